@@ -1,0 +1,105 @@
+// Internal types of the MI355X detection path: the per-frame device
+// descriptor every kernel reads, the shared parameters, and the launch
+// wrappers of each kernel file.
+//
+// Data layout in HBM (one arena per context, one region per frame, all
+// offsets 256-byte aligned; qw x qh is the quad-detection image):
+//   qim     u8  [qh][qw]       decimated (blurred/sharpened) image
+//   tmx/tmn u8  [th][tw]       4x4 tile max / min
+//   t       u8  [qh][qw]       thresholded image (0, 127, 255)
+//   runs    u32 [qh][qw]       row runs, row-strided: start | end << 15 | white << 30
+//   rcnt    u32 [qh]           runs per row; row_off u32 [qh + 1]
+//   parent  u32 [nodes]        union-find links written by the row sweep
+//   fw      u32 [nodes]        component pixel count, valid at final roots
+//   nlab    u32 [nodes]        representative | ok << 31
+//   plab    u32 [qh][qw]       per-pixel nlab (ok 0 where t is 127)
+//   pcnt/poff u32 [qh + 1]     points per row / their offsets
+//   pts     u64 [pcap]         boundary points in emission order
+//   skey/sval, skey2/sval2     radix-sort ping-pong (u64 key, u32 value)
+//   cpts    u64 [pcap]         points grouped by cluster, clusters in order
+//   cstart/ccount u32 [pcap]   cluster starts and sizes
+//   okey    u64 [pcap]         unsorted angle keys; mom f64 [6][pcap] moments
+//   quads   QuadRec [pcap/24]  fitted quads in cluster order; dets at_detection
+// nodes = qw * qh + qh (runs plus one virtual last-column node per row).
+#pragma once
+#include <cstdint>
+
+#include "apriltag_fast.h"
+
+namespace atg {
+
+constexpr int kMaxFamilies = 16;
+constexpr int kSortBlocks = 32;      // radix-sort blocks per frame
+constexpr int kMaxQuadsPerFrame = 4096;
+constexpr int kMaxDetsPerFrame = 4096;
+
+struct DevFamily {
+    int32_t nbits, wab, tw, reversed;
+    int32_t maxhamming, chunk_size, ncodes, pad;
+    int64_t capacity;
+    int32_t bit_x[64], bit_y[64];
+    const uint32_t *offsets;  // [4][capacity + 1]
+    const uint16_t *ids;      // [4][ncodes]
+    const uint64_t *codes;    // [4][ncodes]
+};
+
+struct DevParams {
+    float decimate, sigma;
+    int32_t refine, deglitch;
+    double sharpening;
+    int32_t mcp, max_nmaxima, mwbd, nfam;
+    float cos_crit, max_mse;
+    int32_t tag_width, normal_border, reversed_border, blur_ksz;
+    uint8_t blur_k[64];        // u8 blur kernel (host exp), blur_ksz taps
+    double qsm_kern[7];        // f32-rounded exp(-j^2/2), j = -3..3 (host exp)
+    double rot_cs[4][2];       // cos, sin of r * pi / 2 (host cos/sin)
+    DevFamily fam[kMaxFamilies];
+};
+
+// A quad candidate (fast/common.h:278-282) plus its decode results.
+struct QuadRec {
+    float p[4][2];
+    int32_t reversed;
+    int32_t hok;
+    double H[9];
+};
+
+// counters[] of a frame
+enum {
+    kNRuns = 0, kNPts, kNClusters, kNQuads, kNDets, kError, kNCounters = 16
+};
+
+struct Frame {
+    const uint8_t *img;
+    int64_t stride;
+    int32_t W, H;
+    int32_t qw, qh, tw, th;
+    int32_t nbits_node;   // node ids < 2^nbits_node
+    int32_t key_bits;     // bits of the cluster sort key
+    uint64_t bucket_mask; // cluster hash buckets - 1 (fast/clusters.c:97-103)
+    int32_t max_px;       // 2 * (2 qw + 2 qh) (fast/quads.c:1150)
+    int32_t blurred;      // the quad image was blurred or sharpened
+    int64_t pcap;         // point capacity
+    uint8_t *qim, *t, *tmx, *tmn, *tbuf;
+    uint32_t *runs, *rcnt, *row_off;
+    uint32_t *parent, *fw, *nlab, *plab;
+    uint32_t *pcnt, *poff;
+    uint64_t *pts, *cpts;
+    uint64_t *skey, *skey2;
+    uint32_t *sval, *sval2;
+    uint32_t *rhist;      // [256][kSortBlocks]
+    uint32_t *cstart, *ccount, *cflag;
+    uint64_t *okey;       // unsorted angle keys, by cluster position
+    double *mom;          // [6][pcap] cumulative moments in angle order
+    double *errs, *filt;  // [pcap] window errors, filtered
+    uint32_t *midx;       // [pcap] maxima positions
+    QuadRec *qtmp;        // [kMaxQuadsPerFrame] fitted quads, any order
+    uint32_t *qcl;        // [kMaxQuadsPerFrame] their cluster index
+    QuadRec *quads;       // [kMaxQuadsPerFrame] in cluster order
+    at_detection *dtmp;   // [kMaxQuadsPerFrame * nfam] per (quad, family)
+    uint32_t *dflag;      // [kMaxQuadsPerFrame * nfam]
+    at_detection *dets;   // [kMaxDetsPerFrame] final, sorted by id
+    uint32_t *counters;
+};
+
+}  // namespace atg

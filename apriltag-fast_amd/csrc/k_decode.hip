@@ -1,0 +1,644 @@
+// Stage (d, refine) and (e): per quad, edge refinement on the full image,
+// homography, bit sampling and code lookup; per frame, the overlap
+// reconcile and the stable sort by id.
+//
+//   k_decode   one block per quad. The samples of refine_edges
+//              (fast/decode.c:657-761) run one per thread, each with the
+//              reference's 4-lane step sums (:549-626); their sums over the
+//              edge are then added in sample order by one thread. The bit
+//              sampler (:291-402) gathers its pixel samples in parallel and
+//              accumulates the gray models and scores in the reference's
+//              order in one thread. Lookup walks the quick-decode bucket of
+//              each chunk (:87-107).
+//   k_finish   one block per frame: the detections in (quad, family)
+//              order, reconcile (:860-891), stable sort by id
+//              (fast/detector.c:164-179).
+//
+// atan2f/cosf/sinf must be correctly rounded (the reference's goldens are);
+// they are evaluated in double and rounded once to float.
+#include <hip/hip_runtime.h>
+
+#include "atg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace atg {
+
+#define DADD __dadd_rn
+#define DSUB __dsub_rn
+#define DMUL __dmul_rn
+#define DDIV __ddiv_rn
+
+// C's (int) of a double on x86-64 (fast/decode.c:8-11)
+__device__ __forceinline__ int64_t ctrunc(double x) {
+    if (x > -2147483649.0 && x < 2147483648.0) return (int64_t)x;
+    return -2147483648LL;
+}
+
+struct View {
+    const uint8_t *im;
+    int64_t s, w, h;
+};
+
+// lane_weight (fast/decode.c:440-456): (g2 - g1)^2, or 0 outside the
+// image or where g1 < g2; bilinear sums in the lane order.
+__device__ __forceinline__ double lane_weight(const View &v, double x0, double y0, double nx, double ny, double n) {
+    double x1 = DSUB(DADD(x0, DMUL(DADD(n, 1.0), nx)), 0.5), y1 = DSUB(DADD(y0, DMUL(DADD(n, 1.0), ny)), 0.5);
+    double x2 = DSUB(DADD(x0, DMUL(DSUB(n, 1.0), nx)), 0.5), y2 = DSUB(DADD(y0, DMUL(DSUB(n, 1.0), ny)), 0.5);
+    double wl = (double)(v.w - 1), hl = (double)(v.h - 1);
+    if (!(x1 > -1.0 && x1 < wl && y1 > -1.0 && y1 < hl && x2 > -1.0 && x2 < wl && y2 > -1.0 && y2 < hl)) return 0.0;
+    double tx1 = trunc(x1), ty1 = trunc(y1), tx2 = trunc(x2), ty2 = trunc(y2);
+    const uint8_t *p = v.im + (int64_t)ty1 * v.s + (int64_t)tx1, *q = v.im + (int64_t)ty2 * v.s + (int64_t)tx2;
+    int64_t s = v.s;
+    double a1 = DSUB(x1, tx1), b1 = DSUB(y1, ty1), a2 = DSUB(x2, tx2), b2 = DSUB(y2, ty2);
+    double na1 = DSUB(1.0, a1), nb1 = DSUB(1.0, b1), na2 = DSUB(1.0, a2), nb2 = DSUB(1.0, b2);
+    double g1 = DADD(DADD(DMUL(DMUL(na1, nb1), (double)p[0]), DMUL(DMUL(a1, nb1), (double)p[1])),
+                     DADD(DMUL(DMUL(na1, b1), (double)p[s]), DMUL(DMUL(a1, b1), (double)p[s + 1])));
+    double g2 = DADD(DADD(DMUL(DMUL(na2, nb2), (double)q[0]), DMUL(DMUL(a2, nb2), (double)q[1])),
+                     DADD(DMUL(DMUL(na2, b2), (double)q[s]), DMUL(DMUL(a2, b2), (double)q[s + 1])));
+    double d = DSUB(g2, g1);
+    return !(g1 < g2) ? DMUL(d, d) : 0.0;
+}
+
+// bilinear1 (fast/decode.c:408-411), left to right
+__device__ __forceinline__ double bilinear1(const View &v, int64_t xi, int64_t yi, double a, double b) {
+    const uint8_t *p = v.im + yi * v.s + xi;
+    double na = DSUB(1.0, a), nb = DSUB(1.0, b);
+    double r = DMUL(DMUL(na, nb), (double)p[0]);
+    r = DADD(r, DMUL(DMUL(a, nb), (double)p[1]));
+    r = DADD(r, DMUL(DMUL(na, b), (double)p[v.s]));
+    r = DADD(r, DMUL(DMUL(a, b), (double)p[v.s + 1]));
+    return r;
+}
+
+// One sample point of an edge: the weighted mean offset along the normal.
+__device__ bool refine_sample(const View &v, int64_t range, int64_t max_steps, int64_t vsteps, double x0, double y0,
+                              double nx, double ny, double *bx, double *by) {
+    double rf = (double)-range;
+    double ln[4] = {0, 0, 0, 0}, lc[4] = {0, 0, 0, 0};
+    for (int64_t st = 0; st < vsteps; st++) {
+        double n = DADD(rf, DMUL(0.25, (double)st));
+        double w = lane_weight(v, x0, y0, nx, ny, n);
+        ln[st & 3] = DADD(ln[st & 3], DMUL(w, n));
+        lc[st & 3] = DADD(lc[st & 3], w);
+    }
+    double mn = DADD(0.0, DADD(DADD(ln[0], ln[2]), DADD(ln[1], ln[3])));
+    double mc = DADD(0.0, DADD(DADD(lc[0], lc[2]), DADD(lc[1], lc[3])));
+    for (int64_t st = vsteps; st < max_steps; st++) {
+        double n = DADD(rf, DMUL(0.25, (double)st));
+        double x1 = DSUB(DADD(x0, DMUL(DADD(n, 1.0), nx)), 0.5), y1 = DSUB(DADD(y0, DMUL(DADD(n, 1.0), ny)), 0.5);
+        int64_t x1i = ctrunc(x1), y1i = ctrunc(y1);
+        if (x1i < 0 || x1i + 1 >= v.w || y1i < 0 || y1i + 1 >= v.h) continue;
+        double x2 = DSUB(DADD(x0, DMUL(DSUB(n, 1.0), nx)), 0.5), y2 = DSUB(DADD(y0, DMUL(DSUB(n, 1.0), ny)), 0.5);
+        int64_t x2i = ctrunc(x2), y2i = ctrunc(y2);
+        if (x2i < 0 || x2i + 1 >= v.w || y2i < 0 || y2i + 1 >= v.h) continue;
+        double g1 = bilinear1(v, x1i, y1i, DSUB(x1, (double)x1i), DSUB(y1, (double)y1i));
+        double g2 = bilinear1(v, x2i, y2i, DSUB(x2, (double)x2i), DSUB(y2, (double)y2i));
+        if (g1 < g2) continue;
+        double w = DMUL(DSUB(g2, g1), DSUB(g2, g1));
+        mn = DADD(mn, DMUL(w, n));
+        mc = DADD(mc, w);
+    }
+    if (mc == 0.0) return false;
+    double n0 = DDIV(mn, mc);
+    *bx = DADD(x0, DMUL(n0, nx));
+    *by = DADD(y0, DMUL(n0, ny));
+    return true;
+}
+
+// homography_compute2 (fast/decode.c:114-165)
+__device__ bool homography(const double c[4][4], double hr[9]) {
+    double a[72];
+    for (int i = 0; i < 72; i++) a[i] = 0.0;
+    for (int i = 0; i < 4; i++) {
+        int r0 = 18 * i, r1 = r0 + 9;
+        a[r0 + 0] = c[i][0]; a[r0 + 1] = c[i][1]; a[r0 + 2] = 1.0;
+        a[r0 + 6] = DMUL(-c[i][0], c[i][2]); a[r0 + 7] = DMUL(-c[i][1], c[i][2]); a[r0 + 8] = c[i][2];
+        a[r1 + 3] = c[i][0]; a[r1 + 4] = c[i][1]; a[r1 + 5] = 1.0;
+        a[r1 + 6] = DMUL(-c[i][0], c[i][3]); a[r1 + 7] = DMUL(-c[i][1], c[i][3]); a[r1 + 8] = c[i][3];
+    }
+    for (int col = 0; col < 8; col++) {
+        double mv = 0.0;
+        int mi = -1;
+        for (int row = col; row < 8; row++) {
+            double v = fabs(a[row * 9 + col]);
+            if (v > mv) { mv = v; mi = row; }
+        }
+        if (mi < 0 || mv < 1e-10) return false;
+        if (mi != col)
+            for (int i = col; i < 9; i++) { double t = a[col * 9 + i]; a[col * 9 + i] = a[mi * 9 + i]; a[mi * 9 + i] = t; }
+        for (int i = col + 1; i < 8; i++) {
+            double f = DDIV(a[i * 9 + col], a[col * 9 + col]);
+            a[i * 9 + col] = 0.0;
+            for (int j = col + 1; j < 9; j++) a[i * 9 + j] = DSUB(a[i * 9 + j], DMUL(f, a[col * 9 + j]));
+        }
+    }
+    for (int col = 7; col >= 0; col--) {
+        double sum = 0.0;
+        for (int i = col + 1; i < 8; i++) sum = DADD(sum, DMUL(a[col * 9 + i], a[i * 9 + 8]));
+        a[col * 9 + 8] = DDIV(DSUB(a[col * 9 + 8], sum), a[col * 9 + col]);
+    }
+    hr[0] = a[8]; hr[1] = a[17]; hr[2] = a[26]; hr[3] = a[35]; hr[4] = a[44];
+    hr[5] = a[53]; hr[6] = a[62]; hr[7] = a[71]; hr[8] = 1.0;
+    return true;
+}
+
+// matd_invertible (fast/decode.c:168-196)
+__device__ bool invertible(const double m[9]) {
+    double lu[9];
+    for (int i = 0; i < 9; i++) lu[i] = m[i];
+    for (int j = 0; j < 3; j++) {
+        for (int i = 0; i < 3; i++) {
+            int km = i < j ? i : j;
+            double acc = 0.0;
+            for (int k = 0; k < km; k++) acc = DADD(acc, DMUL(lu[i * 3 + k], lu[k * 3 + j]));
+            lu[i * 3 + j] = DSUB(lu[i * 3 + j], acc);
+        }
+        int p = j;
+        for (int i = j + 1; i < 3; i++) if (fabs(lu[i * 3 + j]) > fabs(lu[p * 3 + j])) p = i;
+        if (p != j)
+            for (int k = 0; k < 3; k++) { double t = lu[p * 3 + k]; lu[p * 3 + k] = lu[j * 3 + k]; lu[j * 3 + k] = t; }
+        double d = lu[j * 3 + j];
+        if (fabs(d) < 1e-8) return false;
+        if (d != 0.0) {
+            double inv = DDIV(1.0, d);
+            for (int i = j + 1; i < 3; i++) lu[i * 3 + j] = DMUL(lu[i * 3 + j], inv);
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ void project(const double h[9], double x, double y, double *ox, double *oy) {
+    double xx = DADD(DADD(DMUL(h[0], x), DMUL(h[1], y)), h[2]);
+    double yy = DADD(DADD(DMUL(h[3], x), DMUL(h[4], y)), h[5]);
+    double zz = DADD(DADD(DMUL(h[6], x), DMUL(h[7], y)), h[8]);
+    *ox = DDIV(xx, zz);
+    *oy = DDIV(yy, zz);
+}
+
+struct Gray {
+    double a[9], b[3], c[3];
+};
+
+__device__ __forceinline__ void gray_add(Gray &g, double x, double y, double v) {
+    g.a[0] = DADD(g.a[0], DMUL(x, x)); g.a[1] = DADD(g.a[1], DMUL(x, y)); g.a[2] = DADD(g.a[2], x);
+    g.a[4] = DADD(g.a[4], DMUL(y, y)); g.a[5] = DADD(g.a[5], y); g.a[8] = DADD(g.a[8], 1.0);
+    g.b[0] = DADD(g.b[0], DMUL(x, v)); g.b[1] = DADD(g.b[1], DMUL(y, v)); g.b[2] = DADD(g.b[2], v);
+}
+
+// graymodel_solve (fast/decode.c:242-265)
+__device__ void gray_solve(Gray &g) {
+    const double *a = g.a;
+    double l[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    l[0] = __dsqrt_rn(a[0]);
+    l[3] = DDIV(a[1], l[0]);
+    l[6] = DDIV(a[2], l[0]);
+    l[4] = __dsqrt_rn(DSUB(a[4], DMUL(l[3], l[3])));
+    l[7] = DDIV(DSUB(a[5], DMUL(l[3], l[6])), l[4]);
+    l[8] = __dsqrt_rn(DSUB(DSUB(a[8], DMUL(l[6], l[6])), DMUL(l[7], l[7])));
+    m[0] = DDIV(1.0, l[0]);
+    m[3] = DDIV(DMUL(-l[3], m[0]), l[4]);
+    m[4] = DDIV(1.0, l[4]);
+    m[6] = DDIV(DSUB(DMUL(-l[6], m[0]), DMUL(l[7], m[3])), l[8]);
+    m[7] = DDIV(DMUL(-l[7], m[4]), l[8]);
+    m[8] = DDIV(1.0, l[8]);
+    const double *b = g.b;
+    double t0 = DMUL(m[0], b[0]);
+    double t1 = DADD(DMUL(m[3], b[0]), DMUL(m[4], b[1]));
+    double t2 = DADD(DADD(DMUL(m[6], b[0]), DMUL(m[7], b[1])), DMUL(m[8], b[2]));
+    g.c[0] = DADD(DADD(DMUL(m[0], t0), DMUL(m[3], t1)), DMUL(m[6], t2));
+    g.c[1] = DADD(DMUL(m[4], t1), DMUL(m[7], t2));
+    g.c[2] = DMUL(m[8], t2);
+}
+
+// value_for_pixel (fast/decode.c:268-278)
+__device__ double pixel_value(const View &v, double px, double py) {
+    double xm = DSUB(px, 0.5), ym = DSUB(py, 0.5);
+    int64_t x1 = ctrunc(floor(xm)), x2 = ctrunc(ceil(xm));
+    double x = DSUB(xm, (double)x1);
+    int64_t y1 = ctrunc(floor(ym)), y2 = ctrunc(ceil(ym));
+    double y = DSUB(ym, (double)y1);
+    if (x1 < 0 || x2 >= v.w || y1 < 0 || y2 >= v.h) return -1.0;
+    const uint8_t *r1 = v.im + y1 * v.s, *r2 = v.im + y2 * v.s;
+    double nx = DSUB(1.0, x), ny = DSUB(1.0, y);
+    double r = DMUL(DMUL((double)r1[x1], nx), ny);
+    r = DADD(r, DMUL(DMUL((double)r1[x2], x), ny));
+    r = DADD(r, DMUL(DMUL((double)r2[x1], nx), y));
+    r = DADD(r, DMUL(DMUL((double)r2[x2], x), y));
+    return r;
+}
+
+// rotate90 (fast/decode.c:61-71)
+__device__ __forceinline__ uint64_t rotate90(uint64_t w0, int64_t nbits) {
+    int64_t p = nbits;
+    uint64_t l = 0;
+    if (nbits % 4 == 1) { p = nbits - 1; l = 1; }
+    uint64_t w = ((w0 >> l) << (p / 4 + (int64_t)l)) | (w0 >> (3 * p / 4 + (int64_t)l) << l) | (w0 & l);
+    return w & (((uint64_t)1 << nbits) - 1);
+}
+
+constexpr int kMaxSamples = 512;  // bit-sampler points: 8 * wab + nbits
+
+__global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevParams *P) {
+    __shared__ double sx[256], sy[256];
+    __shared__ int sv[256];
+    __shared__ double smp[3][kMaxSamples];  // tag x, tag y, value
+    __shared__ int sflag[kMaxSamples];
+    __shared__ float qp[4][2];
+    __shared__ double lines[4][4];
+    __shared__ double sH[9];
+    __shared__ int s_hok;
+    __shared__ Gray g_white, g_black;
+    __shared__ double vals[12 * 12], sharp[12 * 12];
+    const Frame F = frames[blockIdx.y];
+    const uint32_t nq = F.counters[kNQuads];
+    const int t = threadIdx.x;
+    const int nfam = P->nfam;
+    View v = {F.img, F.stride, F.W, F.H};
+    if (F.blurred && !(P->decimate > 1.0f)) v = View{F.qim, F.W, F.W, F.H};
+    for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        QuadRec q = F.quads[qi];
+        if (t == 0) {
+            for (int i = 0; i < 4; i++) {
+                float f = P->decimate;
+                qp[i][0] = f > 1.0f ? __fmul_rn(q.p[i][0], f) : q.p[i][0];
+                qp[i][1] = f > 1.0f ? __fmul_rn(q.p[i][1], f) : q.p[i][1];
+            }
+        }
+        __syncthreads();
+        if (P->refine) {
+            int64_t range = (int64_t)(P->decimate + 1.0f);
+            int64_t max_steps = 2 * 4 * range + 1;
+            int64_t vsteps = max_steps - max_steps % 4;
+            if (vsteps < 0) vsteps = 0;
+            for (int edge = 0; edge < 4; edge++) {
+                int a = edge, b = (edge + 1) & 3;
+                double nx = (double)__fsub_rn(qp[b][1], qp[a][1]);
+                double ny = (double)__fadd_rn(-qp[b][0], qp[a][0]);
+                double mag = __dsqrt_rn(DADD(DMUL(nx, nx), DMUL(ny, ny)));
+                nx = DDIV(nx, mag);
+                ny = DDIV(ny, mag);
+                if (q.reversed) { nx = -nx; ny = -ny; }
+                int64_t ns = ctrunc(DDIV(mag, 8.0));
+                if (ns < 16) ns = 16;
+                double mx = 0.0, my = 0.0, mxx = 0.0, mxy = 0.0, myy = 0.0, nn = 0.0;
+                for (int64_t c0 = 0; c0 < ns; c0 += 256) {
+                    int64_t si = c0 + t;
+                    int ok = 0;
+                    if (si < ns) {
+                        double alpha = DDIV(DADD(1.0, (double)si), (double)(ns + 1));
+                        double x0 = DADD(DMUL(alpha, (double)qp[a][0]), DMUL(DSUB(1.0, alpha), (double)qp[b][0]));
+                        double y0 = DADD(DMUL(alpha, (double)qp[a][1]), DMUL(DSUB(1.0, alpha), (double)qp[b][1]));
+                        double bx = 0, by = 0;
+                        ok = refine_sample(v, range, max_steps, vsteps, x0, y0, nx, ny, &bx, &by) ? 1 : 0;
+                        sx[t] = bx;
+                        sy[t] = by;
+                    }
+                    sv[t] = ok;
+                    __syncthreads();
+                    if (t == 0) {
+                        int64_t len = ns - c0 < 256 ? ns - c0 : 256;
+                        for (int64_t k = 0; k < len; k++) {
+                            if (!sv[k]) continue;
+                            double bx = sx[k], by = sy[k];
+                            mx = DADD(mx, bx); my = DADD(my, by);
+                            mxx = DADD(mxx, DMUL(bx, bx)); mxy = DADD(mxy, DMUL(bx, by)); myy = DADD(myy, DMUL(by, by));
+                            nn = DADD(nn, 1.0);
+                        }
+                    }
+                    __syncthreads();
+                }
+                if (t == 0) {
+                    double ex = DDIV(mx, nn), ey = DDIV(my, nn);
+                    double cxx = DSUB(DDIV(mxx, nn), DMUL(ex, ex));
+                    double cxy = DSUB(DDIV(mxy, nn), DMUL(ex, ey));
+                    double cyy = DSUB(DDIV(myy, nn), DMUL(ey, ey));
+                    float th2 = (float)atan2((double)(float)DMUL(-2.0, cxy), (double)(float)DSUB(cyy, cxx));
+                    double theta = DMUL(0.5, (double)th2);
+                    lines[edge][0] = ex;
+                    lines[edge][1] = ey;
+                    lines[edge][2] = (double)(float)cos((double)(float)theta);
+                    lines[edge][3] = (double)(float)sin((double)(float)theta);
+                }
+                __syncthreads();
+            }
+            if (t == 0) {
+                for (int i = 0; i < 4; i++) {
+                    int j = (i + 1) & 3;
+                    double a00 = lines[i][3], a01 = -lines[j][3], a10 = -lines[i][2], a11 = lines[j][2];
+                    double b0 = DADD(-lines[i][0], lines[j][0]), b1 = DADD(-lines[i][1], lines[j][1]);
+                    double det = DSUB(DMUL(a00, a11), DMUL(a10, a01));
+                    if (fabs(det) > 0.001) {
+                        double w00 = DDIV(a11, det), w01 = DDIV(-a01, det);
+                        double l0 = DADD(DMUL(w00, b0), DMUL(w01, b1));
+                        qp[j][0] = (float)DADD(lines[i][0], DMUL(l0, a00));
+                        qp[j][1] = (float)DADD(lines[i][1], DMUL(l0, a10));
+                    }
+                }
+            }
+        }
+        if (t == 0) {
+            double corr[4][4];
+            for (int i = 0; i < 4; i++) {
+                corr[i][0] = i == 0 || i == 3 ? -1.0 : 1.0;
+                corr[i][1] = i == 0 || i == 1 ? -1.0 : 1.0;
+                corr[i][2] = qp[i][0];
+                corr[i][3] = qp[i][1];
+            }
+            double h[9];
+            int hok = homography(corr, h) ? 1 : 0;
+            if (hok) {
+                for (int i = 0; i < 9; i++) sH[i] = h[i];
+                hok = invertible(h) ? 1 : 0;
+            }
+            s_hok = hok;
+        }
+        __syncthreads();
+        for (int fi = 0; fi < nfam; fi++) {
+            if (t == 0) F.dflag[qi * nfam + fi] = 0;
+            const DevFamily &fm = P->fam[fi];
+            if (!s_hok || fm.reversed != q.reversed) continue;
+            const int wab = fm.wab;
+            const float wf = (float)wab;
+            const int nb = 8 * wab;
+            if (nb + fm.nbits > kMaxSamples || fm.tw > 12) continue;
+            // border samples (fast/decode.c:310-326), in parallel
+            const float pat[40] = {-0.5f, 0.5f, 0.0f, 1.0f, 1.0f,   0.5f, 0.5f, 0.0f, 1.0f, 0.0f,
+                                   (float)(wab + 0.5), 0.5f, 0.0f, 1.0f, 1.0f, (float)(wab - 0.5), 0.5f, 0.0f, 1.0f, 0.0f,
+                                   0.5f, -0.5f, 1.0f, 0.0f, 1.0f,   0.5f, 0.5f, 1.0f, 0.0f, 0.0f,
+                                   0.5f, (float)(wab + 0.5), 1.0f, 0.0f, 1.0f, 0.5f, (float)(wab - 0.5), 1.0f, 0.0f, 0.0f};
+            for (int k = t; k < nb; k += 256) {
+                int pi = k / wab, i = k % wab;
+                double tx01 = (double)__fdiv_rn(__fadd_rn(pat[pi * 5], __fmul_rn((float)i, pat[pi * 5 + 2])), wf);
+                double ty01 = (double)__fdiv_rn(__fadd_rn(pat[pi * 5 + 1], __fmul_rn((float)i, pat[pi * 5 + 3])), wf);
+                double tx = DMUL(2.0, DSUB(tx01, 0.5)), ty = DMUL(2.0, DSUB(ty01, 0.5));
+                double zz = DADD(DADD(DMUL(sH[6], tx), DMUL(sH[7], ty)), sH[8]);
+                double px = DDIV(DADD(DADD(DMUL(sH[0], tx), DMUL(sH[1], ty)), sH[2]), zz);
+                double py = DDIV(DADD(DADD(DMUL(sH[3], tx), DMUL(sH[4], ty)), sH[5]), zz);
+                int64_t ix = ctrunc(px), iy = ctrunc(py);
+                int f = 0;
+                double val = 0.0;
+                if (!(ix < 0 || iy < 0 || ix >= v.w || iy >= v.h)) {
+                    f = 1 | (pat[pi * 5 + 4] != 0.0f ? 2 : 0);
+                    val = (double)v.im[iy * v.s + ix];
+                }
+                smp[0][k] = tx;
+                smp[1][k] = ty;
+                smp[2][k] = val;
+                sflag[k] = f;
+            }
+            // bit samples (fast/decode.c:346-362)
+            const int mc = (wab - fm.tw) / 2;
+            for (int i = t; i < fm.nbits; i += 256) {
+                int k = nb + i;
+                double tx01 = DDIV(DADD((double)fm.bit_x[i], 0.5), (double)wab);
+                double ty01 = DDIV(DADD((double)fm.bit_y[i], 0.5), (double)wab);
+                double tx = DMUL(2.0, DSUB(tx01, 0.5)), ty = DMUL(2.0, DSUB(ty01, 0.5));
+                double zz = DADD(DADD(DMUL(sH[6], tx), DMUL(sH[7], ty)), sH[8]);
+                double px = DDIV(DADD(DADD(DMUL(sH[0], tx), DMUL(sH[1], ty)), sH[2]), zz);
+                double py = DDIV(DADD(DADD(DMUL(sH[3], tx), DMUL(sH[4], ty)), sH[5]), zz);
+                smp[0][k] = tx;
+                smp[1][k] = ty;
+                smp[2][k] = pixel_value(v, px, py);
+            }
+            __syncthreads();
+            if (t == 0) {
+                Gray wh, bl;
+                for (int i = 0; i < 9; i++) { wh.a[i] = 0; bl.a[i] = 0; }
+                for (int i = 0; i < 3; i++) { wh.b[i] = 0; bl.b[i] = 0; wh.c[i] = 0; bl.c[i] = 0; }
+                for (int k = 0; k < nb; k++) {
+                    if (!(sflag[k] & 1)) continue;
+                    gray_add((sflag[k] & 2) ? wh : bl, smp[0][k], smp[1][k], smp[2][k]);
+                }
+                gray_solve(wh);
+                if (wab > 1) gray_solve(bl);
+                else { bl.c[0] = 0.0; bl.c[1] = 0.0; bl.c[2] = DDIV(bl.b[2], 4.0); }
+                double wi = DADD(DADD(DMUL(wh.c[0], 0.0), DMUL(wh.c[1], 0.0)), wh.c[2]);
+                double bi = DADD(DADD(DMUL(bl.c[0], 0.0), DMUL(bl.c[1], 0.0)), bl.c[2]);
+                bool polar = (DSUB(wi, bi) < 0.0) == (fm.reversed != 0);
+                if (polar) {
+                    const int tw = fm.tw;
+                    for (int i = 0; i < tw * tw; i++) vals[i] = 0.0;
+                    for (int i = 0; i < fm.nbits; i++) {
+                        double val = smp[2][nb + i];
+                        if (val == -1.0) continue;
+                        double tx = smp[0][nb + i], ty = smp[1][nb + i];
+                        double th = DDIV(DADD(DADD(DADD(DMUL(bl.c[0], tx), DMUL(bl.c[1], ty)), bl.c[2]),
+                                              DADD(DADD(DMUL(wh.c[0], tx), DMUL(wh.c[1], ty)), wh.c[2])),
+                                         2.0);
+                        vals[tw * (fm.bit_y[i] - mc) + (fm.bit_x[i] - mc)] = DSUB(val, th);
+                    }
+                    for (int y = 0; y < tw; y++)
+                        for (int x = 0; x < tw; x++) {
+                            double acc = 0.0;
+                            if (y > 0) acc = DADD(acc, -vals[(y - 1) * tw + x]);
+                            if (x > 0) acc = DADD(acc, -vals[y * tw + x - 1]);
+                            acc = DADD(acc, DMUL(vals[y * tw + x], 4.0));
+                            if (x + 1 < tw) acc = DADD(acc, -vals[y * tw + x + 1]);
+                            if (y + 1 < tw) acc = DADD(acc, -vals[(y + 1) * tw + x]);
+                            sharp[y * tw + x] = acc;
+                        }
+                    for (int i = 0; i < tw * tw; i++) vals[i] = DADD(vals[i], DMUL(P->sharpening, sharp[i]));
+                    float bs = 0.0f, ws = 0.0f, bc = 1.0f, wc = 1.0f;
+                    uint64_t rc = 0;
+                    for (int i = 0; i < fm.nbits; i++) {
+                        rc <<= 1;
+                        double vv = vals[(fm.bit_y[i] - mc) * tw + (fm.bit_x[i] - mc)];
+                        if (vv > 0.0) { ws = (float)DADD((double)ws, vv); wc = __fadd_rn(wc, 1.0f); rc |= 1; }
+                        else { bs = (float)DSUB((double)bs, vv); bc = __fadd_rn(bc, 1.0f); }
+                    }
+                    double xm = (double)__fdiv_rn(ws, wc), ym = (double)__fdiv_rn(bs, bc);
+                    double mg = xm != xm ? ym : ym != ym ? xm : xm < ym ? xm : ym;
+                    float margin = (float)mg;
+                    // quick decode (fast/decode.c:87-107)
+                    int64_t id = 65535, ham = 255, rot = 0;
+                    uint64_t rcode = rc;
+                    uint64_t cmask = (uint64_t)(fm.capacity - 1);
+                    bool found = false;
+                    for (int r = 0; r < 4 && !found; r++) {
+                        for (int i = 0; i < 4 && !found; i++) {
+                            int64_t val = (int64_t)((rcode >> (i * fm.chunk_size)) & cmask);
+                            int64_t ob = i * (fm.capacity + 1) + val;
+                            int64_t basei = (int64_t)i * fm.ncodes;
+                            for (int64_t j = basei + fm.offsets[ob]; j < basei + fm.offsets[ob + 1]; j++) {
+                                int64_t hm = __popcll(fm.codes[j] ^ rcode);
+                                if (hm <= fm.maxhamming) {
+                                    id = fm.ids[j];
+                                    ham = hm;
+                                    rot = r;
+                                    found = true;
+                                    break;
+                                }
+                            }
+                        }
+                        if (!found) rcode = rotate90(rcode, fm.nbits);
+                    }
+                    if (margin >= 0.0f && ham < 255) {
+                        // make_detection (fast/detector.c:83-107)
+                        at_detection d;
+                        double c = P->rot_cs[rot][0], s = P->rot_cs[rot][1];
+                        const double rr[9] = {c, -s, 0.0, s, c, 0.0, 0.0, 0.0, 1.0};
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++) {
+                                double acc = 0.0;
+                                for (int k = 0; k < 3; k++) acc = DADD(acc, DMUL(sH[i * 3 + k], rr[k * 3 + j]));
+                                d.H[i * 3 + j] = acc;
+                            }
+                        d.family = fi;
+                        d.id = (int32_t)id;
+                        d.hamming = (int32_t)ham;
+                        d.decision_margin = margin;
+                        project(d.H, 0.0, 0.0, &d.c[0], &d.c[1]);
+                        for (int i = 0; i < 4; i++)
+                            project(d.H, i == 1 || i == 2 ? 1.0 : -1.0, i < 2 ? 1.0 : -1.0, &d.p[i][0], &d.p[i][1]);
+                        F.dtmp[qi * nfam + fi] = d;
+                        F.dflag[qi * nfam + fi] = 1;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            QuadRec &o = F.quads[qi];
+            for (int i = 0; i < 4; i++) { o.p[i][0] = qp[i][0]; o.p[i][1] = qp[i][1]; }
+            o.hok = s_hok;
+            for (int i = 0; i < 9; i++) o.H[i] = sH[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ reconcile
+
+struct G2Line {
+    double p[2], u[2];
+};
+
+__device__ G2Line g2_line(const double *a, const double *b) {
+    double u0 = DSUB(b[0], a[0]), u1 = DSUB(b[1], a[1]);
+    double mag = (double)__fsqrt_rn((float)DADD(DMUL(u0, u0), DMUL(u1, u1)));
+    G2Line l;
+    l.p[0] = a[0]; l.p[1] = a[1]; l.u[0] = DDIV(u0, mag); l.u[1] = DDIV(u1, mag);
+    return l;
+}
+__device__ double g2_coord(const G2Line &l, const double *q) {
+    return DADD(DMUL(DSUB(q[0], l.p[0]), l.u[0]), DMUL(DSUB(q[1], l.p[1]), l.u[1]));
+}
+__device__ bool g2_isect(const G2Line &la, const G2Line &lb, double out[2]) {
+    double m00 = la.u[0], m01 = -lb.u[0], m10 = la.u[1], m11 = -lb.u[1];
+    double det = DSUB(DMUL(m00, m11), DMUL(m01, m10));
+    if (fabs(det) < 0.00000001) return false;
+    double i00 = DDIV(m11, det), i01 = DDIV(-m01, det);
+    double b00 = DSUB(lb.p[0], la.p[0]), b10 = DSUB(lb.p[1], la.p[1]);
+    double x00 = DADD(DMUL(i00, b00), DMUL(i01, b10));
+    out[0] = DADD(DMUL(la.u[0], x00), la.p[0]);
+    out[1] = DADD(DMUL(la.u[1], x00), la.p[1]);
+    return true;
+}
+__device__ bool g2_seg(const double *a0, const double *a1, const double *b0, const double *b1) {
+    G2Line la = g2_line(a0, a1), lb = g2_line(b0, b1);
+    double t[2];
+    if (!g2_isect(la, lb, t)) return false;
+    double a = g2_coord(la, la.p), b = g2_coord(la, a1), c = g2_coord(la, t);
+    if ((c < a && c < b) || (c > a && c > b)) return false;
+    a = g2_coord(lb, lb.p); b = g2_coord(lb, b1); c = g2_coord(lb, t);
+    if ((c < a && c < b) || (c > a && c > b)) return false;
+    return true;
+}
+__device__ bool g2_contains(const double poly[4][2], const double q[2]) {
+    int last = 0, acc = 0;
+    for (int i = 0; i < 5; i++) {
+        const double *p = poly[i % 4];
+        int quadrant = p[0] < q[0] ? (p[1] < q[1] ? 2 : 1) : (p[1] < q[1] ? 3 : 0);
+        if (i > 0) {
+            int dq = quadrant - last;
+            if (dq == -3 || dq == 1) acc++;
+            else if (dq == -1 || dq == 3) acc--;
+            else if (dq == -2 || dq == 2) {
+                const double *p0 = poly[i - 1];
+                double nx = DSUB(p[1], q[1]), ny = DADD(-p[0], q[0]);
+                double dot = DADD(DMUL(nx, DSUB(p0[0], q[0])), DMUL(ny, DSUB(p0[1], q[1])));
+                acc += dot < 0.0 ? -2 : 2;
+            }
+        }
+        last = quadrant;
+    }
+    return acc >= 2 || acc <= -2;
+}
+__device__ bool g2_overlaps(const double pa[4][2], const double pb[4][2]) {
+    for (int ia = 0; ia < 4; ia++)
+        for (int ib = 0; ib < 4; ib++)
+            if (g2_seg(pa[ia], pa[(ia + 1) % 4], pb[ib], pb[(ib + 1) % 4])) return true;
+    double qb[2] = {DDIV(DADD(DADD(pb[0][0], pb[1][0]), pb[2][0]), 3.0), DDIV(DADD(DADD(pb[0][1], pb[1][1]), pb[2][1]), 3.0)};
+    if (g2_contains(pa, qb)) return true;
+    double qa[2] = {DDIV(DADD(DADD(pa[0][0], pa[1][0]), pa[2][0]), 3.0), DDIV(DADD(DADD(pa[0][1], pa[1][1]), pa[2][1]), 3.0)};
+    return g2_contains(pb, qa);
+}
+__device__ __forceinline__ int prefer(int p, double a, double b) { return p != 0 ? p : a < b ? -1 : b < a ? 1 : 0; }
+
+__global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevParams *P) {
+    const Frame F = frames[blockIdx.y];
+    if (threadIdx.x != 0) return;
+    const uint32_t nq = F.counters[kNQuads];
+    const int nfam = P->nfam;
+    at_detection *d = F.dets;
+    int64_t n = 0;
+    for (uint32_t k = 0; k < nq * (uint32_t)nfam; k++) {
+        if (!F.dflag[k]) continue;
+        if (n < kMaxDetsPerFrame) d[n++] = F.dtmp[k];
+        else atomicOr(F.counters + kError, 16u);
+    }
+    // reconcile (fast/decode.c:860-891)
+    int64_t i0 = 0;
+    while (i0 < n) {
+        bool removed0 = false;
+        int64_t i1 = i0 + 1;
+        while (i1 < n) {
+            if (d[i0].id != d[i1].id || d[i0].family != d[i1].family) { i1++; continue; }
+            if (g2_overlaps(d[i0].p, d[i1].p)) {
+                int p = 0;
+                p = prefer(p, (double)d[i0].hamming, (double)d[i1].hamming);
+                p = prefer(p, -(double)d[i0].decision_margin, -(double)d[i1].decision_margin);
+                for (int i = 0; i < 4; i++) {
+                    p = prefer(p, d[i0].p[i][0], d[i1].p[i][0]);
+                    p = prefer(p, d[i0].p[i][1], d[i1].p[i][1]);
+                }
+                if (p < 0) {
+                    for (int64_t k = i1; k + 1 < n; k++) d[k] = d[k + 1];
+                    n--;
+                    continue;
+                }
+                for (int64_t k = i0; k + 1 < n; k++) d[k] = d[k + 1];
+                n--;
+                removed0 = true;
+                break;
+            }
+            i1++;
+        }
+        if (!removed0) i0++;
+    }
+    // stable sort by id
+    for (int64_t i = 1; i < n; i++) {
+        at_detection v = d[i];
+        int64_t j = i;
+        while (j > 0 && d[j - 1].id > v.id) { d[j] = d[j - 1]; j--; }
+        d[j] = v;
+    }
+    F.counters[kNDets] = (uint32_t)n;
+}
+
+__global__ void k_init(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    if (threadIdx.x < kNCounters) F.counters[threadIdx.x] = 0;
+}
+
+void launch_init(hipStream_t st, const Frame *dframes, int nframes) {
+    hipLaunchKernelGGL(k_init, dim3(1, nframes), dim3(64), 0, st, dframes);
+}
+
+void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks) {
+    hipLaunchKernelGGL(k_decode, dim3(blocks, nframes), dim3(256), 0, st, dframes, dparams);
+    hipLaunchKernelGGL(k_finish, dim3(1, nframes), dim3(64), 0, st, dframes, dparams);
+}
+
+}  // namespace atg

@@ -1,0 +1,303 @@
+// Stage (c): boundary points, their clusters and the cluster order.
+//
+//   k_point_rows   one wave per row pair (y, y+1), y in 1..qh-2: the 4-bit
+//                  neighbour mask of each pixel (fast/clusters.c:150-174);
+//                  pass 0 counts points per row, pass 1 writes each point
+//                  at its emission index (row offset + wave prefix) with
+//                  its cluster sort key.
+//   k_point_scan   the row offsets of the points.
+//   radix sort     stable LSD sort of (key, index): the keys order the
+//                  clusters by (hash bucket, id) as the reference's cluster
+//                  table does (fast/clusters.c:338-356); stability keeps
+//                  each cluster's points in emission order.
+//   k_cluster_*    the points regrouped and the cluster starts compacted.
+//
+// Cluster sort key of a point: bucket << 2B | max_root << B | min_root
+// with B = bits of the node count (node ids < 2^B); comparing it equals
+// comparing (bucket, id) for id = max_root << 32 | min_root.
+#include <hip/hip_runtime.h>
+
+#include "atg_internal.h"
+
+namespace atg {
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread block; returns the prefix, *total the sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan_u(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int i = 0; i < 4; i++) {
+        if (i < w) pre += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+__device__ __forceinline__ uint32_t opp(uint32_t a, uint32_t b) { return (a ^ b) == 255u ? 1u : 0u; }
+
+// The point mask of pixel x of row a against row b (rows of t and plab).
+__device__ __forceinline__ uint32_t point_mask(const uint8_t *a, const uint8_t *b, const uint32_t *la,
+                                               const uint32_t *lb, int x) {
+    auto oa = [&](int i) { return la[i] >> 31; };
+    auto ob = [&](int i) { return lb[i] >> 31; };
+    if (x == 0) return (opp(a[0], a[1]) && oa(0) && oa(1)) ? 1u : 0u;
+    if (!oa(x)) return 0u;
+    uint32_t v = a[x];
+    uint32_t c10 = opp(v, a[x + 1]) & oa(x + 1);
+    uint32_t c01 = opp(v, b[x]) & ob(x);
+    uint32_t c11 = opp(v, b[x + 1]) & ob(x + 1);
+    uint32_t cm1 = opp(v, b[x - 1]) & ob(x - 1);
+    // the (1,1) point of the pixel to the left is the same pair; x = 1 has
+    // no pixel to the left in the C library's loop (fast/clusters.c:170-171)
+    uint32_t p11 = x > 1 ? (opp(a[x - 1], b[x]) & oa(x - 1) & ob(x)) : 0u;
+    return c10 | (c01 << 1) | ((cm1 & (p11 ^ 1u)) << 2) | (c11 << 3);
+}
+
+__global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pass) {
+    const Frame F = frames[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    int y = 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (y > F.qh - 2) return;
+    const int QW = F.qw;
+    const uint8_t *a = F.t + (int64_t)y * QW, *b = a + QW;
+    const uint32_t *la = F.plab + (int64_t)y * QW, *lb = la + QW;
+    uint32_t count = 0;
+    uint32_t base = pass ? F.poff[y] : 0u;
+    const int NB = F.nbits_node;
+    const uint64_t mask = F.bucket_mask;
+    for (int x0 = 0; x0 < QW - 1; x0 += 64) {
+        int x = x0 + lane;
+        uint32_t m = x < QW - 1 ? point_mask(a, b, la, lb, x) : 0u;
+        uint32_t c = __popc(m);
+        if (pass) {
+            uint32_t inc = wave_incl_scan_u(c);
+            uint32_t at = base + count + inc - c;
+            if (m) {
+                uint32_t av = a[x];
+                int vd = 255 - 2 * (int)av;
+                uint32_t r0 = la[x] & 0x7fffffffu;
+                for (int j = 0; j < 4; j++) {
+                    if (!((m >> j) & 1u)) continue;
+                    if ((int64_t)at >= F.pcap) break;  // flagged by k_point_scan
+                    int dx = j == 0 ? 1 : j == 1 ? 0 : j == 2 ? -1 : 1;
+                    int dy = j == 0 ? 0 : 1;
+                    uint32_t r1 = (dy ? lb[x + dx] : la[x + dx]) & 0x7fffffffu;
+                    uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
+                    uint64_t id = ((uint64_t)hi << 32) + lo;
+                    uint64_t bucket = ((id * 2654435761ull) >> 32) & mask;
+                    uint64_t key = (bucket << (2 * NB)) | ((uint64_t)hi << NB) | lo;
+                    uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
+                    uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
+                    F.pts[at] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
+                    F.skey[at] = key;
+                    F.sval[at] = at;
+                    at++;
+                }
+            }
+            count += __shfl(inc, 63, 64);
+        } else {
+            count += __shfl(wave_incl_scan_u(c), 63, 64);
+        }
+    }
+    if (!pass && lane == 0) F.pcnt[y] = count;
+}
+
+// Row offsets of the points (one block per frame).
+__global__ __launch_bounds__(256) void k_point_scan(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    uint32_t carry = 0;
+    for (int y0 = 0; y0 < F.qh; y0 += 256) {
+        int y = y0 + threadIdx.x;
+        uint32_t c = (y >= 1 && y <= F.qh - 2) ? F.pcnt[y] : 0u;
+        uint32_t tot;
+        uint32_t pre = block_excl_scan(c, &tot);
+        if (y < F.qh) F.poff[y] = carry + pre;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        uint32_t n = carry;
+        if ((int64_t)n > F.pcap) {
+            atomicOr(F.counters + kError, 2u);
+            n = (uint32_t)F.pcap;
+        }
+        F.counters[kNPts] = n;
+        F.poff[F.qh] = carry;
+    }
+}
+
+// ---------------------------------------------------------------- radix sort
+
+struct SortBufs {
+    const uint64_t *kin;
+    const uint32_t *vin;
+    uint64_t *kout;
+    uint32_t *vout;
+};
+
+__device__ __forceinline__ SortBufs sort_bufs(const Frame &F, int parity) {
+    if (parity == 0) return {F.skey, F.sval, F.skey2, F.sval2};
+    return {F.skey2, F.sval2, F.skey, F.sval};
+}
+
+__device__ __forceinline__ void seg_of(uint32_t n, int b, uint32_t *s0, uint32_t *s1) {
+    uint32_t seg = ((n + kSortBlocks - 1) / kSortBlocks + 255u) & ~255u;
+    *s0 = min(n, (uint32_t)b * seg);
+    *s1 = min(n, (uint32_t)(b + 1) * seg);
+}
+
+// count = counters[which]
+__global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int which, int shift, int parity) {
+    __shared__ uint32_t h[256];
+    const Frame F = frames[blockIdx.y];
+    uint32_t n = F.counters[which];
+    SortBufs B = sort_bufs(F, parity);
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t s0, s1;
+    seg_of(n, blockIdx.x, &s0, &s1);
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) atomicAdd(&h[(B.kin[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    F.rhist[threadIdx.x * kSortBlocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: chunks of 256 keys in order; inside a chunk, the rank of
+// a key among the keys of its digit before it: within a wave by ballots
+// (peers = lanes of the same digit), across waves by per-wave counts.
+__global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity) {
+    __shared__ uint32_t off[256];
+    __shared__ uint32_t wcnt[4][256];
+    const Frame F = frames[blockIdx.y];
+    uint32_t n = F.counters[which];
+    SortBufs B = sort_bufs(F, parity);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    {
+        const uint32_t *hd = F.rhist + t * kSortBlocks;
+        uint32_t tot = 0, pre = 0;
+        for (int b = 0; b < kSortBlocks; b++) {
+            uint32_t v = hd[b];
+            if (b < (int)blockIdx.x) pre += v;
+            tot += v;
+        }
+        uint32_t all;
+        uint32_t base = block_excl_scan(tot, &all);
+        off[t] = base + pre;
+        for (int k = 0; k < 4; k++) wcnt[k][t] = 0;
+    }
+    __syncthreads();
+    uint32_t s0, s1;
+    seg_of(n, blockIdx.x, &s0, &s1);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
+        uint32_t i = c0 + t;
+        bool valid = i < s1;
+        uint64_t key = valid ? B.kin[i] : 0ull;
+        uint32_t val = valid ? B.vin[i] : 0u;
+        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            uint64_t bb = __ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = off[d] + rank;
+            for (int k = 0; k < w; k++) pos += wcnt[k][d];
+            B.kout[pos] = key;
+            B.vout[pos] = val;
+        }
+        __syncthreads();
+        off[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        for (int k = 0; k < 4; k++) wcnt[k][t] = 0;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- clusters
+
+// After the sort (result in buffer `parity`): the points in cluster order,
+// and the number of cluster starts in each block's segment.
+__global__ __launch_bounds__(256) void k_cluster_count(const Frame *frames, int parity) {
+    const Frame F = frames[blockIdx.y];
+    uint32_t n = F.counters[kNPts];
+    const uint64_t *key = parity ? F.skey2 : F.skey;
+    const uint32_t *val = parity ? F.sval2 : F.sval;
+    uint32_t s0, s1;
+    seg_of(n, blockIdx.x, &s0, &s1);
+    uint32_t c = 0;
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) {
+        F.cpts[i] = F.pts[val[i]];
+        c += (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+    }
+    uint32_t tot;
+    block_excl_scan(c, &tot);
+    if (threadIdx.x == 0) F.rhist[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames, int parity) {
+    const Frame F = frames[blockIdx.y];
+    uint32_t n = F.counters[kNPts];
+    const uint64_t *key = parity ? F.skey2 : F.skey;
+    uint32_t base = 0, all = 0;
+    for (int b = 0; b < kSortBlocks; b++) {
+        uint32_t v = F.rhist[b];
+        if (b < (int)blockIdx.x) base += v;
+        all += v;
+    }
+    uint32_t s0, s1;
+    seg_of(n, blockIdx.x, &s0, &s1);
+    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
+        uint32_t i = c0 + threadIdx.x;
+        uint32_t f = (i < s1 && (i == 0 || key[i] != key[i - 1])) ? 1u : 0u;
+        uint32_t tot;
+        uint32_t pre = block_excl_scan(f, &tot);
+        if (f) F.cstart[base + pre] = i;
+        base += tot;
+    }
+    if (blockIdx.x == kSortBlocks - 1 && threadIdx.x == 0) {
+        F.counters[kNClusters] = all;
+        F.cstart[all] = n;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity) {
+    for (int shift = 0; shift < nbits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
+                           *parity);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
+                           *parity);
+        *parity ^= 1;
+    }
+}
+
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits) {
+    dim3 g((unsigned)((max_qh + 3) / 4), nframes);
+    hipLaunchKernelGGL(k_point_rows, g, dim3(256), 0, st, dframes, 0);
+    hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_point_rows, g, dim3(256), 0, st, dframes, 1);
+    int parity = 0;
+    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity);
+    hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, parity);
+    hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, parity);
+}
+
+}  // namespace atg

@@ -1,0 +1,723 @@
+// Stage (c, sort keys) and (d): fit a quad to each cluster.
+//
+//   k_fit_prep     one block per cluster: size and bounding-box rejects,
+//                  the angle key of every point (fast/quads.c:438-453), the
+//                  gradient dot product in the reference's 8-lane order
+//                  (:467-512), the border polarity; writes (cluster, slope)
+//                  sort items in reverse point order.
+//   radix sort     all clusters' items at once: stable LSD by (cluster,
+//                  slope), so equal slopes come out in descending point
+//                  index = ascending complemented index: the full-key order.
+//   k_fit_moments  the full keys; the leaf-block fix of runs of equal slope
+//                  (the order of the reference's merge sort, :42-46,
+//                  :241-327), the point weights and the six cumulative
+//                  moment sums, sequential in angle order as the reference
+//                  adds them (:1006-1035), staged through LDS.
+//   k_fit_corners  window errors, the 7-tap filter, maxima, the maxima
+//                  threshold, pair line fits and the best 4 corners, the
+//                  four lines, corners, area and convexity (:539-959,
+//                  :1044-1126).
+//   k_quad_order   the fitted quads back into cluster order.
+//
+// Floating point follows the reference's operation order exactly; the
+// file is compiled without FP contraction.
+#include <hip/hip_runtime.h>
+
+#include "atg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace atg {
+
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+
+__device__ __forceinline__ uint32_t wscan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t bscan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t ws[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wscan(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int i = 0; i < 4; i++) {
+        if (i < w) pre += ws[i];
+        tot += ws[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+// slope_sort_key (fast/quads.c:429-433)
+__device__ __forceinline__ uint32_t slope_key(float s) {
+    uint32_t b = f2u(s), sg = (uint32_t)((int32_t)b >> 31);
+    return b ^ (sg | 0x80000000u);
+}
+
+// The high word of point_key (fast/quads.c:438-453).
+__device__ __forceinline__ uint32_t point_slope(float dx, float dy) {
+    float quadrant = dy > 0.0f ? (dx > 0.0f ? 65536.0f : 131072.0f) : (dx > 0.0f ? 0.0f : -65536.0f);
+    uint32_t neg = dy < 0.0f ? 0x80000000u : 0u;
+    float fx = u2f(f2u(dx) ^ neg), fy = u2f(f2u(dy) ^ neg);
+    float ndx = fx < 0.0f ? fy : fx;
+    float ndy = fx < 0.0f ? -fx : fy;
+    return slope_key(__fadd_rn(quadrant, __fdiv_rn(ndy, ndx)));
+}
+
+__device__ __forceinline__ void pt_unpack(uint64_t p, int *x, int *y, int *gx, int *gy) {
+    *x = (int)(p & 0xffffu);
+    *y = (int)((p >> 16) & 0xffffu);
+    *gx = (int)(int16_t)(uint16_t)((p >> 32) & 0xffffu);
+    *gy = (int)(int16_t)(uint16_t)(p >> 48);
+}
+
+// ------------------------------------------------------------ prep
+
+constexpr int kDotChunk = 4096;
+
+__global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const DevParams *P) {
+    __shared__ float terms[kDotChunk];
+    __shared__ uint32_t red[4][4];
+    __shared__ float lanes8[8];
+    __shared__ int s_ok;
+    const Frame F = frames[blockIdx.y];
+    const uint32_t ncl = F.counters[kNClusters];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
+        const uint64_t *pts = F.cpts + s0;
+        bool in_range = (int64_t)n >= P->mcp && (int64_t)n <= F.max_px;
+        // bounding box (fast/quads.c:515-528)
+        uint32_t xmin = 0xffff, xmax = 0, ymin = 0xffff, ymax = 0;
+        if (in_range)
+            for (uint32_t i = t; i < n; i += 256) {
+                int x, y, gx, gy;
+                pt_unpack(pts[i], &x, &y, &gx, &gy);
+                xmin = min(xmin, (uint32_t)x); xmax = max(xmax, (uint32_t)x);
+                ymin = min(ymin, (uint32_t)y); ymax = max(ymax, (uint32_t)y);
+            }
+        for (int d = 32; d >= 1; d >>= 1) {
+            xmin = min(xmin, (uint32_t)__shfl_xor(xmin, d, 64)); xmax = max(xmax, (uint32_t)__shfl_xor(xmax, d, 64));
+            ymin = min(ymin, (uint32_t)__shfl_xor(ymin, d, 64)); ymax = max(ymax, (uint32_t)__shfl_xor(ymax, d, 64));
+        }
+        if (lane == 0) { red[w][0] = xmin; red[w][1] = xmax; red[w][2] = ymin; red[w][3] = ymax; }
+        __syncthreads();
+        xmin = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
+        xmax = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
+        ymin = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
+        ymax = max(max(red[0][3], red[1][3]), max(red[2][3], red[3][3]));
+        __syncthreads();
+        bool ok = in_range && ((int64_t)xmax - (int64_t)xmin) * ((int64_t)ymax - (int64_t)ymin) >= P->tag_width;
+        float cx = (float)((double)(xmin + xmax) * 0.5 + 0.05118);
+        float cy = (float)((double)(ymin + ymax) * 0.5 + -0.028581);
+        // sort items (reverse point order) and the full keys of the points
+        for (uint32_t i = t; i < n; i += 256) {
+            uint32_t slope = 0;
+            if (ok) {
+                int x, y, gx, gy;
+                pt_unpack(pts[i], &x, &y, &gx, &gy);
+                slope = point_slope(__fsub_rn((float)x, cx), __fsub_rn((float)y, cy));
+            }
+            F.okey[s0 + i] = ((uint64_t)slope << 32) | (uint32_t)~i;
+            uint32_t pos = s0 + (n - 1 - i);
+            F.skey[pos] = ((uint64_t)c << 32) | slope;
+            F.sval[pos] = i;
+        }
+        if (!ok) {
+            if (t == 0) F.cflag[c] = 0;
+            continue;
+        }
+        // dot product: lane l of 8 sums the terms of points l, l+8, ...
+        // below n - n % 8 in order (fast/quads.c:490-504)
+        const uint32_t nv = n - n % 8;
+        float acc = 0.0f;
+        for (uint32_t c0 = 0; c0 < nv; c0 += kDotChunk) {
+            uint32_t len = min((uint32_t)kDotChunk, nv - c0);
+            for (uint32_t j = t; j < len; j += 256) {
+                int x, y, gx, gy;
+                pt_unpack(pts[c0 + j], &x, &y, &gx, &gy);
+                float dx = __fsub_rn((float)x, cx), dy = __fsub_rn((float)y, cy);
+                terms[j] = __fadd_rn(__fmul_rn(dx, (float)gx), __fmul_rn(dy, (float)gy));
+            }
+            __syncthreads();
+            if (t < 8)
+                for (uint32_t j = t; j < len; j += 8) acc = __fadd_rn(acc, terms[j]);
+            __syncthreads();
+        }
+        if (t < 8) lanes8[t] = acc;
+        __syncthreads();
+        if (t == 0) {
+            float dot = __fadd_rn(__fadd_rn(__fadd_rn(lanes8[0], lanes8[4]), __fadd_rn(lanes8[2], lanes8[6])),
+                                  __fadd_rn(__fadd_rn(lanes8[1], lanes8[5]), __fadd_rn(lanes8[3], lanes8[7])));
+            for (uint32_t i = nv; i < n; i++) {
+                int x, y, gx, gy;
+                pt_unpack(pts[i], &x, &y, &gx, &gy);
+                float dx = __fsub_rn((float)x, cx), dy = __fsub_rn((float)y, cy);
+                dot = __fadd_rn(dot, __fadd_rn(__fmul_rn(dx, (float)gx), __fmul_rn(dy, (float)gy)));
+            }
+            bool rev = dot < 0.0f;
+            bool keep = !((!P->reversed_border && rev) || (!P->normal_border && !rev));
+            F.cflag[c] = keep ? (1u | (rev ? 2u : 0u)) : 0u;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ sort fix
+
+__device__ __forceinline__ void kswap(uint64_t *k, int a, int b) {
+    if ((k[a] >> 32) > (k[b] >> 32)) {
+        uint64_t x = k[a];
+        k[a] = k[b];
+        k[b] = x;
+    }
+}
+
+// The compare-exchange networks of the merge sort's leaves, high words
+// only (fast/quads.c:56-85).
+__device__ void leaf_network(uint64_t *k, int n) {
+    if (n <= 1) return;
+    if (n == 2) { kswap(k, 0, 1); return; }
+    if (n == 3) { kswap(k, 0, 1); kswap(k, 1, 2); kswap(k, 0, 1); return; }
+    if (n == 4) { kswap(k, 0, 1); kswap(k, 2, 3); kswap(k, 0, 2); kswap(k, 1, 3); kswap(k, 1, 2); return; }
+    kswap(k, 0, 1); kswap(k, 3, 4); kswap(k, 1, 2); kswap(k, 0, 1); kswap(k, 0, 3);
+    kswap(k, 2, 4); kswap(k, 1, 2); kswap(k, 2, 3); kswap(k, 1, 2);
+}
+
+// The merge sort's leaf holding position p of n (fast/quads.c:215-237):
+// split in four while half the range exceeds 5, then in halves.
+__device__ void leaf_of(int64_t n, int64_t p, int64_t *lo_out, int64_t *len_out) {
+    int64_t lo = 0, sz = n;
+    while (sz / 2 > 5) {
+        int64_t asz = sz / 2, a1 = asz / 2, b1 = asz + (sz - asz) / 2, r = p - lo;
+        if (r < a1) sz = a1;
+        else if (r < asz) { lo += a1; sz = asz - a1; }
+        else if (r < b1) { lo += asz; sz = b1 - asz; }
+        else { lo += b1; sz = sz - b1; }
+    }
+    while (sz > 5) {
+        int64_t asz = sz / 2;
+        if (p - lo >= asz) { lo += asz; sz -= asz; }
+        else sz = asz;
+    }
+    *lo_out = lo;
+    *len_out = sz;
+}
+
+__device__ __forceinline__ int64_t kidx(uint64_t k) { return (int64_t)(uint32_t)~(uint32_t)k; }
+
+// One run of equal slopes (r, full-key order) into the merge sort's order:
+// the leaf keys of the run ordered by the first key of their block
+// (a key larger than all before it in its leaf), then position in the
+// leaf (fast/quads.c:241-302).
+__device__ void order_run(const uint64_t *orig, int64_t n, uint64_t *r, int64_t nr, uint64_t *ent, uint64_t *bh) {
+    uint64_t hi = r[0] & 0xffffffff00000000ull;
+    int64_t m = 0, k = 0;
+    while (k < nr) {
+        if (k + 1 == nr || kidx(r[k]) - kidx(r[k + 1]) >= 5) {
+            ent[m] = r[k];
+            bh[m] = r[k];
+            m++;
+            k++;
+            continue;
+        }
+        int64_t lo, len;
+        leaf_of(n, kidx(r[k]), &lo, &len);
+        int64_t k1 = k + 1;
+        while (k1 < nr && kidx(r[k1]) >= lo) k1++;
+        if (k1 == k + 1) {
+            ent[m] = r[k];
+            bh[m] = r[k];
+            m++;
+        } else {
+            uint64_t lf[5] = {0, 0, 0, 0, 0};
+            for (int64_t q = 0; q < len; q++) lf[q] = orig[lo + q];
+            leaf_network(lf, (int)len);
+            uint64_t top = 0;
+            bool first = true;
+            for (int64_t q = 0; q < len; q++) {
+                uint64_t key = lf[q];
+                if ((key & 0xffffffff00000000ull) != hi) continue;
+                if (first || key > top) top = key;
+                first = false;
+                ent[m] = key;
+                bh[m] = top;
+                m++;
+            }
+        }
+        k = k1;
+    }
+    for (int64_t q = 1; q < m; q++) {
+        uint64_t e = ent[q], b = bh[q];
+        int64_t j = q;
+        while (j > 0 && bh[j - 1] > b) {
+            ent[j] = ent[j - 1];
+            bh[j] = bh[j - 1];
+            j--;
+        }
+        ent[j] = e;
+        bh[j] = b;
+    }
+    for (int64_t q = 0; q < m; q++) r[q] = ent[q];
+}
+
+__device__ void fix_runs(uint64_t *res, int64_t n, const uint64_t *orig, uint64_t *ent, uint64_t *bh) {
+    int64_t i = 1;
+    while (i < n) {
+        if (res[i] - res[i - 1] >= 5) { i++; continue; }
+        uint64_t x = res[i - 1];
+        if ((x >> 32) != (res[i] >> 32)) { i++; continue; }
+        int64_t s = i - 1;
+        while (s > 0 && (res[s - 1] >> 32) == (x >> 32)) s--;
+        int64_t e = i + 1;
+        while (e < n && (res[e] >> 32) == (x >> 32)) e++;
+        order_run(orig, n, res + s, e - s, ent, bh);
+        i = e + 1;
+    }
+}
+
+// ------------------------------------------------------------ moments
+
+constexpr int kMomChunk = 512;
+
+__global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int parity) {
+    __shared__ double sm[6][kMomChunk];
+    __shared__ int s_fix;
+    const Frame F = frames[blockIdx.y];
+    const uint32_t ncl = F.counters[kNClusters];
+    const int t = threadIdx.x;
+    uint64_t *skey = parity ? F.skey2 : F.skey;
+    const uint32_t *sval = parity ? F.sval2 : F.sval;
+    const int64_t pc = F.pcap;
+    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        if (!(F.cflag[c] & 1u)) continue;
+        const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
+        uint64_t *keys = skey + s0;
+        if (t == 0) s_fix = 0;
+        __syncthreads();
+        // the full keys in order: slope << 32 | ~index
+        for (uint32_t k = t; k < n; k += 256) keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
+        __syncthreads();
+        for (uint32_t k = 1 + t; k < n; k += 256)
+            if (keys[k] - keys[k - 1] < 5 && (keys[k] >> 32) == (keys[k - 1] >> 32)) s_fix = 1;
+        __syncthreads();
+        if (s_fix && t == 0)
+            fix_runs(keys, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
+                     reinterpret_cast<uint64_t *>(F.filt + s0));
+        __syncthreads();
+        // weights (fast/quads.c:968-1004) and the six running sums
+        double run[6] = {0, 0, 0, 0, 0, 0};
+        const int iw = F.qw, ih = F.qh;
+        for (uint32_t c0 = 0; c0 < n; c0 += kMomChunk) {
+            uint32_t len = min((uint32_t)kMomChunk, n - c0);
+            for (uint32_t j = t; j < len; j += 256) {
+                uint32_t src = (uint32_t)~(uint32_t)keys[c0 + j];
+                int px, py, gx, gy;
+                pt_unpack(F.cpts[s0 + src], &px, &py, &gx, &gy);
+                int ix = (px >> 1) + (px & 1), iy = (py >> 1) + (py & 1);
+                int g2 = 0;
+                if (ix > 0 && ix + 1 < iw && iy > 0 && iy + 1 < ih) {
+                    const uint8_t *im = F.qim + (int64_t)iy * iw + ix;
+                    int ggx = (int)im[1] - (int)im[-1], ggy = (int)im[iw] - (int)im[-iw];
+                    g2 = ggx * ggx + ggy * ggy;
+                }
+                double wgt = __dadd_rn(__dsqrt_rn((double)g2), 1.0);
+                double fx = __dadd_rn(__dmul_rn((double)px, 0.5), 0.5);
+                double fy = __dadd_rn(__dmul_rn((double)py, 0.5), 0.5);
+                double wfx = __dmul_rn(wgt, fx), wfy = __dmul_rn(wgt, fy);
+                sm[0][j] = wfx;
+                sm[1][j] = wfy;
+                sm[2][j] = __dmul_rn(wfx, fx);
+                sm[3][j] = __dmul_rn(wfx, fy);
+                sm[4][j] = __dmul_rn(wfy, fy);
+                sm[5][j] = wgt;
+            }
+            __syncthreads();
+            if (t < 6) {
+                double s = run[t];
+                double *a = sm[t];
+                for (uint32_t j = 0; j < len; j++) {
+                    s = __dadd_rn(s, a[j]);
+                    a[j] = s;
+                }
+                run[t] = s;
+            }
+            __syncthreads();
+            for (uint32_t j = t; j < len; j += 256)
+                for (int q = 0; q < 6; q++) F.mom[q * pc + s0 + c0 + j] = sm[q][j];
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------------------------------------ corners
+
+struct Mom {
+    const double *a[6];  // mx, my, mxx, mxy, myy, w (cluster-relative)
+};
+
+// The window sums of [i0, i1], wrapping, as fit_line forms them
+// (fast/quads.c:539-572).
+__device__ __forceinline__ void range_sums(const Mom &M, int64_t sz, int64_t i0, int64_t i1, double s[6]) {
+    for (int q = 0; q < 6; q++) {
+        const double *v = M.a[q];
+        if (i0 < i1) {
+            double x = v[i1];
+            if (i0 > 0) x = __dsub_rn(x, v[i0 - 1]);
+            s[q] = x;
+        } else {
+            s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 - 1]), v[i1]);
+        }
+    }
+}
+
+// fit_line (fast/quads.c:539-600) from the sums: line (ex, ey, nx, ny), mse.
+__device__ void line_from_sums(const double s[6], double lp[4], double *mse) {
+    double w = s[5];
+    double ex = __ddiv_rn(s[0], w), ey = __ddiv_rn(s[1], w);
+    double cxx = __dsub_rn(__ddiv_rn(s[2], w), __dmul_rn(ex, ex));
+    double cxy = __dsub_rn(__ddiv_rn(s[3], w), __dmul_rn(ex, ey));
+    double cyy = __dsub_rn(__ddiv_rn(s[4], w), __dmul_rn(ey, ey));
+    double dd = __dsub_rn(cxx, cyy);
+    double arg = __dadd_rn(__dmul_rn(dd, dd), __dmul_rn(__dmul_rn(4.0, cxy), cxy));
+    double root = (double)__fsqrt_rn((float)arg);
+    double es = __dmul_rn(0.5, __dsub_rn(__dadd_rn(cxx, cyy), root));
+    double eig = __dmul_rn(0.5, __dadd_rn(__dadd_rn(cxx, cyy), root));
+    double nx1 = __dsub_rn(cxx, eig), ny1 = cxy, m1 = __dadd_rn(__dmul_rn(nx1, nx1), __dmul_rn(ny1, ny1));
+    double nx2 = cxy, ny2 = __dsub_rn(cyy, eig), m2 = __dadd_rn(__dmul_rn(nx2, nx2), __dmul_rn(ny2, ny2));
+    double nx = nx2, ny = ny2, m = m2;
+    if (m1 > m2) { nx = nx1; ny = ny1; m = m1; }
+    double len = (double)__fsqrt_rn((float)m);
+    lp[0] = ex;
+    lp[1] = ey;
+    lp[2] = 0.0;
+    lp[3] = 0.0;
+    if (!(fabs(len) < 1e-12)) {
+        lp[2] = __ddiv_rn(nx, len);
+        lp[3] = __ddiv_rn(ny, len);
+    }
+    *mse = es;
+}
+
+// The window error of window_errors / moment_errors (fast/quads.c:604-652).
+__device__ __forceinline__ double win_err(const double s[6], double nn) {
+    double w = s[5];
+    double ex = __ddiv_rn(s[0], w), ey = __ddiv_rn(s[1], w);
+    double cxx = __dsub_rn(__ddiv_rn(s[2], w), __dmul_rn(ex, ex));
+    double cxy = __dsub_rn(__ddiv_rn(s[3], w), __dmul_rn(ex, ey));
+    double cyy = __dsub_rn(__ddiv_rn(s[4], w), __dmul_rn(ey, ey));
+    double d = __dsub_rn(cxx, cyy);
+    double root = (double)__fsqrt_rn((float)__dadd_rn(__dmul_rn(d, d), __dmul_rn(4.0, __dmul_rn(cxy, cxy))));
+    return __dmul_rn(nn, __dmul_rn(0.5, __dsub_rn(__dadd_rn(cxx, cyy), root)));
+}
+
+// order-preserving integer of a double
+__device__ __forceinline__ uint64_t dkey(double d) {
+    uint64_t b = (uint64_t)__double_as_longlong(d);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+constexpr int kMaxNm = 32;
+
+__global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const DevParams *P) {
+    __shared__ double pe[kMaxNm * kMaxNm], pm[kMaxNm * kMaxNm], pnx[kMaxNm * kMaxNm], pny[kMaxNm * kMaxNm];
+    __shared__ uint32_t mx[kMaxNm];
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t s_prefix;
+    __shared__ uint32_t s_rank, s_nm;
+    __shared__ double s_best[256];
+    __shared__ uint32_t s_bord[256];
+    const Frame F = frames[blockIdx.y];
+    const uint32_t ncl = F.counters[kNClusters];
+    const int t = threadIdx.x;
+    const int64_t pc = F.pcap;
+    const double max_mse = (double)P->max_mse, max_dot = (double)P->cos_crit;
+    const double INF = __longlong_as_double(0x7ff0000000000000ll);
+    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        uint32_t flag = F.cflag[c];
+        if (!(flag & 1u)) continue;
+        const uint32_t s0 = F.cstart[c];
+        const int64_t sz = F.cstart[c + 1] - s0;
+        Mom M;
+        for (int q = 0; q < 6; q++) M.a[q] = F.mom + q * pc + s0;
+        int64_t ksz = min((int64_t)20, sz / 12);
+        if (ksz < 2) continue;
+        double nn = (double)(2 * ksz + 1);
+        double *errs = F.errs + s0, *y = F.filt + s0;
+        // window errors (fast/quads.c:654-686 edges, :604-635 middle)
+        for (int64_t i = t; i < sz; i += 256) {
+            int64_t i0 = i - ksz, i1 = i + ksz;
+            double s[6];
+            for (int q = 0; q < 6; q++) {
+                const double *v = M.a[q];
+                if (i0 < 0) s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 + sz - 1]), v[i1]);
+                else if (i1 >= sz) s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 - 1]), v[i1 - sz]);
+                else if (i0 > 0) s[q] = __dsub_rn(v[i1], v[i0 - 1]);
+                else s[q] = v[i1];
+            }
+            errs[i] = win_err(s, nn);
+        }
+        __syncthreads();
+        // 7-tap filter, wrapping (fast/quads.c:706-735)
+        for (int64_t i = t; i < sz; i += 256) {
+            double acc;
+            if (i >= 3 && i < sz - 3) {
+                acc = __dmul_rn(errs[i - 3], P->qsm_kern[0]);
+                for (int k = 1; k < 7; k++) acc = __dadd_rn(acc, __dmul_rn(errs[i - 3 + k], P->qsm_kern[k]));
+            } else {
+                acc = 0.0;
+                int64_t j = i - 3;
+                if (j < 0) j += sz;
+                for (int k = 0; k < 7; k++) {
+                    acc = __dadd_rn(acc, __dmul_rn(errs[j], P->qsm_kern[k]));
+                    j++;
+                    if (j == sz) j = 0;
+                }
+            }
+            y[i] = acc;
+        }
+        __syncthreads();
+        // cyclic local maxima, in order (fast/quads.c:805-823)
+        uint32_t base = 0;
+        for (int64_t c0 = 0; c0 < sz; c0 += 256) {
+            int64_t i = c0 + t;
+            uint32_t f = 0;
+            if (i < sz) {
+                double e = y[i], l = y[i == 0 ? sz - 1 : i - 1], r = y[i == sz - 1 ? 0 : i + 1];
+                f = (e > l && e > r) ? 1u : 0u;
+            }
+            uint32_t tot;
+            uint32_t pre = bscan(f, &tot);
+            if (f) F.midx[s0 + base + pre] = (uint32_t)i;
+            base += tot;
+        }
+        uint32_t nm = base;
+        __syncthreads();
+        if (nm < 4) continue;
+        const uint32_t *midx = F.midx + s0;
+        if ((int64_t)nm > P->max_nmaxima) {
+            // the (max_nmaxima+1)-th largest maximum error: radix select
+            // over the order-preserving bits (fast/quads.c:826-862 finds
+            // the same value by quickselect)
+            uint32_t rank = (uint32_t)P->max_nmaxima;  // 0-based, descending
+            uint64_t prefix = 0;
+            for (int shift = 56; shift >= 0; shift -= 8) {
+                hist[t] = 0;
+                __syncthreads();
+                uint64_t hmask = shift == 56 ? 0ull : (~0ull << (shift + 8));
+                for (uint32_t i = t; i < nm; i += 256) {
+                    uint64_t k = dkey(y[midx[i]]);
+                    if ((k & hmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+                }
+                __syncthreads();
+                if (t == 0) {
+                    uint32_t r = rank;
+                    int d = 255;
+                    for (; d > 0; d--) {
+                        if (r < hist[d]) break;
+                        r -= hist[d];
+                    }
+                    s_prefix = prefix | ((uint64_t)d << shift);
+                    s_rank = r;
+                }
+                __syncthreads();
+                prefix = s_prefix;
+                rank = s_rank;
+                __syncthreads();
+            }
+            uint64_t thr = prefix;
+            // keep the maxima above it, in order
+            uint32_t o = 0;
+            for (uint32_t c0 = 0; c0 < nm; c0 += 256) {
+                uint32_t i = c0 + t;
+                uint32_t f = 0, v = 0;
+                if (i < nm) {
+                    v = midx[i];
+                    f = dkey(y[v]) > thr ? 1u : 0u;
+                }
+                uint32_t tot;
+                uint32_t pre = bscan(f, &tot);
+                if (f && o + pre < (uint32_t)kMaxNm) mx[o + pre] = v;
+                o += tot;
+            }
+            nm = o;
+        } else {
+            for (uint32_t i = t; i < nm && i < (uint32_t)kMaxNm; i += 256) mx[i] = midx[i];
+        }
+        __syncthreads();
+        if (nm > (uint32_t)kMaxNm) {
+            if (t == 0) atomicOr(F.counters + kError, 4u);
+            continue;
+        }
+        // pair fits (fast/quads.c:869-928): (a, b) for a < b, closing (b, a) for b >= a + 3
+        for (uint32_t pq = t; pq < nm * nm; pq += 256) {
+            uint32_t p = pq / nm, q = pq % nm;
+            bool fwd = p < q, back = q + 3 <= p;
+            if (!fwd && !back) continue;
+            int64_t i0 = mx[p], i1 = mx[q];
+            double s[6], lp[4], mse;
+            range_sums(M, sz, i0, i1, s);
+            line_from_sums(s, lp, &mse);
+            double cnt = fwd ? (double)(i1 - i0 + 1) : (double)(sz - i0 + i1 + 1);
+            pe[pq] = __dmul_rn(cnt, mse);
+            pm[pq] = mse;
+            pnx[pq] = lp[2];
+            pny[pq] = lp[3];
+        }
+        __syncthreads();
+        // the first minimum over (m0, m1, m2, m3) in loop order (:930-956)
+        double best = INF;
+        uint32_t bord = 0xffffffffu;
+        const uint32_t ntri = nm * nm * nm;
+        for (uint32_t tri = t; tri < ntri; tri += 256) {
+            uint32_t m0 = tri / (nm * nm), m1 = (tri / nm) % nm, m2 = tri % nm;
+            if (!(m0 < m1 && m1 < m2 && m2 + 1 < nm)) continue;
+            uint32_t p01 = m0 * nm + m1, p12 = m1 * nm + m2;
+            if (pm[p01] > max_mse || pm[p12] > max_mse) continue;
+            double dot = __dadd_rn(__dmul_rn(pnx[p01], pnx[p12]), __dmul_rn(pny[p01], pny[p12]));
+            if (fabs(dot) > max_dot) continue;
+            double e012 = __dadd_rn(pe[p01], pe[p12]);
+            for (uint32_t m3 = m2 + 1; m3 < nm; m3++) {
+                uint32_t p23 = m2 * nm + m3, p30 = m3 * nm + m0;
+                double f23 = pm[p23] > max_mse ? INF : pe[p23];
+                double f30 = pm[p30] > max_mse ? INF : pe[p30];
+                double err = __dadd_rn(__dadd_rn(e012, f23), f30);
+                uint32_t ord = tri * nm + m3;
+                if (err < best) {
+                    best = err;
+                    bord = ord;
+                }
+            }
+        }
+        s_best[t] = best;
+        s_bord[t] = bord;
+        __syncthreads();
+        if (t == 0) {
+            double b = INF;
+            uint32_t o = 0xffffffffu;
+            for (int k = 0; k < 256; k++) {
+                double e = s_best[k];
+                if (e < b || (e == b && e < INF && s_bord[k] < o)) {
+                    b = e;
+                    o = s_bord[k];
+                }
+            }
+            bool ok = b < INF && __ddiv_rn(b, (double)sz) < max_mse;
+            int64_t idx[4] = {0, 0, 0, 0};
+            if (ok) {
+                uint32_t m3 = o % nm, tri = o / nm;
+                uint32_t m0 = tri / (nm * nm), m1 = (tri / nm) % nm, m2 = tri % nm;
+                idx[0] = mx[m0];
+                idx[1] = mx[m1];
+                idx[2] = mx[m2];
+                idx[3] = mx[m3];
+            }
+            double lines[4][4];
+            for (int i = 0; ok && i < 4; i++) {
+                double s[6], mse;
+                range_sums(M, sz, idx[i], idx[(i + 1) & 3], s);
+                line_from_sums(s, lines[i], &mse);
+                if (mse > max_mse) ok = false;
+            }
+            QuadRec q;
+            for (int i = 0; ok && i < 4; i++) {
+                int j = (i + 1) & 3;
+                double a00 = lines[i][3], a01 = -lines[j][3], a10 = -lines[i][2], a11 = lines[j][2];
+                double b0 = __dadd_rn(-lines[i][0], lines[j][0]), b1 = __dadd_rn(-lines[i][1], lines[j][1]);
+                double det = __dsub_rn(__dmul_rn(a00, a11), __dmul_rn(a10, a01));
+                if (fabs(det) < 0.001) { ok = false; break; }
+                double w00 = __ddiv_rn(a11, det), w01 = __ddiv_rn(-a01, det);
+                double l0 = __dadd_rn(__dmul_rn(w00, b0), __dmul_rn(w01, b1));
+                q.p[i][0] = (float)__dadd_rn(lines[i][0], __dmul_rn(l0, a00));
+                q.p[i][1] = (float)__dadd_rn(lines[i][1], __dmul_rn(l0, a10));
+            }
+            if (ok) {
+                // area (two Heron triangles) and convexity (:1094-1124);
+                // coordinate differences in float
+                auto sq = [](double v) { return __dmul_rn(v, v); };
+                auto fd = [](float a, float b) { return (double)__fsub_rn(a, b); };
+                double len[3], area = 0.0;
+                for (int i = 0; i < 3; i++) {
+                    int ia = i, ib = (i + 1) % 3;
+                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(q.p[ib][0], q.p[ia][0])), sq(fd(q.p[ib][1], q.p[ia][1]))));
+                }
+                double pp = __ddiv_rn(__dadd_rn(__dadd_rn(len[0], len[1]), len[2]), 2.0);
+                area = __dadd_rn(area, __dsqrt_rn(__dmul_rn(__dmul_rn(__dmul_rn(pp, __dsub_rn(pp, len[0])),
+                                                                      __dsub_rn(pp, len[1])),
+                                                            __dsub_rn(pp, len[2]))));
+                const int idxs[4] = {2, 3, 0, 2};
+                for (int i = 0; i < 3; i++) {
+                    int ia = idxs[i], ib = idxs[i + 1];
+                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(q.p[ib][0], q.p[ia][0])), sq(fd(q.p[ib][1], q.p[ia][1]))));
+                }
+                pp = __ddiv_rn(__dadd_rn(__dadd_rn(len[0], len[1]), len[2]), 2.0);
+                area = __dadd_rn(area, __dsqrt_rn(__dmul_rn(__dmul_rn(__dmul_rn(pp, __dsub_rn(pp, len[0])),
+                                                                      __dsub_rn(pp, len[1])),
+                                                            __dsub_rn(pp, len[2]))));
+                double tw = (double)P->tag_width;
+                if (area < __dmul_rn(__dmul_rn(0.95, tw), tw)) ok = false;
+                double ccr = (double)P->cos_crit;
+                for (int i = 0; ok && i < 4; i++) {
+                    int i0 = i, i1 = (i + 1) & 3, i2 = (i + 2) & 3;
+                    double dx1 = fd(q.p[i1][0], q.p[i0][0]), dy1 = fd(q.p[i1][1], q.p[i0][1]);
+                    double dx2 = fd(q.p[i2][0], q.p[i1][0]), dy2 = fd(q.p[i2][1], q.p[i1][1]);
+                    double den = __dsqrt_rn(__dmul_rn(__dadd_rn(__dmul_rn(dx1, dx1), __dmul_rn(dy1, dy1)),
+                                                      __dadd_rn(__dmul_rn(dx2, dx2), __dmul_rn(dy2, dy2))));
+                    if (den == 0.0) { ok = false; break; }
+                    double cd = __ddiv_rn(__dadd_rn(__dmul_rn(dx1, dx2), __dmul_rn(dy1, dy2)), den);
+                    if ((cd > ccr || cd < -ccr) || __dmul_rn(dx1, dy2) < __dmul_rn(dy1, dx2)) ok = false;
+                }
+            }
+            if (ok) {
+                q.reversed = (flag & 2u) ? 1 : 0;
+                q.hok = 0;
+                uint32_t slot = atomicAdd(F.counters + kNQuads, 1u);
+                if (slot < (uint32_t)kMaxQuadsPerFrame) {
+                    F.qtmp[slot] = q;
+                    F.qcl[slot] = c;
+                } else {
+                    atomicOr(F.counters + kError, 8u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The fitted quads in cluster order (one block per frame): rank of each
+// by its cluster index.
+__global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    uint32_t nq = min(F.counters[kNQuads], (uint32_t)kMaxQuadsPerFrame);
+    for (uint32_t i = threadIdx.x; i < nq; i += 256) {
+        uint32_t ci = F.qcl[i], r = 0;
+        for (uint32_t j = 0; j < nq; j++) r += F.qcl[j] < ci ? 1u : 0u;
+        F.quads[r] = F.qtmp[i];
+    }
+    if (threadIdx.x == 0) F.counters[kNQuads] = nq;
+}
+
+void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity);
+
+void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
+                        int fit_blocks) {
+    dim3 g(fit_blocks, nframes);
+    hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams);
+    int parity = 0;
+    launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);
+    hipLaunchKernelGGL(k_fit_moments, g, dim3(256), 0, st, dframes, parity);
+    hipLaunchKernelGGL(k_fit_corners, g, dim3(256), 0, st, dframes, dparams);
+    hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
+}
+
+}  // namespace atg

@@ -1,0 +1,158 @@
+/* atg: the thin C-ABI of the MI355X AprilTag detection path.
+ *
+ * Plain pointers and sizes only. The native API (apriltag_fast.h) is a
+ * layer over this one; the batched and serving forms are entry points
+ * beside the single-frame one, not changes to it.
+ *
+ * Reference interfaces each entry point replaces or extends:
+ *   atg_ctx_create/destroy  - the device half of at_detector_create /
+ *                             at_detector_destroy (fast/detector.c:4-43)
+ *   atg_set_families        - at_add_family / at_clear_families
+ *                             (fast/detector.c:45-60): uploads the
+ *                             quick-decode tables (fast/decode.c:20-58)
+ *   atg_detect              - at_detect (fast/detector.c:246-335) for one
+ *                             frame, in host or device memory
+ *   atg_detect_batch        - no counterpart: many frames of any sizes in
+ *                             one pass of the kernel chain
+ *   atg_debug_stages        - no counterpart: the stage arrays of the
+ *                             last call, in the layout of the reference's
+ *                             quad_thresh_single intermediates
+ *                             (fast/detector.c:198-244)
+ *
+ * Errors are negative return codes; atg_last_error() describes the last
+ * one. All calls on one context must come from one thread at a time. */
+#ifndef ATG_H
+#define ATG_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#include "apriltag_fast.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATG_OK 0
+#define ATG_ERR_ARG -1     /* bad argument */
+#define ATG_ERR_HIP -2     /* a HIP call failed */
+#define ATG_ERR_LIMIT -3   /* an image beyond the supported size */
+#define ATG_ERR_NOFAM -4   /* no families (the reference returns nothing) */
+
+typedef struct atg_ctx atg_ctx;
+
+/* The detection parameters (the parameter head of at_detector,
+ * fast/apriltag_fast.h:73-87). */
+typedef struct {
+    float quad_decimate;
+    float quad_sigma;
+    bool refine_edges;
+    double decode_sharpening;
+    int32_t min_cluster_pixels;
+    int32_t max_nmaxima;
+    float cos_critical_rad;
+    float max_line_fit_mse;
+    int32_t min_white_black_diff;
+    bool deglitch;
+} atg_params;
+
+typedef struct {
+    const uint8_t *pixels;
+    int64_t width, height, stride;
+} atg_frame;
+
+/* The detections of one frame: dets points into the context (valid until
+ * the next detect call on it), sorted by id as at_detect's. */
+typedef struct {
+    const at_detection *dets;
+    int64_t ndets;
+    int64_t nquads;
+} atg_result;
+
+/* Stage arrays of one frame, malloc'd (free with atg_stages_free). The
+ * layout matches the stage dump of the reference harness. */
+typedef struct {
+    int64_t qw, qh;
+    uint8_t *qim;
+    uint8_t *t;
+    int64_t nruns;
+    uint32_t *runs;     /* nruns x 3: start, end, v */
+    uint32_t *row_off;  /* qh + 1 */
+    int64_t nnodes;
+    uint32_t *rep;      /* representative (union-find root) of each node */
+    uint32_t *csize;    /* pixel count of each node's component */
+    uint8_t *ok;
+    int64_t nclusters;
+    uint64_t *cluster_ids;
+    uint32_t *ccount;
+    int64_t npts;
+    uint64_t *pts;      /* grouped by cluster, clusters in output order */
+    int64_t nquads;
+    float *quads;       /* nquads x 9: p[4][2] on the quad image, reversed flag */
+} atg_stages;
+
+void atg_params_default(atg_params *p);
+
+/* device: HIP device ordinal. Returns ATG_OK and *out, or an error. */
+int atg_ctx_create(int device, atg_ctx **out);
+void atg_ctx_destroy(atg_ctx *ctx);
+const char *atg_last_error(const atg_ctx *ctx);
+
+/* Replace the context's families: fams[i] with bits[i] bits corrected.
+ * The same validation as at_add_family (fast/detector.c:45-54). */
+int atg_set_families(atg_ctx *ctx, const at_family *const *fams, const int32_t *bits, int32_t n);
+
+/* One frame. on_device != 0: pixels is a device pointer. */
+int atg_detect(atg_ctx *ctx, const atg_params *p, const uint8_t *pixels, int64_t width, int64_t height,
+               int64_t stride, int on_device, atg_result *out);
+
+/* n frames of any sizes through one pass of the kernel chain. out[i] is
+ * what atg_detect gives for frames[i] alone. */
+int atg_detect_batch(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device,
+                     atg_result *out);
+
+/* Stage arrays of frame `frame` of the last detect call. */
+int atg_debug_stages(atg_ctx *ctx, int32_t frame, atg_stages *out);
+void atg_stages_free(atg_stages *s);
+
+/* Device time (ms, HIP events on the context's stream) of the last
+ * detect call's kernel chain, and of its longest single-kernel stage. */
+int atg_last_timing(const atg_ctx *ctx, double *chain_ms);
+
+/* ------------------------------------------------------------------
+ * Seeded synthetic frames (host code), for tests and benchmarks. */
+typedef struct {
+    int32_t width, height;
+    int32_t ntags_min, ntags_max;
+    int32_t side_min, side_max;
+    double max_tilt;        /* tan(tilt/2) bound of each of two tilts */
+    int32_t family_a;       /* index into the built-in families (3 = tag36h11) */
+    int32_t family_b;       /* second family or -1 (8 = tagStandard52h13) */
+    int32_t background;     /* 0: uniform + linear gradient, 1: value noise */
+    int32_t bg_amp;
+    double blur_min, blur_max;
+    double noise_min, noise_max;
+    int32_t nbars, nclutter, ncheckers;
+} atg_synth_spec;
+
+typedef struct {
+    int32_t family, id;
+    double corners[8];
+} atg_synth_tag;
+
+void atg_synth_default(atg_synth_spec *s, int config);
+/* Writes a frame into out (stride bytes a row); returns the number of
+ * tags placed (truth gets up to max_truth of them), or -1. */
+int atg_synth_frame(const atg_synth_spec *spec, uint64_t seed, uint8_t *out, int64_t stride, atg_synth_tag *truth,
+                    int32_t max_truth);
+
+/* The built-in families in the order tag16h5, tag25h9, tag36h10,
+ * tag36h11, tagCircle21h7, tagCircle49h12, tagCustom48h12,
+ * tagStandard41h12, tagStandard52h13 (index 0..8), or NULL. */
+const at_family *atg_builtin_family(int32_t index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
