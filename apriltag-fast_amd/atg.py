@@ -1,0 +1,248 @@
+"""Python binding of the MI355X AprilTag detector (ctypes over the C-ABI in
+include/atg.h and include/apriltag_fast.h).
+
+Loads the in-tree libapriltag_amd.so and raises if it is missing: there is
+no CPU fallback.
+
+    ctx = Context()
+    ctx.set_families([("tag36h11", 2)])
+    res = ctx.detect(img)                 # img: uint8 HxW numpy array
+    res["dets"], res["nquads"]
+    outs = ctx.detect_batch([img0, img1]) # frames of any sizes, one pass
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libapriltag_amd.so"
+
+FAMILY_INDEX = {
+    "tag16h5": 0, "tag25h9": 1, "tag36h10": 2, "tag36h11": 3, "tagCircle21h7": 4,
+    "tagCircle49h12": 5, "tagCustom48h12": 6, "tagStandard41h12": 7, "tagStandard52h13": 8,
+}
+
+
+class Family(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("codes", C.POINTER(C.c_uint64)), ("ncodes", C.c_int64),
+                ("nbits", C.c_int32), ("h", C.c_int32), ("width_at_border", C.c_int32),
+                ("total_width", C.c_int32), ("reversed_border", C.c_bool),
+                ("bit_x", C.c_int32 * 64), ("bit_y", C.c_int32 * 64)]
+
+
+class Params(C.Structure):
+    _fields_ = [("quad_decimate", C.c_float), ("quad_sigma", C.c_float), ("refine_edges", C.c_bool),
+                ("decode_sharpening", C.c_double), ("min_cluster_pixels", C.c_int32),
+                ("max_nmaxima", C.c_int32), ("cos_critical_rad", C.c_float),
+                ("max_line_fit_mse", C.c_float), ("min_white_black_diff", C.c_int32),
+                ("deglitch", C.c_bool)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("pixels", C.c_void_p), ("width", C.c_int64), ("height", C.c_int64), ("stride", C.c_int64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("dets", C.c_void_p), ("ndets", C.c_int64), ("nquads", C.c_int64)]
+
+
+class Stages(C.Structure):
+    _fields_ = [("qw", C.c_int64), ("qh", C.c_int64), ("qim", C.POINTER(C.c_uint8)),
+                ("t", C.POINTER(C.c_uint8)), ("nruns", C.c_int64), ("runs", C.POINTER(C.c_uint32)),
+                ("row_off", C.POINTER(C.c_uint32)), ("nnodes", C.c_int64),
+                ("rep", C.POINTER(C.c_uint32)), ("csize", C.POINTER(C.c_uint32)),
+                ("ok", C.POINTER(C.c_uint8)), ("nclusters", C.c_int64),
+                ("cluster_ids", C.POINTER(C.c_uint64)), ("ccount", C.POINTER(C.c_uint32)),
+                ("npts", C.c_int64), ("pts", C.POINTER(C.c_uint64)), ("nquads", C.c_int64),
+                ("quads", C.POINTER(C.c_float))]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("ntags_min", C.c_int32),
+                ("ntags_max", C.c_int32), ("side_min", C.c_int32), ("side_max", C.c_int32),
+                ("max_tilt", C.c_double), ("family_a", C.c_int32), ("family_b", C.c_int32),
+                ("background", C.c_int32), ("bg_amp", C.c_int32), ("blur_min", C.c_double),
+                ("blur_max", C.c_double), ("noise_min", C.c_double), ("noise_max", C.c_double),
+                ("nbars", C.c_int32), ("nclutter", C.c_int32), ("ncheckers", C.c_int32)]
+
+
+class SynthTag(C.Structure):
+    _fields_ = [("family", C.c_int32), ("id", C.c_int32), ("corners", C.c_double * 8)]
+
+
+# at_detection (include/apriltag_fast.h), 168 bytes
+DET_DTYPE = np.dtype([("family", "<i4"), ("id", "<i4"), ("hamming", "<i4"), ("margin", "<f4"),
+                      ("H", "<f8", 9), ("c", "<f8", 2), ("p", "<f8", 8)])
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError("libapriltag_amd.so is not built: run apriltag-fast_amd/build.py")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    L.atg_params_default.argtypes = [C.POINTER(Params)]
+    L.atg_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.atg_ctx_create.restype = C.c_int
+    L.atg_ctx_destroy.argtypes = [vp]
+    L.atg_last_error.argtypes = [vp]
+    L.atg_last_error.restype = C.c_char_p
+    L.atg_set_families.argtypes = [vp, C.POINTER(C.POINTER(Family)), C.POINTER(i32), i32]
+    L.atg_set_families.restype = C.c_int
+    L.atg_detect.argtypes = [vp, C.POINTER(Params), vp, i64, i64, i64, C.c_int, C.POINTER(Result)]
+    L.atg_detect.restype = C.c_int
+    L.atg_detect_batch.argtypes = [vp, C.POINTER(Params), C.POINTER(Frame), i32, C.c_int, C.POINTER(Result)]
+    L.atg_detect_batch.restype = C.c_int
+    L.atg_debug_stages.argtypes = [vp, i32, C.POINTER(Stages)]
+    L.atg_debug_stages.restype = C.c_int
+    L.atg_stages_free.argtypes = [C.POINTER(Stages)]
+    L.atg_last_timing.argtypes = [vp, C.POINTER(C.c_double)]
+    L.atg_last_timing.restype = C.c_int
+    L.atg_builtin_family.argtypes = [i32]
+    L.atg_builtin_family.restype = C.POINTER(Family)
+    L.atg_synth_default.argtypes = [C.POINTER(SynthSpec), C.c_int]
+    L.atg_synth_frame.argtypes = [C.POINTER(SynthSpec), C.c_uint64, vp, i64, C.POINTER(SynthTag), i32]
+    L.atg_synth_frame.restype = C.c_int
+    _lib = L
+    return L
+
+
+def default_params(**kw):
+    p = Params()
+    lib().atg_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def params_from_dict(d):
+    """Parameters from a dict in the at_detector naming (see tests)."""
+    keys = [f for f, _ in Params._fields_]
+    return default_params(**{k: v for k, v in d.items() if k in keys})
+
+
+def _dets(ptr, n):
+    if n <= 0:
+        return np.zeros(0, DET_DTYPE)
+    return np.frombuffer(C.string_at(ptr, n * DET_DTYPE.itemsize), DET_DTYPE).copy()
+
+
+class Context:
+    """One device context (atg_ctx)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        rc = self.L.atg_ctx_create(device, C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError("atg_ctx_create failed (%d)" % rc)
+        self._fams = []
+
+    def close(self):
+        if self.h:
+            self.L.atg_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, self.L.atg_last_error(self.h).decode()))
+
+    def set_families(self, families):
+        """families: list of (name, bits_corrected)."""
+        ptrs = (C.POINTER(Family) * max(1, len(families)))()
+        bits = (C.c_int32 * max(1, len(families)))()
+        for i, (name, b) in enumerate(families):
+            ptrs[i] = self.L.atg_builtin_family(FAMILY_INDEX[name])
+            bits[i] = b
+        self._check(self.L.atg_set_families(self.h, ptrs, bits, len(families)), "atg_set_families")
+
+    def detect(self, img, params=None, on_device_ptr=None):
+        p = params if params is not None else default_params()
+        r = Result()
+        if on_device_ptr is not None:
+            ptr, h, w, stride = on_device_ptr
+            self._check(self.L.atg_detect(self.h, C.byref(p), ptr, w, h, stride, 1, C.byref(r)), "atg_detect")
+        else:
+            img = np.ascontiguousarray(img, np.uint8)
+            self._check(self.L.atg_detect(self.h, C.byref(p), img.ctypes.data, img.shape[1], img.shape[0],
+                                          img.strides[0], 0, C.byref(r)), "atg_detect")
+        return dict(dets=_dets(r.dets, r.ndets), nquads=r.nquads)
+
+    def detect_batch(self, imgs, params=None, on_device=False):
+        """imgs: list of uint8 arrays, or with on_device a list of
+        (device_ptr, height, width, stride)."""
+        p = params if params is not None else default_params()
+        n = len(imgs)
+        fr = (Frame * n)()
+        keep = []
+        for i, im in enumerate(imgs):
+            if on_device:
+                ptr, h, w, stride = im
+                fr[i] = Frame(ptr, w, h, stride)
+            else:
+                a = np.ascontiguousarray(im, np.uint8)
+                keep.append(a)
+                fr[i] = Frame(a.ctypes.data, a.shape[1], a.shape[0], a.strides[0])
+        res = (Result * n)()
+        self._check(self.L.atg_detect_batch(self.h, C.byref(p), fr, n, 1 if on_device else 0, res),
+                    "atg_detect_batch")
+        return [dict(dets=_dets(r.dets, r.ndets), nquads=r.nquads) for r in res]
+
+    def chain_ms(self):
+        v = C.c_double()
+        self.L.atg_last_timing(self.h, C.byref(v))
+        return v.value
+
+    def stages(self, frame=0):
+        s = Stages()
+        self._check(self.L.atg_debug_stages(self.h, frame, C.byref(s)), "atg_debug_stages")
+
+        def arr(ptr, n, dt):
+            if n <= 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ptr, (n,)).astype(dt, copy=True)
+        qw, qh = s.qw, s.qh
+        out = dict(qw=qw, qh=qh,
+                   qim=arr(s.qim, qw * qh, np.uint8).reshape(qh, qw),
+                   t=arr(s.t, qw * qh, np.uint8).reshape(qh, qw),
+                   runs=arr(s.runs, 3 * s.nruns, np.uint32).reshape(-1, 3),
+                   row_off=arr(s.row_off, qh + 1, np.uint32),
+                   rep=arr(s.rep, s.nnodes, np.uint32), csize=arr(s.csize, s.nnodes, np.uint32),
+                   ok=arr(s.ok, s.nnodes, np.uint8),
+                   cluster_ids=arr(s.cluster_ids, s.nclusters, np.uint64),
+                   ccount=arr(s.ccount, s.nclusters, np.uint32),
+                   pts=arr(s.pts, s.npts, np.uint64),
+                   quads=arr(s.quads, 9 * s.nquads, np.float32).reshape(-1, 9))
+        self.L.atg_stages_free(C.byref(s))
+        return out
+
+
+def synth_frame(config, seed, width=None, height=None, ntags=None):
+    """A seeded synthetic frame (uint8 HxW) and its tags (family, id)."""
+    L = lib()
+    s = SynthSpec()
+    L.atg_synth_default(C.byref(s), config)
+    if width:
+        s.width = width
+    if height:
+        s.height = height
+    if ntags is not None:
+        s.ntags_min = s.ntags_max = ntags
+    img = np.zeros((s.height, s.width), np.uint8)
+    truth = (SynthTag * 256)()
+    n = L.atg_synth_frame(C.byref(s), seed, img.ctypes.data, s.width, truth, 256)
+    if n < 0:
+        raise RuntimeError("atg_synth_frame failed")
+    return img, [(truth[i].family, truth[i].id) for i in range(min(n, 256))]

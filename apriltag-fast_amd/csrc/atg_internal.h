@@ -31,7 +31,7 @@ namespace atg {
 constexpr int kMaxFamilies = 16;
 constexpr int kSortBlocks = 32;      // radix-sort blocks per frame
 constexpr int kMaxQuadsPerFrame = 4096;
-constexpr int kMaxDetsPerFrame = 4096;
+constexpr int kMaxDetsPerFrame = 1024;
 
 struct DevFamily {
     int32_t nbits, wab, tw, reversed;
@@ -79,6 +79,7 @@ struct Frame {
     uint64_t bucket_mask; // cluster hash buckets - 1 (fast/clusters.c:97-103)
     int32_t max_px;       // 2 * (2 qw + 2 qh) (fast/quads.c:1150)
     int32_t blurred;      // the quad image was blurred or sharpened
+    int32_t skip;         // a frame the reference returns nothing for: an 8x8 zero image
     int64_t pcap;         // point capacity
     uint8_t *qim, *t, *tmx, *tmn, *tbuf;
     uint32_t *runs, *rcnt, *row_off;
@@ -89,6 +90,7 @@ struct Frame {
     uint32_t *sval, *sval2;
     uint32_t *rhist;      // [256][kSortBlocks]
     uint32_t *cstart, *ccount, *cflag;
+    uint64_t *cid;        // [pcap] cluster ids (max_root << 32 | min_root), in order
     uint64_t *okey;       // unsorted angle keys, by cluster position
     double *mom;          // [6][pcap] cumulative moments in angle order
     double *errs, *filt;  // [pcap] window errors, filtered

@@ -280,7 +280,11 @@ __global__ void k_resolve(const Frame *frames, const DevParams *P) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn) return;
     uint32_t r = (uint32_t)i;
-    for (uint32_t p; (p = F.parent[r]) != r;) r = p;
+    // every link points to a node that was a root when it was made, so the
+    // walk ends; the bound only guards against a corrupted table
+    int64_t steps = 0;
+    for (uint32_t p; (p = F.parent[r]) != r && steps <= nn; steps++) r = p;
+    if (steps > nn) atomicOr(F.counters + kError, 32u);
     uint32_t ok = (int64_t)F.fw[r] >= (int64_t)P->mcp ? 1u : 0u;
     F.nlab[i] = r | (ok << 31);
 }

@@ -24,6 +24,12 @@
 
 namespace atg {
 
+// Correctly rounded float sqrt and division. The device's single-precision
+// sqrt is not correctly rounded; double has more than 2*24+2 bits, so one
+// double operation rounded once to float is the correctly rounded result.
+__device__ __forceinline__ float fsqrt_cr(float x) { return (float)__dsqrt_rn((double)x); }
+__device__ __forceinline__ float fdiv_cr(float a, float b) { return (float)__ddiv_rn((double)a, (double)b); }
+
 #define DADD __dadd_rn
 #define DSUB __dsub_rn
 #define DMUL __dmul_rn
@@ -369,8 +375,8 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                                    0.5f, (float)(wab + 0.5), 1.0f, 0.0f, 1.0f, 0.5f, (float)(wab - 0.5), 1.0f, 0.0f, 0.0f};
             for (int k = t; k < nb; k += 256) {
                 int pi = k / wab, i = k % wab;
-                double tx01 = (double)__fdiv_rn(__fadd_rn(pat[pi * 5], __fmul_rn((float)i, pat[pi * 5 + 2])), wf);
-                double ty01 = (double)__fdiv_rn(__fadd_rn(pat[pi * 5 + 1], __fmul_rn((float)i, pat[pi * 5 + 3])), wf);
+                double tx01 = (double)fdiv_cr(__fadd_rn(pat[pi * 5], __fmul_rn((float)i, pat[pi * 5 + 2])), wf);
+                double ty01 = (double)fdiv_cr(__fadd_rn(pat[pi * 5 + 1], __fmul_rn((float)i, pat[pi * 5 + 3])), wf);
                 double tx = DMUL(2.0, DSUB(tx01, 0.5)), ty = DMUL(2.0, DSUB(ty01, 0.5));
                 double zz = DADD(DADD(DMUL(sH[6], tx), DMUL(sH[7], ty)), sH[8]);
                 double px = DDIV(DADD(DADD(DMUL(sH[0], tx), DMUL(sH[1], ty)), sH[2]), zz);
@@ -447,7 +453,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                         if (vv > 0.0) { ws = (float)DADD((double)ws, vv); wc = __fadd_rn(wc, 1.0f); rc |= 1; }
                         else { bs = (float)DSUB((double)bs, vv); bc = __fadd_rn(bc, 1.0f); }
                     }
-                    double xm = (double)__fdiv_rn(ws, wc), ym = (double)__fdiv_rn(bs, bc);
+                    double xm = (double)fdiv_cr(ws, wc), ym = (double)fdiv_cr(bs, bc);
                     double mg = xm != xm ? ym : ym != ym ? xm : xm < ym ? xm : ym;
                     float margin = (float)mg;
                     // quick decode (fast/decode.c:87-107)
@@ -500,7 +506,6 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
         }
         if (t == 0) {
             QuadRec &o = F.quads[qi];
-            for (int i = 0; i < 4; i++) { o.p[i][0] = qp[i][0]; o.p[i][1] = qp[i][1]; }
             o.hok = s_hok;
             for (int i = 0; i < 9; i++) o.H[i] = sH[i];
         }
@@ -516,7 +521,7 @@ struct G2Line {
 
 __device__ G2Line g2_line(const double *a, const double *b) {
     double u0 = DSUB(b[0], a[0]), u1 = DSUB(b[1], a[1]);
-    double mag = (double)__fsqrt_rn((float)DADD(DMUL(u0, u0), DMUL(u1, u1)));
+    double mag = (double)fsqrt_cr((float)DADD(DMUL(u0, u0), DMUL(u1, u1)));
     G2Line l;
     l.p[0] = a[0]; l.p[1] = a[1]; l.u[0] = DDIV(u0, mag); l.u[1] = DDIV(u1, mag);
     return l;
@@ -630,6 +635,7 @@ __global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevPar
 __global__ void k_init(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     if (threadIdx.x < kNCounters) F.counters[threadIdx.x] = 0;
+    if (F.skip) const_cast<uint8_t *>(F.img)[threadIdx.x] = 0;  // 64 bytes: the 8x8 zero frame
 }
 
 void launch_init(hipStream_t st, const Frame *dframes, int nframes) {

@@ -268,7 +268,13 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames, int 
         uint32_t f = (i < s1 && (i == 0 || key[i] != key[i - 1])) ? 1u : 0u;
         uint32_t tot;
         uint32_t pre = block_excl_scan(f, &tot);
-        if (f) F.cstart[base + pre] = i;
+        if (f) {
+            F.cstart[base + pre] = i;
+            // the cluster id back from the sort key
+            const int NB = F.nbits_node;
+            const uint64_t nm = ((uint64_t)1 << NB) - 1;
+            F.cid[base + pre] = (((key[i] >> NB) & nm) << 32) | (key[i] & nm);
+        }
         base += tot;
     }
     if (blockIdx.x == kSortBlocks - 1 && threadIdx.x == 0) {

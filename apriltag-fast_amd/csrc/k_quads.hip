@@ -29,6 +29,12 @@
 
 namespace atg {
 
+// Correctly rounded float sqrt and division. The device's single-precision
+// sqrt is not correctly rounded; double has more than 2*24+2 bits, so one
+// double operation rounded once to float is the correctly rounded result.
+__device__ __forceinline__ float fsqrt_cr(float x) { return (float)__dsqrt_rn((double)x); }
+__device__ __forceinline__ float fdiv_cr(float a, float b) { return (float)__ddiv_rn((double)a, (double)b); }
+
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 
@@ -71,7 +77,7 @@ __device__ __forceinline__ uint32_t point_slope(float dx, float dy) {
     float fx = u2f(f2u(dx) ^ neg), fy = u2f(f2u(dy) ^ neg);
     float ndx = fx < 0.0f ? fy : fx;
     float ndy = fx < 0.0f ? -fx : fy;
-    return slope_key(__fadd_rn(quadrant, __fdiv_rn(ndy, ndx)));
+    return slope_key(__fadd_rn(quadrant, fdiv_cr(ndy, ndx)));
 }
 
 __device__ __forceinline__ void pt_unpack(uint64_t p, int *x, int *y, int *gx, int *gy) {
@@ -390,14 +396,14 @@ __device__ void line_from_sums(const double s[6], double lp[4], double *mse) {
     double cyy = __dsub_rn(__ddiv_rn(s[4], w), __dmul_rn(ey, ey));
     double dd = __dsub_rn(cxx, cyy);
     double arg = __dadd_rn(__dmul_rn(dd, dd), __dmul_rn(__dmul_rn(4.0, cxy), cxy));
-    double root = (double)__fsqrt_rn((float)arg);
+    double root = (double)fsqrt_cr((float)arg);
     double es = __dmul_rn(0.5, __dsub_rn(__dadd_rn(cxx, cyy), root));
     double eig = __dmul_rn(0.5, __dadd_rn(__dadd_rn(cxx, cyy), root));
     double nx1 = __dsub_rn(cxx, eig), ny1 = cxy, m1 = __dadd_rn(__dmul_rn(nx1, nx1), __dmul_rn(ny1, ny1));
     double nx2 = cxy, ny2 = __dsub_rn(cyy, eig), m2 = __dadd_rn(__dmul_rn(nx2, nx2), __dmul_rn(ny2, ny2));
     double nx = nx2, ny = ny2, m = m2;
     if (m1 > m2) { nx = nx1; ny = ny1; m = m1; }
-    double len = (double)__fsqrt_rn((float)m);
+    double len = (double)fsqrt_cr((float)m);
     lp[0] = ex;
     lp[1] = ey;
     lp[2] = 0.0;
@@ -417,7 +423,7 @@ __device__ __forceinline__ double win_err(const double s[6], double nn) {
     double cxy = __dsub_rn(__ddiv_rn(s[3], w), __dmul_rn(ex, ey));
     double cyy = __dsub_rn(__ddiv_rn(s[4], w), __dmul_rn(ey, ey));
     double d = __dsub_rn(cxx, cyy);
-    double root = (double)__fsqrt_rn((float)__dadd_rn(__dmul_rn(d, d), __dmul_rn(4.0, __dmul_rn(cxy, cxy))));
+    double root = (double)fsqrt_cr((float)__dadd_rn(__dmul_rn(d, d), __dmul_rn(4.0, __dmul_rn(cxy, cxy))));
     return __dmul_rn(nn, __dmul_rn(0.5, __dsub_rn(__dadd_rn(cxx, cyy), root)));
 }
 

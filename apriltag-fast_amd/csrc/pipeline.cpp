@@ -1,0 +1,663 @@
+// Host side of the atg C-ABI: the context, the HBM arena and its per-frame
+// layout, the decode tables, the kernel chain of one (batched) call and
+// the readback of the detections.
+//
+// One call = one pass of the chain over all its frames: every kernel is
+// launched once with blockIdx.y = frame, so a batch of uneven frames costs
+// the same number of launches as one frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "atg.h"
+#include "atg_internal.h"
+
+namespace atg {
+void launch_init(hipStream_t st, const Frame *dframes, int nframes);
+void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
+                        int max_qh, bool blur, bool deglitch);
+void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
+                      int max_qh, uint8_t *gscratch, int64_t gstride);
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits);
+void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
+                        int fit_blocks);
+void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks);
+}  // namespace atg
+
+using namespace atg;
+
+namespace {
+
+constexpr int kResultDets = kMaxDetsPerFrame;  // detection capacity of a frame's result block
+constexpr int kInlineDets = 32;            // read back with the counters
+constexpr size_t kCountersBytes = 64;
+constexpr size_t kResultBlock = kCountersBytes + (size_t)kResultDets * sizeof(at_detection);
+constexpr int kFitBlocks = 256;
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int ceil_log2(uint64_t v) {
+    int b = 0;
+    while (((uint64_t)1 << b) < v) b++;
+    return b;
+}
+
+// fast/image.c:6-20
+void decimated_size(int64_t w, int64_t h, float f, int64_t *ow, int64_t *oh) {
+    if (!(f > 1.0f)) { *ow = w; *oh = h; return; }
+    if (f == 1.5f) { *ow = w / 3 * 2; *oh = h / 3 * 2; return; }
+    int64_t k = (int64_t)f;
+    if (k == 2) { *ow = w / 2; *oh = h / 2; return; }
+    *ow = 1 + (w - 1) / k;
+    *oh = 1 + (h - 1) / k;
+}
+
+// fast/clusters.c:97-103 with one thread
+int64_t cluster_map_size(int64_t w, int64_t h) {
+    int64_t chunk = 1 + (h - 1) / 6, n = 1024;
+    while (n < chunk * w / 64 && n < 65536) n <<= 1;
+    return n;
+}
+
+struct FramePlan {
+    bool skip;            // the reference returns nothing for this frame
+    int64_t W, H, stride;
+    int64_t qw, qh;
+    int64_t pcap;
+    size_t off, size;     // arena region
+    size_t in_off;        // staged input (host frames), within the region
+};
+
+}  // namespace
+
+struct atg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    // families and decode tables
+    std::vector<at_family> fams;
+    std::vector<int32_t> bits;
+    uint8_t *dtables = nullptr;
+    DevParams hp{};
+    DevParams *dparams = nullptr;
+    bool params_valid = false;
+    atg_params last_params{};
+    // arena
+    uint8_t *arena = nullptr;
+    size_t arena_size = 0;
+    uint8_t *results = nullptr;       // result blocks, contiguous
+    size_t results_size = 0;
+    Frame *dframes = nullptr;
+    size_t dframes_cap = 0;
+    uint8_t *pinned = nullptr;        // host staging: frames in
+    size_t pinned_size = 0;
+    uint8_t *pinned_out = nullptr;    // host staging: result prefixes out
+    size_t pinned_out_size = 0;
+    // last call
+    std::vector<Frame> hframes;
+    std::vector<FramePlan> plans;
+    std::vector<std::vector<at_detection>> dets;
+    std::vector<int64_t> nquads;
+    double chain_ms = 0.0;
+    int64_t pcap_scale = 1;           // points per quad-image pixel
+};
+
+namespace {
+
+int fail(atg_ctx *c, int code, const std::string &m) {
+    c->err = m;
+    return code;
+}
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(ctx, ATG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// quick_decode_init (fast/decode.c:20-58): per chunk j, the codes sorted by
+// the chunk's value (counting sort, id order kept).
+void build_tables(const at_family &f, std::vector<uint32_t> &off, std::vector<uint16_t> &ids,
+                  std::vector<uint64_t> &codes, int64_t *chunk, int64_t *cap) {
+    int64_t cs = (f.nbits + 3) / 4, capacity = (int64_t)1 << cs;
+    uint64_t mask = (uint64_t)(capacity - 1);
+    off.assign((size_t)(4 * (capacity + 1)), 0);
+    ids.assign((size_t)(4 * f.ncodes), 0);
+    codes.assign((size_t)(4 * f.ncodes), 0);
+    for (int64_t i = 0; i < f.ncodes; i++)
+        for (int64_t j = 0; j < 4; j++) off[(size_t)(j * (capacity + 1) + (int64_t)((f.codes[i] >> (j * cs)) & mask) + 1)]++;
+    for (int64_t j = 0; j < 4; j++)
+        for (int64_t v = 0; v < capacity; v++) off[(size_t)(j * (capacity + 1) + v + 1)] += off[(size_t)(j * (capacity + 1) + v)];
+    std::vector<uint32_t> cur(off);
+    for (int64_t i = 0; i < f.ncodes; i++)
+        for (int64_t j = 0; j < 4; j++) {
+            int64_t c = j * (capacity + 1) + (int64_t)((f.codes[i] >> (j * cs)) & mask);
+            size_t at = (size_t)(j * f.ncodes + cur[(size_t)c]++);
+            ids[at] = (uint16_t)i;
+            codes[at] = f.codes[i];
+        }
+    *chunk = cs;
+    *cap = capacity;
+}
+
+int upload_params(atg_ctx *ctx, const atg_params *p) {
+    DevParams &h = ctx->hp;
+    memset(&h, 0, sizeof h);
+    h.decimate = p->quad_decimate;
+    h.sigma = p->quad_sigma;
+    h.refine = p->refine_edges ? 1 : 0;
+    h.deglitch = p->deglitch ? 1 : 0;
+    h.sharpening = p->decode_sharpening;
+    h.mcp = p->min_cluster_pixels;
+    h.max_nmaxima = p->max_nmaxima;
+    h.mwbd = p->min_white_black_diff;
+    h.cos_crit = p->cos_critical_rad;
+    h.max_mse = p->max_line_fit_mse;
+    h.nfam = (int32_t)ctx->fams.size();
+    // min_tag_width (fast/detector.c:67-81)
+    int32_t mtw = 1000000;
+    bool nb = false, rb = false;
+    for (auto &f : ctx->fams) {
+        mtw = std::min(mtw, f.width_at_border);
+        nb = nb || !f.reversed_border;
+        rb = rb || f.reversed_border;
+    }
+    if (p->quad_decimate > 1.0f) mtw = (int32_t)((float)mtw / p->quad_decimate);
+    if (mtw < 3) mtw = 3;
+    h.tag_width = mtw;
+    h.normal_border = nb;
+    h.reversed_border = rb;
+    // blur kernel (fast/image.c:111-147): host exp, as the reference
+    int64_t ksz = 0;
+    if (p->quad_sigma != 0.0f) {
+        ksz = (int64_t)(4.0f * fabsf(p->quad_sigma));
+        if ((ksz & 1) == 0) ksz++;
+    }
+    if (ksz > 63) return fail(ctx, ATG_ERR_ARG, "quad_sigma too large (blur kernel over 63 taps)");
+    h.blur_ksz = ksz > 1 ? (int32_t)ksz : 0;
+    if (ksz > 1) {
+        double sigma = fabsf(p->quad_sigma);
+        std::vector<double> dk((size_t)ksz);
+        for (int64_t i = 0; i < ksz; i++) {
+            double t = (double)(-(ksz / 2) + i) / sigma;
+            dk[(size_t)i] = exp(-0.5 * (t * t));
+        }
+        double acc = 0.0;
+        for (int64_t i = 0; i < ksz; i++) acc += dk[(size_t)i];
+        for (int64_t i = 0; i < ksz; i++) h.blur_k[i] = (uint8_t)(dk[(size_t)i] / acc * 255.0);
+    }
+    // qsm_kernel_init (fast/quads.c:740-745), make_detection's rotations
+    // (fast/detector.c:85-86): host libm, as the reference
+    for (int i = 0; i < 7; i++) h.qsm_kern[i] = (float)exp(-(double)((i - 3) * (i - 3)) / 2.0);
+    for (int r = 0; r < 4; r++) {
+        double th = (double)r * M_PI / 2.0;
+        h.rot_cs[r][0] = cos(th);
+        h.rot_cs[r][1] = sin(th);
+    }
+    // families: geometry and table pointers into dtables
+    size_t at = 0;
+    std::vector<uint8_t> blob;
+    for (size_t i = 0; i < ctx->fams.size(); i++) {
+        const at_family &f = ctx->fams[i];
+        DevFamily &d = h.fam[i];
+        std::vector<uint32_t> off;
+        std::vector<uint16_t> ids;
+        std::vector<uint64_t> codes;
+        int64_t cs, cap;
+        build_tables(f, off, ids, codes, &cs, &cap);
+        d.nbits = f.nbits;
+        d.wab = f.width_at_border;
+        d.tw = f.total_width;
+        d.reversed = f.reversed_border ? 1 : 0;
+        d.maxhamming = ctx->bits[i];
+        d.chunk_size = (int32_t)cs;
+        d.ncodes = (int32_t)f.ncodes;
+        d.capacity = cap;
+        for (int k = 0; k < 64; k++) {
+            d.bit_x[k] = f.bit_x[k];
+            d.bit_y[k] = f.bit_y[k];
+        }
+        size_t o_off = align256(at), o_ids = align256(o_off + off.size() * 4), o_codes = align256(o_ids + ids.size() * 2);
+        at = o_codes + codes.size() * 8;
+        blob.resize(align256(at));
+        memcpy(blob.data() + o_off, off.data(), off.size() * 4);
+        memcpy(blob.data() + o_ids, ids.data(), ids.size() * 2);
+        memcpy(blob.data() + o_codes, codes.data(), codes.size() * 8);
+        d.offsets = reinterpret_cast<const uint32_t *>(o_off);  // rebased below
+        d.ids = reinterpret_cast<const uint16_t *>(o_ids);
+        d.codes = reinterpret_cast<const uint64_t *>(o_codes);
+    }
+    if (ctx->dtables) HIPCHK(hipFree(ctx->dtables));
+    ctx->dtables = nullptr;
+    if (!blob.empty()) {
+        HIPCHK(hipMalloc(&ctx->dtables, blob.size()));
+        HIPCHK(hipMemcpy(ctx->dtables, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    }
+    for (size_t i = 0; i < ctx->fams.size(); i++) {
+        DevFamily &d = h.fam[i];
+        d.offsets = reinterpret_cast<const uint32_t *>(ctx->dtables + reinterpret_cast<size_t>(d.offsets));
+        d.ids = reinterpret_cast<const uint16_t *>(ctx->dtables + reinterpret_cast<size_t>(d.ids));
+        d.codes = reinterpret_cast<const uint64_t *>(ctx->dtables + reinterpret_cast<size_t>(d.codes));
+    }
+    if (!ctx->dparams) HIPCHK(hipMalloc(&ctx->dparams, sizeof(DevParams)));
+    HIPCHK(hipMemcpy(ctx->dparams, &h, sizeof h, hipMemcpyHostToDevice));
+    ctx->last_params = *p;
+    ctx->params_valid = true;
+    return ATG_OK;
+}
+
+bool same_params(const atg_params &a, const atg_params &b) {
+    return a.quad_decimate == b.quad_decimate && a.quad_sigma == b.quad_sigma && a.refine_edges == b.refine_edges &&
+           a.decode_sharpening == b.decode_sharpening && a.min_cluster_pixels == b.min_cluster_pixels &&
+           a.max_nmaxima == b.max_nmaxima && a.cos_critical_rad == b.cos_critical_rad &&
+           a.max_line_fit_mse == b.max_line_fit_mse && a.min_white_black_diff == b.min_white_black_diff &&
+           a.deglitch == b.deglitch;
+}
+
+// The region of one frame: offsets of every array (also its size).
+size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t *base) {
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = at;
+        at = align256(at + bytes);
+        return base ? base + o : nullptr;
+    };
+    int64_t qw = fp.qw, qh = fp.qh, px = qw * qh, tw = qw / 4, th = qh / 4;
+    int64_t nodes = px + qh, pc = fp.pcap;
+    uint8_t *in = (host_input || fp.skip) ? take((size_t)(fp.W * fp.H)) : nullptr;
+    if (F) F->img = in;
+    F->qim = take((size_t)px);
+    F->t = take((size_t)px);
+    F->tbuf = take((size_t)px);
+    F->tmx = take((size_t)(tw * th + 1));
+    F->tmn = take((size_t)(tw * th + 1));
+    F->runs = (uint32_t *)take((size_t)px * 4);
+    F->rcnt = (uint32_t *)take((size_t)(qh + 1) * 4);
+    F->row_off = (uint32_t *)take((size_t)(qh + 2) * 4);
+    F->parent = (uint32_t *)take((size_t)nodes * 4);
+    F->fw = (uint32_t *)take((size_t)nodes * 4);
+    F->nlab = (uint32_t *)take((size_t)nodes * 4);
+    F->plab = (uint32_t *)take((size_t)px * 4);
+    F->pcnt = (uint32_t *)take((size_t)(qh + 1) * 4);
+    F->poff = (uint32_t *)take((size_t)(qh + 2) * 4);
+    F->pts = (uint64_t *)take((size_t)pc * 8);
+    F->cpts = (uint64_t *)take((size_t)pc * 8);
+    F->skey = (uint64_t *)take((size_t)pc * 8);
+    F->skey2 = (uint64_t *)take((size_t)pc * 8);
+    F->sval = (uint32_t *)take((size_t)pc * 4);
+    F->sval2 = (uint32_t *)take((size_t)pc * 4);
+    F->rhist = (uint32_t *)take((size_t)256 * kSortBlocks * 4);
+    F->cstart = (uint32_t *)take((size_t)(pc + 2) * 4);
+    F->ccount = nullptr;
+    F->cflag = (uint32_t *)take((size_t)(pc + 1) * 4);
+    F->cid = (uint64_t *)take((size_t)(pc + 1) * 8);
+    F->okey = (uint64_t *)take((size_t)pc * 8);
+    F->mom = (double *)take((size_t)pc * 6 * 8);
+    F->errs = (double *)take((size_t)pc * 8);
+    F->filt = (double *)take((size_t)pc * 8);
+    F->midx = (uint32_t *)take((size_t)pc * 4);
+    F->qtmp = (QuadRec *)take(sizeof(QuadRec) * kMaxQuadsPerFrame);
+    F->qcl = (uint32_t *)take(4 * kMaxQuadsPerFrame);
+    F->quads = (QuadRec *)take(sizeof(QuadRec) * kMaxQuadsPerFrame);
+    F->dtmp = (at_detection *)take(sizeof(at_detection) * kMaxQuadsPerFrame * std::max(nfam, 1));
+    F->dflag = (uint32_t *)take(4 * (size_t)kMaxQuadsPerFrame * std::max(nfam, 1));
+    return at;
+}
+
+}  // namespace
+
+extern "C" void atg_params_default(atg_params *p) {
+    // fast/detector.c:8-20
+    memset(p, 0, sizeof *p);
+    p->quad_decimate = 2.0f;
+    p->quad_sigma = 0.0f;
+    p->refine_edges = true;
+    p->decode_sharpening = 0.25;
+    p->min_cluster_pixels = 24;
+    p->max_nmaxima = 10;
+    p->cos_critical_rad = (float)cos(10.0 * M_PI / 180.0);
+    p->max_line_fit_mse = 10.0f;
+    p->min_white_black_diff = 5;
+    p->deglitch = false;
+}
+
+extern "C" int atg_ctx_create(int device, atg_ctx **out) {
+    *out = nullptr;
+    atg_ctx *ctx = new atg_ctx();
+    ctx->device = device;
+    int rc = ATG_OK;
+    [&]() -> int {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&ctx->ev0));
+        HIPCHK(hipEventCreate(&ctx->ev1));
+        return ATG_OK;
+    }();
+    if (!ctx->err.empty()) {
+        fprintf(stderr, "atg: %s\n", ctx->err.c_str());
+        rc = ATG_ERR_HIP;
+        atg_ctx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return ATG_OK;
+}
+
+extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    hipFree(ctx->arena);
+    hipFree(ctx->results);
+    hipFree(ctx->dframes);
+    hipFree(ctx->dtables);
+    hipFree(ctx->dparams);
+    hipHostFree(ctx->pinned);
+    hipHostFree(ctx->pinned_out);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *atg_last_error(const atg_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int atg_set_families(atg_ctx *ctx, const at_family *const *fams, const int32_t *bits, int32_t n) {
+    if (n < 0 || n > kMaxFamilies) return fail(ctx, ATG_ERR_ARG, "at most 16 families");
+    for (int32_t i = 0; i < n; i++) {
+        // at_add_family's checks (fast/detector.c:45-54)
+        if (bits[i] < 0 || bits[i] > 3) return fail(ctx, ATG_ERR_ARG, "bits_corrected outside 0..3");
+        if (fams[i]->ncodes >= 65536 || fams[i]->nbits <= 0 || fams[i]->nbits > 64)
+            return fail(ctx, ATG_ERR_ARG, "family outside the supported sizes");
+    }
+    ctx->fams.clear();
+    ctx->bits.clear();
+    for (int32_t i = 0; i < n; i++) {
+        ctx->fams.push_back(*fams[i]);
+        ctx->bits.push_back(bits[i]);
+    }
+    ctx->params_valid = false;
+    return ATG_OK;
+}
+
+static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device) {
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->params_valid || !same_params(ctx->last_params, *p)) {
+        int rc = upload_params(ctx, p);
+        if (rc) return rc;
+    }
+    const int nfam = (int)ctx->fams.size();
+    ctx->plans.assign((size_t)n, FramePlan{});
+    ctx->hframes.assign((size_t)n, Frame{});
+    size_t total = 0, pinned_need = 0;
+    int max_qw = 4, max_qh = 4, key_bits = 0, cl_bits = 0;
+    for (int32_t i = 0; i < n; i++) {
+        FramePlan &fp = ctx->plans[(size_t)i];
+        fp.W = frames[i].width;
+        fp.H = frames[i].height;
+        fp.stride = frames[i].stride;
+        fp.skip = nfam == 0 || fp.W < 8 || fp.H < 8 || !frames[i].pixels;
+        decimated_size(fp.W, fp.H, p->quad_decimate, &fp.qw, &fp.qh);
+        if (!fp.skip && (fp.qw < 4 || fp.qh < 4)) fp.skip = true;
+        if (fp.skip) { fp.qw = fp.qh = 4; fp.W = fp.H = 8; }
+        if (fp.qw > 32767 || fp.qh > 32767) return fail(ctx, ATG_ERR_LIMIT, "quad image wider or taller than 32767");
+        int64_t nodes = fp.qw * fp.qh + fp.qh;
+        if (nodes >= ((int64_t)1 << 24)) return fail(ctx, ATG_ERR_LIMIT, "quad image over 2^24 pixels");
+        fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
+        Frame F{};
+        fp.size = layout(fp, nfam, !on_device, &F, nullptr);
+        fp.off = total;
+        total += fp.size;
+        if (!on_device && !fp.skip) pinned_need += align256((size_t)(fp.W * fp.H));
+        max_qw = std::max<int>(max_qw, (int)fp.qw);
+        max_qh = std::max<int>(max_qh, (int)fp.qh);
+        int nb = ceil_log2((uint64_t)nodes);
+        key_bits = std::max(key_bits, ceil_log2((uint64_t)cluster_map_size(fp.qw, fp.qh)) + 2 * nb);
+        cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
+    }
+    // the row sweep's state: in LDS when it fits, else per-frame scratch
+    int64_t sweep_bytes = 40 * (int64_t)max_qw + 64;
+    size_t gstride = sweep_bytes > 64 * 1024 ? align256((size_t)sweep_bytes) : 0;
+    total += gstride * (size_t)n;
+    if (total > ctx->arena_size) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipFree(ctx->arena));
+        ctx->arena = nullptr;
+        HIPCHK(hipMalloc(&ctx->arena, total));
+        ctx->arena_size = total;
+    }
+    size_t rneed = kResultBlock * (size_t)n;
+    if (rneed > ctx->results_size) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipFree(ctx->results));
+        ctx->results = nullptr;
+        HIPCHK(hipMalloc(&ctx->results, rneed));
+        ctx->results_size = rneed;
+    }
+    if (sizeof(Frame) * (size_t)n > ctx->dframes_cap) {
+        HIPCHK(hipFree(ctx->dframes));
+        ctx->dframes = nullptr;
+        HIPCHK(hipMalloc(&ctx->dframes, sizeof(Frame) * (size_t)n));
+        ctx->dframes_cap = sizeof(Frame) * (size_t)n;
+    }
+    size_t out_need = (kCountersBytes + kInlineDets * sizeof(at_detection)) * (size_t)n + sizeof(Frame) * (size_t)n;
+    if (pinned_need > ctx->pinned_size) {
+        HIPCHK(hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        HIPCHK(hipHostMalloc(&ctx->pinned, pinned_need));
+        ctx->pinned_size = pinned_need;
+    }
+    if (out_need > ctx->pinned_out_size) {
+        HIPCHK(hipHostFree(ctx->pinned_out));
+        ctx->pinned_out = nullptr;
+        HIPCHK(hipHostMalloc(&ctx->pinned_out, out_need));
+        ctx->pinned_out_size = out_need;
+    }
+    // frame descriptors, and the host frames into the arena
+    size_t pin_at = 0;
+    uint8_t *gscratch = ctx->arena + (total - gstride * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        FramePlan &fp = ctx->plans[(size_t)i];
+        Frame &F = ctx->hframes[(size_t)i];
+        layout(fp, nfam, !on_device, &F, ctx->arena + fp.off);
+        F.W = (int32_t)fp.W;
+        F.H = (int32_t)fp.H;
+        F.qw = (int32_t)fp.qw;
+        F.qh = (int32_t)fp.qh;
+        F.tw = (int32_t)(fp.qw / 4);
+        F.th = (int32_t)(fp.qh / 4);
+        F.nbits_node = ceil_log2((uint64_t)(fp.qw * fp.qh + fp.qh));
+        F.bucket_mask = (uint64_t)cluster_map_size(fp.qw, fp.qh) - 1;
+        F.key_bits = key_bits;
+        F.max_px = (int32_t)(2 * (2 * fp.qw + 2 * fp.qh));
+        F.blurred = ctx->hp.blur_ksz > 1 ? 1 : 0;
+        F.pcap = fp.pcap;
+        F.counters = reinterpret_cast<uint32_t *>(ctx->results + kResultBlock * (size_t)i);
+        F.dets = reinterpret_cast<at_detection *>(ctx->results + kResultBlock * (size_t)i + kCountersBytes);
+        if (fp.skip) {
+            // an 8x8 zero frame (k_init clears it): no tag can come out of it
+            F.skip = 1;
+            F.stride = 8;
+        } else if (on_device) {
+            F.img = frames[i].pixels;
+            F.stride = fp.stride;
+        } else {
+            uint8_t *dst = ctx->pinned + pin_at;
+            for (int64_t y = 0; y < fp.H; y++) memcpy(dst + y * fp.W, frames[i].pixels + y * fp.stride, (size_t)fp.W);
+            HIPCHK(hipMemcpyAsync((void *)F.img, dst, (size_t)(fp.W * fp.H), hipMemcpyHostToDevice, ctx->stream));
+            pin_at += align256((size_t)(fp.W * fp.H));
+            F.stride = fp.W;
+        }
+    }
+    Frame *fstage = reinterpret_cast<Frame *>(ctx->pinned_out);
+    memcpy(fstage, ctx->hframes.data(), sizeof(Frame) * (size_t)n);
+    HIPCHK(hipMemcpyAsync(ctx->dframes, fstage, sizeof(Frame) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    // the chain
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipEventRecord(ctx->ev0, st));
+    launch_init(st, ctx->dframes, n);
+    launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ctx->hp.blur_ksz > 1, p->deglitch);
+    launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, gscratch, (int64_t)gstride);
+    launch_points_stage(st, ctx->dframes, n, max_qh, key_bits);
+    launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, kFitBlocks);
+    launch_decode_stage(st, ctx->dframes, ctx->dparams, n, kFitBlocks);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, st));
+    // results: counters and the first detections of every frame in one copy
+    size_t pre = kCountersBytes + kInlineDets * sizeof(at_detection);
+    uint8_t *hout = ctx->pinned_out + sizeof(Frame) * (size_t)n;
+    HIPCHK(hipMemcpy2DAsync(hout, pre, ctx->results, kResultBlock, pre, (size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->chain_ms = ms;
+    ctx->dets.assign((size_t)n, {});
+    ctx->nquads.assign((size_t)n, 0);
+    bool overflow = false;
+    for (int32_t i = 0; i < n; i++) {
+        const uint32_t *cnt = reinterpret_cast<const uint32_t *>(hout + pre * (size_t)i);
+        const at_detection *d = reinterpret_cast<const at_detection *>(hout + pre * (size_t)i + kCountersBytes);
+        if (ctx->plans[(size_t)i].skip) continue;
+        if (cnt[kError] & 2u) overflow = true;
+        if (cnt[kError] & ~2u) {
+            char m[128];
+            snprintf(m, sizeof m, "frame %d: device capacity exceeded (flags 0x%x)", i, cnt[kError]);
+            return fail(ctx, ATG_ERR_LIMIT, m);
+        }
+        uint32_t nd = std::min<uint32_t>(cnt[kNDets], kResultDets);
+        ctx->nquads[(size_t)i] = cnt[kNQuads];
+        auto &v = ctx->dets[(size_t)i];
+        v.resize(nd);
+        uint32_t first = std::min<uint32_t>(nd, kInlineDets);
+        if (first) memcpy(v.data(), d, first * sizeof(at_detection));
+        if (nd > first)
+            HIPCHK(hipMemcpy(v.data() + first, ctx->results + kResultBlock * (size_t)i + kCountersBytes +
+                                                   first * sizeof(at_detection),
+                             (nd - first) * sizeof(at_detection), hipMemcpyDeviceToHost));
+    }
+    if (overflow) return 1;  // the caller retries with more point capacity
+    return ATG_OK;
+}
+
+extern "C" int atg_detect_batch(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device,
+                                atg_result *out) {
+    if (!ctx || !p || (n > 0 && (!frames || !out)) || n < 0) return ATG_ERR_ARG;
+    if (n == 0) return ATG_OK;
+    ctx->err.clear();
+    ctx->pcap_scale = 1;
+    int rc = run_chain(ctx, p, frames, n, on_device);
+    if (rc == 1) {
+        // more boundary points than quad-image pixels: the worst case is 4
+        ctx->pcap_scale = 4;
+        rc = run_chain(ctx, p, frames, n, on_device);
+        if (rc == 1) rc = fail(ctx, ATG_ERR_LIMIT, "point capacity exceeded");
+    }
+    if (rc) return rc;
+    for (int32_t i = 0; i < n; i++) {
+        out[i].dets = ctx->dets[(size_t)i].data();
+        out[i].ndets = (int64_t)ctx->dets[(size_t)i].size();
+        out[i].nquads = ctx->nquads[(size_t)i];
+    }
+    return ATG_OK;
+}
+
+extern "C" int atg_detect(atg_ctx *ctx, const atg_params *p, const uint8_t *pixels, int64_t width, int64_t height,
+                          int64_t stride, int on_device, atg_result *out) {
+    atg_frame f{pixels, width, height, stride};
+    return atg_detect_batch(ctx, p, &f, 1, on_device, out);
+}
+
+extern "C" int atg_last_timing(const atg_ctx *ctx, double *chain_ms) {
+    if (!ctx) return ATG_ERR_ARG;
+    *chain_ms = ctx->chain_ms;
+    return ATG_OK;
+}
+
+extern "C" void atg_stages_free(atg_stages *s) {
+    free(s->qim); free(s->t); free(s->runs); free(s->row_off); free(s->rep); free(s->csize); free(s->ok);
+    free(s->cluster_ids); free(s->ccount); free(s->pts); free(s->quads);
+    memset(s, 0, sizeof *s);
+}
+
+extern "C" int atg_debug_stages(atg_ctx *ctx, int32_t frame, atg_stages *out) {
+    memset(out, 0, sizeof *out);
+    if (!ctx || frame < 0 || (size_t)frame >= ctx->hframes.size()) return ATG_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const Frame &F = ctx->hframes[(size_t)frame];
+    int64_t qw = F.qw, qh = F.qh, px = qw * qh;
+    uint32_t cnt[kNCounters];
+    HIPCHK(hipMemcpy(cnt, F.counters, sizeof cnt, hipMemcpyDeviceToHost));
+    out->qw = qw;
+    out->qh = qh;
+    out->qim = (uint8_t *)malloc((size_t)px);
+    out->t = (uint8_t *)malloc((size_t)px);
+    HIPCHK(hipMemcpy(out->qim, F.qim, (size_t)px, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out->t, F.t, (size_t)px, hipMemcpyDeviceToHost));
+    // runs, packed in row order
+    std::vector<uint32_t> rcnt((size_t)qh), runs((size_t)px);
+    HIPCHK(hipMemcpy(rcnt.data(), F.rcnt, 4 * (size_t)qh, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(runs.data(), F.runs, 4 * (size_t)px, hipMemcpyDeviceToHost));
+    int64_t nr = 0;
+    for (int64_t y = 0; y < qh; y++) nr += rcnt[(size_t)y];
+    out->nruns = nr;
+    out->runs = (uint32_t *)malloc(12 * (size_t)nr + 4);
+    out->row_off = (uint32_t *)malloc(4 * (size_t)(qh + 1));
+    int64_t k = 0;
+    for (int64_t y = 0; y < qh; y++) {
+        out->row_off[y] = (uint32_t)k;
+        for (uint32_t i = 0; i < rcnt[(size_t)y]; i++, k++) {
+            uint32_t r = runs[(size_t)(y * qw + i)];
+            out->runs[3 * k] = r & 0x7fff;
+            out->runs[3 * k + 1] = (r >> 15) & 0x7fff;
+            out->runs[3 * k + 2] = (r >> 30) & 1 ? 255 : 0;
+        }
+    }
+    out->row_off[qh] = (uint32_t)k;
+    int64_t nn = nr + qh;
+    out->nnodes = nn;
+    std::vector<uint32_t> nlab((size_t)nn), fw((size_t)nn);
+    HIPCHK(hipMemcpy(nlab.data(), F.nlab, 4 * (size_t)nn, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fw.data(), F.fw, 4 * (size_t)nn, hipMemcpyDeviceToHost));
+    out->rep = (uint32_t *)malloc(4 * (size_t)nn + 4);
+    out->csize = (uint32_t *)malloc(4 * (size_t)nn + 4);
+    out->ok = (uint8_t *)malloc((size_t)nn + 1);
+    for (int64_t i = 0; i < nn; i++) {
+        uint32_t r = nlab[(size_t)i] & 0x7fffffffu;
+        out->rep[i] = r;
+        out->ok[i] = (uint8_t)(nlab[(size_t)i] >> 31);
+        out->csize[i] = r < (uint32_t)nn ? fw[r] : 0;
+    }
+    // clusters and their points in output order
+    uint32_t np = cnt[kNPts], nc = cnt[kNClusters];
+    out->npts = np;
+    out->nclusters = nc;
+    out->pts = (uint64_t *)malloc(8 * (size_t)np + 8);
+    HIPCHK(hipMemcpy(out->pts, F.cpts, 8 * (size_t)np, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> cs((size_t)nc + 1);
+    HIPCHK(hipMemcpy(cs.data(), F.cstart, 4 * ((size_t)nc + 1), hipMemcpyDeviceToHost));
+    out->ccount = (uint32_t *)malloc(4 * (size_t)nc + 4);
+    out->cluster_ids = (uint64_t *)malloc(8 * (size_t)nc + 8);
+    HIPCHK(hipMemcpy(out->cluster_ids, F.cid, 8 * (size_t)nc, hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < nc; c++) out->ccount[c] = cs[c + 1] - cs[c];
+    uint32_t nq = std::min<uint32_t>(cnt[kNQuads], kMaxQuadsPerFrame);
+    out->nquads = nq;
+    std::vector<QuadRec> q(nq);
+    HIPCHK(hipMemcpy(q.data(), F.quads, sizeof(QuadRec) * nq, hipMemcpyDeviceToHost));
+    out->quads = (float *)malloc(36 * (size_t)nq + 4);
+    for (uint32_t i = 0; i < nq; i++) {
+        for (int j = 0; j < 4; j++) {
+            out->quads[9 * i + 2 * j] = q[i].p[j][0];
+            out->quads[9 * i + 2 * j + 1] = q[i].p[j][1];
+        }
+        out->quads[9 * i + 8] = q[i].reversed ? 1.0f : 0.0f;
+    }
+    return ATG_OK;
+}

@@ -339,3 +339,38 @@ def have_ref():
 def same_dets(a, b):
     """Bit-exact equality of two detection arrays (all fields)."""
     return len(a) == len(b) and a.tobytes() == b.tobytes()
+
+
+def load_atg():
+    """The product's Python binding (apriltag-fast_amd/atg.py)."""
+    import importlib.util
+    import sys
+    if "atg" in sys.modules:
+        return sys.modules["atg"]
+    spec = importlib.util.spec_from_file_location("atg", str(PKG / "atg.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["atg"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def frame_hash(img):
+    """A 64-bit FNV-1a digest of a frame's bytes (pins the synthetic generator)."""
+    import hashlib
+    return int.from_bytes(hashlib.blake2b(np.ascontiguousarray(img).tobytes(), digest_size=8).digest(), "little")
+
+
+SYNTH_CASES = [(1, s) for s in range(8)] + [(2, s) for s in range(3)] + [(4, s) for s in range(2)]
+
+
+def synth_case(atg, cfg, seed):
+    """The frame, parameters and families of a seeded synthetic case."""
+    if cfg == 4:
+        sizes = [(640, 480), (1280, 720), (1920, 1080), (2448, 2048)]
+        w, h = sizes[seed % 4]
+        img, _ = atg.synth_frame(4, seed, width=w, height=h)
+    else:
+        img, _ = atg.synth_frame(cfg, seed)
+    fams = [("tag36h11", 2)] if cfg != 3 else [("tag36h11", 2), ("tagStandard52h13", 1)]
+    p = default_params(quad_decimate=1.0 if cfg == 3 else 2.0)
+    return img, p, fams

@@ -1,0 +1,125 @@
+"""Parity of the HIP path with the reference, on the GPU.
+
+Bit-exact bar (integer, index and floating-point fields alike): every
+detection field, the quad count, and the stage arrays (quad image,
+thresholded image, runs, union-find representatives, component sizes,
+ok flags, cluster ids and order, points in cluster order, fitted quads)
+equal the oracle's. The oracle is pinned against the reference by
+tests/test_oracle.py.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from atg_testlib import (GOLDEN, SYNTH_CASES, Detector, default_params, det_rows, frame_hash, golden_cases,
+                         parse_case_args, parse_hex, read_pnm, ref_images, same_dets, synth_case)
+
+pytestmark = pytest.mark.gpu
+
+STAGE_KEYS = ["qim", "t", "runs", "row_off", "rep", "csize", "ok", "cluster_ids", "ccount", "pts", "quads"]
+CASES = golden_cases()
+
+
+def gpu_detect(atg, ctx, img, p, fams):
+    ctx.set_families(fams)
+    return ctx.detect(img, atg.params_from_dict(p))
+
+
+@pytest.mark.parametrize("name,args", CASES, ids=[c[0] for c in CASES])
+def test_reference_golden_hex_on_gpu(atg, ctx, name, args):
+    p, fam, ham = parse_case_args(args)
+    gold = parse_hex(GOLDEN / "ref_hex" / (name + ".hex"))
+    for path in ref_images():
+        print("case", name, path.name, flush=True)
+        got = det_rows(gpu_detect(atg, ctx, read_pnm(path), p, [(fam, ham)])["dets"])
+        assert got == gold.get(path.name, []), (name, path.name)
+
+
+@pytest.mark.parametrize("cfg,seed", SYNTH_CASES, ids=["c%d_%d" % c for c in SYNTH_CASES])
+def test_synth_frames_match_reference_fixtures(atg, ctx, cfg, seed):
+    fx = np.load(GOLDEN / "synth_expected.npz")
+    tag = "c%d_%d" % (cfg, seed)
+    img, p, fams = synth_case(atg, cfg, seed)
+    assert int(fx[tag + "_hash"]) == frame_hash(img)
+    print("case", tag, img.shape, flush=True)
+    g = gpu_detect(atg, ctx, img, p, fams)
+    assert g["nquads"] == int(fx[tag + "_nquads"])
+    assert g["dets"].tobytes() == fx[tag + "_dets"].tobytes()
+
+
+@pytest.mark.parametrize("cfg,seed", [(1, 0), (1, 5), (2, 0), (4, 1)])
+def test_stages_match_oracle(atg, ctx, oracle, cfg, seed):
+    img, p, fams = synth_case(atg, cfg, seed)
+    print("case", cfg, seed, img.shape, flush=True)
+    o = oracle.detect(img, p, fams, stages=True)
+    gpu_detect(atg, ctx, img, p, fams)
+    g = ctx.stages(0)
+    for k in STAGE_KEYS:
+        a, b = o["stages"][k], g[k]
+        assert a.shape == b.shape and np.array_equal(a, b), k
+
+
+def test_batch_of_uneven_frames_equals_single(atg, ctx):
+    """Frames of four sizes in one pass give what each gives alone."""
+    frames = [synth_case(atg, 4, s)[0] for s in range(4)] + [synth_case(atg, 1, 0)[0]]
+    p = default_params()
+    ctx.set_families([("tag36h11", 2)])
+    prm = atg.params_from_dict(p)
+    alone = [ctx.detect(f, prm) for f in frames]
+    batch = ctx.detect_batch(frames, prm)
+    for a, b in zip(alone, batch):
+        assert a["nquads"] == b["nquads"]
+        assert same_dets(a["dets"], b["dets"])
+    assert sum(len(a["dets"]) for a in alone) > 20
+
+
+def test_repeatable(atg, ctx):
+    img, p, fams = synth_case(atg, 2, 1)
+    a = gpu_detect(atg, ctx, img, p, fams)
+    b = gpu_detect(atg, ctx, img, p, fams)
+    assert same_dets(a["dets"], b["dets"]) and a["nquads"] == b["nquads"]
+
+
+def test_native_at_detect_api(atg):
+    """at_detector_create / at_add_family / at_detect as a C caller uses them."""
+    L = atg.lib()
+    L.at_detector_create.restype = C.POINTER(Detector)
+    L.at_detector_destroy.argtypes = [C.POINTER(Detector)]
+    L.at_family_by_name.restype = C.c_void_p
+    L.at_family_by_name.argtypes = [C.c_char_p]
+    L.at_add_family.restype = C.c_bool
+    L.at_add_family.argtypes = [C.POINTER(Detector), C.c_void_p, C.c_int]
+    L.at_detect.argtypes = [C.POINTER(Detector), C.c_void_p, C.c_int64, C.c_int64, C.c_int64]
+    td = L.at_detector_create()
+    try:
+        assert L.at_add_family(td, L.at_family_by_name(b"tag36h11"), 1)
+        assert not L.at_add_family(td, L.at_family_by_name(b"tag36h11"), 4)
+        gold = parse_hex(GOLDEN / "ref_hex" / "default.hex")
+        for path in ref_images():
+            img = read_pnm(path)
+            # a wider row stride than the width
+            buf = np.zeros((img.shape[0], img.shape[1] + 13), np.uint8)
+            buf[:, :img.shape[1]] = img
+            L.at_detect(td, buf.ctypes.data, img.shape[1], img.shape[0], buf.shape[1])
+            d = td.contents
+            raw = C.string_at(d.detections, d.ndetections * 168) if d.ndetections else b""
+            dets = np.frombuffer(raw, atg.DET_DTYPE)
+            assert det_rows(dets) == gold[path.name]
+        # under 8x8: nothing, as the reference
+        L.at_detect(td, img.ctypes.data, 7, 7, img.shape[1])
+        assert td.contents.ndetections == 0 and td.contents.nquads == 0
+    finally:
+        L.at_detector_destroy(td)
+
+
+def test_device_pointer_input(atg, ctx):
+    torch = pytest.importorskip("torch")
+    img, p, fams = synth_case(atg, 1, 2)
+    t = torch.from_numpy(img).cuda()
+    ctx.set_families(fams)
+    prm = atg.params_from_dict(p)
+    a = ctx.detect(img, prm)
+    torch.cuda.synchronize()
+    b = ctx.detect(None, prm, on_device_ptr=(t.data_ptr(), img.shape[0], img.shape[1], img.shape[1]))
+    assert same_dets(a["dets"], b["dets"]) and a["nquads"] == b["nquads"]
