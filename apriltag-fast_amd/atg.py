@@ -104,6 +104,16 @@ def lib():
     L.atg_stages_free.argtypes = [C.POINTER(Stages)]
     L.atg_last_timing.argtypes = [vp, C.POINTER(C.c_double)]
     L.atg_last_timing.restype = C.c_int
+    L.atg_set_profiling.argtypes = [vp, C.c_int]
+    L.atg_set_profiling.restype = C.c_int
+    L.atg_stage_times.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), i32]
+    L.atg_stage_times.restype = C.c_int
+    L.atg_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    L.atg_device_alloc.restype = C.c_int
+    L.atg_device_free.argtypes = [vp, vp]
+    L.atg_device_free.restype = C.c_int
+    L.atg_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
+    L.atg_copy_to_device.restype = C.c_int
     L.atg_builtin_family.argtypes = [i32]
     L.atg_builtin_family.restype = C.POINTER(Family)
     L.atg_synth_default.argtypes = [C.POINTER(SynthSpec), C.c_int]
@@ -204,6 +214,26 @@ class Context:
         v = C.c_double()
         self.L.atg_last_timing(self.h, C.byref(v))
         return v.value
+
+    def to_device(self, img):
+        """Copy a frame into a new device buffer: (ptr, height, width, stride)."""
+        img = np.ascontiguousarray(img, np.uint8)
+        p = C.c_void_p()
+        self._check(self.L.atg_device_alloc(self.h, img.nbytes, C.byref(p)), "atg_device_alloc")
+        self._check(self.L.atg_copy_to_device(self.h, p, img.ctypes.data, img.nbytes), "atg_copy_to_device")
+        return (p.value, img.shape[0], img.shape[1], img.shape[1])
+
+    def free_device(self, ptr):
+        self._check(self.L.atg_device_free(self.h, ptr), "atg_device_free")
+
+    def set_profiling(self, on=True):
+        self._check(self.L.atg_set_profiling(self.h, 1 if on else 0), "atg_set_profiling")
+
+    def stage_times(self):
+        names = (C.c_char_p * 32)()
+        ms = (C.c_double * 32)()
+        n = self.L.atg_stage_times(self.h, names, ms, 32)
+        return [(names[i].decode(), ms[i]) for i in range(min(n, 32))]
 
     def stages(self, frame=0):
         s = Stages()

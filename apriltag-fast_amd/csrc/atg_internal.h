@@ -22,6 +22,8 @@
 //   quads   QuadRec [pcap/24]  fitted quads in cluster order; dets at_detection
 // nodes = qw * qh + qh (runs plus one virtual last-column node per row).
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 #include "apriltag_fast.h"
@@ -102,6 +104,21 @@ struct Frame {
     uint32_t *dflag;      // [kMaxQuadsPerFrame * nfam]
     at_detection *dets;   // [kMaxDetsPerFrame] final, sorted by id
     uint32_t *counters;
+};
+
+// Optional timing marks between the stages of one call (HIP events on
+// the call's stream).
+constexpr int kMaxMarks = 24;
+struct Marks {
+    hipEvent_t ev[kMaxMarks];
+    const char *name[kMaxMarks];
+    int n = 0;
+    bool on = false;
+    void mark(hipStream_t st, const char *what) {
+        if (!on || n >= kMaxMarks) return;
+        name[n] = what;
+        hipEventRecord(ev[n++], st);
+    }
 };
 
 }  // namespace atg

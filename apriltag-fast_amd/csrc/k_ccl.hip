@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
 }
 
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, uint8_t *gscratch, int64_t gstride) {
+                      int max_qh, uint8_t *gscratch, int64_t gstride, Marks *mk) {
     int64_t need = (int64_t)40 * max_qw + 64;
     int lds = need <= 160 * 1024 ? (int)need : 0;
     if (lds > 64 * 1024) {
@@ -332,10 +332,13 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     }
     hipLaunchKernelGGL(k_uf_sweep, dim3(1, nframes), dim3(64), (size_t)lds, st, dframes, dparams, lds, gscratch,
                        gstride);
+    mk->mark(st, "uf_sweep");
     int64_t maxnodes = (int64_t)max_qw * max_qh + max_qh;
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,
                        dparams);
+    mk->mark(st, "resolve");
     hipLaunchKernelGGL(k_labels, dim3((unsigned)((max_qh + 3) / 4), nframes), dim3(256), 0, st, dframes);
+    mk->mark(st, "labels");
 }
 
 }  // namespace atg

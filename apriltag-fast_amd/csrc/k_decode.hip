@@ -642,9 +642,12 @@ void launch_init(hipStream_t st, const Frame *dframes, int nframes) {
     hipLaunchKernelGGL(k_init, dim3(1, nframes), dim3(64), 0, st, dframes);
 }
 
-void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks) {
+void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
+                         Marks *mk) {
     hipLaunchKernelGGL(k_decode, dim3(blocks, nframes), dim3(256), 0, st, dframes, dparams);
+    mk->mark(st, "decode");
     hipLaunchKernelGGL(k_finish, dim3(1, nframes), dim3(64), 0, st, dframes, dparams);
+    mk->mark(st, "finish");
 }
 
 }  // namespace atg

@@ -261,17 +261,19 @@ __global__ void k_deglitch_min(const Frame *frames) {
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                        int max_qh, bool blur, bool deglitch) {
+                        int max_qh, bool blur, bool deglitch, Marks *mk) {
     int64_t maxpx = (int64_t)max_qw * max_qh;
     int64_t slots = (int64_t)((max_qw + 3) / 4) * ((max_qh + 3) / 4);
     dim3 g1(cdiv(slots, 256), nframes);
     hipLaunchKernelGGL(k_decimate_tiles, g1, dim3(256), 0, st, dframes, dparams, blur ? 0 : 1);
+    mk->mark(st, "decimate");
     if (blur) {
         dim3 gp(cdiv(maxpx, 256), nframes);
         hipLaunchKernelGGL(k_blur_rows, gp, dim3(256), 0, st, dframes, dparams);
         hipLaunchKernelGGL(k_blur_cols, gp, dim3(256), 0, st, dframes, dparams);
         dim3 gt(cdiv((int64_t)(max_qw / 4) * (max_qh / 4), 256), nframes);
         hipLaunchKernelGGL(k_tiles, gt, dim3(256), 0, st, dframes);
+        mk->mark(st, "blur");
     }
     int max_th = max_qh / 4;
     size_t lds = (size_t)2 * (max_qw / 4) + 16;
@@ -283,6 +285,7 @@ void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_deglitch_min, gp, dim3(256), 0, st, dframes);
         hipLaunchKernelGGL(k_rle, dim3(max_qh, nframes), dim3(64), 0, st, dframes);
     }
+    mk->mark(st, "threshold_rle");
 }
 
 }  // namespace atg

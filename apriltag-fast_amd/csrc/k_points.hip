@@ -295,15 +295,18 @@ void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int wh
     }
 }
 
-void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits) {
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, Marks *mk) {
     dim3 g((unsigned)((max_qh + 3) / 4), nframes);
     hipLaunchKernelGGL(k_point_rows, g, dim3(256), 0, st, dframes, 0);
     hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_point_rows, g, dim3(256), 0, st, dframes, 1);
+    mk->mark(st, "points");
     int parity = 0;
     launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity);
+    mk->mark(st, "cluster_sort");
     hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, parity);
     hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, parity);
+    mk->mark(st, "clusters");
 }
 
 }  // namespace atg

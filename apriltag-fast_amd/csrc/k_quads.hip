@@ -716,14 +716,18 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity);
 
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks) {
+                        int fit_blocks, Marks *mk) {
     dim3 g(fit_blocks, nframes);
     hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams);
+    mk->mark(st, "fit_prep");
     int parity = 0;
     launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);
+    mk->mark(st, "angle_sort");
     hipLaunchKernelGGL(k_fit_moments, g, dim3(256), 0, st, dframes, parity);
+    mk->mark(st, "moments");
     hipLaunchKernelGGL(k_fit_corners, g, dim3(256), 0, st, dframes, dparams);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
+    mk->mark(st, "corners");
 }
 
 }  // namespace atg

@@ -20,13 +20,14 @@
 namespace atg {
 void launch_init(hipStream_t st, const Frame *dframes, int nframes);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                        int max_qh, bool blur, bool deglitch);
+                        int max_qh, bool blur, bool deglitch, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, uint8_t *gscratch, int64_t gstride);
-void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits);
+                      int max_qh, uint8_t *gscratch, int64_t gstride, Marks *mk);
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks);
-void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks);
+                        int fit_blocks, Marks *mk);
+void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
+                         Marks *mk);
 }  // namespace atg
 
 using namespace atg;
@@ -106,6 +107,10 @@ struct atg_ctx {
     std::vector<int64_t> nquads;
     double chain_ms = 0.0;
     int64_t pcap_scale = 1;           // points per quad-image pixel
+    Marks marks;                      // per-stage events, when profiling
+    bool profiling = false;
+    std::vector<double> stage_ms;
+    std::vector<const char *> stage_names;
 };
 
 namespace {
@@ -502,12 +507,16 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
     // the chain
     hipStream_t st = ctx->stream;
     HIPCHK(hipEventRecord(ctx->ev0, st));
+    Marks *mk = &ctx->marks;
+    mk->n = 0;
+    mk->on = ctx->profiling;
     launch_init(st, ctx->dframes, n);
-    launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ctx->hp.blur_ksz > 1, p->deglitch);
-    launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, gscratch, (int64_t)gstride);
-    launch_points_stage(st, ctx->dframes, n, max_qh, key_bits);
-    launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, kFitBlocks);
-    launch_decode_stage(st, ctx->dframes, ctx->dparams, n, kFitBlocks);
+    mk->mark(st, "init");
+    launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ctx->hp.blur_ksz > 1, p->deglitch, mk);
+    launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, gscratch, (int64_t)gstride, mk);
+    launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, mk);
+    launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, kFitBlocks, mk);
+    launch_decode_stage(st, ctx->dframes, ctx->dparams, n, kFitBlocks, mk);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev1, st));
     // results: counters and the first detections of every frame in one copy
@@ -518,6 +527,14 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->chain_ms = ms;
+    ctx->stage_ms.clear();
+    ctx->stage_names.clear();
+    for (int k = 1; k < mk->n; k++) {
+        float s = 0.0f;
+        HIPCHK(hipEventElapsedTime(&s, mk->ev[k - 1], mk->ev[k]));
+        ctx->stage_ms.push_back(s);
+        ctx->stage_names.push_back(mk->name[k]);
+    }
     ctx->dets.assign((size_t)n, {});
     ctx->nquads.assign((size_t)n, 0);
     bool overflow = false;
@@ -572,6 +589,47 @@ extern "C" int atg_detect(atg_ctx *ctx, const atg_params *p, const uint8_t *pixe
                           int64_t stride, int on_device, atg_result *out) {
     atg_frame f{pixels, width, height, stride};
     return atg_detect_batch(ctx, p, &f, 1, on_device, out);
+}
+
+extern "C" int atg_device_alloc(atg_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return ATG_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMalloc(out, bytes));
+    return ATG_OK;
+}
+
+extern "C" int atg_device_free(atg_ctx *ctx, void *p) {
+    if (!ctx) return ATG_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipFree(p));
+    return ATG_OK;
+}
+
+extern "C" int atg_copy_to_device(atg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return ATG_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ATG_OK;
+}
+
+extern "C" int atg_set_profiling(atg_ctx *ctx, int on) {
+    if (!ctx) return ATG_ERR_ARG;
+    if (on && !ctx->profiling) {
+        HIPCHK(hipSetDevice(ctx->device));
+        for (int k = 0; k < kMaxMarks; k++) HIPCHK(hipEventCreate(&ctx->marks.ev[k]));
+    }
+    ctx->profiling = on != 0;
+    return ATG_OK;
+}
+
+extern "C" int atg_stage_times(const atg_ctx *ctx, const char **names, double *ms, int32_t cap) {
+    if (!ctx) return ATG_ERR_ARG;
+    int32_t n = (int32_t)ctx->stage_ms.size();
+    for (int32_t i = 0; i < n && i < cap; i++) {
+        names[i] = ctx->stage_names[(size_t)i];
+        ms[i] = ctx->stage_ms[(size_t)i];
+    }
+    return n;
 }
 
 extern "C" int atg_last_timing(const atg_ctx *ctx, double *chain_ms) {

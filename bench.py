@@ -1,0 +1,234 @@
+"""Benchmark of the MI355X AprilTag detection path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+
+Workload: 10 seeded synthetic 1920x1080 frames, 20 tag36h11 tags each
+(40-300 px, tilt <= 45 deg, occluding bars, value-noise background,
+blur 0-1.5, noise 2-8), decimate 2, refine_edges 1, bits_corrected 2.
+The frames are resident in HBM before the timed region.
+
+A step is one pass of the detection chain over one batch of B frames
+(the 10 frames repeated); `value` is the batched frames/s. Beside it:
+the single-frame path (one frame per call) frames/s and its p50/p99
+latency, bit-exactness of both against the oracle, per-stage device
+times, the roofline of the dominant kernel, and the reference CPU
+detector timed on the host (1 thread and all cores).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import threading
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def frames_config2(atg, n=10):
+    return [atg.synth_frame(2, seed)[0] for seed in range(n)]
+
+
+def pctl(v, q):
+    return float(np.percentile(np.asarray(v), q))
+
+
+def stage_bytes(st, W, H):
+    """Algorithmic HBM bytes of stages (a)-(c) of one frame: each array
+    written once by its producer and read once by each consumer."""
+    qw, qh = st["qw"], st["qh"]
+    R = len(st["runs"])
+    N = R + qh
+    P = len(st["pts"])
+    a = W * H + 2 * qw * qh
+    b = qw * qh + 10 * R + 4 * N + 4 * qh
+    c = qw * qh + 5 * R + 4 * N + 8 * P
+    return dict(a=a, b=b, c=c, R=R, N=N, P=P)
+
+
+def cpu_baseline(frames, sample_s):
+    """The reference detector (oracle/_ref, built from the reference's own
+    sources) timed on this host at 1 thread and at all cores, on a bounded
+    sample of the same frames; the C restatement if the reference build
+    is absent."""
+    from atg_testlib import Oracle, Reference, default_params, have_ref
+    cores = os.cpu_count() or 1
+    fams = [("tag36h11", 2)]
+    res = {}
+    if have_ref():
+        ref = Reference()
+        for nt in (1, cores):
+            p = default_params(nthreads=nt)
+            td = ref.detector(p, fams)
+            n, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < sample_s:
+                img = frames[n % len(frames)]
+                ref.lib.at_detect(td, img.ctypes.data, img.shape[1], img.shape[0], img.strides[0])
+                n += 1
+            dt = time.perf_counter() - t0
+            ref.lib.at_detector_destroy(td)
+            res[nt] = (n / dt, n)
+        return dict(value=round(res[cores][0], 2), unit="frames/s", cores=cores, kind="reference",
+                    value_1thread=round(res[1][0], 2), cores_1thread=1,
+                    sample="%d and %d frames (the 10 workload frames, cycled; ~%.0f s each) at %d and 1 threads"
+                           % (res[cores][1], res[1][1], sample_s, cores))
+    orc = Oracle()
+    p = default_params()
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < sample_s:
+        orc.detect(frames[n % len(frames)], p, fams)
+        n += 1
+    dt = time.perf_counter() - t0
+    return dict(value=round(n / dt, 2), unit="frames/s", cores=1, kind="port",
+                sample="%d frames of the workload, ~%.0f s, 1 thread" % (n, sample_s))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=40)
+    ap.add_argument("--cpu-sample-s", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: frames are independent, so more GPUs would each run this same "
+              "single-GPU path on their own share of frames (no collective); --gpus %d is not supported"
+              % args.gpus)
+        sys.exit(2)
+
+    from atg_testlib import Oracle, default_params, load_atg, same_dets
+    atg = load_atg()
+    log("generating frames")
+    frames = frames_config2(atg)
+    W, H = frames[0].shape[1], frames[0].shape[0]
+    fams = [("tag36h11", 2)]
+    pdict = default_params()
+    ctx = atg.Context(0)
+    ctx.set_families(fams)
+    prm = atg.params_from_dict(pdict)
+    dev = [ctx.to_device(f) for f in frames]
+
+    # parity: batch and single-frame outputs against the oracle
+    log("checking parity against the oracle")
+    orc = Oracle()
+    want = [orc.detect(f, pdict, fams) for f in frames]
+    got_b = ctx.detect_batch(dev, prm, on_device=True)
+    got_s = [ctx.detect(None, prm, on_device_ptr=d) for d in dev]
+    bit_exact = all(same_dets(w["dets"], g["dets"]) and w["nquads"] == g["nquads"] for w, g in zip(want, got_b)) and \
+        all(same_dets(w["dets"], g["dets"]) and w["nquads"] == g["nquads"] for w, g in zip(want, got_s))
+    ndets = sum(len(w["dets"]) for w in want)
+
+    # single-frame path
+    log("timing the single-frame path")
+    for _ in range(args.warmup):
+        for d in dev:
+            ctx.detect(None, prm, on_device_ptr=d)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        for d in dev:
+            t = time.perf_counter()
+            ctx.detect(None, prm, on_device_ptr=d)
+            lat.append(time.perf_counter() - t)
+    single_fps = len(lat) / (time.perf_counter() - t0)
+
+    # batched path
+    log("timing the batched path")
+    batch = [dev[i % len(dev)] for i in range(args.batch)]
+    for _ in range(args.warmup):
+        ctx.detect_batch(batch, prm, on_device=True)
+    step_t = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        ctx.detect_batch(batch, prm, on_device=True)
+        step_t.append(time.perf_counter() - t)
+    wall = time.perf_counter() - t0
+    batched_fps = args.batch * args.steps / wall
+
+    # per-stage device times of one batched call (HIP events on the stream)
+    ctx.set_profiling(True)
+    ctx.detect_batch(batch, prm, on_device=True)
+    stages = ctx.stage_times()
+    ctx.set_profiling(False)
+    chain_ms = ctx.chain_ms()
+    # algorithmic bytes of the batch, from the first 10 frames' counts
+    ctx.detect_batch(dev, prm, on_device=True)
+    per = [stage_bytes(ctx.stages(i), W, H) for i in range(len(dev))]
+    reps = args.batch / len(dev)
+    bytes_a = sum(x["a"] for x in per) * reps
+    bytes_b = sum(x["b"] for x in per) * reps
+    bytes_c = sum(x["c"] for x in per) * reps
+    sm = dict(stages)
+    dom = max(stages, key=lambda kv: kv[1])
+    # the stage (a)-(c) kernels with their algorithmic bytes
+    kernel_bytes = {"decimate": None, "threshold_rle": bytes_a, "uf_sweep": bytes_b,
+                    "points": bytes_c}
+    if dom[0] in kernel_bytes and kernel_bytes[dom[0]]:
+        kb, kname = kernel_bytes[dom[0]], dom[0]
+    else:
+        kb, kname = None, dom[0]
+    roof = None
+    if kb:
+        ach = kb / (dom[1] * 1e-3) / 1e9
+        roof = dict(kernel=kname, bound="hbm", achieved=round(ach, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 6), traffic=None,
+                    bytes_per_launch=int(kb), ms_per_launch=round(dom[1], 4))
+    else:
+        # the dominant stage is not an (a)-(c) byte stage: report it with the
+        # (a)-(c) bytes over the whole (a)-(c) device time
+        t_ac = sum(v for k, v in stages if k in ("decimate", "threshold_rle", "uf_sweep", "resolve", "labels",
+                                                  "points", "cluster_sort", "clusters"))
+        ach = (bytes_a + bytes_b + bytes_c) / (t_ac * 1e-3) / 1e9
+        roof = dict(kernel="stages a-c (dominant stage: %s %.3f ms)" % dom, bound="hbm", achieved=round(ach, 3),
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 6), traffic=None,
+                    bytes_per_launch=int(bytes_a + bytes_b + bytes_c), ms_per_launch=round(t_ac, 4))
+
+    cpu = None if args.no_cpu else cpu_baseline(frames, args.cpu_sample_s)
+    out = {
+        "metric": "frames/s @1080p tag36h11 batched vs single-frame; p50/p99 latency, bit-exact",
+        "value": round(batched_fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 (stages a-c), f64/f32 as the reference (d-e)",
+        "data": "synthetic",
+        "config": {"workload": "10 seeded 1920x1080 frames, 20 tag36h11 tags each (configs[1]), decimate 2, "
+                               "refine_edges 1, bits_corrected 2; batch of %d frames (the 10 cycled)" % args.batch,
+                   "batch": args.batch, "frame": "1920x1080 u8"},
+        "single_frame_fps": round(single_fps, 2),
+        "batched_over_single": round(batched_fps / single_fps, 2),
+        "latency_ms_single": {"p50": round(1000 * pctl(lat, 50), 3), "p99": round(1000 * pctl(lat, 99), 3)},
+        "latency_ms_batch": {"p50": round(1000 * pctl(step_t, 50), 3), "p99": round(1000 * pctl(step_t, 99), 3)},
+        "bit_exact": bool(bit_exact),
+        "detections_per_10_frames": int(ndets),
+        "device_chain_ms": round(chain_ms, 3),
+        "stage_ms": {k: round(v, 4) for k, v in stages},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    for d in dev:
+        ctx.free_device(d[0])
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

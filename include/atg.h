@@ -25,6 +25,7 @@
 #define ATG_H
 
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "apriltag_fast.h"
@@ -116,8 +117,21 @@ int atg_debug_stages(atg_ctx *ctx, int32_t frame, atg_stages *out);
 void atg_stages_free(atg_stages *s);
 
 /* Device time (ms, HIP events on the context's stream) of the last
- * detect call's kernel chain, and of its longest single-kernel stage. */
+ * detect call's kernel chain. */
 int atg_last_timing(const atg_ctx *ctx, double *chain_ms);
+
+/* Per-stage device times of the following calls: HIP events between the
+ * kernel groups on the context's stream (a little launch overhead each). */
+int atg_set_profiling(atg_ctx *ctx, int on);
+/* The stages of the last call: names[i] and ms[i] for i < the return
+ * value (at most cap are written). */
+int atg_stage_times(const atg_ctx *ctx, const char **names, double *ms, int32_t cap);
+
+/* Device buffers on the context's device, for callers without their own
+ * allocator (frames kept resident in HBM between calls). */
+int atg_device_alloc(atg_ctx *ctx, size_t bytes, void **out);
+int atg_device_free(atg_ctx *ctx, void *p);
+int atg_copy_to_device(atg_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------
  * Seeded synthetic frames (host code), for tests and benchmarks. */
