@@ -102,6 +102,8 @@ def lib():
     L.atg_debug_stages.argtypes = [vp, i32, C.POINTER(Stages)]
     L.atg_debug_stages.restype = C.c_int
     L.atg_stages_free.argtypes = [C.POINTER(Stages)]
+    L.atg_debug_counters.argtypes = [vp, i32, C.POINTER(C.c_uint32)]
+    L.atg_debug_counters.restype = C.c_int
     L.atg_last_timing.argtypes = [vp, C.POINTER(C.c_double)]
     L.atg_last_timing.restype = C.c_int
     L.atg_set_profiling.argtypes = [vp, C.c_int]
@@ -234,6 +236,11 @@ class Context:
         ms = (C.c_double * 32)()
         n = self.L.atg_stage_times(self.h, names, ms, 32)
         return [(names[i].decode(), ms[i]) for i in range(min(n, 32))]
+
+    def counters(self, frame=0):
+        out = (C.c_uint32 * 16)()
+        self._check(self.L.atg_debug_counters(self.h, frame, out), "atg_debug_counters")
+        return list(out)
 
     def stages(self, frame=0):
         s = Stages()

@@ -86,6 +86,7 @@ struct Frame {
     uint8_t *qim, *t, *tmx, *tmn, *tbuf;
     uint32_t *runs, *rcnt, *row_off;
     uint32_t *parent, *fw, *nlab, *plab;
+    uint32_t *uedge, *ecnt;   // [qh][3 qw] union-find links per row, [qh] their counts
     uint32_t *pcnt, *poff;
     uint64_t *pts, *cpts;
     uint64_t *skey, *skey2;

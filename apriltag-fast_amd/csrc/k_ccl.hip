@@ -25,6 +25,8 @@
 //                the run index counted with ballots over the row's changes.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "atg_internal.h"
 
 namespace atg {
@@ -44,30 +46,6 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-struct SweepMem {
-    uint16_t *par;   // [E] local union-find parent
-    uint32_t *W;     // [E] set weight (pixels)
-    uint32_t *R;     // [E] set root (global node id)
-    uint16_t *flag;  // [E] compaction index + 1 of roots holding a current run
-    uint16_t *labA;  // [QW] element of each previous-row run
-    uint16_t *labB;  // [QW] element of each current-row run (next labA)
-    uint32_t *edge;  // [3 QW] (a << 16 | b); reused as [2][QW] u32 temp
-};
-
-__device__ __forceinline__ uint32_t uf_find(uint16_t *par, uint32_t x) {
-    while (par[x] != x) {
-        uint32_t p = par[x], g = par[p];
-        par[x] = (uint16_t)g;
-        x = g;
-    }
-    return x;
-}
-
-__device__ __forceinline__ uint32_t uf_find_ro(const uint16_t *par, uint32_t x) {
-    while (par[x] != x) x = par[x];
-    return x;
-}
-
 // Whether current run a and previous run b of the same value are linked
 // (fast/unionfind.c:130-141): overlap on columns >= 1; white also touches
 // diagonally.
@@ -78,24 +56,147 @@ __device__ __forceinline__ bool runs_join(int a0, int a1, int b0, int b1, bool w
     return max(max(a0, b0 - 1), 1) <= min(a1, b1 - 1);
 }
 
-__global__ __launch_bounds__(64) void k_uf_sweep(const Frame *frames, const DevParams *P, int lds_bytes,
-                                                 uint8_t *gscratch, int64_t gstride) {
-    extern __shared__ __align__(16) uint8_t lds[];
+// The links of every row, in the reference's order (current run
+// ascending, then previous run ascending): one wave per row, all rows at
+// once. uedge[y][e] = i << 16 | k (current run i of row y, previous run k).
+__global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (y < 1 || y >= F.qh) return;
+    const int QW = F.qw;
+    const int ncur = (int)F.rcnt[y], nprev = (int)F.rcnt[y - 1];
+    const uint32_t *cur = F.runs + (int64_t)y * QW, *prv = F.runs + (int64_t)(y - 1) * QW;
+    uint32_t *out = F.uedge + (int64_t)y * 3 * QW;
+    const uint32_t cap = 3u * (uint32_t)QW;
+    uint32_t carry = 0;
+    for (int i0 = 0; i0 < ncur; i0 += 64) {
+        int i = i0 + lane;
+        int a0 = 0, a1 = -1, klo = nprev;
+        bool white = false;
+        uint32_t cnt = 0;
+        if (i < ncur) {
+            uint32_t r = cur[i];
+            a0 = (int)run_start(r);
+            a1 = (int)run_end(r);
+            white = run_white(r) != 0;
+            int lo = 0, hi = nprev;
+            while (lo < hi) {  // the first previous run with end + 1 >= a0
+                int mid = (lo + hi) >> 1;
+                if ((int)run_end(prv[mid]) + 1 < a0) lo = mid + 1; else hi = mid;
+            }
+            klo = lo;
+            for (int k = klo; k < nprev; k++) {
+                uint32_t pk = prv[k];
+                if ((int)run_start(pk) > a1 + 1) break;
+                if ((run_white(pk) != 0) == white && runs_join(a0, a1, (int)run_start(pk), (int)run_end(pk), white))
+                    cnt++;
+            }
+        }
+        uint32_t inc = wave_incl_scan(cnt);
+        uint32_t at = carry + inc - cnt;
+        if (cnt && at + cnt <= cap) {
+            for (int k = klo; k < nprev; k++) {
+                uint32_t pk = prv[k];
+                if ((int)run_start(pk) > a1 + 1) break;
+                if ((run_white(pk) != 0) == white && runs_join(a0, a1, (int)run_start(pk), (int)run_end(pk), white))
+                    out[at++] = ((uint32_t)i << 16) | (uint32_t)k;
+            }
+        }
+        carry += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) {
+        if (carry > cap) atomicOr(F.counters + kError, 1u);
+        F.ecnt[y] = min(carry, cap);
+    }
+}
+
+// The sweep's state, in LDS (or, for very wide frames, in global scratch
+// accessed through volatile pointers, so that lanes see lane 0's writes).
+template <bool G>
+struct SweepMem {
+    using U32 = typename std::conditional<G, volatile uint32_t, uint32_t>::type;
+    using U16 = typename std::conditional<G, volatile uint16_t, uint16_t>::type;
+    U32 *ent;   // [E][4]: union-find parent, weight (pixels), root node, unused
+    U32 *deg;   // [QW] links of each current run
+    U32 *lcnt;  // [QW] links into each set record
+    U16 *flag;  // [E] (over deg/lcnt) compaction index + 1 of roots holding a current run
+    U32 *le;    // [3 QW] links as (i << 16 | record), bit 31 once resolved; then [2][QW + 1] new records
+    U16 *labA;  // [QW] record of each previous-row run
+    U16 *labB;  // [QW] record of each current-row run
+};
+
+// An element's entry in one load: x = parent, y = weight, z = root node.
+template <bool G>
+__device__ __forceinline__ uint4 ld_ent(SweepMem<G> &M, uint32_t x) {
+    if constexpr (G) {
+        uint4 v;
+        v.x = M.ent[4 * x];
+        v.y = M.ent[4 * x + 1];
+        v.z = M.ent[4 * x + 2];
+        v.w = 0;
+        return v;
+    } else {
+        return *reinterpret_cast<const uint4 *>(M.ent + 4 * x);
+    }
+}
+
+// Find with path halving; returns the root's entry with .x = the root.
+template <bool G>
+__device__ __forceinline__ uint4 sw_find(SweepMem<G> &M, uint32_t x) {
+    for (;;) {
+        uint4 e = ld_ent(M, x);
+        if (e.x == x) return e;
+        uint4 pe = ld_ent(M, e.x);
+        if (pe.x == e.x) return pe;
+        M.ent[4 * x] = pe.x;
+        x = pe.x;
+    }
+}
+
+template <bool G>
+__device__ __forceinline__ uint32_t sw_find_ro(SweepMem<G> &M, uint32_t x) {
+    for (uint32_t p; (p = M.ent[4 * x]) != x;) x = p;
+    return x;
+}
+
+// A barrier of the one-wave sweep: LDS operations of a wave execute in
+// order, so only the compiler must not move memory operations across it;
+// the global-scratch variant needs the full barrier.
+template <bool G>
+__device__ __forceinline__ void sw_sync() {
+    if constexpr (G) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// One wave per frame walks the rows in order. Per row, the set records of
+// the previous row and the current runs are union-find elements; a link
+// where one current run meets one record and nothing else links either is
+// independent of all others and all lanes resolve those at once; lane 0
+// resolves the rest in the reference's order; then the roots that hold
+// current runs become the next row's records and the others are final.
+template <bool G>
+__device__ void uf_sweep(const Frame &F, uint8_t *base) {
     const int lane = threadIdx.x;
     const int QW = F.qw, QH = F.qh;
     const int E = 2 * QW + 2;
-    uint8_t *base = lds_bytes > 0 ? lds : gscratch + (int64_t)blockIdx.y * gstride;
-    SweepMem M;
+    SweepMem<G> M;
     {
+        using U32 = typename SweepMem<G>::U32;
+        using U16 = typename SweepMem<G>::U16;
         uint8_t *p = base;
-        M.W = reinterpret_cast<uint32_t *>(p); p += 4 * (size_t)E;
-        M.R = reinterpret_cast<uint32_t *>(p); p += 4 * (size_t)E;
-        M.edge = reinterpret_cast<uint32_t *>(p); p += 4 * (size_t)(3 * QW + 2);
-        M.par = reinterpret_cast<uint16_t *>(p); p += 2 * (size_t)E;
-        M.flag = reinterpret_cast<uint16_t *>(p); p += 2 * (size_t)E;
-        M.labA = reinterpret_cast<uint16_t *>(p); p += 2 * (size_t)QW;
-        M.labB = reinterpret_cast<uint16_t *>(p);
+        M.ent = reinterpret_cast<U32 *>(p); p += 16 * (size_t)E;
+        M.deg = reinterpret_cast<U32 *>(p);
+        M.flag = reinterpret_cast<U16 *>(p); p += 4 * (size_t)QW;
+        M.lcnt = reinterpret_cast<U32 *>(p); p += 4 * (size_t)QW + 8;
+        M.le = reinterpret_cast<U32 *>(p); p += 12 * (size_t)QW;
+        M.labA = reinterpret_cast<U16 *>(p); p += 2 * (size_t)QW;
+        M.labB = reinterpret_cast<U16 *>(p);
     }
     // row offsets: exclusive scan of the per-row run counts
     uint32_t carry = 0;
@@ -115,162 +216,201 @@ __global__ __launch_bounds__(64) void k_uf_sweep(const Frame *frames, const DevP
         F.parent[vb + y] = vb + y;
         F.fw[vb + y] = 1;
     }
-    // row 0: every run is its own set
+    __syncthreads();  // row_off is read back below
     int m;
     {
         int n0 = (int)F.rcnt[0];
-        const uint32_t *rr = F.runs;
         for (int i = lane; i < n0; i += 64) {
-            uint32_t r = rr[i];
+            uint32_t r = F.runs[i];
             M.labA[i] = (uint16_t)i;
-            M.W[i] = run_end(r) - run_start(r) + 1;
-            M.R[i] = (uint32_t)i;
+            M.ent[4 * i + 1] = run_end(r) - run_start(r) + 1;
+            M.ent[4 * i + 2] = (uint32_t)i;
             F.parent[i] = (uint32_t)i;
         }
         m = n0;
     }
-    uint32_t *err = F.counters + kError;
+    uint64_t cyc[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t tc = clock64();
+    auto tick = [&](int k) {
+        uint64_t n = clock64();
+        cyc[k] += n - tc;
+        tc = n;
+    };
     for (int y = 1; y < QH; y++) {
-        __syncthreads();
+        sw_sync<G>();
         const uint32_t cbase = F.row_off[y];
-        const int ncur = (int)F.rcnt[y], nprev = (int)F.rcnt[y - 1];
+        const int ncur = (int)F.rcnt[y];
+        const int ne = (int)F.ecnt[y];
         const uint32_t *cur = F.runs + (int64_t)y * QW;
-        const uint32_t *prv = F.runs + (int64_t)(y - 1) * QW;
+        const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
         const int ev = m + ncur;  // the virtual node of row y-1
-        // elements: set records 0..m-1, current runs m..m+ncur-1, vnode
-        for (int l = lane; l < m; l += 64) M.par[l] = (uint16_t)l;
+        for (int l = lane; l < m; l += 64) {
+            M.ent[4 * l] = (uint32_t)l;
+            M.lcnt[l] = 0;
+        }
         for (int i = lane; i < ncur; i += 64) {
             uint32_t r = cur[i];
-            M.par[m + i] = (uint16_t)(m + i);
-            M.W[m + i] = run_end(r) - run_start(r) + 1;
-            M.R[m + i] = cbase + i;
+            M.ent[4 * (m + i)] = (uint32_t)(m + i);
+            M.ent[4 * (m + i) + 1] = run_end(r) - run_start(r) + 1;
+            M.ent[4 * (m + i) + 2] = cbase + i;
+            M.deg[i] = 0;
             F.parent[cbase + i] = cbase + i;
         }
         if (lane == 0) {
-            M.par[ev] = (uint16_t)ev;
-            M.W[ev] = 1;
-            M.R[ev] = vb + y - 1;
+            M.ent[4 * ev] = (uint32_t)ev;
+            M.ent[4 * ev + 1] = 1;
+            M.ent[4 * ev + 2] = vb + y - 1;
         }
-        // the edge list in the reference's order
-        uint32_t ecarry = 0;
-        for (int i0 = 0; i0 < ncur; i0 += 64) {
-            int i = i0 + lane;
-            int a0 = 0, a1 = -1, klo = nprev;
-            bool white = false;
-            uint32_t cnt = 0;
-            if (i < ncur) {
-                uint32_t r = cur[i];
-                a0 = (int)run_start(r);
-                a1 = (int)run_end(r);
-                white = run_white(r) != 0;
-                // first previous run with end + 1 >= a0
-                int lo = 0, hi = nprev;
-                while (lo < hi) {
-                    int mid = (lo + hi) >> 1;
-                    if ((int)run_end(prv[mid]) + 1 < a0) lo = mid + 1; else hi = mid;
-                }
-                klo = lo;
-                for (int k = klo; k < nprev; k++) {
-                    uint32_t pk = prv[k];
-                    if ((int)run_start(pk) > a1 + 1) break;
-                    if ((run_white(pk) != 0) != white) continue;
-                    if (runs_join(a0, a1, (int)run_start(pk), (int)run_end(pk), white)) cnt++;
-                }
-            }
-            uint32_t inc = wave_incl_scan(cnt);
-            uint32_t at = ecarry + inc - cnt;
-            if (at + cnt > (uint32_t)(3 * QW)) {
-                atomicOr(err, 1u);
-                cnt = 0;
-            }
-            if (cnt) {
-                for (int k = klo; k < nprev; k++) {
-                    uint32_t pk = prv[k];
-                    if ((int)run_start(pk) > a1 + 1) break;
-                    if ((run_white(pk) != 0) != white) continue;
-                    if (runs_join(a0, a1, (int)run_start(pk), (int)run_end(pk), white))
-                        M.edge[at++] = ((uint32_t)(m + i) << 16) | M.labA[k];
-                }
-            }
-            ecarry += __shfl(inc, 63, 64);
+        sw_sync<G>();
+        tick(0);
+        // the links in terms of set records, and how many touch each side
+        for (int e = lane; e < ne; e += 64) {
+            uint32_t ed = edges[e];
+            uint32_t i = ed >> 16, l = M.labA[ed & 0xffffu];
+            M.le[e] = (i << 16) | l;
+            atomicAdd((uint32_t *)&M.deg[i], 1u);
+            atomicAdd((uint32_t *)&M.lcnt[l], 1u);
         }
-        const uint32_t nedges = min(ecarry, (uint32_t)(3 * QW));
-        // the last-column edge (fast/unionfind.c:93-97)
+        sw_sync<G>();
+        tick(1);
+        // one current run meeting one record, nothing else linking either:
+        // an independent link (fast/unionfind.c:36-44)
+        for (int e = lane; e < ne; e += 64) {
+            uint32_t ed = M.le[e];
+            uint32_t i = ed >> 16, l = ed & 0xffffu;
+            if (M.deg[i] != 1u || M.lcnt[l] != 1u) continue;
+            uint32_t a = (uint32_t)m + i;
+            uint32_t wa = M.ent[4 * a + 1], wb = M.ent[4 * l + 1];
+            uint32_t Ra = M.ent[4 * a + 2], Rb = M.ent[4 * l + 2];
+            if (wa > wb) {
+                M.ent[4 * l] = a;
+                M.ent[4 * a + 1] = wa + wb;
+                F.parent[Rb] = Ra;
+            } else {
+                M.ent[4 * a] = l;
+                M.ent[4 * l + 1] = wa + wb;
+                F.parent[Ra] = Rb;
+            }
+            M.le[e] = ed | 0x80000000u;
+        }
         bool lastcol = false;
         if (ncur > 0) {
             uint32_t lr = cur[ncur - 1];
             const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
             lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
         }
-        __syncthreads();
+        sw_sync<G>();
+        tick(2);
+        // the other links, in order, on lane 0; the current run's set is
+        // kept in registers across its links
         if (lane == 0) {
-            uint16_t *par = M.par;
-            uint32_t *W = M.W, *R = M.R;
-            for (uint32_t e = 0; e <= nedges; e++) {
-                uint32_t a, b;
-                if (e < nedges) {
-                    uint32_t ed = M.edge[e];
-                    a = ed >> 16;
-                    b = ed & 0xffffu;
-                } else {
-                    if (!lastcol) break;
-                    a = (uint32_t)(m + ncur - 1);
-                    b = (uint32_t)ev;
+            int cur_i = -1;
+            uint32_t ra = 0, wa = 0, Ra = 0;
+            for (int e0 = 0; e0 < ne; e0 += 8) {
+                uint32_t eb[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) eb[q] = e0 + q < ne ? M.le[e0 + q] : 0x80000000u;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    uint32_t ed = eb[q];
+                    if (ed & 0x80000000u) continue;
+                    uint32_t i = ed >> 16, l = ed & 0xffffu;
+                    if ((int)i != cur_i) {
+                        if (cur_i >= 0) M.ent[4 * ra + 1] = wa;
+                        cur_i = (int)i;
+                        ra = (uint32_t)m + i;  // a current run's first link: a fresh root
+                        uint4 ea = ld_ent(M, ra);
+                        wa = ea.y;
+                        Ra = ea.z;
+                    }
+                    uint4 eb4 = sw_find(M, l);
+                    uint32_t rb = eb4.x;
+                    if (rb == ra) continue;
+                    uint32_t wb = eb4.y, Rb = eb4.z;
+                    if (wa > wb) {
+                        M.ent[4 * rb] = ra;
+                        F.parent[Rb] = Ra;
+                    } else {
+                        M.ent[4 * ra] = rb;
+                        F.parent[Ra] = Rb;
+                        ra = rb;
+                        Ra = Rb;
+                    }
+                    wa += wb;
                 }
-                if (a > (uint32_t)ev || b > (uint32_t)ev) continue;
-                uint32_t ra = uf_find(par, a), rb = uf_find(par, b);
-                if (ra == rb) continue;
-                uint32_t wa = W[ra], wb = W[rb];
-                if (wa > wb) {
-                    par[rb] = (uint16_t)ra;
-                    W[ra] = wa + wb;
-                    F.parent[R[rb]] = R[ra];
+            }
+            if (cur_i >= 0) M.ent[4 * ra + 1] = wa;
+            // the last-column link (fast/unionfind.c:93-97), last in order
+            if (lastcol) {
+                uint4 ea = sw_find(M, (uint32_t)(m + ncur - 1));
+                uint32_t a = ea.x, w1 = ea.y, R1 = ea.z;
+                if (w1 > 1u) {
+                    M.ent[4 * ev] = a;
+                    M.ent[4 * a + 1] = w1 + 1;
+                    F.parent[vb + y - 1] = R1;
                 } else {
-                    par[ra] = (uint16_t)rb;
-                    W[rb] = wa + wb;
-                    F.parent[R[ra]] = R[rb];
+                    M.ent[4 * a] = (uint32_t)ev;
+                    M.ent[4 * ev + 1] = w1 + 1;
+                    F.parent[R1] = vb + y - 1;
                 }
             }
         }
-        __syncthreads();
+        sw_sync<G>();
         // roots that hold a current run go on; the others are final
-        const int ne = ev + 1;
-        for (int e = lane; e < ne; e += 64) M.flag[e] = 0;
-        __syncthreads();
-        for (int i = lane; i < ncur; i += 64) M.flag[uf_find_ro(M.par, m + i)] = 1;
-        __syncthreads();
+        const int nel = ev + 1;
+        for (int e = lane; e < nel; e += 64) M.flag[e] = 0;
+        sw_sync<G>();
+        for (int i = lane; i < ncur; i += 64) M.flag[sw_find_ro(M, (uint32_t)(m + i))] = 1;
+        sw_sync<G>();
         uint32_t ncarry = 0;
-        uint32_t *tW = M.edge, *tR = M.edge + QW + 1;
-        for (int e0 = 0; e0 < ne; e0 += 64) {
+        auto *tW = M.le;
+        auto *tR = M.le + QW + 1;
+        for (int e0 = 0; e0 < nel; e0 += 64) {
             int e = e0 + lane;
-            uint32_t f = e < ne ? M.flag[e] : 0u;
+            uint32_t f = e < nel ? (uint32_t)M.flag[e] : 0u;
             uint32_t inc = wave_incl_scan(f);
-            if (e < ne) {
+            if (e < nel) {
                 if (f) {
                     uint32_t j = ncarry + inc - 1;
                     M.flag[e] = (uint16_t)(j + 1);
-                    tW[j] = M.W[e];
-                    tR[j] = M.R[e];
-                } else if (M.par[e] == e) {
-                    F.fw[M.R[e]] = M.W[e];
+                    tW[j] = M.ent[4 * e + 1];
+                    tR[j] = M.ent[4 * e + 2];
+                } else if (M.ent[4 * e] == (uint32_t)e) {
+                    F.fw[M.ent[4 * e + 2]] = M.ent[4 * e + 1];
                 }
             }
             ncarry += __shfl(inc, 63, 64);
         }
-        __syncthreads();
-        for (int i = lane; i < ncur; i += 64) M.labB[i] = (uint16_t)(M.flag[uf_find_ro(M.par, m + i)] - 1);
-        __syncthreads();
+        sw_sync<G>();
+        tick(4);
+        for (int i = lane; i < ncur; i += 64)
+            M.labB[i] = (uint16_t)(M.flag[sw_find_ro(M, (uint32_t)(m + i))] - 1);
+        sw_sync<G>();
         m = (int)ncarry;
         for (int j = lane; j < m; j += 64) {
-            M.W[j] = tW[j];
-            M.R[j] = tR[j];
+            M.ent[4 * j + 1] = tW[j];
+            M.ent[4 * j + 2] = tR[j];
         }
-        uint16_t *sw = M.labA;
+        auto *sw = M.labA;
         M.labA = M.labB;
         M.labB = sw;
+        tick(5);
     }
-    __syncthreads();
-    for (int l = lane; l < m; l += 64) F.fw[M.R[l]] = M.W[l];
+    sw_sync<G>();
+    for (int l = lane; l < m; l += 64) F.fw[M.ent[4 * l + 2]] = M.ent[4 * l + 1];
+    // cycles per phase (lane 0's clock), for tuning: counters[8..13]
+    if (lane == 0)
+        for (int k = 0; k < 6; k++) F.counters[8 + k] = (uint32_t)min(cyc[k] >> 10, (uint64_t)0xffffffffu);
+}
+
+__global__ __launch_bounds__(64) void k_uf_sweep(const Frame *frames, int lds_bytes, uint8_t *gscratch,
+                                                 int64_t gstride) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const Frame F = frames[blockIdx.y];
+    if (lds_bytes > 0)
+        uf_sweep<false>(F, lds);
+    else
+        uf_sweep<true>(F, gscratch + (int64_t)blockIdx.y * gstride);
 }
 
 // One thread per node: the final root and the ok flag.
@@ -321,17 +461,25 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     if (lane == 0) out[F.qw - 1] = F.nlab[F.row_off[F.qh] + y];
 }
 
+size_t sweep_lds_bytes(int max_qw) { return (size_t)56 * max_qw + 64; }
+
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                       int max_qh, uint8_t *gscratch, int64_t gstride, Marks *mk) {
-    int64_t need = (int64_t)40 * max_qw + 64;
-    int lds = need <= 160 * 1024 ? (int)need : 0;
-    if (lds > 64 * 1024) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_uf_sweep),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
-            lds = 0;
+    hipLaunchKernelGGL(k_uf_edges, dim3((unsigned)((max_qh + 3) / 4), nframes), dim3(256), 0, st, dframes);
+    mk->mark(st, "uf_edges");
+    // LDS when it fits (above 64 KB by the per-kernel limit, up to gfx950's
+    // 160 KB), else the per-frame scratch in HBM
+    size_t need = sweep_lds_bytes(max_qw);
+    int lds = (int)need;
+    if (need > 64 * 1024) {
+        static int big_ok = -1;
+        if (big_ok < 0)
+            big_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_uf_sweep),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+        if (!big_ok || need > 160 * 1024 || gstride == 0) lds = 0;
     }
-    hipLaunchKernelGGL(k_uf_sweep, dim3(1, nframes), dim3(64), (size_t)lds, st, dframes, dparams, lds, gscratch,
-                       gstride);
+    if (lds == 0 && gstride == 0) return;  // cannot happen: the caller sizes the scratch
+    hipLaunchKernelGGL(k_uf_sweep, dim3(1, nframes), dim3(64), (size_t)lds, st, dframes, lds, gscratch, gstride);
     mk->mark(st, "uf_sweep");
     int64_t maxnodes = (int64_t)max_qw * max_qh + max_qh;
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,

@@ -350,9 +350,24 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
             }
             __syncthreads();
             if (t < 6) {
+                // one lane per sum, in order; the loads of 8 terms are
+                // issued together ahead of their adds
                 double s = run[t];
                 double *a = sm[t];
-                for (uint32_t j = 0; j < len; j++) {
+                uint32_t j = 0;
+                for (; j + 8 <= len; j += 8) {
+                    double v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) v[q] = a[j + q];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        s = __dadd_rn(s, v[q]);
+                        v[q] = s;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) a[j + q] = v[q];
+                }
+                for (; j < len; j++) {
                     s = __dadd_rn(s, a[j]);
                     a[j] = s;
                 }
@@ -526,22 +541,24 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                     if ((k & hmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
                 }
                 __syncthreads();
-                if (t == 0) {
-                    uint32_t r = rank;
-                    int d = 255;
-                    for (; d > 0; d--) {
-                        if (r < hist[d]) break;
-                        r -= hist[d];
+                {
+                    // bins in descending order, one per thread: the bin
+                    // that holds the rank-th largest
+                    uint32_t d = 255u - (uint32_t)t, c = hist[d], tot;
+                    uint32_t before = bscan(c, &tot);
+                    if (rank >= before && rank < before + c) {
+                        s_prefix = prefix | ((uint64_t)d << shift);
+                        s_rank = rank - before;
                     }
-                    s_prefix = prefix | ((uint64_t)d << shift);
-                    s_rank = r;
                 }
                 __syncthreads();
                 prefix = s_prefix;
                 rank = s_rank;
                 __syncthreads();
             }
-            uint64_t thr = prefix;
+            // the threshold as a double: maxima are kept if their error is
+            // greater (a comparison of doubles, as the reference's)
+            double thr = __longlong_as_double((long long)((prefix >> 63) ? (prefix ^ 0x8000000000000000ull) : ~prefix));
             // keep the maxima above it, in order
             uint32_t o = 0;
             for (uint32_t c0 = 0; c0 < nm; c0 += 256) {
@@ -549,7 +566,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                 uint32_t f = 0, v = 0;
                 if (i < nm) {
                     v = midx[i];
-                    f = dkey(y[v]) > thr ? 1u : 0u;
+                    f = y[v] > thr ? 1u : 0u;
                 }
                 uint32_t tot;
                 uint32_t pre = bscan(f, &tot);

@@ -19,6 +19,7 @@
 
 namespace atg {
 void launch_init(hipStream_t st, const Frame *dframes, int nframes);
+size_t sweep_lds_bytes(int max_qw);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                         int max_qh, bool blur, bool deglitch, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
@@ -290,6 +291,8 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->fw = (uint32_t *)take((size_t)nodes * 4);
     F->nlab = (uint32_t *)take((size_t)nodes * 4);
     F->plab = (uint32_t *)take((size_t)px * 4);
+    F->uedge = (uint32_t *)take((size_t)px * 12);
+    F->ecnt = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->pcnt = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->poff = (uint32_t *)take((size_t)(qh + 2) * 4);
     F->pts = (uint64_t *)take((size_t)pc * 8);
@@ -428,8 +431,8 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
     }
     // the row sweep's state: in LDS when it fits, else per-frame scratch
-    int64_t sweep_bytes = 40 * (int64_t)max_qw + 64;
-    size_t gstride = sweep_bytes > 64 * 1024 ? align256((size_t)sweep_bytes) : 0;
+    int64_t sweep_bytes = (int64_t)sweep_lds_bytes(max_qw);
+    size_t gstride = sweep_bytes > 64 * 1024 ? align256((size_t)sweep_bytes) : 0;  // fallback if LDS is short
     total += gstride * (size_t)n;
     if (total > ctx->arena_size) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -642,6 +645,14 @@ extern "C" void atg_stages_free(atg_stages *s) {
     free(s->qim); free(s->t); free(s->runs); free(s->row_off); free(s->rep); free(s->csize); free(s->ok);
     free(s->cluster_ids); free(s->ccount); free(s->pts); free(s->quads);
     memset(s, 0, sizeof *s);
+}
+
+extern "C" int atg_debug_counters(atg_ctx *ctx, int32_t frame, uint32_t *out16) {
+    if (!ctx || frame < 0 || (size_t)frame >= ctx->hframes.size()) return ATG_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out16, ctx->hframes[(size_t)frame].counters, 16 * 4, hipMemcpyDeviceToHost));
+    return ATG_OK;
 }
 
 extern "C" int atg_debug_stages(atg_ctx *ctx, int32_t frame, atg_stages *out) {

@@ -115,6 +115,10 @@ int atg_detect_batch(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
 /* Stage arrays of frame `frame` of the last detect call. */
 int atg_debug_stages(atg_ctx *ctx, int32_t frame, atg_stages *out);
 void atg_stages_free(atg_stages *s);
+/* The 16 device counters of frame `frame` of the last call (runs, points,
+ * clusters, quads, detections, error flags, then stage-internal
+ * diagnostics). */
+int atg_debug_counters(atg_ctx *ctx, int32_t frame, uint32_t *out16);
 
 /* Device time (ms, HIP events on the context's stream) of the last
  * detect call's kernel chain. */
