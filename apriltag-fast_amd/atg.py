@@ -108,6 +108,8 @@ def lib():
     L.atg_last_timing.restype = C.c_int
     L.atg_set_profiling.argtypes = [vp, C.c_int]
     L.atg_set_profiling.restype = C.c_int
+    L.atg_debug_force_scratch.argtypes = [vp, C.c_int]
+    L.atg_debug_force_scratch.restype = C.c_int
     L.atg_stage_times.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), i32]
     L.atg_stage_times.restype = C.c_int
     L.atg_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
@@ -230,6 +232,10 @@ class Context:
 
     def set_profiling(self, on=True):
         self._check(self.L.atg_set_profiling(self.h, 1 if on else 0), "atg_set_profiling")
+
+    def force_scratch(self, on=True):
+        """Keep the union-find replay state in HBM scratch (the wide-frame path)."""
+        self._check(self.L.atg_debug_force_scratch(self.h, 1 if on else 0), "atg_debug_force_scratch")
 
     def stage_times(self):
         names = (C.c_char_p * 32)()

@@ -9,9 +9,14 @@
 //   t       u8  [qh][qw]       thresholded image (0, 127, 255)
 //   runs    u32 [qh][qw]       row runs, row-strided: start | end << 15 | white << 30
 //   rcnt    u32 [qh]           runs per row; row_off u32 [qh + 1]
-//   parent  u32 [nodes]        union-find links written by the row sweep
+//   parent  u32 [nodes]        union-find links: a node, or kSym | record slot
 //   fw      u32 [nodes]        component pixel count, valid at final roots
-//   nlab    u32 [nodes]        representative | ok << 31
+//   nlab    u32 [nodes]        representative | ok << 31; before that, the
+//                              record of each run within its row (rlab)
+//   recw/rlost/rsym/rroot u32 [nodes]  per record slot (row_off[y] + record):
+//                              weight, what its root lost to, the root of its
+//                              set as replayed, its actual root
+//   nrec    u32 [qh]           records (sets touching the row) per row
 //   plab    u32 [qh][qw]       per-pixel nlab (ok 0 where t is 127)
 //   pcnt/poff u32 [qh + 1]     points per row / their offsets
 //   pts     u64 [pcap]         boundary points in emission order
@@ -86,6 +91,8 @@ struct Frame {
     uint8_t *qim, *t, *tmx, *tmn, *tbuf;
     uint32_t *runs, *rcnt, *row_off;
     uint32_t *parent, *fw, *nlab, *plab;
+    uint32_t *rlab;           // = nlab until k_resolve: record of each run within its row
+    uint32_t *recw, *rlost, *rsym, *rroot, *nrec;
     uint32_t *uedge, *ecnt;   // [qh][3 qw] union-find links per row, [qh] their counts
     uint32_t *pcnt, *poff;
     uint64_t *pts, *cpts;
@@ -105,6 +112,14 @@ struct Frame {
     uint32_t *dflag;      // [kMaxQuadsPerFrame * nfam]
     at_detection *dets;   // [kMaxDetsPerFrame] final, sorted by id
     uint32_t *counters;
+};
+
+// The replay kernels' memory plan for one call (LDS where it fits, else
+// HBM scratch).
+struct CclPlan {
+    int front, replay, stitch;  // LDS bytes, 0 = scratch
+    int replay_waves;           // waves per frame of k_replay
+    size_t scratch_front, scratch_replay, scratch_stitch;  // HBM bytes per frame
 };
 
 // Optional timing marks between the stages of one call (HIP events on

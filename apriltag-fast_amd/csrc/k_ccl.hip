@@ -6,18 +6,21 @@
 // set is the root of the heavier part, ties to the previous-row side.
 // Those roots become cluster ids, and cluster ids decide the cluster
 // order, so a min-label GPU CCL (right partition, other roots) is not
-// enough. The outcome of each link depends only on the two set weights at
-// that moment, and a set that does not reach row y can never be linked
-// again. So the replay keeps only the sets that touch the previous row:
+// enough. Two facts make the replay parallel: the outcome of a link
+// depends only on the two set weights at that moment, and weights are
+// pixel counts, which do not depend on the order. So:
 //
-//   k_uf_sweep   one wave per frame walks the rows in order. Per row, the
-//                lanes build the row's edge list in the reference's order
-//                (cur run ascending, then previous run ascending, then the
-//                last-column edge) into LDS; lane 0 links them over a small
-//                union-find of set records (weight, root node) held in LDS;
-//                the lanes then relabel the current runs and retire the
-//                sets that ended. Every link that makes a root lose writes
-//                parent[loser] = winner to HBM.
+//   k_uf_edges   one wave per row, all rows at once: the row's links in the
+//                reference's order (current run ascending, then previous
+//                run ascending).
+//   k_front      one wave per frame, an unordered lock-free union-find row
+//                by row: which runs of row y share a set over rows 0..y
+//                (the row's records) and each record's weight.
+//   k_replay     one wave per row, all rows at once: the row's links replayed
+//                in order over the previous row's records (weights known,
+//                roots named symbolically) and the row's runs.
+//   k_stitch     one wave per frame, row by row: turns the symbolic roots
+//                into nodes and writes the final weight of each set that ends.
 //   k_resolve    one thread per node follows parent[] to the final root;
 //                nlab = root | (pixel count >= min_cluster_pixels) << 31
 //                (fast/clusters.c:73-94).
@@ -25,6 +28,7 @@
 //                the run index counted with ballots over the row's changes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "atg_internal.h"
@@ -111,93 +115,87 @@ __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames) {
     }
 }
 
-// The sweep's state, in LDS (or, for very wide frames, in global scratch
-// accessed through volatile pointers, so that lanes see lane 0's writes).
+// Memory of the three replay kernels: LDS, or (for very wide frames) HBM
+// scratch through volatile pointers, so that each lane sees the others'
+// writes after a barrier.
 template <bool G>
-struct SweepMem {
+struct Mem {
     using U32 = typename std::conditional<G, volatile uint32_t, uint32_t>::type;
     using U16 = typename std::conditional<G, volatile uint16_t, uint16_t>::type;
-    U32 *ent;   // [E][4]: union-find parent, weight (pixels), root node, unused
-    U32 *deg;   // [QW] links of each current run
-    U32 *lcnt;  // [QW] links into each set record
-    U16 *flag;  // [E] (over deg/lcnt) compaction index + 1 of roots holding a current run
-    U32 *le;    // [3 QW] links as (i << 16 | record), bit 31 once resolved; then [2][QW + 1] new records
-    U16 *labA;  // [QW] record of each previous-row run
-    U16 *labB;  // [QW] record of each current-row run
 };
 
-// An element's entry in one load: x = parent, y = weight, z = root node.
+// A barrier of a one-wave block. LDS operations of a wave complete in
+// order once lgkmcnt is 0, so the LDS variant only has to keep the
+// compiler (and the LDS queue) in order; the scratch variant orders HBM.
 template <bool G>
-__device__ __forceinline__ uint4 ld_ent(SweepMem<G> &M, uint32_t x) {
-    if constexpr (G) {
-        uint4 v;
-        v.x = M.ent[4 * x];
-        v.y = M.ent[4 * x + 1];
-        v.z = M.ent[4 * x + 2];
-        v.w = 0;
-        return v;
-    } else {
-        return *reinterpret_cast<const uint4 *>(M.ent + 4 * x);
-    }
-}
-
-// Find with path halving; returns the root's entry with .x = the root.
-template <bool G>
-__device__ __forceinline__ uint4 sw_find(SweepMem<G> &M, uint32_t x) {
-    for (;;) {
-        uint4 e = ld_ent(M, x);
-        if (e.x == x) return e;
-        uint4 pe = ld_ent(M, e.x);
-        if (pe.x == e.x) return pe;
-        M.ent[4 * x] = pe.x;
-        x = pe.x;
-    }
-}
-
-template <bool G>
-__device__ __forceinline__ uint32_t sw_find_ro(SweepMem<G> &M, uint32_t x) {
-    for (uint32_t p; (p = M.ent[4 * x]) != x;) x = p;
-    return x;
-}
-
-// A barrier of the one-wave sweep: LDS operations of a wave execute in
-// order, so only the compiler must not move memory operations across it;
-// the global-scratch variant needs the full barrier.
-template <bool G>
-__device__ __forceinline__ void sw_sync() {
+__device__ __forceinline__ void wsync() {
     if constexpr (G) {
         __syncthreads();
     } else {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     }
 }
 
-// One wave per frame walks the rows in order. Per row, the set records of
-// the previous row and the current runs are union-find elements; a link
-// where one current run meets one record and nothing else links either is
-// independent of all others and all lanes resolve those at once; lane 0
-// resolves the rest in the reference's order; then the roots that hold
-// current runs become the next row's records and the others are final.
+constexpr uint32_t kSym = 0x80000000u;     // a record of the previous row, by slot
+constexpr uint32_t kLostNone = 0xffffffffu; // the record's root goes on into the row
+constexpr uint32_t kLostRetired = 0xfffffffeu;  // the record's set ended at the previous row
+
+// ---------------------------------------------------------------------
+// Step 1, k_front: the sets that touch each row, with their weights.
+//
+// Connectivity and pixel counts do not depend on the link order, so one
+// wave per frame sweeps the rows with an unordered lock-free union-find:
+// the elements of row y are the records of row y-1 (distinct sets), the
+// current runs and the row's virtual node; after the unions, the roots
+// that hold a current run become row y's records, numbered in element
+// order. Output per row: rlab[run] = its record, recw[slot] = the record's
+// weight (slot = row_off[y] + record), nrec[y] = the record count.
+
 template <bool G>
-__device__ void uf_sweep(const Frame &F, uint8_t *base) {
+__device__ __forceinline__ uint32_t fr_find(typename Mem<G>::U32 *p, uint32_t x) {
+    for (;;) {  // path halving: only non-roots are rewritten, to an ancestor
+        uint32_t q = p[x];
+        if (q == x) return x;
+        uint32_t g = p[q];
+        if (g == q) return q;
+        p[x] = g;
+        x = g;
+    }
+}
+
+template <bool G>
+__device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, uint32_t b) {
+    for (;;) {  // hook the higher root under the lower; retry if it moved
+        a = fr_find<G>(p, a);
+        b = fr_find<G>(p, b);
+        if (a == b) return;
+        if (a < b) {
+            uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS((uint32_t *)&p[a], a, b) == a) return;
+    }
+}
+
+size_t front_bytes(int qw) { return (size_t)36 * qw + 128; }
+
+template <bool G>
+__device__ void front_sweep(const Frame &F, uint8_t *base) {
+    using U32 = typename Mem<G>::U32;
+    using U16 = typename Mem<G>::U16;
     const int lane = threadIdx.x;
     const int QW = F.qw, QH = F.qh;
-    const int E = 2 * QW + 2;
-    SweepMem<G> M;
-    {
-        using U32 = typename SweepMem<G>::U32;
-        using U16 = typename SweepMem<G>::U16;
-        uint8_t *p = base;
-        M.ent = reinterpret_cast<U32 *>(p); p += 16 * (size_t)E;
-        M.deg = reinterpret_cast<U32 *>(p);
-        M.flag = reinterpret_cast<U16 *>(p); p += 4 * (size_t)QW;
-        M.lcnt = reinterpret_cast<U32 *>(p); p += 4 * (size_t)QW + 8;
-        M.le = reinterpret_cast<U32 *>(p); p += 12 * (size_t)QW;
-        M.labA = reinterpret_cast<U16 *>(p); p += 2 * (size_t)QW;
-        M.labB = reinterpret_cast<U16 *>(p);
-    }
+    const int EM = 2 * QW + 2;
+    U32 *p = reinterpret_cast<U32 *>(base);
+    U32 *acc = p + EM;
+    U32 *flag = acc + EM;
+    U32 *W = flag + EM;
+    U32 *Wn = W + QW + 1;
+    U16 *labA = reinterpret_cast<U16 *>(Wn + QW + 1);
+    U16 *labB = labA + QW;
     // row offsets: exclusive scan of the per-row run counts
     uint32_t carry = 0;
     for (int y0 = 0; y0 < QH; y0 += 64) {
@@ -216,215 +214,371 @@ __device__ void uf_sweep(const Frame &F, uint8_t *base) {
         F.parent[vb + y] = vb + y;
         F.fw[vb + y] = 1;
     }
-    __syncthreads();  // row_off is read back below
-    int m;
-    {
-        int n0 = (int)F.rcnt[0];
-        for (int i = lane; i < n0; i += 64) {
-            uint32_t r = F.runs[i];
-            M.labA[i] = (uint16_t)i;
-            M.ent[4 * i + 1] = run_end(r) - run_start(r) + 1;
-            M.ent[4 * i + 2] = (uint32_t)i;
-            F.parent[i] = (uint32_t)i;
-        }
-        m = n0;
+    // row 0: each run is its own set
+    int m = (int)F.rcnt[0];
+    for (int i = lane; i < m; i += 64) {
+        uint32_t r = F.runs[i], len = run_end(r) - run_start(r) + 1;
+        labA[i] = (uint16_t)i;
+        W[i] = len;
+        F.rlab[i] = (uint32_t)i;
+        F.recw[i] = len;
+        F.parent[i] = (uint32_t)i;
     }
-    uint64_t cyc[6] = {0, 0, 0, 0, 0, 0};
-    uint64_t tc = clock64();
-    auto tick = [&](int k) {
-        uint64_t n = clock64();
-        cyc[k] += n - tc;
-        tc = n;
-    };
+    if (lane == 0) F.nrec[0] = (uint32_t)m;
+    // per 64 rows: run counts, link counts and row offsets in registers
+    uint32_t c_rc = 0, c_ec = 0, c_ro = 0, ro_carry = 0;
     for (int y = 1; y < QH; y++) {
-        sw_sync<G>();
-        const uint32_t cbase = F.row_off[y];
-        const int ncur = (int)F.rcnt[y];
-        const int ne = (int)F.ecnt[y];
+        if (y == 1 || (y & 63) == 0) {
+            int yy = (y & ~63) + lane;
+            c_rc = yy < QH ? F.rcnt[yy] : 0u;
+            c_ec = yy < QH ? F.ecnt[yy] : 0u;
+            uint32_t inc = wave_incl_scan(c_rc);
+            if (y == 1) ro_carry = 0;
+            c_ro = ro_carry + inc - c_rc;
+            ro_carry += __shfl(inc, 63, 64);
+        }
+        const int ncur = (int)__shfl(c_rc, y & 63, 64);
+        const int ne = (int)__shfl(c_ec, y & 63, 64);
+        const uint32_t cbase = __shfl(c_ro, y & 63, 64);
         const uint32_t *cur = F.runs + (int64_t)y * QW;
         const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
-        const int ev = m + ncur;  // the virtual node of row y-1
-        for (int l = lane; l < m; l += 64) {
-            M.ent[4 * l] = (uint32_t)l;
-            M.lcnt[l] = 0;
-        }
-        for (int i = lane; i < ncur; i += 64) {
-            uint32_t r = cur[i];
-            M.ent[4 * (m + i)] = (uint32_t)(m + i);
-            M.ent[4 * (m + i) + 1] = run_end(r) - run_start(r) + 1;
-            M.ent[4 * (m + i) + 2] = cbase + i;
-            M.deg[i] = 0;
-            F.parent[cbase + i] = cbase + i;
-        }
-        if (lane == 0) {
-            M.ent[4 * ev] = (uint32_t)ev;
-            M.ent[4 * ev + 1] = 1;
-            M.ent[4 * ev + 2] = vb + y - 1;
-        }
-        sw_sync<G>();
-        tick(0);
-        // the links in terms of set records, and how many touch each side
-        for (int e = lane; e < ne; e += 64) {
-            uint32_t ed = edges[e];
-            uint32_t i = ed >> 16, l = M.labA[ed & 0xffffu];
-            M.le[e] = (i << 16) | l;
-            atomicAdd((uint32_t *)&M.deg[i], 1u);
-            atomicAdd((uint32_t *)&M.lcnt[l], 1u);
-        }
-        sw_sync<G>();
-        tick(1);
-        // one current run meeting one record, nothing else linking either:
-        // an independent link (fast/unionfind.c:36-44)
-        for (int e = lane; e < ne; e += 64) {
-            uint32_t ed = M.le[e];
-            uint32_t i = ed >> 16, l = ed & 0xffffu;
-            if (M.deg[i] != 1u || M.lcnt[l] != 1u) continue;
-            uint32_t a = (uint32_t)m + i;
-            uint32_t wa = M.ent[4 * a + 1], wb = M.ent[4 * l + 1];
-            uint32_t Ra = M.ent[4 * a + 2], Rb = M.ent[4 * l + 2];
-            if (wa > wb) {
-                M.ent[4 * l] = a;
-                M.ent[4 * a + 1] = wa + wb;
-                F.parent[Rb] = Ra;
-            } else {
-                M.ent[4 * a] = l;
-                M.ent[4 * l + 1] = wa + wb;
-                F.parent[Ra] = Rb;
-            }
-            M.le[e] = ed | 0x80000000u;
-        }
-        bool lastcol = false;
+        const int ev = m + ncur, ne_all = ev + 1;
+        bool lastcol = false;  // fast/unionfind.c:93-97
         if (ncur > 0) {
             uint32_t lr = cur[ncur - 1];
             const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
             lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
         }
-        sw_sync<G>();
-        tick(2);
-        // the other links, in order, on lane 0; the current run's set is
-        // kept in registers across its links
-        if (lane == 0) {
-            int cur_i = -1;
-            uint32_t ra = 0, wa = 0, Ra = 0;
-            for (int e0 = 0; e0 < ne; e0 += 8) {
-                uint32_t eb[8];
-#pragma unroll
-                for (int q = 0; q < 8; q++) eb[q] = e0 + q < ne ? M.le[e0 + q] : 0x80000000u;
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    uint32_t ed = eb[q];
-                    if (ed & 0x80000000u) continue;
-                    uint32_t i = ed >> 16, l = ed & 0xffffu;
-                    if ((int)i != cur_i) {
-                        if (cur_i >= 0) M.ent[4 * ra + 1] = wa;
-                        cur_i = (int)i;
-                        ra = (uint32_t)m + i;  // a current run's first link: a fresh root
-                        uint4 ea = ld_ent(M, ra);
-                        wa = ea.y;
-                        Ra = ea.z;
-                    }
-                    uint4 eb4 = sw_find(M, l);
-                    uint32_t rb = eb4.x;
-                    if (rb == ra) continue;
-                    uint32_t wb = eb4.y, Rb = eb4.z;
-                    if (wa > wb) {
-                        M.ent[4 * rb] = ra;
-                        F.parent[Rb] = Ra;
-                    } else {
-                        M.ent[4 * ra] = rb;
-                        F.parent[Ra] = Rb;
-                        ra = rb;
-                        Ra = Rb;
-                    }
-                    wa += wb;
-                }
-            }
-            if (cur_i >= 0) M.ent[4 * ra + 1] = wa;
-            // the last-column link (fast/unionfind.c:93-97), last in order
-            if (lastcol) {
-                uint4 ea = sw_find(M, (uint32_t)(m + ncur - 1));
-                uint32_t a = ea.x, w1 = ea.y, R1 = ea.z;
-                if (w1 > 1u) {
-                    M.ent[4 * ev] = a;
-                    M.ent[4 * a + 1] = w1 + 1;
-                    F.parent[vb + y - 1] = R1;
-                } else {
-                    M.ent[4 * a] = (uint32_t)ev;
-                    M.ent[4 * ev + 1] = w1 + 1;
-                    F.parent[R1] = vb + y - 1;
-                }
+        wsync<G>();
+        for (int x = lane; x < ne_all; x += 64) {
+            uint32_t w;
+            if (x < m) w = W[x];
+            else if (x < ev) { uint32_t r = cur[x - m]; w = run_end(r) - run_start(r) + 1; }
+            else w = 1;
+            p[x] = (uint32_t)x;
+            acc[x] = w;
+            flag[x] = 0;
+        }
+        wsync<G>();
+        for (int e = lane; e < ne + (lastcol ? 1 : 0); e += 64) {
+            if (e < ne) {
+                uint32_t ed = edges[e];
+                fr_unite<G>(p, (uint32_t)m + (ed >> 16), (uint32_t)labA[ed & 0xffffu]);
+            } else {
+                fr_unite<G>(p, (uint32_t)(ev - 1), (uint32_t)ev);
             }
         }
-        sw_sync<G>();
-        // roots that hold a current run go on; the others are final
-        const int nel = ev + 1;
-        for (int e = lane; e < nel; e += 64) M.flag[e] = 0;
-        sw_sync<G>();
-        for (int i = lane; i < ncur; i += 64) M.flag[sw_find_ro(M, (uint32_t)(m + i))] = 1;
-        sw_sync<G>();
-        uint32_t ncarry = 0;
-        auto *tW = M.le;
-        auto *tR = M.le + QW + 1;
-        for (int e0 = 0; e0 < nel; e0 += 64) {
-            int e = e0 + lane;
-            uint32_t f = e < nel ? (uint32_t)M.flag[e] : 0u;
-            uint32_t inc = wave_incl_scan(f);
-            if (e < nel) {
-                if (f) {
-                    uint32_t j = ncarry + inc - 1;
-                    M.flag[e] = (uint16_t)(j + 1);
-                    tW[j] = M.ent[4 * e + 1];
-                    tR[j] = M.ent[4 * e + 2];
-                } else if (M.ent[4 * e] == (uint32_t)e) {
-                    F.fw[M.ent[4 * e + 2]] = M.ent[4 * e + 1];
-                }
+        wsync<G>();
+        for (int x = lane; x < ne_all; x += 64) {
+            uint32_t r = fr_find<G>(p, (uint32_t)x);
+            if (r != (uint32_t)x) {
+                p[x] = r;
+                atomicAdd((uint32_t *)&acc[r], (uint32_t)acc[x]);  // acc of a non-root never changes
             }
-            ncarry += __shfl(inc, 63, 64);
+            if (x >= m && x < ev) flag[r] = 1;
         }
-        sw_sync<G>();
-        tick(4);
-        for (int i = lane; i < ncur; i += 64)
-            M.labB[i] = (uint16_t)(M.flag[sw_find_ro(M, (uint32_t)(m + i))] - 1);
-        sw_sync<G>();
-        m = (int)ncarry;
-        for (int j = lane; j < m; j += 64) {
-            M.ent[4 * j + 1] = tW[j];
-            M.ent[4 * j + 2] = tR[j];
+        wsync<G>();
+        // row y's records: the roots holding a current run, in element order
+        uint32_t nrec = 0;
+        for (int x0 = 0; x0 < ne_all; x0 += 64) {
+            int x = x0 + lane;
+            bool f = x < ne_all && flag[x] != 0;
+            uint64_t bal = __ballot(f);
+            if (f) {
+                uint32_t id = nrec + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                flag[x] = id + 1;
+                uint32_t w = acc[x];
+                Wn[id] = w;
+                F.recw[cbase + id] = w;
+            }
+            nrec += (uint32_t)__popcll(bal);
         }
-        auto *sw = M.labA;
-        M.labA = M.labB;
-        M.labB = sw;
-        tick(5);
+        wsync<G>();
+        for (int i = lane; i < ncur; i += 64) {
+            uint32_t l = flag[p[m + i]] - 1;
+            labB[i] = (uint16_t)l;
+            F.rlab[cbase + i] = l;
+        }
+        if (lane == 0) F.nrec[y] = nrec;
+        m = (int)nrec;
+        U32 *tw = W; W = Wn; Wn = tw;
+        U16 *tl = labA; labA = labB; labB = tl;
     }
-    sw_sync<G>();
-    for (int l = lane; l < m; l += 64) F.fw[M.ent[4 * l + 2]] = M.ent[4 * l + 1];
-    // cycles per phase (lane 0's clock), for tuning: counters[8..13]
-    if (lane == 0)
-        for (int k = 0; k < 6; k++) F.counters[8 + k] = (uint32_t)min(cyc[k] >> 10, (uint64_t)0xffffffffu);
 }
 
-__global__ __launch_bounds__(64) void k_uf_sweep(const Frame *frames, int lds_bytes, uint8_t *gscratch,
-                                                 int64_t gstride) {
+__global__ __launch_bounds__(64) void k_front(const Frame *frames, int lds_bytes, uint8_t *gscratch,
+                                              int64_t gstride) {
     extern __shared__ __align__(16) uint8_t lds[];
     const Frame F = frames[blockIdx.y];
     if (lds_bytes > 0)
-        uf_sweep<false>(F, lds);
+        front_sweep<false>(F, lds);
     else
-        uf_sweep<true>(F, gscratch + (int64_t)blockIdx.y * gstride);
+        front_sweep<true>(F, gscratch + (int64_t)blockIdx.y * gstride);
 }
 
-// One thread per node: the final root and the ok flag.
+// ---------------------------------------------------------------------
+// Step 2, k_replay: every row's links in the reference's order, all rows
+// at once.
+//
+// At the start of row y the sets touching row y-1 are distinct, and their
+// weights are the ones the sequential replay has there (a weight is a
+// pixel count: step 1 knows it). So row y can be replayed on its own: the
+// elements are those records (weights from step 1, roots unknown: named
+// by slot, kSym | slot), the current runs and the virtual node. Links
+// where one current run meets one record and nothing else links either
+// are independent and resolved by all lanes; lane 0 takes the rest in
+// order (fast/unionfind.c:15-45, 112-146), the last-column link last.
+// Output: parent[] of the row's runs and virtual node (a node or a
+// record slot), rlost[slot] of each previous record (the winner it lost
+// to, or kLostNone / kLostRetired), rsym[slot] of each row-y record (the
+// root of its set: a node or a previous record's slot).
+
+size_t replay_bytes(int qw) { return (size_t)34 * qw + 128; }
+
+template <bool G>
+__device__ __forceinline__ uint2 rp_ent(typename Mem<G>::U32 *ent, uint32_t x) {
+    if constexpr (G) {
+        uint2 v;
+        v.x = ent[2 * x];
+        v.y = ent[2 * x + 1];
+        return v;
+    } else {
+        return *reinterpret_cast<const uint2 *>(ent + 2 * x);
+    }
+}
+
+// Find with path halving; returns the root's entry with .x = the root.
+template <bool G>
+__device__ __forceinline__ uint2 rp_find(typename Mem<G>::U32 *ent, uint32_t x) {
+    for (;;) {
+        uint2 e = rp_ent<G>(ent, x);
+        if (e.x == x) return e;
+        uint2 pe = rp_ent<G>(ent, e.x);
+        if (pe.x == e.x) return pe;
+        ent[2 * x] = pe.x;
+        x = pe.x;
+    }
+}
+
+template <bool G>
+__device__ __forceinline__ uint32_t rp_find_ro(typename Mem<G>::U32 *ent, uint32_t x) {
+    for (uint32_t q; (q = ent[2 * x]) != x;) x = q;
+    return x;
+}
+
+template <bool G>
+__device__ void replay_row(const Frame &F, int y, uint8_t *base) {
+    using U32 = typename Mem<G>::U32;
+    using U16 = typename Mem<G>::U16;
+    const int lane = threadIdx.x;
+    const int QW = F.qw;
+    U32 *ent = reinterpret_cast<U32 *>(base);      // [2 QW + 2][2]: parent, weight
+    U32 *lcnt = ent + 2 * (2 * QW + 2);             // [QW] links into each record
+    U32 *le = lcnt + QW;                            // [3 QW] links: cur << 16 | record, bit 31 once done
+    U16 *labA = reinterpret_cast<U16 *>(le + 3 * QW);  // [QW] record of each previous-row run
+    const uint32_t pb = F.row_off[y - 1], cb = F.row_off[y], vb = F.row_off[F.qh];
+    const int m = (int)F.nrec[y - 1], nprev = (int)F.rcnt[y - 1];
+    const int ncur = (int)F.rcnt[y], ne = (int)F.ecnt[y];
+    const uint32_t *cur = F.runs + (int64_t)y * QW;
+    const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
+    const int ev = m + ncur;
+    for (int l = lane; l < m; l += 64) {
+        ent[2 * l] = (uint32_t)l;
+        ent[2 * l + 1] = F.recw[pb + l];
+        lcnt[l] = 0;
+    }
+    for (int i = lane; i < ncur; i += 64) {
+        uint32_t r = cur[i];
+        ent[2 * (m + i)] = (uint32_t)(m + i);
+        ent[2 * (m + i) + 1] = run_end(r) - run_start(r) + 1;
+    }
+    for (int k = lane; k < nprev; k += 64) labA[k] = (uint16_t)F.rlab[pb + k];
+    if (lane == 0) {
+        ent[2 * ev] = (uint32_t)ev;
+        ent[2 * ev + 1] = 1;
+    }
+    bool lastcol = false;
+    if (ncur > 0) {
+        uint32_t lr = cur[ncur - 1];
+        const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
+        lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
+    }
+    wsync<G>();
+    for (int e = lane; e < ne; e += 64) {
+        uint32_t ed = edges[e];
+        uint32_t l = labA[ed & 0xffffu];
+        le[e] = (ed & 0xffff0000u) | l;
+        atomicAdd((uint32_t *)&lcnt[l], 1u);
+    }
+    wsync<G>();
+    // one current run meeting one record, nothing else linking either: an
+    // independent link (links are sorted by current run, so the run's only
+    // link has different neighbours)
+    for (int e = lane; e < ne; e += 64) {
+        uint32_t ed = le[e];
+        uint32_t i = (ed >> 16) & 0x7fffu, l = ed & 0xffffu;
+        bool only_i = (e == 0 || ((le[e - 1] >> 16) & 0x7fffu) != i) &&
+                      (e == ne - 1 || ((le[e + 1] >> 16) & 0x7fffu) != i);
+        if (!only_i || lcnt[l] != 1u) continue;
+        uint32_t a = (uint32_t)m + i;
+        uint32_t wa = ent[2 * a + 1], wb = ent[2 * l + 1];
+        if (wa > wb) {
+            ent[2 * l] = a;
+            ent[2 * a + 1] = wa + wb;
+        } else {
+            ent[2 * a] = l;
+            ent[2 * l + 1] = wa + wb;
+        }
+        le[e] = ed | 0x80000000u;
+    }
+    wsync<G>();
+    // the other links in order on lane 0; the current run's set is kept in
+    // registers across its links
+    if (lane == 0) {
+        int cur_i = -1;
+        uint32_t ra = 0, wa = 0;
+        for (int e0 = 0; e0 < ne; e0 += 8) {
+            uint32_t eb[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) eb[q] = e0 + q < ne ? le[e0 + q] : 0x80000000u;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                uint32_t ed = eb[q];
+                if (ed & 0x80000000u) continue;
+                uint32_t i = ed >> 16, l = ed & 0xffffu;
+                if ((int)i != cur_i) {
+                    if (cur_i >= 0) ent[2 * ra + 1] = wa;
+                    cur_i = (int)i;
+                    ra = (uint32_t)m + i;  // a current run's first link: a fresh root
+                    wa = ent[2 * ra + 1];
+                }
+                uint2 b = rp_find<G>(ent, l);
+                if (b.x == ra) continue;
+                if (wa > b.y) {
+                    ent[2 * b.x] = ra;
+                } else {
+                    ent[2 * ra] = b.x;
+                    ra = b.x;
+                }
+                wa += b.y;
+            }
+        }
+        if (cur_i >= 0) ent[2 * ra + 1] = wa;
+        if (lastcol) {  // fast/unionfind.c:93-97, last in order
+            uint2 a = rp_find<G>(ent, (uint32_t)(ev - 1));
+            if (a.y > 1u) {
+                ent[2 * ev] = a.x;
+                ent[2 * a.x + 1] = a.y + 1;
+            } else {
+                ent[2 * a.x] = (uint32_t)ev;
+                ent[2 * ev + 1] = a.y + 1;
+            }
+        }
+    }
+    wsync<G>();
+    auto name = [&](uint32_t x) -> uint32_t {
+        if ((int)x < m) return kSym | (pb + x);
+        if ((int)x < ev) return cb + (x - (uint32_t)m);
+        return vb + (uint32_t)y - 1;
+    };
+    for (int l = lane; l < m; l += 64) {
+        uint32_t q = ent[2 * l];
+        F.rlost[pb + l] = q == (uint32_t)l ? (lcnt[l] ? kLostNone : kLostRetired) : name(q);
+    }
+    for (int x = m + lane; x <= ev; x += 64) F.parent[name((uint32_t)x)] = name(ent[2 * x]);
+    for (int i = lane; i < ncur; i += 64)
+        F.rsym[cb + F.rlab[cb + i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
+}
+
+__global__ __launch_bounds__(64) void k_replay(const Frame *frames, int lds_bytes, uint8_t *gscratch,
+                                               int64_t fstride, int64_t wstride) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const Frame F = frames[blockIdx.y];
+    for (int y = 1 + (int)blockIdx.x; y < F.qh; y += (int)gridDim.x) {
+        if (lds_bytes > 0) {
+            replay_row<false>(F, y, lds);
+        } else {
+            replay_row<true>(F, y, gscratch + (int64_t)blockIdx.y * fstride + (int64_t)blockIdx.x * wstride);
+        }
+        __syncthreads();  // the next row reuses the memory
+    }
+}
+
+// ---------------------------------------------------------------------
+// Step 3, k_stitch: the actual roots of the records, row by row (one wave
+// per frame). root[slot] of row y-1's records is known; each lost record's
+// root gets parent = its winner; each retired record's root gets its final
+// weight; row y's records take the root their rsym names.
+
+size_t stitch_bytes(int qw) { return (size_t)8 * qw + 64; }
+
+template <bool G>
+__device__ void stitch(const Frame &F, uint8_t *base) {
+    using U32 = typename Mem<G>::U32;
+    const int lane = threadIdx.x;
+    const int QW = F.qw, QH = F.qh;
+    U32 *rootA = reinterpret_cast<U32 *>(base);
+    U32 *rootB = rootA + QW + 1;
+    int m = (int)F.nrec[0];
+    for (int l = lane; l < m; l += 64) {
+        rootA[l] = (uint32_t)l;
+        F.rroot[l] = (uint32_t)l;
+    }
+    uint32_t pb = 0;
+    for (int y = 1; y < QH; y++) {
+        const uint32_t cb = F.row_off[y];
+        const int m2 = (int)F.nrec[y];
+        wsync<G>();
+        for (int l = lane; l < m; l += 64) {
+            uint32_t x = F.rlost[pb + l];
+            if (x == kLostRetired) {
+                F.fw[rootA[l]] = F.recw[pb + l];
+            } else if (x != kLostNone) {
+                F.parent[rootA[l]] = (x & kSym) ? rootA[(x & ~kSym) - pb] : x;
+            }
+        }
+        for (int l = lane; l < m2; l += 64) {
+            uint32_t s = F.rsym[cb + l];
+            uint32_t r = (s & kSym) ? rootA[(s & ~kSym) - pb] : s;
+            rootB[l] = r;
+            F.rroot[cb + l] = r;
+        }
+        wsync<G>();
+        U32 *t = rootA; rootA = rootB; rootB = t;
+        m = m2;
+        pb = cb;
+    }
+    for (int l = lane; l < m; l += 64) F.fw[rootA[l]] = F.recw[pb + l];
+}
+
+__global__ __launch_bounds__(64) void k_stitch(const Frame *frames, int lds_bytes, uint8_t *gscratch,
+                                               int64_t gstride) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const Frame F = frames[blockIdx.y];
+    if (lds_bytes > 0)
+        stitch<false>(F, lds);
+    else
+        stitch<true>(F, gscratch + (int64_t)blockIdx.y * gstride);
+}
+
+// One thread per node: the final root (a parent naming a record slot goes
+// to that record's root) and the ok flag (fast/clusters.c:73-94).
 __global__ void k_resolve(const Frame *frames, const DevParams *P) {
     const Frame F = frames[blockIdx.y];
     int64_t nn = (int64_t)F.counters[kNRuns] + F.qh;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn) return;
     uint32_t r = (uint32_t)i;
-    // every link points to a node that was a root when it was made, so the
-    // walk ends; the bound only guards against a corrupted table
+    // every link points to a node or record that was a root when it was
+    // made, so the walk ends; the bound only guards against a corrupted table
     int64_t steps = 0;
-    for (uint32_t p; (p = F.parent[r]) != r && steps <= nn; steps++) r = p;
-    if (steps > nn) atomicOr(F.counters + kError, 32u);
+    for (; steps <= 2 * nn; steps++) {
+        uint32_t q = F.parent[r];
+        if (q & kSym) q = F.rroot[q & ~kSym];
+        if (q == r) break;
+        r = q;
+    }
+    if (steps > 2 * nn) atomicOr(F.counters + kError, 32u);
     uint32_t ok = (int64_t)F.fw[r] >= (int64_t)P->mcp ? 1u : 0u;
     F.nlab[i] = r | (ok << 31);
 }
@@ -461,26 +615,46 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     if (lane == 0) out[F.qw - 1] = F.nlab[F.row_off[F.qh] + y];
 }
 
-size_t sweep_lds_bytes(int max_qw) { return (size_t)56 * max_qw + 64; }
+// LDS bytes of the three replay kernels for the widest frame of a call, or
+// 0 where it does not fit (then HBM scratch, ccl_scratch_bytes per frame).
+static bool lds_ok(const void *fn, size_t need, bool force_scratch) {
+    if (force_scratch || need > 160 * 1024) return false;
+    if (need <= 64 * 1024) return true;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) == hipSuccess;
+}
+
+CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch) {
+    CclPlan c{};
+    size_t f = front_bytes(max_qw), r = replay_bytes(max_qw), s = stitch_bytes(max_qw);
+    c.front = lds_ok(reinterpret_cast<const void *>(&k_front), f, force_scratch) ? (int)f : 0;
+    c.replay = lds_ok(reinterpret_cast<const void *>(&k_replay), r, force_scratch) ? (int)r : 0;
+    c.stitch = lds_ok(reinterpret_cast<const void *>(&k_stitch), s, force_scratch) ? (int)s : 0;
+    c.replay_waves = c.replay ? std::max(1, max_qh - 1) : std::min(std::max(1, max_qh - 1), 64);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    c.scratch_front = c.front ? 0 : al(f);
+    c.scratch_stitch = c.stitch ? 0 : al(s);
+    c.scratch_replay = c.replay ? 0 : al(r) * (size_t)c.replay_waves;
+    return c;
+}
+
+size_t ccl_scratch_bytes(const CclPlan &c) { return c.scratch_front + c.scratch_replay + c.scratch_stitch; }
 
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, uint8_t *gscratch, int64_t gstride, Marks *mk) {
+                      int max_qh, const CclPlan &c, uint8_t *scratch, Marks *mk) {
     hipLaunchKernelGGL(k_uf_edges, dim3((unsigned)((max_qh + 3) / 4), nframes), dim3(256), 0, st, dframes);
     mk->mark(st, "uf_edges");
-    // LDS when it fits (above 64 KB by the per-kernel limit, up to gfx950's
-    // 160 KB), else the per-frame scratch in HBM
-    size_t need = sweep_lds_bytes(max_qw);
-    int lds = (int)need;
-    if (need > 64 * 1024) {
-        static int big_ok = -1;
-        if (big_ok < 0)
-            big_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_uf_sweep),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
-        if (!big_ok || need > 160 * 1024 || gstride == 0) lds = 0;
-    }
-    if (lds == 0 && gstride == 0) return;  // cannot happen: the caller sizes the scratch
-    hipLaunchKernelGGL(k_uf_sweep, dim3(1, nframes), dim3(64), (size_t)lds, st, dframes, lds, gscratch, gstride);
-    mk->mark(st, "uf_sweep");
+    // scratch per frame: [front][replay waves][stitch], frames back to back
+    int64_t per = (int64_t)ccl_scratch_bytes(c);
+    uint8_t *s_front = scratch, *s_replay = scratch + c.scratch_front,
+            *s_stitch = scratch + c.scratch_front + c.scratch_replay;
+    int64_t rstride = c.replay ? 0 : (int64_t)(c.scratch_replay / (size_t)c.replay_waves);
+    hipLaunchKernelGGL(k_front, dim3(1, nframes), dim3(64), (size_t)c.front, st, dframes, c.front, s_front, per);
+    mk->mark(st, "uf_front");
+    hipLaunchKernelGGL(k_replay, dim3((unsigned)c.replay_waves, nframes), dim3(64), (size_t)c.replay, st, dframes,
+                       c.replay, s_replay, per, rstride);
+    mk->mark(st, "uf_replay");
+    hipLaunchKernelGGL(k_stitch, dim3(1, nframes), dim3(64), (size_t)c.stitch, st, dframes, c.stitch, s_stitch, per);
+    mk->mark(st, "uf_stitch");
     int64_t maxnodes = (int64_t)max_qw * max_qh + max_qh;
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,
                        dparams);

@@ -19,11 +19,12 @@
 
 namespace atg {
 void launch_init(hipStream_t st, const Frame *dframes, int nframes);
-size_t sweep_lds_bytes(int max_qw);
+CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch);
+size_t ccl_scratch_bytes(const CclPlan &c);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                         int max_qh, bool blur, bool deglitch, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, uint8_t *gscratch, int64_t gstride, Marks *mk);
+                      int max_qh, const CclPlan &c, uint8_t *scratch, Marks *mk);
 void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
                         int fit_blocks, Marks *mk);
@@ -110,6 +111,7 @@ struct atg_ctx {
     int64_t pcap_scale = 1;           // points per quad-image pixel
     Marks marks;                      // per-stage events, when profiling
     bool profiling = false;
+    bool force_scratch = false;       // union-find replay state in HBM even where LDS fits
     std::vector<double> stage_ms;
     std::vector<const char *> stage_names;
 };
@@ -290,6 +292,12 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->parent = (uint32_t *)take((size_t)nodes * 4);
     F->fw = (uint32_t *)take((size_t)nodes * 4);
     F->nlab = (uint32_t *)take((size_t)nodes * 4);
+    F->rlab = F->nlab;
+    F->recw = (uint32_t *)take((size_t)nodes * 4);
+    F->rlost = (uint32_t *)take((size_t)nodes * 4);
+    F->rsym = (uint32_t *)take((size_t)nodes * 4);
+    F->rroot = (uint32_t *)take((size_t)nodes * 4);
+    F->nrec = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->plab = (uint32_t *)take((size_t)px * 4);
     F->uedge = (uint32_t *)take((size_t)px * 12);
     F->ecnt = (uint32_t *)take((size_t)(qh + 1) * 4);
@@ -430,9 +438,9 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
         key_bits = std::max(key_bits, ceil_log2((uint64_t)cluster_map_size(fp.qw, fp.qh)) + 2 * nb);
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
     }
-    // the row sweep's state: in LDS when it fits, else per-frame scratch
-    int64_t sweep_bytes = (int64_t)sweep_lds_bytes(max_qw);
-    size_t gstride = sweep_bytes > 64 * 1024 ? align256((size_t)sweep_bytes) : 0;  // fallback if LDS is short
+    // the union-find replay's state: in LDS when it fits, else per-frame scratch
+    const CclPlan ccl = ccl_plan(max_qw, max_qh, ctx->force_scratch);
+    const size_t gstride = ccl_scratch_bytes(ccl);
     total += gstride * (size_t)n;
     if (total > ctx->arena_size) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -516,7 +524,7 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
     launch_init(st, ctx->dframes, n);
     mk->mark(st, "init");
     launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ctx->hp.blur_ksz > 1, p->deglitch, mk);
-    launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, gscratch, (int64_t)gstride, mk);
+    launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, gscratch, mk);
     launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, mk);
     launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, kFitBlocks, mk);
     launch_decode_stage(st, ctx->dframes, ctx->dparams, n, kFitBlocks, mk);
@@ -622,6 +630,12 @@ extern "C" int atg_set_profiling(atg_ctx *ctx, int on) {
         for (int k = 0; k < kMaxMarks; k++) HIPCHK(hipEventCreate(&ctx->marks.ev[k]));
     }
     ctx->profiling = on != 0;
+    return ATG_OK;
+}
+
+extern "C" int atg_debug_force_scratch(atg_ctx *ctx, int on) {
+    if (!ctx) return ATG_ERR_ARG;
+    ctx->force_scratch = on != 0;
     return ATG_OK;
 }
 

@@ -120,6 +120,11 @@ void atg_stages_free(atg_stages *s);
  * diagnostics). */
 int atg_debug_counters(atg_ctx *ctx, int32_t frame, uint32_t *out16);
 
+/* on != 0: the following calls keep the union-find replay's state in HBM
+ * scratch, the path frames too wide for LDS take, so that tests can check
+ * it on any frame. */
+int atg_debug_force_scratch(atg_ctx *ctx, int on);
+
 /* Device time (ms, HIP events on the context's stream) of the last
  * detect call's kernel chain. */
 int atg_last_timing(const atg_ctx *ctx, double *chain_ms);

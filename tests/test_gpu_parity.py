@@ -60,6 +60,24 @@ def test_stages_match_oracle(atg, ctx, oracle, cfg, seed):
         assert a.shape == b.shape and np.array_equal(a, b), k
 
 
+@pytest.mark.parametrize("cfg,seed", [(1, 1), (4, 3)])
+def test_union_find_scratch_path_matches_oracle(atg, ctx, oracle, cfg, seed):
+    """The union-find replay with its state in HBM scratch (the path of
+    frames too wide for LDS) gives the oracle's roots and detections."""
+    img, p, fams = synth_case(atg, cfg, seed)
+    print("case", cfg, seed, img.shape, flush=True)
+    o = oracle.detect(img, p, fams, stages=True)
+    ctx.force_scratch(True)
+    try:
+        g = gpu_detect(atg, ctx, img, p, fams)
+        st = ctx.stages(0)
+    finally:
+        ctx.force_scratch(False)
+    for k in ("rep", "csize", "ok", "cluster_ids"):
+        assert np.array_equal(o["stages"][k], st[k]), k
+    assert same_dets(o["dets"], g["dets"]) and o["nquads"] == g["nquads"]
+
+
 def test_batch_of_uneven_frames_equals_single(atg, ctx):
     """Frames of four sizes in one pass give what each gives alone."""
     frames = [synth_case(atg, 4, s)[0] for s in range(4)] + [synth_case(atg, 1, 0)[0]]
