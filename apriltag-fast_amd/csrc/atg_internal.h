@@ -13,9 +13,8 @@
 //   fw      u32 [nodes]        component pixel count, valid at final roots
 //   nlab    u32 [nodes]        representative | ok << 31; before that, the
 //                              record of each run within its row (rlab)
-//   recw/rlost/rsym/rroot u32 [nodes]  per record slot (row_off[y] + record):
-//                              weight, what its root lost to, the root of its
-//                              set as replayed, its actual root
+//   recw/rlost/rroot u32 [nodes]  per record slot (row_off[y] + record):
+//                              weight, what its root lost to, its root
 //   nrec    u32 [qh]           records (sets touching the row) per row
 //   plab    u32 [qh][qw]       per-pixel nlab (ok 0 where t is 127)
 //   pcnt/poff u32 [qh + 1]     points per row / their offsets
@@ -92,7 +91,7 @@ struct Frame {
     uint32_t *runs, *rcnt, *row_off;
     uint32_t *parent, *fw, *nlab, *plab;
     uint32_t *rlab;           // = nlab until k_resolve: record of each run within its row
-    uint32_t *recw, *rlost, *rsym, *rroot, *nrec;
+    uint32_t *recw, *rlost, *rroot, *nrec;
     uint32_t *uedge, *ecnt;   // [qh][3 qw] union-find links per row, [qh] their counts
     uint32_t *pcnt, *poff;
     uint64_t *pts, *cpts;
@@ -117,9 +116,9 @@ struct Frame {
 // The replay kernels' memory plan for one call (LDS where it fits, else
 // HBM scratch).
 struct CclPlan {
-    int front, replay, stitch;  // LDS bytes, 0 = scratch
+    int front, replay;          // LDS bytes, 0 = scratch
     int replay_waves;           // waves per frame of k_replay
-    size_t scratch_front, scratch_replay, scratch_stitch;  // HBM bytes per frame
+    size_t scratch_front, scratch_replay;  // HBM bytes per frame
 };
 
 // Optional timing marks between the stages of one call (HIP events on

@@ -13,14 +13,15 @@
 //   k_uf_edges   one wave per row, all rows at once: the row's links in the
 //                reference's order (current run ascending, then previous
 //                run ascending).
-//   k_front      one wave per frame, an unordered lock-free union-find row
+//   k_front      one block per frame, an unordered lock-free union-find row
 //                by row: which runs of row y share a set over rows 0..y
 //                (the row's records) and each record's weight.
 //   k_replay     one wave per row, all rows at once: the row's links replayed
 //                in order over the previous row's records (weights known,
 //                roots named symbolically) and the row's runs.
-//   k_stitch     one wave per frame, row by row: turns the symbolic roots
-//                into nodes and writes the final weight of each set that ends.
+//   k_root_jump  pointer jumping over the records, all rows at once: the
+//   k_lost       symbolic roots become nodes; each record root that lost
+//                gets its parent, each set that ends its final weight.
 //   k_resolve    one thread per node follows parent[] to the final root;
 //                nlab = root | (pixel count >= min_cluster_pixels) << 31
 //                (fast/clusters.c:73-94).
@@ -146,7 +147,7 @@ constexpr uint32_t kLostRetired = 0xfffffffeu;  // the record's set ended at the
 // Step 1, k_front: the sets that touch each row, with their weights.
 //
 // Connectivity and pixel counts do not depend on the link order, so one
-// wave per frame sweeps the rows with an unordered lock-free union-find:
+// block per frame sweeps the rows with an unordered lock-free union-find:
 // the elements of row y are the records of row y-1 (distinct sets), the
 // current runs and the row's virtual node; after the unions, the roots
 // that hold a current run become row y's records, numbered in element
@@ -180,13 +181,31 @@ __device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, ui
     }
 }
 
-size_t front_bytes(int qw) { return (size_t)36 * qw + 128; }
+// Staged rows: the first kStage runs and links of a row come from
+// registers loaded one row ahead; the rest (rare) from HBM at use.
+constexpr int kFrontThreads = 256;
+constexpr int kStage = 1024;
+
+size_t front_bytes(int qw) { return (size_t)36 * qw + (size_t)8 * kStage + 128; }
+
+// A barrier of the whole block over LDS only (the scratch variant orders
+// HBM too).
+template <bool G>
+__device__ __forceinline__ void bsync() {
+    if constexpr (G) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
+}
 
 template <bool G>
-__device__ void front_sweep(const Frame &F, uint8_t *base) {
+__device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) {
     using U32 = typename Mem<G>::U32;
     using U16 = typename Mem<G>::U16;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int QW = F.qw, QH = F.qh;
     const int EM = 2 * QW + 2;
     U32 *p = reinterpret_cast<U32 *>(base);
@@ -194,82 +213,112 @@ __device__ void front_sweep(const Frame &F, uint8_t *base) {
     U32 *flag = acc + EM;
     U32 *W = flag + EM;
     U32 *Wn = W + QW + 1;
-    U16 *labA = reinterpret_cast<U16 *>(Wn + QW + 1);
+    U32 *srun = Wn + QW + 1;       // [kStage] the row's runs
+    U32 *sedge = srun + kStage;    // [kStage] the row's links
+    U16 *labA = reinterpret_cast<U16 *>(sedge + kStage);
     U16 *labB = labA + QW;
-    // row offsets: exclusive scan of the per-row run counts
-    uint32_t carry = 0;
-    for (int y0 = 0; y0 < QH; y0 += 64) {
-        int y = y0 + lane;
-        uint32_t c = y < QH ? F.rcnt[y] : 0u;
-        uint32_t inc = wave_incl_scan(c);
-        if (y < QH) F.row_off[y] = carry + inc - c;
-        carry += __shfl(inc, 63, 64);
-    }
-    const uint32_t vb = carry;  // the virtual last-column nodes start here
-    if (lane == 0) {
-        F.row_off[QH] = vb;
-        F.counters[kNRuns] = vb;
-    }
-    for (int y = lane; y < QH; y += 64) {
-        F.parent[vb + y] = vb + y;
-        F.fw[vb + y] = 1;
+    uint32_t *wsum = lds_small;    // [4] per-wave totals of a block scan
+    // row offsets: exclusive scan of the per-row run counts (wave 0)
+    if (wave == 0) {
+        uint32_t carry = 0;
+        for (int y0 = 0; y0 < QH; y0 += 64) {
+            int y = y0 + lane;
+            uint32_t c = y < QH ? F.rcnt[y] : 0u;
+            uint32_t inc = wave_incl_scan(c);
+            if (y < QH) F.row_off[y] = carry + inc - c;
+            carry += __shfl(inc, 63, 64);
+        }
+        if (lane == 0) {
+            F.row_off[QH] = carry;
+            F.counters[kNRuns] = carry;
+        }
+        for (int y = lane; y < QH; y += 64) {
+            F.parent[carry + y] = carry + y;
+            F.fw[carry + y] = 1;
+        }
     }
     // row 0: each run is its own set
     int m = (int)F.rcnt[0];
-    for (int i = lane; i < m; i += 64) {
-        uint32_t r = F.runs[i], len = run_end(r) - run_start(r) + 1;
+    for (int i = tid; i < m; i += kFrontThreads) {
+        uint32_t r = F.runs[i];
         labA[i] = (uint16_t)i;
-        W[i] = len;
-        F.rlab[i] = (uint32_t)i;
-        F.recw[i] = len;
+        W[i] = run_end(r) - run_start(r) + 1;
+        F.rroot[i] = (uint32_t)i;
         F.parent[i] = (uint32_t)i;
     }
-    if (lane == 0) F.nrec[0] = (uint32_t)m;
-    // per 64 rows: run counts, link counts and row offsets in registers
-    uint32_t c_rc = 0, c_ec = 0, c_ro = 0, ro_carry = 0;
-    for (int y = 1; y < QH; y++) {
-        if (y == 1 || (y & 63) == 0) {
-            int yy = (y & ~63) + lane;
-            c_rc = yy < QH ? F.rcnt[yy] : 0u;
-            c_ec = yy < QH ? F.ecnt[yy] : 0u;
-            uint32_t inc = wave_incl_scan(c_rc);
-            if (y == 1) ro_carry = 0;
-            c_ro = ro_carry + inc - c_rc;
-            ro_carry += __shfl(inc, 63, 64);
-        }
-        const int ncur = (int)__shfl(c_rc, y & 63, 64);
-        const int ne = (int)__shfl(c_ec, y & 63, 64);
-        const uint32_t cbase = __shfl(c_ro, y & 63, 64);
+    uint32_t pbase = 0;  // row_off of the previous row
+    int pcur = m;        // runs of the previous row
+    // the next row, loaded one row ahead
+    uint32_t nx_rc = 0, nx_ec = 0, nx_r[4], nx_e[4];
+    uint8_t nx_t1 = 0, nx_t2 = 0;
+    auto prefetch = [&](int y) {
         const uint32_t *cur = F.runs + (int64_t)y * QW;
         const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
+        nx_rc = F.rcnt[y];
+        nx_ec = F.ecnt[y];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int i = tid + j * kFrontThreads;
+            nx_r[j] = i < QW ? cur[i] : 0u;
+            nx_e[j] = i < 3 * QW ? edges[i] : 0u;
+        }
+        const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
+        nx_t1 = trow[QW - 1];
+        nx_t2 = trow[QW - 2];
+    };
+    if (QH > 1) prefetch(1);
+    bsync<G>();
+    for (int y = 1; y < QH; y++) {
+        // stage the row
+        const int ncur = (int)nx_rc, ne = (int)nx_ec;
+        const uint8_t t1 = nx_t1, t2 = nx_t2;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int i = tid + j * kFrontThreads;
+            if (i < ncur) srun[i] = nx_r[j];
+            if (i < ne) sedge[i] = nx_e[j];
+        }
+        // the previous row's outputs
+        for (int i = tid; i < pcur; i += kFrontThreads) F.rlab[pbase + i] = labA[i];
+        for (int l = tid; l < m; l += kFrontThreads) F.recw[pbase + l] = W[l];
+        if (tid == 0) F.nrec[y - 1] = (uint32_t)m;
+        const uint32_t cbase = pbase + (uint32_t)pcur;
+        if (y + 1 < QH) prefetch(y + 1);
+        const uint32_t *cur = F.runs + (int64_t)y * QW;
+        const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
+        auto run_at = [&](int i) -> uint32_t { return i < kStage ? (uint32_t)srun[i] : cur[i]; };
         const int ev = m + ncur, ne_all = ev + 1;
+        bsync<G>();
         bool lastcol = false;  // fast/unionfind.c:93-97
         if (ncur > 0) {
-            uint32_t lr = cur[ncur - 1];
-            const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
-            lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
+            uint32_t lr = run_at(ncur - 1);
+            lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && t1 == 255 && t2 != 255;
         }
-        wsync<G>();
-        for (int x = lane; x < ne_all; x += 64) {
+        for (int x = tid; x < ne_all; x += kFrontThreads) {
             uint32_t w;
-            if (x < m) w = W[x];
-            else if (x < ev) { uint32_t r = cur[x - m]; w = run_end(r) - run_start(r) + 1; }
-            else w = 1;
+            if (x < m) {
+                w = W[x];
+            } else if (x < ev) {
+                uint32_t r = run_at(x - m);
+                w = run_end(r) - run_start(r) + 1;
+            } else {
+                w = 1;
+            }
             p[x] = (uint32_t)x;
             acc[x] = w;
             flag[x] = 0;
         }
-        wsync<G>();
-        for (int e = lane; e < ne + (lastcol ? 1 : 0); e += 64) {
+        bsync<G>();
+        for (int e = tid; e < ne + (lastcol ? 1 : 0); e += kFrontThreads) {
             if (e < ne) {
-                uint32_t ed = edges[e];
+                uint32_t ed = e < kStage ? (uint32_t)sedge[e] : edges[e];
                 fr_unite<G>(p, (uint32_t)m + (ed >> 16), (uint32_t)labA[ed & 0xffffu]);
             } else {
                 fr_unite<G>(p, (uint32_t)(ev - 1), (uint32_t)ev);
             }
         }
-        wsync<G>();
-        for (int x = lane; x < ne_all; x += 64) {
+        bsync<G>();
+        for (int x = tid; x < ne_all; x += kFrontThreads) {
             uint32_t r = fr_find<G>(p, (uint32_t)x);
             if (r != (uint32_t)x) {
                 p[x] = r;
@@ -277,43 +326,50 @@ __device__ void front_sweep(const Frame &F, uint8_t *base) {
             }
             if (x >= m && x < ev) flag[r] = 1;
         }
-        wsync<G>();
-        // row y's records: the roots holding a current run, in element order
-        uint32_t nrec = 0;
-        for (int x0 = 0; x0 < ne_all; x0 += 64) {
-            int x = x0 + lane;
-            bool f = x < ne_all && flag[x] != 0;
-            uint64_t bal = __ballot(f);
-            if (f) {
-                uint32_t id = nrec + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        bsync<G>();
+        // row y's records: the roots holding a current run, in element
+        // order; each thread numbers a contiguous block of elements
+        const int per = (ne_all + kFrontThreads - 1) / kFrontThreads;
+        const int x0 = tid * per, x1 = min(x0 + per, ne_all);
+        uint32_t cnt = 0;
+        for (int x = x0; x < x1; x++) cnt += flag[x] != 0 ? 1u : 0u;
+        uint32_t inc = wave_incl_scan(cnt);
+        if (lane == 63) wsum[wave] = inc;
+        bsync<G>();
+        uint32_t id = inc - cnt;
+        for (int w = 0; w < wave; w++) id += wsum[w];
+        const uint32_t nrec = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        for (int x = x0; x < x1; x++) {
+            if (flag[x] != 0) {
                 flag[x] = id + 1;
-                uint32_t w = acc[x];
-                Wn[id] = w;
-                F.recw[cbase + id] = w;
+                Wn[id] = acc[x];
+                id++;
             }
-            nrec += (uint32_t)__popcll(bal);
         }
-        wsync<G>();
-        for (int i = lane; i < ncur; i += 64) {
-            uint32_t l = flag[p[m + i]] - 1;
-            labB[i] = (uint16_t)l;
-            F.rlab[cbase + i] = l;
-        }
-        if (lane == 0) F.nrec[y] = nrec;
+        bsync<G>();
+        for (int i = tid; i < ncur; i += kFrontThreads) labB[i] = (uint16_t)(flag[p[m + i]] - 1);
+        bsync<G>();
         m = (int)nrec;
+        pbase = cbase;
+        pcur = ncur;
         U32 *tw = W; W = Wn; Wn = tw;
         U16 *tl = labA; labA = labB; labB = tl;
     }
+    // the last row's outputs
+    for (int i = tid; i < pcur; i += kFrontThreads) F.rlab[pbase + i] = labA[i];
+    for (int l = tid; l < m; l += kFrontThreads) F.recw[pbase + l] = W[l];
+    if (tid == 0) F.nrec[QH - 1] = (uint32_t)m;
 }
 
-__global__ __launch_bounds__(64) void k_front(const Frame *frames, int lds_bytes, uint8_t *gscratch,
-                                              int64_t gstride) {
+__global__ __launch_bounds__(256) void k_front(const Frame *frames, int lds_bytes, uint8_t *gscratch,
+                                               int64_t gstride) {
     extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t small[4];
     const Frame F = frames[blockIdx.y];
     if (lds_bytes > 0)
-        front_sweep<false>(F, lds);
+        front_sweep<false>(F, lds, small);
     else
-        front_sweep<true>(F, gscratch + (int64_t)blockIdx.y * gstride);
+        front_sweep<true>(F, gscratch + (int64_t)blockIdx.y * gstride, small);
 }
 
 // ---------------------------------------------------------------------
@@ -330,8 +386,8 @@ __global__ __launch_bounds__(64) void k_front(const Frame *frames, int lds_bytes
 // order (fast/unionfind.c:15-45, 112-146), the last-column link last.
 // Output: parent[] of the row's runs and virtual node (a node or a
 // record slot), rlost[slot] of each previous record (the winner it lost
-// to, or kLostNone / kLostRetired), rsym[slot] of each row-y record (the
-// root of its set: a node or a previous record's slot).
+// to, or kLostNone / kLostRetired), rroot[slot] of each row-y record (the
+// root of its set as replayed: a node or kSym | a previous record's slot).
 
 size_t replay_bytes(int qw) { return (size_t)34 * qw + 128; }
 
@@ -487,7 +543,7 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base) {
     }
     for (int x = m + lane; x <= ev; x += 64) F.parent[name((uint32_t)x)] = name(ent[2 * x]);
     for (int i = lane; i < ncur; i += 64)
-        F.rsym[cb + F.rlab[cb + i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
+        F.rroot[cb + F.rlab[cb + i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
 }
 
 __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int lds_bytes, uint8_t *gscratch,
@@ -505,60 +561,42 @@ __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int lds_byte
 }
 
 // ---------------------------------------------------------------------
-// Step 3, k_stitch: the actual roots of the records, row by row (one wave
-// per frame). root[slot] of row y-1's records is known; each lost record's
-// root gets parent = its winner; each retired record's root gets its final
-// weight; row y's records take the root their rsym names.
+// Step 3: the actual roots. rroot[slot] names a node or a record of the
+// previous row, so the roots resolve by pointer jumping over the slots
+// (k_root_jump, one pass per doubling of the resolved distance, every row
+// at once); then k_lost gives each record root that lost in the next row
+// its winner as parent, and each set that ended its final weight.
 
-size_t stitch_bytes(int qw) { return (size_t)8 * qw + 64; }
-
-template <bool G>
-__device__ void stitch(const Frame &F, uint8_t *base) {
-    using U32 = typename Mem<G>::U32;
-    const int lane = threadIdx.x;
-    const int QW = F.qw, QH = F.qh;
-    U32 *rootA = reinterpret_cast<U32 *>(base);
-    U32 *rootB = rootA + QW + 1;
-    int m = (int)F.nrec[0];
-    for (int l = lane; l < m; l += 64) {
-        rootA[l] = (uint32_t)l;
-        F.rroot[l] = (uint32_t)l;
+__global__ __launch_bounds__(256) void k_root_jump(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int y = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (y < 1 || y >= F.qh) return;
+    const uint32_t cb = F.row_off[y];
+    const int n = (int)F.nrec[y];
+    for (int l = lane; l < n; l += 64) {
+        uint32_t v = F.rroot[cb + l];
+        if (v & kSym) F.rroot[cb + l] = F.rroot[v & ~kSym];  // an ancestor's value: never a wrong root
     }
-    uint32_t pb = 0;
-    for (int y = 1; y < QH; y++) {
-        const uint32_t cb = F.row_off[y];
-        const int m2 = (int)F.nrec[y];
-        wsync<G>();
-        for (int l = lane; l < m; l += 64) {
-            uint32_t x = F.rlost[pb + l];
-            if (x == kLostRetired) {
-                F.fw[rootA[l]] = F.recw[pb + l];
-            } else if (x != kLostNone) {
-                F.parent[rootA[l]] = (x & kSym) ? rootA[(x & ~kSym) - pb] : x;
-            }
-        }
-        for (int l = lane; l < m2; l += 64) {
-            uint32_t s = F.rsym[cb + l];
-            uint32_t r = (s & kSym) ? rootA[(s & ~kSym) - pb] : s;
-            rootB[l] = r;
-            F.rroot[cb + l] = r;
-        }
-        wsync<G>();
-        U32 *t = rootA; rootA = rootB; rootB = t;
-        m = m2;
-        pb = cb;
-    }
-    for (int l = lane; l < m; l += 64) F.fw[rootA[l]] = F.recw[pb + l];
 }
 
-__global__ __launch_bounds__(64) void k_stitch(const Frame *frames, int lds_bytes, uint8_t *gscratch,
-                                               int64_t gstride) {
-    extern __shared__ __align__(16) uint8_t lds[];
+// Row y (1..qh) settles the records of row y-1; row qh ends them all.
+__global__ __launch_bounds__(256) void k_lost(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
-    if (lds_bytes > 0)
-        stitch<false>(F, lds);
-    else
-        stitch<true>(F, gscratch + (int64_t)blockIdx.y * gstride);
+    const int lane = threadIdx.x & 63;
+    const int y = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (y < 1 || y > F.qh) return;
+    const uint32_t pb = F.row_off[y - 1];
+    const int m = (int)F.nrec[y - 1];
+    for (int l = lane; l < m; l += 64) {
+        uint32_t root = F.rroot[pb + l];
+        uint32_t x = y < F.qh ? F.rlost[pb + l] : kLostRetired;
+        if (x == kLostRetired) {
+            F.fw[root] = F.recw[pb + l];
+        } else if (x != kLostNone) {
+            F.parent[root] = (x & kSym) ? F.rroot[x & ~kSym] : x;
+        }
+    }
 }
 
 // One thread per node: the final root (a parent naming a record slot goes
@@ -625,35 +663,35 @@ static bool lds_ok(const void *fn, size_t need, bool force_scratch) {
 
 CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch) {
     CclPlan c{};
-    size_t f = front_bytes(max_qw), r = replay_bytes(max_qw), s = stitch_bytes(max_qw);
+    size_t f = front_bytes(max_qw), r = replay_bytes(max_qw);
     c.front = lds_ok(reinterpret_cast<const void *>(&k_front), f, force_scratch) ? (int)f : 0;
     c.replay = lds_ok(reinterpret_cast<const void *>(&k_replay), r, force_scratch) ? (int)r : 0;
-    c.stitch = lds_ok(reinterpret_cast<const void *>(&k_stitch), s, force_scratch) ? (int)s : 0;
     c.replay_waves = c.replay ? std::max(1, max_qh - 1) : std::min(std::max(1, max_qh - 1), 64);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     c.scratch_front = c.front ? 0 : al(f);
-    c.scratch_stitch = c.stitch ? 0 : al(s);
     c.scratch_replay = c.replay ? 0 : al(r) * (size_t)c.replay_waves;
     return c;
 }
 
-size_t ccl_scratch_bytes(const CclPlan &c) { return c.scratch_front + c.scratch_replay + c.scratch_stitch; }
+size_t ccl_scratch_bytes(const CclPlan &c) { return c.scratch_front + c.scratch_replay; }
 
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                       int max_qh, const CclPlan &c, uint8_t *scratch, Marks *mk) {
     hipLaunchKernelGGL(k_uf_edges, dim3((unsigned)((max_qh + 3) / 4), nframes), dim3(256), 0, st, dframes);
     mk->mark(st, "uf_edges");
-    // scratch per frame: [front][replay waves][stitch], frames back to back
+    // scratch per frame: [front][replay waves], frames back to back
     int64_t per = (int64_t)ccl_scratch_bytes(c);
-    uint8_t *s_front = scratch, *s_replay = scratch + c.scratch_front,
-            *s_stitch = scratch + c.scratch_front + c.scratch_replay;
+    uint8_t *s_front = scratch, *s_replay = scratch + c.scratch_front;
     int64_t rstride = c.replay ? 0 : (int64_t)(c.scratch_replay / (size_t)c.replay_waves);
-    hipLaunchKernelGGL(k_front, dim3(1, nframes), dim3(64), (size_t)c.front, st, dframes, c.front, s_front, per);
+    hipLaunchKernelGGL(k_front, dim3(1, nframes), dim3(kFrontThreads), (size_t)c.front, st, dframes, c.front, s_front,
+                       per);
     mk->mark(st, "uf_front");
     hipLaunchKernelGGL(k_replay, dim3((unsigned)c.replay_waves, nframes), dim3(64), (size_t)c.replay, st, dframes,
                        c.replay, s_replay, per, rstride);
     mk->mark(st, "uf_replay");
-    hipLaunchKernelGGL(k_stitch, dim3(1, nframes), dim3(64), (size_t)c.stitch, st, dframes, c.stitch, s_stitch, per);
+    const dim3 rows4((unsigned)((max_qh + 4) / 4), nframes);
+    for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rows4, dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_lost, rows4, dim3(256), 0, st, dframes);
     mk->mark(st, "uf_stitch");
     int64_t maxnodes = (int64_t)max_qw * max_qh + max_qh;
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,

@@ -295,7 +295,6 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->rlab = F->nlab;
     F->recw = (uint32_t *)take((size_t)nodes * 4);
     F->rlost = (uint32_t *)take((size_t)nodes * 4);
-    F->rsym = (uint32_t *)take((size_t)nodes * 4);
     F->rroot = (uint32_t *)take((size_t)nodes * 4);
     F->nrec = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->plab = (uint32_t *)take((size_t)px * 4);
