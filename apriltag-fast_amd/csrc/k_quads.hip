@@ -277,28 +277,12 @@ __device__ void order_run(const uint64_t *orig, int64_t n, uint64_t *r, int64_t 
     for (int64_t q = 0; q < m; q++) r[q] = ent[q];
 }
 
-__device__ void fix_runs(uint64_t *res, int64_t n, const uint64_t *orig, uint64_t *ent, uint64_t *bh) {
-    int64_t i = 1;
-    while (i < n) {
-        if (res[i] - res[i - 1] >= 5) { i++; continue; }
-        uint64_t x = res[i - 1];
-        if ((x >> 32) != (res[i] >> 32)) { i++; continue; }
-        int64_t s = i - 1;
-        while (s > 0 && (res[s - 1] >> 32) == (x >> 32)) s--;
-        int64_t e = i + 1;
-        while (e < n && (res[e] >> 32) == (x >> 32)) e++;
-        order_run(orig, n, res + s, e - s, ent, bh);
-        i = e + 1;
-    }
-}
-
 // ------------------------------------------------------------ moments
 
 constexpr int kMomChunk = 512;
 
 __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int parity) {
     __shared__ double sm[6][kMomChunk];
-    __shared__ int s_fix;
     const Frame F = frames[blockIdx.y];
     const uint32_t ncl = F.counters[kNClusters];
     const int t = threadIdx.x;
@@ -309,17 +293,26 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
         if (!(F.cflag[c] & 1u)) continue;
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         uint64_t *keys = skey + s0;
-        if (t == 0) s_fix = 0;
-        __syncthreads();
         // the full keys in order: slope << 32 | ~index
         for (uint32_t k = t; k < n; k += 256) keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
         __syncthreads();
-        for (uint32_t k = 1 + t; k < n; k += 256)
-            if (keys[k] - keys[k - 1] < 5 && (keys[k] >> 32) == (keys[k - 1] >> 32)) s_fix = 1;
-        __syncthreads();
-        if (s_fix && t == 0)
-            fix_runs(keys, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
-                     reinterpret_cast<uint64_t *>(F.filt + s0));
+        // fix_runs (fast/quads.c:304-327) reorders each maximal run of equal
+        // slopes holding two neighbours less than 5 apart; the runs are
+        // disjoint and each reads only its own keys, so the thread of the
+        // run's first such pair orders it, with scratch at the run's offset
+        for (uint32_t k = 1 + t; k < n; k += 256) {
+            const uint64_t hk = keys[k] >> 32;
+            if (!(keys[k] - keys[k - 1] < 5 && hk == (keys[k - 1] >> 32))) continue;
+            uint32_t s = k - 1;
+            while (s > 0 && (keys[s - 1] >> 32) == hk) s--;
+            bool first = true;
+            for (uint32_t j = s + 1; j < k && first; j++) first = !(keys[j] - keys[j - 1] < 5);
+            if (!first) continue;
+            uint32_t e = k + 1;
+            while (e < n && (keys[e] >> 32) == hk) e++;
+            order_run(F.okey + s0, n, keys + s, e - s, reinterpret_cast<uint64_t *>(F.errs + s0) + s,
+                      reinterpret_cast<uint64_t *>(F.filt + s0) + s);
+        }
         __syncthreads();
         // weights (fast/quads.c:968-1004) and the six running sums
         double run[6] = {0, 0, 0, 0, 0, 0};
