@@ -72,7 +72,10 @@ struct QuadRec {
 
 // counters[] of a frame
 enum {
-    kNRuns = 0, kNPts, kNClusters, kNQuads, kNDets, kError, kNCounters = 16
+    kNRuns = 0, kNPts, kNClusters, kNQuads, kNDets, kError,
+    kNEdges,    // union-find links (for the byte counts of the measurements)
+    kNKeptPts,  // points of the clusters that go on to the quad fit
+    kNCounters = 16
 };
 
 struct Frame {

@@ -113,6 +113,7 @@ __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames) {
     if (lane == 0) {
         if (carry > cap) atomicOr(F.counters + kError, 1u);
         F.ecnt[y] = min(carry, cap);
+        atomicAdd(F.counters + kNEdges, min(carry, cap));
     }
 }
 

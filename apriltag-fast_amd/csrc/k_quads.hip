@@ -174,6 +174,7 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
             bool rev = dot < 0.0f;
             bool keep = !((!P->reversed_border && rev) || (!P->normal_border && !rev));
             F.cflag[c] = keep ? (1u | (rev ? 2u : 0u)) : 0u;
+            if (keep) atomicAdd(F.counters + kNKeptPts, n);
         }
         __syncthreads();
     }

@@ -15,11 +15,9 @@ times, the roofline of the dominant kernel, and the reference CPU
 detector timed on the host (1 thread and all cores).
 """
 import argparse
-import ctypes as C
 import json
 import os
 import sys
-import threading
 import time
 from pathlib import Path
 
@@ -43,17 +41,48 @@ def pctl(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
-def stage_bytes(st, W, H):
-    """Algorithmic HBM bytes of stages (a)-(c) of one frame: each array
-    written once by its producer and read once by each consumer."""
+def frame_counts(ctx, i, W, H):
+    """The data-dependent sizes of frame i of the last call: R runs, N
+    nodes, E union-find links, P points, C clusters, K points of the
+    clusters that reach the quad fit, Q quads."""
+    c = ctx.counters(i)
+    st = ctx.stages(i)
     qw, qh = st["qw"], st["qh"]
-    R = len(st["runs"])
-    N = R + qh
-    P = len(st["pts"])
-    a = W * H + 2 * qw * qh
-    b = qw * qh + 10 * R + 4 * N + 4 * qh
-    c = qw * qh + 5 * R + 4 * N + 8 * P
-    return dict(a=a, b=b, c=c, R=R, N=N, P=P)
+    R = c[0]
+    return dict(W=W, H=H, qw=qw, qh=qh, R=R, N=R + qh, E=c[6], P=c[1], C=c[2], K=c[7], Q=c[3])
+
+
+def kernel_bytes(n, sort_passes=7):
+    """Algorithmic HBM bytes per frame of each kernel group: every array
+    it reads or writes, once (a 4-byte word per run, node, link and record
+    slot; an 8-byte word per point; 48 bytes of moments per fitted point).
+    Stages (a)-(c) follow the survey's per-frame formulas split by kernel;
+    (d)-(e) count the arrays of the fit and, for decode, the pixels each
+    quad samples (refine: 4 edges x 16 samples x 25 steps x 2 bilinear
+    reads of 4 pixels)."""
+    W, H, qw, qh = n["W"], n["H"], n["qw"], n["qh"]
+    R, N, E, P, K, Q = n["R"], n["N"], n["E"], n["P"], n["K"], n["Q"]
+    px, tiles = qw * qh, (qw // 4) * (qh // 4)
+    lg = max(1, (qh - 1).bit_length())
+    return {
+        "decimate": W * H + px + 2 * tiles,
+        "threshold_rle": px + 2 * tiles + px + 4 * R + 4 * qh,
+        "uf_edges": 8 * R + 4 * E + 4 * qh,
+        "uf_front": 4 * R + 4 * E + 4 * R + 4 * N + 12 * qh,
+        "uf_replay": 4 * R + 4 * E + 4 * R + 4 * N + 12 * N,
+        "uf_stitch": lg * 8 * N + 16 * N,
+        "resolve": 16 * N,
+        "labels": px + 4 * px + 4 * N,
+        "points": 2 * (px + 4 * px) + 12 * P,
+        "cluster_sort": sort_passes * 32 * P,
+        "clusters": 24 * P,
+        "fit_prep": 8 * P + 20 * P,
+        "angle_sort": sort_passes * 32 * P,
+        "moments": 72 * K,
+        "corners": 80 * K,
+        "decode": Q * 4 * 16 * 25 * 8,
+        "finish": Q * 168,
+    }
 
 
 def cpu_baseline(frames, sample_s):
@@ -98,7 +127,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--cpu-sample-s", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -166,35 +195,23 @@ def main():
     chain_ms = ctx.chain_ms()
     # algorithmic bytes of the batch, from the first 10 frames' counts
     ctx.detect_batch(dev, prm, on_device=True)
-    per = [stage_bytes(ctx.stages(i), W, H) for i in range(len(dev))]
+    counts = [frame_counts(ctx, i, W, H) for i in range(len(dev))]
     reps = args.batch / len(dev)
-    bytes_a = sum(x["a"] for x in per) * reps
-    bytes_b = sum(x["b"] for x in per) * reps
-    bytes_c = sum(x["c"] for x in per) * reps
-    sm = dict(stages)
-    dom = max(stages, key=lambda kv: kv[1])
-    # the stage (a)-(c) kernels with their algorithmic bytes
-    kernel_bytes = {"decimate": None, "threshold_rle": bytes_a, "uf_sweep": bytes_b,
-                    "points": bytes_c}
-    if dom[0] in kernel_bytes and kernel_bytes[dom[0]]:
-        kb, kname = kernel_bytes[dom[0]], dom[0]
-    else:
-        kb, kname = None, dom[0]
-    roof = None
-    if kb:
-        ach = kb / (dom[1] * 1e-3) / 1e9
-        roof = dict(kernel=kname, bound="hbm", achieved=round(ach, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 6), traffic=None,
-                    bytes_per_launch=int(kb), ms_per_launch=round(dom[1], 4))
-    else:
-        # the dominant stage is not an (a)-(c) byte stage: report it with the
-        # (a)-(c) bytes over the whole (a)-(c) device time
-        t_ac = sum(v for k, v in stages if k in ("decimate", "threshold_rle", "uf_sweep", "resolve", "labels",
-                                                  "points", "cluster_sort", "clusters"))
-        ach = (bytes_a + bytes_b + bytes_c) / (t_ac * 1e-3) / 1e9
-        roof = dict(kernel="stages a-c (dominant stage: %s %.3f ms)" % dom, bound="hbm", achieved=round(ach, 3),
-                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 6), traffic=None,
-                    bytes_per_launch=int(bytes_a + bytes_b + bytes_c), ms_per_launch=round(t_ac, 4))
+    kb = {}
+    for n in counts:
+        for k, v in kernel_bytes(n).items():
+            kb[k] = kb.get(k, 0) + v * reps
+    dom_name, dom_ms = max(stages, key=lambda kv: kv[1])
+    ach = kb[dom_name] / (dom_ms * 1e-3) / 1e9
+    bound = "hbm"
+    roof = dict(kernel=dom_name, bound=bound, achieved=round(ach, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(ach / HBM_PEAK_GBS, 6), traffic=None, bytes_per_launch=int(kb[dom_name]),
+                ms_per_launch=round(dom_ms, 4))
+    ac = ("decimate", "threshold_rle", "uf_edges", "uf_front", "uf_replay", "uf_stitch", "resolve", "labels",
+          "points", "cluster_sort", "clusters")
+    t_ac = sum(v for k, v in stages if k in ac)
+    b_ac = sum(kb[k] for k in ac if k in kb)
+    per_kernel = {k: {"ms": round(v, 4), "GB/s": round(kb.get(k, 0) / (v * 1e-3) / 1e9, 2)} for k, v in stages}
 
     cpu = None if args.no_cpu else cpu_baseline(frames, args.cpu_sample_s)
     out = {
@@ -221,6 +238,8 @@ def main():
         "detections_per_10_frames": int(ndets),
         "device_chain_ms": round(chain_ms, 3),
         "stage_ms": {k: round(v, 4) for k, v in stages},
+        "kernel_gbs": per_kernel,
+        "stages_a_c": {"bytes": int(b_ac), "ms": round(t_ac, 4), "GB/s": round(b_ac / (t_ac * 1e-3) / 1e9, 2)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
