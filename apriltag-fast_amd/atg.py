@@ -71,6 +71,28 @@ class SynthTag(C.Structure):
     _fields_ = [("family", C.c_int32), ("id", C.c_int32), ("corners", C.c_double * 8)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("budget_bytes", C.c_size_t), ("lanes", C.c_int32),
+                ("max_batch", C.c_int32), ("slots_per_stream", C.c_int32)]
+
+
+class Served(C.Structure):
+    _fields_ = [("stream", C.c_int32), ("status", C.c_int32), ("tag", C.c_uint64), ("arrival_ns", C.c_int64),
+                ("admit_ns", C.c_int64), ("done_ns", C.c_int64), ("nquads", C.c_int64), ("ndets", C.c_int64),
+                ("dets", C.c_void_p), ("batch_frames", C.c_int32)]
+
+
+class Arrival(C.Structure):
+    _fields_ = [("stream", C.c_int32), ("at_ns", C.c_int64), ("pixels", C.c_void_p), ("stride", C.c_int64),
+                ("tag", C.c_uint64)]
+
+
+class ServerCounts(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("dropped", C.c_int64), ("completed", C.c_int64),
+                ("batches", C.c_int64), ("max_batch_seen", C.c_int64), ("lanes", C.c_int32),
+                ("lane_arena_bytes", C.c_size_t), ("slot_bytes", C.c_size_t)]
+
+
 # at_detection (include/apriltag_fast.h), 168 bytes
 DET_DTYPE = np.dtype([("family", "<i4"), ("id", "<i4"), ("hamming", "<i4"), ("margin", "<f4"),
                       ("H", "<f8", 9), ("c", "<f8", 2), ("p", "<f8", 8)])
@@ -123,6 +145,33 @@ def lib():
     L.atg_synth_default.argtypes = [C.POINTER(SynthSpec), C.c_int]
     L.atg_synth_frame.argtypes = [C.POINTER(SynthSpec), C.c_uint64, vp, i64, C.POINTER(SynthTag), i32]
     L.atg_synth_frame.restype = C.c_int
+    L.atg_synth_frame_drift.argtypes = [C.POINTER(SynthSpec), C.c_uint64, i64, vp, i64, C.POINTER(SynthTag), i32]
+    L.atg_synth_frame_drift.restype = C.c_int
+    L.atg_server_config_default.argtypes = [C.POINTER(ServerConfig)]
+    L.atg_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(Params), C.POINTER(C.POINTER(Family)),
+                                    C.POINTER(i32), i32, C.POINTER(vp)]
+    L.atg_server_create.restype = C.c_int
+    L.atg_server_destroy.argtypes = [vp]
+    L.atg_server_error.argtypes = [vp]
+    L.atg_server_error.restype = C.c_char_p
+    L.atg_server_now_ns.argtypes = [vp]
+    L.atg_server_now_ns.restype = i64
+    L.atg_server_add_stream.argtypes = [vp, i32, i32, C.POINTER(i32)]
+    L.atg_server_add_stream.restype = C.c_int
+    L.atg_server_submit.argtypes = [vp, i32, vp, i64, C.c_uint64]
+    L.atg_server_submit.restype = C.c_int
+    L.atg_server_feed.argtypes = [vp, C.POINTER(Arrival), i64]
+    L.atg_server_feed.restype = C.c_int
+    L.atg_server_poll.argtypes = [vp, C.POINTER(Served), i32]
+    L.atg_server_poll.restype = C.c_int
+    L.atg_server_drain.argtypes = [vp, i64]
+    L.atg_server_drain.restype = C.c_int
+    L.atg_server_stats.argtypes = [vp, C.POINTER(ServerCounts)]
+    L.atg_server_stats.restype = C.c_int
+    L.atg_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.atg_host_alloc.restype = C.c_int
+    L.atg_host_free.argtypes = [vp]
+    L.atg_host_free.restype = C.c_int
     _lib = L
     return L
 
@@ -270,6 +319,123 @@ class Context:
                    quads=arr(s.quads, 9 * s.nquads, np.float32).reshape(-1, 9))
         self.L.atg_stages_free(C.byref(s))
         return out
+
+
+def _family_arrays(families):
+    L = lib()
+    ptrs = (C.POINTER(Family) * max(1, len(families)))()
+    bits = (C.c_int32 * max(1, len(families)))()
+    for i, (name, b) in enumerate(families):
+        ptrs[i] = L.atg_builtin_family(FAMILY_INDEX[name])
+        bits[i] = b
+    return ptrs, bits
+
+
+class PinnedFrame:
+    """A uint8 HxW frame in pinned host memory (DMA-able without staging)."""
+
+    def __init__(self, h, w):
+        self.ptr = C.c_void_p()
+        if lib().atg_host_alloc(h * w, C.byref(self.ptr)) != 0:
+            raise RuntimeError("atg_host_alloc failed")
+        self.array = np.ctypeslib.as_array((C.c_uint8 * (h * w)).from_address(self.ptr.value)).reshape(h, w)
+
+    def free(self):
+        if self.ptr:
+            lib().atg_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+class Server:
+    """The serving loop (atg_server): streams, arrivals, polled results."""
+
+    def __init__(self, families, params=None, budget_bytes=16 << 30, lanes=3, max_batch=128, slots=2, device=0):
+        self.L = lib()
+        cfg = ServerConfig()
+        self.L.atg_server_config_default(C.byref(cfg))
+        cfg.device, cfg.budget_bytes, cfg.lanes, cfg.max_batch, cfg.slots_per_stream = (
+            device, budget_bytes, lanes, max_batch, slots)
+        self.params = params if params is not None else default_params()
+        ptrs, bits = _family_arrays(families)
+        self.h = C.c_void_p()
+        rc = self.L.atg_server_create(C.byref(cfg), C.byref(self.params), ptrs, bits, len(families),
+                                      C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError("atg_server_create failed (%d)" % rc)
+        self._keep = []
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise RuntimeError("%s: %s (%d)" % (what, self.L.atg_server_error(self.h).decode(), rc))
+        return rc
+
+    def add_stream(self, width, height):
+        sid = C.c_int32()
+        self._check(self.L.atg_server_add_stream(self.h, width, height, C.byref(sid)), "atg_server_add_stream")
+        return sid.value
+
+    def now_ns(self):
+        return self.L.atg_server_now_ns(self.h)
+
+    def submit(self, stream, img, tag=0):
+        """img: a uint8 HxW array that stays alive until its result is polled."""
+        self._keep.append(img)
+        return self.L.atg_server_submit(self.h, stream, img.ctypes.data, img.strides[0], tag)
+
+    def feed(self, arrivals):
+        """arrivals: list of (stream, at_ns, img, tag); imgs stay alive until polled."""
+        arr = (Arrival * max(1, len(arrivals)))()
+        for i, (sid, at, img, tag) in enumerate(arrivals):
+            arr[i] = Arrival(sid, int(at), img.ctypes.data, img.strides[0], tag)
+            self._keep.append(img)
+        self._check(self.L.atg_server_feed(self.h, arr, len(arrivals)), "atg_server_feed")
+
+    def poll(self, cap=1024):
+        out = (Served * cap)()
+        n = self._check(self.L.atg_server_poll(self.h, out, cap), "atg_server_poll")
+        res = []
+        for i in range(n):
+            o = out[i]
+            res.append(dict(stream=o.stream, status=o.status, tag=o.tag, arrival_ns=o.arrival_ns,
+                            admit_ns=o.admit_ns, done_ns=o.done_ns, nquads=o.nquads, batch_frames=o.batch_frames,
+                            dets=_dets(o.dets, o.ndets)))
+        return res
+
+    def drain(self, timeout_ms=60000):
+        self._check(self.L.atg_server_drain(self.h, timeout_ms), "atg_server_drain")
+
+    def counts(self):
+        c = ServerCounts()
+        self._check(self.L.atg_server_stats(self.h, C.byref(c)), "atg_server_stats")
+        return {f: getattr(c, f) for f, _ in ServerCounts._fields_}
+
+    def close(self):
+        if self.h:
+            self.L.atg_server_destroy(self.h)
+            self.h = C.c_void_p()
+            self._keep = []
+
+
+def synth_spec(config, width=None, height=None, ntags=None):
+    s = SynthSpec()
+    lib().atg_synth_default(C.byref(s), config)
+    if width:
+        s.width = width
+    if height:
+        s.height = height
+    if ntags is not None:
+        s.ntags_min = s.ntags_max = ntags
+    return s
+
+
+def synth_stream_frame(spec, seed, step, out=None):
+    """Frame `step` of the camera stream `seed` (tags drifting slowly) into
+    out (a uint8 array, e.g. a PinnedFrame's) or a new array."""
+    img = out if out is not None else np.zeros((spec.height, spec.width), np.uint8)
+    n = lib().atg_synth_frame_drift(C.byref(spec), seed, step, img.ctypes.data, img.strides[0], None, 0)
+    if n < 0:
+        raise RuntimeError("atg_synth_frame_drift failed")
+    return img
 
 
 def synth_frame(config, seed, width=None, height=None, ntags=None):

@@ -16,6 +16,7 @@
 
 #include "atg.h"
 #include "atg_internal.h"
+#include "chain.h"
 
 namespace atg {
 void launch_init(hipStream_t st, const Frame *dframes, int nframes);
@@ -112,6 +113,10 @@ struct atg_ctx {
     Marks marks;                      // per-stage events, when profiling
     bool profiling = false;
     bool force_scratch = false;       // union-find replay state in HBM even where LDS fits
+    bool arena_fixed = false;         // the arena never grows (a serving lane's budget)
+    hipEvent_t ev2 = nullptr;         // the end of the last enqueued chain (with its readback)
+    int32_t inflight = 0;             // frames of the enqueued chain
+    std::vector<int> fstatus;         // per frame of the last call: ATG_OK, 1 (point capacity), ATG_ERR_LIMIT
     std::vector<double> stage_ms;
     std::vector<const char *> stage_names;
 };
@@ -353,6 +358,7 @@ extern "C" int atg_ctx_create(int device, atg_ctx **out) {
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ctx->ev0));
         HIPCHK(hipEventCreate(&ctx->ev1));
+        HIPCHK(hipEventCreateWithFlags(&ctx->ev2, hipEventDisableTiming));
         return ATG_OK;
     }();
     if (!ctx->err.empty()) {
@@ -378,6 +384,7 @@ extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
     hipHostFree(ctx->pinned_out);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->ev2) hipEventDestroy(ctx->ev2);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -402,7 +409,13 @@ extern "C" int atg_set_families(atg_ctx *ctx, const at_family *const *fams, cons
     return ATG_OK;
 }
 
-static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device) {
+namespace atg {
+
+// The first half of a call: plans, arena, frame descriptors, the kernel
+// chain and the readback copy, all queued on the context's stream; the
+// event ev2 marks the end. chain_collect reads the results once it has
+// passed.
+int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device) {
     HIPCHK(hipSetDevice(ctx->device));
     if (!ctx->params_valid || !same_params(ctx->last_params, *p)) {
         int rc = upload_params(ctx, p);
@@ -441,6 +454,7 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
     const CclPlan ccl = ccl_plan(max_qw, max_qh, ctx->force_scratch);
     const size_t gstride = ccl_scratch_bytes(ccl);
     total += gstride * (size_t)n;
+    if (total > ctx->arena_size && ctx->arena_fixed) return fail(ctx, ATG_ERR_LIMIT, "batch over the arena budget");
     if (total > ctx->arena_size) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(hipFree(ctx->arena));
@@ -533,7 +547,24 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
     size_t pre = kCountersBytes + kInlineDets * sizeof(at_detection);
     uint8_t *hout = ctx->pinned_out + sizeof(Frame) * (size_t)n;
     HIPCHK(hipMemcpy2DAsync(hout, pre, ctx->results, kResultBlock, pre, (size_t)n, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipEventRecord(ctx->ev2, st));
+    ctx->inflight = n;
+    return ATG_OK;
+}
+
+bool chain_done(atg_ctx *ctx) { return hipEventQuery(ctx->ev2) == hipSuccess; }
+
+// The second half: after ev2, the counters and detections of every frame.
+// Returns 1 if a frame ran out of point capacity (the caller re-runs the
+// call with more).
+int chain_collect(atg_ctx *ctx) {
+    const int32_t n = ctx->inflight;
+    ctx->inflight = 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipEventSynchronize(ctx->ev2));
+    Marks *mk = &ctx->marks;
+    size_t pre = kCountersBytes + kInlineDets * sizeof(at_detection);
+    uint8_t *hout = ctx->pinned_out + sizeof(Frame) * (size_t)n;
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->chain_ms = ms;
@@ -547,16 +578,23 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
     }
     ctx->dets.assign((size_t)n, {});
     ctx->nquads.assign((size_t)n, 0);
+    ctx->fstatus.assign((size_t)n, ATG_OK);
     bool overflow = false;
+    std::string first_err;
     for (int32_t i = 0; i < n; i++) {
         const uint32_t *cnt = reinterpret_cast<const uint32_t *>(hout + pre * (size_t)i);
         const at_detection *d = reinterpret_cast<const at_detection *>(hout + pre * (size_t)i + kCountersBytes);
         if (ctx->plans[(size_t)i].skip) continue;
-        if (cnt[kError] & 2u) overflow = true;
+        if (cnt[kError] & 2u) {
+            overflow = true;
+            ctx->fstatus[(size_t)i] = 1;
+        }
         if (cnt[kError] & ~2u) {
             char m[128];
             snprintf(m, sizeof m, "frame %d: device capacity exceeded (flags 0x%x)", i, cnt[kError]);
-            return fail(ctx, ATG_ERR_LIMIT, m);
+            if (first_err.empty()) first_err = m;
+            ctx->fstatus[(size_t)i] = ATG_ERR_LIMIT;
+            continue;
         }
         uint32_t nd = std::min<uint32_t>(cnt[kNDets], kResultDets);
         ctx->nquads[(size_t)i] = cnt[kNQuads];
@@ -569,9 +607,72 @@ static int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames,
                                                    first * sizeof(at_detection),
                              (nd - first) * sizeof(at_detection), hipMemcpyDeviceToHost));
     }
+    if (!first_err.empty()) return fail(ctx, ATG_ERR_LIMIT, first_err);
     if (overflow) return 1;  // the caller retries with more point capacity
     return ATG_OK;
 }
+
+int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device) {
+    int rc = chain_enqueue(ctx, p, frames, n, on_device);
+    if (rc) return rc;
+    return chain_collect(ctx);
+}
+
+// Arena bytes of a call over these frames (what chain_enqueue allocates).
+size_t batch_arena_bytes(const atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n,
+                         int on_device) {
+    const int nfam = (int)ctx->fams.size();
+    size_t total = 0;
+    int max_qw = 4, max_qh = 4;
+    for (int32_t i = 0; i < n; i++) {
+        FramePlan fp{};
+        fp.W = frames[i].width;
+        fp.H = frames[i].height;
+        fp.skip = nfam == 0 || fp.W < 8 || fp.H < 8 || !frames[i].pixels;
+        decimated_size(fp.W, fp.H, p->quad_decimate, &fp.qw, &fp.qh);
+        if (!fp.skip && (fp.qw < 4 || fp.qh < 4)) fp.skip = true;
+        if (fp.skip) { fp.qw = fp.qh = 4; fp.W = fp.H = 8; }
+        fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
+        Frame F{};
+        total += layout(fp, nfam, !on_device, &F, nullptr);
+        max_qw = std::max<int>(max_qw, (int)fp.qw);
+        max_qh = std::max<int>(max_qh, (int)fp.qh);
+    }
+    return total + ccl_scratch_bytes(ccl_plan(max_qw, max_qh, ctx->force_scratch)) * (size_t)n;
+}
+
+// A fixed arena of `bytes` and result space for `max_frames`: later calls
+// that need more fail instead of growing it.
+int ctx_reserve(atg_ctx *ctx, size_t bytes, int32_t max_frames) {
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipFree(ctx->arena));
+    ctx->arena = nullptr;
+    HIPCHK(hipMalloc(&ctx->arena, bytes));
+    ctx->arena_size = bytes;
+    ctx->arena_fixed = true;
+    HIPCHK(hipFree(ctx->results));
+    ctx->results = nullptr;
+    ctx->results_size = kResultBlock * (size_t)max_frames;
+    HIPCHK(hipMalloc(&ctx->results, ctx->results_size));
+    HIPCHK(hipFree(ctx->dframes));
+    ctx->dframes = nullptr;
+    ctx->dframes_cap = sizeof(Frame) * (size_t)max_frames;
+    HIPCHK(hipMalloc(&ctx->dframes, ctx->dframes_cap));
+    size_t out_need = (kCountersBytes + kInlineDets * sizeof(at_detection) + sizeof(Frame)) * (size_t)max_frames;
+    HIPCHK(hipHostFree(ctx->pinned_out));
+    ctx->pinned_out = nullptr;
+    HIPCHK(hipHostMalloc(&ctx->pinned_out, out_need));
+    ctx->pinned_out_size = out_need;
+    return ATG_OK;
+}
+
+hipStream_t ctx_stream(atg_ctx *ctx) { return ctx->stream; }
+const std::vector<at_detection> &ctx_dets(const atg_ctx *ctx, int32_t i) { return ctx->dets[(size_t)i]; }
+int ctx_frame_status(const atg_ctx *ctx, int32_t i) { return ctx->fstatus[(size_t)i]; }
+int64_t ctx_nquads(const atg_ctx *ctx, int32_t i) { return ctx->nquads[(size_t)i]; }
+size_t ctx_result_bytes(int32_t max_frames) { return kResultBlock * (size_t)max_frames; }
+
+}  // namespace atg
 
 extern "C" int atg_detect_batch(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device,
                                 atg_result *out) {

@@ -270,8 +270,12 @@ extern "C" void atg_synth_default(atg_synth_spec *s, int config) {
     }
 }
 
-extern "C" int atg_synth_frame(const atg_synth_spec *spec, uint64_t seed, uint8_t *out, int64_t stride,
-                               atg_synth_tag *truth, int32_t max_truth) {
+// The frame of `seed` with its tags moved `step` frames along their drift:
+// each tag has its own velocity (up to 1.5 px a frame) and spin (up to
+// about 0.6 degrees a frame) from a generator of its own, so step 0 is the
+// plain frame; the noise of a step > 0 comes from its own generator.
+static int synth_frame(const atg_synth_spec *spec, uint64_t seed, int64_t step, uint8_t *out, int64_t stride,
+                       atg_synth_tag *truth, int32_t max_truth) {
     Rng r{seed * 0x2545f4914f6cdd1dull + 0x1234567ull};
     Canvas cv{spec->width, spec->height, std::vector<int32_t>((size_t)spec->width * spec->height)};
     int64_t W = spec->width, H = spec->height;
@@ -355,6 +359,14 @@ extern "C" int atg_synth_frame(const atg_synth_spec *spec, uint64_t seed, uint8_
         Rot rot = random_rotation(r);
         Rot t0 = rational_rot(r.uniform(-spec->max_tilt, spec->max_tilt));
         Rot t1 = rational_rot(r.uniform(-spec->max_tilt, spec->max_tilt));
+        if (step != 0) {
+            Rng d{(seed + 0x9e3779b97f4a7c15ull * (uint64_t)(i + 1)) ^ 0xd1b54a32d192ed03ull};
+            double vx = d.uniform(-1.5, 1.5), vy = d.uniform(-1.5, 1.5), w = d.uniform(-0.005, 0.005);
+            cx += vx * (double)step;
+            cy += vy * (double)step;
+            Rot dr = rational_rot(w * (double)step);
+            rot = Rot{rot.c * dr.c - rot.s * dr.s, rot.s * dr.c + rot.c * dr.s};
+        }
         double dst[4][2];
         place_square(cx, cy, side, rot, t0, t1, focal, dst);
         const double src[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
@@ -399,6 +411,7 @@ extern "C" int atg_synth_frame(const atg_synth_spec *spec, uint64_t seed, uint8_
     double blur = r.uniform(spec->blur_min, spec->blur_max);
     if (spec->blur_max > 0.0) box_blur_gauss(cv, blur);
     double noise = r.uniform(spec->noise_min, spec->noise_max);
+    if (step != 0) r = Rng{(seed ^ 0x6a09e667f3bcc909ull) * 0x2545f4914f6cdd1dull + (uint64_t)step * 0x9e3779b97f4a7c15ull};
     int64_t nq = (int64_t)(noise * 256.0 + 0.5);  // sigma in 1/256 gray levels
     for (int64_t y = 0; y < H; y++)
         for (int64_t x = 0; x < W; x++) {
@@ -407,4 +420,14 @@ extern "C" int atg_synth_frame(const atg_synth_spec *spec, uint64_t seed, uint8_
             out[y * stride + x] = (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v);
         }
     return nt;
+}
+
+extern "C" int atg_synth_frame(const atg_synth_spec *spec, uint64_t seed, uint8_t *out, int64_t stride,
+                               atg_synth_tag *truth, int32_t max_truth) {
+    return synth_frame(spec, seed, 0, out, stride, truth, max_truth);
+}
+
+extern "C" int atg_synth_frame_drift(const atg_synth_spec *spec, uint64_t seed, int64_t step, uint8_t *out,
+                                     int64_t stride, atg_synth_tag *truth, int32_t max_truth) {
+    return synth_frame(spec, seed, step, out, stride, truth, max_truth);
 }

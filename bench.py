@@ -85,6 +85,94 @@ def kernel_bytes(n, sort_passes=7):
     }
 
 
+SERVE_SIZES = [(640, 480), (1280, 720), (1920, 1080), (2448, 2048)]
+
+
+def serving_sweep(atg, ctx, loads, seconds, nstreams=32, nframes=4, check_ref=True):
+    """Serving: nstreams camera streams (seeded sizes from SERVE_SIZES, rates
+    30-120 frames/s scaled to each offered load), frames of drifting tags
+    (config-1 scenes) in pinned host memory, Poisson arrivals fed to the
+    server; per load: sustained frames/s and p50/p99 arrival-to-result
+    latency. Every served result is compared with the frame's result alone
+    (and those with the reference CPU detector)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from atg_testlib import Reference, default_params, have_ref, same_dets
+    fams = [("tag36h11", 2)]
+    rng = np.random.default_rng(2024)
+    sizes = [SERVE_SIZES[i] for i in rng.integers(0, 4, nstreams)]
+    rates = rng.uniform(30.0, 120.0, nstreams)
+
+    def gen(k):
+        w, h = sizes[k]
+        spec = atg.synth_spec(1, w, h)
+        out = []
+        for j in range(nframes):
+            pf = atg.PinnedFrame(h, w)
+            atg.synth_stream_frame(spec, 5000 + k, 8 * j, pf.array)
+            out.append(pf)
+        return out
+
+    with ThreadPoolExecutor(16) as ex:
+        frames = list(ex.map(gen, range(nstreams)))
+    ctx.set_families(fams)
+    prm = atg.default_params()
+    alone = {(k, j): ctx.detect(frames[k][j].array, prm) for k in range(nstreams) for j in range(nframes)}
+    ref_ok = None
+    if check_ref and have_ref():
+        ref = Reference()
+        ref_ok = True
+        for (k, j), a in alone.items():
+            r = ref.detect(frames[k][j].array, default_params(), fams)
+            ref_ok = ref_ok and same_dets(r["dets"], a["dets"]) and r["nquads"] == a["nquads"]
+    points, all_exact = [], True
+    for load in loads:
+        srv = atg.Server(fams, lanes=3, max_batch=128, budget_bytes=16 << 30)
+        try:
+            ids = [srv.add_stream(*sizes[k]) for k in range(nstreams)]
+            scale = load / rates.sum()
+            arr = []
+            for k in range(nstreams):
+                t, j = 0.0, 0
+                gap = 1e9 / (rates[k] * scale)
+                while True:
+                    t += rng.exponential(gap)
+                    if t >= seconds * 1e9:
+                        break
+                    arr.append((ids[k], t, frames[k][j % nframes].array, k * 1000000 + j))
+                    j += 1
+            log("serving: offered %d frames/s, %d arrivals" % (load, len(arr)))
+            srv.feed(arr)
+            srv.drain(600000)
+            res = []
+            while True:
+                got = srv.poll(4096)
+                if not got:
+                    break
+                res.extend(got)
+            counts = srv.counts()
+        finally:
+            srv.close()
+        exact = all(r["status"] == 0 and same_dets(r["dets"], alone[(r["tag"] // 1000000, (r["tag"] % 1000000) % nframes)]["dets"])
+                    for r in res)
+        all_exact = all_exact and exact
+        lat = np.array([(r["done_ns"] - r["arrival_ns"]) * 1e-6 for r in res])
+        t0 = min(r["arrival_ns"] for r in res)
+        t1 = max(r["done_ns"] for r in res)
+        points.append(dict(offered_fps=load, arrivals=len(arr), completed=len(res), dropped=counts["dropped"],
+                           sustained_fps=round(len(res) / ((t1 - t0) * 1e-9), 1),
+                           p50_ms=round(float(np.percentile(lat, 50)), 3),
+                           p99_ms=round(float(np.percentile(lat, 99)), 3),
+                           mean_batch=round(float(np.mean([r["batch_frames"] for r in res])), 1),
+                           batches=counts["batches"], bit_exact_vs_alone=bool(exact)))
+    for fr in frames:
+        for pf in fr:
+            pf.free()
+    return dict(streams=nstreams, frame_sizes={"%dx%d" % s: sizes.count(s) for s in SERVE_SIZES},
+                rates_fps="30-120 per stream, scaled to the offered load", budget_gib=16, lanes=3, max_batch=128,
+                seconds_per_point=seconds, points=points, bit_exact_vs_alone=bool(all_exact),
+                alone_bit_exact_vs_cpu_reference=ref_ok)
+
+
 def cpu_baseline(frames, sample_s):
     """The reference detector (oracle/_ref, built from the reference's own
     sources) timed on this host at 1 thread and at all cores, on a bounded
@@ -130,6 +218,9 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--cpu-sample-s", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-loads", default="500,1000,2000,4000,8000,20000")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: frames are independent, so more GPUs would each run this same "
@@ -213,6 +304,10 @@ def main():
     b_ac = sum(kb[k] for k in ac if k in kb)
     per_kernel = {k: {"ms": round(v, 4), "GB/s": round(kb.get(k, 0) / (v * 1e-3) / 1e9, 2)} for k, v in stages}
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_sweep(atg, ctx, [int(x) for x in args.serve_loads.split(",")], args.serve_seconds,
+                                check_ref=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline(frames, args.cpu_sample_s)
     out = {
         "metric": "frames/s @1080p tag36h11 batched vs single-frame; p50/p99 latency, bit-exact",
@@ -241,6 +336,7 @@ def main():
         "kernel_gbs": per_kernel,
         "stages_a_c": {"bytes": int(b_ac), "ms": round(t_ac, 4), "GB/s": round(b_ac / (t_ac * 1e-3) / 1e9, 2)},
         "roofline": roof,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     for d in dev:

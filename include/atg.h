@@ -143,6 +143,76 @@ int atg_device_free(atg_ctx *ctx, void *p);
 int atg_copy_to_device(atg_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------
+ * Serving (no counterpart in the reference): frames from many camera
+ * streams, admitted as they arrive into batches that overlap on the
+ * device. Each stream keeps slots_per_stream input buffers resident in
+ * HBM; `lanes` batches run at once, each a kernel chain on its own stream
+ * with a fixed arena; frames are admitted while the lane's arena holds
+ * them. The budget covers the lanes' arenas and result buffers and the
+ * streams' slots. A frame's detections are the ones atg_detect gives for
+ * it alone. */
+typedef struct atg_server atg_server;
+
+typedef struct {
+    int32_t device;
+    size_t budget_bytes;       /* HBM for frames in flight */
+    int32_t lanes;             /* batches in flight at once (1..8) */
+    int32_t max_batch;         /* frames per batch at most */
+    int32_t slots_per_stream;  /* resident input buffers per stream */
+} atg_server_config;
+
+/* One completed frame. Times are on the server clock (atg_server_now_ns). */
+typedef struct {
+    int32_t stream;
+    int32_t status;            /* ATG_OK or an error code */
+    uint64_t tag;
+    int64_t arrival_ns, admit_ns, done_ns;
+    int64_t nquads;
+    int64_t ndets;
+    const at_detection *dets;  /* valid until the next atg_server_poll */
+    int32_t batch_frames;      /* frames of the batch it ran in */
+} atg_served;
+
+/* A frame to arrive at_ns after atg_server_feed is called. */
+typedef struct {
+    int32_t stream;
+    int64_t at_ns;
+    const uint8_t *pixels;     /* host memory, borrowed until the result is polled */
+    int64_t stride;
+    uint64_t tag;
+} atg_arrival;
+
+typedef struct {
+    int64_t submitted, dropped, completed, batches, max_batch_seen;
+    int32_t lanes;
+    size_t lane_arena_bytes, slot_bytes;
+} atg_server_counts;
+
+void atg_server_config_default(atg_server_config *c);  /* device 0, 16 GiB, 3 lanes, 128, 2 */
+int atg_server_create(const atg_server_config *c, const atg_params *p, const at_family *const *fams,
+                      const int32_t *bits, int32_t nfam, atg_server **out);
+/* Stops the worker; frames not yet arrived are cancelled, admitted ones finish. */
+void atg_server_destroy(atg_server *s);
+const char *atg_server_error(const atg_server *s);
+int64_t atg_server_now_ns(const atg_server *s);
+/* Streams are added before the first frame (the lanes are sized then). */
+int atg_server_add_stream(atg_server *s, int32_t width, int32_t height, int32_t *stream_id);
+/* A frame of `stream` arrives now (pixels borrowed until its result is
+ * polled). ATG_ERR_LIMIT: the stream's queue is full and the frame dropped. */
+int atg_server_submit(atg_server *s, int32_t stream, const uint8_t *pixels, int64_t stride, uint64_t tag);
+/* Frames that arrive at their times, released by the server's own feeder
+ * (precise arrival processes without a caller thread). */
+int atg_server_feed(atg_server *s, const atg_arrival *arrivals, int64_t n);
+/* Completed frames, in completion order: up to cap into out; returns the count. */
+int atg_server_poll(atg_server *s, atg_served *out, int32_t cap);
+/* Waits until every submitted and fed frame is done, at most timeout_ms. */
+int atg_server_drain(atg_server *s, int64_t timeout_ms);
+int atg_server_stats(atg_server *s, atg_server_counts *out);
+/* Pinned host memory (frames that the server copies in by DMA). */
+int atg_host_alloc(size_t bytes, void **out);
+int atg_host_free(void *p);
+
+/* ------------------------------------------------------------------
  * Seeded synthetic frames (host code), for tests and benchmarks. */
 typedef struct {
     int32_t width, height;
@@ -168,6 +238,11 @@ void atg_synth_default(atg_synth_spec *s, int config);
  * tags placed (truth gets up to max_truth of them), or -1. */
 int atg_synth_frame(const atg_synth_spec *spec, uint64_t seed, uint8_t *out, int64_t stride, atg_synth_tag *truth,
                     int32_t max_truth);
+
+/* Frame `step` of a camera stream: the frame of `seed` with each tag moved
+ * step frames along its own slow drift (step 0 = atg_synth_frame). */
+int atg_synth_frame_drift(const atg_synth_spec *spec, uint64_t seed, int64_t step, uint8_t *out, int64_t stride,
+                          atg_synth_tag *truth, int32_t max_truth);
 
 /* The built-in families in the order tag16h5, tag25h9, tag36h10,
  * tag36h11, tagCircle21h7, tagCircle49h12, tagCustom48h12,
