@@ -10,7 +10,15 @@ import pytest
 
 from atg_testlib import PKG, REPO, frame_hash, load_atg, load_families
 
-HEADERS = [REPO / "include" / "apriltag_fast.h", REPO / "include" / "atg.h"]
+HEADERS = [REPO / "include" / "apriltag_fast.h", REPO / "include" / "atg.h", REPO / "include" / "apriltag.h",
+           REPO / "include" / "common" / "matd.h", REPO / "include" / "common" / "image_u8.h"] + \
+    sorted((REPO / "include").glob("tag*.h"))
+
+# sizeof of the upstream API's public structs in the reference (its
+# apriltag.h and common/ headers, x86-64 gcc): code built against them
+# must see the same layouts here.
+UPSTREAM_SIZES = {"apriltag_detector_t": 160, "apriltag_detection_t": 112, "apriltag_family_t": 72,
+                  "image_u8_t": 24, "zarray_t": 24, "matd_t": 16}
 
 
 def declared_functions():
@@ -18,7 +26,7 @@ def declared_functions():
     for h in HEADERS:
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;{]*\)\s*;", text, re.M):
-            if not m.group(0).startswith("typedef"):
+            if not m.group(0).startswith(("typedef", "static")):
                 names.add(m.group(1))
     return sorted(names)
 
@@ -104,3 +112,20 @@ def test_pnm_roundtrip(tmp_path):
     got = np.frombuffer(C.string_at(px, w.value * h.value), np.uint8).reshape(h.value, w.value)
     assert np.array_equal(got, img)
     C.CDLL(None).free(px)
+
+
+def build_upstream_demo(out):
+    """tests/c/upstream_demo.c, built as a user's C program against
+    include/ and the library."""
+    lib_dir = PKG
+    cmd = ["gcc", "-O1", "-Wall", "-Werror", "-I", str(REPO / "include"), str(REPO / "tests" / "c" / "upstream_demo.c"),
+           "-L", str(lib_dir), "-lapriltag_amd", "-Wl,-rpath," + str(lib_dir), "-o", str(out)]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return out
+
+
+def test_upstream_api_builds_with_reference_layouts(tmp_path):
+    exe = build_upstream_demo(tmp_path / "upstream_demo")
+    out = subprocess.run([str(exe), "-s"], check=True, capture_output=True, text=True).stdout
+    sizes = {l.split()[0]: int(l.split()[1]) for l in out.splitlines()}
+    assert sizes == UPSTREAM_SIZES

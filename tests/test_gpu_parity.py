@@ -144,3 +144,26 @@ def test_device_pointer_input(atg, ctx):
     finally:
         ctx.free_device(d[0])
     assert same_dets(a["dets"], b["dets"]) and a["nquads"] == b["nquads"]
+
+
+def test_upstream_api_golden_hex(tmp_path):
+    """The upstream API (apriltag.h) from a C program, every golden case:
+    apriltag_detector_detect gives the reference's detections."""
+    import subprocess
+    from test_abi import build_upstream_demo
+    exe = build_upstream_demo(tmp_path / "upstream_demo")
+    imgs = [str(p) for p in ref_images()]
+    for name, args in CASES:
+        print("case", name, flush=True)
+        out = subprocess.run([str(exe)] + args.split() + imgs, check=True, capture_output=True, text=True,
+                             timeout=120).stdout
+        got = {}
+        for line in out.splitlines():
+            if line.startswith("#"):
+                continue
+            tok = line.split()
+            img = tok[0].rsplit("/", 1)[-1]
+            got.setdefault(img, []).append([int(tok[1]), int(tok[2])] + [float.fromhex(t) for t in tok[3:]])
+        gold = parse_hex(GOLDEN / "ref_hex" / (name + ".hex"))
+        for path in ref_images():
+            assert got.get(path.name, []) == gold.get(path.name, []), (name, path.name)
