@@ -360,7 +360,7 @@ def frame_hash(img):
     return int.from_bytes(hashlib.blake2b(np.ascontiguousarray(img).tobytes(), digest_size=8).digest(), "little")
 
 
-SYNTH_CASES = [(1, s) for s in range(8)] + [(2, s) for s in range(3)] + [(4, s) for s in range(2)]
+SYNTH_CASES = [(1, s) for s in range(8)] + [(2, s) for s in range(3)] + [(4, s) for s in range(2)] + [(3, 0), (3, 1)]
 
 
 def synth_case(atg, cfg, seed):

@@ -48,7 +48,7 @@ def test_synth_frames_match_reference_fixtures(atg, ctx, cfg, seed):
     assert g["dets"].tobytes() == fx[tag + "_dets"].tobytes()
 
 
-@pytest.mark.parametrize("cfg,seed", [(1, 0), (1, 5), (2, 0), (4, 1)])
+@pytest.mark.parametrize("cfg,seed", [(1, 0), (1, 5), (2, 0), (4, 1), (3, 0)])
 def test_stages_match_oracle(atg, ctx, oracle, cfg, seed):
     img, p, fams = synth_case(atg, cfg, seed)
     print("case", cfg, seed, img.shape, flush=True)
