@@ -71,6 +71,11 @@ class SynthTag(C.Structure):
     _fields_ = [("family", C.c_int32), ("id", C.c_int32), ("corners", C.c_double * 8)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("thr_tile_rows", C.c_int32), ("row_waves", C.c_int32), ("front_threads", C.c_int32),
+                ("fit_blocks", C.c_int32)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("budget_bytes", C.c_size_t), ("lanes", C.c_int32),
                 ("max_batch", C.c_int32), ("slots_per_stream", C.c_int32)]
@@ -132,6 +137,15 @@ def lib():
     L.atg_set_profiling.restype = C.c_int
     L.atg_debug_force_scratch.argtypes = [vp, C.c_int]
     L.atg_debug_force_scratch.restype = C.c_int
+    L.atg_get_tune.argtypes = [vp, C.POINTER(Tune)]
+    L.atg_get_tune.restype = C.c_int
+    L.atg_set_tune.argtypes = [vp, C.POINTER(Tune)]
+    L.atg_set_tune.restype = C.c_int
+    L.atg_set_graphs.argtypes = [vp, C.c_int]
+    L.atg_set_graphs.restype = C.c_int
+    L.atg_autotune.argtypes = [vp, C.POINTER(Params), C.POINTER(Frame), i32, C.c_int, C.POINTER(Tune),
+                               C.POINTER(C.c_double)]
+    L.atg_autotune.restype = C.c_int
     L.atg_stage_times.argtypes = [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), i32]
     L.atg_stage_times.restype = C.c_int
     L.atg_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
@@ -243,10 +257,8 @@ class Context:
                                           img.strides[0], 0, C.byref(r)), "atg_detect")
         return dict(dets=_dets(r.dets, r.ndets), nquads=r.nquads)
 
-    def detect_batch(self, imgs, params=None, on_device=False):
-        """imgs: list of uint8 arrays, or with on_device a list of
-        (device_ptr, height, width, stride)."""
-        p = params if params is not None else default_params()
+    @staticmethod
+    def _frames(imgs, on_device):
         n = len(imgs)
         fr = (Frame * n)()
         keep = []
@@ -258,6 +270,14 @@ class Context:
                 a = np.ascontiguousarray(im, np.uint8)
                 keep.append(a)
                 fr[i] = Frame(a.ctypes.data, a.shape[1], a.shape[0], a.strides[0])
+        return fr, keep
+
+    def detect_batch(self, imgs, params=None, on_device=False):
+        """imgs: list of uint8 arrays, or with on_device a list of
+        (device_ptr, height, width, stride)."""
+        p = params if params is not None else default_params()
+        n = len(imgs)
+        fr, keep = self._frames(imgs, on_device)
         res = (Result * n)()
         self._check(self.L.atg_detect_batch(self.h, C.byref(p), fr, n, 1 if on_device else 0, res),
                     "atg_detect_batch")
@@ -281,6 +301,28 @@ class Context:
 
     def set_profiling(self, on=True):
         self._check(self.L.atg_set_profiling(self.h, 1 if on else 0), "atg_set_profiling")
+
+    def get_tune(self):
+        t = Tune()
+        self._check(self.L.atg_get_tune(self.h, C.byref(t)), "atg_get_tune")
+        return (t.thr_tile_rows, t.row_waves, t.front_threads, t.fit_blocks)
+
+    def set_tune(self, thr_tile_rows, row_waves, front_threads, fit_blocks):
+        t = Tune(thr_tile_rows, row_waves, front_threads, fit_blocks)
+        self._check(self.L.atg_set_tune(self.h, C.byref(t)), "atg_set_tune")
+
+    def set_graphs(self, on=True):
+        self._check(self.L.atg_set_graphs(self.h, 1 if on else 0), "atg_set_graphs")
+
+    def autotune(self, imgs, params=None, on_device=False):
+        """Times the work splits on this batch; returns (tune tuple, ms)."""
+        p = params if params is not None else default_params()
+        fr, keep = self._frames(imgs, on_device)
+        t = Tune()
+        ms = C.c_double()
+        self._check(self.L.atg_autotune(self.h, C.byref(p), fr, len(imgs), 1 if on_device else 0, C.byref(t),
+                                        C.byref(ms)), "atg_autotune")
+        return (t.thr_tile_rows, t.row_waves, t.front_threads, t.fit_blocks), ms.value
 
     def force_scratch(self, on=True):
         """Keep the union-find replay state in HBM scratch (the wide-frame path)."""

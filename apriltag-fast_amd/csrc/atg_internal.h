@@ -116,6 +116,16 @@ struct Frame {
     uint32_t *counters;
 };
 
+// Work splits of the kernel chain (the autotuner's choices); any values
+// give the same results.
+struct Tune {
+    int32_t thr_tile_rows;  // tile rows per k_threshold block: 1, 2, 4
+    int32_t row_waves;      // rows (one wave each) per block of the per-row kernels: 1, 2, 4
+                            // (their launch bounds are 256 threads)
+    int32_t front_threads;  // k_front block: 128, 256, 512
+    int32_t fit_blocks;     // blocks per frame of the fit and decode kernels: 32..1024
+};
+
 // The replay kernels' memory plan for one call (LDS where it fits, else
 // HBM scratch).
 struct CclPlan {

@@ -67,7 +67,7 @@ __device__ __forceinline__ bool runs_join(int a0, int a1, int b0, int b1, bool w
 __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63;
-    int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int y = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (y < 1 || y >= F.qh) return;
     const int QW = F.qw;
     const int ncur = (int)F.rcnt[y], nprev = (int)F.rcnt[y - 1];
@@ -184,7 +184,6 @@ __device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, ui
 
 // Staged rows: the first kStage runs and links of a row come from
 // registers loaded one row ahead; the rest (rare) from HBM at use.
-constexpr int kFrontThreads = 256;
 constexpr int kStage = 1024;
 
 size_t front_bytes(int qw) { return (size_t)36 * qw + (size_t)8 * kStage + 128; }
@@ -202,8 +201,10 @@ __device__ __forceinline__ void bsync() {
     }
 }
 
-template <bool G>
+template <bool G, int kFrontThreads>
 __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) {
+    constexpr int kPer = kStage / kFrontThreads;  // prefetched runs and links per thread
+    constexpr int kWaves = kFrontThreads / 64;
     using U32 = typename Mem<G>::U32;
     using U16 = typename Mem<G>::U16;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -218,7 +219,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
     U32 *sedge = srun + kStage;    // [kStage] the row's links
     U16 *labA = reinterpret_cast<U16 *>(sedge + kStage);
     U16 *labB = labA + QW;
-    uint32_t *wsum = lds_small;    // [4] per-wave totals of a block scan
+    uint32_t *wsum = lds_small;    // [kWaves] per-wave totals of a block scan
     // row offsets: exclusive scan of the per-row run counts (wave 0)
     if (wave == 0) {
         uint32_t carry = 0;
@@ -250,7 +251,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
     uint32_t pbase = 0;  // row_off of the previous row
     int pcur = m;        // runs of the previous row
     // the next row, loaded one row ahead
-    uint32_t nx_rc = 0, nx_ec = 0, nx_r[4], nx_e[4];
+    uint32_t nx_rc = 0, nx_ec = 0, nx_r[kPer], nx_e[kPer];
     uint8_t nx_t1 = 0, nx_t2 = 0;
     auto prefetch = [&](int y) {
         const uint32_t *cur = F.runs + (int64_t)y * QW;
@@ -258,7 +259,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
         nx_rc = F.rcnt[y];
         nx_ec = F.ecnt[y];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < kPer; j++) {
             int i = tid + j * kFrontThreads;
             nx_r[j] = i < QW ? cur[i] : 0u;
             nx_e[j] = i < 3 * QW ? edges[i] : 0u;
@@ -274,7 +275,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
         const int ncur = (int)nx_rc, ne = (int)nx_ec;
         const uint8_t t1 = nx_t1, t2 = nx_t2;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < kPer; j++) {
             int i = tid + j * kFrontThreads;
             if (i < ncur) srun[i] = nx_r[j];
             if (i < ne) sedge[i] = nx_e[j];
@@ -339,7 +340,9 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
         bsync<G>();
         uint32_t id = inc - cnt;
         for (int w = 0; w < wave; w++) id += wsum[w];
-        const uint32_t nrec = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        uint32_t nrec = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) nrec += wsum[w];
         for (int x = x0; x < x1; x++) {
             if (flag[x] != 0) {
                 flag[x] = id + 1;
@@ -362,15 +365,15 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
     if (tid == 0) F.nrec[QH - 1] = (uint32_t)m;
 }
 
-__global__ __launch_bounds__(256) void k_front(const Frame *frames, int lds_bytes, uint8_t *gscratch,
-                                               int64_t gstride) {
+template <int T>
+__global__ __launch_bounds__(T) void k_front(const Frame *frames, int lds_bytes, uint8_t *gscratch, int64_t gstride) {
     extern __shared__ __align__(16) uint8_t lds[];
-    __shared__ uint32_t small[4];
+    __shared__ uint32_t small[T / 64];
     const Frame F = frames[blockIdx.y];
     if (lds_bytes > 0)
-        front_sweep<false>(F, lds, small);
+        front_sweep<false, T>(F, lds, small);
     else
-        front_sweep<true>(F, gscratch + (int64_t)blockIdx.y * gstride, small);
+        front_sweep<true, T>(F, gscratch + (int64_t)blockIdx.y * gstride, small);
 }
 
 // ---------------------------------------------------------------------
@@ -571,7 +574,7 @@ __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int lds_byte
 __global__ __launch_bounds__(256) void k_root_jump(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63;
-    const int y = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int y = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (y < 1 || y >= F.qh) return;
     const uint32_t cb = F.row_off[y];
     const int n = (int)F.nrec[y];
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(256) void k_root_jump(const Frame *frames) {
 __global__ __launch_bounds__(256) void k_lost(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63;
-    const int y = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int y = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (y < 1 || y > F.qh) return;
     const uint32_t pb = F.row_off[y - 1];
     const int m = (int)F.nrec[y - 1];
@@ -626,7 +629,7 @@ __global__ void k_resolve(const Frame *frames, const DevParams *P) {
 // t is 127), the row's virtual node at x = qw-1 (fast/clusters.c:123-142).
 __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
-    int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int y = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (y >= F.qh) return;
     const int lane = threadIdx.x & 63;
     const uint8_t *trow = F.t + (int64_t)y * F.qw;
@@ -665,7 +668,10 @@ static bool lds_ok(const void *fn, size_t need, bool force_scratch) {
 CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch) {
     CclPlan c{};
     size_t f = front_bytes(max_qw), r = replay_bytes(max_qw);
-    c.front = lds_ok(reinterpret_cast<const void *>(&k_front), f, force_scratch) ? (int)f : 0;
+    bool fok = lds_ok(reinterpret_cast<const void *>(&k_front<128>), f, force_scratch) &&
+               lds_ok(reinterpret_cast<const void *>(&k_front<256>), f, force_scratch) &&
+               lds_ok(reinterpret_cast<const void *>(&k_front<512>), f, force_scratch);
+    c.front = fok ? (int)f : 0;
     c.replay = lds_ok(reinterpret_cast<const void *>(&k_replay), r, force_scratch) ? (int)r : 0;
     c.replay_waves = c.replay ? std::max(1, max_qh - 1) : std::min(std::max(1, max_qh - 1), 64);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -677,28 +683,42 @@ CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch) {
 size_t ccl_scratch_bytes(const CclPlan &c) { return c.scratch_front + c.scratch_replay; }
 
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, const CclPlan &c, uint8_t *scratch, Marks *mk) {
-    hipLaunchKernelGGL(k_uf_edges, dim3((unsigned)((max_qh + 3) / 4), nframes), dim3(256), 0, st, dframes);
+                      int max_qh, const CclPlan &c, uint8_t *scratch, const Tune &tu, Marks *mk) {
+    // per-row kernels: tu.row_waves rows (one wave each) per block, at
+    // most 4: their launch bounds are 256 threads
+    const int rw = std::min(std::max(tu.row_waves, 1), 4);
+    const dim3 rb(64 * rw), rg((unsigned)((max_qh + rw) / rw), nframes);
+    hipLaunchKernelGGL(k_uf_edges, rg, rb, 0, st, dframes);
     mk->mark(st, "uf_edges");
     // scratch per frame: [front][replay waves], frames back to back
     int64_t per = (int64_t)ccl_scratch_bytes(c);
     uint8_t *s_front = scratch, *s_replay = scratch + c.scratch_front;
     int64_t rstride = c.replay ? 0 : (int64_t)(c.scratch_replay / (size_t)c.replay_waves);
-    hipLaunchKernelGGL(k_front, dim3(1, nframes), dim3(kFrontThreads), (size_t)c.front, st, dframes, c.front, s_front,
-                       per);
+    switch (tu.front_threads) {
+    case 128:
+        hipLaunchKernelGGL(k_front<128>, dim3(1, nframes), dim3(128), (size_t)c.front, st, dframes, c.front, s_front,
+                           per);
+        break;
+    case 512:
+        hipLaunchKernelGGL(k_front<512>, dim3(1, nframes), dim3(512), (size_t)c.front, st, dframes, c.front, s_front,
+                           per);
+        break;
+    default:
+        hipLaunchKernelGGL(k_front<256>, dim3(1, nframes), dim3(256), (size_t)c.front, st, dframes, c.front, s_front,
+                           per);
+    }
     mk->mark(st, "uf_front");
     hipLaunchKernelGGL(k_replay, dim3((unsigned)c.replay_waves, nframes), dim3(64), (size_t)c.replay, st, dframes,
                        c.replay, s_replay, per, rstride);
     mk->mark(st, "uf_replay");
-    const dim3 rows4((unsigned)((max_qh + 4) / 4), nframes);
-    for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rows4, dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_lost, rows4, dim3(256), 0, st, dframes);
+    for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);
+    hipLaunchKernelGGL(k_lost, rg, rb, 0, st, dframes);
     mk->mark(st, "uf_stitch");
     int64_t maxnodes = (int64_t)max_qw * max_qh + max_qh;
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,
                        dparams);
     mk->mark(st, "resolve");
-    hipLaunchKernelGGL(k_labels, dim3((unsigned)((max_qh + 3) / 4), nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_labels, rg, rb, 0, st, dframes);
     mk->mark(st, "labels");
 }
 

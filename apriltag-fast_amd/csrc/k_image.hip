@@ -175,50 +175,55 @@ __device__ __forceinline__ void row_runs(const Frame &F, int y, TF tfun) {
     if (lane == 0) F.rcnt[y] = (uint32_t)count;
 }
 
-// One block (4 waves) per tile row: the 3x3 max/min of the tile grid
-// around each tile of the row (clipped at the grid edge) into LDS, then
-// each wave thresholds one pixel row (fast/threshold.c:71-150) and, with
-// do_rle, extracts its runs. Block th takes the rows below the last whole
-// tile row (tile row th-1, never flat).
-__global__ __launch_bounds__(256) void k_threshold(const Frame *frames, const DevParams *P, int do_rle) {
+// One block (4 waves) per `trows` tile rows: the 3x3 max/min of the tile
+// grid around each tile of those rows (clipped at the grid edge) into LDS,
+// then each wave thresholds pixel rows (fast/threshold.c:71-150) and, with
+// do_rle, extracts their runs. The rows below the last whole tile row use
+// tile row th-1 and are never flat (fast/threshold.c:144-148).
+__global__ __launch_bounds__(256) void k_threshold(const Frame *frames, const DevParams *P, int do_rle, int trows) {
     extern __shared__ uint8_t lds[];
     const Frame F = frames[blockIdx.y];
-    int ty = blockIdx.x;
-    if (ty > F.th) return;
-    bool tail = ty == F.th;
-    if (tail && 4 * F.th >= F.qh) return;
-    int tyd = tail ? F.th - 1 : ty;
-    uint8_t *hx = lds, *hn = lds + F.tw;
-    int up = max(tyd - 1, 0), dn = min(tyd + 1, F.th - 1);
-    for (int tx = threadIdx.x; tx < F.tw; tx += blockDim.x) {
-        int l = max(tx - 1, 0), r = min(tx + 1, F.tw - 1);
-        uint8_t hi = 0, lo = 255;
-        for (int yy = up; yy <= dn; yy++)
-            for (int xx = l; xx <= r; xx++) {
-                hi = max(hi, F.tmx[yy * F.tw + xx]);
-                lo = min(lo, F.tmn[yy * F.tw + xx]);
-            }
-        hx[tx] = hi;
-        hn[tx] = lo;
+    const int t0 = (int)blockIdx.x * trows;
+    if (t0 > F.th) return;
+    const int t1 = min(t0 + trows, F.th + 1);  // tile rows [t0, t1), th = the tail
+    const int tw = F.tw;
+    for (int j = 0; j < t1 - t0; j++) {
+        const int tyd = min(t0 + j, F.th - 1);
+        uint8_t *hx = lds + 2 * tw * j, *hn = hx + tw;
+        const int up = max(tyd - 1, 0), dn = min(tyd + 1, F.th - 1);
+        for (int tx = threadIdx.x; tx < tw; tx += blockDim.x) {
+            int l = max(tx - 1, 0), r = min(tx + 1, tw - 1);
+            uint8_t hi = 0, lo = 255;
+            for (int yy = up; yy <= dn; yy++)
+                for (int xx = l; xx <= r; xx++) {
+                    hi = max(hi, F.tmx[yy * tw + xx]);
+                    lo = min(lo, F.tmn[yy * tw + xx]);
+                }
+            hx[tx] = hi;
+            hn[tx] = lo;
+        }
     }
     __syncthreads();
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int y = tail ? 4 * F.th + wave : 4 * ty + wave;
-    if (y >= F.qh || (!tail && wave >= 4)) return;
-    const uint8_t *row = F.qim + (int64_t)y * F.qw;
-    uint8_t *trow = F.t + (int64_t)y * F.qw;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int y1 = min(4 * t1, F.qh);
     const int diff = P->mwbd;
-    const int tw = F.tw;
-    auto tfun = [&](int x) -> uint32_t {
-        bool xin = x < 4 * tw;
-        int tx = xin ? x >> 2 : tw - 1;
-        int hi = hx[tx], lo = hn[tx];
-        int thr = lo + (hi - lo) / 2;
-        bool flat = !tail && xin && hi - lo < diff;
-        return flat ? 127u : (row[x] > thr ? 255u : 0u);
-    };
-    for (int x = lane; x < F.qw; x += 64) trow[x] = (uint8_t)tfun(x);
-    if (do_rle) row_runs(F, y, tfun);
+    for (int y = 4 * t0 + wave; y < y1; y += 4) {
+        const int ty = y >> 2;
+        const bool tail = ty == F.th;
+        const uint8_t *hx = lds + 2 * tw * (ty - t0), *hn = hx + tw;
+        const uint8_t *row = F.qim + (int64_t)y * F.qw;
+        uint8_t *trow = F.t + (int64_t)y * F.qw;
+        auto tfun = [&](int x) -> uint32_t {
+            bool xin = x < 4 * tw;
+            int tx = xin ? x >> 2 : tw - 1;
+            int hi = hx[tx], lo = hn[tx];
+            int thr = lo + (hi - lo) / 2;
+            bool flat = !tail && xin && hi - lo < diff;
+            return flat ? 127u : (row[x] > thr ? 255u : 0u);
+        };
+        for (int x = lane; x < F.qw; x += 64) trow[x] = (uint8_t)tfun(x);
+        if (do_rle) row_runs(F, y, tfun);
+    }
 }
 
 // Runs of the thresholded image (after deglitch), one wave per row.
@@ -261,7 +266,7 @@ __global__ void k_deglitch_min(const Frame *frames) {
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                        int max_qh, bool blur, bool deglitch, Marks *mk) {
+                        int max_qh, bool blur, bool deglitch, const Tune &tu, Marks *mk) {
     int64_t maxpx = (int64_t)max_qw * max_qh;
     int64_t slots = (int64_t)((max_qw + 3) / 4) * ((max_qh + 3) / 4);
     dim3 g1(cdiv(slots, 256), nframes);
@@ -275,10 +280,10 @@ void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_tiles, gt, dim3(256), 0, st, dframes);
         mk->mark(st, "blur");
     }
-    int max_th = max_qh / 4;
-    size_t lds = (size_t)2 * (max_qw / 4) + 16;
-    dim3 g2(max_th + 1, nframes);
-    hipLaunchKernelGGL(k_threshold, g2, dim3(256), lds, st, dframes, dparams, deglitch ? 0 : 1);
+    int max_th = max_qh / 4, trows = tu.thr_tile_rows;
+    size_t lds = (size_t)2 * (max_qw / 4) * trows + 16;
+    dim3 g2(cdiv(max_th + 1, trows), nframes);
+    hipLaunchKernelGGL(k_threshold, g2, dim3(256), lds, st, dframes, dparams, deglitch ? 0 : 1, trows);
     if (deglitch) {
         dim3 gp(cdiv(maxpx, 256), nframes);
         hipLaunchKernelGGL(k_deglitch_max, gp, dim3(256), 0, st, dframes);

@@ -71,7 +71,7 @@ __device__ __forceinline__ uint32_t point_mask(const uint8_t *a, const uint8_t *
 __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pass) {
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63;
-    int y = 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    int y = 1 + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (y > F.qh - 2) return;
     const int QW = F.qw;
     const uint8_t *a = F.t + (int64_t)y * QW, *b = a + QW;
@@ -295,11 +295,13 @@ void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int wh
     }
 }
 
-void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, Marks *mk) {
-    dim3 g((unsigned)((max_qh + 3) / 4), nframes);
-    hipLaunchKernelGGL(k_point_rows, g, dim3(256), 0, st, dframes, 0);
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
+                         Marks *mk) {
+    const int rw = tu.row_waves < 1 ? 1 : tu.row_waves > 4 ? 4 : tu.row_waves;  // launch bounds: 256 threads
+    dim3 g((unsigned)((max_qh + rw - 1) / rw), nframes), b(64 * rw);
+    hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 0);
     hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_point_rows, g, dim3(256), 0, st, dframes, 1);
+    hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);
     mk->mark(st, "points");
     int parity = 0;
     launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity);

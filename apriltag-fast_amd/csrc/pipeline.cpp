@@ -23,10 +23,11 @@ void launch_init(hipStream_t st, const Frame *dframes, int nframes);
 CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch);
 size_t ccl_scratch_bytes(const CclPlan &c);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                        int max_qh, bool blur, bool deglitch, Marks *mk);
+                        int max_qh, bool blur, bool deglitch, const Tune &tu, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, const CclPlan &c, uint8_t *scratch, Marks *mk);
-void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, Marks *mk);
+                      int max_qh, const CclPlan &c, uint8_t *scratch, const Tune &tu, Marks *mk);
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
+                         Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
                         int fit_blocks, Marks *mk);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
@@ -41,7 +42,7 @@ constexpr int kResultDets = kMaxDetsPerFrame;  // detection capacity of a frame'
 constexpr int kInlineDets = 32;            // read back with the counters
 constexpr size_t kCountersBytes = 64;
 constexpr size_t kResultBlock = kCountersBytes + (size_t)kResultDets * sizeof(at_detection);
-constexpr int kFitBlocks = 256;
+constexpr Tune kDefaultTune = {1, 4, 256, 256};
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -116,6 +117,10 @@ struct atg_ctx {
     bool arena_fixed = false;         // the arena never grows (a serving lane's budget)
     hipEvent_t ev2 = nullptr;         // the end of the last enqueued chain (with its readback)
     int32_t inflight = 0;             // frames of the enqueued chain
+    Tune tune = kDefaultTune;         // work splits of the chain
+    bool graphs = true;               // replay the chain as a HIP graph when the call shape repeats
+    hipGraphExec_t gexec = nullptr;   // the graph of the last call shape
+    std::vector<int64_t> gkey;        // that shape
     std::vector<int> fstatus;         // per frame of the last call: ATG_OK, 1 (point capacity), ATG_ERR_LIMIT
     std::vector<double> stage_ms;
     std::vector<const char *> stage_names;
@@ -385,6 +390,7 @@ extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->ev2) hipEventDestroy(ctx->ev2);
+    if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -489,7 +495,14 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         HIPCHK(hipHostMalloc(&ctx->pinned_out, out_need));
         ctx->pinned_out_size = out_need;
     }
-    // frame descriptors, and the host frames into the arena
+    // frame descriptors, and the host frames into pinned staging; the
+    // stream work (copies in, kernels, results out) is issued below
+    struct Copy {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    std::vector<Copy> copies;
     size_t pin_at = 0;
     uint8_t *gscratch = ctx->arena + (total - gstride * (size_t)n);
     for (int32_t i = 0; i < n; i++) {
@@ -520,33 +533,80 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         } else {
             uint8_t *dst = ctx->pinned + pin_at;
             for (int64_t y = 0; y < fp.H; y++) memcpy(dst + y * fp.W, frames[i].pixels + y * fp.stride, (size_t)fp.W);
-            HIPCHK(hipMemcpyAsync((void *)F.img, dst, (size_t)(fp.W * fp.H), hipMemcpyHostToDevice, ctx->stream));
+            copies.push_back(Copy{(void *)F.img, dst, (size_t)(fp.W * fp.H)});
             pin_at += align256((size_t)(fp.W * fp.H));
             F.stride = fp.W;
         }
     }
     Frame *fstage = reinterpret_cast<Frame *>(ctx->pinned_out);
     memcpy(fstage, ctx->hframes.data(), sizeof(Frame) * (size_t)n);
-    HIPCHK(hipMemcpyAsync(ctx->dframes, fstage, sizeof(Frame) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
-    // the chain
+    copies.push_back(Copy{ctx->dframes, fstage, sizeof(Frame) * (size_t)n});
+    const size_t pre = kCountersBytes + kInlineDets * sizeof(at_detection);
+    uint8_t *hout = ctx->pinned_out + sizeof(Frame) * (size_t)n;
     hipStream_t st = ctx->stream;
-    HIPCHK(hipEventRecord(ctx->ev0, st));
     Marks *mk = &ctx->marks;
     mk->n = 0;
     mk->on = ctx->profiling;
-    launch_init(st, ctx->dframes, n);
-    mk->mark(st, "init");
-    launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ctx->hp.blur_ksz > 1, p->deglitch, mk);
-    launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, gscratch, mk);
-    launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, mk);
-    launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, kFitBlocks, mk);
-    launch_decode_stage(st, ctx->dframes, ctx->dparams, n, kFitBlocks, mk);
-    HIPCHK(hipGetLastError());
+    const Tune tu = ctx->tune;
+    const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
+    auto issue = [&]() -> hipError_t {
+        for (const Copy &c : copies) {
+            hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return e;
+        }
+        launch_init(st, ctx->dframes, n);
+        mk->mark(st, "init");
+        launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, blur, deglitch, tu, mk);
+        launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, gscratch, tu, mk);
+        launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
+        launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, mk);
+        launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, mk);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        // results: counters and the first detections of every frame in one copy
+        return hipMemcpy2DAsync(hout, pre, ctx->results, kResultBlock, pre, (size_t)n, hipMemcpyDeviceToHost, st);
+    };
+    HIPCHK(hipEventRecord(ctx->ev0, st));
+    if (ctx->graphs && !ctx->profiling) {
+        // one graph per call shape: everything the stream work depends on
+        std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
+                                    blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
+                                    ccl.replay_waves,        (int64_t)ccl.scratch_front, (int64_t)ccl.scratch_replay,
+                                    tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks,
+                                    (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
+                                    (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
+                                    (int64_t)(intptr_t)ctx->dparams, (int64_t)(intptr_t)gscratch};
+        for (const Copy &c : copies) {
+            key.push_back((int64_t)(intptr_t)c.dst);
+            key.push_back((int64_t)(intptr_t)c.src);
+            key.push_back((int64_t)c.bytes);
+        }
+        if (!ctx->gexec || key != ctx->gkey) {
+            if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
+            ctx->gexec = nullptr;
+            ctx->gkey.clear();
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipError_t e = issue();
+            hipError_t e2 = hipStreamEndCapture(st, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                return fail(ctx, ATG_ERR_HIP, std::string("graph capture: ") +
+                                                  hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ctx->gexec = nullptr;
+                return fail(ctx, ATG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            }
+            ctx->gkey = key;
+        }
+        HIPCHK(hipGraphLaunch(ctx->gexec, st));
+    } else {
+        HIPCHK(issue());
+    }
     HIPCHK(hipEventRecord(ctx->ev1, st));
-    // results: counters and the first detections of every frame in one copy
-    size_t pre = kCountersBytes + kInlineDets * sizeof(at_detection);
-    uint8_t *hout = ctx->pinned_out + sizeof(Frame) * (size_t)n;
-    HIPCHK(hipMemcpy2DAsync(hout, pre, ctx->results, kResultBlock, pre, (size_t)n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev2, st));
     ctx->inflight = n;
     return ATG_OK;
@@ -730,6 +790,104 @@ extern "C" int atg_set_profiling(atg_ctx *ctx, int on) {
         for (int k = 0; k < kMaxMarks; k++) HIPCHK(hipEventCreate(&ctx->marks.ev[k]));
     }
     ctx->profiling = on != 0;
+    return ATG_OK;
+}
+
+static bool valid_tune(const Tune &t) {
+    auto in = [](int v, std::initializer_list<int> l) { return std::find(l.begin(), l.end(), v) != l.end(); };
+    // row_waves: the per-row kernels are compiled for blocks of at most 256
+    // threads (4 waves)
+    return in(t.thr_tile_rows, {1, 2, 4}) && in(t.row_waves, {1, 2, 4}) && in(t.front_threads, {128, 256, 512}) &&
+           t.fit_blocks >= 32 && t.fit_blocks <= 1024;
+}
+
+extern "C" int atg_get_tune(const atg_ctx *ctx, atg_tune *out) {
+    if (!ctx || !out) return ATG_ERR_ARG;
+    static_assert(sizeof(atg_tune) == sizeof(Tune), "atg_tune mirrors Tune");
+    memcpy(out, &ctx->tune, sizeof(Tune));
+    return ATG_OK;
+}
+
+extern "C" int atg_set_tune(atg_ctx *ctx, const atg_tune *t) {
+    if (!ctx || !t) return ATG_ERR_ARG;
+    Tune v;
+    memcpy(&v, t, sizeof v);
+    if (!valid_tune(v)) return fail(ctx, ATG_ERR_ARG, "work split outside the supported values");
+    ctx->tune = v;
+    return ATG_OK;
+}
+
+extern "C" int atg_set_graphs(atg_ctx *ctx, int on) {
+    if (!ctx) return ATG_ERR_ARG;
+    ctx->graphs = on != 0;
+    return ATG_OK;
+}
+
+// Coordinate descent over the work splits on the caller's batch: each
+// value of one split with the others at their best so far, the device
+// time of the chain (median of three runs) deciding; every candidate's
+// detections must equal the first run's.
+extern "C" int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device,
+                            atg_tune *best_out, double *best_ms) {
+    if (!ctx || !p || !frames || n < 1) return ATG_ERR_ARG;
+    const Tune start = ctx->tune;
+    const bool graphs = ctx->graphs, prof = ctx->profiling;
+    ctx->graphs = false;
+    ctx->profiling = false;
+    std::vector<std::vector<at_detection>> want;
+    std::vector<int64_t> want_q;
+    auto measure = [&](const Tune &t, double *ms) -> int {
+        ctx->tune = t;
+        ctx->pcap_scale = 1;
+        std::vector<double> v;
+        for (int rep = 0; rep < 4; rep++) {
+            int rc = run_chain(ctx, p, frames, n, on_device);
+            if (rc == 1) {
+                ctx->pcap_scale = 4;
+                rc = run_chain(ctx, p, frames, n, on_device);
+            }
+            if (rc) return rc < 0 ? rc : ATG_ERR_LIMIT;
+            if (rep > 0) v.push_back(ctx->chain_ms);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        if (want.empty()) {
+            want = ctx->dets;
+            want_q = ctx->nquads;
+            return ATG_OK;
+        }
+        for (int32_t i = 0; i < n; i++)
+            if (ctx->nquads[(size_t)i] != want_q[(size_t)i] || ctx->dets[(size_t)i].size() != want[(size_t)i].size() ||
+                memcmp(ctx->dets[(size_t)i].data(), want[(size_t)i].data(),
+                       want[(size_t)i].size() * sizeof(at_detection)) != 0)
+                return fail(ctx, ATG_ERR_LIMIT, "a work split changed the detections");
+        return ATG_OK;
+    };
+    Tune best = start;
+    double bms = 0.0;
+    int rc = measure(best, &bms);
+    const std::vector<int> cand[4] = {{1, 2, 4}, {1, 2, 4}, {128, 256, 512}, {64, 128, 256, 512, 1024}};
+    for (int k = 0; k < 4 && rc == ATG_OK; k++) {
+        for (int v : cand[k]) {
+            Tune t = best;
+            int32_t *f[4] = {&t.thr_tile_rows, &t.row_waves, &t.front_threads, &t.fit_blocks};
+            if (*f[k] == v) continue;
+            *f[k] = v;
+            double ms = 0.0;
+            rc = measure(t, &ms);
+            if (rc) break;
+            if (ms < bms) {
+                bms = ms;
+                best = t;
+            }
+        }
+    }
+    ctx->graphs = graphs;
+    ctx->profiling = prof;
+    ctx->tune = rc == ATG_OK ? best : start;
+    if (rc) return rc;
+    if (best_out) memcpy(best_out, &best, sizeof best);
+    if (best_ms) *best_ms = bms;
     return ATG_OK;
 }
 

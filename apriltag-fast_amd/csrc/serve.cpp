@@ -259,6 +259,7 @@ int start(atg_server *s) {
         int rc = atg_ctx_create(s->cfg.device, &L.ctx);
         if (rc) return sfail(s, rc, "lane context");
         s->lanes.push_back(L);
+        atg_set_graphs(L.ctx, 0);  // batches differ from one launch to the next
         rc = atg_set_families(L.ctx, s->fams.data(), s->bits.data(), (int32_t)s->fams.size());
         if (rc) return sfail(s, rc, std::string("lane families: ") + atg_last_error(L.ctx));
     }

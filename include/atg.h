@@ -120,6 +120,25 @@ void atg_stages_free(atg_stages *s);
  * diagnostics). */
 int atg_debug_counters(atg_ctx *ctx, int32_t frame, uint32_t *out16);
 
+/* Work splits of the kernel chain (any values give the same detections):
+ * threshold tile rows per block (1, 2, 4), rows per block of the per-row
+ * kernels (1, 2, 4), threads of the union-find front sweep (128, 256,
+ * 512), blocks per frame of the fit and decode kernels (32..1024). */
+typedef struct {
+    int32_t thr_tile_rows, row_waves, front_threads, fit_blocks;
+} atg_tune;
+int atg_get_tune(const atg_ctx *ctx, atg_tune *out);
+int atg_set_tune(atg_ctx *ctx, const atg_tune *t);
+/* Chooses the work splits for this batch by timing candidates on it
+ * (coordinate descent, device time), checks that each gives the same
+ * detections, and keeps the fastest in the context. */
+int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device,
+                 atg_tune *best, double *best_ms);
+/* on != 0 (the default): calls whose shape (frame sizes, parameters, work
+ * splits, buffers) repeats replay the captured HIP graph of the chain
+ * instead of launching each kernel. */
+int atg_set_graphs(atg_ctx *ctx, int on);
+
 /* on != 0: the following calls keep the union-find replay's state in HBM
  * scratch, the path frames too wide for LDS take, so that tests can check
  * it on any frame. */
