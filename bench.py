@@ -173,6 +173,44 @@ def serving_sweep(atg, ctx, loads, seconds, nstreams=32, nframes=4, check_ref=Tr
                 alone_bit_exact_vs_cpu_reference=ref_ok)
 
 
+def batching_config4(atg, ctx, steps):
+    """Uneven batch: 64 frames, 16 each of 640x480, 1280x720, 1920x1080
+    and 2448x2048 (config-1 scenes with 5-30 tags), resident in HBM, one
+    call of the chain; frames/s with the default work splits and with the
+    autotuner's, and that both equal each frame's single-frame result."""
+    from atg_testlib import same_dets
+    sizes = [(640, 480), (1280, 720), (1920, 1080), (2448, 2048)]
+    frames = [atg.synth_frame(4, 100 + i, width=sizes[i % 4][0], height=sizes[i % 4][1])[0] for i in range(64)]
+    ctx.set_families([("tag36h11", 2)])
+    prm = atg.default_params()
+    dev = [ctx.to_device(f) for f in frames]
+    try:
+        alone = [ctx.detect(None, prm, on_device_ptr=d) for d in dev]
+
+        def timed():
+            ctx.detect_batch(dev, prm, on_device=True)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                out = ctx.detect_batch(dev, prm, on_device=True)
+            return 64 * steps / (time.perf_counter() - t0), out
+
+        default_tune = ctx.get_tune()
+        fps0, out0 = timed()
+        tune, _ = ctx.autotune(dev, prm, on_device=True)
+        fps1, out1 = timed()
+        exact = all(same_dets(a["dets"], b["dets"]) and a["nquads"] == b["nquads"] and
+                    same_dets(a["dets"], c["dets"]) for a, b, c in zip(alone, out0, out1))
+        ctx.set_tune(*default_tune)
+    finally:
+        for d in dev:
+            ctx.free_device(d[0])
+    return dict(frames="64: 16 each of 640x480, 1280x720, 1920x1080, 2448x2048", fps_default=round(fps0, 1),
+                fps_autotuned=round(fps1, 1), autotuned=dict(zip(("thr_tile_rows", "row_waves", "front_threads",
+                                                                    "fit_blocks"), tune)),
+                default=dict(zip(("thr_tile_rows", "row_waves", "front_threads", "fit_blocks"), default_tune)),
+                bit_exact_vs_single=bool(exact), detections=int(sum(len(a["dets"]) for a in alone)))
+
+
 def cpu_baseline(frames, sample_s):
     """The reference detector (oracle/_ref, built from the reference's own
     sources) timed on this host at 1 thread and at all cores, on a bounded
@@ -304,6 +342,9 @@ def main():
     b_ac = sum(kb[k] for k in ac if k in kb)
     per_kernel = {k: {"ms": round(v, 4), "GB/s": round(kb.get(k, 0) / (v * 1e-3) / 1e9, 2)} for k, v in stages}
 
+    log("config 4: uneven batch")
+    batching = batching_config4(atg, ctx, max(2, args.steps // 4))
+    ctx.set_families(fams)
     serving = None
     if not args.no_serve:
         serving = serving_sweep(atg, ctx, [int(x) for x in args.serve_loads.split(",")], args.serve_seconds,
@@ -336,6 +377,7 @@ def main():
         "kernel_gbs": per_kernel,
         "stages_a_c": {"bytes": int(b_ac), "ms": round(t_ac, 4), "GB/s": round(b_ac / (t_ac * 1e-3) / 1e9, 2)},
         "roofline": roof,
+        "batching_uneven": batching,
         "serving": serving,
         "cpu_baseline": cpu,
     }
