@@ -135,6 +135,8 @@ def lib():
     L.atg_last_timing.restype = C.c_int
     L.atg_set_profiling.argtypes = [vp, C.c_int]
     L.atg_set_profiling.restype = C.c_int
+    L.atg_debug_copy.argtypes = [vp, vp, vp, i64, C.POINTER(C.c_double)]
+    L.atg_debug_copy.restype = C.c_int
     L.atg_debug_force_scratch.argtypes = [vp, C.c_int]
     L.atg_debug_force_scratch.restype = C.c_int
     L.atg_get_tune.argtypes = [vp, C.POINTER(Tune)]
@@ -323,6 +325,23 @@ class Context:
         self._check(self.L.atg_autotune(self.h, C.byref(p), fr, len(imgs), 1 if on_device else 0, C.byref(t),
                                         C.byref(ms)), "atg_autotune")
         return (t.thr_tile_rows, t.row_waves, t.front_threads, t.fit_blocks), ms.value
+
+    def copy_bandwidth(self, nbytes=1 << 29, reps=3):
+        """A plain copy kernel over nbytes, reps times: (GB/s moved, ms each)."""
+        a, b = C.c_void_p(), C.c_void_p()
+        self._check(self.L.atg_device_alloc(self.h, nbytes, C.byref(a)), "atg_device_alloc")
+        self._check(self.L.atg_device_alloc(self.h, nbytes, C.byref(b)), "atg_device_alloc")
+        try:
+            ms = C.c_double()
+            times = []
+            for _ in range(reps):
+                self._check(self.L.atg_debug_copy(self.h, b, a, nbytes, C.byref(ms)), "atg_debug_copy")
+                times.append(ms.value)
+        finally:
+            self.L.atg_device_free(self.h, a)
+            self.L.atg_device_free(self.h, b)
+        best = min(times)
+        return 2 * nbytes / (best * 1e-3) / 1e9, times
 
     def force_scratch(self, on=True):
         """Keep the union-find replay state in HBM scratch (the wide-frame path)."""

@@ -75,6 +75,8 @@ enum {
     kNRuns = 0, kNPts, kNClusters, kNQuads, kNDets, kError,
     kNEdges,    // union-find links (for the byte counts of the measurements)
     kNKeptPts,  // points of the clusters that go on to the quad fit
+    kNLarge = 8,  // kept clusters above kLargeCluster points
+    kNSmall = 9,  // the other kept clusters
     kNCounters = 16
 };
 
@@ -102,6 +104,8 @@ struct Frame {
     uint32_t *sval, *sval2;
     uint32_t *rhist;      // [256][kSortBlocks]
     uint32_t *cstart, *ccount, *cflag;
+    uint32_t *klist;      // [2][kcap] kept clusters: large ones, then (at kcap) the others
+    uint32_t kcap;
     uint64_t *cid;        // [pcap] cluster ids (max_root << 32 | min_root), in order
     uint64_t *okey;       // unsorted angle keys, by cluster position
     double *mom;          // [6][pcap] cumulative moments in angle order
@@ -116,6 +120,20 @@ struct Frame {
     uint32_t *counters;
 };
 
+// The work of the persistent kernels, across the frames of a call: list 0
+// the kept clusters above kLargeCluster points (taken first, so that the
+// longest items start early), list 1 the other kept clusters, list 2 the
+// quads. Each frame lists its own items (klist); off[l][f] is where frame
+// f's items start in list l's global index, total[l] its size. Blocks take
+// global indices in strides, without atomics.
+constexpr uint32_t kLargeCluster = 1024;
+struct WorkQueue {
+    uint32_t total[4];
+    uint32_t nframes, pad[3];
+    uint32_t *off[3];   // [nframes + 1] each
+    uint64_t pad2;
+};
+
 // Work splits of the kernel chain (the autotuner's choices); any values
 // give the same results.
 struct Tune {
@@ -123,7 +141,7 @@ struct Tune {
     int32_t row_waves;      // rows (one wave each) per block of the per-row kernels: 1, 2, 4
                             // (their launch bounds are 256 threads)
     int32_t front_threads;  // k_front block: 128, 256, 512
-    int32_t fit_blocks;     // blocks per frame of the fit and decode kernels: 32..1024
+    int32_t fit_blocks;     // blocks of the persistent fit and decode kernels: 32..4096
 };
 
 // The replay kernels' memory plan for one call (LDS where it fits, else

@@ -245,7 +245,9 @@ __device__ __forceinline__ uint64_t rotate90(uint64_t w0, int64_t nbits) {
 
 constexpr int kMaxSamples = 512;  // bit-sampler points: 8 * wab + nbits
 
-__global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevParams *P) {
+// Persistent: the quads of every frame, block b taking global indices b,
+// b + G, ... (a frame cursor).
+__global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
     __shared__ double sx[256], sy[256];
     __shared__ int sv[256];
     __shared__ double smp[3][kMaxSamples];  // tag x, tag y, value
@@ -256,13 +258,17 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
     __shared__ int s_hok;
     __shared__ Gray g_white, g_black;
     __shared__ double vals[12 * 12], sharp[12 * 12];
-    const Frame F = frames[blockIdx.y];
-    const uint32_t nq = F.counters[kNQuads];
     const int t = threadIdx.x;
     const int nfam = P->nfam;
-    View v = {F.img, F.stride, F.W, F.H};
-    if (F.blurred && !(P->decimate > 1.0f)) v = View{F.qim, F.W, F.W, F.H};
-    for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    const uint32_t total = Q->total[2];
+    const uint32_t *off = Q->off[2];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const uint32_t qi = g - off[fi];
+        View v = {F.img, F.stride, F.W, F.H};
+        if (F.blurred && !(P->decimate > 1.0f)) v = View{F.qim, F.W, F.W, F.H};
         QuadRec q = F.quads[qi];
         if (t == 0) {
             for (int i = 0; i < 4; i++) {
@@ -632,19 +638,20 @@ __global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevPar
     F.counters[kNDets] = (uint32_t)n;
 }
 
-__global__ void k_init(const Frame *frames) {
+__global__ void k_init(const Frame *frames, WorkQueue *Q) {
     const Frame F = frames[blockIdx.y];
     if (threadIdx.x < kNCounters) F.counters[threadIdx.x] = 0;
     if (F.skip) const_cast<uint8_t *>(F.img)[threadIdx.x] = 0;  // 64 bytes: the 8x8 zero frame
+    if (blockIdx.y == 0 && threadIdx.x < 4) Q->total[threadIdx.x] = 0;
 }
 
-void launch_init(hipStream_t st, const Frame *dframes, int nframes) {
-    hipLaunchKernelGGL(k_init, dim3(1, nframes), dim3(64), 0, st, dframes);
+void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q) {
+    hipLaunchKernelGGL(k_init, dim3(1, nframes), dim3(64), 0, st, dframes, Q);
 }
 
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
-                         Marks *mk) {
-    hipLaunchKernelGGL(k_decode, dim3(blocks, nframes), dim3(256), 0, st, dframes, dparams);
+                         WorkQueue *Q, Marks *mk) {
+    hipLaunchKernelGGL(k_decode, dim3(blocks), dim3(256), 0, st, dframes, dparams, Q);
     mk->mark(st, "decode");
     hipLaunchKernelGGL(k_finish, dim3(1, nframes), dim3(64), 0, st, dframes, dparams);
     mk->mark(st, "finish");

@@ -261,6 +261,19 @@ __global__ void k_deglitch_min(const Frame *frames) {
     F.t[i] = m;
 }
 
+// A plain copy (16 bytes a thread, grid-stride): the measurements' known-size
+// reference for HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy(uint4 *dst, const uint4 *src, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy(hipStream_t st, void *dst, const void *src, int64_t bytes) {
+    int64_t n16 = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, static_cast<uint4 *>(dst), static_cast<const uint4 *>(src),
+                       n16);
+}
+
 // ---------------------------------------------------------------- launchers
 
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }

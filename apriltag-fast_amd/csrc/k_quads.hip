@@ -278,20 +278,72 @@ __device__ void order_run(const uint64_t *orig, int64_t n, uint64_t *r, int64_t 
     for (int64_t q = 0; q < m; q++) r[q] = ent[q];
 }
 
+// Per frame (a block each): its kept clusters in cluster order, those
+// above kLargeCluster points first (block scans over the flags), and
+// their counts.
+__global__ __launch_bounds__(256) void k_keep_lists(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    const uint32_t ncl = F.counters[kNClusters];
+    const int t = threadIdx.x;
+    uint32_t nl = 0, ns = 0;
+    for (uint32_t c0 = 0; c0 < ncl; c0 += 256) {
+        const uint32_t c = c0 + t;
+        const bool kept = c < ncl && (F.cflag[c] & 1u);
+        const bool large = kept && F.cstart[c + 1] - F.cstart[c] > kLargeCluster;
+        uint32_t tl, ts;
+        const uint32_t pl = bscan(large ? 1u : 0u, &tl);
+        const uint32_t ps = bscan(kept && !large ? 1u : 0u, &ts);
+        if (large && nl + pl < F.kcap) F.klist[nl + pl] = c;
+        if (kept && !large && ns + ps < F.kcap) F.klist[F.kcap + ns + ps] = c;
+        nl += tl;
+        ns += ts;
+    }
+    if (t == 0) {
+        if (nl > F.kcap || ns > F.kcap) atomicOr(F.counters + kError, 64u);
+        F.counters[kNLarge] = min(nl, F.kcap);
+        F.counters[kNSmall] = min(ns, F.kcap);
+    }
+}
+
+// One block: where each frame's items start in list l (counter `which` of
+// each frame), and the list's size.
+__global__ __launch_bounds__(256) void k_work_offsets(const Frame *frames, int nframes, WorkQueue *Q, int l,
+                                                     int which) {
+    uint32_t carry = 0;
+    for (int f0 = 0; f0 < nframes; f0 += 256) {
+        const int f = f0 + (int)threadIdx.x;
+        const uint32_t v = f < nframes ? frames[f].counters[which] : 0u;
+        uint32_t tot;
+        const uint32_t pre = bscan(v, &tot);
+        if (f < nframes) Q->off[l][f] = carry + pre;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        Q->off[l][nframes] = carry;
+        Q->total[l] = carry;
+    }
+}
+
 // ------------------------------------------------------------ moments
 
 constexpr int kMomChunk = 512;
 
-__global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int parity) {
+// Persistent: the kept clusters of every frame, large ones first, block b
+// taking global indices b, b + G, ... (a frame cursor per list).
+__global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q) {
     __shared__ double sm[6][kMomChunk];
-    const Frame F = frames[blockIdx.y];
-    const uint32_t ncl = F.counters[kNClusters];
     const int t = threadIdx.x;
-    uint64_t *skey = parity ? F.skey2 : F.skey;
-    const uint32_t *sval = parity ? F.sval2 : F.sval;
-    const int64_t pc = F.pcap;
-    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
-        if (!(F.cflag[c] & 1u)) continue;
+    for (int l = 0; l < 2; l++) {
+    const uint32_t total = Q->total[l];
+    const uint32_t *off = Q->off[l];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const uint32_t c = F.klist[(l ? F.kcap : 0u) + (g - off[fi])];
+        uint64_t *skey = parity ? F.skey2 : F.skey;
+        const uint32_t *sval = parity ? F.sval2 : F.sval;
+        const int64_t pc = F.pcap;
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         uint64_t *keys = skey + s0;
         // the full keys in order: slope << 32 | ~index
@@ -373,6 +425,7 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
             __syncthreads();
         }
     }
+    }
 }
 
 // ------------------------------------------------------------ corners
@@ -443,22 +496,35 @@ __device__ __forceinline__ uint64_t dkey(double d) {
 }
 
 constexpr int kMaxNm = 32;
+constexpr int kLdsPts = 1024;  // clusters up to this size keep their errors in LDS
 
-__global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const DevParams *P) {
-    __shared__ double pe[kMaxNm * kMaxNm], pm[kMaxNm * kMaxNm], pnx[kMaxNm * kMaxNm], pny[kMaxNm * kMaxNm];
+// Persistent, as k_fit_moments.
+__global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
+    // The errors, filtered errors and maxima values of a cluster (first
+    // part) share LDS with the pair-fit tables (second part).
+    __shared__ double sh[4 * kMaxNm * kMaxNm];
+    __shared__ uint32_t lidx[kLdsPts];
     __shared__ uint32_t mx[kMaxNm];
     __shared__ uint32_t hist[256];
     __shared__ uint64_t s_prefix;
-    __shared__ uint32_t s_rank, s_nm;
-    __shared__ double s_best[256];
-    __shared__ uint32_t s_bord[256];
-    const Frame F = frames[blockIdx.y];
-    const uint32_t ncl = F.counters[kNClusters];
-    const int t = threadIdx.x;
-    const int64_t pc = F.pcap;
+    __shared__ uint32_t s_rank;
+    __shared__ double s_thr;
+    __shared__ double s_wbest[4];
+    __shared__ uint32_t s_word[4];
+    static_assert(3 * kLdsPts <= 4 * kMaxNm * kMaxNm, "errors, filtered errors and maxima fit the shared area");
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const double max_mse = (double)P->max_mse, max_dot = (double)P->cos_crit;
     const double INF = __longlong_as_double(0x7ff0000000000000ll);
-    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+    double *pe = sh, *pm = sh + kMaxNm * kMaxNm, *pnx = sh + 2 * kMaxNm * kMaxNm, *pny = sh + 3 * kMaxNm * kMaxNm;
+    for (int l = 0; l < 2; l++) {
+    const uint32_t total = Q->total[l];
+    const uint32_t *off = Q->off[l];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const uint32_t c = F.klist[(l ? F.kcap : 0u) + (g - off[fi])];
+        const int64_t pc = F.pcap;
         uint32_t flag = F.cflag[c];
         if (!(flag & 1u)) continue;
         const uint32_t s0 = F.cstart[c];
@@ -468,7 +534,9 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         int64_t ksz = min((int64_t)20, sz / 12);
         if (ksz < 2) continue;
         double nn = (double)(2 * ksz + 1);
-        double *errs = F.errs + s0, *y = F.filt + s0;
+        const bool inl = sz <= kLdsPts;
+        double *errs = inl ? sh : F.errs + s0;
+        double *y = inl ? sh + kLdsPts : F.filt + s0;
         // window errors (fast/quads.c:654-686 edges, :604-635 middle)
         for (int64_t i = t; i < sz; i += 256) {
             int64_t i0 = i - ksz, i1 = i + ksz;
@@ -503,6 +571,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         }
         __syncthreads();
         // cyclic local maxima, in order (fast/quads.c:805-823)
+        uint32_t *midx = inl ? lidx : F.midx + s0;
         uint32_t base = 0;
         for (int64_t c0 = 0; c0 < sz; c0 += 256) {
             int64_t i = c0 + t;
@@ -513,47 +582,66 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
             }
             uint32_t tot;
             uint32_t pre = bscan(f, &tot);
-            if (f) F.midx[s0 + base + pre] = (uint32_t)i;
+            if (f) midx[base + pre] = (uint32_t)i;
             base += tot;
         }
         uint32_t nm = base;
         __syncthreads();
         if (nm < 4) continue;
-        const uint32_t *midx = F.midx + s0;
         if ((int64_t)nm > P->max_nmaxima) {
-            // the (max_nmaxima+1)-th largest maximum error: radix select
-            // over the order-preserving bits (fast/quads.c:826-862 finds
-            // the same value by quickselect)
-            uint32_t rank = (uint32_t)P->max_nmaxima;  // 0-based, descending
-            uint64_t prefix = 0;
-            for (int shift = 56; shift >= 0; shift -= 8) {
-                hist[t] = 0;
+            // the (max_nmaxima+1)-th largest maximum error (fast/quads.c:
+            // 826-862 finds the same value by quickselect); the maxima
+            // above it are kept, in order
+            const uint32_t rank = (uint32_t)P->max_nmaxima;  // 0-based, descending
+            double thr;
+            if (nm <= (uint32_t)kLdsPts) {
+                // the rank of each value among the maxima (ties by position)
+                double *mv = sh + 2 * kLdsPts;
+                for (uint32_t i = t; i < nm; i += 256) mv[i] = y[midx[i]];
                 __syncthreads();
-                uint64_t hmask = shift == 56 ? 0ull : (~0ull << (shift + 8));
                 for (uint32_t i = t; i < nm; i += 256) {
-                    uint64_t k = dkey(y[midx[i]]);
-                    if ((k & hmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
-                }
-                __syncthreads();
-                {
-                    // bins in descending order, one per thread: the bin
-                    // that holds the rank-th largest
-                    uint32_t d = 255u - (uint32_t)t, c = hist[d], tot;
-                    uint32_t before = bscan(c, &tot);
-                    if (rank >= before && rank < before + c) {
-                        s_prefix = prefix | ((uint64_t)d << shift);
-                        s_rank = rank - before;
+                    const double v = mv[i];
+                    uint32_t r = 0;
+                    for (uint32_t j = 0; j < nm; j++) {
+                        const double u = mv[j];
+                        r += (u > v || (u == v && j < i)) ? 1u : 0u;
                     }
+                    if (r == rank) s_thr = v;
                 }
                 __syncthreads();
-                prefix = s_prefix;
-                rank = s_rank;
-                __syncthreads();
+                thr = s_thr;
+            } else {
+                // radix select over the order-preserving bits
+                uint32_t rk = rank;
+                uint64_t prefix = 0;
+                for (int shift = 56; shift >= 0; shift -= 8) {
+                    hist[t] = 0;
+                    __syncthreads();
+                    uint64_t hmask = shift == 56 ? 0ull : (~0ull << (shift + 8));
+                    for (uint32_t i = t; i < nm; i += 256) {
+                        uint64_t k = dkey(y[midx[i]]);
+                        if ((k & hmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+                    }
+                    __syncthreads();
+                    {
+                        // bins in descending order, one per thread: the bin
+                        // that holds the rk-th largest
+                        uint32_t d = 255u - (uint32_t)t, cnt = hist[d], tot;
+                        uint32_t before = bscan(cnt, &tot);
+                        if (rk >= before && rk < before + cnt) {
+                            s_prefix = prefix | ((uint64_t)d << shift);
+                            s_rank = rk - before;
+                        }
+                    }
+                    __syncthreads();
+                    prefix = s_prefix;
+                    rk = s_rank;
+                    __syncthreads();
+                }
+                thr = __longlong_as_double((long long)((prefix >> 63) ? (prefix ^ 0x8000000000000000ull) : ~prefix));
             }
-            // the threshold as a double: maxima are kept if their error is
-            // greater (a comparison of doubles, as the reference's)
-            double thr = __longlong_as_double((long long)((prefix >> 63) ? (prefix ^ 0x8000000000000000ull) : ~prefix));
-            // keep the maxima above it, in order
+            // keep the maxima above the threshold (a comparison of doubles,
+            // as the reference's), in order
             uint32_t o = 0;
             for (uint32_t c0 = 0; c0 < nm; c0 += 256) {
                 uint32_t i = c0 + t;
@@ -616,17 +704,29 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                 }
             }
         }
-        s_best[t] = best;
-        s_bord[t] = bord;
+        // the least (error, loop order) over the block (equal infinite
+        // errors reject the cluster whatever their order)
+        for (int d = 32; d >= 1; d >>= 1) {
+            double e2 = __shfl_xor(best, d, 64);
+            uint32_t o2 = (uint32_t)__shfl_xor((int)bord, d, 64);
+            if (e2 < best || (e2 == best && o2 < bord)) {
+                best = e2;
+                bord = o2;
+            }
+        }
+        if (lane == 0) {
+            s_wbest[w] = best;
+            s_word[w] = bord;
+        }
         __syncthreads();
         if (t == 0) {
             double b = INF;
             uint32_t o = 0xffffffffu;
-            for (int k = 0; k < 256; k++) {
-                double e = s_best[k];
-                if (e < b || (e == b && e < INF && s_bord[k] < o)) {
+            for (int k = 0; k < 4; k++) {
+                double e = s_wbest[k];
+                if (e < b || (e == b && s_word[k] < o)) {
                     b = e;
-                    o = s_bord[k];
+                    o = s_word[k];
                 }
             }
             bool ok = b < INF && __ddiv_rn(b, (double)sz) < max_mse;
@@ -709,6 +809,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         }
         __syncthreads();
     }
+    }
 }
 
 // The fitted quads in cluster order (one block per frame): rank of each
@@ -727,17 +828,23 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity);
 
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks, Marks *mk) {
-    dim3 g(fit_blocks, nframes);
+                        int fit_blocks, WorkQueue *Q, Marks *mk) {
+    // k_fit_prep: every cluster of every frame, a grid per frame; the fit
+    // kernels after it: persistent, fit_blocks blocks over the work lists
+    dim3 g(256, nframes), gp(fit_blocks);
     hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams);
+    hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 0, (int)kNLarge);
+    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 1, (int)kNSmall);
     mk->mark(st, "fit_prep");
     int parity = 0;
     launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);
     mk->mark(st, "angle_sort");
-    hipLaunchKernelGGL(k_fit_moments, g, dim3(256), 0, st, dframes, parity);
+    hipLaunchKernelGGL(k_fit_moments, gp, dim3(256), 0, st, dframes, parity, Q);
     mk->mark(st, "moments");
-    hipLaunchKernelGGL(k_fit_corners, g, dim3(256), 0, st, dframes, dparams);
+    hipLaunchKernelGGL(k_fit_corners, gp, dim3(256), 0, st, dframes, dparams, Q);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 2, (int)kNQuads);
     mk->mark(st, "corners");
 }
 

@@ -19,7 +19,8 @@
 #include "chain.h"
 
 namespace atg {
-void launch_init(hipStream_t st, const Frame *dframes, int nframes);
+void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q);
+void launch_copy(hipStream_t st, void *dst, const void *src, int64_t bytes);
 CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch);
 size_t ccl_scratch_bytes(const CclPlan &c);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
@@ -29,9 +30,9 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
 void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
                          Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks, Marks *mk);
+                        int fit_blocks, WorkQueue *Q, Marks *mk);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
-                         Marks *mk);
+                         WorkQueue *Q, Marks *mk);
 }  // namespace atg
 
 using namespace atg;
@@ -42,7 +43,7 @@ constexpr int kResultDets = kMaxDetsPerFrame;  // detection capacity of a frame'
 constexpr int kInlineDets = 32;            // read back with the counters
 constexpr size_t kCountersBytes = 64;
 constexpr size_t kResultBlock = kCountersBytes + (size_t)kResultDets * sizeof(at_detection);
-constexpr Tune kDefaultTune = {1, 4, 256, 256};
+constexpr Tune kDefaultTune = {1, 4, 256, 1024};
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -68,6 +69,16 @@ int64_t cluster_map_size(int64_t w, int64_t h) {
     while (n < chunk * w / 64 && n < 65536) n <<= 1;
     return n;
 }
+
+// The persistent kernels' work lists of a call (in the arena, after the
+// frames' regions): the header and three offset arrays over the frames.
+size_t queue_bytes(int64_t nframes) {
+    return align256(sizeof(WorkQueue)) + 3 * align256((size_t)(nframes + 1) * 4);
+}
+
+// Kept clusters need at least a few points each, so a quarter of the point
+// capacity bounds a frame's list of them.
+int64_t cluster_cap(int64_t pcap) { return pcap / 4 + 64; }
 
 struct FramePlan {
     bool skip;            // the reference returns nothing for this frame
@@ -117,6 +128,7 @@ struct atg_ctx {
     bool arena_fixed = false;         // the arena never grows (a serving lane's budget)
     hipEvent_t ev2 = nullptr;         // the end of the last enqueued chain (with its readback)
     int32_t inflight = 0;             // frames of the enqueued chain
+    WorkQueue qstage{};               // the work lists' header of the call
     Tune tune = kDefaultTune;         // work splits of the chain
     bool graphs = true;               // replay the chain as a HIP graph when the call shape repeats
     hipGraphExec_t gexec = nullptr;   // the graph of the last call shape
@@ -322,6 +334,8 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->cstart = (uint32_t *)take((size_t)(pc + 2) * 4);
     F->ccount = nullptr;
     F->cflag = (uint32_t *)take((size_t)(pc + 1) * 4);
+    F->kcap = (uint32_t)cluster_cap(pc);
+    F->klist = (uint32_t *)take((size_t)F->kcap * 8);
     F->cid = (uint64_t *)take((size_t)(pc + 1) * 8);
     F->okey = (uint64_t *)take((size_t)pc * 8);
     F->mom = (double *)take((size_t)pc * 6 * 8);
@@ -460,6 +474,8 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     const CclPlan ccl = ccl_plan(max_qw, max_qh, ctx->force_scratch);
     const size_t gstride = ccl_scratch_bytes(ccl);
     total += gstride * (size_t)n;
+    const size_t qoff = total;
+    total += queue_bytes(n);
     if (total > ctx->arena_size && ctx->arena_fixed) return fail(ctx, ATG_ERR_LIMIT, "batch over the arena budget");
     if (total > ctx->arena_size) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -482,7 +498,8 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         HIPCHK(hipMalloc(&ctx->dframes, sizeof(Frame) * (size_t)n));
         ctx->dframes_cap = sizeof(Frame) * (size_t)n;
     }
-    size_t out_need = (kCountersBytes + kInlineDets * sizeof(at_detection)) * (size_t)n + sizeof(Frame) * (size_t)n;
+    size_t out_need = (kCountersBytes + kInlineDets * sizeof(at_detection)) * (size_t)n + sizeof(Frame) * (size_t)n +
+                      align256(sizeof(WorkQueue)) + 256;
     if (pinned_need > ctx->pinned_size) {
         HIPCHK(hipHostFree(ctx->pinned));
         ctx->pinned = nullptr;
@@ -504,7 +521,17 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     };
     std::vector<Copy> copies;
     size_t pin_at = 0;
-    uint8_t *gscratch = ctx->arena + (total - gstride * (size_t)n);
+    uint8_t *gscratch = ctx->arena + qoff - gstride * (size_t)n;
+    // the work lists: the header staged in pinned memory, copied in with the frames
+    WorkQueue *dq = reinterpret_cast<WorkQueue *>(ctx->arena + qoff);
+    {
+        uint8_t *qb = ctx->arena + qoff + align256(sizeof(WorkQueue));
+        WorkQueue hq{};
+        hq.nframes = (uint32_t)n;
+        for (int l = 0; l < 3; l++)
+            hq.off[l] = reinterpret_cast<uint32_t *>(qb + (size_t)l * align256((size_t)(n + 1) * 4));
+        ctx->qstage = hq;
+    }
     for (int32_t i = 0; i < n; i++) {
         FramePlan &fp = ctx->plans[(size_t)i];
         Frame &F = ctx->hframes[(size_t)i];
@@ -543,6 +570,9 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     copies.push_back(Copy{ctx->dframes, fstage, sizeof(Frame) * (size_t)n});
     const size_t pre = kCountersBytes + kInlineDets * sizeof(at_detection);
     uint8_t *hout = ctx->pinned_out + sizeof(Frame) * (size_t)n;
+    uint8_t *qstage = ctx->pinned_out + align256(sizeof(Frame) * (size_t)n + pre * (size_t)n);
+    memcpy(qstage, &ctx->qstage, sizeof(WorkQueue));
+    copies.push_back(Copy{dq, qstage, sizeof(WorkQueue)});
     hipStream_t st = ctx->stream;
     Marks *mk = &ctx->marks;
     mk->n = 0;
@@ -554,13 +584,13 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
             hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return e;
         }
-        launch_init(st, ctx->dframes, n);
+        launch_init(st, ctx->dframes, n, dq);
         mk->mark(st, "init");
         launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, blur, deglitch, tu, mk);
         launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, gscratch, tu, mk);
         launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
-        launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, mk);
-        launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, mk);
+        launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, dq, mk);
+        launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, dq, mk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         // results: counters and the first detections of every frame in one copy
@@ -698,7 +728,7 @@ size_t batch_arena_bytes(const atg_ctx *ctx, const atg_params *p, const atg_fram
         max_qw = std::max<int>(max_qw, (int)fp.qw);
         max_qh = std::max<int>(max_qh, (int)fp.qh);
     }
-    return total + ccl_scratch_bytes(ccl_plan(max_qw, max_qh, ctx->force_scratch)) * (size_t)n;
+    return total + ccl_scratch_bytes(ccl_plan(max_qw, max_qh, ctx->force_scratch)) * (size_t)n + queue_bytes(n);
 }
 
 // A fixed arena of `bytes` and result space for `max_frames`: later calls
@@ -718,7 +748,8 @@ int ctx_reserve(atg_ctx *ctx, size_t bytes, int32_t max_frames) {
     ctx->dframes = nullptr;
     ctx->dframes_cap = sizeof(Frame) * (size_t)max_frames;
     HIPCHK(hipMalloc(&ctx->dframes, ctx->dframes_cap));
-    size_t out_need = (kCountersBytes + kInlineDets * sizeof(at_detection) + sizeof(Frame)) * (size_t)max_frames;
+    size_t out_need = (kCountersBytes + kInlineDets * sizeof(at_detection) + sizeof(Frame)) * (size_t)max_frames +
+                      align256(sizeof(WorkQueue)) + 256;
     HIPCHK(hipHostFree(ctx->pinned_out));
     ctx->pinned_out = nullptr;
     HIPCHK(hipHostMalloc(&ctx->pinned_out, out_need));
@@ -798,7 +829,7 @@ static bool valid_tune(const Tune &t) {
     // row_waves: the per-row kernels are compiled for blocks of at most 256
     // threads (4 waves)
     return in(t.thr_tile_rows, {1, 2, 4}) && in(t.row_waves, {1, 2, 4}) && in(t.front_threads, {128, 256, 512}) &&
-           t.fit_blocks >= 32 && t.fit_blocks <= 1024;
+           t.fit_blocks >= 32 && t.fit_blocks <= 4096;
 }
 
 extern "C" int atg_get_tune(const atg_ctx *ctx, atg_tune *out) {
@@ -866,7 +897,7 @@ extern "C" int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *
     Tune best = start;
     double bms = 0.0;
     int rc = measure(best, &bms);
-    const std::vector<int> cand[4] = {{1, 2, 4}, {1, 2, 4}, {128, 256, 512}, {64, 128, 256, 512, 1024}};
+    const std::vector<int> cand[4] = {{1, 2, 4}, {1, 2, 4}, {128, 256, 512}, {256, 512, 1024, 2048, 4096}};
     for (int k = 0; k < 4 && rc == ATG_OK; k++) {
         for (int v : cand[k]) {
             Tune t = best;
@@ -888,6 +919,20 @@ extern "C" int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *
     if (rc) return rc;
     if (best_out) memcpy(best_out, &best, sizeof best);
     if (best_ms) *best_ms = bms;
+    return ATG_OK;
+}
+
+extern "C" int atg_debug_copy(atg_ctx *ctx, void *dst, const void *src, int64_t bytes, double *ms) {
+    if (!ctx || !dst || !src || bytes < 16 || bytes % 16) return ATG_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    launch_copy(ctx->stream, dst, src, bytes);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+    if (ms) *ms = f;
     return ATG_OK;
 }
 

@@ -41,6 +41,22 @@ def pctl(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
+def measured_traffic(batch):
+    """HBM bytes per launch of each kernel group at this batch size, as the
+    rocprofv3 FETCH_SIZE and WRITE_SIZE runs measured them (scaled by the
+    copy kernel's readings): the table of profiles/traffic_1080p_batch<B>.txt,
+    if that profile exists."""
+    path = REPO / "profiles" / ("traffic_1080p_batch%d.txt" % batch)
+    out = {}
+    if not path.exists():
+        return out
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0].startswith("group:"):
+            out[cells[0][len("group:"):].strip()] = int(cells[3])
+    return out
+
+
 def frame_counts(ctx, i, W, H):
     """The data-dependent sizes of frame i of the last call: R runs, N
     nodes, E union-find links, P points, C clusters, K points of the
@@ -257,6 +273,7 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (config 4) measurement")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-loads", default="500,1000,2000,4000,8000,20000")
     args = ap.parse_args()
@@ -333,8 +350,9 @@ def main():
     dom_name, dom_ms = max(stages, key=lambda kv: kv[1])
     ach = kb[dom_name] / (dom_ms * 1e-3) / 1e9
     bound = "hbm"
+    traffic = measured_traffic(args.batch).get(dom_name)
     roof = dict(kernel=dom_name, bound=bound, achieved=round(ach, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(ach / HBM_PEAK_GBS, 6), traffic=None, bytes_per_launch=int(kb[dom_name]),
+                frac=round(ach / HBM_PEAK_GBS, 6), traffic=traffic, bytes_per_launch=int(kb[dom_name]),
                 ms_per_launch=round(dom_ms, 4))
     ac = ("decimate", "threshold_rle", "uf_edges", "uf_front", "uf_replay", "uf_stitch", "resolve", "labels",
           "points", "cluster_sort", "clusters")
@@ -342,9 +360,14 @@ def main():
     b_ac = sum(kb[k] for k in ac if k in kb)
     per_kernel = {k: {"ms": round(v, 4), "GB/s": round(kb.get(k, 0) / (v * 1e-3) / 1e9, 2)} for k, v in stages}
 
-    log("config 4: uneven batch")
-    batching = batching_config4(atg, ctx, max(2, args.steps // 4))
-    ctx.set_families(fams)
+    batching = None
+    if not args.no_batching:
+        log("config 4: uneven batch")
+        batching = batching_config4(atg, ctx, max(2, args.steps // 4))
+        ctx.set_families(fams)
+    # a plain copy kernel of known size: the reference for the traffic
+    # counters and for what HBM delivers to a simple kernel here
+    copy_gbs, _ = ctx.copy_bandwidth(1 << 29)
     serving = None
     if not args.no_serve:
         serving = serving_sweep(atg, ctx, [int(x) for x in args.serve_loads.split(",")], args.serve_seconds,
@@ -377,6 +400,7 @@ def main():
         "kernel_gbs": per_kernel,
         "stages_a_c": {"bytes": int(b_ac), "ms": round(t_ac, 4), "GB/s": round(b_ac / (t_ac * 1e-3) / 1e9, 2)},
         "roofline": roof,
+        "copy_kernel_gbs": round(copy_gbs, 1),
         "batching_uneven": batching,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -139,6 +139,11 @@ int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int
  * instead of launching each kernel. */
 int atg_set_graphs(atg_ctx *ctx, int on);
 
+/* A plain device-to-device copy kernel of `bytes` (a multiple of 16) on
+ * the context's stream, its device time in *ms: the known-size reference
+ * that HBM traffic counters are scaled by. */
+int atg_debug_copy(atg_ctx *ctx, void *dst, const void *src, int64_t bytes, double *ms);
+
 /* on != 0: the following calls keep the union-find replay's state in HBM
  * scratch, the path frames too wide for LDS take, so that tests can check
  * it on any frame. */
