@@ -2,14 +2,16 @@
 // homography, bit sampling and code lookup; per frame, the overlap
 // reconcile and the stable sort by id.
 //
-//   k_decode   one block per quad. The samples of refine_edges
-//              (fast/decode.c:657-761) run one per thread, each with the
-//              reference's 4-lane step sums (:549-626); their sums over the
-//              edge are then added in sample order by one thread. The bit
-//              sampler (:291-402) gathers its pixel samples in parallel and
-//              accumulates the gray models and scores in the reference's
-//              order in one thread. Lookup walks the quick-decode bucket of
-//              each chunk (:87-107).
+//   k_decode   persistent blocks over the quads of all frames. refine_edges
+//              (fast/decode.c:657-761): a wave per edge, a sample per lane
+//              with the reference's 4-lane step sums (:549-626), the edge's
+//              sums added in sample order by its lane 0. The homography's
+//              elimination runs in LDS, its row updates in parallel. The
+//              bit sampler (:291-402) gathers its pixel samples in parallel;
+//              the gray models and the scores are added in the reference's
+//              order by one thread; each cell's threshold and sharpening are
+//              a thread's. Lookup walks the quick-decode bucket of each
+//              chunk (:87-107).
 //   k_finish   one block per frame: the detections in (quad, family)
 //              order, reconcile (:860-891), stable sort by id
 //              (fast/detector.c:164-179).
@@ -82,11 +84,15 @@ __device__ bool refine_sample(const View &v, int64_t range, int64_t max_steps, i
                               double nx, double ny, double *bx, double *by) {
     double rf = (double)-range;
     double ln[4] = {0, 0, 0, 0}, lc[4] = {0, 0, 0, 0};
-    for (int64_t st = 0; st < vsteps; st++) {
-        double n = DADD(rf, DMUL(0.25, (double)st));
-        double w = lane_weight(v, x0, y0, nx, ny, n);
-        ln[st & 3] = DADD(ln[st & 3], DMUL(w, n));
-        lc[st & 3] = DADD(lc[st & 3], w);
+    // vsteps is a multiple of 4: step st adds to lane st & 3, in order
+    for (int64_t s0 = 0; s0 < vsteps; s0 += 4) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            double n = DADD(rf, DMUL(0.25, (double)(s0 + q)));
+            double w = lane_weight(v, x0, y0, nx, ny, n);
+            ln[q] = DADD(ln[q], DMUL(w, n));
+            lc[q] = DADD(lc[q], w);
+        }
     }
     double mn = DADD(0.0, DADD(DADD(ln[0], ln[2]), DADD(ln[1], ln[3])));
     double mc = DADD(0.0, DADD(DADD(lc[0], lc[2]), DADD(lc[1], lc[3])));
@@ -112,46 +118,79 @@ __device__ bool refine_sample(const View &v, int64_t range, int64_t max_steps, i
     return true;
 }
 
-// homography_compute2 (fast/decode.c:114-165)
-__device__ bool homography(const double c[4][4], double hr[9]) {
-    double a[72];
-    for (int i = 0; i < 72; i++) a[i] = 0.0;
-    for (int i = 0; i < 4; i++) {
-        int r0 = 18 * i, r1 = r0 + 9;
-        a[r0 + 0] = c[i][0]; a[r0 + 1] = c[i][1]; a[r0 + 2] = 1.0;
-        a[r0 + 6] = DMUL(-c[i][0], c[i][2]); a[r0 + 7] = DMUL(-c[i][1], c[i][2]); a[r0 + 8] = c[i][2];
-        a[r1 + 3] = c[i][0]; a[r1 + 4] = c[i][1]; a[r1 + 5] = 1.0;
-        a[r1 + 6] = DMUL(-c[i][0], c[i][3]); a[r1 + 7] = DMUL(-c[i][1], c[i][3]); a[r1 + 8] = c[i][3];
+// homography_compute2 (fast/decode.c:114-165): the 8x9 system of the four
+// corner correspondences, Gaussian elimination with partial pivoting, in
+// LDS. Each pivot step's row updates are independent element by element,
+// so the block's threads make them (the same operations per element as
+// the reference's loops); the pivot search and back substitution are one
+// thread's. c[i] = (x, y) of the tag corner, qp[i] its image point. Every
+// thread returns the same result; hr is written by thread 0.
+__device__ bool homography_block(const float qp[4][2], double *A, int *s_piv, double *hr) {
+    const int t = threadIdx.x;
+    if (t < 72) {
+        const int row = t / 9, col = t % 9, i = row >> 1;
+        const double cx = (i == 0 || i == 3) ? -1.0 : 1.0, cy = (i == 0 || i == 1) ? -1.0 : 1.0;
+        const double px = (double)qp[i][0], py = (double)qp[i][1];
+        const double pv = (row & 1) ? py : px;  // the image coordinate of this row
+        const int o = (row & 1) ? 3 : 0;        // where (x, y, 1) sits
+        double v = 0.0;
+        if (col == o) v = cx;
+        else if (col == o + 1) v = cy;
+        else if (col == o + 2) v = 1.0;
+        else if (col == 6) v = DMUL(-cx, pv);
+        else if (col == 7) v = DMUL(-cy, pv);
+        else if (col == 8) v = pv;
+        A[t] = v;
     }
+    __syncthreads();
+    bool ok = true;
     for (int col = 0; col < 8; col++) {
-        double mv = 0.0;
-        int mi = -1;
-        for (int row = col; row < 8; row++) {
-            double v = fabs(a[row * 9 + col]);
-            if (v > mv) { mv = v; mi = row; }
+        if (t == 0) {
+            double mv = 0.0;
+            int mi = -1;
+            for (int row = col; row < 8; row++) {
+                double v = fabs(A[row * 9 + col]);
+                if (v > mv) { mv = v; mi = row; }
+            }
+            *s_piv = (mi < 0 || mv < 1e-10) ? -1 : mi;
         }
-        if (mi < 0 || mv < 1e-10) return false;
-        if (mi != col)
-            for (int i = col; i < 9; i++) { double t = a[col * 9 + i]; a[col * 9 + i] = a[mi * 9 + i]; a[mi * 9 + i] = t; }
-        for (int i = col + 1; i < 8; i++) {
-            double f = DDIV(a[i * 9 + col], a[col * 9 + col]);
-            a[i * 9 + col] = 0.0;
-            for (int j = col + 1; j < 9; j++) a[i * 9 + j] = DSUB(a[i * 9 + j], DMUL(f, a[col * 9 + j]));
+        __syncthreads();
+        const int mi = *s_piv;
+        if (mi < 0) { ok = false; break; }
+        if (mi != col && t >= col && t < 9) {
+            double x = A[col * 9 + t];
+            A[col * 9 + t] = A[mi * 9 + t];
+            A[mi * 9 + t] = x;
         }
+        __syncthreads();
+        double nv = 0.0;
+        bool wr = false;
+        if (t < 72) {
+            const int i = t / 9, j = t % 9;
+            if (i > col && j >= col) {
+                double f = DDIV(A[i * 9 + col], A[col * 9 + col]);
+                nv = j == col ? 0.0 : DSUB(A[i * 9 + j], DMUL(f, A[col * 9 + j]));
+                wr = true;
+            }
+        }
+        __syncthreads();
+        if (wr) A[t] = nv;
+        __syncthreads();
     }
-    for (int col = 7; col >= 0; col--) {
-        double sum = 0.0;
-        for (int i = col + 1; i < 8; i++) sum = DADD(sum, DMUL(a[col * 9 + i], a[i * 9 + 8]));
-        a[col * 9 + 8] = DDIV(DSUB(a[col * 9 + 8], sum), a[col * 9 + col]);
+    if (ok && t == 0) {
+        for (int col = 7; col >= 0; col--) {
+            double sum = 0.0;
+            for (int i = col + 1; i < 8; i++) sum = DADD(sum, DMUL(A[col * 9 + i], A[i * 9 + 8]));
+            A[col * 9 + 8] = DDIV(DSUB(A[col * 9 + 8], sum), A[col * 9 + col]);
+        }
+        for (int r = 0; r < 8; r++) hr[r] = A[r * 9 + 8];
+        hr[8] = 1.0;
     }
-    hr[0] = a[8]; hr[1] = a[17]; hr[2] = a[26]; hr[3] = a[35]; hr[4] = a[44];
-    hr[5] = a[53]; hr[6] = a[62]; hr[7] = a[71]; hr[8] = 1.0;
-    return true;
+    return ok;
 }
 
 // matd_invertible (fast/decode.c:168-196)
-__device__ bool invertible(const double m[9]) {
-    double lu[9];
+__device__ bool invertible(const double m[9], double *lu) {
     for (int i = 0; i < 9; i++) lu[i] = m[i];
     for (int j = 0; j < 3; j++) {
         for (int i = 0; i < 3; i++) {
@@ -245,20 +284,42 @@ __device__ __forceinline__ uint64_t rotate90(uint64_t w0, int64_t nbits) {
 
 constexpr int kMaxSamples = 512;  // bit-sampler points: 8 * wab + nbits
 
+// The border sampling patterns of quad_sample (fast/decode.c:296-305):
+// start x, start y, step x, step y, and whether the side is white.
+__device__ __forceinline__ void border_pattern(int pi, int wab, float *x0, float *y0, float *dx, float *dy,
+                                               bool *white) {
+    const float hi = (float)((double)wab + 0.5), lo = (float)((double)wab - 0.5);
+    const bool vert = pi < 4;  // the first four walk down a column
+    const int k = pi & 3;
+    const float a = k == 0 ? -0.5f : k == 1 ? 0.5f : k == 2 ? hi : lo;
+    *x0 = vert ? a : 0.5f;
+    *y0 = vert ? 0.5f : a;
+    *dx = vert ? 0.0f : 1.0f;
+    *dy = vert ? 1.0f : 0.0f;
+    *white = (k & 1) == 0;
+}
+
+// A barrier of one wave over its LDS operations.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Persistent: the quads of every frame, block b taking global indices b,
 // b + G, ... (a frame cursor).
 __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
-    __shared__ double sx[256], sy[256];
-    __shared__ int sv[256];
+    __shared__ double wsx[4][64], wsy[4][64];
+    __shared__ int wsv[4][64];
     __shared__ double smp[3][kMaxSamples];  // tag x, tag y, value
     __shared__ int sflag[kMaxSamples];
     __shared__ float qp[4][2];
     __shared__ double lines[4][4];
-    __shared__ double sH[9];
-    __shared__ int s_hok;
+    __shared__ double sH[9], sA[72], sLU[9];
+    __shared__ int s_hok, s_piv, s_polar;
     __shared__ Gray g_white, g_black;
     __shared__ double vals[12 * 12], sharp[12 * 12];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int nfam = P->nfam;
     const uint32_t total = Q->total[2];
     const uint32_t *off = Q->off[2];
@@ -279,61 +340,61 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
         }
         __syncthreads();
         if (P->refine) {
-            int64_t range = (int64_t)(P->decimate + 1.0f);
-            int64_t max_steps = 2 * 4 * range + 1;
+            // refine_edges (fast/decode.c:657-761): wave e takes edge e; its
+            // lanes sample, lane 0 adds the samples up in order
+            const int64_t range = (int64_t)(P->decimate + 1.0f);
+            const int64_t max_steps = 2 * 4 * range + 1;
             int64_t vsteps = max_steps - max_steps % 4;
             if (vsteps < 0) vsteps = 0;
-            for (int edge = 0; edge < 4; edge++) {
-                int a = edge, b = (edge + 1) & 3;
-                double nx = (double)__fsub_rn(qp[b][1], qp[a][1]);
-                double ny = (double)__fadd_rn(-qp[b][0], qp[a][0]);
-                double mag = __dsqrt_rn(DADD(DMUL(nx, nx), DMUL(ny, ny)));
-                nx = DDIV(nx, mag);
-                ny = DDIV(ny, mag);
-                if (q.reversed) { nx = -nx; ny = -ny; }
-                int64_t ns = ctrunc(DDIV(mag, 8.0));
-                if (ns < 16) ns = 16;
-                double mx = 0.0, my = 0.0, mxx = 0.0, mxy = 0.0, myy = 0.0, nn = 0.0;
-                for (int64_t c0 = 0; c0 < ns; c0 += 256) {
-                    int64_t si = c0 + t;
-                    int ok = 0;
-                    if (si < ns) {
-                        double alpha = DDIV(DADD(1.0, (double)si), (double)(ns + 1));
-                        double x0 = DADD(DMUL(alpha, (double)qp[a][0]), DMUL(DSUB(1.0, alpha), (double)qp[b][0]));
-                        double y0 = DADD(DMUL(alpha, (double)qp[a][1]), DMUL(DSUB(1.0, alpha), (double)qp[b][1]));
-                        double bx = 0, by = 0;
-                        ok = refine_sample(v, range, max_steps, vsteps, x0, y0, nx, ny, &bx, &by) ? 1 : 0;
-                        sx[t] = bx;
-                        sy[t] = by;
-                    }
-                    sv[t] = ok;
-                    __syncthreads();
-                    if (t == 0) {
-                        int64_t len = ns - c0 < 256 ? ns - c0 : 256;
-                        for (int64_t k = 0; k < len; k++) {
-                            if (!sv[k]) continue;
-                            double bx = sx[k], by = sy[k];
-                            mx = DADD(mx, bx); my = DADD(my, by);
-                            mxx = DADD(mxx, DMUL(bx, bx)); mxy = DADD(mxy, DMUL(bx, by)); myy = DADD(myy, DMUL(by, by));
-                            nn = DADD(nn, 1.0);
-                        }
-                    }
-                    __syncthreads();
+            const int ea = wv, eb = (wv + 1) & 3;
+            double nx = (double)__fsub_rn(qp[eb][1], qp[ea][1]);
+            double ny = (double)__fadd_rn(-qp[eb][0], qp[ea][0]);
+            const double mag = __dsqrt_rn(DADD(DMUL(nx, nx), DMUL(ny, ny)));
+            nx = DDIV(nx, mag);
+            ny = DDIV(ny, mag);
+            if (q.reversed) { nx = -nx; ny = -ny; }
+            int64_t ns = ctrunc(DDIV(mag, 8.0));
+            if (ns < 16) ns = 16;
+            double mx = 0.0, my = 0.0, mxx = 0.0, mxy = 0.0, myy = 0.0, nn = 0.0;
+            for (int64_t c0 = 0; c0 < ns; c0 += 64) {
+                const int64_t si = c0 + lane;
+                int ok = 0;
+                if (si < ns) {
+                    double alpha = DDIV(DADD(1.0, (double)si), (double)(ns + 1));
+                    double x0 = DADD(DMUL(alpha, (double)qp[ea][0]), DMUL(DSUB(1.0, alpha), (double)qp[eb][0]));
+                    double y0 = DADD(DMUL(alpha, (double)qp[ea][1]), DMUL(DSUB(1.0, alpha), (double)qp[eb][1]));
+                    double bx = 0, by = 0;
+                    ok = refine_sample(v, range, max_steps, vsteps, x0, y0, nx, ny, &bx, &by) ? 1 : 0;
+                    wsx[wv][lane] = bx;
+                    wsy[wv][lane] = by;
                 }
-                if (t == 0) {
-                    double ex = DDIV(mx, nn), ey = DDIV(my, nn);
-                    double cxx = DSUB(DDIV(mxx, nn), DMUL(ex, ex));
-                    double cxy = DSUB(DDIV(mxy, nn), DMUL(ex, ey));
-                    double cyy = DSUB(DDIV(myy, nn), DMUL(ey, ey));
-                    float th2 = (float)atan2((double)(float)DMUL(-2.0, cxy), (double)(float)DSUB(cyy, cxx));
-                    double theta = DMUL(0.5, (double)th2);
-                    lines[edge][0] = ex;
-                    lines[edge][1] = ey;
-                    lines[edge][2] = (double)(float)cos((double)(float)theta);
-                    lines[edge][3] = (double)(float)sin((double)(float)theta);
+                wsv[wv][lane] = ok;
+                wave_sync();
+                if (lane == 0) {
+                    const int64_t len = ns - c0 < 64 ? ns - c0 : 64;
+                    for (int64_t k = 0; k < len; k++) {
+                        if (!wsv[wv][k]) continue;
+                        const double bx = wsx[wv][k], by = wsy[wv][k];
+                        mx = DADD(mx, bx); my = DADD(my, by);
+                        mxx = DADD(mxx, DMUL(bx, bx)); mxy = DADD(mxy, DMUL(bx, by)); myy = DADD(myy, DMUL(by, by));
+                        nn = DADD(nn, 1.0);
+                    }
                 }
-                __syncthreads();
+                wave_sync();
             }
+            if (lane == 0) {
+                double ex = DDIV(mx, nn), ey = DDIV(my, nn);
+                double cxx = DSUB(DDIV(mxx, nn), DMUL(ex, ex));
+                double cxy = DSUB(DDIV(mxy, nn), DMUL(ex, ey));
+                double cyy = DSUB(DDIV(myy, nn), DMUL(ey, ey));
+                float th2 = (float)atan2((double)(float)DMUL(-2.0, cxy), (double)(float)DSUB(cyy, cxx));
+                double theta = DMUL(0.5, (double)th2);
+                lines[wv][0] = ex;
+                lines[wv][1] = ey;
+                lines[wv][2] = (double)(float)cos((double)(float)theta);
+                lines[wv][3] = (double)(float)sin((double)(float)theta);
+            }
+            __syncthreads();
             if (t == 0) {
                 for (int i = 0; i < 4; i++) {
                     int j = (i + 1) & 3;
@@ -348,41 +409,28 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     }
                 }
             }
+            __syncthreads();
         }
-        if (t == 0) {
-            double corr[4][4];
-            for (int i = 0; i < 4; i++) {
-                corr[i][0] = i == 0 || i == 3 ? -1.0 : 1.0;
-                corr[i][1] = i == 0 || i == 1 ? -1.0 : 1.0;
-                corr[i][2] = qp[i][0];
-                corr[i][3] = qp[i][1];
-            }
-            double h[9];
-            int hok = homography(corr, h) ? 1 : 0;
-            if (hok) {
-                for (int i = 0; i < 9; i++) sH[i] = h[i];
-                hok = invertible(h) ? 1 : 0;
-            }
-            s_hok = hok;
-        }
+        // homography (fast/decode.c:198-210) and its invertibility
+        bool hok = homography_block(qp, sA, &s_piv, sH);
+        if (t == 0) s_hok = hok && invertible(sH, sLU) ? 1 : 0;
         __syncthreads();
-        for (int fi = 0; fi < nfam; fi++) {
-            if (t == 0) F.dflag[qi * nfam + fi] = 0;
-            const DevFamily &fm = P->fam[fi];
+        for (int fam = 0; fam < nfam; fam++) {
+            if (t == 0) F.dflag[qi * nfam + fam] = 0;
+            const DevFamily &fm = P->fam[fam];
             if (!s_hok || fm.reversed != q.reversed) continue;
             const int wab = fm.wab;
             const float wf = (float)wab;
             const int nb = 8 * wab;
             if (nb + fm.nbits > kMaxSamples || fm.tw > 12) continue;
             // border samples (fast/decode.c:310-326), in parallel
-            const float pat[40] = {-0.5f, 0.5f, 0.0f, 1.0f, 1.0f,   0.5f, 0.5f, 0.0f, 1.0f, 0.0f,
-                                   (float)(wab + 0.5), 0.5f, 0.0f, 1.0f, 1.0f, (float)(wab - 0.5), 0.5f, 0.0f, 1.0f, 0.0f,
-                                   0.5f, -0.5f, 1.0f, 0.0f, 1.0f,   0.5f, 0.5f, 1.0f, 0.0f, 0.0f,
-                                   0.5f, (float)(wab + 0.5), 1.0f, 0.0f, 1.0f, 0.5f, (float)(wab - 0.5), 1.0f, 0.0f, 0.0f};
             for (int k = t; k < nb; k += 256) {
-                int pi = k / wab, i = k % wab;
-                double tx01 = (double)fdiv_cr(__fadd_rn(pat[pi * 5], __fmul_rn((float)i, pat[pi * 5 + 2])), wf);
-                double ty01 = (double)fdiv_cr(__fadd_rn(pat[pi * 5 + 1], __fmul_rn((float)i, pat[pi * 5 + 3])), wf);
+                const int pi = k / wab, i = k % wab;
+                float bx0, by0, bdx, bdy;
+                bool white;
+                border_pattern(pi, wab, &bx0, &by0, &bdx, &bdy, &white);
+                double tx01 = (double)fdiv_cr(__fadd_rn(bx0, __fmul_rn((float)i, bdx)), wf);
+                double ty01 = (double)fdiv_cr(__fadd_rn(by0, __fmul_rn((float)i, bdy)), wf);
                 double tx = DMUL(2.0, DSUB(tx01, 0.5)), ty = DMUL(2.0, DSUB(ty01, 0.5));
                 double zz = DADD(DADD(DMUL(sH[6], tx), DMUL(sH[7], ty)), sH[8]);
                 double px = DDIV(DADD(DADD(DMUL(sH[0], tx), DMUL(sH[1], ty)), sH[2]), zz);
@@ -391,7 +439,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                 int f = 0;
                 double val = 0.0;
                 if (!(ix < 0 || iy < 0 || ix >= v.w || iy >= v.h)) {
-                    f = 1 | (pat[pi * 5 + 4] != 0.0f ? 2 : 0);
+                    f = 1 | (white ? 2 : 0);
                     val = (double)v.im[iy * v.s + ix];
                 }
                 smp[0][k] = tx;
@@ -414,43 +462,58 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                 smp[2][k] = pixel_value(v, px, py);
             }
             __syncthreads();
+            // the gray models, in sample order (fast/decode.c:229-265)
             if (t == 0) {
                 Gray wh, bl;
                 for (int i = 0; i < 9; i++) { wh.a[i] = 0; bl.a[i] = 0; }
                 for (int i = 0; i < 3; i++) { wh.b[i] = 0; bl.b[i] = 0; wh.c[i] = 0; bl.c[i] = 0; }
                 for (int k = 0; k < nb; k++) {
-                    if (!(sflag[k] & 1)) continue;
-                    gray_add((sflag[k] & 2) ? wh : bl, smp[0][k], smp[1][k], smp[2][k]);
+                    const int f = sflag[k];
+                    if (!(f & 1)) continue;
+                    if (f & 2) gray_add(wh, smp[0][k], smp[1][k], smp[2][k]);
+                    else gray_add(bl, smp[0][k], smp[1][k], smp[2][k]);
                 }
                 gray_solve(wh);
                 if (wab > 1) gray_solve(bl);
                 else { bl.c[0] = 0.0; bl.c[1] = 0.0; bl.c[2] = DDIV(bl.b[2], 4.0); }
                 double wi = DADD(DADD(DMUL(wh.c[0], 0.0), DMUL(wh.c[1], 0.0)), wh.c[2]);
                 double bi = DADD(DADD(DMUL(bl.c[0], 0.0), DMUL(bl.c[1], 0.0)), bl.c[2]);
-                bool polar = (DSUB(wi, bi) < 0.0) == (fm.reversed != 0);
-                if (polar) {
-                    const int tw = fm.tw;
-                    for (int i = 0; i < tw * tw; i++) vals[i] = 0.0;
-                    for (int i = 0; i < fm.nbits; i++) {
-                        double val = smp[2][nb + i];
-                        if (val == -1.0) continue;
-                        double tx = smp[0][nb + i], ty = smp[1][nb + i];
-                        double th = DDIV(DADD(DADD(DADD(DMUL(bl.c[0], tx), DMUL(bl.c[1], ty)), bl.c[2]),
-                                              DADD(DADD(DMUL(wh.c[0], tx), DMUL(wh.c[1], ty)), wh.c[2])),
-                                         2.0);
-                        vals[tw * (fm.bit_y[i] - mc) + (fm.bit_x[i] - mc)] = DSUB(val, th);
-                    }
-                    for (int y = 0; y < tw; y++)
-                        for (int x = 0; x < tw; x++) {
-                            double acc = 0.0;
-                            if (y > 0) acc = DADD(acc, -vals[(y - 1) * tw + x]);
-                            if (x > 0) acc = DADD(acc, -vals[y * tw + x - 1]);
-                            acc = DADD(acc, DMUL(vals[y * tw + x], 4.0));
-                            if (x + 1 < tw) acc = DADD(acc, -vals[y * tw + x + 1]);
-                            if (y + 1 < tw) acc = DADD(acc, -vals[(y + 1) * tw + x]);
-                            sharp[y * tw + x] = acc;
-                        }
-                    for (int i = 0; i < tw * tw; i++) vals[i] = DADD(vals[i], DMUL(P->sharpening, sharp[i]));
+                s_polar = (DSUB(wi, bi) < 0.0) == (fm.reversed != 0) ? 1 : 0;
+                g_white = wh;
+                g_black = bl;
+            }
+            __syncthreads();
+            if (s_polar) {
+                const int tw = fm.tw;
+                for (int i = t; i < tw * tw; i += 256) vals[i] = 0.0;
+                __syncthreads();
+                // each bit's value against the models' threshold, in its cell
+                for (int i = t; i < fm.nbits; i += 256) {
+                    double val = smp[2][nb + i];
+                    if (val == -1.0) continue;
+                    double tx = smp[0][nb + i], ty = smp[1][nb + i];
+                    const double *bc = g_black.c, *wc = g_white.c;
+                    double th = DDIV(DADD(DADD(DADD(DMUL(bc[0], tx), DMUL(bc[1], ty)), bc[2]),
+                                          DADD(DADD(DMUL(wc[0], tx), DMUL(wc[1], ty)), wc[2])),
+                                     2.0);
+                    vals[tw * (fm.bit_y[i] - mc) + (fm.bit_x[i] - mc)] = DSUB(val, th);
+                }
+                __syncthreads();
+                // sharpening (fast/decode.c:370-385), each cell in tap order
+                for (int i = t; i < tw * tw; i += 256) {
+                    const int y = i / tw, x = i % tw;
+                    double acc = 0.0;
+                    if (y > 0) acc = DADD(acc, -vals[(y - 1) * tw + x]);
+                    if (x > 0) acc = DADD(acc, -vals[y * tw + x - 1]);
+                    acc = DADD(acc, DMUL(vals[y * tw + x], 4.0));
+                    if (x + 1 < tw) acc = DADD(acc, -vals[y * tw + x + 1]);
+                    if (y + 1 < tw) acc = DADD(acc, -vals[(y + 1) * tw + x]);
+                    sharp[i] = acc;
+                }
+                __syncthreads();
+                for (int i = t; i < tw * tw; i += 256) vals[i] = DADD(vals[i], DMUL(P->sharpening, sharp[i]));
+                __syncthreads();
+                if (t == 0) {
                     float bs = 0.0f, ws = 0.0f, bc = 1.0f, wc = 1.0f;
                     uint64_t rc = 0;
                     for (int i = 0; i < fm.nbits; i++) {
@@ -488,23 +551,23 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     if (margin >= 0.0f && ham < 255) {
                         // make_detection (fast/detector.c:83-107)
                         at_detection d;
-                        double c = P->rot_cs[rot][0], s = P->rot_cs[rot][1];
-                        const double rr[9] = {c, -s, 0.0, s, c, 0.0, 0.0, 0.0, 1.0};
+                        double c = P->rot_cs[rot][0], sn = P->rot_cs[rot][1];
+                        const double rr[9] = {c, -sn, 0.0, sn, c, 0.0, 0.0, 0.0, 1.0};
                         for (int i = 0; i < 3; i++)
                             for (int j = 0; j < 3; j++) {
                                 double acc = 0.0;
                                 for (int k = 0; k < 3; k++) acc = DADD(acc, DMUL(sH[i * 3 + k], rr[k * 3 + j]));
                                 d.H[i * 3 + j] = acc;
                             }
-                        d.family = fi;
+                        d.family = fam;
                         d.id = (int32_t)id;
                         d.hamming = (int32_t)ham;
                         d.decision_margin = margin;
                         project(d.H, 0.0, 0.0, &d.c[0], &d.c[1]);
                         for (int i = 0; i < 4; i++)
                             project(d.H, i == 1 || i == 2 ? 1.0 : -1.0, i < 2 ? 1.0 : -1.0, &d.p[i][0], &d.p[i][1]);
-                        F.dtmp[qi * nfam + fi] = d;
-                        F.dflag[qi * nfam + fi] = 1;
+                        F.dtmp[qi * nfam + fam] = d;
+                        F.dflag[qi * nfam + fam] = 1;
                     }
                 }
             }
