@@ -430,15 +430,20 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
 
 // ------------------------------------------------------------ corners
 
+// The six cumulative moment arrays of a cluster (mx, my, mxx, mxy, myy, w),
+// cluster-relative: array q starts at b + q * stride.
 struct Mom {
-    const double *a[6];  // mx, my, mxx, mxy, myy, w (cluster-relative)
+    const double *b;
+    int64_t stride;
+    __device__ __forceinline__ const double *a(int q) const { return b + q * stride; }
 };
 
 // The window sums of [i0, i1], wrapping, as fit_line forms them
 // (fast/quads.c:539-572).
 __device__ __forceinline__ void range_sums(const Mom &M, int64_t sz, int64_t i0, int64_t i1, double s[6]) {
+    #pragma unroll
     for (int q = 0; q < 6; q++) {
-        const double *v = M.a[q];
+        const double *v = M.a(q);
         if (i0 < i1) {
             double x = v[i1];
             if (i0 > 0) x = __dsub_rn(x, v[i0 - 1]);
@@ -450,7 +455,7 @@ __device__ __forceinline__ void range_sums(const Mom &M, int64_t sz, int64_t i0,
 }
 
 // fit_line (fast/quads.c:539-600) from the sums: line (ex, ey, nx, ny), mse.
-__device__ void line_from_sums(const double s[6], double lp[4], double *mse) {
+__device__ __forceinline__ void line_from_sums(const double s[6], double lp[4], double *mse) {
     double w = s[5];
     double ex = __ddiv_rn(s[0], w), ey = __ddiv_rn(s[1], w);
     double cxx = __dsub_rn(__ddiv_rn(s[2], w), __dmul_rn(ex, ex));
@@ -511,6 +516,10 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
     __shared__ double s_thr;
     __shared__ double s_wbest[4];
     __shared__ uint32_t s_word[4];
+    __shared__ int64_t s_idx[4];
+    __shared__ double s_lines[4][4];
+    __shared__ float s_qp[4][2];
+    __shared__ int s_ok, s_lok[4];
     static_assert(3 * kLdsPts <= 4 * kMaxNm * kMaxNm, "errors, filtered errors and maxima fit the shared area");
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const double max_mse = (double)P->max_mse, max_dot = (double)P->cos_crit;
@@ -529,8 +538,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         if (!(flag & 1u)) continue;
         const uint32_t s0 = F.cstart[c];
         const int64_t sz = F.cstart[c + 1] - s0;
-        Mom M;
-        for (int q = 0; q < 6; q++) M.a[q] = F.mom + q * pc + s0;
+        const Mom M = {F.mom + s0, pc};
         int64_t ksz = min((int64_t)20, sz / 12);
         if (ksz < 2) continue;
         double nn = (double)(2 * ksz + 1);
@@ -541,8 +549,9 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         for (int64_t i = t; i < sz; i += 256) {
             int64_t i0 = i - ksz, i1 = i + ksz;
             double s[6];
+            #pragma unroll
             for (int q = 0; q < 6; q++) {
-                const double *v = M.a[q];
+                const double *v = M.a(q);
                 if (i0 < 0) s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 + sz - 1]), v[i1]);
                 else if (i1 >= sz) s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 - 1]), v[i1 - sz]);
                 else if (i0 > 0) s[q] = __dsub_rn(v[i1], v[i0 - 1]);
@@ -729,35 +738,47 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                     o = s_word[k];
                 }
             }
-            bool ok = b < INF && __ddiv_rn(b, (double)sz) < max_mse;
-            int64_t idx[4] = {0, 0, 0, 0};
-            if (ok) {
+            const bool ok0 = b < INF && __ddiv_rn(b, (double)sz) < max_mse;
+            s_ok = ok0 ? 1 : 0;
+            if (ok0) {
                 uint32_t m3 = o % nm, tri = o / nm;
                 uint32_t m0 = tri / (nm * nm), m1 = (tri / nm) % nm, m2 = tri % nm;
-                idx[0] = mx[m0];
-                idx[1] = mx[m1];
-                idx[2] = mx[m2];
-                idx[3] = mx[m3];
+                s_idx[0] = mx[m0];
+                s_idx[1] = mx[m1];
+                s_idx[2] = mx[m2];
+                s_idx[3] = mx[m3];
             }
-            double lines[4][4];
-            for (int i = 0; ok && i < 4; i++) {
-                double s[6], mse;
-                range_sums(M, sz, idx[i], idx[(i + 1) & 3], s);
-                line_from_sums(s, lines[i], &mse);
-                if (mse > max_mse) ok = false;
-            }
-            QuadRec q;
+        }
+        __syncthreads();
+        if (!s_ok) continue;
+        // the four lines (fast/quads.c:1058-1075), one thread each
+        if (t < 4) {
+            double s[6], lp[4], mse;
+            range_sums(M, sz, s_idx[t], s_idx[(t + 1) & 3], s);
+            line_from_sums(s, lp, &mse);
+            for (int k = 0; k < 4; k++) s_lines[t][k] = lp[k];
+            s_lok[t] = mse > max_mse ? 0 : 1;
+        }
+        __syncthreads();
+        if (t == 0) {
+            bool ok = s_lok[0] && s_lok[1] && s_lok[2] && s_lok[3];
             for (int i = 0; ok && i < 4; i++) {
                 int j = (i + 1) & 3;
-                double a00 = lines[i][3], a01 = -lines[j][3], a10 = -lines[i][2], a11 = lines[j][2];
-                double b0 = __dadd_rn(-lines[i][0], lines[j][0]), b1 = __dadd_rn(-lines[i][1], lines[j][1]);
+                double a00 = s_lines[i][3], a01 = -s_lines[j][3], a10 = -s_lines[i][2], a11 = s_lines[j][2];
+                double b0 = __dadd_rn(-s_lines[i][0], s_lines[j][0]), b1 = __dadd_rn(-s_lines[i][1], s_lines[j][1]);
                 double det = __dsub_rn(__dmul_rn(a00, a11), __dmul_rn(a10, a01));
                 if (fabs(det) < 0.001) { ok = false; break; }
                 double w00 = __ddiv_rn(a11, det), w01 = __ddiv_rn(-a01, det);
                 double l0 = __dadd_rn(__dmul_rn(w00, b0), __dmul_rn(w01, b1));
-                q.p[i][0] = (float)__dadd_rn(lines[i][0], __dmul_rn(l0, a00));
-                q.p[i][1] = (float)__dadd_rn(lines[i][1], __dmul_rn(l0, a10));
+                s_qp[i][0] = (float)__dadd_rn(s_lines[i][0], __dmul_rn(l0, a00));
+                s_qp[i][1] = (float)__dadd_rn(s_lines[i][1], __dmul_rn(l0, a10));
             }
+            QuadRec q;
+            if (ok)
+                for (int i = 0; i < 4; i++) {
+                    q.p[i][0] = s_qp[i][0];
+                    q.p[i][1] = s_qp[i][1];
+                }
             if (ok) {
                 // area (two Heron triangles) and convexity (:1094-1124);
                 // coordinate differences in float
@@ -766,7 +787,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                 double len[3], area = 0.0;
                 for (int i = 0; i < 3; i++) {
                     int ia = i, ib = (i + 1) % 3;
-                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(q.p[ib][0], q.p[ia][0])), sq(fd(q.p[ib][1], q.p[ia][1]))));
+                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(s_qp[ib][0], s_qp[ia][0])), sq(fd(s_qp[ib][1], s_qp[ia][1]))));
                 }
                 double pp = __ddiv_rn(__dadd_rn(__dadd_rn(len[0], len[1]), len[2]), 2.0);
                 area = __dadd_rn(area, __dsqrt_rn(__dmul_rn(__dmul_rn(__dmul_rn(pp, __dsub_rn(pp, len[0])),
@@ -775,7 +796,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                 const int idxs[4] = {2, 3, 0, 2};
                 for (int i = 0; i < 3; i++) {
                     int ia = idxs[i], ib = idxs[i + 1];
-                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(q.p[ib][0], q.p[ia][0])), sq(fd(q.p[ib][1], q.p[ia][1]))));
+                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(s_qp[ib][0], s_qp[ia][0])), sq(fd(s_qp[ib][1], s_qp[ia][1]))));
                 }
                 pp = __ddiv_rn(__dadd_rn(__dadd_rn(len[0], len[1]), len[2]), 2.0);
                 area = __dadd_rn(area, __dsqrt_rn(__dmul_rn(__dmul_rn(__dmul_rn(pp, __dsub_rn(pp, len[0])),
@@ -786,8 +807,8 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                 double ccr = (double)P->cos_crit;
                 for (int i = 0; ok && i < 4; i++) {
                     int i0 = i, i1 = (i + 1) & 3, i2 = (i + 2) & 3;
-                    double dx1 = fd(q.p[i1][0], q.p[i0][0]), dy1 = fd(q.p[i1][1], q.p[i0][1]);
-                    double dx2 = fd(q.p[i2][0], q.p[i1][0]), dy2 = fd(q.p[i2][1], q.p[i1][1]);
+                    double dx1 = fd(s_qp[i1][0], s_qp[i0][0]), dy1 = fd(s_qp[i1][1], s_qp[i0][1]);
+                    double dx2 = fd(s_qp[i2][0], s_qp[i1][0]), dy2 = fd(s_qp[i2][1], s_qp[i1][1]);
                     double den = __dsqrt_rn(__dmul_rn(__dadd_rn(__dmul_rn(dx1, dx1), __dmul_rn(dy1, dy1)),
                                                       __dadd_rn(__dmul_rn(dx2, dx2), __dmul_rn(dy2, dy2))));
                     if (den == 0.0) { ok = false; break; }
