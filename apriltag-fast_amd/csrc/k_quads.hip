@@ -91,7 +91,7 @@ __device__ __forceinline__ void pt_unpack(uint64_t p, int *x, int *y, int *gx, i
 
 constexpr int kDotChunk = 4096;
 
-__global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const DevParams *P) {
+__global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const DevParams *P, int items) {
     __shared__ float terms[kDotChunk];
     __shared__ uint32_t red[4][4];
     __shared__ float lanes8[8];
@@ -135,9 +135,11 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
                 slope = point_slope(__fsub_rn((float)x, cx), __fsub_rn((float)y, cy));
             }
             F.okey[s0 + i] = ((uint64_t)slope << 32) | (uint32_t)~i;
-            uint32_t pos = s0 + (n - 1 - i);
-            F.skey[pos] = ((uint64_t)c << 32) | slope;
-            F.sval[pos] = i;
+            if (items) {  // for the sort of all points at once
+                uint32_t pos = s0 + (n - 1 - i);
+                F.skey[pos] = ((uint64_t)c << 32) | slope;
+                F.sval[pos] = i;
+            }
         }
         if (!ok) {
             if (t == 0) F.cflag[c] = 0;
@@ -324,13 +326,56 @@ __global__ __launch_bounds__(256) void k_work_offsets(const Frame *frames, int n
     }
 }
 
+// ------------------------------------------------------------ angle sort
+
+// Persistent over list l of kept clusters: the full keys of a cluster
+// (slope << 32 | ~index, written by k_fit_prep in point order) sorted
+// ascending in LDS by a bitonic network. The keys are distinct, so any
+// sort gives the full-key order the reference's sort starts from
+// (fast/quads.c:379-427); its tie fix follows in k_fit_moments. The
+// clusters of list l have at most `cap` points (a power of two).
+__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q, int l) {
+    extern __shared__ uint64_t sk[];
+    const int t = threadIdx.x;
+    const uint32_t total = Q->total[l];
+    const uint32_t *off = Q->off[l];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const uint32_t c = F.klist[(l ? F.kcap : 0u) + (g - off[fi])];
+        const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
+        uint32_t m = 1;
+        while (m < n) m <<= 1;
+        for (uint32_t i = t; i < m; i += 256) sk[i] = i < n ? F.okey[s0 + i] : ~0ull;
+        __syncthreads();
+        for (uint32_t k = 2; k <= m; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = t; i < m; i += 256) {
+                    const uint32_t p = i ^ j;
+                    if (p > i) {
+                        const uint64_t a = sk[i], b = sk[p];
+                        if ((a > b) == ((i & k) == 0)) {
+                            sk[i] = b;
+                            sk[p] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t i = t; i < n; i += 256) F.skey[s0 + i] = sk[i];
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ moments
 
 constexpr int kMomChunk = 512;
 
 // Persistent: the kept clusters of every frame, large ones first, block b
 // taking global indices b, b + G, ... (a frame cursor per list).
-__global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q) {
+__global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
+                                                    int keys_ready) {
     __shared__ double sm[6][kMomChunk];
     const int t = threadIdx.x;
     for (int l = 0; l < 2; l++) {
@@ -341,14 +386,18 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
         while (off[fi + 1] <= g) fi++;
         const Frame &F = frames[fi];
         const uint32_t c = F.klist[(l ? F.kcap : 0u) + (g - off[fi])];
-        uint64_t *skey = parity ? F.skey2 : F.skey;
+        uint64_t *skey = parity && !keys_ready ? F.skey2 : F.skey;
         const uint32_t *sval = parity ? F.sval2 : F.sval;
         const int64_t pc = F.pcap;
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         uint64_t *keys = skey + s0;
-        // the full keys in order: slope << 32 | ~index
-        for (uint32_t k = t; k < n; k += 256) keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
-        __syncthreads();
+        if (!keys_ready) {
+            // the full keys in order: slope << 32 | ~index, from the sorted
+            // (cluster << 32 | slope, index) items
+            for (uint32_t k = t; k < n; k += 256)
+                keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
+            __syncthreads();
+        }
         // fix_runs (fast/quads.c:304-327) reorders each maximal run of equal
         // slopes holding two neighbours less than 5 apart; the runs are
         // disjoint and each reads only its own keys, so the thread of the
@@ -848,20 +897,41 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity);
 
+// The angle sort's LDS capacity for the large clusters: the power of two
+// at or above the largest cluster of the call (max_px), or 0 when that
+// exceeds the per-kernel LDS limit (the sort of all points in HBM then).
+int angle_sort_cap(int max_cluster, bool force_global) {
+    if (force_global) return 0;
+    int m = kLargeCluster;
+    while (m < max_cluster) m <<= 1;
+    if ((size_t)m * 8 > 128 * 1024) return 0;
+    static int attr = -1;
+    if (attr < 0)
+        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_angle_sort),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024) == hipSuccess;
+    if ((size_t)m * 8 > 64 * 1024 && !attr) return 0;
+    return m;
+}
+
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks, WorkQueue *Q, Marks *mk) {
+                        int fit_blocks, int sort_cap, WorkQueue *Q, Marks *mk) {
     // k_fit_prep: every cluster of every frame, a grid per frame; the fit
     // kernels after it: persistent, fit_blocks blocks over the work lists
     dim3 g(256, nframes), gp(fit_blocks);
-    hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams);
+    hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 0, (int)kNLarge);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 1, (int)kNSmall);
     mk->mark(st, "fit_prep");
     int parity = 0;
-    launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);
+    if (sort_cap) {
+        hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), (size_t)sort_cap * 8, st, dframes, Q, 0);
+        hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), (size_t)kLargeCluster * 8, st, dframes, Q, 1);
+    } else {
+        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);
+    }
     mk->mark(st, "angle_sort");
-    hipLaunchKernelGGL(k_fit_moments, gp, dim3(256), 0, st, dframes, parity, Q);
+    hipLaunchKernelGGL(k_fit_moments, gp, dim3(256), 0, st, dframes, parity, Q, sort_cap ? 1 : 0);
     mk->mark(st, "moments");
     hipLaunchKernelGGL(k_fit_corners, gp, dim3(256), 0, st, dframes, dparams, Q);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);

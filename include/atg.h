@@ -144,9 +144,9 @@ int atg_set_graphs(atg_ctx *ctx, int on);
  * that HBM traffic counters are scaled by. */
 int atg_debug_copy(atg_ctx *ctx, void *dst, const void *src, int64_t bytes, double *ms);
 
-/* on != 0: the following calls keep the union-find replay's state in HBM
- * scratch, the path frames too wide for LDS take, so that tests can check
- * it on any frame. */
+/* on != 0: the following calls take the paths of frames too large for
+ * LDS (the union-find replay's state in HBM scratch, the angle sort of all
+ * points at once in HBM), so that tests can check them on any frame. */
 int atg_debug_force_scratch(atg_ctx *ctx, int on);
 
 /* Device time (ms, HIP events on the context's stream) of the last
