@@ -148,7 +148,8 @@ struct Tune {
 // HBM scratch).
 struct CclPlan {
     int front, replay;          // LDS bytes, 0 = scratch
-    int replay_waves;           // waves per frame of k_replay
+    int replay_cap;             // runs per row replayed in LDS, 0 = none
+    int replay_waves;           // waves per frame of k_replay_hbm
     size_t scratch_front, scratch_replay;  // HBM bytes per frame
 };
 

@@ -384,16 +384,50 @@ __global__ __launch_bounds__(T) void k_front(const Frame *frames, int lds_bytes,
 // weights are the ones the sequential replay has there (a weight is a
 // pixel count: step 1 knows it). So row y can be replayed on its own: the
 // elements are those records (weights from step 1, roots unknown: named
-// by slot, kSym | slot), the current runs and the virtual node. Links
-// where one current run meets one record and nothing else links either
-// are independent and resolved by all lanes; lane 0 takes the rest in
-// order (fast/unionfind.c:15-45, 112-146), the last-column link last.
+// by slot, kSym | slot), the current runs and the virtual node. A link
+// only ever touches the sets of one row-y record (the group of its
+// current run, rlab from step 1), so the groups are independent: each
+// lane replays the links of one group in order (fast/unionfind.c:15-45,
+// 112-146), and the last-column link comes last.
 // Output: parent[] of the row's runs and virtual node (a node or a
 // record slot), rlost[slot] of each previous record (the winner it lost
 // to, or kLostNone / kLostRetired), rroot[slot] of each row-y record (the
 // root of its set as replayed: a node or kSym | a previous record's slot).
+//
+// Rows of at most C runs (each of the two rows) and 2C links replay in
+// LDS (k_replay, a wave per row); other rows, and all rows when asked,
+// in HBM scratch sized for the row width (k_replay_hbm).
 
-size_t replay_bytes(int qw) { return (size_t)34 * qw + 128; }
+// Byte offsets of a row's replay memory for C runs, L links and ix-byte
+// link indices.
+struct ReplayLayout {
+    size_t le, nxt, head, labA, hasl, total;
+};
+__host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix) {
+    ReplayLayout r;
+    size_t o = (size_t)8 * (2 * C + 2);  // ent: [2C + 2] x (parent, weight)
+    r.le = o;
+    o += (size_t)4 * L;                  // link: current run << 16 | record
+    r.nxt = o;
+    o = (o + (size_t)ix * L + 3) & ~(size_t)3;  // next link of the same group
+    r.head = o;
+    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // first link of each group
+    r.labA = o;
+    o = (o + (size_t)2 * C + 3) & ~(size_t)3;   // record of each previous-row run
+    r.hasl = o;
+    o = (o + (size_t)C + 15) & ~(size_t)15;     // record has a link
+    r.total = o;
+    return r;
+}
+
+// LDS capacity (runs per row) for frames up to max_qw wide.
+int replay_cap(int max_qw) { return std::min(512, (std::max(max_qw, 1) + 63) & ~63); }
+size_t replay_lds_bytes(int cap) { return replay_layout(cap, 2 * cap, 2).total; }
+size_t replay_hbm_bytes(int qw) { return replay_layout(qw, 3 * qw, 4).total; }
+
+__device__ __forceinline__ bool replay_fits(const Frame &F, int y, int cap) {
+    return (int)F.rcnt[y - 1] <= cap && (int)F.rcnt[y] <= cap && (int)F.ecnt[y] <= 2 * cap;
+}
 
 template <bool G>
 __device__ __forceinline__ uint2 rp_ent(typename Mem<G>::U32 *ent, uint32_t x) {
@@ -426,26 +460,34 @@ __device__ __forceinline__ uint32_t rp_find_ro(typename Mem<G>::U32 *ent, uint32
     return x;
 }
 
+// Replay of row y with capacity C runs and L links (the row fits).
 template <bool G>
-__device__ void replay_row(const Frame &F, int y, uint8_t *base) {
+__device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     using U32 = typename Mem<G>::U32;
     using U16 = typename Mem<G>::U16;
+    using IX = typename std::conditional<G, volatile uint32_t, uint16_t>::type;
+    using U8 = typename std::conditional<G, volatile uint8_t, uint8_t>::type;
+    constexpr uint32_t kNone = G ? 0xffffffffu : 0xffffu;
     const int lane = threadIdx.x;
     const int QW = F.qw;
-    U32 *ent = reinterpret_cast<U32 *>(base);      // [2 QW + 2][2]: parent, weight
-    U32 *lcnt = ent + 2 * (2 * QW + 2);             // [QW] links into each record
-    U32 *le = lcnt + QW;                            // [3 QW] links: cur << 16 | record, bit 31 once done
-    U16 *labA = reinterpret_cast<U16 *>(le + 3 * QW);  // [QW] record of each previous-row run
+    const ReplayLayout ly = replay_layout(C, L, G ? 4 : 2);
+    U32 *ent = reinterpret_cast<U32 *>(base);
+    U32 *le = reinterpret_cast<U32 *>(base + ly.le);
+    IX *nxt = reinterpret_cast<IX *>(base + ly.nxt);
+    IX *head = reinterpret_cast<IX *>(base + ly.head);
+    U16 *labA = reinterpret_cast<U16 *>(base + ly.labA);
+    U8 *hasl = reinterpret_cast<U8 *>(base + ly.hasl);
     const uint32_t pb = F.row_off[y - 1], cb = F.row_off[y], vb = F.row_off[F.qh];
     const int m = (int)F.nrec[y - 1], nprev = (int)F.rcnt[y - 1];
-    const int ncur = (int)F.rcnt[y], ne = (int)F.ecnt[y];
+    const int ncur = (int)F.rcnt[y], ne = (int)F.ecnt[y], ng = (int)F.nrec[y];
     const uint32_t *cur = F.runs + (int64_t)y * QW;
     const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
+    const uint32_t *rlab_cur = F.rlab + cb;
     const int ev = m + ncur;
     for (int l = lane; l < m; l += 64) {
         ent[2 * l] = (uint32_t)l;
         ent[2 * l + 1] = F.recw[pb + l];
-        lcnt[l] = 0;
+        hasl[l] = 0;
     }
     for (int i = lane; i < ncur; i += 64) {
         uint32_t r = cur[i];
@@ -453,6 +495,7 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base) {
         ent[2 * (m + i) + 1] = run_end(r) - run_start(r) + 1;
     }
     for (int k = lane; k < nprev; k += 64) labA[k] = (uint16_t)F.rlab[pb + k];
+    for (int g = lane; g < ng; g += 64) head[g] = kNone;
     if (lane == 0) {
         ent[2 * ev] = (uint32_t)ev;
         ent[2 * ev + 1] = 1;
@@ -464,56 +507,58 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base) {
         lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
     }
     wsync<G>();
+    // links with the previous run's record; nxt[e] holds the link's group
+    // until the next pass replaces it
     for (int e = lane; e < ne; e += 64) {
         uint32_t ed = edges[e];
         uint32_t l = labA[ed & 0xffffu];
         le[e] = (ed & 0xffff0000u) | l;
-        atomicAdd((uint32_t *)&lcnt[l], 1u);
+        hasl[l] = 1;
+        nxt[e] = rlab_cur[ed >> 16];
     }
     wsync<G>();
-    // one current run meeting one record, nothing else linking either: an
-    // independent link (links are sorted by current run, so the run's only
-    // link has different neighbours)
-    for (int e = lane; e < ne; e += 64) {
-        uint32_t ed = le[e];
-        uint32_t i = (ed >> 16) & 0x7fffu, l = ed & 0xffffu;
-        bool only_i = (e == 0 || ((le[e - 1] >> 16) & 0x7fffu) != i) &&
-                      (e == ne - 1 || ((le[e + 1] >> 16) & 0x7fffu) != i);
-        if (!only_i || lcnt[l] != 1u) continue;
-        uint32_t a = (uint32_t)m + i;
-        uint32_t wa = ent[2 * a + 1], wb = ent[2 * l + 1];
-        if (wa > wb) {
-            ent[2 * l] = a;
-            ent[2 * a + 1] = wa + wb;
-        } else {
-            ent[2 * a] = l;
-            ent[2 * l + 1] = wa + wb;
+    // per group, its links in order as a list: chunks of 64 links from the
+    // last, the lanes of each group in a chunk found by ballot
+    for (int b0 = (ne - 1) & ~63; b0 >= 0; b0 -= 64) {
+        const int e = b0 + lane;
+        const uint32_t g = e < ne ? (uint32_t)nxt[e] : 0xffffffffu;
+        uint64_t todo = __ballot(e < ne);
+        while (todo) {
+            const int lead = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t gl = __builtin_amdgcn_readlane(g, lead);
+            const uint64_t same = __ballot(g == gl);
+            todo &= ~same;
+            if (g == gl) {
+                const uint64_t hi = lane == 63 ? 0ull : same >> (lane + 1);
+                nxt[e] = hi ? (uint32_t)(e + __ffsll((unsigned long long)hi)) : (uint32_t)head[gl];
+            }
+            if (lane == lead) head[gl] = (uint32_t)e;
         }
-        le[e] = ed | 0x80000000u;
     }
     wsync<G>();
-    // the other links in order on lane 0; the current run's set is kept in
-    // registers across its links
-    if (lane == 0) {
+    // each lane replays its groups' links in order; the current run's set
+    // is kept in registers across its links, the next link is read ahead
+    for (int g = lane; g < ng; g += 64) {
+        uint32_t e = head[g];
+        if (e == kNone) continue;
         int cur_i = -1;
         uint32_t ra = 0, wa = 0;
-        for (int e0 = 0; e0 < ne; e0 += 8) {
-            uint32_t eb[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) eb[q] = e0 + q < ne ? le[e0 + q] : 0x80000000u;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                uint32_t ed = eb[q];
-                if (ed & 0x80000000u) continue;
-                uint32_t i = ed >> 16, l = ed & 0xffffu;
-                if ((int)i != cur_i) {
-                    if (cur_i >= 0) ent[2 * ra + 1] = wa;
-                    cur_i = (int)i;
-                    ra = (uint32_t)m + i;  // a current run's first link: a fresh root
-                    wa = ent[2 * ra + 1];
-                }
-                uint2 b = rp_find<G>(ent, l);
-                if (b.x == ra) continue;
+        uint32_t ed = le[e], en = nxt[e];
+        for (;;) {
+            uint32_t edn = 0, enn = kNone;
+            if (en != kNone) {
+                edn = le[en];
+                enn = nxt[en];
+            }
+            const uint32_t i = ed >> 16, l = ed & 0xffffu;
+            if ((int)i != cur_i) {
+                if (cur_i >= 0) ent[2 * ra + 1] = wa;
+                cur_i = (int)i;
+                ra = (uint32_t)m + i;  // a current run's first link: a fresh root
+                wa = ent[2 * ra + 1];
+            }
+            uint2 b = rp_find<G>(ent, l);
+            if (b.x != ra) {
                 if (wa > b.y) {
                     ent[2 * b.x] = ra;
                 } else {
@@ -522,17 +567,21 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base) {
                 }
                 wa += b.y;
             }
+            if (en == kNone) break;
+            ed = edn;
+            en = enn;
         }
-        if (cur_i >= 0) ent[2 * ra + 1] = wa;
-        if (lastcol) {  // fast/unionfind.c:93-97, last in order
-            uint2 a = rp_find<G>(ent, (uint32_t)(ev - 1));
-            if (a.y > 1u) {
-                ent[2 * ev] = a.x;
-                ent[2 * a.x + 1] = a.y + 1;
-            } else {
-                ent[2 * a.x] = (uint32_t)ev;
-                ent[2 * ev + 1] = a.y + 1;
-            }
+        ent[2 * ra + 1] = wa;
+    }
+    wsync<G>();
+    if (lane == 0 && lastcol) {  // fast/unionfind.c:93-97, last in order
+        uint2 a = rp_find<G>(ent, (uint32_t)(ev - 1));
+        if (a.y > 1u) {
+            ent[2 * ev] = a.x;
+            ent[2 * a.x + 1] = a.y + 1;
+        } else {
+            ent[2 * a.x] = (uint32_t)ev;
+            ent[2 * ev + 1] = a.y + 1;
         }
     }
     wsync<G>();
@@ -543,24 +592,37 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base) {
     };
     for (int l = lane; l < m; l += 64) {
         uint32_t q = ent[2 * l];
-        F.rlost[pb + l] = q == (uint32_t)l ? (lcnt[l] ? kLostNone : kLostRetired) : name(q);
+        F.rlost[pb + l] = q == (uint32_t)l ? (hasl[l] ? kLostNone : kLostRetired) : name(q);
     }
     for (int x = m + lane; x <= ev; x += 64) F.parent[name((uint32_t)x)] = name(ent[2 * x]);
-    for (int i = lane; i < ncur; i += 64)
-        F.rroot[cb + F.rlab[cb + i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
+    for (int i = lane; i < ncur; i += 64) F.rroot[cb + rlab_cur[i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
 }
 
-__global__ __launch_bounds__(64) void k_replay(const Frame *frames, int lds_bytes, uint8_t *gscratch,
-                                               int64_t fstride, int64_t wstride) {
+// A wave per row, the rows that fit C = cap runs.
+__global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const Frame F = frames[blockIdx.y];
-    for (int y = 1 + (int)blockIdx.x; y < F.qh; y += (int)gridDim.x) {
-        if (lds_bytes > 0) {
-            replay_row<false>(F, y, lds);
-        } else {
-            replay_row<true>(F, y, gscratch + (int64_t)blockIdx.y * fstride + (int64_t)blockIdx.x * wstride);
+    const Frame &F = frames[blockIdx.y];
+    const int y = 1 + (int)blockIdx.x;
+    if (y >= F.qh || !replay_fits(F, y, cap)) return;
+    replay_row<false>(F, y, lds, cap, 2 * cap);
+}
+
+// The other rows (all rows for cap 0): wave b takes rows 1 + b + k W,
+// 64 at a time checked by the lanes, in its own HBM scratch.
+__global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap, uint8_t *gscratch, int64_t fstride,
+                                                   int64_t wstride) {
+    const Frame &F = frames[blockIdx.y];
+    const int lane = threadIdx.x, W = (int)gridDim.x, b = (int)blockIdx.x;
+    uint8_t *mem = gscratch + (int64_t)blockIdx.y * fstride + (int64_t)b * wstride;
+    for (int k0 = 0; 1 + b + W * k0 < F.qh; k0 += 64) {
+        const int y = 1 + b + W * (k0 + lane);
+        uint64_t todo = __ballot(y < F.qh && (cap == 0 || !replay_fits(F, y, cap)));
+        while (todo) {
+            const int j = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            replay_row<true>(F, 1 + b + W * (k0 + j), mem, F.qw, 3 * F.qw);
+            __syncthreads();  // the next row reuses the memory
         }
-        __syncthreads();  // the next row reuses the memory
     }
 }
 
@@ -667,16 +729,19 @@ static bool lds_ok(const void *fn, size_t need, bool force_scratch) {
 
 CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch) {
     CclPlan c{};
-    size_t f = front_bytes(max_qw), r = replay_bytes(max_qw);
+    size_t f = front_bytes(max_qw);
     bool fok = lds_ok(reinterpret_cast<const void *>(&k_front<128>), f, force_scratch) &&
                lds_ok(reinterpret_cast<const void *>(&k_front<256>), f, force_scratch) &&
                lds_ok(reinterpret_cast<const void *>(&k_front<512>), f, force_scratch);
     c.front = fok ? (int)f : 0;
-    c.replay = lds_ok(reinterpret_cast<const void *>(&k_replay), r, force_scratch) ? (int)r : 0;
-    c.replay_waves = c.replay ? std::max(1, max_qh - 1) : std::min(std::max(1, max_qh - 1), 64);
+    // rows of up to replay_cap runs in LDS, the others (all when forced) in
+    // HBM: 16 waves per frame for the odd wide row, 64 when they are all
+    c.replay_cap = force_scratch ? 0 : replay_cap(max_qw);
+    c.replay = c.replay_cap ? (int)replay_lds_bytes(c.replay_cap) : 0;
+    c.replay_waves = std::min(std::max(1, max_qh - 1), c.replay_cap ? 16 : 64);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     c.scratch_front = c.front ? 0 : al(f);
-    c.scratch_replay = c.replay ? 0 : al(r) * (size_t)c.replay_waves;
+    c.scratch_replay = al(replay_hbm_bytes(max_qw)) * (size_t)c.replay_waves;
     return c;
 }
 
@@ -693,7 +758,7 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     // scratch per frame: [front][replay waves], frames back to back
     int64_t per = (int64_t)ccl_scratch_bytes(c);
     uint8_t *s_front = scratch, *s_replay = scratch + c.scratch_front;
-    int64_t rstride = c.replay ? 0 : (int64_t)(c.scratch_replay / (size_t)c.replay_waves);
+    int64_t rstride = (int64_t)(c.scratch_replay / (size_t)c.replay_waves);
     switch (tu.front_threads) {
     case 128:
         hipLaunchKernelGGL(k_front<128>, dim3(1, nframes), dim3(128), (size_t)c.front, st, dframes, c.front, s_front,
@@ -708,8 +773,11 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
                            per);
     }
     mk->mark(st, "uf_front");
-    hipLaunchKernelGGL(k_replay, dim3((unsigned)c.replay_waves, nframes), dim3(64), (size_t)c.replay, st, dframes,
-                       c.replay, s_replay, per, rstride);
+    if (c.replay_cap)
+        hipLaunchKernelGGL(k_replay, dim3((unsigned)std::max(1, max_qh - 1), nframes), dim3(64), (size_t)c.replay, st,
+                           dframes, c.replay_cap);
+    hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, c.replay_cap,
+                       s_replay, per, rstride);
     mk->mark(st, "uf_replay");
     for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);
     hipLaunchKernelGGL(k_lost, rg, rb, 0, st, dframes);
