@@ -126,7 +126,7 @@ struct Frame {
 // quads. Each frame lists its own items (klist); off[l][f] is where frame
 // f's items start in list l's global index, total[l] its size. Blocks take
 // global indices in strides, without atomics.
-constexpr uint32_t kLargeCluster = 1024;
+constexpr uint32_t kLargeCluster = 512;
 struct WorkQueue {
     uint32_t total[4];
     uint32_t nframes, pad[3];

@@ -549,32 +549,74 @@ __device__ __forceinline__ uint64_t dkey(double d) {
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
-constexpr int kMaxNm = 32;
-constexpr int kLdsPts = 1024;  // clusters up to this size keep their errors in LDS
+// The triples m0 < m1 < m2 in colex order (by m2, then m1, then m0): the
+// first C(k, 3) of them are those with m2 < k. Packed m0 | m1 << 5 | m2 << 10.
+struct TriTable {
+    uint16_t v[4495];  // C(31, 3): m2 up to 30
+    constexpr TriTable() : v() {
+        int n = 0;
+        for (int m2 = 2; m2 < 31; m2++)
+            for (int m1 = 1; m1 < m2; m1++)
+                for (int m0 = 0; m0 < m1; m0++) v[n++] = (uint16_t)(m0 | m1 << 5 | m2 << 10);
+    }
+};
+__constant__ const TriTable kTri{};
 
-// Persistent, as k_fit_moments.
-__global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
+// Block shapes of the corner search: 256 threads for the large clusters,
+// a wave for the rest (at most kLargeCluster points, all in LDS).
+template <int NT>
+struct CornerShape;
+template <>
+struct CornerShape<256> {
+    static constexpr int kMaxNm = 32;     // maxima kept at most
+    static constexpr int kLdsPts = 1024;  // clusters up to this size keep their errors in LDS
+    static constexpr int kMaxima = 1024;  // maxima values in LDS at most
+};
+template <>
+struct CornerShape<64> {
+    static constexpr int kMaxNm = 16;
+    static constexpr int kLdsPts = (int)kLargeCluster;
+    static constexpr int kMaxima = (int)kLargeCluster / 2;  // strict cyclic maxima of n points: at most n / 2
+};
+
+template <int NT>
+__device__ __forceinline__ uint32_t bscan_t(uint32_t v, uint32_t *total) {
+    if constexpr (NT == 64) {
+        const uint32_t inc = wscan(v);
+        *total = __shfl(inc, 63, 64);
+        return inc - v;
+    } else {
+        return bscan(v, total);
+    }
+}
+
+// Persistent over list l of kept clusters, a cluster per block of NT threads.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q, int l) {
+    constexpr int kMaxNm = CornerShape<NT>::kMaxNm, kLdsPts = CornerShape<NT>::kLdsPts;
+    constexpr int kMaxima = CornerShape<NT>::kMaxima;
+    constexpr int kShared = 2 * kLdsPts + kMaxima > 4 * kMaxNm * kMaxNm ? 2 * kLdsPts + kMaxima : 4 * kMaxNm * kMaxNm;
+    constexpr int kWaves = NT / 64;
     // The errors, filtered errors and maxima values of a cluster (first
     // part) share LDS with the pair-fit tables (second part).
-    __shared__ double sh[4 * kMaxNm * kMaxNm];
-    __shared__ uint32_t lidx[kLdsPts];
+    __shared__ double sh[kShared];
+    __shared__ uint32_t lidx[kMaxima];
     __shared__ uint32_t mx[kMaxNm];
-    __shared__ uint32_t hist[256];
+    __shared__ uint32_t hist[NT == 256 ? 256 : 1];
     __shared__ uint64_t s_prefix;
     __shared__ uint32_t s_rank;
     __shared__ double s_thr;
-    __shared__ double s_wbest[4];
-    __shared__ uint32_t s_word[4];
+    __shared__ double s_wbest[kWaves];
+    __shared__ uint32_t s_word[kWaves];
     __shared__ int64_t s_idx[4];
     __shared__ double s_lines[4][4];
     __shared__ float s_qp[4][2];
     __shared__ int s_ok, s_lok[4];
-    static_assert(3 * kLdsPts <= 4 * kMaxNm * kMaxNm, "errors, filtered errors and maxima fit the shared area");
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const double max_mse = (double)P->max_mse, max_dot = (double)P->cos_crit;
     const double INF = __longlong_as_double(0x7ff0000000000000ll);
     double *pe = sh, *pm = sh + kMaxNm * kMaxNm, *pnx = sh + 2 * kMaxNm * kMaxNm, *pny = sh + 3 * kMaxNm * kMaxNm;
-    for (int l = 0; l < 2; l++) {
+    {
     const uint32_t total = Q->total[l];
     const uint32_t *off = Q->off[l];
     uint32_t fi = 0;
@@ -592,10 +634,14 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         if (ksz < 2) continue;
         double nn = (double)(2 * ksz + 1);
         const bool inl = sz <= kLdsPts;
+        if (NT == 64 && !inl) {  // the list holds no such cluster
+            if (t == 0) atomicOr(F.counters + kError, 4u);
+            continue;
+        }
         double *errs = inl ? sh : F.errs + s0;
         double *y = inl ? sh + kLdsPts : F.filt + s0;
         // window errors (fast/quads.c:654-686 edges, :604-635 middle)
-        for (int64_t i = t; i < sz; i += 256) {
+        for (int64_t i = t; i < sz; i += NT) {
             int64_t i0 = i - ksz, i1 = i + ksz;
             double s[6];
             #pragma unroll
@@ -610,7 +656,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         }
         __syncthreads();
         // 7-tap filter, wrapping (fast/quads.c:706-735)
-        for (int64_t i = t; i < sz; i += 256) {
+        for (int64_t i = t; i < sz; i += NT) {
             double acc;
             if (i >= 3 && i < sz - 3) {
                 acc = __dmul_rn(errs[i - 3], P->qsm_kern[0]);
@@ -631,7 +677,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         // cyclic local maxima, in order (fast/quads.c:805-823)
         uint32_t *midx = inl ? lidx : F.midx + s0;
         uint32_t base = 0;
-        for (int64_t c0 = 0; c0 < sz; c0 += 256) {
+        for (int64_t c0 = 0; c0 < sz; c0 += NT) {
             int64_t i = c0 + t;
             uint32_t f = 0;
             if (i < sz) {
@@ -639,7 +685,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                 f = (e > l && e > r) ? 1u : 0u;
             }
             uint32_t tot;
-            uint32_t pre = bscan(f, &tot);
+            uint32_t pre = bscan_t<NT>(f, &tot);
             if (f) midx[base + pre] = (uint32_t)i;
             base += tot;
         }
@@ -652,12 +698,12 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
             // above it are kept, in order
             const uint32_t rank = (uint32_t)P->max_nmaxima;  // 0-based, descending
             double thr;
-            if (nm <= (uint32_t)kLdsPts) {
+            if (nm <= (uint32_t)kMaxima) {
                 // the rank of each value among the maxima (ties by position)
                 double *mv = sh + 2 * kLdsPts;
-                for (uint32_t i = t; i < nm; i += 256) mv[i] = y[midx[i]];
+                for (uint32_t i = t; i < nm; i += NT) mv[i] = y[midx[i]];
                 __syncthreads();
-                for (uint32_t i = t; i < nm; i += 256) {
+                for (uint32_t i = t; i < nm; i += NT) {
                     const double v = mv[i];
                     uint32_t r = 0;
                     for (uint32_t j = 0; j < nm; j++) {
@@ -668,7 +714,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                 }
                 __syncthreads();
                 thr = s_thr;
-            } else {
+            } else if constexpr (NT == 256) {
                 // radix select over the order-preserving bits
                 uint32_t rk = rank;
                 uint64_t prefix = 0;
@@ -697,11 +743,13 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                     __syncthreads();
                 }
                 thr = __longlong_as_double((long long)((prefix >> 63) ? (prefix ^ 0x8000000000000000ull) : ~prefix));
+            } else {
+                thr = 0.0;  // the wave shape holds at most kMaxima maxima
             }
             // keep the maxima above the threshold (a comparison of doubles,
             // as the reference's), in order
             uint32_t o = 0;
-            for (uint32_t c0 = 0; c0 < nm; c0 += 256) {
+            for (uint32_t c0 = 0; c0 < nm; c0 += NT) {
                 uint32_t i = c0 + t;
                 uint32_t f = 0, v = 0;
                 if (i < nm) {
@@ -709,13 +757,13 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
                     f = y[v] > thr ? 1u : 0u;
                 }
                 uint32_t tot;
-                uint32_t pre = bscan(f, &tot);
+                uint32_t pre = bscan_t<NT>(f, &tot);
                 if (f && o + pre < (uint32_t)kMaxNm) mx[o + pre] = v;
                 o += tot;
             }
             nm = o;
         } else {
-            for (uint32_t i = t; i < nm && i < (uint32_t)kMaxNm; i += 256) mx[i] = midx[i];
+            for (uint32_t i = t; i < nm && i < (uint32_t)kMaxNm; i += NT) mx[i] = midx[i];
         }
         __syncthreads();
         if (nm > (uint32_t)kMaxNm) {
@@ -723,7 +771,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
             continue;
         }
         // pair fits (fast/quads.c:869-928): (a, b) for a < b, closing (b, a) for b >= a + 3
-        for (uint32_t pq = t; pq < nm * nm; pq += 256) {
+        for (uint32_t pq = t; pq < nm * nm; pq += NT) {
             uint32_t p = pq / nm, q = pq % nm;
             bool fwd = p < q, back = q + 3 <= p;
             if (!fwd && !back) continue;
@@ -741,10 +789,13 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         // the first minimum over (m0, m1, m2, m3) in loop order (:930-956)
         double best = INF;
         uint32_t bord = 0xffffffffu;
-        const uint32_t ntri = nm * nm * nm;
-        for (uint32_t tri = t; tri < ntri; tri += 256) {
-            uint32_t m0 = tri / (nm * nm), m1 = (tri / nm) % nm, m2 = tri % nm;
-            if (!(m0 < m1 && m1 < m2 && m2 + 1 < nm)) continue;
+        // the triples with m2 + 1 < nm (an m3 after m2), in any order: the
+        // loop order is in the key
+        const uint32_t ntri = (nm - 1) * (nm - 2) * (nm - 3) / 6;
+        for (uint32_t k3 = t; k3 < ntri; k3 += NT) {
+            const uint32_t pk = kTri.v[k3];
+            const uint32_t m0 = pk & 31u, m1 = (pk >> 5) & 31u, m2 = pk >> 10;
+            const uint32_t tri = (m0 * nm + m1) * nm + m2;
             uint32_t p01 = m0 * nm + m1, p12 = m1 * nm + m2;
             if (pm[p01] > max_mse || pm[p12] > max_mse) continue;
             double dot = __dadd_rn(__dmul_rn(pnx[p01], pnx[p12]), __dmul_rn(pny[p01], pny[p12]));
@@ -780,7 +831,7 @@ __global__ __launch_bounds__(256) void k_fit_corners(const Frame *frames, const 
         if (t == 0) {
             double b = INF;
             uint32_t o = 0xffffffffu;
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < kWaves; k++) {
                 double e = s_wbest[k];
                 if (e < b || (e == b && s_word[k] < o)) {
                     b = e;
@@ -914,7 +965,7 @@ int angle_sort_cap(int max_cluster, bool force_global) {
 }
 
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks, int sort_cap, WorkQueue *Q, Marks *mk) {
+                        int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk) {
     // k_fit_prep: every cluster of every frame, a grid per frame; the fit
     // kernels after it: persistent, fit_blocks blocks over the work lists
     dim3 g(256, nframes), gp(fit_blocks);
@@ -933,7 +984,11 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     mk->mark(st, "angle_sort");
     hipLaunchKernelGGL(k_fit_moments, gp, dim3(256), 0, st, dframes, parity, Q, sort_cap ? 1 : 0);
     mk->mark(st, "moments");
-    hipLaunchKernelGGL(k_fit_corners, gp, dim3(256), 0, st, dframes, dparams, Q);
+    hipLaunchKernelGGL(k_fit_corners<256>, gp, dim3(256), 0, st, dframes, dparams, Q, 0);
+    if (wave_corners)
+        hipLaunchKernelGGL(k_fit_corners<64>, dim3(4 * fit_blocks), dim3(64), 0, st, dframes, dparams, Q, 1);
+    else
+        hipLaunchKernelGGL(k_fit_corners<256>, gp, dim3(256), 0, st, dframes, dparams, Q, 1);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 2, (int)kNQuads);
     mk->mark(st, "corners");

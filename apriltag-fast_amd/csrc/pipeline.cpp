@@ -30,7 +30,7 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
 void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
                          Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks, int sort_cap, WorkQueue *Q, Marks *mk);
+                        int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk);
 int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          WorkQueue *Q, Marks *mk);
@@ -583,6 +583,9 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     int max_cluster = 0;
     for (int32_t i = 0; i < n; i++) max_cluster = std::max(max_cluster, ctx->hframes[(size_t)i].max_px);
     const int sort_cap = angle_sort_cap(max_cluster, ctx->force_scratch);
+    // the corner search of the small clusters a wave each, when its pair
+    // tables hold the maxima kept
+    const bool wave_corners = ctx->hp.max_nmaxima <= 16;
     const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
     auto issue = [&]() -> hipError_t {
         for (const Copy &c : copies) {
@@ -594,7 +597,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, blur, deglitch, tu, mk);
         launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, gscratch, tu, mk);
         launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
-        launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, sort_cap, dq, mk);
+        launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
         launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, dq, mk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -607,7 +610,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
                                     ccl.replay_cap, ccl.replay_waves,        (int64_t)ccl.scratch_front, (int64_t)ccl.scratch_replay,
-                                    tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap,
+                                    tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
                                     (int64_t)(intptr_t)ctx->dparams, (int64_t)(intptr_t)gscratch};
