@@ -368,17 +368,92 @@ __global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const W
     }
 }
 
+// A wave's bitonic sort of 64 R keys in registers: key i = r * 64 + lane
+// sits in v[r]; partners within a register row by shuffles, across rows in
+// the lane's own registers. Ascending.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[R]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64 * R; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int jr = j >> 6;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (r & jr) continue;
+                    const int r2 = r | jr;
+                    const bool up = ((r << 6) & k) == 0;
+                    const uint64_t a = v[r], b = v[r2];
+                    const bool sw = (a > b) == up;
+                    v[r] = sw ? b : a;
+                    v[r2] = sw ? a : b;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const uint64_t o = __shfl_xor(v[r], j, 64);
+                    const bool up = (((r << 6) | lane) & k) == 0;
+                    const bool keep_min = ((lane & j) == 0) == up;
+                    v[r] = keep_min ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
+                }
+            }
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void wave_sort_keys(const uint64_t *in, uint64_t *out, uint32_t n) {
+    const int lane = threadIdx.x & 63;
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)(r * 64 + lane);
+        v[r] = i < n ? in[i] : ~0ull;
+    }
+    wave_bitonic<R>(v);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)(r * 64 + lane);
+        if (i < n) out[i] = v[r];
+    }
+}
+
+// The angle sort of list 1 (at most kLargeCluster points), a wave per
+// cluster, persistent.
+__global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, const WorkQueue *Q) {
+    static_assert(kLargeCluster <= 512, "eight keys a lane at most");
+    const uint32_t total = Q->total[1];
+    const uint32_t *off = Q->off[1];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const uint32_t c = F.klist[F.kcap + (g - off[fi])];
+        const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
+        if (n <= 64)
+            wave_sort_keys<1>(F.okey + s0, F.skey + s0, n);
+        else if (n <= 128)
+            wave_sort_keys<2>(F.okey + s0, F.skey + s0, n);
+        else if (n <= 256)
+            wave_sort_keys<4>(F.okey + s0, F.skey + s0, n);
+        else
+            wave_sort_keys<8>(F.okey + s0, F.skey + s0, n);
+    }
+}
+
 // ------------------------------------------------------------ moments
 
-constexpr int kMomChunk = 512;
-
-// Persistent: the kept clusters of every frame, large ones first, block b
-// taking global indices b, b + G, ... (a frame cursor per list).
-__global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
-                                                    int keys_ready) {
-    __shared__ double sm[6][kMomChunk];
+// Persistent over list l of kept clusters, block b taking global indices
+// b, b + G, ... (a frame cursor): NT threads a cluster, the running sums
+// in chunks of CH points.
+template <int NT, int CH>
+__global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
+                                                   int keys_ready, int l) {
+    __shared__ double sm[6][CH];
     const int t = threadIdx.x;
-    for (int l = 0; l < 2; l++) {
+    {
     const uint32_t total = Q->total[l];
     const uint32_t *off = Q->off[l];
     uint32_t fi = 0;
@@ -394,7 +469,7 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
         if (!keys_ready) {
             // the full keys in order: slope << 32 | ~index, from the sorted
             // (cluster << 32 | slope, index) items
-            for (uint32_t k = t; k < n; k += 256)
+            for (uint32_t k = t; k < n; k += NT)
                 keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
             __syncthreads();
         }
@@ -402,7 +477,7 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
         // slopes holding two neighbours less than 5 apart; the runs are
         // disjoint and each reads only its own keys, so the thread of the
         // run's first such pair orders it, with scratch at the run's offset
-        for (uint32_t k = 1 + t; k < n; k += 256) {
+        for (uint32_t k = 1 + t; k < n; k += NT) {
             const uint64_t hk = keys[k] >> 32;
             if (!(keys[k] - keys[k - 1] < 5 && hk == (keys[k - 1] >> 32))) continue;
             uint32_t s = k - 1;
@@ -419,9 +494,9 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
         // weights (fast/quads.c:968-1004) and the six running sums
         double run[6] = {0, 0, 0, 0, 0, 0};
         const int iw = F.qw, ih = F.qh;
-        for (uint32_t c0 = 0; c0 < n; c0 += kMomChunk) {
-            uint32_t len = min((uint32_t)kMomChunk, n - c0);
-            for (uint32_t j = t; j < len; j += 256) {
+        for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+            uint32_t len = min((uint32_t)CH, n - c0);
+            for (uint32_t j = t; j < len; j += NT) {
                 uint32_t src = (uint32_t)~(uint32_t)keys[c0 + j];
                 int px, py, gx, gy;
                 pt_unpack(F.cpts[s0 + src], &px, &py, &gx, &gy);
@@ -469,7 +544,7 @@ __global__ __launch_bounds__(256) void k_fit_moments(const Frame *frames, int pa
                 run[t] = s;
             }
             __syncthreads();
-            for (uint32_t j = t; j < len; j += 256)
+            for (uint32_t j = t; j < len; j += NT)
                 for (int q = 0; q < 6; q++) F.mom[q * pc + s0 + c0 + j] = sm[q][j];
             __syncthreads();
         }
@@ -968,7 +1043,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
                         int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk) {
     // k_fit_prep: every cluster of every frame, a grid per frame; the fit
     // kernels after it: persistent, fit_blocks blocks over the work lists
-    dim3 g(256, nframes), gp(fit_blocks);
+    dim3 g(256, nframes), gp(fit_blocks), gw(4 * fit_blocks);
     hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 0, (int)kNLarge);
@@ -977,16 +1052,17 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     int parity = 0;
     if (sort_cap) {
         hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), (size_t)sort_cap * 8, st, dframes, Q, 0);
-        hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), (size_t)kLargeCluster * 8, st, dframes, Q, 1);
+        hipLaunchKernelGGL(k_angle_sort_wave, gw, dim3(64), 0, st, dframes, Q);
     } else {
         launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);
     }
     mk->mark(st, "angle_sort");
-    hipLaunchKernelGGL(k_fit_moments, gp, dim3(256), 0, st, dframes, parity, Q, sort_cap ? 1 : 0);
+    hipLaunchKernelGGL((k_fit_moments<256, 512>), gp, dim3(256), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+    hipLaunchKernelGGL((k_fit_moments<64, 128>), gw, dim3(64), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 1);
     mk->mark(st, "moments");
     hipLaunchKernelGGL(k_fit_corners<256>, gp, dim3(256), 0, st, dframes, dparams, Q, 0);
     if (wave_corners)
-        hipLaunchKernelGGL(k_fit_corners<64>, dim3(4 * fit_blocks), dim3(64), 0, st, dframes, dparams, Q, 1);
+        hipLaunchKernelGGL(k_fit_corners<64>, gw, dim3(64), 0, st, dframes, dparams, Q, 1);
     else
         hipLaunchKernelGGL(k_fit_corners<256>, gp, dim3(256), 0, st, dframes, dparams, Q, 1);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
