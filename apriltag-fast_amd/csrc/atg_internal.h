@@ -36,6 +36,7 @@ namespace atg {
 
 constexpr int kMaxFamilies = 16;
 constexpr int kSortBlocks = 32;      // radix-sort blocks per frame
+constexpr int kSortHistBins = 4096;  // digits of the cluster sort's bucket pass
 constexpr int kMaxQuadsPerFrame = 4096;
 constexpr int kMaxDetsPerFrame = 1024;
 
@@ -93,6 +94,8 @@ struct Frame {
     int32_t skip;         // a frame the reference returns nothing for: an 8x8 zero image
     int64_t pcap;         // point capacity
     uint8_t *qim, *t, *tmx, *tmn, *tbuf;
+    uint8_t *fscr;        // union-find front state in HBM (frames too wide for LDS), else null
+    uint8_t *rscr;        // union-find replay state of rows too wide for LDS: [kReplayHbmWaves][..]
     uint32_t *runs, *rcnt, *row_off;
     uint32_t *parent, *fw, *nlab, *plab;
     uint32_t *rlab;           // = nlab until k_resolve: record of each run within its row
@@ -102,7 +105,7 @@ struct Frame {
     uint64_t *pts, *cpts;
     uint64_t *skey, *skey2;
     uint32_t *sval, *sval2;
-    uint32_t *rhist;      // [256][kSortBlocks]
+    uint32_t *rhist;      // [kSortHistBins][kSortBlocks]
     uint32_t *cstart, *ccount, *cflag;
     uint32_t *klist;      // [2][kcap] kept clusters: large ones, then (at kcap) the others
     uint32_t kcap;
@@ -147,11 +150,16 @@ struct Tune {
 // The replay kernels' memory plan for one call (LDS where it fits, else
 // HBM scratch).
 struct CclPlan {
-    int front, replay;          // LDS bytes, 0 = scratch
+    int front;                  // LDS bytes of the front (frames without fscr)
+    int replay;                 // LDS bytes of the replay
     int replay_cap;             // runs per row replayed in LDS, 0 = none
     int replay_waves;           // waves per frame of k_replay_hbm
-    size_t scratch_front, scratch_replay;  // HBM bytes per frame
 };
+
+// HBM scratch of one frame's union-find, by its own width: the front's
+// when it does not fit LDS (or forced), the replay's for wide rows.
+size_t front_scratch_bytes(int qw, bool force);
+size_t replay_scratch_bytes(int qw, bool force);
 
 // Optional timing marks between the stages of one call (HIP events on
 // the call's stream).

@@ -366,14 +366,14 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
 }
 
 template <int T>
-__global__ __launch_bounds__(T) void k_front(const Frame *frames, int lds_bytes, uint8_t *gscratch, int64_t gstride) {
+__global__ __launch_bounds__(T) void k_front(const Frame *frames) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t small[T / 64];
     const Frame F = frames[blockIdx.y];
-    if (lds_bytes > 0)
-        front_sweep<false, T>(F, lds, small);
+    if (F.fscr)
+        front_sweep<true, T>(F, F.fscr, small);
     else
-        front_sweep<true, T>(F, gscratch + (int64_t)blockIdx.y * gstride, small);
+        front_sweep<false, T>(F, lds, small);
 }
 
 // ---------------------------------------------------------------------
@@ -423,7 +423,12 @@ __host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix) {
 // LDS capacity (runs per row) for frames up to max_qw wide.
 int replay_cap(int max_qw) { return std::min(512, (std::max(max_qw, 1) + 63) & ~63); }
 size_t replay_lds_bytes(int cap) { return replay_layout(cap, 2 * cap, 2).total; }
-size_t replay_hbm_bytes(int qw) { return replay_layout(qw, 3 * qw, 4).total; }
+__host__ __device__ inline size_t replay_hbm_bytes_d(int qw) { return replay_layout(qw, 3 * qw, 4).total; }
+size_t replay_hbm_bytes(int qw) { return replay_hbm_bytes_d(qw); }
+
+// Waves of k_replay_hbm per frame: a few for the odd wide row, 64 when
+// every row takes it.
+constexpr int kReplayHbmWaves = 16, kReplayHbmWavesAll = 64;
 
 __device__ __forceinline__ bool replay_fits(const Frame &F, int y, int cap) {
     return (int)F.rcnt[y - 1] <= cap && (int)F.rcnt[y] <= cap && (int)F.ecnt[y] <= 2 * cap;
@@ -608,12 +613,12 @@ __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
 }
 
 // The other rows (all rows for cap 0): wave b takes rows 1 + b + k W,
-// 64 at a time checked by the lanes, in its own HBM scratch.
-__global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap, uint8_t *gscratch, int64_t fstride,
-                                                   int64_t wstride) {
+// 64 at a time checked by the lanes, in its own part of the frame's rscr.
+__global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap) {
     const Frame &F = frames[blockIdx.y];
     const int lane = threadIdx.x, W = (int)gridDim.x, b = (int)blockIdx.x;
-    uint8_t *mem = gscratch + (int64_t)blockIdx.y * fstride + (int64_t)b * wstride;
+    const size_t wstride = (replay_hbm_bytes_d(F.qw) + 255) & ~(size_t)255;
+    uint8_t *mem = F.rscr + (size_t)b * wstride;
     for (int k0 = 0; 1 + b + W * k0 < F.qh; k0 += 64) {
         const int y = 1 + b + W * (k0 + lane);
         uint64_t todo = __ballot(y < F.qh && (cap == 0 || !replay_fits(F, y, cap)));
@@ -719,65 +724,71 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     if (lane == 0) out[F.qw - 1] = F.nlab[F.row_off[F.qh] + y];
 }
 
-// LDS bytes of the three replay kernels for the widest frame of a call, or
-// 0 where it does not fit (then HBM scratch, ccl_scratch_bytes per frame).
-static bool lds_ok(const void *fn, size_t need, bool force_scratch) {
-    if (force_scratch || need > 160 * 1024) return false;
-    if (need <= 64 * 1024) return true;
-    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) == hipSuccess;
+// The front's dynamic LDS limit: 64 KiB, or most of the 160 KiB of a CU
+// where the kernels accept that much (asked once; a refusal is not left
+// behind as the runtime's last error).
+static size_t front_lds_limit() {
+    static size_t lim = 0;
+    if (!lim) {
+        const int big = 159 * 1024;
+        bool ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<128>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
+                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<256>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
+                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<512>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess;
+        (void)hipGetLastError();
+        lim = ok ? (size_t)big : (size_t)64 * 1024;
+    }
+    return lim;
 }
 
-CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch) {
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t front_scratch_bytes(int qw, bool force) {
+    size_t f = front_bytes(qw);
+    return (force || f > front_lds_limit()) ? al256(f) : 0;
+}
+
+size_t replay_scratch_bytes(int qw, bool force) {
+    return al256(replay_hbm_bytes(qw)) * (size_t)(force ? kReplayHbmWavesAll : kReplayHbmWaves);
+}
+
+// The launch shapes of a call: max_lds_qw is the widest frame whose front
+// runs in LDS (0: none), max_qw the widest frame.
+CclPlan ccl_plan(int max_lds_qw, int max_qw, bool force_scratch) {
     CclPlan c{};
-    size_t f = front_bytes(max_qw);
-    bool fok = lds_ok(reinterpret_cast<const void *>(&k_front<128>), f, force_scratch) &&
-               lds_ok(reinterpret_cast<const void *>(&k_front<256>), f, force_scratch) &&
-               lds_ok(reinterpret_cast<const void *>(&k_front<512>), f, force_scratch);
-    c.front = fok ? (int)f : 0;
-    // rows of up to replay_cap runs in LDS, the others (all when forced) in
-    // HBM: 16 waves per frame for the odd wide row, 64 when they are all
+    c.front = max_lds_qw > 0 ? (int)front_bytes(max_lds_qw) : 0;
+    // rows of up to replay_cap runs in LDS, the others (all when forced) in HBM
     c.replay_cap = force_scratch ? 0 : replay_cap(max_qw);
     c.replay = c.replay_cap ? (int)replay_lds_bytes(c.replay_cap) : 0;
-    c.replay_waves = std::min(std::max(1, max_qh - 1), c.replay_cap ? 16 : 64);
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    c.scratch_front = c.front ? 0 : al(f);
-    c.scratch_replay = al(replay_hbm_bytes(max_qw)) * (size_t)c.replay_waves;
+    c.replay_waves = force_scratch ? kReplayHbmWavesAll : kReplayHbmWaves;
     return c;
 }
 
-size_t ccl_scratch_bytes(const CclPlan &c) { return c.scratch_front + c.scratch_replay; }
-
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, const CclPlan &c, uint8_t *scratch, const Tune &tu, Marks *mk) {
+                      int max_qh, const CclPlan &c, const Tune &tu, Marks *mk) {
     // per-row kernels: tu.row_waves rows (one wave each) per block, at
     // most 4: their launch bounds are 256 threads
     const int rw = std::min(std::max(tu.row_waves, 1), 4);
     const dim3 rb(64 * rw), rg((unsigned)((max_qh + rw) / rw), nframes);
     hipLaunchKernelGGL(k_uf_edges, rg, rb, 0, st, dframes);
     mk->mark(st, "uf_edges");
-    // scratch per frame: [front][replay waves], frames back to back
-    int64_t per = (int64_t)ccl_scratch_bytes(c);
-    uint8_t *s_front = scratch, *s_replay = scratch + c.scratch_front;
-    int64_t rstride = (int64_t)(c.scratch_replay / (size_t)c.replay_waves);
     switch (tu.front_threads) {
     case 128:
-        hipLaunchKernelGGL(k_front<128>, dim3(1, nframes), dim3(128), (size_t)c.front, st, dframes, c.front, s_front,
-                           per);
+        hipLaunchKernelGGL(k_front<128>, dim3(1, nframes), dim3(128), (size_t)c.front, st, dframes);
         break;
     case 512:
-        hipLaunchKernelGGL(k_front<512>, dim3(1, nframes), dim3(512), (size_t)c.front, st, dframes, c.front, s_front,
-                           per);
+        hipLaunchKernelGGL(k_front<512>, dim3(1, nframes), dim3(512), (size_t)c.front, st, dframes);
         break;
     default:
-        hipLaunchKernelGGL(k_front<256>, dim3(1, nframes), dim3(256), (size_t)c.front, st, dframes, c.front, s_front,
-                           per);
+        hipLaunchKernelGGL(k_front<256>, dim3(1, nframes), dim3(256), (size_t)c.front, st, dframes);
     }
     mk->mark(st, "uf_front");
     if (c.replay_cap)
         hipLaunchKernelGGL(k_replay, dim3((unsigned)std::max(1, max_qh - 1), nframes), dim3(64), (size_t)c.replay, st,
                            dframes, c.replay_cap);
-    hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, c.replay_cap,
-                       s_replay, per, rstride);
+    hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, c.replay_cap);
     mk->mark(st, "uf_replay");
     for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);
     hipLaunchKernelGGL(k_lost, rg, rb, 0, st, dframes);

@@ -328,43 +328,97 @@ __global__ __launch_bounds__(256) void k_work_offsets(const Frame *frames, int n
 
 // ------------------------------------------------------------ angle sort
 
-// Persistent over list l of kept clusters: the full keys of a cluster
-// (slope << 32 | ~index, written by k_fit_prep in point order) sorted
-// ascending in LDS by a bitonic network. The keys are distinct, so any
-// sort gives the full-key order the reference's sort starts from
-// (fast/quads.c:379-427); its tie fix follows in k_fit_moments. The
-// clusters of list l have at most `cap` points (a power of two).
-__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q, int l) {
-    extern __shared__ uint64_t sk[];
-    const int t = threadIdx.x;
-    const uint32_t total = Q->total[l];
-    const uint32_t *off = Q->off[l];
+// The pairs of one step of a bitonic network in which every comparison
+// puts the smaller key first (the first step of each merge compares
+// mirrored positions): keys past n act as +infinity and never move, so
+// the network sorts any n in place. Pair q of the step (k, j) is (i, p):
+// j = 0 is the mirror step of merge size k, j > 0 the step of distance j.
+__device__ __forceinline__ void bitonic_pair(uint32_t q, uint32_t k, uint32_t j, uint32_t *i, uint32_t *p) {
+    if (j == 0) {
+        const uint32_t h = k >> 1, blk = q / h, r = q - blk * h;
+        *i = blk * k + r;
+        *p = blk * k + k - 1 - r;
+    } else {
+        *i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+        *p = *i + j;
+    }
+}
+
+constexpr uint32_t kSortTile = 2048;  // keys of a cluster sorted together in LDS
+
+// Steps (k, j) with partners inside tiles of kSortTile keys, on every tile
+// of keys[0, n) in turn: from merge size k0 up to kSortTile when k0 > 0,
+// else only the steps of distance below kSortTile of the merge of size kk.
+__device__ void sort_tiles(uint64_t *keys, uint32_t n, uint64_t *sk, uint32_t k0, uint32_t kk) {
+    const uint32_t t = threadIdx.x;
+    for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
+        const uint32_t tn = min(kSortTile, n - t0);
+        for (uint32_t i = t; i < kSortTile; i += 256) sk[i] = i < tn ? keys[t0 + i] : ~0ull;
+        __syncthreads();
+        auto step = [&](uint32_t k, uint32_t j) {
+            for (uint32_t q = t; q < kSortTile / 2; q += 256) {
+                uint32_t i, p;
+                bitonic_pair(q, k, j, &i, &p);
+                const uint64_t a = sk[i], b = sk[p];
+                if (a > b) {
+                    sk[i] = b;
+                    sk[p] = a;
+                }
+            }
+            __syncthreads();
+        };
+        if (k0) {
+            for (uint32_t k = k0; k <= kSortTile; k <<= 1) {
+                step(k, 0);
+                for (uint32_t j = k >> 2; j > 0; j >>= 1) step(k, j);
+            }
+        } else {
+            for (uint32_t j = kSortTile >> 1; j > 0; j >>= 1) step(kk, j);
+        }
+        for (uint32_t i = t; i < tn; i += 256) keys[t0 + i] = sk[i];
+        __syncthreads();
+    }
+}
+
+// Persistent over list 0 of kept clusters (above kLargeCluster points):
+// the full keys of a cluster (slope << 32 | ~index, written by k_fit_prep
+// in point order) sorted ascending, the steps within a tile in LDS, the
+// wider ones over the cluster's keys in HBM. The keys are distinct, so
+// any sort gives the full-key order the reference's sort starts from
+// (fast/quads.c:379-427); its tie fix follows in k_fit_moments.
+__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q) {
+    __shared__ uint64_t sk[kSortTile];
+    const uint32_t t = threadIdx.x;
+    const uint32_t total = Q->total[0];
+    const uint32_t *off = Q->off[0];
     uint32_t fi = 0;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         while (off[fi + 1] <= g) fi++;
         const Frame &F = frames[fi];
-        const uint32_t c = F.klist[(l ? F.kcap : 0u) + (g - off[fi])];
+        const uint32_t c = F.klist[g - off[fi]];
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
-        uint32_t m = 1;
-        while (m < n) m <<= 1;
-        for (uint32_t i = t; i < m; i += 256) sk[i] = i < n ? F.okey[s0 + i] : ~0ull;
+        uint64_t *keys = F.skey + s0;
+        for (uint32_t i = t; i < n; i += 256) keys[i] = F.okey[s0 + i];
         __syncthreads();
-        for (uint32_t k = 2; k <= m; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = t; i < m; i += 256) {
-                    const uint32_t p = i ^ j;
-                    if (p > i) {
-                        const uint64_t a = sk[i], b = sk[p];
-                        if ((a > b) == ((i & k) == 0)) {
-                            sk[i] = b;
-                            sk[p] = a;
-                        }
+        sort_tiles(keys, n, sk, 2, 0);
+        uint32_t m = kSortTile;
+        while (m < n) m <<= 1;
+        for (uint32_t k = 2 * kSortTile; k <= m; k <<= 1) {
+            for (uint32_t j = 0; j == 0 || j >= kSortTile; j = j ? j >> 1 : k >> 2) {
+                for (uint32_t q = t; q < m / 2; q += 256) {
+                    uint32_t i, p;
+                    bitonic_pair(q, k, j, &i, &p);
+                    if (p >= n) continue;
+                    const uint64_t a = keys[i], b = keys[p];
+                    if (a > b) {
+                        keys[i] = b;
+                        keys[p] = a;
                     }
                 }
                 __syncthreads();
             }
-        for (uint32_t i = t; i < n; i += 256) F.skey[s0 + i] = sk[i];
-        __syncthreads();
+            sort_tiles(keys, n, sk, 0, k);
+        }
     }
 }
 
@@ -1023,20 +1077,10 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity);
 
-// The angle sort's LDS capacity for the large clusters: the power of two
-// at or above the largest cluster of the call (max_px), or 0 when that
-// exceeds the per-kernel LDS limit (the sort of all points in HBM then).
+// Whether the angle sort runs per cluster (else over all points at once in
+// HBM): clusters up to 16384 points.
 int angle_sort_cap(int max_cluster, bool force_global) {
-    if (force_global) return 0;
-    int m = kLargeCluster;
-    while (m < max_cluster) m <<= 1;
-    if ((size_t)m * 8 > 128 * 1024) return 0;
-    static int attr = -1;
-    if (attr < 0)
-        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_angle_sort),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024) == hipSuccess;
-    if ((size_t)m * 8 > 64 * 1024 && !attr) return 0;
-    return m;
+    return (force_global || max_cluster > 16384) ? 0 : 1;
 }
 
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
@@ -1051,7 +1095,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     mk->mark(st, "fit_prep");
     int parity = 0;
     if (sort_cap) {
-        hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), (size_t)sort_cap * 8, st, dframes, Q, 0);
+        hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), 0, st, dframes, Q);
         hipLaunchKernelGGL(k_angle_sort_wave, gw, dim3(64), 0, st, dframes, Q);
     } else {
         launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);

@@ -21,14 +21,13 @@
 namespace atg {
 void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q);
 void launch_copy(hipStream_t st, void *dst, const void *src, int64_t bytes);
-CclPlan ccl_plan(int max_qw, int max_qh, bool force_scratch);
-size_t ccl_scratch_bytes(const CclPlan &c);
+CclPlan ccl_plan(int max_lds_qw, int max_qw, bool force_scratch);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                         int max_qh, bool blur, bool deglitch, const Tune &tu, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, const CclPlan &c, uint8_t *scratch, const Tune &tu, Marks *mk);
-void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
-                         Marks *mk);
+                      int max_qh, const CclPlan &c, const Tune &tu, Marks *mk);
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, bool bucket_pass,
+                         const Tune &tu, Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
                         int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk);
 int angle_sort_cap(int max_cluster, bool force_global);
@@ -86,6 +85,7 @@ struct FramePlan {
     int64_t W, H, stride;
     int64_t qw, qh;
     int64_t pcap;
+    bool force;           // the HBM paths of wide frames for every frame
     size_t off, size;     // arena region
     size_t in_off;        // staged input (host frames), within the region
 };
@@ -331,7 +331,10 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->skey2 = (uint64_t *)take((size_t)pc * 8);
     F->sval = (uint32_t *)take((size_t)pc * 4);
     F->sval2 = (uint32_t *)take((size_t)pc * 4);
-    F->rhist = (uint32_t *)take((size_t)256 * kSortBlocks * 4);
+    F->rhist = (uint32_t *)take((size_t)kSortHistBins * kSortBlocks * 4);
+    const size_t fs = front_scratch_bytes((int)qw, fp.force);
+    F->fscr = fs ? take(fs) : nullptr;
+    F->rscr = take(replay_scratch_bytes((int)qw, fp.force));
     F->cstart = (uint32_t *)take((size_t)(pc + 2) * 4);
     F->ccount = nullptr;
     F->cflag = (uint32_t *)take((size_t)(pc + 1) * 4);
@@ -447,6 +450,8 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     ctx->hframes.assign((size_t)n, Frame{});
     size_t total = 0, pinned_need = 0;
     int max_qw = 4, max_qh = 4, key_bits = 0, cl_bits = 0;
+    int64_t max_buckets = 0;
+    int max_lds_qw = 0;
     for (int32_t i = 0; i < n; i++) {
         FramePlan &fp = ctx->plans[(size_t)i];
         fp.W = frames[i].width;
@@ -460,8 +465,10 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         int64_t nodes = fp.qw * fp.qh + fp.qh;
         if (nodes >= ((int64_t)1 << 24)) return fail(ctx, ATG_ERR_LIMIT, "quad image over 2^24 pixels");
         fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
+        fp.force = ctx->force_scratch;
         Frame F{};
         fp.size = layout(fp, nfam, !on_device, &F, nullptr);
+        if (!F.fscr) max_lds_qw = std::max<int>(max_lds_qw, (int)fp.qw);
         fp.off = total;
         total += fp.size;
         if (!on_device && !fp.skip) pinned_need += align256((size_t)(fp.W * fp.H));
@@ -470,11 +477,10 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         int nb = ceil_log2((uint64_t)nodes);
         key_bits = std::max(key_bits, ceil_log2((uint64_t)cluster_map_size(fp.qw, fp.qh)) + 2 * nb);
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
+        max_buckets = std::max(max_buckets, cluster_map_size(fp.qw, fp.qh));
     }
-    // the union-find replay's state: in LDS when it fits, else per-frame scratch
-    const CclPlan ccl = ccl_plan(max_qw, max_qh, ctx->force_scratch);
-    const size_t gstride = ccl_scratch_bytes(ccl);
-    total += gstride * (size_t)n;
+    // the union-find's launch shapes (its HBM state is in each frame's region)
+    const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, ctx->force_scratch);
     const size_t qoff = total;
     total += queue_bytes(n);
     if (total > ctx->arena_size && ctx->arena_fixed) return fail(ctx, ATG_ERR_LIMIT, "batch over the arena budget");
@@ -522,7 +528,6 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     };
     std::vector<Copy> copies;
     size_t pin_at = 0;
-    uint8_t *gscratch = ctx->arena + qoff - gstride * (size_t)n;
     // the work lists: the header staged in pinned memory, copied in with the frames
     WorkQueue *dq = reinterpret_cast<WorkQueue *>(ctx->arena + qoff);
     {
@@ -586,6 +591,9 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     // the corner search of the small clusters a wave each, when its pair
     // tables hold the maxima kept
     const bool wave_corners = ctx->hp.max_nmaxima <= 16;
+    // the cluster sort as one pass over the hash bucket and a grouping by
+    // id, when the buckets fit its digit; else LSD passes over every bit
+    const bool bucket_pass = max_buckets <= kSortHistBins && !ctx->force_scratch;
     const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
     auto issue = [&]() -> hipError_t {
         for (const Copy &c : copies) {
@@ -595,8 +603,8 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         launch_init(st, ctx->dframes, n, dq);
         mk->mark(st, "init");
         launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, blur, deglitch, tu, mk);
-        launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, gscratch, tu, mk);
-        launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
+        launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, tu, mk);
+        launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, bucket_pass, tu, mk);
         launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
         launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, dq, mk);
         hipError_t e = hipGetLastError();
@@ -609,11 +617,12 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         // one graph per call shape: everything the stream work depends on
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
-                                    ccl.replay_cap, ccl.replay_waves,        (int64_t)ccl.scratch_front, (int64_t)ccl.scratch_replay,
+                                    ccl.replay_cap, ccl.replay_waves,        max_lds_qw,
                                     tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners,
+                                    bucket_pass,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
-                                    (int64_t)(intptr_t)ctx->dparams, (int64_t)(intptr_t)gscratch};
+                                    (int64_t)(intptr_t)ctx->dparams};
         for (const Copy &c : copies) {
             key.push_back((int64_t)(intptr_t)c.dst);
             key.push_back((int64_t)(intptr_t)c.src);
@@ -731,12 +740,11 @@ size_t batch_arena_bytes(const atg_ctx *ctx, const atg_params *p, const atg_fram
         if (!fp.skip && (fp.qw < 4 || fp.qh < 4)) fp.skip = true;
         if (fp.skip) { fp.qw = fp.qh = 4; fp.W = fp.H = 8; }
         fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
+        fp.force = ctx->force_scratch;
         Frame F{};
         total += layout(fp, nfam, !on_device, &F, nullptr);
-        max_qw = std::max<int>(max_qw, (int)fp.qw);
-        max_qh = std::max<int>(max_qh, (int)fp.qh);
     }
-    return total + ccl_scratch_bytes(ccl_plan(max_qw, max_qh, ctx->force_scratch)) * (size_t)n + queue_bytes(n);
+    return total + queue_bytes(n);
 }
 
 // A fixed arena of `bytes` and result space for `max_frames`: later calls
