@@ -36,7 +36,6 @@ namespace atg {
 
 constexpr int kMaxFamilies = 16;
 constexpr int kSortBlocks = 32;      // radix-sort blocks per frame
-constexpr int kSortHistBins = 4096;  // digits of the cluster sort's bucket pass
 constexpr int kMaxQuadsPerFrame = 4096;
 constexpr int kMaxDetsPerFrame = 1024;
 
@@ -78,6 +77,8 @@ enum {
     kNKeptPts,  // points of the clusters that go on to the quad fit
     kNLarge = 8,  // kept clusters above kLargeCluster points
     kNSmall = 9,  // the other kept clusters
+    kRootBits = 10,  // bits of the dense ok-root numbers
+    kSortBits = 11,  // bits of the cluster sort key
     kNCounters = 16
 };
 
@@ -105,7 +106,9 @@ struct Frame {
     uint64_t *pts, *cpts;
     uint64_t *skey, *skey2;
     uint32_t *sval, *sval2;
-    uint32_t *rhist;      // [kSortHistBins][kSortBlocks]
+    uint32_t *rhist;      // [256][kSortBlocks]
+    uint32_t *rdense;     // [nodes] dense number of each ok root, in node order
+    uint32_t *rinv;       // [nodes] the ok root of each dense number
     uint32_t *cstart, *ccount, *cflag;
     uint32_t *klist;      // [2][kcap] kept clusters: large ones, then (at kcap) the others
     uint32_t kcap;

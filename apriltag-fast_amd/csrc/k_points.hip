@@ -6,15 +6,19 @@
 //                  at its emission index (row offset + wave prefix) with
 //                  its cluster sort key.
 //   k_point_scan   the row offsets of the points.
+//   k_root_*       the roots of clusters that can hold points (the ok
+//                  roots) numbered densely in node order.
 //   radix sort     stable LSD sort of (key, index): the keys order the
 //                  clusters by (hash bucket, id) as the reference's cluster
 //                  table does (fast/clusters.c:338-356); stability keeps
 //                  each cluster's points in emission order.
 //   k_cluster_*    the points regrouped and the cluster starts compacted.
 //
-// Cluster sort key of a point: bucket << 2B | max_root << B | min_root
-// with B = bits of the node count (node ids < 2^B); comparing it equals
-// comparing (bucket, id) for id = max_root << 32 | min_root.
+// Cluster sort key of a point: bucket << 2D | dense(max_root) << D |
+// dense(min_root), D the bits of the ok-root count. The dense numbers keep
+// the roots' order, so comparing keys equals comparing (bucket, id) for
+// id = max_root << 32 | min_root; the passes beyond the key's 2D + bucket
+// bits (known on the device only) are skipped.
 #include <hip/hip_runtime.h>
 
 #include "atg_internal.h"
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
     const uint32_t *la = F.plab + (int64_t)y * QW, *lb = la + QW;
     uint32_t count = 0;
     uint32_t base = pass ? F.poff[y] : 0u;
-    const int NB = F.nbits_node;
+    const uint32_t D = pass ? F.counters[kRootBits] : 0u;
     const uint64_t mask = F.bucket_mask;
     for (int x0 = 0; x0 < QW - 1; x0 += 64) {
         int x = x0 + lane;
@@ -100,7 +104,7 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
                     uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
                     uint64_t id = ((uint64_t)hi << 32) + lo;
                     uint64_t bucket = ((id * 2654435761ull) >> 32) & mask;
-                    uint64_t key = (bucket << (2 * NB)) | ((uint64_t)hi << NB) | lo;
+                    uint64_t key = (bucket << (2 * D)) | ((uint64_t)F.rdense[hi] << D) | F.rdense[lo];
                     uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
                     uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
                     F.pts[at] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
@@ -161,9 +165,61 @@ __device__ __forceinline__ void seg_of(uint32_t n, int b, uint32_t *s0, uint32_t
 }
 
 // count = counters[which]
-__global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int which, int shift, int parity) {
+// The ok roots (nlab = root | ok << 31 names itself) of each segment of
+// nodes: counts, then dense numbers in node order (rdense) and back
+// (rinv); D and the sort key's bits into the counters.
+__device__ __forceinline__ bool ok_root(const Frame &F, uint32_t i) {
+    const uint32_t v = F.nlab[i];
+    return (v >> 31) && (v & 0x7fffffffu) == i;
+}
+
+__global__ __launch_bounds__(256) void k_root_count(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    const uint32_t nn = F.counters[kNRuns] + (uint32_t)F.qh;
+    uint32_t s0, s1;
+    seg_of(nn, blockIdx.x, &s0, &s1);
+    uint32_t c = 0;
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) c += ok_root(F, i) ? 1u : 0u;
+    uint32_t tot;
+    block_excl_scan(c, &tot);
+    if (threadIdx.x == 0) F.rhist[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_root_dense(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    const uint32_t nn = F.counters[kNRuns] + (uint32_t)F.qh;
+    uint32_t base = 0, all = 0;
+    for (int b = 0; b < kSortBlocks; b++) {
+        const uint32_t v = F.rhist[b];
+        if (b < (int)blockIdx.x) base += v;
+        all += v;
+    }
+    uint32_t s0, s1;
+    seg_of(nn, blockIdx.x, &s0, &s1);
+    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t f = (i < s1 && ok_root(F, i)) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pre = block_excl_scan(f, &tot);
+        if (f) {
+            F.rdense[i] = base + pre;
+            F.rinv[base + pre] = i;
+        }
+        base += tot;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t d = all <= 1 ? 1u : 32u - (uint32_t)__clz(all - 1);
+        F.counters[kRootBits] = d;
+        F.counters[kSortBits] = (uint32_t)__popcll(F.bucket_mask) + 2 * d;
+    }
+}
+
+// dyn: the pass is skipped where the frame's key (counters[kSortBits]) has
+// no bits at shift; the skipped passes are the last ones.
+__global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int which, int shift, int parity, int dyn) {
     __shared__ uint32_t h[256];
     const Frame F = frames[blockIdx.y];
+    if (dyn && shift >= (int)F.counters[kSortBits]) return;
     uint32_t n = F.counters[which];
     SortBufs B = sort_bufs(F, parity);
     h[threadIdx.x] = 0;
@@ -178,10 +234,12 @@ __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int whi
 // Stable scatter: chunks of 256 keys in order; inside a chunk, the rank of
 // a key among the keys of its digit before it: within a wave by ballots
 // (peers = lanes of the same digit), across waves by per-wave counts.
-__global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity) {
+__global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity,
+                                                       int dyn) {
     __shared__ uint32_t off[256];
     __shared__ uint32_t wcnt[4][256];
     const Frame F = frames[blockIdx.y];
+    if (dyn && shift >= (int)F.counters[kSortBits]) return;
     uint32_t n = F.counters[which];
     SortBufs B = sort_bufs(F, parity);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -230,170 +288,16 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
     }
 }
 
-// ------------------------------------------------- bucket pass + id groups
-//
-// Frames with at most 2^12 hash buckets sort in two steps instead: one
-// stable counting pass over the bucket digit (the key's top bits), then a
-// wave per bucket groups the bucket's points by id, ascending, each
-// group in emission order. That is the same stable order by (bucket, id)
-// as the LSD passes over every bit, since a bucket holds few clusters.
-
-constexpr int kBucketBits = 12;
-constexpr int kBuckets = kSortHistBins;
-static_assert(kBuckets == 1 << kBucketBits && kBuckets % 256 == 0, "bucket digit");
-
-__device__ __forceinline__ uint32_t bucket_of(const Frame &F, uint64_t key) {
-    return (uint32_t)(key >> (2 * F.nbits_node)) & (uint32_t)(kBuckets - 1);
-}
-
-// Counts per (bucket, segment): rhist[bucket * kSortBlocks + segment].
-__global__ __launch_bounds__(256) void k_bucket_hist(const Frame *frames) {
-    __shared__ uint32_t h[kBuckets];
-    const Frame F = frames[blockIdx.y];
-    const uint32_t n = F.counters[kNPts];
-    for (int d = threadIdx.x; d < kBuckets; d += 256) h[d] = 0;
-    __syncthreads();
-    uint32_t s0, s1;
-    seg_of(n, blockIdx.x, &s0, &s1);
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) atomicAdd(&h[bucket_of(F, F.skey[i])], 1u);
-    __syncthreads();
-    for (int d = threadIdx.x; d < kBuckets; d += 256) F.rhist[d * kSortBlocks + blockIdx.x] = h[d];
-}
-
-// The counts become exclusive offsets in (bucket, segment) order, in place
-// (one block per frame, 16 buckets a thread).
-__global__ __launch_bounds__(256) void k_bucket_scan(const Frame *frames) {
-    const Frame F = frames[blockIdx.y];
-    constexpr int kPer = kBuckets / 256 * kSortBlocks;
-    uint32_t *h = F.rhist + threadIdx.x * kPer;
-    uint32_t sum = 0;
-    for (int j = 0; j < kPer; j++) sum += h[j];
-    uint32_t tot;
-    uint32_t run = block_excl_scan(sum, &tot);
-    for (int j = 0; j < kPer; j++) {
-        const uint32_t v = h[j];
-        h[j] = run;
-        run += v;
-    }
-}
-
-// Stable scatter by bucket, buffer 0 -> 1: chunks of 256 keys in order;
-// inside a chunk the waves take their places in turn, the lanes of one
-// bucket (peers, found by ballots) by rank.
-__global__ __launch_bounds__(256) void k_bucket_scatter(const Frame *frames) {
-    __shared__ uint32_t off[kBuckets];
-    __shared__ uint32_t lbase[4][64];
-    const Frame F = frames[blockIdx.y];
-    const uint32_t n = F.counters[kNPts];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    for (int d = t; d < kBuckets; d += 256) off[d] = F.rhist[d * kSortBlocks + blockIdx.x];
-    __syncthreads();
-    uint32_t s0, s1;
-    seg_of(n, blockIdx.x, &s0, &s1);
-    const uint64_t lt = (1ull << lane) - 1ull;
-    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
-        const uint32_t i = c0 + t;
-        const bool valid = i < s1;
-        const uint64_t key = valid ? F.skey[i] : 0ull;
-        const uint32_t val = valid ? F.sval[i] : 0u;
-        const uint32_t d = bucket_of(F, key);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < kBucketBits; bit++) {
-            const uint64_t bb = __ballot((d >> bit) & 1u);
-            peers &= ((d >> bit) & 1u) ? bb : ~bb;
-        }
-        const uint32_t rank = __popcll(peers & lt);
-        const int leader = __ffsll((unsigned long long)peers) - 1;
-        for (int k = 0; k < 4; k++) {
-            if (w == k && valid && rank == 0) {
-                lbase[w][lane] = off[d];
-                off[d] += (uint32_t)__popcll(peers);
-            }
-            __syncthreads();
-        }
-        if (valid) {
-            const uint32_t pos = lbase[w][leader] + rank;
-            F.skey2[pos] = key;
-            F.sval2[pos] = val;
-        }
-        __syncthreads();  // lbase is rewritten by the next chunk
-    }
-}
-
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const uint64_t o = __shfl_xor(v, d, 64);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-// A wave per bucket, in buffer 1: a bucket of one id stays as the bucket
-// pass left it; the others are grouped by id, ids ascending, each group in
-// that order (written to buffer 0 at the bucket's place, then back).
-__global__ __launch_bounds__(256) void k_bucket_groups(const Frame *frames) {
-    const Frame F = frames[blockIdx.y];
-    const uint32_t n = F.counters[kNPts];
-    const int lane = threadIdx.x & 63;
-    const int d = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (d >= kBuckets) return;
-    const uint32_t bs = F.rhist[d * kSortBlocks];
-    const uint32_t be = d + 1 < kBuckets ? F.rhist[(d + 1) * kSortBlocks] : n;
-    if (be - bs < 2 || bs >= be) return;
-    const uint64_t idm = ((uint64_t)1 << (2 * F.nbits_node)) - 1;
-    constexpr uint64_t kMax = ~0ull;
-    const uint64_t lt = (1ull << lane) - 1ull;
-    // the least and greatest id, four loads in flight a lane
-    uint64_t m = kMax, mx = 0;
-    uint32_t i = bs + lane;
-    for (; i + 192 < be; i += 256) {
-        const uint64_t k0 = F.skey2[i] & idm, k1 = F.skey2[i + 64] & idm;
-        const uint64_t k2 = F.skey2[i + 128] & idm, k3 = F.skey2[i + 192] & idm;
-        m = min(min(m, k0), min(k1, min(k2, k3)));
-        mx = max(max(mx, k0), max(k1, max(k2, k3)));
-    }
-    for (; i < be; i += 64) {
-        const uint64_t id = F.skey2[i] & idm;
-        m = min(m, id);
-        mx = max(mx, id);
-    }
-    m = wave_min_u64(m);
-    mx = ~wave_min_u64(~mx);
-    if (m == mx) return;
-    uint32_t written = bs;
-    while (m != kMax) {
-        uint64_t next = kMax;
-        for (uint32_t c0 = bs; c0 < be; c0 += 64) {
-            const uint32_t j = c0 + lane;
-            const bool valid = j < be;
-            const uint64_t key = valid ? F.skey2[j] : 0ull;
-            const uint64_t id = key & idm;
-            const bool f = valid && id == m;
-            if (valid && id > m && id < next) next = id;
-            const uint64_t fb = __ballot(f);
-            if (f) {
-                const uint32_t pos = written + __popcll(fb & lt);
-                F.skey[pos] = key;
-                F.sval[pos] = F.sval2[j];
-            }
-            written += (uint32_t)__popcll(fb);
-        }
-        m = wave_min_u64(next);
-    }
-    for (uint32_t j = bs + lane; j < be; j += 64) {
-        F.skey2[j] = F.skey[j];
-        F.sval2[j] = F.sval[j];
-    }
-}
-
 // ---------------------------------------------------------------- clusters
 
-// After the sort (result in buffer `parity`): the points in cluster order,
-// and the number of cluster starts in each block's segment.
-__global__ __launch_bounds__(256) void k_cluster_count(const Frame *frames, int parity) {
+// The buffer of the cluster sort's result: one swap per executed pass.
+__device__ __forceinline__ int sorted_parity(const Frame &F) { return (int)(((F.counters[kSortBits] + 7) / 8) & 1); }
+
+// After the sort: the points in cluster order, and the number of cluster
+// starts in each block's segment.
+__global__ __launch_bounds__(256) void k_cluster_count(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
+    const int parity = sorted_parity(F);
     uint32_t n = F.counters[kNPts];
     const uint64_t *key = parity ? F.skey2 : F.skey;
     const uint32_t *val = parity ? F.sval2 : F.sval;
@@ -409,8 +313,9 @@ __global__ __launch_bounds__(256) void k_cluster_count(const Frame *frames, int 
     if (threadIdx.x == 0) F.rhist[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames, int parity) {
+__global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
+    const int parity = sorted_parity(F);
     uint32_t n = F.counters[kNPts];
     const uint64_t *key = parity ? F.skey2 : F.skey;
     uint32_t base = 0, all = 0;
@@ -428,10 +333,10 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames, int 
         uint32_t pre = block_excl_scan(f, &tot);
         if (f) {
             F.cstart[base + pre] = i;
-            // the cluster id back from the sort key
-            const int NB = F.nbits_node;
-            const uint64_t nm = ((uint64_t)1 << NB) - 1;
-            F.cid[base + pre] = (((key[i] >> NB) & nm) << 32) | (key[i] & nm);
+            // the cluster id back from the sort key's dense roots
+            const uint32_t D = F.counters[kRootBits];
+            const uint64_t dm = ((uint64_t)1 << D) - 1;
+            F.cid[base + pre] = ((uint64_t)F.rinv[(key[i] >> D) & dm] << 32) | F.rinv[key[i] & dm];
         }
         base += tot;
     }
@@ -443,37 +348,33 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames, int 
 
 // ---------------------------------------------------------------- launchers
 
-void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity) {
+void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
+                       int dyn) {
     for (int shift = 0; shift < nbits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity);
+                           *parity, dyn);
         hipLaunchKernelGGL(k_radix_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity);
+                           *parity, dyn);
         *parity ^= 1;
     }
 }
 
-void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, bool bucket_pass,
-                         const Tune &tu, Marks *mk) {
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
+                         Marks *mk) {
     const int rw = tu.row_waves < 1 ? 1 : tu.row_waves > 4 ? 4 : tu.row_waves;  // launch bounds: 256 threads
     dim3 g((unsigned)((max_qh + rw - 1) / rw), nframes), b(64 * rw);
+    hipLaunchKernelGGL(k_root_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_root_dense, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 0);
     hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);
     mk->mark(st, "points");
+    // key_bits: the widest key the call's frames can have
     int parity = 0;
-    if (bucket_pass) {
-        hipLaunchKernelGGL(k_bucket_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-        hipLaunchKernelGGL(k_bucket_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
-        hipLaunchKernelGGL(k_bucket_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-        hipLaunchKernelGGL(k_bucket_groups, dim3(kBuckets / 4, nframes), dim3(256), 0, st, dframes);
-        parity = 1;
-    } else {
-        launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity);
-    }
+    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1);
     mk->mark(st, "cluster_sort");
-    hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, parity);
-    hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, parity);
+    hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
     mk->mark(st, "clusters");
 }
 

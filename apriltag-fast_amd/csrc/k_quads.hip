@@ -1075,7 +1075,8 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
     if (threadIdx.x == 0) F.counters[kNQuads] = nq;
 }
 
-void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity);
+void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
+                       int dyn);
 
 // Whether the angle sort runs per cluster (else over all points at once in
 // HBM): clusters up to 16384 points.
@@ -1098,7 +1099,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), 0, st, dframes, Q);
         hipLaunchKernelGGL(k_angle_sort_wave, gw, dim3(64), 0, st, dframes, Q);
     } else {
-        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity);
+        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0);
     }
     mk->mark(st, "angle_sort");
     hipLaunchKernelGGL((k_fit_moments<256, 512>), gp, dim3(256), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);

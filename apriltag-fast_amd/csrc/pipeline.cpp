@@ -26,8 +26,8 @@ void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *d
                         int max_qh, bool blur, bool deglitch, const Tune &tu, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                       int max_qh, const CclPlan &c, const Tune &tu, Marks *mk);
-void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, bool bucket_pass,
-                         const Tune &tu, Marks *mk);
+void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
+                         Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
                         int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk);
 int angle_sort_cap(int max_cluster, bool force_global);
@@ -331,7 +331,9 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->skey2 = (uint64_t *)take((size_t)pc * 8);
     F->sval = (uint32_t *)take((size_t)pc * 4);
     F->sval2 = (uint32_t *)take((size_t)pc * 4);
-    F->rhist = (uint32_t *)take((size_t)kSortHistBins * kSortBlocks * 4);
+    F->rhist = (uint32_t *)take((size_t)256 * kSortBlocks * 4);
+    F->rdense = (uint32_t *)take((size_t)nodes * 4);
+    F->rinv = (uint32_t *)take((size_t)nodes * 4);
     const size_t fs = front_scratch_bytes((int)qw, fp.force);
     F->fscr = fs ? take(fs) : nullptr;
     F->rscr = take(replay_scratch_bytes((int)qw, fp.force));
@@ -450,7 +452,6 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     ctx->hframes.assign((size_t)n, Frame{});
     size_t total = 0, pinned_need = 0;
     int max_qw = 4, max_qh = 4, key_bits = 0, cl_bits = 0;
-    int64_t max_buckets = 0;
     int max_lds_qw = 0;
     for (int32_t i = 0; i < n; i++) {
         FramePlan &fp = ctx->plans[(size_t)i];
@@ -477,7 +478,6 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         int nb = ceil_log2((uint64_t)nodes);
         key_bits = std::max(key_bits, ceil_log2((uint64_t)cluster_map_size(fp.qw, fp.qh)) + 2 * nb);
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
-        max_buckets = std::max(max_buckets, cluster_map_size(fp.qw, fp.qh));
     }
     // the union-find's launch shapes (its HBM state is in each frame's region)
     const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, ctx->force_scratch);
@@ -591,9 +591,6 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     // the corner search of the small clusters a wave each, when its pair
     // tables hold the maxima kept
     const bool wave_corners = ctx->hp.max_nmaxima <= 16;
-    // the cluster sort as one pass over the hash bucket and a grouping by
-    // id, when the buckets fit its digit; else LSD passes over every bit
-    const bool bucket_pass = max_buckets <= kSortHistBins && !ctx->force_scratch;
     const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
     auto issue = [&]() -> hipError_t {
         for (const Copy &c : copies) {
@@ -604,7 +601,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         mk->mark(st, "init");
         launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, blur, deglitch, tu, mk);
         launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, tu, mk);
-        launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, bucket_pass, tu, mk);
+        launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
         launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
         launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, dq, mk);
         hipError_t e = hipGetLastError();
@@ -619,7 +616,6 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
                                     ccl.replay_cap, ccl.replay_waves,        max_lds_qw,
                                     tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners,
-                                    bucket_pass,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
                                     (int64_t)(intptr_t)ctx->dparams};
