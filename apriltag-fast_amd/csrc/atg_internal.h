@@ -164,6 +164,14 @@ struct CclPlan {
 size_t front_scratch_bytes(int qw, bool force);
 size_t replay_scratch_bytes(int qw, bool force);
 
+// The grid of a persistent kernel: the blocks of `fn` resident at once on
+// the current device (occupancy x CUs), at most `want`, so that no block
+// starts only when another has done its whole share. Looked up once per
+// kernel and block shape (warm_persistent_grids does so before any stream
+// capture).
+unsigned persistent_grid(const void *fn, int threads, size_t lds, int want);
+void warm_persistent_grids(int fit_blocks);
+
 // Optional timing marks between the stages of one call (HIP events on
 // the call's stream).
 constexpr int kMaxMarks = 24;

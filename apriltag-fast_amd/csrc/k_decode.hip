@@ -712,9 +712,12 @@ void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q
     hipLaunchKernelGGL(k_init, dim3(1, nframes), dim3(64), 0, st, dframes, Q);
 }
 
+void warm_decode_grid(int fit_blocks) { persistent_grid(reinterpret_cast<const void *>(&k_decode), 256, 0, fit_blocks); }
+
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          WorkQueue *Q, Marks *mk) {
-    hipLaunchKernelGGL(k_decode, dim3(blocks), dim3(256), 0, st, dframes, dparams, Q);
+    hipLaunchKernelGGL(k_decode, dim3(persistent_grid(reinterpret_cast<const void *>(&k_decode), 256, 0, blocks)),
+                       dim3(256), 0, st, dframes, dparams, Q);
     mk->mark(st, "decode");
     hipLaunchKernelGGL(k_finish, dim3(1, nframes), dim3(64), 0, st, dframes, dparams);
     mk->mark(st, "finish");

@@ -23,6 +23,10 @@
 // file is compiled without FP contraction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "atg_internal.h"
 
 #pragma clang fp contract(off)
@@ -1078,6 +1082,44 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
                        int dyn);
 
+void warm_decode_grid(int fit_blocks);
+
+unsigned persistent_grid(const void *fn, int threads, size_t lds, int want) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, int> cap;
+    int resident;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cap.find({fn, threads});
+        if (it == cap.end()) {
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess)
+                cus = per_cu = 0;
+            (void)hipGetLastError();
+            it = cap.emplace(std::make_pair(fn, threads), std::max(1, per_cu * cus)).first;
+        }
+        resident = it->second;
+    }
+    return (unsigned)std::max(1, std::min(want, resident));
+}
+
+template <typename K>
+static unsigned pgrid(K *fn, int threads, int want) {
+    return persistent_grid(reinterpret_cast<const void *>(fn), threads, 0, want);
+}
+
+void warm_persistent_grids(int fit_blocks) {
+    pgrid(k_angle_sort, 256, fit_blocks);
+    pgrid(k_angle_sort_wave, 64, fit_blocks);
+    pgrid(k_fit_moments<256, 512>, 256, fit_blocks);
+    pgrid(k_fit_moments<64, 128>, 64, fit_blocks);
+    pgrid(k_fit_corners<256>, 256, fit_blocks);
+    pgrid(k_fit_corners<64>, 64, fit_blocks);
+    warm_decode_grid(fit_blocks);
+}
+
 // Whether the angle sort runs per cluster (else over all points at once in
 // HBM): clusters up to 16384 points.
 int angle_sort_cap(int max_cluster, bool force_global) {
@@ -1088,7 +1130,10 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
                         int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk) {
     // k_fit_prep: every cluster of every frame, a grid per frame; the fit
     // kernels after it: persistent, fit_blocks blocks over the work lists
-    dim3 g(256, nframes), gp(fit_blocks), gw(4 * fit_blocks);
+    // the persistent kernels: fit_blocks blocks of 256 threads, or four
+    // times as many waves, at most as many as are resident at once
+    dim3 g(256, nframes);
+    const int wf = 4 * fit_blocks;
     hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 0, (int)kNLarge);
@@ -1096,20 +1141,25 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     mk->mark(st, "fit_prep");
     int parity = 0;
     if (sort_cap) {
-        hipLaunchKernelGGL(k_angle_sort, gp, dim3(256), 0, st, dframes, Q);
-        hipLaunchKernelGGL(k_angle_sort_wave, gw, dim3(64), 0, st, dframes, Q);
+        hipLaunchKernelGGL(k_angle_sort, dim3(pgrid(k_angle_sort, 256, fit_blocks)), dim3(256), 0, st, dframes, Q);
+        hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, st, dframes, Q);
     } else {
         launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0);
     }
     mk->mark(st, "angle_sort");
-    hipLaunchKernelGGL((k_fit_moments<256, 512>), gp, dim3(256), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
-    hipLaunchKernelGGL((k_fit_moments<64, 128>), gw, dim3(64), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 1);
+    hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256), 0,
+                       st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+    hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, st,
+                       dframes, parity, Q, sort_cap ? 1 : 0, 1);
     mk->mark(st, "moments");
-    hipLaunchKernelGGL(k_fit_corners<256>, gp, dim3(256), 0, st, dframes, dparams, Q, 0);
+    hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st, dframes,
+                       dparams, Q, 0);
     if (wave_corners)
-        hipLaunchKernelGGL(k_fit_corners<64>, gw, dim3(64), 0, st, dframes, dparams, Q, 1);
+        hipLaunchKernelGGL(k_fit_corners<64>, dim3(pgrid(k_fit_corners<64>, 64, wf)), dim3(64), 0, st, dframes, dparams,
+                           Q, 1);
     else
-        hipLaunchKernelGGL(k_fit_corners<256>, gp, dim3(256), 0, st, dframes, dparams, Q, 1);
+        hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st,
+                           dframes, dparams, Q, 1);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 2, (int)kNQuads);
     mk->mark(st, "corners");

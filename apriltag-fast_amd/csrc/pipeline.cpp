@@ -592,6 +592,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     // tables hold the maxima kept
     const bool wave_corners = ctx->hp.max_nmaxima <= 16;
     const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
+    warm_persistent_grids(tu.fit_blocks);  // device queries stay out of stream capture
     auto issue = [&]() -> hipError_t {
         for (const Copy &c : copies) {
             hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st);
