@@ -107,6 +107,8 @@ struct Frame {
     uint64_t *skey, *skey2;
     uint32_t *sval, *sval2;
     uint32_t *rhist;      // [256][kSortBlocks]
+    uint64_t *pw;         // [pcap] kept clusters' points: x | y << 16 | squared gradient << 32,
+                          // in cluster order (the memory of pts, free by then)
     uint32_t *rdense;     // [nodes] dense number of each ok root, in node order
     uint32_t *rinv;       // [nodes] the ok root of each dense number
     uint32_t *cstart, *ccount, *cflag;

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
     __shared__ float terms[kDotChunk];
     __shared__ uint32_t red[4][4];
     __shared__ float lanes8[8];
-    __shared__ int s_ok;
+    __shared__ int s_keep;
     const Frame F = frames[blockIdx.y];
     const uint32_t ncl = F.counters[kNClusters];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -181,8 +181,26 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
             bool keep = !((!P->reversed_border && rev) || (!P->normal_border && !rev));
             F.cflag[c] = keep ? (1u | (rev ? 2u : 0u)) : 0u;
             if (keep) atomicAdd(F.counters + kNKeptPts, n);
+            s_keep = keep ? 1 : 0;
         }
         __syncthreads();
+        if (s_keep) {
+            // for the moments: x, y and the squared image gradient at the
+            // point's pixel (fast/quads.c:968-985), in cluster order
+            const int iw = F.qw, ih = F.qh;
+            for (uint32_t i = t; i < n; i += 256) {
+                int px, py, gx, gy;
+                pt_unpack(pts[i], &px, &py, &gx, &gy);
+                int ix = (px >> 1) + (px & 1), iy = (py >> 1) + (py & 1);
+                uint32_t g2 = 0;
+                if (ix > 0 && ix + 1 < iw && iy > 0 && iy + 1 < ih) {
+                    const uint8_t *im = F.qim + (int64_t)iy * iw + ix;
+                    int ggx = (int)im[1] - (int)im[-1], ggy = (int)im[iw] - (int)im[-iw];
+                    g2 = (uint32_t)(ggx * ggx + ggy * ggy);
+                }
+                F.pw[s0 + i] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)g2 << 32);
+            }
+        }
     }
 }
 
@@ -198,7 +216,7 @@ __device__ __forceinline__ void kswap(uint64_t *k, int a, int b) {
 
 // The compare-exchange networks of the merge sort's leaves, high words
 // only (fast/quads.c:56-85).
-__device__ void leaf_network(uint64_t *k, int n) {
+__device__ __forceinline__ void leaf_network(uint64_t *k, int n) {
     if (n <= 1) return;
     if (n == 2) { kswap(k, 0, 1); return; }
     if (n == 3) { kswap(k, 0, 1); kswap(k, 1, 2); kswap(k, 0, 1); return; }
@@ -229,59 +247,160 @@ __device__ void leaf_of(int64_t n, int64_t p, int64_t *lo_out, int64_t *len_out)
 
 __device__ __forceinline__ int64_t kidx(uint64_t k) { return (int64_t)(uint32_t)~(uint32_t)k; }
 
-// One run of equal slopes (r, full-key order) into the merge sort's order:
-// the leaf keys of the run ordered by the first key of their block
-// (a key larger than all before it in its leaf), then position in the
-// leaf (fast/quads.c:241-302).
-__device__ void order_run(const uint64_t *orig, int64_t n, uint64_t *r, int64_t nr, uint64_t *ent, uint64_t *bh) {
-    uint64_t hi = r[0] & 0xffffffff00000000ull;
-    int64_t m = 0, k = 0;
-    while (k < nr) {
-        if (k + 1 == nr || kidx(r[k]) - kidx(r[k + 1]) >= 5) {
-            ent[m] = r[k];
-            bh[m] = r[k];
-            m++;
-            k++;
+// The entry of key i of a run (keys[s, s + m) of slope hi) in the merge
+// sort's order: the run's keys fall into groups of one merge-sort leaf
+// each (consecutive, as the indices descend along the run, at most 5 long);
+// the keys of a group are emitted in its leaf network's order, each with
+// the running maximum as its block head (fast/quads.c:241-302).
+__device__ __forceinline__ void run_entry(const uint64_t *orig, int64_t n, const uint64_t *keys, uint32_t s,
+                                          uint32_t m, uint64_t hi, uint32_t i, uint64_t key, int64_t lo, int64_t len,
+                                          int g0, int g1, uint64_t *ent, uint64_t *bh) {
+    *ent = key;
+    *bh = key;
+    if (g1 - g0 <= 1) return;
+    uint64_t lf[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) lf[q] = q < len ? orig[lo + q] : 0ull;
+    leaf_network(lf, (int)len);
+    const int j = (int)i - g0;
+    uint64_t top = 0;
+    int c = 0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        if (q < len && (lf[q] & 0xffffffff00000000ull) == hi) {
+            top = (c == 0 || lf[q] > top) ? lf[q] : top;
+            if (c == j) {
+                *ent = lf[q];
+                *bh = top;
+            }
+            c++;
+        }
+    }
+}
+
+// order_run for a run of m <= 64 keys, a lane per key, in registers; the
+// entries ranked by block head, stably.
+__device__ __forceinline__ void order_run_wave(const uint64_t *orig, int64_t n, uint64_t *keys, uint32_t s,
+                                               uint32_t m) {
+    const int lane = threadIdx.x & 63;
+    const bool act = lane < (int)m;
+    const uint64_t key = act ? keys[s + lane] : 0ull;
+    const uint64_t hi = __shfl(key, 0, 64) & 0xffffffff00000000ull;
+    int64_t lo = -1, len = 0;
+    if (act) leaf_of(n, kidx(key), &lo, &len);
+    const int64_t lo_prev = __shfl_up(lo, 1, 64);
+    const uint64_t starts = __ballot(act && (lane == 0 || lo_prev != lo));
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int g0 = 63 - __clzll((long long)(starts & upto));
+    const uint64_t after = starts & ~upto;
+    const int g1 = after ? __ffsll((unsigned long long)after) - 1 : (int)m;
+    uint64_t ent = key, bh = key;
+    if (act) run_entry(orig, n, keys, s, m, hi, (uint32_t)lane, key, lo, len, g0, g1, &ent, &bh);
+    uint32_t rank = 0;
+    for (int l2 = 0; l2 < (int)m; l2++) {
+        const uint64_t b2 = __shfl(bh, l2, 64);
+        rank += (b2 < bh || (b2 == bh && l2 < lane)) ? 1u : 0u;
+    }
+    if (act) keys[s + rank] = ent;
+}
+
+// order_run for a longer run, 64 keys at a time, the entries and heads in
+// HBM scratch (ent, bh at the run's positions).
+__device__ __noinline__ void order_run_long(const uint64_t *orig, int64_t n, uint64_t *keys, uint32_t s, uint32_t m,
+                               uint64_t *ent_s, uint64_t *bh_s) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t hi = keys[s] & 0xffffffff00000000ull;
+    auto leaf_lo = [&](uint32_t i) {
+        int64_t lo, len;
+        leaf_of(n, kidx(keys[s + i]), &lo, &len);
+        return lo;
+    };
+    for (uint32_t r0 = 0; r0 < m; r0 += 64) {
+        const uint32_t i = r0 + (uint32_t)lane;
+        if (i < m) {
+            const uint64_t key = keys[s + i];
+            int64_t lo, len;
+            leaf_of(n, kidx(key), &lo, &len);
+            int g0 = (int)i, g1 = (int)i + 1;
+            while (g0 > 0 && leaf_lo((uint32_t)g0 - 1) == lo) g0--;
+            while (g1 < (int)m && leaf_lo((uint32_t)g1) == lo) g1++;
+            uint64_t e, b;
+            run_entry(orig, n, keys, s, m, hi, i, key, lo, len, g0, g1, &e, &b);
+            ent_s[i] = e;
+            bh_s[i] = b;
+        }
+    }
+    __threadfence_block();
+    for (uint32_t r0 = 0; r0 < m; r0 += 64) {
+        const uint32_t i = r0 + (uint32_t)lane;
+        const uint64_t b = i < m ? bh_s[i] : 0ull;
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < m; j++) {
+            const uint64_t b2 = bh_s[j];
+            rank += (b2 < b || (b2 == b && j < i)) ? 1u : 0u;
+        }
+        if (i < m) keys[s + rank] = ent_s[i];
+    }
+}
+
+// The first index at or after p whose slope differs from hk (or n).
+__device__ __forceinline__ uint32_t slope_run_end(const uint64_t *keys, uint32_t n, uint32_t p, uint64_t hk) {
+    const int lane = threadIdx.x & 63;
+    for (; p < n; p += 64) {
+        const uint32_t k = p + lane;
+        const uint64_t b = __ballot(k >= n || (keys[k] >> 32) != hk);
+        if (b) return min(n, p + (uint32_t)__ffsll((unsigned long long)b) - 1);
+    }
+    return n;
+}
+
+// The first index of the run of slope hk that holds index p.
+__device__ __forceinline__ uint32_t slope_run_start(const uint64_t *keys, uint32_t p, uint64_t hk) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t q = p;; q -= 64) {
+        const int64_t k = q - lane;
+        const uint64_t b = __ballot(k < 0 || (keys[k] >> 32) != hk);
+        if (b) return (uint32_t)(q - (__ffsll((unsigned long long)b) - 1) + 1);
+    }
+}
+
+// fix_runs (fast/quads.c:304-327) by one wave over the runs of equal
+// slope that start in [from, to): each run holding two neighbours less
+// than 5 apart is put in the merge sort's order (order_run_wave, or
+// order_run_long past 64 keys). A run that starts
+// before `from` belongs to the wave before.
+__device__ void fix_runs_wave(uint64_t *keys, uint32_t n, const uint64_t *orig, uint32_t from, uint32_t to,
+                              uint64_t *ent, uint64_t *bh) {
+    const int lane = threadIdx.x & 63;
+    uint32_t pos = from;
+    if (pos >= to) return;
+    if (pos > 0 && (keys[pos] >> 32) == (keys[pos - 1] >> 32)) pos = slope_run_end(keys, n, pos, keys[pos] >> 32);
+    while (pos < to) {
+        const uint32_t k = pos + lane;
+        bool pr = false;
+        uint64_t kk = 0;
+        if (k < n && k >= 1) {
+            kk = keys[k];
+            const uint64_t kp = keys[k - 1];
+            pr = kk - kp < 5 && (kk >> 32) == (kp >> 32);
+        }
+        const uint64_t P = __ballot(pr);
+        if (!P) {
+            pos += 64;
             continue;
         }
-        int64_t lo, len;
-        leaf_of(n, kidx(r[k]), &lo, &len);
-        int64_t k1 = k + 1;
-        while (k1 < nr && kidx(r[k1]) >= lo) k1++;
-        if (k1 == k + 1) {
-            ent[m] = r[k];
-            bh[m] = r[k];
-            m++;
-        } else {
-            uint64_t lf[5] = {0, 0, 0, 0, 0};
-            for (int64_t q = 0; q < len; q++) lf[q] = orig[lo + q];
-            leaf_network(lf, (int)len);
-            uint64_t top = 0;
-            bool first = true;
-            for (int64_t q = 0; q < len; q++) {
-                uint64_t key = lf[q];
-                if ((key & 0xffffffff00000000ull) != hi) continue;
-                if (first || key > top) top = key;
-                first = false;
-                ent[m] = key;
-                bh[m] = top;
-                m++;
-            }
-        }
-        k = k1;
+        const int p0 = __ffsll((unsigned long long)P) - 1;
+        const uint32_t k0 = pos + (uint32_t)p0;
+        const uint64_t hk = __shfl(kk, p0, 64) >> 32;
+        const uint32_t s = slope_run_start(keys, k0 - 1, hk);
+        if (s >= to) break;
+        const uint32_t e = slope_run_end(keys, n, k0, hk);
+        if (e - s <= 64)
+            order_run_wave(orig, n, keys, s, e - s);
+        else
+            order_run_long(orig, n, keys, s, e - s, ent + s, bh + s);
+        pos = e;
     }
-    for (int64_t q = 1; q < m; q++) {
-        uint64_t e = ent[q], b = bh[q];
-        int64_t j = q;
-        while (j > 0 && bh[j - 1] > b) {
-            ent[j] = ent[j - 1];
-            bh[j] = bh[j - 1];
-            j--;
-        }
-        ent[j] = e;
-        bh[j] = b;
-    }
-    for (int64_t q = 0; q < m; q++) r[q] = ent[q];
 }
 
 // Per frame (a block each): its kept clusters in cluster order, those
@@ -531,40 +650,44 @@ __global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int par
                 keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
             __syncthreads();
         }
-        // fix_runs (fast/quads.c:304-327) reorders each maximal run of equal
-        // slopes holding two neighbours less than 5 apart; the runs are
-        // disjoint and each reads only its own keys, so the thread of the
-        // run's first such pair orders it, with scratch at the run's offset
-        for (uint32_t k = 1 + t; k < n; k += NT) {
-            const uint64_t hk = keys[k] >> 32;
-            if (!(keys[k] - keys[k - 1] < 5 && hk == (keys[k - 1] >> 32))) continue;
-            uint32_t s = k - 1;
-            while (s > 0 && (keys[s - 1] >> 32) == hk) s--;
-            bool first = true;
-            for (uint32_t j = s + 1; j < k && first; j++) first = !(keys[j] - keys[j - 1] < 5);
-            if (!first) continue;
-            uint32_t e = k + 1;
-            while (e < n && (keys[e] >> 32) == hk) e++;
-            order_run(F.okey + s0, n, keys + s, e - s, reinterpret_cast<uint64_t *>(F.errs + s0) + s,
-                      reinterpret_cast<uint64_t *>(F.filt + s0) + s);
+        // fix_runs (fast/quads.c:304-327): the runs of equal slopes split
+        // among the waves by where they start; the keys in LDS (the moment
+        // buffer, free until then) when they fit
+        {
+            constexpr int kW = NT / 64;
+            constexpr uint32_t kFixLds = 6 * CH;
+            uint64_t *kl = reinterpret_cast<uint64_t *>(&sm[0][0]);
+            const bool in_lds = n <= kFixLds;
+            if (in_lds) {
+                for (uint32_t i = t; i < n; i += NT) kl[i] = keys[i];
+                __syncthreads();
+            }
+            const uint32_t seg = (n + kW - 1) / kW, w = (uint32_t)(t >> 6);
+            fix_runs_wave(in_lds ? kl : keys, n, F.okey + s0, min(n, w * seg), min(n, (w + 1) * seg),
+                          reinterpret_cast<uint64_t *>(F.errs + s0), reinterpret_cast<uint64_t *>(F.filt + s0));
+            __syncthreads();
+            if (in_lds) {
+                for (uint32_t i = t; i < n; i += NT) keys[i] = kl[i];
+                __syncthreads();
+            }
         }
-        __syncthreads();
         // weights (fast/quads.c:968-1004) and the six running sums
         double run[6] = {0, 0, 0, 0, 0, 0};
-        const int iw = F.qw, ih = F.qh;
         for (uint32_t c0 = 0; c0 < n; c0 += CH) {
             uint32_t len = min((uint32_t)CH, n - c0);
-            for (uint32_t j = t; j < len; j += NT) {
-                uint32_t src = (uint32_t)~(uint32_t)keys[c0 + j];
-                int px, py, gx, gy;
-                pt_unpack(F.cpts[s0 + src], &px, &py, &gx, &gy);
-                int ix = (px >> 1) + (px & 1), iy = (py >> 1) + (py & 1);
-                int g2 = 0;
-                if (ix > 0 && ix + 1 < iw && iy > 0 && iy + 1 < ih) {
-                    const uint8_t *im = F.qim + (int64_t)iy * iw + ix;
-                    int ggx = (int)im[1] - (int)im[-1], ggy = (int)im[iw] - (int)im[-iw];
-                    g2 = ggx * ggx + ggy * ggy;
-                }
+            constexpr int kPer = CH / NT;  // points a thread, their loads issued together
+            uint64_t pwv[kPer];
+#pragma unroll
+            for (int r = 0; r < kPer; r++) {
+                const uint32_t j = t + r * NT;
+                pwv[r] = j < len ? F.pw[s0 + (uint32_t)~(uint32_t)keys[c0 + j]] : 0ull;
+            }
+#pragma unroll
+            for (int r = 0; r < kPer; r++) {
+                const uint32_t j = t + r * NT;
+                if (j >= len) break;
+                const int px = (int)(pwv[r] & 0xffffu), py = (int)((pwv[r] >> 16) & 0xffffu);
+                const uint32_t g2 = (uint32_t)(pwv[r] >> 32);
                 double wgt = __dadd_rn(__dsqrt_rn((double)g2), 1.0);
                 double fx = __dadd_rn(__dmul_rn((double)px, 0.5), 0.5);
                 double fy = __dadd_rn(__dmul_rn((double)py, 0.5), 0.5);

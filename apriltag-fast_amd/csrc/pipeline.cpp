@@ -327,6 +327,7 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->poff = (uint32_t *)take((size_t)(qh + 2) * 4);
     F->pts = (uint64_t *)take((size_t)pc * 8);
     F->cpts = (uint64_t *)take((size_t)pc * 8);
+    F->pw = F->pts;
     F->skey = (uint64_t *)take((size_t)pc * 8);
     F->skey2 = (uint64_t *)take((size_t)pc * 8);
     F->sval = (uint32_t *)take((size_t)pc * 4);
