@@ -65,16 +65,18 @@ def frame_counts(ctx, i, W, H):
     st = ctx.stages(i)
     qw, qh = st["qw"], st["qh"]
     R = c[0]
-    return dict(W=W, H=H, qw=qw, qh=qh, R=R, N=R + qh, E=c[6], P=c[1], C=c[2], K=c[7], Q=c[3])
+    return dict(W=W, H=H, qw=qw, qh=qh, R=R, N=R + qh, E=c[6], P=c[1], C=c[2], K=c[7], Q=c[3],
+                S=(c[11] + 7) // 8)
 
 
-def kernel_bytes(n, sort_passes=7):
+def kernel_bytes(n):
     """Algorithmic HBM bytes per frame of each kernel group: every array
     it reads or writes, once (a 4-byte word per run, node, link and record
     slot; an 8-byte word per point; 48 bytes of moments per fitted point).
-    Stages (a)-(c) follow the survey's per-frame formulas split by kernel;
-    (d)-(e) count the arrays of the fit and, for decode, the pixels each
-    quad samples (refine: 4 edges x 16 samples x 25 steps x 2 bilinear
+    Stages (a)-(c) follow the per-frame formulas of the byte model split by
+    kernel, the cluster sort with the frame's own number of radix passes
+    (S); (d)-(e) count the arrays of the fit and, for decode, the pixels
+    each quad samples (refine: 4 edges x 16 samples x 25 steps x 2 bilinear
     reads of 4 pixels)."""
     W, H, qw, qh = n["W"], n["H"], n["qw"], n["qh"]
     R, N, E, P, K, Q = n["R"], n["N"], n["E"], n["P"], n["K"], n["Q"]
@@ -89,11 +91,11 @@ def kernel_bytes(n, sort_passes=7):
         "uf_stitch": lg * 8 * N + 16 * N,
         "resolve": 16 * N,
         "labels": px + 4 * px + 4 * N,
-        "points": 2 * (px + 4 * px) + 12 * P,
-        "cluster_sort": sort_passes * 32 * P,
+        "points": 8 * N + 2 * (px + 4 * px) + 12 * P,
+        "cluster_sort": n["S"] * 32 * P,
         "clusters": 24 * P,
-        "fit_prep": 8 * P + 20 * P,
-        "angle_sort": sort_passes * 32 * P,
+        "fit_prep": 8 * P + 20 * P + 12 * K,
+        "angle_sort": 16 * K,
         "moments": 72 * K,
         "corners": 80 * K,
         "decode": Q * 4 * 16 * 25 * 8,
@@ -386,7 +388,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u8 (stages a-c), f64/f32 as the reference (d-e)",
         "data": "synthetic",
-        "config": {"workload": "10 seeded 1920x1080 frames, 20 tag36h11 tags each (configs[1]), decimate 2, "
+        "config": {"workload": "10 seeded 1920x1080 frames, 20 tag36h11 tags each, decimate 2, "
                                "refine_edges 1, bits_corrected 2; batch of %d frames (the 10 cycled)" % args.batch,
                    "batch": args.batch, "frame": "1920x1080 u8"},
         "single_frame_fps": round(single_fps, 2),
