@@ -394,9 +394,11 @@ __global__ __launch_bounds__(T) void k_front(const Frame *frames) {
 // to, or kLostNone / kLostRetired), rroot[slot] of each row-y record (the
 // root of its set as replayed: a node or kSym | a previous record's slot).
 //
-// Rows of at most C runs (each of the two rows) and 2C links replay in
-// LDS (k_replay, a wave per row); other rows, and all rows when asked,
-// in HBM scratch sized for the row width (k_replay_hbm).
+// Rows of at most C = 512 runs (each of the two rows) and 2C links replay
+// in LDS (k_replay, a wave per row); the wider rows in a second k_replay
+// launch with LDS for the widest row of the call (C = qw, 3C links); rows
+// beyond that, and all rows when asked, in HBM scratch sized for the row
+// width (k_replay_hbm).
 
 // Byte offsets of a row's replay memory for C runs, L links and ix-byte
 // link indices.
@@ -420,9 +422,12 @@ __host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix) {
     return r;
 }
 
-// LDS capacity (runs per row) for frames up to max_qw wide.
+// LDS capacity (runs per row) of the first replay launch for frames up to
+// max_qw wide, and of the second (the rows wider than the first's).
 int replay_cap(int max_qw) { return std::min(512, (std::max(max_qw, 1) + 63) & ~63); }
+int replay_wide_cap(int max_qw) { return (std::max(max_qw, 1) + 63) & ~63; }
 size_t replay_lds_bytes(int cap) { return replay_layout(cap, 2 * cap, 2).total; }
+size_t replay_wide_lds_bytes(int cap) { return replay_layout(cap, 3 * cap, 2).total; }
 __host__ __device__ inline size_t replay_hbm_bytes_d(int qw) { return replay_layout(qw, 3 * qw, 4).total; }
 size_t replay_hbm_bytes(int qw) { return replay_hbm_bytes_d(qw); }
 
@@ -430,8 +435,8 @@ size_t replay_hbm_bytes(int qw) { return replay_hbm_bytes_d(qw); }
 // every row takes it.
 constexpr int kReplayHbmWaves = 16, kReplayHbmWavesAll = 64;
 
-__device__ __forceinline__ bool replay_fits(const Frame &F, int y, int cap) {
-    return (int)F.rcnt[y - 1] <= cap && (int)F.rcnt[y] <= cap && (int)F.ecnt[y] <= 2 * cap;
+__device__ __forceinline__ bool replay_fits(const Frame &F, int y, int cap, int lcap) {
+    return (int)F.rcnt[y - 1] <= cap && (int)F.rcnt[y] <= cap && (int)F.ecnt[y] <= lcap;
 }
 
 template <bool G>
@@ -603,25 +608,28 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     for (int i = lane; i < ncur; i += 64) F.rroot[cb + rlab_cur[i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
 }
 
-// A wave per row, the rows that fit C = cap runs.
-__global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
+// A wave per row, the rows that fit C = cap runs and lcap links but not
+// (when prev_cap > 0) the previous launch's prev_cap runs and 2 prev_cap links.
+__global__ __launch_bounds__(64) void k_replay(const Frame *frames, int prev_cap, int cap, int lcap) {
     extern __shared__ __align__(16) uint8_t lds[];
     const Frame &F = frames[blockIdx.y];
     const int y = 1 + (int)blockIdx.x;
-    if (y >= F.qh || !replay_fits(F, y, cap)) return;
-    replay_row<false>(F, y, lds, cap, 2 * cap);
+    if (y >= F.qh || !replay_fits(F, y, cap, lcap)) return;
+    if (prev_cap > 0 && replay_fits(F, y, prev_cap, 2 * prev_cap)) return;
+    replay_row<false>(F, y, lds, cap, lcap);
 }
 
 // The other rows (all rows for cap 0): wave b takes rows 1 + b + k W,
 // 64 at a time checked by the lanes, in its own part of the frame's rscr.
-__global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap) {
+// cap, lcap: the largest runs and links the LDS launches took.
+__global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap, int lcap) {
     const Frame &F = frames[blockIdx.y];
     const int lane = threadIdx.x, W = (int)gridDim.x, b = (int)blockIdx.x;
     const size_t wstride = (replay_hbm_bytes_d(F.qw) + 255) & ~(size_t)255;
     uint8_t *mem = F.rscr + (size_t)b * wstride;
     for (int k0 = 0; 1 + b + W * k0 < F.qh; k0 += 64) {
         const int y = 1 + b + W * (k0 + lane);
-        uint64_t todo = __ballot(y < F.qh && (cap == 0 || !replay_fits(F, y, cap)));
+        uint64_t todo = __ballot(y < F.qh && (cap == 0 || !replay_fits(F, y, cap, lcap)));
         while (todo) {
             const int j = __ffsll((unsigned long long)todo) - 1;
             todo &= todo - 1;
@@ -762,6 +770,22 @@ CclPlan ccl_plan(int max_lds_qw, int max_qw, bool force_scratch) {
     // rows of up to replay_cap runs in LDS, the others (all when forced) in HBM
     c.replay_cap = force_scratch ? 0 : replay_cap(max_qw);
     c.replay = c.replay_cap ? (int)replay_lds_bytes(c.replay_cap) : 0;
+    // the wider rows: a second LDS launch when the widest row fits LDS
+    c.replay_wide_cap = 0;
+    c.replay_wide = 0;
+    if (c.replay_cap && replay_wide_cap(max_qw) > c.replay_cap) {
+        const size_t wb = replay_wide_lds_bytes(replay_wide_cap(max_qw));
+        bool ok = wb <= 64 * 1024;
+        if (!ok && wb <= 159 * 1024) {
+            ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_replay),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)wb) == hipSuccess;
+            (void)hipGetLastError();
+        }
+        if (ok) {
+            c.replay_wide_cap = replay_wide_cap(max_qw);
+            c.replay_wide = (int)wb;
+        }
+    }
     c.replay_waves = force_scratch ? kReplayHbmWavesAll : kReplayHbmWaves;
     return c;
 }
@@ -785,10 +809,15 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
         hipLaunchKernelGGL(k_front<256>, dim3(1, nframes), dim3(256), (size_t)c.front, st, dframes);
     }
     mk->mark(st, "uf_front");
+    const dim3 rows((unsigned)std::max(1, max_qh - 1), nframes);
     if (c.replay_cap)
-        hipLaunchKernelGGL(k_replay, dim3((unsigned)std::max(1, max_qh - 1), nframes), dim3(64), (size_t)c.replay, st,
-                           dframes, c.replay_cap);
-    hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, c.replay_cap);
+        hipLaunchKernelGGL(k_replay, rows, dim3(64), (size_t)c.replay, st, dframes, 0, c.replay_cap, 2 * c.replay_cap);
+    if (c.replay_wide_cap)
+        hipLaunchKernelGGL(k_replay, rows, dim3(64), (size_t)c.replay_wide, st, dframes, c.replay_cap,
+                           c.replay_wide_cap, 3 * c.replay_wide_cap);
+    const int hcap = c.replay_wide_cap ? c.replay_wide_cap : c.replay_cap;
+    const int hlcap = c.replay_wide_cap ? 3 * c.replay_wide_cap : 2 * c.replay_cap;
+    hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, hcap, hlcap);
     mk->mark(st, "uf_replay");
     for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);
     hipLaunchKernelGGL(k_lost, rg, rb, 0, st, dframes);
