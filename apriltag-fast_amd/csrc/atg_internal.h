@@ -79,6 +79,7 @@ enum {
     kNSmall = 9,  // the other kept clusters
     kRootBits = 10,  // bits of the dense ok-root numbers
     kSortBits = 11,  // bits of the cluster sort key
+    kNWideRows = 12,  // rows the first replay launch left (over 512 runs), listed in rwide
     kNCounters = 16
 };
 
@@ -109,6 +110,7 @@ struct Frame {
     uint32_t *rhist;      // [256][kSortBlocks]
     uint64_t *pw;         // [pcap] kept clusters' points: x | y << 16 | squared gradient << 32,
                           // in cluster order (the memory of pts, free by then)
+    uint32_t *rwide;      // [qh] rows for the replay's wide launch
     uint32_t *rdense;     // [nodes] dense number of each ok root, in node order
     uint32_t *rinv;       // [nodes] the ok root of each dense number
     uint32_t *cstart, *ccount, *cflag;
@@ -131,15 +133,14 @@ struct Frame {
 // The work of the persistent kernels, across the frames of a call: list 0
 // the kept clusters above kLargeCluster points (taken first, so that the
 // longest items start early), list 1 the other kept clusters, list 2 the
-// quads. Each frame lists its own items (klist); off[l][f] is where frame
+// quads, list 3 the rows of the union-find replay's wide launch. Each frame lists its own items (klist); off[l][f] is where frame
 // f's items start in list l's global index, total[l] its size. Blocks take
 // global indices in strides, without atomics.
 constexpr uint32_t kLargeCluster = 512;
 struct WorkQueue {
     uint32_t total[4];
     uint32_t nframes, pad[3];
-    uint32_t *off[3];   // [nframes + 1] each
-    uint64_t pad2;
+    uint32_t *off[4];   // [nframes + 1] each
 };
 
 // Work splits of the kernel chain (the autotuner's choices); any values
@@ -159,6 +160,7 @@ struct CclPlan {
     int replay;                 // LDS bytes of the replay
     int replay_cap;             // runs per row replayed in LDS, 0 = none
     int replay_wide, replay_wide_cap;  // the second LDS launch (wider rows): bytes, runs; 0 = none
+    int replay_wide_grid;       // its persistent grid
     int replay_waves;           // waves per frame of k_replay_hbm
 };
 

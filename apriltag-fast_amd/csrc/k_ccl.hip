@@ -182,11 +182,11 @@ __device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, ui
     }
 }
 
-// Staged rows: the first kStage runs and links of a row come from
-// registers loaded one row ahead; the rest (rare) from HBM at use.
+// The first kStage runs and links of a row come from registers loaded one
+// row ahead; the rest (rare) from HBM at use.
 constexpr int kStage = 1024;
 
-size_t front_bytes(int qw) { return (size_t)36 * qw + (size_t)8 * kStage + 128; }
+size_t front_bytes(int qw) { return (size_t)36 * qw + 128; }
 
 // A barrier of the whole block over LDS only (the scratch variant orders
 // HBM too).
@@ -204,22 +204,23 @@ __device__ __forceinline__ void bsync() {
 template <bool G, int kFrontThreads>
 __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) {
     constexpr int kPer = kStage / kFrontThreads;  // prefetched runs and links per thread
-    constexpr int kWaves = kFrontThreads / 64;
     using U32 = typename Mem<G>::U32;
     using U16 = typename Mem<G>::U16;
+    (void)lds_small;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int QW = F.qw, QH = F.qh;
+    // elements of row y: the records of row y-1 at [0, QW) (each named by
+    // its first run in row y-1), the runs of row y at QW + i, the virtual
+    // node at 2 QW
     const int EM = 2 * QW + 2;
+    const uint32_t VN = 2u * (uint32_t)QW;
     U32 *p = reinterpret_cast<U32 *>(base);
     U32 *acc = p + EM;
-    U32 *flag = acc + EM;
-    U32 *W = flag + EM;
+    U32 *minrun = acc + EM;
+    U32 *W = minrun + EM;          // [QW + 1] weight of each record, at its first run
     U32 *Wn = W + QW + 1;
-    U32 *srun = Wn + QW + 1;       // [kStage] the row's runs
-    U32 *sedge = srun + kStage;    // [kStage] the row's links
-    U16 *labA = reinterpret_cast<U16 *>(sedge + kStage);
+    U16 *labA = reinterpret_cast<U16 *>(Wn + QW + 1);  // [QW] record (first run) of each run
     U16 *labB = labA + QW;
-    uint32_t *wsum = lds_small;    // [kWaves] per-wave totals of a block scan
     // row offsets: exclusive scan of the per-row run counts (wave 0)
     if (wave == 0) {
         uint32_t carry = 0;
@@ -239,18 +240,17 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             F.fw[carry + y] = 1;
         }
     }
-    // row 0: each run is its own set
-    int m = (int)F.rcnt[0];
-    for (int i = tid; i < m; i += kFrontThreads) {
+    // row 0: each run is its own set (named by itself)
+    int pcur = (int)F.rcnt[0];
+    for (int i = tid; i < pcur; i += kFrontThreads) {
         uint32_t r = F.runs[i];
         labA[i] = (uint16_t)i;
         W[i] = run_end(r) - run_start(r) + 1;
-        F.rroot[i] = (uint32_t)i;
         F.parent[i] = (uint32_t)i;
     }
     uint32_t pbase = 0;  // row_off of the previous row
-    int pcur = m;        // runs of the previous row
-    // the next row, loaded one row ahead
+    // the next row, loaded one row ahead: thread tid holds runs and links
+    // tid + j kFrontThreads
     uint32_t nx_rc = 0, nx_ec = 0, nx_r[kPer], nx_e[kPer];
     uint8_t nx_t1 = 0, nx_t2 = 0;
     auto prefetch = [&](int y) {
@@ -271,98 +271,100 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
     if (QH > 1) prefetch(1);
     bsync<G>();
     for (int y = 1; y < QH; y++) {
-        // stage the row
         const int ncur = (int)nx_rc, ne = (int)nx_ec;
         const uint8_t t1 = nx_t1, t2 = nx_t2;
+        uint32_t rr[kPer], ee[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; j++) {
-            int i = tid + j * kFrontThreads;
-            if (i < ncur) srun[i] = nx_r[j];
-            if (i < ne) sedge[i] = nx_e[j];
+            rr[j] = nx_r[j];
+            ee[j] = nx_e[j];
         }
-        // the previous row's outputs
-        for (int i = tid; i < pcur; i += kFrontThreads) F.rlab[pbase + i] = labA[i];
-        for (int l = tid; l < m; l += kFrontThreads) F.recw[pbase + l] = W[l];
-        if (tid == 0) F.nrec[y - 1] = (uint32_t)m;
+        // the previous row's outputs: each run's record (its first run)
+        // and, at a record's first run, its weight
+        for (int i = tid; i < pcur; i += kFrontThreads) {
+            const uint32_t l = labA[i];
+            F.rlab[pbase + i] = l;
+            if (l == (uint32_t)i) F.recw[pbase + i] = W[i];
+        }
         const uint32_t cbase = pbase + (uint32_t)pcur;
         if (y + 1 < QH) prefetch(y + 1);
         const uint32_t *cur = F.runs + (int64_t)y * QW;
         const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
-        auto run_at = [&](int i) -> uint32_t { return i < kStage ? (uint32_t)srun[i] : cur[i]; };
-        const int ev = m + ncur, ne_all = ev + 1;
-        bsync<G>();
-        bool lastcol = false;  // fast/unionfind.c:93-97
-        if (ncur > 0) {
-            uint32_t lr = run_at(ncur - 1);
-            lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && t1 == 255 && t2 != 255;
-        }
-        for (int x = tid; x < ne_all; x += kFrontThreads) {
-            uint32_t w;
-            if (x < m) {
-                w = W[x];
-            } else if (x < ev) {
-                uint32_t r = run_at(x - m);
-                w = run_end(r) - run_start(r) + 1;
-            } else {
-                w = 1;
-            }
+        // elements: records of row y-1, runs of row y, the virtual node
+        for (int x = tid; x < pcur; x += kFrontThreads) {
             p[x] = (uint32_t)x;
-            acc[x] = w;
-            flag[x] = 0;
+            acc[x] = W[x];
         }
-        bsync<G>();
-        for (int e = tid; e < ne + (lastcol ? 1 : 0); e += kFrontThreads) {
-            if (e < ne) {
-                uint32_t ed = e < kStage ? (uint32_t)sedge[e] : edges[e];
-                fr_unite<G>(p, (uint32_t)m + (ed >> 16), (uint32_t)labA[ed & 0xffffu]);
-            } else {
-                fr_unite<G>(p, (uint32_t)(ev - 1), (uint32_t)ev);
+        bool lastcol = false;  // fast/unionfind.c:93-97, on the thread of the last run
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const int i = tid + j * kFrontThreads;
+            if (i < ncur) {
+                const uint32_t r = rr[j];
+                p[QW + i] = (uint32_t)(QW + i);
+                acc[QW + i] = run_end(r) - run_start(r) + 1;
+                minrun[QW + i] = 0xffffffffu;
+                if (i == ncur - 1) lastcol = (int)run_end(r) == QW - 2 && run_white(r) && t1 == 255 && t2 != 255;
             }
         }
+        for (int i = kStage + tid; i < ncur; i += kFrontThreads) {
+            const uint32_t r = cur[i];
+            p[QW + i] = (uint32_t)(QW + i);
+            acc[QW + i] = run_end(r) - run_start(r) + 1;
+            minrun[QW + i] = 0xffffffffu;
+            if (i == ncur - 1) lastcol = (int)run_end(r) == QW - 2 && run_white(r) && t1 == 255 && t2 != 255;
+        }
+        for (int x = tid; x < pcur; x += kFrontThreads) minrun[x] = 0xffffffffu;
+        if (tid == 0) {
+            p[VN] = VN;
+            acc[VN] = 1;
+            minrun[VN] = 0xffffffffu;
+        }
         bsync<G>();
-        for (int x = tid; x < ne_all; x += kFrontThreads) {
-            uint32_t r = fr_find<G>(p, (uint32_t)x);
-            if (r != (uint32_t)x) {
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const int e = tid + j * kFrontThreads;
+            if (e < ne) fr_unite<G>(p, (uint32_t)QW + (ee[j] >> 16), (uint32_t)labA[ee[j] & 0xffffu]);
+        }
+        for (int e = kStage + tid; e < ne; e += kFrontThreads) {
+            const uint32_t ed = edges[e];
+            fr_unite<G>(p, (uint32_t)QW + (ed >> 16), (uint32_t)labA[ed & 0xffffu]);
+        }
+        if (lastcol) fr_unite<G>(p, (uint32_t)(QW + ncur - 1), VN);
+        bsync<G>();
+        // each element under its root: weights into the roots, and the
+        // least run of each set that holds one
+        auto settle = [&](uint32_t x, bool run) {
+            const uint32_t r = fr_find<G>(p, x);
+            if (r != x) {
                 p[x] = r;
                 atomicAdd((uint32_t *)&acc[r], (uint32_t)acc[x]);  // acc of a non-root never changes
             }
-            if (x >= m && x < ev) flag[r] = 1;
+            if (run) atomicMin((uint32_t *)&minrun[r], x - (uint32_t)QW);
+        };
+        for (int x = tid; x < pcur; x += kFrontThreads) settle((uint32_t)x, false);
+        for (int i = tid; i < ncur; i += kFrontThreads) settle((uint32_t)(QW + i), true);
+        if (tid == 0) settle(VN, false);
+        bsync<G>();
+        // row y's records: each run's set named by its first run
+        for (int i = tid; i < ncur; i += kFrontThreads) {
+            const uint32_t r = p[QW + i];
+            const uint32_t l = minrun[r];
+            labB[i] = (uint16_t)l;
+            if (l == (uint32_t)i) Wn[i] = acc[r];
         }
         bsync<G>();
-        // row y's records: the roots holding a current run, in element
-        // order; each thread numbers a contiguous block of elements
-        const int per = (ne_all + kFrontThreads - 1) / kFrontThreads;
-        const int x0 = tid * per, x1 = min(x0 + per, ne_all);
-        uint32_t cnt = 0;
-        for (int x = x0; x < x1; x++) cnt += flag[x] != 0 ? 1u : 0u;
-        uint32_t inc = wave_incl_scan(cnt);
-        if (lane == 63) wsum[wave] = inc;
-        bsync<G>();
-        uint32_t id = inc - cnt;
-        for (int w = 0; w < wave; w++) id += wsum[w];
-        uint32_t nrec = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) nrec += wsum[w];
-        for (int x = x0; x < x1; x++) {
-            if (flag[x] != 0) {
-                flag[x] = id + 1;
-                Wn[id] = acc[x];
-                id++;
-            }
-        }
-        bsync<G>();
-        for (int i = tid; i < ncur; i += kFrontThreads) labB[i] = (uint16_t)(flag[p[m + i]] - 1);
-        bsync<G>();
-        m = (int)nrec;
         pbase = cbase;
         pcur = ncur;
         U32 *tw = W; W = Wn; Wn = tw;
         U16 *tl = labA; labA = labB; labB = tl;
     }
     // the last row's outputs
-    for (int i = tid; i < pcur; i += kFrontThreads) F.rlab[pbase + i] = labA[i];
-    for (int l = tid; l < m; l += kFrontThreads) F.recw[pbase + l] = W[l];
-    if (tid == 0) F.nrec[QH - 1] = (uint32_t)m;
+    for (int i = tid; i < pcur; i += kFrontThreads) {
+        const uint32_t l = labA[i];
+        F.rlab[pbase + i] = l;
+        if (l == (uint32_t)i) F.recw[pbase + i] = W[i];
+    }
 }
 
 template <int T>
@@ -374,6 +376,36 @@ __global__ __launch_bounds__(T) void k_front(const Frame *frames) {
         front_sweep<true, T>(F, F.fscr, small);
     else
         front_sweep<false, T>(F, lds, small);
+}
+
+// A wave per row: the front names each record of a row by its first run;
+// here the records are numbered densely in that order (rlab per run, recw
+// per record slot, nrec), the names mapped through rroot (free until the
+// replay, and for row 0, whose records are its runs, the identity it
+// keeps).
+__global__ __launch_bounds__(256) void k_front_dense(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int y = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (y >= F.qh) return;
+    const uint32_t b = F.row_off[y], n = F.rcnt[y];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t i = c0 + (uint32_t)lane;
+        const bool first = i < n && F.rlab[b + i] == i;
+        const uint64_t fb = __ballot(first);
+        const uint32_t d = carry + (uint32_t)__popcll(fb & lt);
+        const uint32_t w = first ? F.recw[b + i] : 0u;  // read before any lane of the chunk writes
+        if (first) {
+            F.rroot[b + i] = d;
+            F.recw[b + d] = w;  // d <= i: slots already read
+        }
+        carry += (uint32_t)__popcll(fb);
+    }
+    __threadfence_block();
+    for (uint32_t i = (uint32_t)lane; i < n; i += 64) F.rlab[b + i] = F.rroot[b + F.rlab[b + i]];
+    if (lane == 0) F.nrec[y] = carry;
 }
 
 // ---------------------------------------------------------------------
@@ -395,8 +427,8 @@ __global__ __launch_bounds__(T) void k_front(const Frame *frames) {
 // root of its set as replayed: a node or kSym | a previous record's slot).
 //
 // Rows of at most C = 512 runs (each of the two rows) and 2C links replay
-// in LDS (k_replay, a wave per row); the wider rows in a second k_replay
-// launch with LDS for the widest row of the call (C = qw, 3C links); rows
+// in LDS (k_replay, a wave per row); the wider rows, which it lists, in
+// k_replay_wide with LDS for the widest row of the call (C = qw, 3C links); rows
 // beyond that, and all rows when asked, in HBM scratch sized for the row
 // width (k_replay_hbm).
 
@@ -546,11 +578,22 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
         }
     }
     wsync<G>();
+    // the groups that have links, listed (in labA, free by now), so that
+    // the lanes share out those and not the rows' many linkless records
+    U16 *glist = labA;
+    int ngl = 0;
+    for (int g0 = 0; g0 < ng; g0 += 64) {
+        const int g = g0 + lane;
+        const bool has = g < ng && (uint32_t)head[g] != kNone;
+        const uint64_t hb = __ballot(has);
+        if (has) glist[ngl + __popcll(hb & ((1ull << lane) - 1ull))] = (uint16_t)g;
+        ngl += __popcll(hb);
+    }
+    wsync<G>();
     // each lane replays its groups' links in order; the current run's set
     // is kept in registers across its links, the next link is read ahead
-    for (int g = lane; g < ng; g += 64) {
-        uint32_t e = head[g];
-        if (e == kNone) continue;
+    for (int k = lane; k < ngl; k += 64) {
+        uint32_t e = head[glist[k]];
         int cur_i = -1;
         uint32_t ra = 0, wa = 0;
         uint32_t ed = le[e], en = nxt[e];
@@ -608,15 +651,35 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     for (int i = lane; i < ncur; i += 64) F.rroot[cb + rlab_cur[i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
 }
 
-// A wave per row, the rows that fit C = cap runs and lcap links but not
-// (when prev_cap > 0) the previous launch's prev_cap runs and 2 prev_cap links.
-__global__ __launch_bounds__(64) void k_replay(const Frame *frames, int prev_cap, int cap, int lcap) {
+// A wave per row, the rows that fit C = cap runs and 2C links; the others
+// are listed (rwide) for the wide launch.
+__global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
     extern __shared__ __align__(16) uint8_t lds[];
     const Frame &F = frames[blockIdx.y];
     const int y = 1 + (int)blockIdx.x;
-    if (y >= F.qh || !replay_fits(F, y, cap, lcap)) return;
-    if (prev_cap > 0 && replay_fits(F, y, prev_cap, 2 * prev_cap)) return;
-    replay_row<false>(F, y, lds, cap, lcap);
+    if (y >= F.qh) return;
+    if (!replay_fits(F, y, cap, 2 * cap)) {
+        if (threadIdx.x == 0) F.rwide[atomicAdd(F.counters + kNWideRows, 1u)] = (uint32_t)y;
+        return;
+    }
+    replay_row<false>(F, y, lds, cap, 2 * cap);
+}
+
+// The listed rows of every frame (list 3 of the work queue) that fit
+// C = cap runs and lcap links: persistent, blocks taking list entries in
+// strides (a frame cursor).
+__global__ __launch_bounds__(64) void k_replay_wide(const Frame *frames, const WorkQueue *Q, int cap, int lcap) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t total = Q->total[3];
+    const uint32_t *off = Q->off[3];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const int y = (int)F.rwide[g - off[fi]];
+        if (replay_fits(F, y, cap, lcap)) replay_row<false>(F, y, lds, cap, lcap);
+        __syncthreads();  // the next row reuses the memory
+    }
 }
 
 // The other rows (all rows for cap 0): wave b takes rows 1 + b + k W,
@@ -777,21 +840,24 @@ CclPlan ccl_plan(int max_lds_qw, int max_qw, bool force_scratch) {
         const size_t wb = replay_wide_lds_bytes(replay_wide_cap(max_qw));
         bool ok = wb <= 64 * 1024;
         if (!ok && wb <= 159 * 1024) {
-            ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_replay),
+            ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_replay_wide),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)wb) == hipSuccess;
             (void)hipGetLastError();
         }
         if (ok) {
             c.replay_wide_cap = replay_wide_cap(max_qw);
             c.replay_wide = (int)wb;
+            c.replay_wide_grid = (int)persistent_grid(reinterpret_cast<const void *>(&k_replay_wide), 64, wb, 1 << 16);
         }
     }
     c.replay_waves = force_scratch ? kReplayHbmWavesAll : kReplayHbmWaves;
     return c;
 }
 
+void launch_work_offsets(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q, int l, int which);
+
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, const CclPlan &c, const Tune &tu, Marks *mk) {
+                      int max_qh, const CclPlan &c, WorkQueue *Q, const Tune &tu, Marks *mk) {
     // per-row kernels: tu.row_waves rows (one wave each) per block, at
     // most 4: their launch bounds are 256 threads
     const int rw = std::min(std::max(tu.row_waves, 1), 4);
@@ -808,13 +874,15 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     default:
         hipLaunchKernelGGL(k_front<256>, dim3(1, nframes), dim3(256), (size_t)c.front, st, dframes);
     }
+    hipLaunchKernelGGL(k_front_dense, rg, rb, 0, st, dframes);
     mk->mark(st, "uf_front");
     const dim3 rows((unsigned)std::max(1, max_qh - 1), nframes);
-    if (c.replay_cap)
-        hipLaunchKernelGGL(k_replay, rows, dim3(64), (size_t)c.replay, st, dframes, 0, c.replay_cap, 2 * c.replay_cap);
-    if (c.replay_wide_cap)
-        hipLaunchKernelGGL(k_replay, rows, dim3(64), (size_t)c.replay_wide, st, dframes, c.replay_cap,
-                           c.replay_wide_cap, 3 * c.replay_wide_cap);
+    if (c.replay_cap) hipLaunchKernelGGL(k_replay, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
+    if (c.replay_wide_cap) {
+        launch_work_offsets(st, dframes, nframes, Q, 3, (int)kNWideRows);
+        hipLaunchKernelGGL(k_replay_wide, dim3((unsigned)c.replay_wide_grid), dim3(64), (size_t)c.replay_wide, st,
+                           dframes, Q, c.replay_wide_cap, 3 * c.replay_wide_cap);
+    }
     const int hcap = c.replay_wide_cap ? c.replay_wide_cap : c.replay_cap;
     const int hlcap = c.replay_wide_cap ? 3 * c.replay_wide_cap : 2 * c.replay_cap;
     hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, hcap, hlcap);

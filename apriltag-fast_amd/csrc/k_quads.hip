@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "atg_internal.h"
 
@@ -447,6 +448,10 @@ __global__ __launch_bounds__(256) void k_work_offsets(const Frame *frames, int n
         Q->off[l][nframes] = carry;
         Q->total[l] = carry;
     }
+}
+
+void launch_work_offsets(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q, int l, int which) {
+    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, l, which);
 }
 
 // ------------------------------------------------------------ angle sort
@@ -1209,11 +1214,11 @@ void warm_decode_grid(int fit_blocks);
 
 unsigned persistent_grid(const void *fn, int threads, size_t lds, int want) {
     static std::mutex mu;
-    static std::map<std::pair<const void *, int>, int> cap;
+    static std::map<std::tuple<const void *, int, size_t>, int> cap;
     int resident;
     {
         std::lock_guard<std::mutex> g(mu);
-        auto it = cap.find({fn, threads});
+        auto it = cap.find({fn, threads, lds});
         if (it == cap.end()) {
             int dev = 0, cus = 0, per_cu = 0;
             if (hipGetDevice(&dev) != hipSuccess ||
@@ -1221,7 +1226,7 @@ unsigned persistent_grid(const void *fn, int threads, size_t lds, int want) {
                 hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess)
                 cus = per_cu = 0;
             (void)hipGetLastError();
-            it = cap.emplace(std::make_pair(fn, threads), std::max(1, per_cu * cus)).first;
+            it = cap.emplace(std::make_tuple(fn, threads, lds), std::max(1, per_cu * cus)).first;
         }
         resident = it->second;
     }

@@ -25,7 +25,7 @@ CclPlan ccl_plan(int max_lds_qw, int max_qw, bool force_scratch);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                         int max_qh, bool blur, bool deglitch, const Tune &tu, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
-                      int max_qh, const CclPlan &c, const Tune &tu, Marks *mk);
+                      int max_qh, const CclPlan &c, WorkQueue *Q, const Tune &tu, Marks *mk);
 void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
                          Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
@@ -73,7 +73,7 @@ int64_t cluster_map_size(int64_t w, int64_t h) {
 // The persistent kernels' work lists of a call (in the arena, after the
 // frames' regions): the header and three offset arrays over the frames.
 size_t queue_bytes(int64_t nframes) {
-    return align256(sizeof(WorkQueue)) + 3 * align256((size_t)(nframes + 1) * 4);
+    return align256(sizeof(WorkQueue)) + 4 * align256((size_t)(nframes + 1) * 4);
 }
 
 // Kept clusters need at least a few points each, so a quarter of the point
@@ -333,6 +333,7 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->sval = (uint32_t *)take((size_t)pc * 4);
     F->sval2 = (uint32_t *)take((size_t)pc * 4);
     F->rhist = (uint32_t *)take((size_t)256 * kSortBlocks * 4);
+    F->rwide = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->rdense = (uint32_t *)take((size_t)nodes * 4);
     F->rinv = (uint32_t *)take((size_t)nodes * 4);
     const size_t fs = front_scratch_bytes((int)qw, fp.force);
@@ -535,7 +536,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         uint8_t *qb = ctx->arena + qoff + align256(sizeof(WorkQueue));
         WorkQueue hq{};
         hq.nframes = (uint32_t)n;
-        for (int l = 0; l < 3; l++)
+        for (int l = 0; l < 4; l++)
             hq.off[l] = reinterpret_cast<uint32_t *>(qb + (size_t)l * align256((size_t)(n + 1) * 4));
         ctx->qstage = hq;
     }
@@ -602,7 +603,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         launch_init(st, ctx->dframes, n, dq);
         mk->mark(st, "init");
         launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, blur, deglitch, tu, mk);
-        launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, tu, mk);
+        launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, dq, tu, mk);
         launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
         launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
         launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, dq, mk);
@@ -616,7 +617,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         // one graph per call shape: everything the stream work depends on
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
-                                    ccl.replay_cap, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_waves,        max_lds_qw,
+                                    ccl.replay_cap, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid, ccl.replay_waves,        max_lds_qw,
                                     tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
