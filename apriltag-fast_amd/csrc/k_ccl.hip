@@ -802,11 +802,15 @@ static size_t front_lds_limit() {
     static size_t lim = 0;
     if (!lim) {
         const int big = 159 * 1024;
-        bool ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<128>),
+        bool ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<64>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
+                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<128>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
                   hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<256>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
                   hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<512>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
+                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<1024>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess;
         (void)hipGetLastError();
         lim = ok ? (size_t)big : (size_t)64 * 1024;
@@ -865,11 +869,17 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     hipLaunchKernelGGL(k_uf_edges, rg, rb, 0, st, dframes);
     mk->mark(st, "uf_edges");
     switch (tu.front_threads) {
+    case 64:
+        hipLaunchKernelGGL(k_front<64>, dim3(1, nframes), dim3(64), (size_t)c.front, st, dframes);
+        break;
     case 128:
         hipLaunchKernelGGL(k_front<128>, dim3(1, nframes), dim3(128), (size_t)c.front, st, dframes);
         break;
     case 512:
         hipLaunchKernelGGL(k_front<512>, dim3(1, nframes), dim3(512), (size_t)c.front, st, dframes);
+        break;
+    case 1024:
+        hipLaunchKernelGGL(k_front<1024>, dim3(1, nframes), dim3(1024), (size_t)c.front, st, dframes);
         break;
     default:
         hipLaunchKernelGGL(k_front<256>, dim3(1, nframes), dim3(256), (size_t)c.front, st, dframes);

@@ -43,7 +43,7 @@ constexpr int kResultDets = kMaxDetsPerFrame;  // detection capacity of a frame'
 constexpr int kInlineDets = 32;            // read back with the counters
 constexpr size_t kCountersBytes = 64;
 constexpr size_t kResultBlock = kCountersBytes + (size_t)kResultDets * sizeof(at_detection);
-constexpr Tune kDefaultTune = {1, 4, 256, 1024};
+constexpr Tune kDefaultTune = {1, 4, 1024, 1024};
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -843,7 +843,7 @@ static bool valid_tune(const Tune &t) {
     auto in = [](int v, std::initializer_list<int> l) { return std::find(l.begin(), l.end(), v) != l.end(); };
     // row_waves: the per-row kernels are compiled for blocks of at most 256
     // threads (4 waves)
-    return in(t.thr_tile_rows, {1, 2, 4}) && in(t.row_waves, {1, 2, 4}) && in(t.front_threads, {128, 256, 512}) &&
+    return in(t.thr_tile_rows, {1, 2, 4}) && in(t.row_waves, {1, 2, 4}) && in(t.front_threads, {64, 128, 256, 512, 1024}) &&
            t.fit_blocks >= 32 && t.fit_blocks <= 4096;
 }
 
@@ -912,7 +912,7 @@ extern "C" int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *
     Tune best = start;
     double bms = 0.0;
     int rc = measure(best, &bms);
-    const std::vector<int> cand[4] = {{1, 2, 4}, {1, 2, 4}, {128, 256, 512}, {256, 512, 1024, 2048, 4096}};
+    const std::vector<int> cand[4] = {{1, 2, 4}, {1, 2, 4}, {128, 256, 512, 1024}, {256, 512, 1024, 2048, 4096}};
     for (int k = 0; k < 4 && rc == ATG_OK; k++) {
         for (int v : cand[k]) {
             Tune t = best;

@@ -122,8 +122,9 @@ int atg_debug_counters(atg_ctx *ctx, int32_t frame, uint32_t *out16);
 
 /* Work splits of the kernel chain (any values give the same detections):
  * threshold tile rows per block (1, 2, 4), rows per block of the per-row
- * kernels (1, 2, 4), threads of the union-find front sweep (128, 256,
- * 512), blocks per frame of the fit and decode kernels (32..1024). */
+ * kernels (1, 2, 4), threads of the union-find front sweep (64, 128, 256,
+ * 512, 1024), blocks of the persistent fit and decode kernels (32..4096; at most
+ * as many as are resident at once are launched). */
 typedef struct {
     int32_t thr_tile_rows, row_waves, front_threads, fit_blocks;
 } atg_tune;
