@@ -79,21 +79,28 @@ __device__ __forceinline__ double bilinear1(const View &v, int64_t xi, int64_t y
     return r;
 }
 
-// One sample point of an edge: the weighted mean offset along the normal.
-__device__ bool refine_sample(const View &v, int64_t range, int64_t max_steps, int64_t vsteps, double x0, double y0,
-                              double nx, double ny, double *bx, double *by) {
-    double rf = (double)-range;
-    double ln[4] = {0, 0, 0, 0}, lc[4] = {0, 0, 0, 0};
-    // vsteps is a multiple of 4: step st adds to lane st & 3, in order
+// One sample point of an edge: the weighted mean offset along the normal
+// (fast/decode.c:690-730). The steps below vsteps (a multiple of 4) add to
+// four partial sums by step & 3, in order; refine_partial makes partial q
+// (on its own lane), refine_finish combines the four as the reference
+// does, adds the remaining steps in order and gives the sample's point.
+__device__ __forceinline__ void refine_partial(const View &v, int64_t range, int64_t vsteps, double x0, double y0,
+                                               double nx, double ny, int q, double *ln, double *lc) {
+    const double rf = (double)-range;
+    double a = 0.0, c = 0.0;
     for (int64_t s0 = 0; s0 < vsteps; s0 += 4) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            double n = DADD(rf, DMUL(0.25, (double)(s0 + q)));
-            double w = lane_weight(v, x0, y0, nx, ny, n);
-            ln[q] = DADD(ln[q], DMUL(w, n));
-            lc[q] = DADD(lc[q], w);
-        }
+        const double n = DADD(rf, DMUL(0.25, (double)(s0 + q)));
+        const double w = lane_weight(v, x0, y0, nx, ny, n);
+        a = DADD(a, DMUL(w, n));
+        c = DADD(c, w);
     }
+    *ln = a;
+    *lc = c;
+}
+
+__device__ bool refine_finish(const View &v, int64_t range, int64_t max_steps, int64_t vsteps, double x0, double y0,
+                              double nx, double ny, const double ln[4], const double lc[4], double *bx, double *by) {
+    const double rf = (double)-range;
     double mn = DADD(0.0, DADD(DADD(ln[0], ln[2]), DADD(ln[1], ln[3])));
     double mc = DADD(0.0, DADD(DADD(lc[0], lc[2]), DADD(lc[1], lc[3])));
     for (int64_t st = vsteps; st < max_steps; st++) {
@@ -319,6 +326,9 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
     __shared__ int s_hok, s_piv, s_polar;
     __shared__ Gray g_white, g_black;
     __shared__ double vals[12 * 12], sharp[12 * 12];
+    __shared__ uint64_t s_rc;
+    __shared__ float s_margin;
+    __shared__ int s_mj[16], s_mh[16];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int nfam = P->nfam;
     const uint32_t total = Q->total[2];
@@ -340,8 +350,9 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
         }
         __syncthreads();
         if (P->refine) {
-            // refine_edges (fast/decode.c:657-761): wave e takes edge e; its
-            // lanes sample, lane 0 adds the samples up in order
+            // refine_edges (fast/decode.c:657-761): wave e takes edge e;
+            // four lanes a sample (one partial sum each), 16 samples at a
+            // time; lane 0 adds the samples up in order
             const int64_t range = (int64_t)(P->decimate + 1.0f);
             const int64_t max_steps = 2 * 4 * range + 1;
             int64_t vsteps = max_steps - max_steps % 4;
@@ -356,22 +367,35 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
             int64_t ns = ctrunc(DDIV(mag, 8.0));
             if (ns < 16) ns = 16;
             double mx = 0.0, my = 0.0, mxx = 0.0, mxy = 0.0, myy = 0.0, nn = 0.0;
-            for (int64_t c0 = 0; c0 < ns; c0 += 64) {
-                const int64_t si = c0 + lane;
-                int ok = 0;
+            const int part = lane & 3, sl = lane >> 2;
+            for (int64_t c0 = 0; c0 < ns; c0 += 16) {
+                const int64_t si = c0 + sl;
+                double x0 = 0.0, y0 = 0.0, ln = 0.0, lc = 0.0;
                 if (si < ns) {
                     double alpha = DDIV(DADD(1.0, (double)si), (double)(ns + 1));
-                    double x0 = DADD(DMUL(alpha, (double)qp[ea][0]), DMUL(DSUB(1.0, alpha), (double)qp[eb][0]));
-                    double y0 = DADD(DMUL(alpha, (double)qp[ea][1]), DMUL(DSUB(1.0, alpha), (double)qp[eb][1]));
-                    double bx = 0, by = 0;
-                    ok = refine_sample(v, range, max_steps, vsteps, x0, y0, nx, ny, &bx, &by) ? 1 : 0;
-                    wsx[wv][lane] = bx;
-                    wsy[wv][lane] = by;
+                    x0 = DADD(DMUL(alpha, (double)qp[ea][0]), DMUL(DSUB(1.0, alpha), (double)qp[eb][0]));
+                    y0 = DADD(DMUL(alpha, (double)qp[ea][1]), DMUL(DSUB(1.0, alpha), (double)qp[eb][1]));
+                    refine_partial(v, range, vsteps, x0, y0, nx, ny, part, &ln, &lc);
                 }
-                wsv[wv][lane] = ok;
+                double lns[4], lcs[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    lns[k] = __shfl(ln, (lane & ~3) + k, 64);
+                    lcs[k] = __shfl(lc, (lane & ~3) + k, 64);
+                }
+                if (part == 0) {
+                    int ok = 0;
+                    if (si < ns) {
+                        double bx = 0, by = 0;
+                        ok = refine_finish(v, range, max_steps, vsteps, x0, y0, nx, ny, lns, lcs, &bx, &by) ? 1 : 0;
+                        wsx[wv][sl] = bx;
+                        wsy[wv][sl] = by;
+                    }
+                    wsv[wv][sl] = ok;
+                }
                 wave_sync();
                 if (lane == 0) {
-                    const int64_t len = ns - c0 < 64 ? ns - c0 : 64;
+                    const int64_t len = ns - c0 < 16 ? ns - c0 : 16;
                     for (int64_t k = 0; k < len; k++) {
                         if (!wsv[wv][k]) continue;
                         const double bx = wsx[wv][k], by = wsy[wv][k];
@@ -524,29 +548,45 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     }
                     double xm = (double)fdiv_cr(ws, wc), ym = (double)fdiv_cr(bs, bc);
                     double mg = xm != xm ? ym : ym != ym ? xm : xm < ym ? xm : ym;
-                    float margin = (float)mg;
-                    // quick decode (fast/decode.c:87-107)
-                    int64_t id = 65535, ham = 255, rot = 0;
-                    uint64_t rcode = rc;
-                    uint64_t cmask = (uint64_t)(fm.capacity - 1);
-                    bool found = false;
-                    for (int r = 0; r < 4 && !found; r++) {
-                        for (int i = 0; i < 4 && !found; i++) {
-                            int64_t val = (int64_t)((rcode >> (i * fm.chunk_size)) & cmask);
-                            int64_t ob = i * (fm.capacity + 1) + val;
-                            int64_t basei = (int64_t)i * fm.ncodes;
-                            for (int64_t j = basei + fm.offsets[ob]; j < basei + fm.offsets[ob + 1]; j++) {
-                                int64_t hm = __popcll(fm.codes[j] ^ rcode);
-                                if (hm <= fm.maxhamming) {
-                                    id = fm.ids[j];
-                                    ham = hm;
-                                    rot = r;
-                                    found = true;
-                                    break;
-                                }
-                            }
+                    s_rc = rc;
+                    s_margin = (float)mg;
+                }
+                __syncthreads();
+                // quick decode (fast/decode.c:87-107): the 16 (rotation,
+                // chunk) lookups at once, thread 4 r + i each; the first
+                // match in the reference's order (rotation, chunk, entry)
+                // is the lowest thread's
+                if (t < 16) {
+                    const int r = t >> 2, i = t & 3;
+                    uint64_t rcode = s_rc;
+                    for (int k = 0; k < r; k++) rcode = rotate90(rcode, fm.nbits);
+                    const uint64_t cmask = (uint64_t)(fm.capacity - 1);
+                    const int64_t val = (int64_t)((rcode >> (i * fm.chunk_size)) & cmask);
+                    const int64_t ob = i * (fm.capacity + 1) + val;
+                    const int64_t basei = (int64_t)i * fm.ncodes;
+                    int mj = -1, mh = 255;
+                    for (int64_t j = basei + fm.offsets[ob]; j < basei + fm.offsets[ob + 1]; j++) {
+                        const int64_t hm = __popcll(fm.codes[j] ^ rcode);
+                        if (hm <= fm.maxhamming) {
+                            mj = (int)j;
+                            mh = (int)hm;
+                            break;
                         }
-                        if (!found) rcode = rotate90(rcode, fm.nbits);
+                    }
+                    s_mj[t] = mj;
+                    s_mh[t] = mh;
+                }
+                __syncthreads();
+                if (t == 0) {
+                    const float margin = s_margin;
+                    int64_t id = 65535, ham = 255, rot = 0;
+                    for (int k = 0; k < 16; k++) {
+                        if (s_mj[k] >= 0) {
+                            id = fm.ids[s_mj[k]];
+                            ham = s_mh[k];
+                            rot = k >> 2;
+                            break;
+                        }
                     }
                     if (margin >= 0.0f && ham < 255) {
                         // make_detection (fast/detector.c:83-107)
