@@ -2,16 +2,18 @@
 // homography, bit sampling and code lookup; per frame, the overlap
 // reconcile and the stable sort by id.
 //
-//   k_decode   persistent blocks over the quads of all frames. refine_edges
-//              (fast/decode.c:657-761): a wave per edge, a sample per lane
-//              with the reference's 4-lane step sums (:549-626), the edge's
-//              sums added in sample order by its lane 0. The homography's
-//              elimination runs in LDS, its row updates in parallel. The
-//              bit sampler (:291-402) gathers its pixel samples in parallel;
-//              the gray models and the scores are added in the reference's
-//              order by one thread; each cell's threshold and sharpening are
-//              a thread's. Lookup walks the quick-decode bucket of each
-//              chunk (:87-107).
+//   k_decode   persistent, a wave a quad (or, for calls of a few frames, a
+//              256-thread block with a wave per edge), over the quads of all
+//              frames. refine_edges (fast/decode.c:657-761): the edges in turn, four
+//              lanes a sample, each lane one of the reference's four step
+//              sums (:549-626), the edge's sums added in sample order by
+//              lane 0. The homography's elimination runs in LDS, its row
+//              updates in parallel. The bit sampler (:291-402) gathers its
+//              pixel samples in parallel; the gray models and the scores are
+//              added in the reference's order by one lane; each cell's
+//              threshold and sharpening are a lane's. Lookup (:87-107): the
+//              quick-decode buckets of the 16 (rotation, chunk) pairs at
+//              once, the first match in the reference's order kept.
 //   k_finish   one block per frame: the detections in (quad, family)
 //              order, reconcile (:860-891), stable sort by id
 //              (fast/detector.c:164-179).
@@ -125,6 +127,19 @@ __device__ bool refine_finish(const View &v, int64_t range, int64_t max_steps, i
     return true;
 }
 
+// A barrier of a block of NT threads over LDS; for one wave, only the
+// ordering of its LDS operations.
+template <int NT>
+__device__ __forceinline__ void bsync_t() {
+    if constexpr (NT == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // homography_compute2 (fast/decode.c:114-165): the 8x9 system of the four
 // corner correspondences, Gaussian elimination with partial pivoting, in
 // LDS. Each pivot step's row updates are independent element by element,
@@ -132,10 +147,11 @@ __device__ bool refine_finish(const View &v, int64_t range, int64_t max_steps, i
 // the reference's loops); the pivot search and back substitution are one
 // thread's. c[i] = (x, y) of the tag corner, qp[i] its image point. Every
 // thread returns the same result; hr is written by thread 0.
+template <int NT>
 __device__ bool homography_block(const float qp[4][2], double *A, int *s_piv, double *hr) {
     const int t = threadIdx.x;
-    if (t < 72) {
-        const int row = t / 9, col = t % 9, i = row >> 1;
+    for (int e = t; e < 72; e += NT) {
+        const int row = e / 9, col = e % 9, i = row >> 1;
         const double cx = (i == 0 || i == 3) ? -1.0 : 1.0, cy = (i == 0 || i == 1) ? -1.0 : 1.0;
         const double px = (double)qp[i][0], py = (double)qp[i][1];
         const double pv = (row & 1) ? py : px;  // the image coordinate of this row
@@ -147,9 +163,9 @@ __device__ bool homography_block(const float qp[4][2], double *A, int *s_piv, do
         else if (col == 6) v = DMUL(-cx, pv);
         else if (col == 7) v = DMUL(-cy, pv);
         else if (col == 8) v = pv;
-        A[t] = v;
+        A[e] = v;
     }
-    __syncthreads();
+    bsync_t<NT>();
     bool ok = true;
     for (int col = 0; col < 8; col++) {
         if (t == 0) {
@@ -161,7 +177,7 @@ __device__ bool homography_block(const float qp[4][2], double *A, int *s_piv, do
             }
             *s_piv = (mi < 0 || mv < 1e-10) ? -1 : mi;
         }
-        __syncthreads();
+        bsync_t<NT>();
         const int mi = *s_piv;
         if (mi < 0) { ok = false; break; }
         if (mi != col && t >= col && t < 9) {
@@ -169,20 +185,29 @@ __device__ bool homography_block(const float qp[4][2], double *A, int *s_piv, do
             A[col * 9 + t] = A[mi * 9 + t];
             A[mi * 9 + t] = x;
         }
-        __syncthreads();
-        double nv = 0.0;
-        bool wr = false;
-        if (t < 72) {
-            const int i = t / 9, j = t % 9;
-            if (i > col && j >= col) {
-                double f = DDIV(A[i * 9 + col], A[col * 9 + col]);
-                nv = j == col ? 0.0 : DSUB(A[i * 9 + j], DMUL(f, A[col * 9 + j]));
-                wr = true;
+        bsync_t<NT>();
+        constexpr int kR = (72 + NT - 1) / NT;  // elements a thread
+        double nv[kR];
+        bool wr[kR];
+#pragma unroll
+        for (int r = 0; r < kR; r++) {
+            const int e = t + r * NT;
+            nv[r] = 0.0;
+            wr[r] = false;
+            if (e < 72) {
+                const int i = e / 9, j = e % 9;
+                if (i > col && j >= col) {
+                    double f = DDIV(A[i * 9 + col], A[col * 9 + col]);
+                    nv[r] = j == col ? 0.0 : DSUB(A[i * 9 + j], DMUL(f, A[col * 9 + j]));
+                    wr[r] = true;
+                }
             }
         }
-        __syncthreads();
-        if (wr) A[t] = nv;
-        __syncthreads();
+        bsync_t<NT>();
+#pragma unroll
+        for (int r = 0; r < kR; r++)
+            if (wr[r]) A[t + r * NT] = nv[r];
+        bsync_t<NT>();
     }
     if (ok && t == 0) {
         for (int col = 7; col >= 0; col--) {
@@ -289,7 +314,9 @@ __device__ __forceinline__ uint64_t rotate90(uint64_t w0, int64_t nbits) {
     return w & (((uint64_t)1 << nbits) - 1);
 }
 
-constexpr int kMaxSamples = 512;  // bit-sampler points: 8 * wab + nbits
+// Bit-sampler points, 8 wab + nbits: at most 8 * 12 + 64 for the families
+// decoded here (total width up to 12, width at border no more).
+constexpr int kMaxSamples = 160;
 
 // The border sampling patterns of quad_sample (fast/decode.c:296-305):
 // start x, start y, step x, step y, and whether the side is white.
@@ -306,18 +333,15 @@ __device__ __forceinline__ void border_pattern(int pi, int wab, float *x0, float
     *white = (k & 1) == 0;
 }
 
-// A barrier of one wave over its LDS operations.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Persistent: the quads of every frame, block b taking global indices b,
 // b + G, ... (a frame cursor).
-__global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
-    __shared__ double wsx[4][64], wsy[4][64];
-    __shared__ int wsv[4][64];
+// NT threads a quad: 256 (a wave per edge in the refinement) or one wave
+// (the edges in turn).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
+    __shared__ double wsx[4][16], wsy[4][16];  // an edge's refined sample points, 16 at a time
+    __shared__ int wsv[4][16];
     __shared__ double smp[3][kMaxSamples];  // tag x, tag y, value
     __shared__ int sflag[kMaxSamples];
     __shared__ float qp[4][2];
@@ -329,7 +353,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
     __shared__ uint64_t s_rc;
     __shared__ float s_margin;
     __shared__ int s_mj[16], s_mh[16];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int t = threadIdx.x, lane = t & 63;
     const int nfam = P->nfam;
     const uint32_t total = Q->total[2];
     const uint32_t *off = Q->off[2];
@@ -348,77 +372,85 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                 qp[i][1] = f > 1.0f ? __fmul_rn(q.p[i][1], f) : q.p[i][1];
             }
         }
-        __syncthreads();
+        bsync_t<NT>();
         if (P->refine) {
-            // refine_edges (fast/decode.c:657-761): wave e takes edge e;
+            // refine_edges (fast/decode.c:657-761): edge e on wave e of a
+            // 256-thread block, or the edges in turn on a one-wave block;
             // four lanes a sample (one partial sum each), 16 samples at a
             // time; lane 0 adds the samples up in order
-            const int64_t range = (int64_t)(P->decimate + 1.0f);
-            const int64_t max_steps = 2 * 4 * range + 1;
-            int64_t vsteps = max_steps - max_steps % 4;
-            if (vsteps < 0) vsteps = 0;
-            const int ea = wv, eb = (wv + 1) & 3;
-            double nx = (double)__fsub_rn(qp[eb][1], qp[ea][1]);
-            double ny = (double)__fadd_rn(-qp[eb][0], qp[ea][0]);
-            const double mag = __dsqrt_rn(DADD(DMUL(nx, nx), DMUL(ny, ny)));
-            nx = DDIV(nx, mag);
-            ny = DDIV(ny, mag);
-            if (q.reversed) { nx = -nx; ny = -ny; }
-            int64_t ns = ctrunc(DDIV(mag, 8.0));
-            if (ns < 16) ns = 16;
-            double mx = 0.0, my = 0.0, mxx = 0.0, mxy = 0.0, myy = 0.0, nn = 0.0;
-            const int part = lane & 3, sl = lane >> 2;
-            for (int64_t c0 = 0; c0 < ns; c0 += 16) {
-                const int64_t si = c0 + sl;
-                double x0 = 0.0, y0 = 0.0, ln = 0.0, lc = 0.0;
-                if (si < ns) {
-                    double alpha = DDIV(DADD(1.0, (double)si), (double)(ns + 1));
-                    x0 = DADD(DMUL(alpha, (double)qp[ea][0]), DMUL(DSUB(1.0, alpha), (double)qp[eb][0]));
-                    y0 = DADD(DMUL(alpha, (double)qp[ea][1]), DMUL(DSUB(1.0, alpha), (double)qp[eb][1]));
-                    refine_partial(v, range, vsteps, x0, y0, nx, ny, part, &ln, &lc);
-                }
-                double lns[4], lcs[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    lns[k] = __shfl(ln, (lane & ~3) + k, 64);
-                    lcs[k] = __shfl(lc, (lane & ~3) + k, 64);
-                }
-                if (part == 0) {
-                    int ok = 0;
+            auto refine_edge = [&](const int e) {
+                const int64_t range = (int64_t)(P->decimate + 1.0f);
+                const int64_t max_steps = 2 * 4 * range + 1;
+                int64_t vsteps = max_steps - max_steps % 4;
+                if (vsteps < 0) vsteps = 0;
+                const int ea = e, eb = (e + 1) & 3;
+                double nx = (double)__fsub_rn(qp[eb][1], qp[ea][1]);
+                double ny = (double)__fadd_rn(-qp[eb][0], qp[ea][0]);
+                const double mag = __dsqrt_rn(DADD(DMUL(nx, nx), DMUL(ny, ny)));
+                nx = DDIV(nx, mag);
+                ny = DDIV(ny, mag);
+                if (q.reversed) { nx = -nx; ny = -ny; }
+                int64_t ns = ctrunc(DDIV(mag, 8.0));
+                if (ns < 16) ns = 16;
+                double mx = 0.0, my = 0.0, mxx = 0.0, mxy = 0.0, myy = 0.0, nn = 0.0;
+                const int part = lane & 3, sl = lane >> 2;
+                for (int64_t c0 = 0; c0 < ns; c0 += 16) {
+                    const int64_t si = c0 + sl;
+                    double x0 = 0.0, y0 = 0.0, ln = 0.0, lc = 0.0;
                     if (si < ns) {
-                        double bx = 0, by = 0;
-                        ok = refine_finish(v, range, max_steps, vsteps, x0, y0, nx, ny, lns, lcs, &bx, &by) ? 1 : 0;
-                        wsx[wv][sl] = bx;
-                        wsy[wv][sl] = by;
+                        double alpha = DDIV(DADD(1.0, (double)si), (double)(ns + 1));
+                        x0 = DADD(DMUL(alpha, (double)qp[ea][0]), DMUL(DSUB(1.0, alpha), (double)qp[eb][0]));
+                        y0 = DADD(DMUL(alpha, (double)qp[ea][1]), DMUL(DSUB(1.0, alpha), (double)qp[eb][1]));
+                        refine_partial(v, range, vsteps, x0, y0, nx, ny, part, &ln, &lc);
                     }
-                    wsv[wv][sl] = ok;
+                    double lns[4], lcs[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        lns[k] = __shfl(ln, (lane & ~3) + k, 64);
+                        lcs[k] = __shfl(lc, (lane & ~3) + k, 64);
+                    }
+                    if (part == 0) {
+                        int ok = 0;
+                        if (si < ns) {
+                            double bx = 0, by = 0;
+                            ok = refine_finish(v, range, max_steps, vsteps, x0, y0, nx, ny, lns, lcs, &bx, &by) ? 1 : 0;
+                            wsx[e][sl] = bx;
+                            wsy[e][sl] = by;
+                        }
+                        wsv[e][sl] = ok;
+                    }
+                    bsync_t<64>();
+                    if (lane == 0) {
+                        const int64_t len = ns - c0 < 16 ? ns - c0 : 16;
+                        for (int64_t k = 0; k < len; k++) {
+                            if (!wsv[e][k]) continue;
+                            const double bx = wsx[e][k], by = wsy[e][k];
+                            mx = DADD(mx, bx); my = DADD(my, by);
+                            mxx = DADD(mxx, DMUL(bx, bx)); mxy = DADD(mxy, DMUL(bx, by)); myy = DADD(myy, DMUL(by, by));
+                            nn = DADD(nn, 1.0);
+                        }
+                    }
+                    bsync_t<64>();
                 }
-                wave_sync();
                 if (lane == 0) {
-                    const int64_t len = ns - c0 < 16 ? ns - c0 : 16;
-                    for (int64_t k = 0; k < len; k++) {
-                        if (!wsv[wv][k]) continue;
-                        const double bx = wsx[wv][k], by = wsy[wv][k];
-                        mx = DADD(mx, bx); my = DADD(my, by);
-                        mxx = DADD(mxx, DMUL(bx, bx)); mxy = DADD(mxy, DMUL(bx, by)); myy = DADD(myy, DMUL(by, by));
-                        nn = DADD(nn, 1.0);
-                    }
+                    double ex = DDIV(mx, nn), ey = DDIV(my, nn);
+                    double cxx = DSUB(DDIV(mxx, nn), DMUL(ex, ex));
+                    double cxy = DSUB(DDIV(mxy, nn), DMUL(ex, ey));
+                    double cyy = DSUB(DDIV(myy, nn), DMUL(ey, ey));
+                    float th2 = (float)atan2((double)(float)DMUL(-2.0, cxy), (double)(float)DSUB(cyy, cxx));
+                    double theta = DMUL(0.5, (double)th2);
+                    lines[e][0] = ex;
+                    lines[e][1] = ey;
+                    lines[e][2] = (double)(float)cos((double)(float)theta);
+                    lines[e][3] = (double)(float)sin((double)(float)theta);
                 }
-                wave_sync();
+            };
+            if constexpr (NT == 64) {
+                for (int e = 0; e < 4; e++) refine_edge(e);
+            } else {
+                refine_edge(t >> 6);
             }
-            if (lane == 0) {
-                double ex = DDIV(mx, nn), ey = DDIV(my, nn);
-                double cxx = DSUB(DDIV(mxx, nn), DMUL(ex, ex));
-                double cxy = DSUB(DDIV(mxy, nn), DMUL(ex, ey));
-                double cyy = DSUB(DDIV(myy, nn), DMUL(ey, ey));
-                float th2 = (float)atan2((double)(float)DMUL(-2.0, cxy), (double)(float)DSUB(cyy, cxx));
-                double theta = DMUL(0.5, (double)th2);
-                lines[wv][0] = ex;
-                lines[wv][1] = ey;
-                lines[wv][2] = (double)(float)cos((double)(float)theta);
-                lines[wv][3] = (double)(float)sin((double)(float)theta);
-            }
-            __syncthreads();
+            bsync_t<NT>();
             if (t == 0) {
                 for (int i = 0; i < 4; i++) {
                     int j = (i + 1) & 3;
@@ -433,12 +465,12 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     }
                 }
             }
-            __syncthreads();
+            bsync_t<NT>();
         }
         // homography (fast/decode.c:198-210) and its invertibility
-        bool hok = homography_block(qp, sA, &s_piv, sH);
+        bool hok = homography_block<NT>(qp, sA, &s_piv, sH);
         if (t == 0) s_hok = hok && invertible(sH, sLU) ? 1 : 0;
-        __syncthreads();
+        bsync_t<NT>();
         for (int fam = 0; fam < nfam; fam++) {
             if (t == 0) F.dflag[qi * nfam + fam] = 0;
             const DevFamily &fm = P->fam[fam];
@@ -448,7 +480,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
             const int nb = 8 * wab;
             if (nb + fm.nbits > kMaxSamples || fm.tw > 12) continue;
             // border samples (fast/decode.c:310-326), in parallel
-            for (int k = t; k < nb; k += 256) {
+            for (int k = t; k < nb; k += NT) {
                 const int pi = k / wab, i = k % wab;
                 float bx0, by0, bdx, bdy;
                 bool white;
@@ -473,7 +505,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
             }
             // bit samples (fast/decode.c:346-362)
             const int mc = (wab - fm.tw) / 2;
-            for (int i = t; i < fm.nbits; i += 256) {
+            for (int i = t; i < fm.nbits; i += NT) {
                 int k = nb + i;
                 double tx01 = DDIV(DADD((double)fm.bit_x[i], 0.5), (double)wab);
                 double ty01 = DDIV(DADD((double)fm.bit_y[i], 0.5), (double)wab);
@@ -485,7 +517,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                 smp[1][k] = ty;
                 smp[2][k] = pixel_value(v, px, py);
             }
-            __syncthreads();
+            bsync_t<NT>();
             // the gray models, in sample order (fast/decode.c:229-265)
             if (t == 0) {
                 Gray wh, bl;
@@ -506,13 +538,13 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                 g_white = wh;
                 g_black = bl;
             }
-            __syncthreads();
+            bsync_t<NT>();
             if (s_polar) {
                 const int tw = fm.tw;
-                for (int i = t; i < tw * tw; i += 256) vals[i] = 0.0;
-                __syncthreads();
+                for (int i = t; i < tw * tw; i += NT) vals[i] = 0.0;
+                bsync_t<NT>();
                 // each bit's value against the models' threshold, in its cell
-                for (int i = t; i < fm.nbits; i += 256) {
+                for (int i = t; i < fm.nbits; i += NT) {
                     double val = smp[2][nb + i];
                     if (val == -1.0) continue;
                     double tx = smp[0][nb + i], ty = smp[1][nb + i];
@@ -522,9 +554,9 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                                      2.0);
                     vals[tw * (fm.bit_y[i] - mc) + (fm.bit_x[i] - mc)] = DSUB(val, th);
                 }
-                __syncthreads();
+                bsync_t<NT>();
                 // sharpening (fast/decode.c:370-385), each cell in tap order
-                for (int i = t; i < tw * tw; i += 256) {
+                for (int i = t; i < tw * tw; i += NT) {
                     const int y = i / tw, x = i % tw;
                     double acc = 0.0;
                     if (y > 0) acc = DADD(acc, -vals[(y - 1) * tw + x]);
@@ -534,9 +566,9 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     if (y + 1 < tw) acc = DADD(acc, -vals[(y + 1) * tw + x]);
                     sharp[i] = acc;
                 }
-                __syncthreads();
-                for (int i = t; i < tw * tw; i += 256) vals[i] = DADD(vals[i], DMUL(P->sharpening, sharp[i]));
-                __syncthreads();
+                bsync_t<NT>();
+                for (int i = t; i < tw * tw; i += NT) vals[i] = DADD(vals[i], DMUL(P->sharpening, sharp[i]));
+                bsync_t<NT>();
                 if (t == 0) {
                     float bs = 0.0f, ws = 0.0f, bc = 1.0f, wc = 1.0f;
                     uint64_t rc = 0;
@@ -551,7 +583,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     s_rc = rc;
                     s_margin = (float)mg;
                 }
-                __syncthreads();
+                bsync_t<NT>();
                 // quick decode (fast/decode.c:87-107): the 16 (rotation,
                 // chunk) lookups at once, thread 4 r + i each; the first
                 // match in the reference's order (rotation, chunk, entry)
@@ -576,7 +608,7 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     s_mj[t] = mj;
                     s_mh[t] = mh;
                 }
-                __syncthreads();
+                bsync_t<NT>();
                 if (t == 0) {
                     const float margin = s_margin;
                     int64_t id = 65535, ham = 255, rot = 0;
@@ -611,14 +643,14 @@ __global__ __launch_bounds__(256) void k_decode(const Frame *frames, const DevPa
                     }
                 }
             }
-            __syncthreads();
+            bsync_t<NT>();
         }
         if (t == 0) {
             QuadRec &o = F.quads[qi];
             o.hok = s_hok;
             for (int i = 0; i < 9; i++) o.H[i] = sH[i];
         }
-        __syncthreads();
+        bsync_t<NT>();
     }
 }
 
@@ -752,12 +784,24 @@ void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q
     hipLaunchKernelGGL(k_init, dim3(1, nframes), dim3(64), 0, st, dframes, Q);
 }
 
-void warm_decode_grid(int fit_blocks) { persistent_grid(reinterpret_cast<const void *>(&k_decode), 256, 0, fit_blocks); }
+void warm_decode_grid(int fit_blocks) {
+    persistent_grid(reinterpret_cast<const void *>(&k_decode<64>), 64, 0, 4 * fit_blocks);
+    persistent_grid(reinterpret_cast<const void *>(&k_decode<256>), 256, 0, fit_blocks);
+}
 
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          WorkQueue *Q, Marks *mk) {
-    hipLaunchKernelGGL(k_decode, dim3(persistent_grid(reinterpret_cast<const void *>(&k_decode), 256, 0, blocks)),
-                       dim3(256), 0, st, dframes, dparams, Q);
+    // a wave a quad (four times as many quads in flight as 256-thread
+    // blocks); for a few frames, whose quads all fit at once, a 256-thread
+    // block a quad, its edges refined at the same time
+    if (nframes > 4)
+        hipLaunchKernelGGL(k_decode<64>,
+                           dim3(persistent_grid(reinterpret_cast<const void *>(&k_decode<64>), 64, 0, 4 * blocks)),
+                           dim3(64), 0, st, dframes, dparams, Q);
+    else
+        hipLaunchKernelGGL(k_decode<256>,
+                           dim3(persistent_grid(reinterpret_cast<const void *>(&k_decode<256>), 256, 0, blocks)),
+                           dim3(256), 0, st, dframes, dparams, Q);
     mk->mark(st, "decode");
     hipLaunchKernelGGL(k_finish, dim3(1, nframes), dim3(64), 0, st, dframes, dparams);
     mk->mark(st, "finish");
