@@ -72,9 +72,13 @@ __device__ __forceinline__ uint32_t point_mask(const uint8_t *a, const uint8_t *
     return c10 | (c01 << 1) | ((cm1 & (p11 ^ 1u)) << 2) | (c11 << 3);
 }
 
+// Pass 1 stages each 64-pixel step's points (at most 4 a pixel) in LDS and
+// writes them out contiguously; the sort value of a point is its index,
+// which the sort's first pass takes from the position.
 __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pass) {
+    __shared__ uint64_t st_pt[4][256], st_key[4][256];
     const Frame F = frames[blockIdx.y];
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int y = 1 + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (y > F.qh - 2) return;
     const int QW = F.qw;
@@ -89,31 +93,41 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
         uint32_t m = x < QW - 1 ? point_mask(a, b, la, lb, x) : 0u;
         uint32_t c = __popc(m);
         if (pass) {
-            uint32_t inc = wave_incl_scan_u(c);
-            uint32_t at = base + count + inc - c;
+            const uint32_t inc = wave_incl_scan_u(c), tot = __shfl(inc, 63, 64);
+            uint32_t k = inc - c;  // this lane's first point in the step
             if (m) {
                 uint32_t av = a[x];
                 int vd = 255 - 2 * (int)av;
                 uint32_t r0 = la[x] & 0x7fffffffu;
                 for (int j = 0; j < 4; j++) {
                     if (!((m >> j) & 1u)) continue;
-                    if ((int64_t)at >= F.pcap) break;  // flagged by k_point_scan
                     int dx = j == 0 ? 1 : j == 1 ? 0 : j == 2 ? -1 : 1;
                     int dy = j == 0 ? 0 : 1;
                     uint32_t r1 = (dy ? lb[x + dx] : la[x + dx]) & 0x7fffffffu;
                     uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
                     uint64_t id = ((uint64_t)hi << 32) + lo;
                     uint64_t bucket = ((id * 2654435761ull) >> 32) & mask;
-                    uint64_t key = (bucket << (2 * D)) | ((uint64_t)F.rdense[hi] << D) | F.rdense[lo];
                     uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
                     uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
-                    F.pts[at] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
-                    F.skey[at] = key;
-                    F.sval[at] = at;
-                    at++;
+                    st_pt[wv][k] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
+                    st_key[wv][k] = (bucket << (2 * D)) | ((uint64_t)F.rdense[hi] << D) | F.rdense[lo];
+                    k++;
                 }
             }
-            count += __shfl(inc, 63, 64);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            const uint32_t at0 = base + count;
+            for (uint32_t q = (uint32_t)lane; q < tot; q += 64) {
+                const int64_t at = (int64_t)at0 + q;
+                if (at >= F.pcap) break;  // flagged by k_point_scan
+                F.pts[at] = st_pt[wv][q];
+                F.skey[at] = st_key[wv][q];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            count += tot;
         } else {
             count += __shfl(wave_incl_scan_u(c), 63, 64);
         }
@@ -234,8 +248,10 @@ __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int whi
 // Stable scatter: chunks of 256 keys in order; inside a chunk, the rank of
 // a key among the keys of its digit before it: within a wave by ballots
 // (peers = lanes of the same digit), across waves by per-wave counts.
+// iota: the values are the keys' positions (the first pass of the cluster
+// sort; its points were emitted in order), not read.
 __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity,
-                                                       int dyn) {
+                                                       int dyn, int iota) {
     __shared__ uint32_t off[256];
     __shared__ uint32_t wcnt[4][256];
     const Frame F = frames[blockIdx.y];
@@ -264,7 +280,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
         uint32_t i = c0 + t;
         bool valid = i < s1;
         uint64_t key = valid ? B.kin[i] : 0ull;
-        uint32_t val = valid ? B.vin[i] : 0u;
+        uint32_t val = valid ? (iota ? i : B.vin[i]) : 0u;
         uint32_t d = (uint32_t)(key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -349,12 +365,12 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
 // ---------------------------------------------------------------- launchers
 
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
-                       int dyn) {
+                       int dyn, int iota) {
     for (int shift = 0; shift < nbits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
                            *parity, dyn);
         hipLaunchKernelGGL(k_radix_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity, dyn);
+                           *parity, dyn, shift == 0 ? iota : 0);
         *parity ^= 1;
     }
 }
@@ -371,7 +387,7 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     mk->mark(st, "points");
     // key_bits: the widest key the call's frames can have
     int parity = 0;
-    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1);
+    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 1);
     mk->mark(st, "cluster_sort");
     hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);

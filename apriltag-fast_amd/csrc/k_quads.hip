@@ -853,7 +853,8 @@ __device__ __forceinline__ uint32_t bscan_t(uint32_t v, uint32_t *total) {
 
 // Persistent over list l of kept clusters, a cluster per block of NT threads.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q, int l) {
+__global__ __launch_bounds__(NT) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q,
+                                                    int l) {
     constexpr int kMaxNm = CornerShape<NT>::kMaxNm, kLdsPts = CornerShape<NT>::kLdsPts;
     constexpr int kMaxima = CornerShape<NT>::kMaxima;
     constexpr int kShared = 2 * kLdsPts + kMaxima > 4 * kMaxNm * kMaxNm ? 2 * kLdsPts + kMaxima : 4 * kMaxNm * kMaxNm;
@@ -1208,7 +1209,7 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 }
 
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
-                       int dyn);
+                       int dyn, int iota);
 
 void warm_decode_grid(int fit_blocks);
 
@@ -1272,7 +1273,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_angle_sort, dim3(pgrid(k_angle_sort, 256, fit_blocks)), dim3(256), 0, st, dframes, Q);
         hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, st, dframes, Q);
     } else {
-        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0);
+        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0);
     }
     mk->mark(st, "angle_sort");
     hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256), 0,
