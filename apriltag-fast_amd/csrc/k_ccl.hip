@@ -64,14 +64,27 @@ __device__ __forceinline__ bool runs_join(int a0, int a1, int b0, int b1, bool w
 // The links of every row, in the reference's order (current run
 // ascending, then previous run ascending): one wave per row, all rows at
 // once. uedge[y][e] = i << 16 | k (current run i of row y, previous run k).
-__global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames) {
+// L: the previous row's runs are first copied to LDS (lds_qw words a wave),
+// where the search and scans read them.
+template <bool L>
+__global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames, int lds_qw) {
+    extern __shared__ uint32_t sprv[];
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63;
     int y = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (y < 1 || y >= F.qh) return;
     const int QW = F.qw;
     const int ncur = (int)F.rcnt[y], nprev = (int)F.rcnt[y - 1];
-    const uint32_t *cur = F.runs + (int64_t)y * QW, *prv = F.runs + (int64_t)(y - 1) * QW;
+    const uint32_t *cur = F.runs + (int64_t)y * QW;
+    const uint32_t *prv = F.runs + (int64_t)(y - 1) * QW;
+    if constexpr (L) {
+        uint32_t *sp = sprv + (threadIdx.x >> 6) * lds_qw;
+        for (int k = lane; k < nprev; k += 64) sp[k] = prv[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        prv = sp;
+    }
     uint32_t *out = F.uedge + (int64_t)y * 3 * QW;
     const uint32_t cap = 3u * (uint32_t)QW;
     uint32_t carry = 0;
@@ -866,7 +879,13 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     // most 4: their launch bounds are 256 threads
     const int rw = std::min(std::max(tu.row_waves, 1), 4);
     const dim3 rb(64 * rw), rg((unsigned)((max_qh + rw) / rw), nframes);
-    hipLaunchKernelGGL(k_uf_edges, rg, rb, 0, st, dframes);
+    // the previous rows in LDS when rw rows of the widest frame fit 64 KiB
+    // (and the LDS paths are not switched off: replay_cap 0)
+    const size_t elds = (size_t)rw * (size_t)max_qw * 4;
+    if (elds <= 64 * 1024 && c.replay_cap > 0)
+        hipLaunchKernelGGL(k_uf_edges<true>, rg, rb, elds, st, dframes, max_qw);
+    else
+        hipLaunchKernelGGL(k_uf_edges<false>, rg, rb, 0, st, dframes, 0);
     mk->mark(st, "uf_edges");
     switch (tu.front_threads) {
     case 64:
