@@ -94,11 +94,14 @@ __device__ __forceinline__ void pt_unpack(uint64_t p, int *x, int *y, int *gx, i
 
 // ------------------------------------------------------------ prep
 
-constexpr int kDotChunk = 4096;
-
-__global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const DevParams *P, int items) {
+// Every cluster of every frame, NT threads a cluster (a wave: most clusters
+// are a few hundred points), blocks taking clusters in strides.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevParams *P, int items) {
+    constexpr int kDotChunk = NT * 16;  // dot-product terms staged at a time
+    constexpr int kW = NT / 64;
     __shared__ float terms[kDotChunk];
-    __shared__ uint32_t red[4][4];
+    __shared__ uint32_t red[kW][4];
     __shared__ float lanes8[8];
     __shared__ int s_keep;
     const Frame F = frames[blockIdx.y];
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
         // bounding box (fast/quads.c:515-528)
         uint32_t xmin = 0xffff, xmax = 0, ymin = 0xffff, ymax = 0;
         if (in_range)
-            for (uint32_t i = t; i < n; i += 256) {
+            for (uint32_t i = t; i < n; i += NT) {
                 int x, y, gx, gy;
                 pt_unpack(pts[i], &x, &y, &gx, &gy);
                 xmin = min(xmin, (uint32_t)x); xmax = max(xmax, (uint32_t)x);
@@ -121,18 +124,20 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
             xmin = min(xmin, (uint32_t)__shfl_xor(xmin, d, 64)); xmax = max(xmax, (uint32_t)__shfl_xor(xmax, d, 64));
             ymin = min(ymin, (uint32_t)__shfl_xor(ymin, d, 64)); ymax = max(ymax, (uint32_t)__shfl_xor(ymax, d, 64));
         }
-        if (lane == 0) { red[w][0] = xmin; red[w][1] = xmax; red[w][2] = ymin; red[w][3] = ymax; }
-        __syncthreads();
-        xmin = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
-        xmax = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
-        ymin = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
-        ymax = max(max(red[0][3], red[1][3]), max(red[2][3], red[3][3]));
-        __syncthreads();
+        if constexpr (kW > 1) {
+            if (lane == 0) { red[w][0] = xmin; red[w][1] = xmax; red[w][2] = ymin; red[w][3] = ymax; }
+            __syncthreads();
+            for (int k2 = 0; k2 < kW; k2++) {
+                xmin = min(xmin, red[k2][0]); xmax = max(xmax, red[k2][1]);
+                ymin = min(ymin, red[k2][2]); ymax = max(ymax, red[k2][3]);
+            }
+            __syncthreads();
+        }
         bool ok = in_range && ((int64_t)xmax - (int64_t)xmin) * ((int64_t)ymax - (int64_t)ymin) >= P->tag_width;
         float cx = (float)((double)(xmin + xmax) * 0.5 + 0.05118);
         float cy = (float)((double)(ymin + ymax) * 0.5 + -0.028581);
         // sort items (reverse point order) and the full keys of the points
-        for (uint32_t i = t; i < n; i += 256) {
+        for (uint32_t i = t; i < n; i += NT) {
             uint32_t slope = 0;
             if (ok) {
                 int x, y, gx, gy;
@@ -156,7 +161,7 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
         float acc = 0.0f;
         for (uint32_t c0 = 0; c0 < nv; c0 += kDotChunk) {
             uint32_t len = min((uint32_t)kDotChunk, nv - c0);
-            for (uint32_t j = t; j < len; j += 256) {
+            for (uint32_t j = t; j < len; j += NT) {
                 int x, y, gx, gy;
                 pt_unpack(pts[c0 + j], &x, &y, &gx, &gy);
                 float dx = __fsub_rn((float)x, cx), dy = __fsub_rn((float)y, cy);
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(256) void k_fit_prep(const Frame *frames, const Dev
             // for the moments: x, y and the squared image gradient at the
             // point's pixel (fast/quads.c:968-985), in cluster order
             const int iw = F.qw, ih = F.qh;
-            for (uint32_t i = t; i < n; i += 256) {
+            for (uint32_t i = t; i < n; i += NT) {
                 int px, py, gx, gy;
                 pt_unpack(pts[i], &px, &py, &gx, &gy);
                 int ix = (px >> 1) + (px & 1), iy = (py >> 1) + (py & 1);
@@ -1258,12 +1263,16 @@ int angle_sort_cap(int max_cluster, bool force_global) {
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
                         int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk) {
     // k_fit_prep: every cluster of every frame, a grid per frame; the fit
-    // kernels after it: persistent, fit_blocks blocks over the work lists
-    // the persistent kernels: fit_blocks blocks of 256 threads, or four
-    // times as many waves, at most as many as are resident at once
-    dim3 g(256, nframes);
+    // kernels after it: persistent over the work lists, fit_blocks blocks of
+    // 256 threads or four times as many waves, at most as many as are
+    // resident at once
     const int wf = 4 * fit_blocks;
-    hipLaunchKernelGGL(k_fit_prep, g, dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
+    // k_fit_prep: a wave a cluster for batches; for a few frames, whose
+    // clusters all fit at once, 256 threads a cluster (the large ones sooner)
+    if (nframes > 4)
+        hipLaunchKernelGGL(k_fit_prep<64>, dim3(512, nframes), dim3(64), 0, st, dframes, dparams, sort_cap ? 0 : 1);
+    else
+        hipLaunchKernelGGL(k_fit_prep<256>, dim3(256, nframes), dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 0, (int)kNLarge);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 1, (int)kNSmall);
