@@ -284,30 +284,34 @@ __device__ __forceinline__ void run_entry(const uint64_t *orig, int64_t n, const
     }
 }
 
-// order_run for a run of m <= 64 keys, a lane per key, in registers; the
-// entries ranked by block head, stably.
-__device__ __forceinline__ void order_run_wave(const uint64_t *orig, int64_t n, uint64_t *keys, uint32_t s,
-                                               uint32_t m) {
+// order_run for a batch of runs of at most 64 keys together, a lane per
+// key: lane l holds key j of the run starting at keys[rs] with m keys,
+// whose first key is on lane base (lanes past the batch: m = 0). Each run's
+// entries are ranked by block head, stably, among its own lanes.
+__device__ __noinline__ void order_runs_batch(const uint64_t *orig, int64_t n, uint64_t *keys, uint32_t rs,
+                                                 uint32_t m, int base) {
     const int lane = threadIdx.x & 63;
-    const bool act = lane < (int)m;
-    const uint64_t key = act ? keys[s + lane] : 0ull;
-    const uint64_t hi = __shfl(key, 0, 64) & 0xffffffff00000000ull;
+    const int j = lane - base;
+    const bool act = m > 0;
+    const uint64_t key = act ? keys[rs + (uint32_t)j] : 0ull;
+    const uint64_t hi = key & 0xffffffff00000000ull;
     int64_t lo = -1, len = 0;
     if (act) leaf_of(n, kidx(key), &lo, &len);
     const int64_t lo_prev = __shfl_up(lo, 1, 64);
-    const uint64_t starts = __ballot(act && (lane == 0 || lo_prev != lo));
+    const uint64_t starts = __ballot(act && (j == 0 || lo_prev != lo));
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     const int g0 = 63 - __clzll((long long)(starts & upto));
     const uint64_t after = starts & ~upto;
-    const int g1 = after ? __ffsll((unsigned long long)after) - 1 : (int)m;
+    int g1 = after ? __ffsll((unsigned long long)after) - 1 : 64;
+    g1 = min(g1, base + (int)m);
     uint64_t ent = key, bh = key;
-    if (act) run_entry(orig, n, keys, s, m, hi, (uint32_t)lane, key, lo, len, g0, g1, &ent, &bh);
+    if (act) run_entry(orig, n, keys, rs, m, hi, (uint32_t)j, key, lo, len, g0 - base, g1 - base, &ent, &bh);
     uint32_t rank = 0;
-    for (int l2 = 0; l2 < (int)m; l2++) {
-        const uint64_t b2 = __shfl(bh, l2, 64);
-        rank += (b2 < bh || (b2 == bh && l2 < lane)) ? 1u : 0u;
+    for (int d = 0; d < 64; d++) {
+        const uint64_t b2 = __shfl(bh, min(base + d, 63), 64);
+        rank += (act && d < (int)m && (b2 < bh || (b2 == bh && d < j))) ? 1u : 0u;
     }
-    if (act) keys[s + rank] = ent;
+    if (act) keys[rs + rank] = ent;
 }
 
 // order_run for a longer run, 64 keys at a time, the entries and heads in
@@ -372,15 +376,26 @@ __device__ __forceinline__ uint32_t slope_run_start(const uint64_t *keys, uint32
 
 // fix_runs (fast/quads.c:304-327) by one wave over the runs of equal
 // slope that start in [from, to): each run holding two neighbours less
-// than 5 apart is put in the merge sort's order (order_run_wave, or
-// order_run_long past 64 keys). A run that starts
-// before `from` belongs to the wave before.
+// than 5 apart is put in the merge sort's order (order_runs_batch, several
+// runs at once, or order_run_long past 64 keys). A run that starts before
+// `from` belongs to the wave before.
 __device__ void fix_runs_wave(uint64_t *keys, uint32_t n, const uint64_t *orig, uint32_t from, uint32_t to,
                               uint64_t *ent, uint64_t *bh) {
     const int lane = threadIdx.x & 63;
     uint32_t pos = from;
     if (pos >= to) return;
     if (pos > 0 && (keys[pos] >> 32) == (keys[pos - 1] >> 32)) pos = slope_run_end(keys, n, pos, keys[pos] >> 32);
+    // runs of at most 64 keys are packed onto the lanes and ordered
+    // together when the lanes are full (a run's rewrite does not change
+    // what the search reads: slopes and run ends stay)
+    int used = 0;
+    uint32_t my_rs = 0, my_m = 0;
+    int my_base = 0;
+    auto flush = [&]() {
+        if (used) order_runs_batch(orig, n, keys, my_rs, my_m, my_base);
+        used = 0;
+        my_m = 0;
+    };
     while (pos < to) {
         const uint32_t k = pos + lane;
         bool pr = false;
@@ -401,12 +416,22 @@ __device__ void fix_runs_wave(uint64_t *keys, uint32_t n, const uint64_t *orig, 
         const uint32_t s = slope_run_start(keys, k0 - 1, hk);
         if (s >= to) break;
         const uint32_t e = slope_run_end(keys, n, k0, hk);
-        if (e - s <= 64)
-            order_run_wave(orig, n, keys, s, e - s);
-        else
-            order_run_long(orig, n, keys, s, e - s, ent + s, bh + s);
+        const uint32_t m = e - s;
+        if (m <= 64) {
+            if (used + (int)m > 64) flush();
+            if (lane >= used && lane < used + (int)m) {
+                my_rs = s;
+                my_m = m;
+                my_base = used;
+            }
+            used += (int)m;
+        } else {
+            flush();
+            order_run_long(orig, n, keys, s, m, ent + s, bh + s);
+        }
         pos = e;
     }
+    flush();
 }
 
 // Per frame (a block each): its kept clusters in cluster order, those
