@@ -159,6 +159,7 @@ struct CclPlan {
     int front;                  // LDS bytes of the front (frames without fscr)
     int replay;                 // LDS bytes of the replay
     int replay_cap;             // runs per row replayed in LDS, 0 = none
+    bool replay_pack;           // its elements packed (parent, weight) in one word
     int replay_wide, replay_wide_cap;  // the second LDS launch (wider rows): bytes, runs; 0 = none
     int replay_wide_grid;       // its persistent grid
     int replay_waves;           // waves per frame of k_replay_hbm

@@ -450,9 +450,9 @@ __global__ __launch_bounds__(256) void k_front_dense(const Frame *frames) {
 struct ReplayLayout {
     size_t le, nxt, head, labA, hasl, total;
 };
-__host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix) {
+__host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix, int eb = 8) {
     ReplayLayout r;
-    size_t o = (size_t)8 * (2 * C + 2);  // ent: [2C + 2] x (parent, weight)
+    size_t o = (size_t)eb * (2 * C + 2);  // ent: [2C + 2] x (parent, weight), 8 bytes or packed in 4
     r.le = o;
     o += (size_t)4 * L;                  // link: current run << 16 | record
     r.nxt = o;
@@ -471,7 +471,7 @@ __host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix) {
 // max_qw wide, and of the second (the rows wider than the first's).
 int replay_cap(int max_qw) { return std::min(512, (std::max(max_qw, 1) + 63) & ~63); }
 int replay_wide_cap(int max_qw) { return (std::max(max_qw, 1) + 63) & ~63; }
-size_t replay_lds_bytes(int cap) { return replay_layout(cap, 2 * cap, 2).total; }
+size_t replay_lds_bytes(int cap, bool pk) { return replay_layout(cap, 2 * cap, 2, pk ? 4 : 8).total; }
 size_t replay_wide_lds_bytes(int cap) { return replay_layout(cap, 3 * cap, 2).total; }
 __host__ __device__ inline size_t replay_hbm_bytes_d(int qw) { return replay_layout(qw, 3 * qw, 4).total; }
 size_t replay_hbm_bytes(int qw) { return replay_hbm_bytes_d(qw); }
@@ -484,39 +484,71 @@ __device__ __forceinline__ bool replay_fits(const Frame &F, int y, int cap, int 
     return (int)F.rcnt[y - 1] <= cap && (int)F.rcnt[y] <= cap && (int)F.ecnt[y] <= lcap;
 }
 
-template <bool G>
-__device__ __forceinline__ uint2 rp_ent(typename Mem<G>::U32 *ent, uint32_t x) {
-    if constexpr (G) {
-        uint2 v;
-        v.x = ent[2 * x];
-        v.y = ent[2 * x + 1];
-        return v;
-    } else {
-        return *reinterpret_cast<const uint2 *>(ent + 2 * x);
+// A row's elements: (parent, weight) pairs, or (PK) packed in one word as
+// parent << 21 | weight, for rows of at most 1023 runs in frames of under
+// 2^21 pixels (every weight is a pixel count of the frame).
+template <bool G, bool PK>
+struct Ents {
+    using U32 = typename Mem<G>::U32;
+    static constexpr uint32_t kWB = 21, kWM = (1u << kWB) - 1u;
+    U32 *e;
+    __device__ __forceinline__ uint2 get(uint32_t x) const {
+        if constexpr (PK) {
+            const uint32_t v = e[x];
+            return make_uint2(v >> kWB, v & kWM);
+        } else if constexpr (G) {
+            uint2 v;
+            v.x = e[2 * x];
+            v.y = e[2 * x + 1];
+            return v;
+        } else {
+            return *reinterpret_cast<const uint2 *>(e + 2 * x);
+        }
     }
-}
-
-// Find with path halving; returns the root's entry with .x = the root.
-template <bool G>
-__device__ __forceinline__ uint2 rp_find(typename Mem<G>::U32 *ent, uint32_t x) {
-    for (;;) {
-        uint2 e = rp_ent<G>(ent, x);
-        if (e.x == x) return e;
-        uint2 pe = rp_ent<G>(ent, e.x);
-        if (pe.x == e.x) return pe;
-        ent[2 * x] = pe.x;
-        x = pe.x;
+    __device__ __forceinline__ uint32_t par(uint32_t x) const {
+        if constexpr (PK) return e[x] >> kWB;
+        else return e[2 * x];
     }
-}
-
-template <bool G>
-__device__ __forceinline__ uint32_t rp_find_ro(typename Mem<G>::U32 *ent, uint32_t x) {
-    for (uint32_t q; (q = ent[2 * x]) != x;) x = q;
-    return x;
-}
+    __device__ __forceinline__ uint32_t wt(uint32_t x) const {
+        if constexpr (PK) return e[x] & kWM;
+        else return e[2 * x + 1];
+    }
+    __device__ __forceinline__ void set(uint32_t x, uint32_t p, uint32_t w) const {
+        if constexpr (PK) {
+            e[x] = (p << kWB) | w;
+        } else {
+            e[2 * x] = p;
+            e[2 * x + 1] = w;
+        }
+    }
+    __device__ __forceinline__ void set_par(uint32_t x, uint32_t p) const {
+        if constexpr (PK) e[x] = (p << kWB) | (e[x] & kWM);
+        else e[2 * x] = p;
+    }
+    __device__ __forceinline__ void set_wt(uint32_t x, uint32_t w) const {
+        if constexpr (PK) e[x] = (e[x] & ~kWM) | w;
+        else e[2 * x + 1] = w;
+    }
+    // find with path halving; returns the root's entry with .x = the root
+    __device__ __forceinline__ uint2 find(uint32_t x) const {
+        for (;;) {
+            const uint2 v = get(x);
+            if (v.x == x) return v;
+            const uint2 pv = get(v.x);
+            if (pv.x == v.x) return pv;
+            if constexpr (PK) e[x] = (pv.x << kWB) | v.y;
+            else e[2 * x] = pv.x;
+            x = pv.x;
+        }
+    }
+    __device__ __forceinline__ uint32_t find_ro(uint32_t x) const {
+        for (uint32_t q; (q = par(x)) != x;) x = q;
+        return x;
+    }
+};
 
 // Replay of row y with capacity C runs and L links (the row fits).
-template <bool G>
+template <bool G, bool PK>
 __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     using U32 = typename Mem<G>::U32;
     using U16 = typename Mem<G>::U16;
@@ -525,8 +557,8 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     constexpr uint32_t kNone = G ? 0xffffffffu : 0xffffu;
     const int lane = threadIdx.x;
     const int QW = F.qw;
-    const ReplayLayout ly = replay_layout(C, L, G ? 4 : 2);
-    U32 *ent = reinterpret_cast<U32 *>(base);
+    const ReplayLayout ly = replay_layout(C, L, G ? 4 : 2, PK ? 4 : 8);
+    const Ents<G, PK> ent{reinterpret_cast<U32 *>(base)};
     U32 *le = reinterpret_cast<U32 *>(base + ly.le);
     IX *nxt = reinterpret_cast<IX *>(base + ly.nxt);
     IX *head = reinterpret_cast<IX *>(base + ly.head);
@@ -540,20 +572,17 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     const uint32_t *rlab_cur = F.rlab + cb;
     const int ev = m + ncur;
     for (int l = lane; l < m; l += 64) {
-        ent[2 * l] = (uint32_t)l;
-        ent[2 * l + 1] = F.recw[pb + l];
+        ent.set((uint32_t)l, (uint32_t)l, F.recw[pb + l]);
         hasl[l] = 0;
     }
     for (int i = lane; i < ncur; i += 64) {
         uint32_t r = cur[i];
-        ent[2 * (m + i)] = (uint32_t)(m + i);
-        ent[2 * (m + i) + 1] = run_end(r) - run_start(r) + 1;
+        ent.set((uint32_t)(m + i), (uint32_t)(m + i), run_end(r) - run_start(r) + 1);
     }
     for (int k = lane; k < nprev; k += 64) labA[k] = (uint16_t)F.rlab[pb + k];
     for (int g = lane; g < ng; g += 64) head[g] = kNone;
     if (lane == 0) {
-        ent[2 * ev] = (uint32_t)ev;
-        ent[2 * ev + 1] = 1;
+        ent.set((uint32_t)ev, (uint32_t)ev, 1u);
     }
     bool lastcol = false;
     if (ncur > 0) {
@@ -618,17 +647,17 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
             }
             const uint32_t i = ed >> 16, l = ed & 0xffffu;
             if ((int)i != cur_i) {
-                if (cur_i >= 0) ent[2 * ra + 1] = wa;
+                if (cur_i >= 0) ent.set_wt(ra, wa);
                 cur_i = (int)i;
                 ra = (uint32_t)m + i;  // a current run's first link: a fresh root
-                wa = ent[2 * ra + 1];
+                wa = ent.wt(ra);
             }
-            uint2 b = rp_find<G>(ent, l);
+            uint2 b = ent.find(l);
             if (b.x != ra) {
                 if (wa > b.y) {
-                    ent[2 * b.x] = ra;
+                    ent.set_par(b.x, ra);
                 } else {
-                    ent[2 * ra] = b.x;
+                    ent.set_par(ra, b.x);
                     ra = b.x;
                 }
                 wa += b.y;
@@ -637,17 +666,17 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
             ed = edn;
             en = enn;
         }
-        ent[2 * ra + 1] = wa;
+        ent.set_wt(ra, wa);
     }
     wsync<G>();
     if (lane == 0 && lastcol) {  // fast/unionfind.c:93-97, last in order
-        uint2 a = rp_find<G>(ent, (uint32_t)(ev - 1));
+        uint2 a = ent.find((uint32_t)(ev - 1));
         if (a.y > 1u) {
-            ent[2 * ev] = a.x;
-            ent[2 * a.x + 1] = a.y + 1;
+            ent.set_par((uint32_t)ev, a.x);
+            ent.set_wt(a.x, a.y + 1);
         } else {
-            ent[2 * a.x] = (uint32_t)ev;
-            ent[2 * ev + 1] = a.y + 1;
+            ent.set_par(a.x, (uint32_t)ev);
+            ent.set_wt((uint32_t)ev, a.y + 1);
         }
     }
     wsync<G>();
@@ -657,15 +686,16 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
         return vb + (uint32_t)y - 1;
     };
     for (int l = lane; l < m; l += 64) {
-        uint32_t q = ent[2 * l];
+        uint32_t q = ent.par((uint32_t)l);
         F.rlost[pb + l] = q == (uint32_t)l ? (hasl[l] ? kLostNone : kLostRetired) : name(q);
     }
-    for (int x = m + lane; x <= ev; x += 64) F.parent[name((uint32_t)x)] = name(ent[2 * x]);
-    for (int i = lane; i < ncur; i += 64) F.rroot[cb + rlab_cur[i]] = name(rp_find_ro<G>(ent, (uint32_t)(m + i)));
+    for (int x = m + lane; x <= ev; x += 64) F.parent[name((uint32_t)x)] = name(ent.par((uint32_t)x));
+    for (int i = lane; i < ncur; i += 64) F.rroot[cb + rlab_cur[i]] = name(ent.find_ro((uint32_t)(m + i)));
 }
 
 // A wave per row, the rows that fit C = cap runs and 2C links; the others
-// are listed (rwide) for the wide launch.
+// are listed (rwide) for the wide launch. PK: packed elements.
+template <bool PK>
 __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
     extern __shared__ __align__(16) uint8_t lds[];
     const Frame &F = frames[blockIdx.y];
@@ -675,7 +705,7 @@ __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
         if (threadIdx.x == 0) F.rwide[atomicAdd(F.counters + kNWideRows, 1u)] = (uint32_t)y;
         return;
     }
-    replay_row<false>(F, y, lds, cap, 2 * cap);
+    replay_row<false, PK>(F, y, lds, cap, 2 * cap);
 }
 
 // The listed rows of every frame (list 3 of the work queue) that fit
@@ -690,7 +720,7 @@ __global__ __launch_bounds__(64) void k_replay_wide(const Frame *frames, const W
         while (off[fi + 1] <= g) fi++;
         const Frame &F = frames[fi];
         const int y = (int)F.rwide[g - off[fi]];
-        if (replay_fits(F, y, cap, lcap)) replay_row<false>(F, y, lds, cap, lcap);
+        if (replay_fits(F, y, cap, lcap)) replay_row<false, false>(F, y, lds, cap, lcap);
         __syncthreads();  // the next row reuses the memory
     }
 }
@@ -709,7 +739,7 @@ __global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap,
         while (todo) {
             const int j = __ffsll((unsigned long long)todo) - 1;
             todo &= todo - 1;
-            replay_row<true>(F, 1 + b + W * (k0 + j), mem, F.qw, 3 * F.qw);
+            replay_row<true, false>(F, 1 + b + W * (k0 + j), mem, F.qw, 3 * F.qw);
             __syncthreads();  // the next row reuses the memory
         }
     }
@@ -843,13 +873,16 @@ size_t replay_scratch_bytes(int qw, bool force) {
 }
 
 // The launch shapes of a call: max_lds_qw is the widest frame whose front
-// runs in LDS (0: none), max_qw the widest frame.
-CclPlan ccl_plan(int max_lds_qw, int max_qw, bool force_scratch) {
+// runs in LDS (0: none), max_qw the widest frame, max_weight the most
+// pixels a set of a frame can weigh (qw qh plus the virtual nodes).
+CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch) {
     CclPlan c{};
     c.front = max_lds_qw > 0 ? (int)front_bytes(max_lds_qw) : 0;
     // rows of up to replay_cap runs in LDS, the others (all when forced) in HBM
     c.replay_cap = force_scratch ? 0 : replay_cap(max_qw);
-    c.replay = c.replay_cap ? (int)replay_lds_bytes(c.replay_cap) : 0;
+    // packed elements when every weight fits 21 bits (and a parent 11)
+    c.replay_pack = c.replay_cap && c.replay_cap < 1023 && max_weight < (1 << 21);
+    c.replay = c.replay_cap ? (int)replay_lds_bytes(c.replay_cap, c.replay_pack) : 0;
     // the wider rows: a second LDS launch when the widest row fits LDS
     c.replay_wide_cap = 0;
     c.replay_wide = 0;
@@ -906,7 +939,10 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     hipLaunchKernelGGL(k_front_dense, rg, rb, 0, st, dframes);
     mk->mark(st, "uf_front");
     const dim3 rows((unsigned)std::max(1, max_qh - 1), nframes);
-    if (c.replay_cap) hipLaunchKernelGGL(k_replay, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
+    if (c.replay_cap && c.replay_pack)
+        hipLaunchKernelGGL(k_replay<true>, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
+    else if (c.replay_cap)
+        hipLaunchKernelGGL(k_replay<false>, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
     if (c.replay_wide_cap) {
         launch_work_offsets(st, dframes, nframes, Q, 3, (int)kNWideRows);
         hipLaunchKernelGGL(k_replay_wide, dim3((unsigned)c.replay_wide_grid), dim3(64), (size_t)c.replay_wide, st,

@@ -21,7 +21,7 @@
 namespace atg {
 void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q);
 void launch_copy(hipStream_t st, void *dst, const void *src, int64_t bytes);
-CclPlan ccl_plan(int max_lds_qw, int max_qw, bool force_scratch);
+CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                         int max_qh, bool blur, bool deglitch, const Tune &tu, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
@@ -455,6 +455,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     size_t total = 0, pinned_need = 0;
     int max_qw = 4, max_qh = 4, key_bits = 0, cl_bits = 0;
     int max_lds_qw = 0;
+    int64_t max_weight = 0;
     for (int32_t i = 0; i < n; i++) {
         FramePlan &fp = ctx->plans[(size_t)i];
         fp.W = frames[i].width;
@@ -472,6 +473,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         Frame F{};
         fp.size = layout(fp, nfam, !on_device, &F, nullptr);
         if (!F.fscr) max_lds_qw = std::max<int>(max_lds_qw, (int)fp.qw);
+        max_weight = std::max<int64_t>(max_weight, fp.qw * fp.qh + fp.qh);
         fp.off = total;
         total += fp.size;
         if (!on_device && !fp.skip) pinned_need += align256((size_t)(fp.W * fp.H));
@@ -482,7 +484,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
     }
     // the union-find's launch shapes (its HBM state is in each frame's region)
-    const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, ctx->force_scratch);
+    const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, max_weight, ctx->force_scratch);
     const size_t qoff = total;
     total += queue_bytes(n);
     if (total > ctx->arena_size && ctx->arena_fixed) return fail(ctx, ATG_ERR_LIMIT, "batch over the arena budget");
@@ -617,7 +619,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         // one graph per call shape: everything the stream work depends on
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
-                                    ccl.replay_cap, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid, ccl.replay_waves,        max_lds_qw,
+                                    ccl.replay_cap, ccl.replay_pack, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid, ccl.replay_waves,        max_lds_qw,
                                     tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
