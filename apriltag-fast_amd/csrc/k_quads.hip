@@ -504,82 +504,6 @@ __device__ __forceinline__ void bitonic_pair(uint32_t q, uint32_t k, uint32_t j,
 
 constexpr uint32_t kSortTile = 2048;  // keys of a cluster sorted together in LDS
 
-// Steps (k, j) with partners inside tiles of kSortTile keys, on every tile
-// of keys[0, n) in turn: from merge size k0 up to kSortTile when k0 > 0,
-// else only the steps of distance below kSortTile of the merge of size kk.
-__device__ void sort_tiles(uint64_t *keys, uint32_t n, uint64_t *sk, uint32_t k0, uint32_t kk) {
-    const uint32_t t = threadIdx.x;
-    for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
-        const uint32_t tn = min(kSortTile, n - t0);
-        for (uint32_t i = t; i < kSortTile; i += 256) sk[i] = i < tn ? keys[t0 + i] : ~0ull;
-        __syncthreads();
-        auto step = [&](uint32_t k, uint32_t j) {
-            for (uint32_t q = t; q < kSortTile / 2; q += 256) {
-                uint32_t i, p;
-                bitonic_pair(q, k, j, &i, &p);
-                const uint64_t a = sk[i], b = sk[p];
-                if (a > b) {
-                    sk[i] = b;
-                    sk[p] = a;
-                }
-            }
-            __syncthreads();
-        };
-        if (k0) {
-            for (uint32_t k = k0; k <= kSortTile; k <<= 1) {
-                step(k, 0);
-                for (uint32_t j = k >> 2; j > 0; j >>= 1) step(k, j);
-            }
-        } else {
-            for (uint32_t j = kSortTile >> 1; j > 0; j >>= 1) step(kk, j);
-        }
-        for (uint32_t i = t; i < tn; i += 256) keys[t0 + i] = sk[i];
-        __syncthreads();
-    }
-}
-
-// Persistent over list 0 of kept clusters (above kLargeCluster points):
-// the full keys of a cluster (slope << 32 | ~index, written by k_fit_prep
-// in point order) sorted ascending, the steps within a tile in LDS, the
-// wider ones over the cluster's keys in HBM. The keys are distinct, so
-// any sort gives the full-key order the reference's sort starts from
-// (fast/quads.c:379-427); its tie fix follows in k_fit_moments.
-__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q) {
-    __shared__ uint64_t sk[kSortTile];
-    const uint32_t t = threadIdx.x;
-    const uint32_t total = Q->total[0];
-    const uint32_t *off = Q->off[0];
-    uint32_t fi = 0;
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (off[fi + 1] <= g) fi++;
-        const Frame &F = frames[fi];
-        const uint32_t c = F.klist[g - off[fi]];
-        const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
-        uint64_t *keys = F.skey + s0;
-        for (uint32_t i = t; i < n; i += 256) keys[i] = F.okey[s0 + i];
-        __syncthreads();
-        sort_tiles(keys, n, sk, 2, 0);
-        uint32_t m = kSortTile;
-        while (m < n) m <<= 1;
-        for (uint32_t k = 2 * kSortTile; k <= m; k <<= 1) {
-            for (uint32_t j = 0; j == 0 || j >= kSortTile; j = j ? j >> 1 : k >> 2) {
-                for (uint32_t q = t; q < m / 2; q += 256) {
-                    uint32_t i, p;
-                    bitonic_pair(q, k, j, &i, &p);
-                    if (p >= n) continue;
-                    const uint64_t a = keys[i], b = keys[p];
-                    if (a > b) {
-                        keys[i] = b;
-                        keys[p] = a;
-                    }
-                }
-                __syncthreads();
-            }
-            sort_tiles(keys, n, sk, 0, k);
-        }
-    }
-}
-
 // A wave's bitonic sort of 64 R keys in registers: key i = r * 64 + lane
 // sits in v[r]; partners within a register row by shuffles, across rows in
 // the lane's own registers. Ascending.
@@ -629,6 +553,126 @@ __device__ __forceinline__ void wave_sort_keys(const uint64_t *in, uint64_t *out
     for (int r = 0; r < R; r++) {
         const uint32_t i = (uint32_t)(r * 64 + lane);
         if (i < n) out[i] = v[r];
+    }
+}
+
+// The half-cleaner steps of distance 256 down to 1 of the all-ascending
+// network on a wave's 512 keys in registers (key r * 64 + lane in v[r]):
+// the smaller key to the lower position.
+__device__ __forceinline__ void reg_half_clean(uint64_t (&v)[8]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 256; j > 0; j >>= 1) {
+        if (j >= 64) {
+            const int jr = j >> 6;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                if (r & jr) continue;
+                const int r2 = r | jr;
+                const uint64_t a = v[r], b = v[r2];
+                v[r] = a < b ? a : b;
+                v[r2] = a < b ? b : a;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const uint64_t o = __shfl_xor(v[r], j, 64);
+                const bool lower = (lane & j) == 0;
+                v[r] = lower ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
+            }
+        }
+    }
+}
+
+// One step of the network whose partners sit in other waves: through LDS
+// (sk, kSortTile keys), partner p = position ^ x.
+__device__ __forceinline__ void lds_xor_step(uint64_t (&v)[8], uint64_t *sk, uint32_t x) {
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < 8; r++) sk[w * 512 + r * 64 + lane] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const uint32_t i = w * 512 + r * 64 + lane, p = i ^ x;
+        const uint64_t o = sk[p];
+        v[r] = i < p ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
+    }
+    __syncthreads();
+}
+
+// The keys of each tile of kSortTile = 4 x 512 (a block's waves, 8 a lane in
+// registers) of src[0, n) into dst: sorted (kk = 0), or the steps of
+// distance below kSortTile of the merge of size kk. The merges across
+// waves go through LDS, everything inside a wave in registers.
+__device__ __forceinline__ void sort_tiles(const uint64_t *src, uint64_t *dst, uint32_t n, uint64_t *sk, uint32_t kk) {
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
+        const uint32_t tn = min(kSortTile, n - t0);
+        uint64_t v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t i = w * 512 + r * 64 + lane;
+            v[r] = i < tn ? src[t0 + i] : ~0ull;
+        }
+        if (kk == 0) {
+            wave_bitonic<8>(v);        // each wave's 512 keys, ascending
+            lds_xor_step(v, sk, 1023);  // merge of 1024: the mirror step
+            reg_half_clean(v);
+            lds_xor_step(v, sk, 2047);  // merge of 2048: the mirror step
+            lds_xor_step(v, sk, 512);
+            reg_half_clean(v);
+        } else {
+            lds_xor_step(v, sk, 1024);
+            lds_xor_step(v, sk, 512);
+            reg_half_clean(v);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t i = w * 512 + r * 64 + lane;
+            if (i < tn) dst[t0 + i] = v[r];
+        }
+        __syncthreads();
+    }
+}
+
+// Persistent over list 0 of kept clusters (above kLargeCluster points):
+// the full keys of a cluster (slope << 32 | ~index, written by k_fit_prep
+// in point order) sorted ascending, tiles of 2048 in registers and LDS, the
+// wider steps over the cluster's keys in HBM. The keys are distinct, so
+// any sort gives the full-key order the reference's sort starts from
+// (fast/quads.c:379-427); its tie fix follows in k_fit_moments.
+__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q) {
+    static_assert(kSortTile == 2048, "4 waves x 512 keys");
+    __shared__ uint64_t sk[kSortTile];
+    const uint32_t t = threadIdx.x;
+    const uint32_t total = Q->total[0];
+    const uint32_t *off = Q->off[0];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const uint32_t c = F.klist[g - off[fi]];
+        const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
+        uint64_t *keys = F.skey + s0;
+        uint32_t m = kSortTile;
+        while (m < n) m <<= 1;
+        sort_tiles(F.okey + s0, keys, n, sk, 0);  // the unsorted keys stay for the tie fix
+        for (uint32_t k = 2 * kSortTile; k <= m; k <<= 1) {
+            for (uint32_t j = 0; j == 0 || j >= kSortTile; j = j ? j >> 1 : k >> 2) {
+                for (uint32_t q = t; q < m / 2; q += 256) {
+                    uint32_t i, p;
+                    bitonic_pair(q, k, j, &i, &p);
+                    if (p >= n) continue;
+                    const uint64_t a = keys[i], b = keys[p];
+                    if (a > b) {
+                        keys[i] = b;
+                        keys[p] = a;
+                    }
+                }
+                __syncthreads();
+            }
+            sort_tiles(keys, keys, n, sk, k);
+        }
     }
 }
 
