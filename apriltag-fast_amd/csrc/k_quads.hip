@@ -823,18 +823,24 @@ struct Mom {
 };
 
 // The window sums of [i0, i1], wrapping, as fit_line forms them
-// (fast/quads.c:539-572).
+// (fast/quads.c:539-572); a wrapping range has i0 > i1 >= 0. Every operand
+// is loaded whichever case applies (the loads of all six sums in flight at
+// once, without divergent branches), the case picks the result.
 __device__ __forceinline__ void range_sums(const Mom &M, int64_t sz, int64_t i0, int64_t i1, double s[6]) {
+    const bool wrap = i0 >= i1, head = i0 > 0;
+    const int64_t ib = head ? i0 - 1 : 0;
+    double a[6], b[6], c[6];
     #pragma unroll
     for (int q = 0; q < 6; q++) {
         const double *v = M.a(q);
-        if (i0 < i1) {
-            double x = v[i1];
-            if (i0 > 0) x = __dsub_rn(x, v[i0 - 1]);
-            s[q] = x;
-        } else {
-            s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 - 1]), v[i1]);
-        }
+        a[q] = v[sz - 1];
+        b[q] = v[ib];
+        c[q] = v[i1];
+    }
+    #pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const double r1 = __dadd_rn(__dsub_rn(a[q], b[q]), c[q]), r2 = __dsub_rn(c[q], b[q]);
+        s[q] = wrap ? r1 : (head ? r2 : c[q]);
     }
 }
 
@@ -976,17 +982,26 @@ __global__ __launch_bounds__(NT) void k_fit_corners(const Frame *frames, const D
         }
         double *errs = inl ? sh : F.errs + s0;
         double *y = inl ? sh + kLdsPts : F.filt + s0;
-        // window errors (fast/quads.c:654-686 edges, :604-635 middle)
+        // window errors (fast/quads.c:654-686 edges, :604-635 middle); the
+        // operands of all four cases loaded at once, the case picks the sum
+        double tot[6];
+        #pragma unroll
+        for (int q = 0; q < 6; q++) tot[q] = M.a(q)[sz - 1];
         for (int64_t i = t; i < sz; i += NT) {
-            int64_t i0 = i - ksz, i1 = i + ksz;
-            double s[6];
+            const int64_t i0 = i - ksz, i1 = i + ksz;
+            const bool lo = i0 < 0, hi = i1 >= sz, head = i0 > 0;  // ksz <= sz / 12: not both
+            const int64_t ib = lo ? i0 + sz - 1 : (head ? i0 - 1 : 0), ic = hi ? i1 - sz : i1;
+            double b[6], c[6], s[6];
             #pragma unroll
             for (int q = 0; q < 6; q++) {
                 const double *v = M.a(q);
-                if (i0 < 0) s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 + sz - 1]), v[i1]);
-                else if (i1 >= sz) s[q] = __dadd_rn(__dsub_rn(v[sz - 1], v[i0 - 1]), v[i1 - sz]);
-                else if (i0 > 0) s[q] = __dsub_rn(v[i1], v[i0 - 1]);
-                else s[q] = v[i1];
+                b[q] = v[ib];
+                c[q] = v[ic];
+            }
+            #pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const double r1 = __dadd_rn(__dsub_rn(tot[q], b[q]), c[q]), r2 = __dsub_rn(c[q], b[q]);
+                s[q] = (lo || hi) ? r1 : (head ? r2 : c[q]);
             }
             errs[i] = win_err(s, nn);
         }
