@@ -75,11 +75,12 @@ enum {
     kNRuns = 0, kNPts, kNClusters, kNQuads, kNDets, kError,
     kNEdges,    // union-find links (for the byte counts of the measurements)
     kNKeptPts,  // points of the clusters that go on to the quad fit
-    kNLarge = 8,  // kept clusters above kLargeCluster points
-    kNSmall = 9,  // the other kept clusters
+    kNLarge = 8,  // kept clusters above kLargeCluster points, at most kHugeCluster
+    kNSmall = 9,  // kept clusters of at most kLargeCluster points
     kRootBits = 10,  // bits of the dense ok-root numbers
     kSortBits = 11,  // bits of the cluster sort key
     kNWideRows = 12,  // rows the first replay launch left (over 512 runs), listed in rwide
+    kNHuge = 13,  // kept clusters above kHugeCluster points
     kNCounters = 16
 };
 
@@ -114,7 +115,7 @@ struct Frame {
     uint32_t *rdense;     // [nodes] dense number of each ok root, in node order
     uint32_t *rinv;       // [nodes] the ok root of each dense number
     uint32_t *cstart, *ccount, *cflag;
-    uint32_t *klist;      // [2][kcap] kept clusters: large ones, then (at kcap) the others
+    uint32_t *klist;      // [2][kcap] kept clusters: the huge ones, then the large ones; at kcap the small ones
     uint32_t kcap;
     uint64_t *cid;        // [pcap] cluster ids (max_root << 32 | min_root), in order
     uint64_t *okey;       // unsorted angle keys, by cluster position
@@ -131,16 +132,21 @@ struct Frame {
 };
 
 // The work of the persistent kernels, across the frames of a call: list 0
-// the kept clusters above kLargeCluster points (taken first, so that the
-// longest items start early), list 1 the other kept clusters, list 2 the
-// quads, list 3 the rows of the union-find replay's wide launch. Each frame lists its own items (klist); off[l][f] is where frame
-// f's items start in list l's global index, total[l] its size. Blocks take
-// global indices in strides, without atomics.
+// the kept clusters of kLargeCluster to kHugeCluster points, list 1 the
+// smaller kept clusters, list 2 the quads, list 3 the rows of the
+// union-find replay's wide launch, list 4 the kept clusters above
+// kHugeCluster points. Each frame lists its own items (klist, rwide);
+// off[l][f] is where frame f's items start in list l's global index,
+// total[l] its size. Blocks take global indices in strides, without
+// atomics; the kernels of one cluster a block go over the huge clusters of
+// every frame first, then list 0, so that the longest items start early.
 constexpr uint32_t kLargeCluster = 512;
+constexpr uint32_t kHugeCluster = 1024;
+constexpr int kLists = 5;
 struct WorkQueue {
-    uint32_t total[4];
-    uint32_t nframes, pad[3];
-    uint32_t *off[4];   // [nframes + 1] each
+    uint32_t total[kLists];
+    uint32_t nframes, pad[2];
+    uint32_t *off[kLists];   // [nframes + 1] each
 };
 
 // Work splits of the kernel chain (the autotuner's choices); any values

@@ -777,7 +777,7 @@ __global__ void k_init(const Frame *frames, WorkQueue *Q) {
     const Frame F = frames[blockIdx.y];
     if (threadIdx.x < kNCounters) F.counters[threadIdx.x] = 0;
     if (F.skip) const_cast<uint8_t *>(F.img)[threadIdx.x] = 0;  // 64 bytes: the 8x8 zero frame
-    if (blockIdx.y == 0 && threadIdx.x < 4) Q->total[threadIdx.x] = 0;
+    if (blockIdx.y == 0 && threadIdx.x < kLists) Q->total[threadIdx.x] = 0;
 }
 
 void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q) {

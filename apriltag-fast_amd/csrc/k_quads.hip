@@ -353,112 +353,158 @@ __device__ __noinline__ void order_run_long(const uint64_t *orig, int64_t n, uin
     }
 }
 
-// The first index at or after p whose slope differs from hk (or n).
-__device__ __forceinline__ uint32_t slope_run_end(const uint64_t *keys, uint32_t n, uint32_t p, uint64_t hk) {
-    const int lane = threadIdx.x & 63;
-    for (; p < n; p += 64) {
-        const uint32_t k = p + lane;
-        const uint64_t b = __ballot(k >= n || (keys[k] >> 32) != hk);
-        if (b) return min(n, p + (uint32_t)__ffsll((unsigned long long)b) - 1);
-    }
-    return n;
-}
-
-// The first index of the run of slope hk that holds index p.
-__device__ __forceinline__ uint32_t slope_run_start(const uint64_t *keys, uint32_t p, uint64_t hk) {
-    const int lane = threadIdx.x & 63;
-    for (int64_t q = p;; q -= 64) {
-        const int64_t k = q - lane;
-        const uint64_t b = __ballot(k < 0 || (keys[k] >> 32) != hk);
-        if (b) return (uint32_t)(q - (__ffsll((unsigned long long)b) - 1) + 1);
-    }
-}
-
-// fix_runs (fast/quads.c:304-327) by one wave over the runs of equal
-// slope that start in [from, to): each run holding two neighbours less
-// than 5 apart is put in the merge sort's order (order_runs_batch, several
-// runs at once, or order_run_long past 64 keys). A run that starts before
-// `from` belongs to the wave before.
-__device__ void fix_runs_wave(uint64_t *keys, uint32_t n, const uint64_t *orig, uint32_t from, uint32_t to,
-                              uint64_t *ent, uint64_t *bh) {
-    const int lane = threadIdx.x & 63;
-    uint32_t pos = from;
-    if (pos >= to) return;
-    if (pos > 0 && (keys[pos] >> 32) == (keys[pos - 1] >> 32)) pos = slope_run_end(keys, n, pos, keys[pos] >> 32);
-    // runs of at most 64 keys are packed onto the lanes and ordered
-    // together when the lanes are full (a run's rewrite does not change
-    // what the search reads: slopes and run ends stay)
-    int used = 0;
-    uint32_t my_rs = 0, my_m = 0;
-    int my_base = 0;
-    auto flush = [&]() {
-        if (used) order_runs_batch(orig, n, keys, my_rs, my_m, my_base);
-        used = 0;
-        my_m = 0;
-    };
-    while (pos < to) {
-        const uint32_t k = pos + lane;
-        bool pr = false;
-        uint64_t kk = 0;
-        if (k < n && k >= 1) {
-            kk = keys[k];
-            const uint64_t kp = keys[k - 1];
-            pr = kk - kp < 5 && (kk >> 32) == (kp >> 32);
-        }
-        const uint64_t P = __ballot(pr);
-        if (!P) {
-            pos += 64;
-            continue;
-        }
-        const int p0 = __ffsll((unsigned long long)P) - 1;
-        const uint32_t k0 = pos + (uint32_t)p0;
-        const uint64_t hk = __shfl(kk, p0, 64) >> 32;
-        const uint32_t s = slope_run_start(keys, k0 - 1, hk);
-        if (s >= to) break;
-        const uint32_t e = slope_run_end(keys, n, k0, hk);
-        const uint32_t m = e - s;
-        if (m <= 64) {
-            if (used + (int)m > 64) flush();
-            if (lane >= used && lane < used + (int)m) {
-                my_rs = s;
-                my_m = m;
-                my_base = used;
+// fix_runs (fast/quads.c:304-327) by a block of NT threads: each run of
+// equal slope holding two neighbours less than 5 apart is put in the merge
+// sort's order. The keys go in segments of 8 NT positions: a lane at each
+// run start in the segment reads its run (it may reach past the segment)
+// and lists it if it needs the fix; the waves then split the list, pack
+// runs of at most 64 keys onto their lanes (order_runs_batch) and take the
+// longer ones one at a time (order_run_long). Runs are disjoint and a
+// run's rewrite keeps every slope, so the runs go in any order. ls/lm hold
+// 4 NT + 1 entries; ends with a barrier.
+template <int NT>
+__device__ void fix_runs_block(uint64_t *keys, uint32_t n, const uint64_t *orig, uint64_t *ent, uint64_t *bh,
+                               uint32_t *ls, uint32_t *lm, uint32_t *lcnt) {
+    constexpr uint32_t kSeg = 8 * NT;
+    constexpr int kW = NT / 64;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (uint32_t g0 = 0; g0 < n; g0 += kSeg) {
+        const uint32_t g1 = min(n, g0 + kSeg);
+        if (t == 0) *lcnt = 0;
+        __syncthreads();
+        for (uint32_t k = g0 + t; k < g1; k += NT) {
+            const uint64_t key = keys[k];
+            const uint64_t hk = key >> 32;
+            if (k > 0 && (keys[k - 1] >> 32) == hk) continue;  // not a run start
+            uint32_t e = k + 1;
+            uint64_t prev = key;
+            bool need = false, more = true;
+            while (more) {
+                uint64_t v[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q] = e + q < n ? keys[e + q] : 0ull;
+                const uint32_t e0 = e;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (more && e0 + q < n && (v[q] >> 32) == hk) {
+                        need = need || v[q] - prev < 5;
+                        prev = v[q];
+                        e++;
+                    } else {
+                        more = false;
+                    }
+                }
             }
-            used += (int)m;
-        } else {
-            flush();
-            order_run_long(orig, n, keys, s, m, ent + s, bh + s);
+            if (need) {
+                const uint32_t slot = atomicAdd(lcnt, 1u);
+                ls[slot] = k;
+                lm[slot] = e - k;
+            }
         }
-        pos = e;
+        __syncthreads();
+        const uint32_t L = *lcnt;
+        const uint32_t a = L * (uint32_t)w / kW, b = L * (uint32_t)(w + 1) / kW;
+        int used = 0, my_base = 0;
+        uint32_t my_rs = 0, my_m = 0;
+        for (uint32_t i0 = a; i0 < b; i0 += 64) {
+            const uint32_t i = i0 + (uint32_t)lane;
+            const uint32_t rs_l = i < b ? ls[i] : 0u, m_l = i < b ? lm[i] : 0u;
+            const uint32_t cnt = min(64u, b - i0);
+            for (uint32_t q = 0; q < cnt; q++) {
+                const uint32_t rs = (uint32_t)__shfl((int)rs_l, (int)q, 64), m = (uint32_t)__shfl((int)m_l, (int)q, 64);
+                if (m <= 64) {
+                    if (used + (int)m > 64) {
+                        order_runs_batch(orig, n, keys, my_rs, my_m, my_base);
+                        used = 0;
+                        my_m = 0;
+                    }
+                    if (lane >= used && lane < used + (int)m) {
+                        my_rs = rs;
+                        my_m = m;
+                        my_base = used;
+                    }
+                    used += (int)m;
+                } else {
+                    if (used) order_runs_batch(orig, n, keys, my_rs, my_m, my_base);
+                    used = 0;
+                    my_m = 0;
+                    order_run_long(orig, n, keys, rs, m, ent + rs, bh + rs);
+                }
+            }
+        }
+        if (used) order_runs_batch(orig, n, keys, my_rs, my_m, my_base);
+        __syncthreads();
     }
-    flush();
 }
 
-// Per frame (a block each): its kept clusters in cluster order, those
-// above kLargeCluster points first (block scans over the flags), and
-// their counts.
+// Per frame (a block each): its kept clusters in cluster order, the huge
+// ones (above kHugeCluster points) first, then the other large ones (above
+// kLargeCluster), and the small ones at kcap (block scans over the flags),
+// and their counts.
 __global__ __launch_bounds__(256) void k_keep_lists(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     const uint32_t ncl = F.counters[kNClusters];
     const int t = threadIdx.x;
-    uint32_t nl = 0, ns = 0;
-    for (uint32_t c0 = 0; c0 < ncl; c0 += 256) {
-        const uint32_t c = c0 + t;
-        const bool kept = c < ncl && (F.cflag[c] & 1u);
-        const bool large = kept && F.cstart[c + 1] - F.cstart[c] > kLargeCluster;
-        uint32_t tl, ts;
-        const uint32_t pl = bscan(large ? 1u : 0u, &tl);
-        const uint32_t ps = bscan(kept && !large ? 1u : 0u, &ts);
-        if (large && nl + pl < F.kcap) F.klist[nl + pl] = c;
-        if (kept && !large && ns + ps < F.kcap) F.klist[F.kcap + ns + ps] = c;
-        nl += tl;
-        ns += ts;
+    uint32_t nh = 0, nl = 0, ns = 0;
+    // the huge ones, then the large ones after them
+    for (int pass = 0; pass < 2; pass++) {
+        for (uint32_t c0 = 0; c0 < ncl; c0 += 256) {
+            const uint32_t c = c0 + t;
+            const bool kept = c < ncl && (F.cflag[c] & 1u);
+            const uint32_t sz = kept ? F.cstart[c + 1] - F.cstart[c] : 0u;
+            const bool huge = sz > kHugeCluster, large = sz > kLargeCluster && !huge;
+            const bool mine = pass == 0 ? huge : large;
+            uint32_t tot;
+            const uint32_t pre = bscan(mine ? 1u : 0u, &tot);
+            const uint32_t at = (pass == 0 ? 0u : nh) + (pass == 0 ? nh : nl) + pre;
+            if (mine && at < F.kcap) F.klist[at] = c;
+            if (pass == 0) {
+                nh += tot;
+            } else {
+                nl += tot;
+                uint32_t ts;
+                const uint32_t ps = bscan(kept && sz <= kLargeCluster ? 1u : 0u, &ts);
+                if (kept && sz <= kLargeCluster && ns + ps < F.kcap) F.klist[F.kcap + ns + ps] = c;
+                ns += ts;
+            }
+        }
     }
     if (t == 0) {
-        if (nl > F.kcap || ns > F.kcap) atomicOr(F.counters + kError, 64u);
-        F.counters[kNLarge] = min(nl, F.kcap);
+        if (nh + nl > F.kcap || ns > F.kcap) atomicOr(F.counters + kError, 64u);
+        nh = min(nh, F.kcap);
+        F.counters[kNHuge] = nh;
+        F.counters[kNLarge] = min(nl, F.kcap - nh);
         F.counters[kNSmall] = min(ns, F.kcap);
     }
+}
+
+// The cluster at global index g of the kernels of one cluster a block
+// (list 0: the huge clusters of every frame, list 4, then those of list 0;
+// list 1: the small ones): frame fi and cluster c, false past the end.
+// The cursors follow g, which only grows.
+struct TakeCursor {
+    uint32_t fh = 0, fl = 0;
+};
+__device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue *Q, int l, uint32_t g,
+                                           TakeCursor &cur, uint32_t *fi, uint32_t *c) {
+    if (l == 0) {
+        const uint32_t th = Q->total[4];
+        if (g < th) {
+            const uint32_t *off = Q->off[4];
+            while (off[cur.fh + 1] <= g) cur.fh++;
+            *fi = cur.fh;
+            *c = frames[cur.fh].klist[g - off[cur.fh]];
+            return true;
+        }
+        g -= th;
+    }
+    if (g >= Q->total[l]) return false;
+    const uint32_t *off = Q->off[l];
+    while (off[cur.fl + 1] <= g) cur.fl++;
+    const Frame &F = frames[cur.fl];
+    *fi = cur.fl;
+    *c = F.klist[(l ? F.kcap : F.counters[kNHuge]) + (g - off[cur.fl])];
+    return true;
 }
 
 // One block: where each frame's items start in list l (counter `which` of
@@ -645,13 +691,10 @@ __global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const W
     static_assert(kSortTile == 2048, "4 waves x 512 keys");
     __shared__ uint64_t sk[kSortTile];
     const uint32_t t = threadIdx.x;
-    const uint32_t total = Q->total[0];
-    const uint32_t *off = Q->off[0];
-    uint32_t fi = 0;
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (off[fi + 1] <= g) fi++;
+    TakeCursor cur;
+    uint32_t fi, c;
+    for (uint32_t g = blockIdx.x; cluster_at(frames, Q, 0, g, cur, &fi, &c); g += gridDim.x) {
         const Frame &F = frames[fi];
-        const uint32_t c = F.klist[g - off[fi]];
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         uint64_t *keys = F.skey + s0;
         uint32_t m = kSortTile;
@@ -707,16 +750,14 @@ __global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, con
 template <int NT, int CH>
 __global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
                                                    int keys_ready, int l) {
-    __shared__ double sm[6][CH];
+    __shared__ double sm[6][CH + 2];  // rows 2 doubles apart in the banks: the scan's lanes conflict-free
+    __shared__ uint32_t fls[4 * NT + 1], flm[4 * NT + 1], fcnt;
     const int t = threadIdx.x;
     {
-    const uint32_t total = Q->total[l];
-    const uint32_t *off = Q->off[l];
-    uint32_t fi = 0;
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (off[fi + 1] <= g) fi++;
+    TakeCursor cur;
+    uint32_t fi, c;
+    for (uint32_t g = blockIdx.x; cluster_at(frames, Q, l, g, cur, &fi, &c); g += gridDim.x) {
         const Frame &F = frames[fi];
-        const uint32_t c = F.klist[(l ? F.kcap : 0u) + (g - off[fi])];
         uint64_t *skey = parity && !keys_ready ? F.skey2 : F.skey;
         const uint32_t *sval = parity ? F.sval2 : F.sval;
         const int64_t pc = F.pcap;
@@ -729,11 +770,9 @@ __global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int par
                 keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
             __syncthreads();
         }
-        // fix_runs (fast/quads.c:304-327): the runs of equal slopes split
-        // among the waves by where they start; the keys in LDS (the moment
+        // fix_runs (fast/quads.c:304-327), the keys in LDS (the moment
         // buffer, free until then) when they fit
         {
-            constexpr int kW = NT / 64;
             constexpr uint32_t kFixLds = 6 * CH;
             uint64_t *kl = reinterpret_cast<uint64_t *>(&sm[0][0]);
             const bool in_lds = n <= kFixLds;
@@ -741,17 +780,15 @@ __global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int par
                 for (uint32_t i = t; i < n; i += NT) kl[i] = keys[i];
                 __syncthreads();
             }
-            const uint32_t seg = (n + kW - 1) / kW, w = (uint32_t)(t >> 6);
-            fix_runs_wave(in_lds ? kl : keys, n, F.okey + s0, min(n, w * seg), min(n, (w + 1) * seg),
-                          reinterpret_cast<uint64_t *>(F.errs + s0), reinterpret_cast<uint64_t *>(F.filt + s0));
-            __syncthreads();
+            fix_runs_block<NT>(in_lds ? kl : keys, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
+                               reinterpret_cast<uint64_t *>(F.filt + s0), fls, flm, &fcnt);
             if (in_lds) {
                 for (uint32_t i = t; i < n; i += NT) keys[i] = kl[i];
                 __syncthreads();
             }
         }
         // weights (fast/quads.c:968-1004) and the six running sums
-        double run[6] = {0, 0, 0, 0, 0, 0};
+        double run = 0.0;  // lane q < 6: running sum q
         for (uint32_t c0 = 0; c0 < n; c0 += CH) {
             uint32_t len = min((uint32_t)CH, n - c0);
             constexpr int kPer = CH / NT;  // points a thread, their loads issued together
@@ -780,28 +817,36 @@ __global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int par
             }
             __syncthreads();
             if (t < 6) {
-                // one lane per sum, in order; the loads of 8 terms are
-                // issued together ahead of their adds
-                double s = run[t];
+                // one lane per sum, in order; each of the 8 registers takes
+                // the next group's term as soon as its add has used it
+                double s = run;
                 double *a = sm[t];
+                const uint32_t len8 = len & ~7u;
                 uint32_t j = 0;
-                for (; j + 8 <= len; j += 8) {
-                    double v[8];
+                double v[8];
+                if (len8) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) v[q] = a[j + q];
+                    for (int q = 0; q < 8; q++) v[q] = a[q];
+                    for (; j + 8 < len8; j += 8) {
+#pragma unroll
+                        for (int q = 0; q < 8; q++) {
+                            s = __dadd_rn(s, v[q]);
+                            a[j + q] = s;
+                            v[q] = a[j + 8 + q];
+                        }
+                    }
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
                         s = __dadd_rn(s, v[q]);
-                        v[q] = s;
+                        a[j + q] = s;
                     }
-#pragma unroll
-                    for (int q = 0; q < 8; q++) a[j + q] = v[q];
+                    j += 8;
                 }
                 for (; j < len; j++) {
                     s = __dadd_rn(s, a[j]);
                     a[j] = s;
                 }
-                run[t] = s;
+                run = s;
             }
             __syncthreads();
             for (uint32_t j = t; j < len; j += NT)
@@ -959,13 +1004,10 @@ __global__ __launch_bounds__(NT) void k_fit_corners(const Frame *frames, const D
     const double INF = __longlong_as_double(0x7ff0000000000000ll);
     double *pe = sh, *pm = sh + kMaxNm * kMaxNm, *pnx = sh + 2 * kMaxNm * kMaxNm, *pny = sh + 3 * kMaxNm * kMaxNm;
     {
-    const uint32_t total = Q->total[l];
-    const uint32_t *off = Q->off[l];
-    uint32_t fi = 0;
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (off[fi + 1] <= g) fi++;
+    TakeCursor cur;
+    uint32_t fi, c;
+    for (uint32_t g = blockIdx.x; cluster_at(frames, Q, l, g, cur, &fi, &c); g += gridDim.x) {
         const Frame &F = frames[fi];
-        const uint32_t c = F.klist[(l ? F.kcap : 0u) + (g - off[fi])];
         const int64_t pc = F.pcap;
         uint32_t flag = F.cflag[c];
         if (!(flag & 1u)) continue;
@@ -1358,6 +1400,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     else
         hipLaunchKernelGGL(k_fit_prep<256>, dim3(256, nframes), dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 4, (int)kNHuge);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 0, (int)kNLarge);
     hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 1, (int)kNSmall);
     mk->mark(st, "fit_prep");

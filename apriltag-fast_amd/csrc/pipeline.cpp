@@ -71,9 +71,9 @@ int64_t cluster_map_size(int64_t w, int64_t h) {
 }
 
 // The persistent kernels' work lists of a call (in the arena, after the
-// frames' regions): the header and three offset arrays over the frames.
+// frames' regions): the header and an offset array over the frames per list.
 size_t queue_bytes(int64_t nframes) {
-    return align256(sizeof(WorkQueue)) + 4 * align256((size_t)(nframes + 1) * 4);
+    return align256(sizeof(WorkQueue)) + kLists * align256((size_t)(nframes + 1) * 4);
 }
 
 // Kept clusters need at least a few points each, so a quarter of the point
@@ -538,7 +538,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         uint8_t *qb = ctx->arena + qoff + align256(sizeof(WorkQueue));
         WorkQueue hq{};
         hq.nframes = (uint32_t)n;
-        for (int l = 0; l < 4; l++)
+        for (int l = 0; l < kLists; l++)
             hq.off[l] = reinterpret_cast<uint32_t *>(qb + (size_t)l * align256((size_t)(n + 1) * 4));
         ctx->qstage = hq;
     }
