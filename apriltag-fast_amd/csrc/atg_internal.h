@@ -36,6 +36,8 @@ namespace atg {
 
 constexpr int kMaxFamilies = 16;
 constexpr int kSortBlocks = 32;      // radix-sort blocks per frame
+constexpr int kRadixBits = 11;       // bits of a radix-sort digit
+constexpr int kRadixBins = 1 << kRadixBits;
 constexpr int kMaxQuadsPerFrame = 4096;
 constexpr int kMaxDetsPerFrame = 1024;
 
@@ -108,7 +110,7 @@ struct Frame {
     uint64_t *pts, *cpts;
     uint64_t *skey, *skey2;
     uint32_t *sval, *sval2;
-    uint32_t *rhist;      // [256][kSortBlocks]
+    uint32_t *rhist;      // [kSortBlocks][kRadixBins]
     uint64_t *pw;         // [pcap] kept clusters' points: x | y << 16 | squared gradient << 32,
                           // in cluster order (the memory of pts, free by then)
     uint32_t *rwide;      // [qh] rows for the replay's wide launch

@@ -8,10 +8,13 @@
 //   k_point_scan   the row offsets of the points.
 //   k_root_*       the roots of clusters that can hold points (the ok
 //                  roots) numbered densely in node order.
-//   radix sort     stable LSD sort of (key, index): the keys order the
-//                  clusters by (hash bucket, id) as the reference's cluster
-//                  table does (fast/clusters.c:338-356); stability keeps
-//                  each cluster's points in emission order.
+//   radix sort     stable LSD sort of (key, index), kRadixBits a pass: the
+//                  keys order the clusters by (hash bucket, id) as the
+//                  reference's cluster table does (fast/clusters.c:338-356);
+//                  stability keeps each cluster's points in emission order.
+//                  Where the key and the point index fit one word (the
+//                  index above the key's bits: a packed frame) the sort
+//                  moves that word alone.
 //   k_cluster_*    the points regrouped and the cluster starts compacted.
 //
 // Cluster sort key of a point: bucket << 2D | dense(max_root) << D |
@@ -72,6 +75,14 @@ __device__ __forceinline__ uint32_t point_mask(const uint8_t *a, const uint8_t *
     return c10 | (c01 << 1) | ((cm1 & (p11 ^ 1u)) << 2) | (c11 << 3);
 }
 
+// Whether the frame's cluster sort is packed: its key bits (counters
+// [kSortBits], known once the ok roots are numbered) and the bits of a
+// point index (below pcap) fit one word.
+__device__ __forceinline__ bool sort_packed(const Frame &F) {
+    return F.counters[kSortBits] + (64u - (uint32_t)__clzll((unsigned long long)F.pcap)) <= 64u;
+}
+__device__ __forceinline__ uint64_t low_bits(uint32_t b) { return b >= 64 ? ~0ull : (1ull << b) - 1ull; }
+
 // Pass 1 stages each 64-pixel step's points (at most 4 a pixel) in LDS and
 // writes them out contiguously; the sort value of a point is its index,
 // which the sort's first pass takes from the position.
@@ -87,6 +98,8 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
     uint32_t count = 0;
     uint32_t base = pass ? F.poff[y] : 0u;
     const uint32_t D = pass ? F.counters[kRootBits] : 0u;
+    const uint32_t sbits = pass ? F.counters[kSortBits] : 0u;
+    const bool packed = pass && sort_packed(F);
     const uint64_t mask = F.bucket_mask;
     for (int x0 = 0; x0 < QW - 1; x0 += 64) {
         int x = x0 + lane;
@@ -122,7 +135,7 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
                 const int64_t at = (int64_t)at0 + q;
                 if (at >= F.pcap) break;  // flagged by k_point_scan
                 F.pts[at] = st_pt[wv][q];
-                F.skey[at] = st_key[wv][q];
+                F.skey[at] = st_key[wv][q] | (packed ? (uint64_t)at << sbits : 0ull);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             __builtin_amdgcn_wave_barrier();
@@ -228,78 +241,150 @@ __global__ __launch_bounds__(256) void k_root_dense(const Frame *frames) {
     }
 }
 
-// dyn: the pass is skipped where the frame's key (counters[kSortBits]) has
-// no bits at shift; the skipped passes are the last ones.
-__global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int which, int shift, int parity, int dyn) {
-    __shared__ uint32_t h[256];
-    const Frame F = frames[blockIdx.y];
-    if (dyn && shift >= (int)F.counters[kSortBits]) return;
-    uint32_t n = F.counters[which];
-    SortBufs B = sort_bufs(F, parity);
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t s0, s1;
-    seg_of(n, blockIdx.x, &s0, &s1);
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) atomicAdd(&h[(B.kin[i] >> shift) & 255u], 1u);
-    __syncthreads();
-    F.rhist[threadIdx.x * kSortBlocks + blockIdx.x] = h[threadIdx.x];
+// The lanes of the wave holding the same digit as this lane (of those
+// with `valid`).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < kRadixBits; bit++) {
+        const uint64_t bb = __ballot((d >> bit) & 1u);
+        peers &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    return peers;
 }
 
-// Stable scatter: chunks of 256 keys in order; inside a chunk, the rank of
-// a key among the keys of its digit before it: within a wave by ballots
-// (peers = lanes of the same digit), across waves by per-wave counts.
-// iota: the values are the keys' positions (the first pass of the cluster
-// sort; its points were emitted in order), not read.
-__global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity,
-                                                       int dyn, int iota) {
-    __shared__ uint32_t off[256];
-    __shared__ uint32_t wcnt[4][256];
+// dyn: the pass is skipped where the frame's key (counters[kSortBits]) has
+// no bits at shift; the skipped passes are the last ones. pk: the frame's
+// words are packed where sort_packed says so (the key is their low
+// kSortBits bits).
+// Digit counts of each block's segment, rhist[block][digit]; the lanes of
+// a wave with the same digit add once.
+__global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int which, int shift, int parity, int dyn,
+                                                    int pk) {
+    __shared__ uint32_t h[kRadixBins];
     const Frame F = frames[blockIdx.y];
-    if (dyn && shift >= (int)F.counters[kSortBits]) return;
-    uint32_t n = F.counters[which];
-    SortBufs B = sort_bufs(F, parity);
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    {
-        const uint32_t *hd = F.rhist + t * kSortBlocks;
-        uint32_t tot = 0, pre = 0;
-        for (int b = 0; b < kSortBlocks; b++) {
-            uint32_t v = hd[b];
-            if (b < (int)blockIdx.x) pre += v;
-            tot += v;
-        }
-        uint32_t all;
-        uint32_t base = block_excl_scan(tot, &all);
-        off[t] = base + pre;
-        for (int k = 0; k < 4; k++) wcnt[k][t] = 0;
-    }
+    const uint32_t sb = F.counters[kSortBits];
+    if (dyn && shift >= (int)sb) return;
+    const uint64_t km = pk && sort_packed(F) ? low_bits(sb) : ~0ull;
+    const uint32_t n = F.counters[which];
+    const SortBufs B = sort_bufs(F, parity);
+    const int t = threadIdx.x, lane = t & 63;
+    for (int i = t; i < kRadixBins; i += 256) h[i] = 0;
     __syncthreads();
     uint32_t s0, s1;
     seg_of(n, blockIdx.x, &s0, &s1);
     const uint64_t lt = (1ull << lane) - 1ull;
     for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
-        uint32_t i = c0 + t;
-        bool valid = i < s1;
-        uint64_t key = valid ? B.kin[i] : 0ull;
-        uint32_t val = valid ? (iota ? i : B.vin[i]) : 0u;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
+        const uint32_t i = c0 + t;
+        const bool valid = i < s1;
+        const uint32_t d = valid ? (uint32_t)((B.kin[i] & km) >> shift) & (kRadixBins - 1) : 0u;
+        const uint64_t peers = digit_peers(d, valid);
+        if (valid && !(peers & lt)) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    for (int i = t; i < kRadixBins; i += 256) F.rhist[blockIdx.x * kRadixBins + i] = h[i];
+}
+
+// Exclusive scan over a block of NT threads; *total the sum.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t ws[NT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan_u(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int i = 0; i < NT / 64; i++) {
+        pre += i < w ? ws[i] : 0u;
+        tot += ws[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+// Per frame (a block of kOffThreads): the digit counts turned into each
+// block's first output position per digit, in place (digits in order,
+// blocks in order within a digit). Thread t holds the digits t,
+// t + kOffThreads, ...: a digit's total, its base among the digits (a
+// block scan per group), then its blocks' positions.
+constexpr int kOffThreads = 1024;
+__global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *frames, int shift, int dyn) {
+    constexpr int kPer = kRadixBins / kOffThreads;
+    static_assert(kPer * kOffThreads == kRadixBins, "whole digits a thread");
+    const Frame F = frames[blockIdx.y];
+    if (dyn && shift >= (int)F.counters[kSortBits]) return;
+    const int t = threadIdx.x;
+    uint32_t carry = 0;
+    for (int q = 0; q < kPer; q++) {
+        const int d = q * kOffThreads + t;
+        uint32_t v[kSortBlocks], tot = 0;
 #pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-            uint64_t bb = __ballot((d >> bit) & 1u);
-            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        for (int b = 0; b < kSortBlocks; b++) {
+            v[b] = F.rhist[b * kRadixBins + d];
+            tot += v[b];
         }
-        uint32_t rank = __popcll(peers & lt);
-        if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+        uint32_t all;
+        uint32_t run = carry + block_excl_scan_t<kOffThreads>(tot, &all);
+        carry += all;
+#pragma unroll
+        for (int b = 0; b < kSortBlocks; b++) {
+            F.rhist[b * kRadixBins + d] = run;
+            run += v[b];
+        }
+    }
+}
+
+// Stable scatter: chunks of 256 keys in order; inside a chunk, the rank of
+// a key among the keys of its digit before it: within a wave by ballots
+// (peers = lanes of the same digit), across waves by per-wave counts. The
+// counts carry their chunk's number (a count of another chunk reads as 0),
+// so that no chunk clears them. iota: the values are the keys' positions
+// (the first pass of an unpacked cluster sort; its points were emitted in
+// order), not read.
+__global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity,
+                                                       int dyn, int iota, int pk) {
+    __shared__ uint32_t off[kRadixBins];
+    __shared__ uint32_t wcnt[4][kRadixBins];  // chunk << 16 | count
+    const Frame F = frames[blockIdx.y];
+    const uint32_t sb = F.counters[kSortBits];
+    if (dyn && shift >= (int)sb) return;
+    const bool packed = pk && sort_packed(F);
+    const uint64_t km = packed ? low_bits(sb) : ~0ull;
+    const uint32_t n = F.counters[which];
+    const SortBufs B = sort_bufs(F, parity);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int i = t; i < kRadixBins; i += 256) {
+        off[i] = F.rhist[blockIdx.x * kRadixBins + i];
+        for (int k = 0; k < 4; k++) wcnt[k][i] = 0xffff0000u;
+    }
+    __syncthreads();
+    uint32_t s0, s1;
+    seg_of(n, blockIdx.x, &s0, &s1);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t chunk = 0;  // below 0xffff: segments of at most 2^24 keys
+    for (uint32_t c0 = s0; c0 < s1; c0 += 256, chunk++) {
+        const uint32_t i = c0 + t;
+        const bool valid = i < s1;
+        const uint64_t key = valid ? B.kin[i] : 0ull;
+        const uint32_t val = (valid && !packed) ? (iota ? i : B.vin[i]) : 0u;
+        const uint32_t d = (uint32_t)((key & km) >> shift) & (kRadixBins - 1);
+        const uint64_t peers = digit_peers(d, valid);
+        const uint32_t rank = (uint32_t)__popcll(peers & lt), cnt = (uint32_t)__popcll(peers);
+        const bool lead = valid && rank == 0;
+        if (lead) wcnt[w][d] = (chunk << 16) | cnt;
         __syncthreads();
         if (valid) {
             uint32_t pos = off[d] + rank;
-            for (int k = 0; k < w; k++) pos += wcnt[k][d];
+            for (int k = 0; k < w; k++) {
+                const uint32_t e = wcnt[k][d];
+                pos += (e >> 16) == chunk ? (e & 0xffffu) : 0u;
+            }
             B.kout[pos] = key;
-            B.vout[pos] = val;
+            if (!packed) B.vout[pos] = val;
         }
         __syncthreads();
-        off[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
-        for (int k = 0; k < 4; k++) wcnt[k][t] = 0;
+        if (lead) atomicAdd(&off[d], cnt);
         __syncthreads();
     }
 }
@@ -307,7 +392,9 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
 // ---------------------------------------------------------------- clusters
 
 // The buffer of the cluster sort's result: one swap per executed pass.
-__device__ __forceinline__ int sorted_parity(const Frame &F) { return (int)(((F.counters[kSortBits] + 7) / 8) & 1); }
+__device__ __forceinline__ int sorted_parity(const Frame &F) {
+    return (int)(((F.counters[kSortBits] + kRadixBits - 1) / kRadixBits) & 1);
+}
 
 // After the sort: the points in cluster order, and the number of cluster
 // starts in each block's segment.
@@ -317,12 +404,16 @@ __global__ __launch_bounds__(256) void k_cluster_count(const Frame *frames) {
     uint32_t n = F.counters[kNPts];
     const uint64_t *key = parity ? F.skey2 : F.skey;
     const uint32_t *val = parity ? F.sval2 : F.sval;
+    const uint32_t sb = F.counters[kSortBits];
+    const bool packed = sort_packed(F);
+    const uint64_t km = packed ? low_bits(sb) : ~0ull;
     uint32_t s0, s1;
     seg_of(n, blockIdx.x, &s0, &s1);
     uint32_t c = 0;
     for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) {
-        F.cpts[i] = F.pts[val[i]];
-        c += (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+        const uint64_t k = key[i];
+        F.cpts[i] = F.pts[packed ? (uint32_t)(k >> sb) : val[i]];
+        c += (i == 0 || (k & km) != (key[i - 1] & km)) ? 1u : 0u;
     }
     uint32_t tot;
     block_excl_scan(c, &tot);
@@ -334,6 +425,7 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
     const int parity = sorted_parity(F);
     uint32_t n = F.counters[kNPts];
     const uint64_t *key = parity ? F.skey2 : F.skey;
+    const uint64_t km = sort_packed(F) ? low_bits(F.counters[kSortBits]) : ~0ull;
     uint32_t base = 0, all = 0;
     for (int b = 0; b < kSortBlocks; b++) {
         uint32_t v = F.rhist[b];
@@ -344,7 +436,8 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
     seg_of(n, blockIdx.x, &s0, &s1);
     for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
         uint32_t i = c0 + threadIdx.x;
-        uint32_t f = (i < s1 && (i == 0 || key[i] != key[i - 1])) ? 1u : 0u;
+        const uint64_t k = i < s1 ? key[i] & km : 0ull;
+        uint32_t f = (i < s1 && (i == 0 || k != (key[i - 1] & km))) ? 1u : 0u;
         uint32_t tot;
         uint32_t pre = block_excl_scan(f, &tot);
         if (f) {
@@ -352,7 +445,7 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
             // the cluster id back from the sort key's dense roots
             const uint32_t D = F.counters[kRootBits];
             const uint64_t dm = ((uint64_t)1 << D) - 1;
-            F.cid[base + pre] = ((uint64_t)F.rinv[(key[i] >> D) & dm] << 32) | F.rinv[key[i] & dm];
+            F.cid[base + pre] = ((uint64_t)F.rinv[(k >> D) & dm] << 32) | F.rinv[k & dm];
         }
         base += tot;
     }
@@ -364,13 +457,16 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
 
 // ---------------------------------------------------------------- launchers
 
+// pk: the cluster sort (packed frames move one word); else (key, value)
+// pairs throughout.
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
-                       int dyn, int iota) {
-    for (int shift = 0; shift < nbits; shift += 8) {
+                       int dyn, int iota, int pk) {
+    for (int shift = 0; shift < nbits; shift += kRadixBits) {
         hipLaunchKernelGGL(k_radix_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity, dyn);
+                           *parity, dyn, pk);
+        hipLaunchKernelGGL(k_radix_offsets, dim3(1, nframes), dim3(kOffThreads), 0, st, dframes, shift, dyn);
         hipLaunchKernelGGL(k_radix_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity, dyn, shift == 0 ? iota : 0);
+                           *parity, dyn, shift == 0 ? iota : 0, pk);
         *parity ^= 1;
     }
 }
@@ -387,7 +483,7 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     mk->mark(st, "points");
     // key_bits: the widest key the call's frames can have
     int parity = 0;
-    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 1);
+    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 1, 1);
     mk->mark(st, "cluster_sort");
     hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);

@@ -1340,7 +1340,7 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 }
 
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
-                       int dyn, int iota);
+                       int dyn, int iota, int pk);
 
 void warm_decode_grid(int fit_blocks);
 
@@ -1409,7 +1409,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_angle_sort, dim3(pgrid(k_angle_sort, 256, fit_blocks)), dim3(256), 0, st, dframes, Q);
         hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, st, dframes, Q);
     } else {
-        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0);
+        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0, 0);
     }
     mk->mark(st, "angle_sort");
     hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256), 0,

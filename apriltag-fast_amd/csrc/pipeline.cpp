@@ -332,7 +332,7 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->skey2 = (uint64_t *)take((size_t)pc * 8);
     F->sval = (uint32_t *)take((size_t)pc * 4);
     F->sval2 = (uint32_t *)take((size_t)pc * 4);
-    F->rhist = (uint32_t *)take((size_t)256 * kSortBlocks * 4);
+    F->rhist = (uint32_t *)take((size_t)kRadixBins * kSortBlocks * 4);
     F->rwide = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->rdense = (uint32_t *)take((size_t)nodes * 4);
     F->rinv = (uint32_t *)take((size_t)nodes * 4);
