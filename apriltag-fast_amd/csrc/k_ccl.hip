@@ -454,11 +454,11 @@ __host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix, int 
     ReplayLayout r;
     size_t o = (size_t)eb * (2 * C + 2);  // ent: [2C + 2] x (parent, weight), 8 bytes or packed in 4
     r.le = o;
-    o += (size_t)4 * L;                  // link: current run << 16 | record
+    o += (size_t)4 * L;                  // link: current run << 16 | record, by group
     r.nxt = o;
-    o = (o + (size_t)ix * L + 3) & ~(size_t)3;  // next link of the same group
+    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // each group's end in le
     r.head = o;
-    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // first link of each group
+    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // each group's start in le
     r.labA = o;
     o = (o + (size_t)2 * C + 3) & ~(size_t)3;   // record of each previous-row run
     r.hasl = o;
@@ -554,7 +554,6 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     using U16 = typename Mem<G>::U16;
     using IX = typename std::conditional<G, volatile uint32_t, uint16_t>::type;
     using U8 = typename std::conditional<G, volatile uint8_t, uint8_t>::type;
-    constexpr uint32_t kNone = G ? 0xffffffffu : 0xffffu;
     const int lane = threadIdx.x;
     const int QW = F.qw;
     const ReplayLayout ly = replay_layout(C, L, G ? 4 : 2, PK ? 4 : 8);
@@ -580,7 +579,6 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
         ent.set((uint32_t)(m + i), (uint32_t)(m + i), run_end(r) - run_start(r) + 1);
     }
     for (int k = lane; k < nprev; k += 64) labA[k] = (uint16_t)F.rlab[pb + k];
-    for (int g = lane; g < ng; g += 64) head[g] = kNone;
     if (lane == 0) {
         ent.set((uint32_t)ev, (uint32_t)ev, 1u);
     }
@@ -591,60 +589,82 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
         lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
     }
     wsync<G>();
-    // links with the previous run's record; nxt[e] holds the link's group
-    // until the next pass replaces it
-    for (int e = lane; e < ne; e += 64) {
-        uint32_t ed = edges[e];
-        uint32_t l = labA[ed & 0xffffu];
-        le[e] = (ed & 0xffff0000u) | l;
-        hasl[l] = 1;
-        nxt[e] = rlab_cur[ed >> 16];
-    }
-    wsync<G>();
-    // per group, its links in order as a list: chunks of 64 links from the
-    // last, the lanes of each group in a chunk found by ballot
-    for (int b0 = (ne - 1) & ~63; b0 >= 0; b0 -= 64) {
-        const int e = b0 + lane;
-        const uint32_t g = e < ne ? (uint32_t)nxt[e] : 0xffffffffu;
-        uint64_t todo = __ballot(e < ne);
-        while (todo) {
-            const int lead = __ffsll((unsigned long long)todo) - 1;
-            const uint32_t gl = __builtin_amdgcn_readlane(g, lead);
-            const uint64_t same = __ballot(g == gl);
-            todo &= ~same;
-            if (g == gl) {
-                const uint64_t hi = lane == 63 ? 0ull : same >> (lane + 1);
-                nxt[e] = hi ? (uint32_t)(e + __ffsll((unsigned long long)hi)) : (uint32_t)head[gl];
-            }
-            if (lane == lead) head[gl] = (uint32_t)e;
+    // the links by group, stably (each group's links in order, contiguous
+    // in le): counts per group (head), their starts (head) and running ends
+    // (nxt) by a scan, then each link at its group's end plus its rank among
+    // the group's lanes of its chunk of 64 (peers by ballot over the group
+    // bits; one lane of each group adds)
+    const int gbits = ng <= 1 ? 1 : 32 - __clz(ng - 1);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    auto group_peers = [&](uint32_t g, bool valid) {
+        uint64_t peers = __ballot(valid);
+        for (int bit = 0; bit < gbits; bit++) {
+            const uint64_t bb = __ballot((g >> bit) & 1u);
+            peers &= ((g >> bit) & 1u) ? bb : ~bb;
         }
+        return peers;
+    };
+    for (int g = lane; g < ng; g += 64) head[g] = 0;
+    wsync<G>();
+    for (int b0 = 0; b0 < ne; b0 += 64) {
+        const int e = b0 + lane;
+        const bool valid = e < ne;
+        const uint32_t ed = valid ? edges[e] : 0u;
+        const uint32_t g = valid ? rlab_cur[ed >> 16] : 0u;
+        if (valid) hasl[labA[ed & 0xffffu]] = 1;
+        const uint64_t peers = group_peers(g, valid);
+        if (valid && !(peers & lt)) head[g] = (uint32_t)head[g] + (uint32_t)__popcll(peers);
+        wsync<G>();
+    }
+    for (int g0 = 0, carry = 0; g0 < ng; g0 += 64) {
+        const int g = g0 + lane;
+        const uint32_t c = g < ng ? (uint32_t)head[g] : 0u;
+        uint32_t inc = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (g < ng) {
+            head[g] = (uint32_t)carry + inc - c;
+            nxt[g] = (uint32_t)carry + inc - c;
+        }
+        carry += (int)__shfl(inc, 63, 64);
     }
     wsync<G>();
+    for (int b0 = 0; b0 < ne; b0 += 64) {
+        const int e = b0 + lane;
+        const bool valid = e < ne;
+        const uint32_t ed = valid ? edges[e] : 0u;
+        const uint32_t g = valid ? rlab_cur[ed >> 16] : 0u;
+        const uint64_t peers = group_peers(g, valid);
+        if (valid) le[(uint32_t)nxt[g] + (uint32_t)__popcll(peers & lt)] = (ed & 0xffff0000u) | labA[ed & 0xffffu];
+        wsync<G>();
+        if (valid && !(peers & lt)) nxt[g] = (uint32_t)nxt[g] + (uint32_t)__popcll(peers);
+        wsync<G>();
+    }
     // the groups that have links, listed (in labA, free by now), so that
     // the lanes share out those and not the rows' many linkless records
     U16 *glist = labA;
     int ngl = 0;
     for (int g0 = 0; g0 < ng; g0 += 64) {
         const int g = g0 + lane;
-        const bool has = g < ng && (uint32_t)head[g] != kNone;
+        const bool has = g < ng && (uint32_t)nxt[g] != (uint32_t)head[g];
         const uint64_t hb = __ballot(has);
-        if (has) glist[ngl + __popcll(hb & ((1ull << lane) - 1ull))] = (uint16_t)g;
+        if (has) glist[ngl + __popcll(hb & lt)] = (uint16_t)g;
         ngl += __popcll(hb);
     }
     wsync<G>();
     // each lane replays its groups' links in order; the current run's set
     // is kept in registers across its links, the next link is read ahead
     for (int k = lane; k < ngl; k += 64) {
-        uint32_t e = head[glist[k]];
+        const uint32_t g = glist[k];
+        uint32_t e = head[g];
+        const uint32_t e1 = nxt[g];
         int cur_i = -1;
         uint32_t ra = 0, wa = 0;
-        uint32_t ed = le[e], en = nxt[e];
+        uint32_t ed = le[e];
         for (;;) {
-            uint32_t edn = 0, enn = kNone;
-            if (en != kNone) {
-                edn = le[en];
-                enn = nxt[en];
-            }
+            const uint32_t edn = e + 1 < e1 ? (uint32_t)le[e + 1] : 0u;
             const uint32_t i = ed >> 16, l = ed & 0xffffu;
             if ((int)i != cur_i) {
                 if (cur_i >= 0) ent.set_wt(ra, wa);
@@ -662,9 +682,8 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
                 }
                 wa += b.y;
             }
-            if (en == kNone) break;
+            if (++e == e1) break;
             ed = edn;
-            en = enn;
         }
         ent.set_wt(ra, wa);
     }
