@@ -655,24 +655,47 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     }
     wsync<G>();
     // each lane replays its groups' links in order; the current run's set
-    // is kept in registers across its links, the next link is read ahead
+    // is kept in registers across its links, the next link is read ahead.
+    // cr: the group's last root, cw its weight (as stored, since the run
+    // change that left it; while cr == ra the stored weight lags, unread):
+    // a record whose parent is cr needs no read of cr (find's first step)
     for (int k = lane; k < ngl; k += 64) {
         const uint32_t g = glist[k];
         uint32_t e = head[g];
         const uint32_t e1 = nxt[g];
         int cur_i = -1;
-        uint32_t ra = 0, wa = 0;
+        uint32_t ra = 0, wa = 0, cr = 0xffffffffu, cw = 0;
         uint32_t ed = le[e];
         for (;;) {
             const uint32_t edn = e + 1 < e1 ? (uint32_t)le[e + 1] : 0u;
             const uint32_t i = ed >> 16, l = ed & 0xffffu;
             if ((int)i != cur_i) {
-                if (cur_i >= 0) ent.set_wt(ra, wa);
+                if (cur_i >= 0) {
+                    ent.set_wt(ra, wa);
+                    cr = ra;
+                    cw = wa;
+                }
                 cur_i = (int)i;
                 ra = (uint32_t)m + i;  // a current run's first link: a fresh root
                 wa = ent.wt(ra);
             }
-            uint2 b = ent.find(l);
+            uint2 b;
+            {
+                const uint2 v = ent.get(l);
+                if (v.x == l) {
+                    b = v;
+                } else if (v.x == cr) {
+                    b = make_uint2(cr, cw);
+                } else {
+                    const uint2 pv = ent.get(v.x);
+                    if (pv.x == v.x) {
+                        b = pv;
+                    } else {
+                        ent.set(l, pv.x, v.y);  // find's halving step on l
+                        b = ent.find(pv.x);
+                    }
+                }
+            }
             if (b.x != ra) {
                 if (wa > b.y) {
                     ent.set_par(b.x, ra);
@@ -681,6 +704,7 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
                     ra = b.x;
                 }
                 wa += b.y;
+                cr = ra;
             }
             if (++e == e1) break;
             ed = edn;
