@@ -456,9 +456,9 @@ __host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix, int 
     r.le = o;
     o += (size_t)4 * L;                  // link: current run << 16 | record, by group
     r.nxt = o;
-    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // each group's end in le
+    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // each current run's record
     r.head = o;
-    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // each group's start in le
+    o = (o + (size_t)ix * C + 3) & ~(size_t)3;  // each group's end in le
     r.labA = o;
     o = (o + (size_t)2 * C + 3) & ~(size_t)3;   // record of each previous-row run
     r.hasl = o;
@@ -570,15 +570,34 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
     const uint32_t *rlab_cur = F.rlab + cb;
     const int ev = m + ncur;
-    for (int l = lane; l < m; l += 64) {
-        ent.set((uint32_t)l, (uint32_t)l, F.recw[pb + l]);
-        hasl[l] = 0;
+    // the elements (records with their weights, current runs with their
+    // lengths), each previous run's record (labA) and each current run's
+    // (nxt, until the group ends take it over), 4 entries a lane of each
+    // array at a time, their loads issued together
+    for (int b0 = 0; b0 < m || b0 < ncur || b0 < nprev; b0 += 256) {
+        uint32_t rw[4], rr[4], rp[4], rc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = b0 + 64 * q + lane;
+            rw[q] = i < m ? F.recw[pb + i] : 0u;
+            rr[q] = i < ncur ? cur[i] : 0u;
+            rc[q] = i < ncur ? rlab_cur[i] : 0u;
+            rp[q] = i < nprev ? F.rlab[pb + i] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = b0 + 64 * q + lane;
+            if (i < m) {
+                ent.set((uint32_t)i, (uint32_t)i, rw[q]);
+                hasl[i] = 0;
+            }
+            if (i < ncur) {
+                ent.set((uint32_t)(m + i), (uint32_t)(m + i), run_end(rr[q]) - run_start(rr[q]) + 1);
+                nxt[i] = rc[q];
+            }
+            if (i < nprev) labA[i] = (uint16_t)rp[q];
+        }
     }
-    for (int i = lane; i < ncur; i += 64) {
-        uint32_t r = cur[i];
-        ent.set((uint32_t)(m + i), (uint32_t)(m + i), run_end(r) - run_start(r) + 1);
-    }
-    for (int k = lane; k < nprev; k += 64) labA[k] = (uint16_t)F.rlab[pb + k];
     if (lane == 0) {
         ent.set((uint32_t)ev, (uint32_t)ev, 1u);
     }
@@ -590,10 +609,12 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     }
     wsync<G>();
     // the links by group, stably (each group's links in order, contiguous
-    // in le): counts per group (head), their starts (head) and running ends
-    // (nxt) by a scan, then each link at its group's end plus its rank among
-    // the group's lanes of its chunk of 64 (peers by ballot over the group
-    // bits; one lane of each group adds)
+    // in le; group g's links from head[g - 1] (0 for g = 0) to head[g]):
+    // counts per group, their starts by a scan, then each link at its
+    // group's running end plus its rank among the group's lanes of its
+    // chunk of 64 (peers by ballot over the group bits; one lane of each
+    // group adds). A link's group: its current run's record (nxt). Each
+    // chunk's edges are loaded while the chunk before it runs.
     const int gbits = ng <= 1 ? 1 : 32 - __clz(ng - 1);
     const uint64_t lt = (1ull << lane) - 1ull;
     auto group_peers = [&](uint32_t g, bool valid) {
@@ -606,11 +627,13 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     };
     for (int g = lane; g < ng; g += 64) head[g] = 0;
     wsync<G>();
+    uint32_t edn = lane < ne ? edges[lane] : 0u;
     for (int b0 = 0; b0 < ne; b0 += 64) {
         const int e = b0 + lane;
         const bool valid = e < ne;
-        const uint32_t ed = valid ? edges[e] : 0u;
-        const uint32_t g = valid ? rlab_cur[ed >> 16] : 0u;
+        const uint32_t ed = edn;
+        edn = e + 64 < ne ? edges[e + 64] : 0u;
+        const uint32_t g = valid ? (uint32_t)nxt[ed >> 16] : 0u;
         if (valid) hasl[labA[ed & 0xffffu]] = 1;
         const uint64_t peers = group_peers(g, valid);
         if (valid && !(peers & lt)) head[g] = (uint32_t)head[g] + (uint32_t)__popcll(peers);
@@ -624,31 +647,31 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
             const uint32_t o = __shfl_up(inc, d, 64);
             if (lane >= d) inc += o;
         }
-        if (g < ng) {
-            head[g] = (uint32_t)carry + inc - c;
-            nxt[g] = (uint32_t)carry + inc - c;
-        }
+        if (g < ng) head[g] = (uint32_t)carry + inc - c;
         carry += (int)__shfl(inc, 63, 64);
     }
     wsync<G>();
+    edn = lane < ne ? edges[lane] : 0u;
     for (int b0 = 0; b0 < ne; b0 += 64) {
         const int e = b0 + lane;
         const bool valid = e < ne;
-        const uint32_t ed = valid ? edges[e] : 0u;
-        const uint32_t g = valid ? rlab_cur[ed >> 16] : 0u;
+        const uint32_t ed = edn;
+        edn = e + 64 < ne ? edges[e + 64] : 0u;
+        const uint32_t g = valid ? (uint32_t)nxt[ed >> 16] : 0u;
         const uint64_t peers = group_peers(g, valid);
-        if (valid) le[(uint32_t)nxt[g] + (uint32_t)__popcll(peers & lt)] = (ed & 0xffff0000u) | labA[ed & 0xffffu];
+        if (valid) le[(uint32_t)head[g] + (uint32_t)__popcll(peers & lt)] = (ed & 0xffff0000u) | labA[ed & 0xffffu];
         wsync<G>();
-        if (valid && !(peers & lt)) nxt[g] = (uint32_t)nxt[g] + (uint32_t)__popcll(peers);
+        if (valid && !(peers & lt)) head[g] = (uint32_t)head[g] + (uint32_t)__popcll(peers);
         wsync<G>();
     }
+    auto gstart = [&](uint32_t g) -> uint32_t { return g ? (uint32_t)head[g - 1] : 0u; };
     // the groups that have links, listed (in labA, free by now), so that
     // the lanes share out those and not the rows' many linkless records
     U16 *glist = labA;
     int ngl = 0;
     for (int g0 = 0; g0 < ng; g0 += 64) {
         const int g = g0 + lane;
-        const bool has = g < ng && (uint32_t)nxt[g] != (uint32_t)head[g];
+        const bool has = g < ng && (uint32_t)head[g] != gstart((uint32_t)g);
         const uint64_t hb = __ballot(has);
         if (has) glist[ngl + __popcll(hb & lt)] = (uint16_t)g;
         ngl += __popcll(hb);
@@ -661,8 +684,8 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     // a record whose parent is cr needs no read of cr (find's first step)
     for (int k = lane; k < ngl; k += 64) {
         const uint32_t g = glist[k];
-        uint32_t e = head[g];
-        const uint32_t e1 = nxt[g];
+        uint32_t e = gstart(g);
+        const uint32_t e1 = head[g];
         int cur_i = -1;
         uint32_t ra = 0, wa = 0, cr = 0xffffffffu, cw = 0;
         uint32_t ed = le[e];
@@ -733,7 +756,7 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
         F.rlost[pb + l] = q == (uint32_t)l ? (hasl[l] ? kLostNone : kLostRetired) : name(q);
     }
     for (int x = m + lane; x <= ev; x += 64) F.parent[name((uint32_t)x)] = name(ent.par((uint32_t)x));
-    for (int i = lane; i < ncur; i += 64) F.rroot[cb + rlab_cur[i]] = name(ent.find_ro((uint32_t)(m + i)));
+    for (int i = lane; i < ncur; i += 64) F.rroot[cb + (uint32_t)nxt[i]] = name(ent.find_ro((uint32_t)(m + i)));
 }
 
 // A wave per row, the rows that fit C = cap runs and 2C links; the others
