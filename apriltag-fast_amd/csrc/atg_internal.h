@@ -17,6 +17,7 @@
 //                              weight, what its root lost to, its root
 //   nrec    u32 [qh]           records (sets touching the row) per row
 //   plab    u32 [qh][qw]       per-pixel nlab (ok 0 where t is 127)
+//   dlab    u32 [qh][qw]       per-pixel dense number of the root (ok pixels)
 //   pcnt/poff u32 [qh + 1]     points per row / their offsets
 //   pts     u64 [pcap]         boundary points in emission order
 //   skey/sval, skey2/sval2     radix-sort ping-pong (u64 key, u32 value)
@@ -106,6 +107,7 @@ struct Frame {
     uint32_t *rlab;           // = nlab until k_resolve: record of each run within its row
     uint32_t *recw, *rlost, *rroot, *nrec;
     uint32_t *uedge, *ecnt;   // [qh][3 qw] union-find links per row, [qh] their counts
+    uint32_t *dlab;           // [qh][qw] the dense number of each ok pixel's root (uedge's memory, free by then)
     uint32_t *pcnt, *poff;
     uint64_t *pts, *cpts;
     uint64_t *skey, *skey2;

@@ -26,7 +26,8 @@
 //                nlab = root | (pixel count >= min_cluster_pixels) << 31
 //                (fast/clusters.c:73-94).
 //   k_labels     one wave per row: the nlab of the run under each pixel,
-//                the run index counted with ballots over the row's changes.
+//                the run index counted with ballots over the row's changes,
+//                and the dense number of its root (numbered just before).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -873,7 +874,8 @@ __global__ void k_resolve(const Frame *frames, const DevParams *P) {
 }
 
 // One wave per row: plab[y][x] = nlab of the run under pixel x (0 where
-// t is 127), the row's virtual node at x = qw-1 (fast/clusters.c:123-142).
+// t is 127), the row's virtual node at x = qw-1 (fast/clusters.c:123-142);
+// dlab[y][x] = the dense number of its root where that root is ok.
 __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     int y = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -881,6 +883,7 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     const int lane = threadIdx.x & 63;
     const uint8_t *trow = F.t + (int64_t)y * F.qw;
     uint32_t *out = F.plab + (int64_t)y * F.qw;
+    uint32_t *dout = F.dlab + (int64_t)y * F.qw;
     const uint32_t *nl = F.nlab + F.row_off[y];
     const int xmax = F.qw - 2;
     int count = 0;
@@ -896,12 +899,18 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
         uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
         if (valid) {
             int idx = count + __popcll(sm & le) - 1;
-            out[x] = tv == 127u ? 0u : nl[idx];
+            const uint32_t v = tv == 127u ? 0u : nl[idx];
+            out[x] = v;
+            dout[x] = (v >> 31) ? F.rdense[v & 0x7fffffffu] : 0u;
         }
         count += __popcll(sm);
         carry_t = __shfl(tv, 63, 64);
     }
-    if (lane == 0) out[F.qw - 1] = F.nlab[F.row_off[F.qh] + y];
+    if (lane == 0) {
+        const uint32_t v = F.nlab[F.row_off[F.qh] + y];
+        out[F.qw - 1] = v;
+        dout[F.qw - 1] = (v >> 31) ? F.rdense[v & 0x7fffffffu] : 0u;
+    }
 }
 
 // The front's dynamic LDS limit: 64 KiB, or most of the 160 KiB of a CU
@@ -971,6 +980,7 @@ CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scra
 }
 
 void launch_work_offsets(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q, int l, int which);
+void launch_root_numbers(hipStream_t st, const Frame *dframes, int nframes);
 
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                       int max_qh, const CclPlan &c, WorkQueue *Q, const Tune &tu, Marks *mk) {
@@ -1025,6 +1035,7 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,
                        dparams);
     mk->mark(st, "resolve");
+    launch_root_numbers(st, dframes, nframes);  // the dense numbers k_labels writes
     hipLaunchKernelGGL(k_labels, rg, rb, 0, st, dframes);
     mk->mark(st, "labels");
 }

@@ -1,13 +1,14 @@
 // Stage (c): boundary points, their clusters and the cluster order.
 //
+//   k_root_*       the roots of clusters that can hold points (the ok
+//                  roots) numbered densely in node order (launched before
+//                  k_labels, which writes each pixel's number to dlab).
 //   k_point_rows   one wave per row pair (y, y+1), y in 1..qh-2: the 4-bit
 //                  neighbour mask of each pixel (fast/clusters.c:150-174);
 //                  pass 0 counts points per row, pass 1 writes each point
 //                  at its emission index (row offset + wave prefix) with
 //                  its cluster sort key.
 //   k_point_scan   the row offsets of the points.
-//   k_root_*       the roots of clusters that can hold points (the ok
-//                  roots) numbered densely in node order.
 //   radix sort     stable LSD sort of (key, index), kRadixBits a pass: the
 //                  keys order the clusters by (hash bucket, id) as the
 //                  reference's cluster table does (fast/clusters.c:338-356);
@@ -57,24 +58,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
 
 __device__ __forceinline__ uint32_t opp(uint32_t a, uint32_t b) { return (a ^ b) == 255u ? 1u : 0u; }
 
-// The point mask of pixel x of row a against row b (rows of t and plab).
-__device__ __forceinline__ uint32_t point_mask(const uint8_t *a, const uint8_t *b, const uint32_t *la,
-                                               const uint32_t *lb, int x) {
-    auto oa = [&](int i) { return la[i] >> 31; };
-    auto ob = [&](int i) { return lb[i] >> 31; };
-    if (x == 0) return (opp(a[0], a[1]) && oa(0) && oa(1)) ? 1u : 0u;
-    if (!oa(x)) return 0u;
-    uint32_t v = a[x];
-    uint32_t c10 = opp(v, a[x + 1]) & oa(x + 1);
-    uint32_t c01 = opp(v, b[x]) & ob(x);
-    uint32_t c11 = opp(v, b[x + 1]) & ob(x + 1);
-    uint32_t cm1 = opp(v, b[x - 1]) & ob(x - 1);
-    // the (1,1) point of the pixel to the left is the same pair; x = 1 has
-    // no pixel to the left in the C library's loop (fast/clusters.c:170-171)
-    uint32_t p11 = x > 1 ? (opp(a[x - 1], b[x]) & oa(x - 1) & ob(x)) : 0u;
-    return c10 | (c01 << 1) | ((cm1 & (p11 ^ 1u)) << 2) | (c11 << 3);
-}
-
 // Whether the frame's cluster sort is packed: its key bits (counters
 // [kSortBits], known once the ok roots are numbered) and the bits of a
 // point index (below pcap) fit one word.
@@ -83,9 +66,42 @@ __device__ __forceinline__ bool sort_packed(const Frame &F) {
 }
 __device__ __forceinline__ uint64_t low_bits(uint32_t b) { return b >= 64 ? ~0ull : (1ull << b) - 1ull; }
 
-// Pass 1 stages each 64-pixel step's points (at most 4 a pixel) in LDS and
-// writes them out contiguously; the sort value of a point is its index,
-// which the sort's first pass takes from the position.
+// One pixel column of the row pair: t of rows a and b, their plab (root
+// | ok << 31) and, for the write pass, their dlab (dense root numbers).
+struct PixCol {
+    uint32_t ta, tb, la, lb, da, db;
+};
+
+// The point mask of pixel x of row a against row b (fast/clusters.c:
+// 150-174) from its column c, the columns left (l) and right (r) of it.
+__device__ __forceinline__ uint32_t point_mask(const PixCol &c, const PixCol &l, const PixCol &r, int x) {
+    if (x == 0) return (opp(c.ta, r.ta) && (c.la >> 31) && (r.la >> 31)) ? 1u : 0u;
+    if (!(c.la >> 31)) return 0u;
+    const uint32_t v = c.ta;
+    const uint32_t c10 = opp(v, r.ta) & (r.la >> 31);
+    const uint32_t c01 = opp(v, c.tb) & (c.lb >> 31);
+    const uint32_t c11 = opp(v, r.tb) & (r.lb >> 31);
+    const uint32_t cm1 = opp(v, l.tb) & (l.lb >> 31);
+    // the (1,1) point of the pixel to the left is the same pair; x = 1 has
+    // no pixel to the left in the C library's loop (fast/clusters.c:170-171)
+    const uint32_t p11 = x > 1 ? (opp(l.ta, c.tb) & (l.la >> 31) & (c.lb >> 31)) : 0u;
+    return c10 | (c01 << 1) | ((cm1 & (p11 ^ 1u)) << 2) | (c11 << 3);
+}
+
+__device__ __forceinline__ PixCol shfl_col(const PixCol &c, int src) {
+    return {(uint32_t)__shfl((int)c.ta, src, 64), (uint32_t)__shfl((int)c.tb, src, 64),
+            (uint32_t)__shfl((int)c.la, src, 64), (uint32_t)__shfl((int)c.lb, src, 64),
+            (uint32_t)__shfl((int)c.da, src, 64), (uint32_t)__shfl((int)c.db, src, 64)};
+}
+
+// A wave per row pair (y, y + 1), 64 pixels a step: each lane's column of
+// the step, the column right of the step (for lane 63) and both of the
+// next step loaded during this one; neighbours by shuffles, the left one
+// of lane 0 kept from the step before. Pass 0 counts the points of each
+// row; pass 1 stages each step's points (at most 4 a pixel) in LDS and
+// writes them out contiguously with their cluster sort keys (bucket << 2D
+// | dense(max root) << D | dense(min root), and the point's index above
+// the key's bits when the frame's sort is packed).
 __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pass) {
     __shared__ uint64_t st_pt[4][256], st_key[4][256];
     const Frame F = frames[blockIdx.y];
@@ -95,35 +111,58 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
     const int QW = F.qw;
     const uint8_t *a = F.t + (int64_t)y * QW, *b = a + QW;
     const uint32_t *la = F.plab + (int64_t)y * QW, *lb = la + QW;
+    const uint32_t *da = F.dlab + (int64_t)y * QW, *db = da + QW;
+    auto load = [&](int x) -> PixCol {
+        PixCol c{0, 0, 0, 0, 0, 0};
+        if (x < QW) {
+            c.ta = a[x];
+            c.tb = b[x];
+            c.la = la[x];
+            c.lb = lb[x];
+            if (pass) {
+                c.da = da[x];
+                c.db = db[x];
+            }
+        }
+        return c;
+    };
     uint32_t count = 0;
     uint32_t base = pass ? F.poff[y] : 0u;
     const uint32_t D = pass ? F.counters[kRootBits] : 0u;
     const uint32_t sbits = pass ? F.counters[kSortBits] : 0u;
     const bool packed = pass && sort_packed(F);
     const uint64_t mask = F.bucket_mask;
+    PixCol cur = load(lane), cr = load(64), left{0, 0, 0, 0, 0, 0};
     for (int x0 = 0; x0 < QW - 1; x0 += 64) {
-        int x = x0 + lane;
-        uint32_t m = x < QW - 1 ? point_mask(a, b, la, lb, x) : 0u;
-        uint32_t c = __popc(m);
+        const PixCol nxt = load(x0 + 64 + lane), nr = load(x0 + 128);
+        const int x = x0 + lane;
+        PixCol r = shfl_col(cur, (lane + 1) & 63), l = shfl_col(cur, (lane + 63) & 63);
+        if (lane == 63) r = cr;
+        if (lane == 0) l = left;
+        const uint32_t m = x < QW - 1 ? point_mask(cur, l, r, x) : 0u;
+        const uint32_t c = __popc(m);
         if (pass) {
             const uint32_t inc = wave_incl_scan_u(c), tot = __shfl(inc, 63, 64);
             uint32_t k = inc - c;  // this lane's first point in the step
             if (m) {
-                uint32_t av = a[x];
-                int vd = 255 - 2 * (int)av;
-                uint32_t r0 = la[x] & 0x7fffffffu;
+                const int vd = 255 - 2 * (int)cur.ta;
+                const uint32_t r0 = cur.la & 0x7fffffffu, d0 = cur.da;
                 for (int j = 0; j < 4; j++) {
                     if (!((m >> j) & 1u)) continue;
-                    int dx = j == 0 ? 1 : j == 1 ? 0 : j == 2 ? -1 : 1;
-                    int dy = j == 0 ? 0 : 1;
-                    uint32_t r1 = (dy ? lb[x + dx] : la[x + dx]) & 0x7fffffffu;
-                    uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
-                    uint64_t id = ((uint64_t)hi << 32) + lo;
-                    uint64_t bucket = ((id * 2654435761ull) >> 32) & mask;
-                    uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
-                    uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
+                    // partner (dx, dy): (1, 0), (0, 1), (-1, 1), (1, 1)
+                    const int dx = j == 0 ? 1 : j == 1 ? 0 : j == 2 ? -1 : 1;
+                    const int dy = j == 0 ? 0 : 1;
+                    const uint32_t r1 = (j == 0 ? r.la : j == 1 ? cur.lb : j == 2 ? l.lb : r.lb) & 0x7fffffffu;
+                    const uint32_t d1 = j == 0 ? r.da : j == 1 ? cur.db : j == 2 ? l.db : r.db;
+                    // dense numbers keep the roots' order
+                    const uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
+                    const uint32_t dh = r0 > r1 ? d0 : d1, dl = r0 > r1 ? d1 : d0;
+                    const uint64_t id = ((uint64_t)hi << 32) + lo;
+                    const uint64_t bucket = ((id * 2654435761ull) >> 32) & mask;
+                    const uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
+                    const uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
                     st_pt[wv][k] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
-                    st_key[wv][k] = (bucket << (2 * D)) | ((uint64_t)F.rdense[hi] << D) | F.rdense[lo];
+                    st_key[wv][k] = (bucket << (2 * D)) | ((uint64_t)dh << D) | dl;
                     k++;
                 }
             }
@@ -144,6 +183,9 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
         } else {
             count += __shfl(wave_incl_scan_u(c), 63, 64);
         }
+        left = shfl_col(cur, 63);
+        cur = nxt;
+        cr = nr;
     }
     if (!pass && lane == 0) F.pcnt[y] = count;
 }
@@ -457,6 +499,13 @@ __global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
 
 // ---------------------------------------------------------------- launchers
 
+// The ok roots numbered densely in node order (before k_labels, which
+// writes each pixel's number).
+void launch_root_numbers(hipStream_t st, const Frame *dframes, int nframes) {
+    hipLaunchKernelGGL(k_root_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_root_dense, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+}
+
 // pk: the cluster sort (packed frames move one word); else (key, value)
 // pairs throughout.
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
@@ -475,8 +524,6 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
                          Marks *mk) {
     const int rw = tu.row_waves < 1 ? 1 : tu.row_waves > 4 ? 4 : tu.row_waves;  // launch bounds: 256 threads
     dim3 g((unsigned)((max_qh + rw - 1) / rw), nframes), b(64 * rw);
-    hipLaunchKernelGGL(k_root_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_root_dense, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 0);
     hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);

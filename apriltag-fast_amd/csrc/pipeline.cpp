@@ -322,6 +322,7 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->nrec = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->plab = (uint32_t *)take((size_t)px * 4);
     F->uedge = (uint32_t *)take((size_t)px * 12);
+    F->dlab = F->uedge;  // written after the union-find's last use of uedge
     F->ecnt = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->pcnt = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->poff = (uint32_t *)take((size_t)(qh + 2) * 4);
