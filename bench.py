@@ -65,8 +65,12 @@ def frame_counts(ctx, i, W, H):
     st = ctx.stages(i)
     qw, qh = st["qw"], st["qh"]
     R = c[0]
+    # the cluster sort: 11-bit radix passes over the key's bits; a frame
+    # whose key and point index fit one 64-bit word (pcap = qw qh + 64)
+    # sorts that word alone
+    packed = c[11] + (qw * qh + 64).bit_length() <= 64
     return dict(W=W, H=H, qw=qw, qh=qh, R=R, N=R + qh, E=c[6], P=c[1], C=c[2], K=c[7], Q=c[3],
-                S=(c[11] + 7) // 8)
+                S=(c[11] + 10) // 11, SB=24 if packed else 32)
 
 
 def kernel_bytes(n):
@@ -75,9 +79,14 @@ def kernel_bytes(n):
     slot; an 8-byte word per point; 48 bytes of moments per fitted point).
     Stages (a)-(c) follow the per-frame formulas of the byte model split by
     kernel, the cluster sort with the frame's own number of radix passes
-    (S); (d)-(e) count the arrays of the fit and, for decode, the pixels
-    each quad samples (refine: 4 edges x 16 samples x 25 steps x 2 bilinear
-    reads of 4 pixels)."""
+    (S) and bytes a point per pass (SB: 8-byte words read by the histogram
+    and read and written by the scatter, 24, or 12-byte key and index
+    pairs, 32); labels count the ok roots' numbering (8N), the gathers of
+    run labels and dense numbers (8N), t and the two label planes; points
+    read t and plab (count pass) and t, plab and dlab (write pass) and
+    write 16 bytes a point; (d)-(e) count the arrays of the fit and, for
+    decode, the pixels each quad samples (refine: 4 edges x 16 samples x 25
+    steps x 2 bilinear reads of 4 pixels)."""
     W, H, qw, qh = n["W"], n["H"], n["qw"], n["qh"]
     R, N, E, P, K, Q = n["R"], n["N"], n["E"], n["P"], n["K"], n["Q"]
     px, tiles = qw * qh, (qw // 4) * (qh // 4)
@@ -90,9 +99,9 @@ def kernel_bytes(n):
         "uf_replay": 4 * R + 4 * E + 4 * R + 4 * N + 12 * N,
         "uf_stitch": lg * 8 * N + 16 * N,
         "resolve": 16 * N,
-        "labels": px + 4 * px + 4 * N,
-        "points": 8 * N + 2 * (px + 4 * px) + 12 * P,
-        "cluster_sort": n["S"] * 32 * P,
+        "labels": 16 * N + px + 8 * px,
+        "points": (px + 4 * px) + (px + 8 * px) + 16 * P,
+        "cluster_sort": n["S"] * n["SB"] * P,
         "clusters": 24 * P,
         "fit_prep": 8 * P + 20 * P + 12 * K,
         "angle_sort": 16 * K,
