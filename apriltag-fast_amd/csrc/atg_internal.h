@@ -129,6 +129,8 @@ struct Frame {
     QuadRec *qtmp;        // [kMaxQuadsPerFrame] fitted quads, any order
     uint32_t *qcl;        // [kMaxQuadsPerFrame] their cluster index
     QuadRec *quads;       // [kMaxQuadsPerFrame] in cluster order
+    double *qline;        // [kMaxQuadsPerFrame][4][4] per quad edge: refined line (x, y, cos, sin);
+                          // before k_lines: mean x, mean y and atan2's two arguments
     at_detection *dtmp;   // [kMaxQuadsPerFrame * nfam] per (quad, family)
     uint32_t *dflag;      // [kMaxQuadsPerFrame * nfam]
     at_detection *dets;   // [kMaxDetsPerFrame] final, sorted by id

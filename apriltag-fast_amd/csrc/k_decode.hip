@@ -335,26 +335,17 @@ __device__ __forceinline__ void border_pattern(int pi, int wab, float *x0, float
 
 
 // Persistent: the quads of every frame, block b taking global indices b,
-// b + G, ... (a frame cursor).
-// NT threads a quad: 256 (a wave per edge in the refinement) or one wave
-// (the edges in turn).
+// b + G, ... (a frame cursor). NT threads a quad: 256 (a wave per edge) or
+// one wave (the edges in turn). The edges' sample sums of refine_edges
+// (fast/decode.c:657-761) up to the line's angle: per edge, the mean point
+// and atan2's two arguments into qline (k_lines takes the angle; its
+// library calls need far more registers than the rest of the decode).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
+__global__ __launch_bounds__(NT) void k_refine(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
     __shared__ double wsx[4][16], wsy[4][16];  // an edge's refined sample points, 16 at a time
     __shared__ int wsv[4][16];
-    __shared__ double smp[3][kMaxSamples];  // tag x, tag y, value
-    __shared__ int sflag[kMaxSamples];
     __shared__ float qp[4][2];
-    __shared__ double lines[4][4];
-    __shared__ double sH[9], sA[72], sLU[9];
-    __shared__ int s_hok, s_piv, s_polar;
-    __shared__ Gray g_white, g_black;
-    __shared__ double vals[12 * 12], sharp[12 * 12];
-    __shared__ uint64_t s_rc;
-    __shared__ float s_margin;
-    __shared__ int s_mj[16], s_mh[16];
     const int t = threadIdx.x, lane = t & 63;
-    const int nfam = P->nfam;
     const uint32_t total = Q->total[2];
     const uint32_t *off = Q->off[2];
     uint32_t fi = 0;
@@ -437,12 +428,11 @@ __global__ __launch_bounds__(NT) void k_decode(const Frame *frames, const DevPar
                     double cxx = DSUB(DDIV(mxx, nn), DMUL(ex, ex));
                     double cxy = DSUB(DDIV(mxy, nn), DMUL(ex, ey));
                     double cyy = DSUB(DDIV(myy, nn), DMUL(ey, ey));
-                    float th2 = (float)atan2((double)(float)DMUL(-2.0, cxy), (double)(float)DSUB(cyy, cxx));
-                    double theta = DMUL(0.5, (double)th2);
-                    lines[e][0] = ex;
-                    lines[e][1] = ey;
-                    lines[e][2] = (double)(float)cos((double)(float)theta);
-                    lines[e][3] = (double)(float)sin((double)(float)theta);
+                    double *ln = F.qline + (size_t)qi * 16 + e * 4;
+                    ln[0] = ex;
+                    ln[1] = ey;
+                    ln[2] = (double)(float)DMUL(-2.0, cxy);
+                    ln[3] = (double)(float)DSUB(cyy, cxx);
                 }
             };
             if constexpr (NT == 64) {
@@ -450,6 +440,69 @@ __global__ __launch_bounds__(NT) void k_decode(const Frame *frames, const DevPar
             } else {
                 refine_edge(t >> 6);
             }
+            bsync_t<NT>();
+        }
+        bsync_t<NT>();
+    }
+}
+
+// atan2, cos and sin of refine_edges' lines (fast/decode.c:750-760), a
+// thread per quad edge of every frame (the quads' list 2, four edges each):
+// qline's atan2 arguments become the line's direction.
+__global__ __launch_bounds__(256) void k_lines(const Frame *frames, const WorkQueue *Q) {
+    const uint32_t total = 4 * Q->total[2];
+    const uint32_t *off = Q->off[2];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < total; g += gridDim.x * 256) {
+        while (4 * off[fi + 1] <= g) fi++;
+        double *ln = frames[fi].qline + (size_t)(g - 4 * off[fi]) * 4;
+        float th2 = (float)atan2(ln[2], ln[3]);
+        double theta = DMUL(0.5, (double)th2);
+        ln[2] = (double)(float)cos((double)(float)theta);
+        ln[3] = (double)(float)sin((double)(float)theta);
+    }
+}
+
+// Persistent: the quads of every frame, block b taking global indices b,
+// b + G, ... (a frame cursor).
+// NT threads a quad: 256 or one wave. The corners from the refined lines
+// (k_refine, k_lines), the homography, the samples and the decoding.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
+    __shared__ double smp[3][kMaxSamples];  // tag x, tag y, value
+    __shared__ int sflag[kMaxSamples];
+    __shared__ float qp[4][2];
+    __shared__ double lines[4][4];
+    __shared__ double sH[9], sA[72], sLU[9];
+    __shared__ int s_hok, s_piv, s_polar;
+    __shared__ Gray g_white, g_black;
+    __shared__ double vals[12 * 12], sharp[12 * 12];
+    __shared__ uint64_t s_rc;
+    __shared__ float s_margin;
+    __shared__ int s_mj[16], s_mh[16];
+    const int t = threadIdx.x, lane = t & 63;
+    const int nfam = P->nfam;
+    const uint32_t total = Q->total[2];
+    const uint32_t *off = Q->off[2];
+    uint32_t fi = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (off[fi + 1] <= g) fi++;
+        const Frame &F = frames[fi];
+        const uint32_t qi = g - off[fi];
+        View v = {F.img, F.stride, F.W, F.H};
+        if (F.blurred && !(P->decimate > 1.0f)) v = View{F.qim, F.W, F.W, F.H};
+        QuadRec q = F.quads[qi];
+        if (t == 0) {
+            for (int i = 0; i < 4; i++) {
+                float f = P->decimate;
+                qp[i][0] = f > 1.0f ? __fmul_rn(q.p[i][0], f) : q.p[i][0];
+                qp[i][1] = f > 1.0f ? __fmul_rn(q.p[i][1], f) : q.p[i][1];
+            }
+        }
+        bsync_t<NT>();
+        if (P->refine) {
+            // the refined lines (k_refine, k_lines) and their intersections
+            if (t < 16) lines[t >> 2][t & 3] = F.qline[(size_t)qi * 16 + t];
             bsync_t<NT>();
             if (t == 0) {
                 for (int i = 0; i < 4; i++) {
@@ -787,13 +840,28 @@ void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q
 void warm_decode_grid(int fit_blocks) {
     persistent_grid(reinterpret_cast<const void *>(&k_decode<64>), 64, 0, 4 * fit_blocks);
     persistent_grid(reinterpret_cast<const void *>(&k_decode<256>), 256, 0, fit_blocks);
+    persistent_grid(reinterpret_cast<const void *>(&k_refine<64>), 64, 0, 4 * fit_blocks);
+    persistent_grid(reinterpret_cast<const void *>(&k_refine<256>), 256, 0, fit_blocks);
+    persistent_grid(reinterpret_cast<const void *>(&k_lines), 256, 0, fit_blocks);
 }
 
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
-                         WorkQueue *Q, Marks *mk) {
+                         bool P_refine, WorkQueue *Q, Marks *mk) {
     // a wave a quad (four times as many quads in flight as 256-thread
     // blocks); for a few frames, whose quads all fit at once, a 256-thread
     // block a quad, its edges refined at the same time
+    if (P_refine) {
+        if (nframes > 4)
+            hipLaunchKernelGGL(k_refine<64>,
+                               dim3(persistent_grid(reinterpret_cast<const void *>(&k_refine<64>), 64, 0, 4 * blocks)),
+                               dim3(64), 0, st, dframes, dparams, Q);
+        else
+            hipLaunchKernelGGL(k_refine<256>,
+                               dim3(persistent_grid(reinterpret_cast<const void *>(&k_refine<256>), 256, 0, blocks)),
+                               dim3(256), 0, st, dframes, dparams, Q);
+        hipLaunchKernelGGL(k_lines, dim3(persistent_grid(reinterpret_cast<const void *>(&k_lines), 256, 0, blocks)),
+                           dim3(256), 0, st, dframes, Q);
+    }
     if (nframes > 4)
         hipLaunchKernelGGL(k_decode<64>,
                            dim3(persistent_grid(reinterpret_cast<const void *>(&k_decode<64>), 64, 0, 4 * blocks)),

@@ -32,7 +32,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
                         int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk);
 int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
-                         WorkQueue *Q, Marks *mk);
+                         bool refine, WorkQueue *Q, Marks *mk);
 }  // namespace atg
 
 using namespace atg;
@@ -352,6 +352,7 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->filt = (double *)take((size_t)pc * 8);
     F->midx = (uint32_t *)take((size_t)pc * 4);
     F->qtmp = (QuadRec *)take(sizeof(QuadRec) * kMaxQuadsPerFrame);
+    F->qline = (double *)take(sizeof(double) * 16 * kMaxQuadsPerFrame);
     F->qcl = (uint32_t *)take(4 * kMaxQuadsPerFrame);
     F->quads = (QuadRec *)take(sizeof(QuadRec) * kMaxQuadsPerFrame);
     F->dtmp = (at_detection *)take(sizeof(at_detection) * kMaxQuadsPerFrame * std::max(nfam, 1));
@@ -609,7 +610,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, dq, tu, mk);
         launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
         launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
-        launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, dq, mk);
+        launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, ctx->hp.refine != 0, dq, mk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         // results: counters and the first detections of every frame in one copy
@@ -621,7 +622,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
                                     ccl.replay_cap, ccl.replay_pack, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid, ccl.replay_waves,        max_lds_qw,
-                                    tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners,
+                                    tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners, ctx->hp.refine,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
                                     (int64_t)(intptr_t)ctx->dparams};
