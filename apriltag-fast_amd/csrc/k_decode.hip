@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256) void k_lines(const Frame *frames, const WorkQu
 // NT threads a quad: 256 or one wave. The corners from the refined lines
 // (k_refine, k_lines), the homography, the samples and the decoding.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
+__global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_decode(const Frame *frames, const DevParams *P, const WorkQueue *Q) {
     __shared__ double smp[3][kMaxSamples];  // tag x, tag y, value
     __shared__ int sflag[kMaxSamples];
     __shared__ float qp[4][2];

@@ -978,7 +978,8 @@ __device__ __forceinline__ uint32_t bscan_t(uint32_t v, uint32_t *total) {
 
 // Persistent over list l of kept clusters, a cluster per block of NT threads.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q,
+// (the 256-thread shape held to 128 registers: four blocks a CU, as its LDS allows)
+__global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q,
                                                     int l) {
     constexpr int kMaxNm = CornerShape<NT>::kMaxNm, kLdsPts = CornerShape<NT>::kLdsPts;
     constexpr int kMaxima = CornerShape<NT>::kMaxima;
