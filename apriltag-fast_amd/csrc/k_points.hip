@@ -379,15 +379,15 @@ __global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *fram
 
 // Stable scatter: chunks of 256 keys in order; inside a chunk, the rank of
 // a key among the keys of its digit before it: within a wave by ballots
-// (peers = lanes of the same digit), across waves by per-wave counts. The
-// counts carry their chunk's number (a count of another chunk reads as 0),
-// so that no chunk clears them. iota: the values are the keys' positions
-// (the first pass of an unpacked cluster sort; its points were emitted in
+// (peers = lanes of the same digit), across waves by per-wave counts (a
+// byte each: at most 64; the lane that set one clears it once the chunk's
+// positions are taken). iota: the values are the keys' positions (the
+// first pass of an unpacked cluster sort; its points were emitted in
 // order), not read.
 __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity,
                                                        int dyn, int iota, int pk) {
     __shared__ uint32_t off[kRadixBins];
-    __shared__ uint32_t wcnt[4][kRadixBins];  // chunk << 16 | count
+    __shared__ uint8_t wcnt[4][kRadixBins];
     const Frame F = frames[blockIdx.y];
     const uint32_t sb = F.counters[kSortBits];
     if (dyn && shift >= (int)sb) return;
@@ -398,14 +398,13 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int i = t; i < kRadixBins; i += 256) {
         off[i] = F.rhist[blockIdx.x * kRadixBins + i];
-        for (int k = 0; k < 4; k++) wcnt[k][i] = 0xffff0000u;
+        for (int k = 0; k < 4; k++) wcnt[k][i] = 0;
     }
     __syncthreads();
     uint32_t s0, s1;
     seg_of(n, blockIdx.x, &s0, &s1);
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t chunk = 0;  // below 0xffff: segments of at most 2^24 keys
-    for (uint32_t c0 = s0; c0 < s1; c0 += 256, chunk++) {
+    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
         const uint32_t i = c0 + t;
         const bool valid = i < s1;
         const uint64_t key = valid ? B.kin[i] : 0ull;
@@ -414,19 +413,19 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
         const uint64_t peers = digit_peers(d, valid);
         const uint32_t rank = (uint32_t)__popcll(peers & lt), cnt = (uint32_t)__popcll(peers);
         const bool lead = valid && rank == 0;
-        if (lead) wcnt[w][d] = (chunk << 16) | cnt;
+        if (lead) wcnt[w][d] = (uint8_t)cnt;
         __syncthreads();
         if (valid) {
             uint32_t pos = off[d] + rank;
-            for (int k = 0; k < w; k++) {
-                const uint32_t e = wcnt[k][d];
-                pos += (e >> 16) == chunk ? (e & 0xffffu) : 0u;
-            }
+            for (int k = 0; k < w; k++) pos += wcnt[k][d];
             B.kout[pos] = key;
             if (!packed) B.vout[pos] = val;
         }
         __syncthreads();
-        if (lead) atomicAdd(&off[d], cnt);
+        if (lead) {
+            atomicAdd(&off[d], cnt);
+            wcnt[w][d] = 0;
+        }
         __syncthreads();
     }
 }
