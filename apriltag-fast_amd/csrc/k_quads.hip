@@ -550,39 +550,67 @@ __device__ __forceinline__ void bitonic_pair(uint32_t q, uint32_t k, uint32_t j,
 
 constexpr uint32_t kSortTile = 2048;  // keys of a cluster sorted together in LDS
 
-// A wave's bitonic sort of 64 R keys in registers: key i = r * 64 + lane
-// sits in v[r]; partners within a register row by shuffles, across rows in
-// the lane's own registers. Ascending.
-template <int R>
-__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[R]) {
+// The value of lane ^ J of a wave (J a power of two below 64): a DPP quad
+// permutation for 1 and 2, a swizzle over 32 lanes up to 16, a permute for
+// 32.
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t x) {
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);  // [1,0,3,2]
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
+    else if constexpr (J < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (J << 10) | 0x1F);  // xor J
+    else return (uint32_t)__shfl_xor((int)x, J, 64);
+}
+template <int J>
+__device__ __forceinline__ uint64_t lane_xor64(uint64_t v) {
+    return ((uint64_t)lane_xor32<J>((uint32_t)(v >> 32)) << 32) | lane_xor32<J>((uint32_t)v);
+}
+
+// One compare-exchange of distinct keys (or equal padding): the lane takes
+// its partner's key when the partner is the smaller one and the lane keeps
+// the minimum, or the larger one and it keeps the maximum.
+__device__ __forceinline__ uint64_t cx_take(uint64_t mine, uint64_t other, bool keep_min) {
+    return (other < mine) == keep_min ? other : mine;
+}
+
+// Step (K, J) of a wave's bitonic sort of 64 R keys in registers (key
+// i = r * 64 + lane in v[r]), then the steps after it.
+template <int R, int K, int J>
+__device__ __forceinline__ void wave_bitonic_step(uint64_t (&v)[R]) {
     const int lane = threadIdx.x & 63;
+    if constexpr (J >= 64) {
+        constexpr int jr = J >> 6;
 #pragma unroll
-    for (int k = 2; k <= 64 * R; k <<= 1) {
+        for (int r = 0; r < R; r++) {
+            if (r & jr) continue;
+            const int r2 = r | jr;
+            const bool up = ((r << 6) & K) == 0;
+            const uint64_t a = v[r], b = v[r2];
+            const bool sw = (a > b) == up;
+            v[r] = sw ? b : a;
+            v[r2] = sw ? a : b;
+        }
+    } else {
 #pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
-                const int jr = j >> 6;
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    if (r & jr) continue;
-                    const int r2 = r | jr;
-                    const bool up = ((r << 6) & k) == 0;
-                    const uint64_t a = v[r], b = v[r2];
-                    const bool sw = (a > b) == up;
-                    v[r] = sw ? b : a;
-                    v[r2] = sw ? a : b;
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const uint64_t o = __shfl_xor(v[r], j, 64);
-                    const bool up = (((r << 6) | lane) & k) == 0;
-                    const bool keep_min = ((lane & j) == 0) == up;
-                    v[r] = keep_min ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
-                }
-            }
+        for (int r = 0; r < R; r++) {
+            const uint64_t o = lane_xor64<J>(v[r]);
+            const bool up = (((r << 6) | lane) & K) == 0;
+            v[r] = cx_take(v[r], o, ((lane & J) == 0) == up);
         }
     }
+    if constexpr (J > 1) wave_bitonic_step<R, K, J / 2>(v);
+}
+template <int R, int K>
+__device__ __forceinline__ void wave_bitonic_merge(uint64_t (&v)[R]) {
+    wave_bitonic_step<R, K, K / 2>(v);
+    if constexpr (K < 64 * R) wave_bitonic_merge<R, 2 * K>(v);
+}
+
+// A wave's bitonic sort of 64 R keys in registers: key i = r * 64 + lane
+// sits in v[r]; partners within a register row by lane exchanges, across
+// rows in the lane's own registers. Ascending.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[R]) {
+    wave_bitonic_merge<R, 2>(v);
 }
 
 template <int R>
@@ -602,32 +630,27 @@ __device__ __forceinline__ void wave_sort_keys(const uint64_t *in, uint64_t *out
     }
 }
 
-// The half-cleaner steps of distance 256 down to 1 of the all-ascending
+// The half-cleaner steps of distance J down to 1 of the all-ascending
 // network on a wave's 512 keys in registers (key r * 64 + lane in v[r]):
 // the smaller key to the lower position.
+template <int J = 256>
 __device__ __forceinline__ void reg_half_clean(uint64_t (&v)[8]) {
     const int lane = threadIdx.x & 63;
+    if constexpr (J >= 64) {
+        constexpr int jr = J >> 6;
 #pragma unroll
-    for (int j = 256; j > 0; j >>= 1) {
-        if (j >= 64) {
-            const int jr = j >> 6;
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                if (r & jr) continue;
-                const int r2 = r | jr;
-                const uint64_t a = v[r], b = v[r2];
-                v[r] = a < b ? a : b;
-                v[r2] = a < b ? b : a;
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                const uint64_t o = __shfl_xor(v[r], j, 64);
-                const bool lower = (lane & j) == 0;
-                v[r] = lower ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
-            }
+        for (int r = 0; r < 8; r++) {
+            if (r & jr) continue;
+            const int r2 = r | jr;
+            const uint64_t a = v[r], b = v[r2];
+            v[r] = a < b ? a : b;
+            v[r2] = a < b ? b : a;
         }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = cx_take(v[r], lane_xor64<J>(v[r]), (lane & J) == 0);
     }
+    if constexpr (J > 1) reg_half_clean<J / 2>(v);
 }
 
 // One step of the network whose partners sit in other waves: through LDS
