@@ -73,6 +73,28 @@ struct QuadRec {
     double H[9];
 };
 
+// Wave-wide data movement by DPP (VALU moves, no LDS round trip; the
+// controls of the GFX9 family).
+// Inclusive sum over the 64 lanes: shifts of 1, 2, 4, 8 within rows of 16,
+// then row 15's and lane 31's totals into the rows after them.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1, 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31, rows 2, 3
+    return v;
+}
+// x of lane i - 1; lane 0 gets fill
+__device__ __forceinline__ uint32_t wave_from_left(uint32_t x, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)x, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+// x of lane i + 1; lane 63 gets fill
+__device__ __forceinline__ uint32_t wave_from_right(uint32_t x, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)x, 0x130, 0xF, 0xF, false);  // wave_shl:1
+}
+
 // counters[] of a frame
 enum {
     kNRuns = 0, kNPts, kNClusters, kNQuads, kNDets, kError,

@@ -42,15 +42,7 @@ __device__ __forceinline__ uint32_t run_end(uint32_t r) { return (r >> 15) & 0x7
 __device__ __forceinline__ uint32_t run_white(uint32_t r) { return (r >> 30) & 1u; }
 
 // inclusive wave scan of v (64 lanes)
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) { return wave_scan_incl(v); }
 
 // Whether current run a and previous run b of the same value are linked
 // (fast/unionfind.c:130-141): overlap on columns >= 1; white also touches

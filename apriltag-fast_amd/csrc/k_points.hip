@@ -29,15 +29,7 @@
 
 namespace atg {
 
-__device__ __forceinline__ uint32_t wave_incl_scan_u(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
+__device__ __forceinline__ uint32_t wave_incl_scan_u(uint32_t v) { return wave_scan_incl(v); }
 
 // Exclusive scan over a 256-thread block; returns the prefix, *total the sum.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
@@ -88,10 +80,20 @@ __device__ __forceinline__ uint32_t point_mask(const PixCol &c, const PixCol &l,
     return c10 | (c01 << 1) | ((cm1 & (p11 ^ 1u)) << 2) | (c11 << 3);
 }
 
-__device__ __forceinline__ PixCol shfl_col(const PixCol &c, int src) {
-    return {(uint32_t)__shfl((int)c.ta, src, 64), (uint32_t)__shfl((int)c.tb, src, 64),
-            (uint32_t)__shfl((int)c.la, src, 64), (uint32_t)__shfl((int)c.lb, src, 64),
-            (uint32_t)__shfl((int)c.da, src, 64), (uint32_t)__shfl((int)c.db, src, 64)};
+// The columns of the neighbour lanes (the step's edge lanes take `fill`),
+// and lane 63's column in every lane.
+__device__ __forceinline__ PixCol col_from_left(const PixCol &c, const PixCol &fill) {
+    return {wave_from_left(c.ta, fill.ta), wave_from_left(c.tb, fill.tb), wave_from_left(c.la, fill.la),
+            wave_from_left(c.lb, fill.lb), wave_from_left(c.da, fill.da), wave_from_left(c.db, fill.db)};
+}
+__device__ __forceinline__ PixCol col_from_right(const PixCol &c, const PixCol &fill) {
+    return {wave_from_right(c.ta, fill.ta), wave_from_right(c.tb, fill.tb), wave_from_right(c.la, fill.la),
+            wave_from_right(c.lb, fill.lb), wave_from_right(c.da, fill.da), wave_from_right(c.db, fill.db)};
+}
+__device__ __forceinline__ PixCol col_of_lane63(const PixCol &c) {
+    return {(uint32_t)__builtin_amdgcn_readlane((int)c.ta, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.tb, 63),
+            (uint32_t)__builtin_amdgcn_readlane((int)c.la, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.lb, 63),
+            (uint32_t)__builtin_amdgcn_readlane((int)c.da, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.db, 63)};
 }
 
 // A wave per row pair (y, y + 1), 64 pixels a step: each lane's column of
@@ -136,13 +138,11 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
     for (int x0 = 0; x0 < QW - 1; x0 += 64) {
         const PixCol nxt = load(x0 + 64 + lane), nr = load(x0 + 128);
         const int x = x0 + lane;
-        PixCol r = shfl_col(cur, (lane + 1) & 63), l = shfl_col(cur, (lane + 63) & 63);
-        if (lane == 63) r = cr;
-        if (lane == 0) l = left;
+        const PixCol r = col_from_right(cur, cr), l = col_from_left(cur, left);
         const uint32_t m = x < QW - 1 ? point_mask(cur, l, r, x) : 0u;
         const uint32_t c = __popc(m);
         if (pass) {
-            const uint32_t inc = wave_incl_scan_u(c), tot = __shfl(inc, 63, 64);
+            const uint32_t inc = wave_incl_scan_u(c), tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
             uint32_t k = inc - c;  // this lane's first point in the step
             if (m) {
                 const int vd = 255 - 2 * (int)cur.ta;
@@ -181,9 +181,9 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             count += tot;
         } else {
-            count += __shfl(wave_incl_scan_u(c), 63, 64);
+            count += (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_u(c), 63);
         }
-        left = shfl_col(cur, 63);
+        left = col_of_lane63(cur);
         cur = nxt;
         cr = nr;
     }

@@ -43,15 +43,7 @@ __device__ __forceinline__ float fdiv_cr(float a, float b) { return (float)__ddi
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 
-__device__ __forceinline__ uint32_t wscan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
+__device__ __forceinline__ uint32_t wscan(uint32_t v) { return wave_scan_incl(v); }
 
 __device__ __forceinline__ uint32_t bscan(uint32_t v, uint32_t *total) {
     __shared__ uint32_t ws[4];
