@@ -95,6 +95,21 @@ __device__ __forceinline__ uint32_t wave_from_right(uint32_t x, uint32_t fill) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)x, 0x130, 0xF, 0xF, false);  // wave_shl:1
 }
 
+// The value of lane ^ J of a wave (J a power of two below 64): a DPP quad
+// permutation for 1 and 2, a swizzle over 32 lanes up to 16, a permute for
+// 32.
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t x) {
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);  // [1,0,3,2]
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
+    else if constexpr (J < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (J << 10) | 0x1F);  // xor J
+    else return (uint32_t)__shfl_xor((int)x, J, 64);
+}
+template <int J>
+__device__ __forceinline__ uint64_t lane_xor64(uint64_t v) {
+    return ((uint64_t)lane_xor32<J>((uint32_t)(v >> 32)) << 32) | lane_xor32<J>((uint32_t)v);
+}
+
 // counters[] of a frame
 enum {
     kNRuns = 0, kNPts, kNClusters, kNQuads, kNDets, kError,

@@ -635,13 +635,9 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     for (int g0 = 0, carry = 0; g0 < ng; g0 += 64) {
         const int g = g0 + lane;
         const uint32_t c = g < ng ? (uint32_t)head[g] : 0u;
-        uint32_t inc = c;
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += o;
-        }
+        const uint32_t inc = wave_scan_incl(c);
         if (g < ng) head[g] = (uint32_t)carry + inc - c;
-        carry += (int)__shfl(inc, 63, 64);
+        carry += __builtin_amdgcn_readlane((int)inc, 63);
     }
     wsync<G>();
     edn = lane < ne ? edges[lane] : 0u;
@@ -884,8 +880,7 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
         int x = x0 + lane;
         bool valid = x <= xmax;
         uint32_t tv = valid ? trow[x] : 0u;
-        uint32_t tp = __shfl_up(tv, 1, 64);
-        if (lane == 0) tp = carry_t;
+        const uint32_t tp = wave_from_left(tv, carry_t);
         bool st = valid && (x == 0 || tv != tp) && tv != 127u;
         uint64_t sm = __ballot(st);
         uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
@@ -896,7 +891,7 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
             dout[x] = (v >> 31) ? F.rdense[v & 0x7fffffffu] : 0u;
         }
         count += __popcll(sm);
-        carry_t = __shfl(tv, 63, 64);
+        carry_t = (uint32_t)__builtin_amdgcn_readlane((int)tv, 63);
     }
     if (lane == 0) {
         const uint32_t v = F.nlab[F.row_off[F.qh] + y];

@@ -86,13 +86,25 @@ __device__ __forceinline__ void pt_unpack(uint64_t p, int *x, int *y, int *gx, i
 
 // ------------------------------------------------------------ prep
 
+// Min and max over a wave by lane exchanges (xor 32 .. 1).
+template <int J = 32>
+__device__ __forceinline__ void wave_minmax(uint32_t &mn, uint32_t &mx) {
+    mn = min(mn, lane_xor32<J>(mn));
+    mx = max(mx, lane_xor32<J>(mx));
+    if constexpr (J > 1) wave_minmax<J / 2>(mn, mx);
+}
+
 // Every cluster of every frame, NT threads a cluster (a wave: most clusters
-// are a few hundred points), blocks taking clusters in strides.
+// are a few hundred points), blocks taking clusters in strides. The first
+// kR NT points stay in registers through the passes (bounding box, keys,
+// dot product, weights); the others are read again at each.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevParams *P, int items) {
     constexpr int kDotChunk = NT * 16;  // dot-product terms staged at a time
     constexpr int kW = NT / 64;
+    constexpr int kR = 4;
     __shared__ float terms[kDotChunk];
+    __shared__ float tail[8];
     __shared__ uint32_t red[kW][4];
     __shared__ float lanes8[8];
     __shared__ int s_keep;
@@ -103,19 +115,28 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         const uint64_t *pts = F.cpts + s0;
         bool in_range = (int64_t)n >= P->mcp && (int64_t)n <= F.max_px;
+        uint64_t pv[kR];
+#pragma unroll
+        for (int r = 0; r < kR; r++) {
+            const uint32_t i = t + r * NT;
+            pv[r] = i < n ? pts[i] : 0ull;
+        }
+        // point i, for this thread's indices only (i = t + r NT)
+        auto pt_at = [&](uint32_t i) -> uint64_t {
+            const uint32_t r = (i - t) / NT;
+            return r == 0 ? pv[0] : r == 1 ? pv[1] : r == 2 ? pv[2] : r == 3 ? pv[3] : pts[i];
+        };
         // bounding box (fast/quads.c:515-528)
         uint32_t xmin = 0xffff, xmax = 0, ymin = 0xffff, ymax = 0;
         if (in_range)
             for (uint32_t i = t; i < n; i += NT) {
                 int x, y, gx, gy;
-                pt_unpack(pts[i], &x, &y, &gx, &gy);
+                pt_unpack(pt_at(i), &x, &y, &gx, &gy);
                 xmin = min(xmin, (uint32_t)x); xmax = max(xmax, (uint32_t)x);
                 ymin = min(ymin, (uint32_t)y); ymax = max(ymax, (uint32_t)y);
             }
-        for (int d = 32; d >= 1; d >>= 1) {
-            xmin = min(xmin, (uint32_t)__shfl_xor(xmin, d, 64)); xmax = max(xmax, (uint32_t)__shfl_xor(xmax, d, 64));
-            ymin = min(ymin, (uint32_t)__shfl_xor(ymin, d, 64)); ymax = max(ymax, (uint32_t)__shfl_xor(ymax, d, 64));
-        }
+        wave_minmax(xmin, xmax);
+        wave_minmax(ymin, ymax);
         if constexpr (kW > 1) {
             if (lane == 0) { red[w][0] = xmin; red[w][1] = xmax; red[w][2] = ymin; red[w][3] = ymax; }
             __syncthreads();
@@ -133,7 +154,7 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
             uint32_t slope = 0;
             if (ok) {
                 int x, y, gx, gy;
-                pt_unpack(pts[i], &x, &y, &gx, &gy);
+                pt_unpack(pt_at(i), &x, &y, &gx, &gy);
                 slope = point_slope(__fsub_rn((float)x, cx), __fsub_rn((float)y, cy));
             }
             F.okey[s0 + i] = ((uint64_t)slope << 32) | (uint32_t)~i;
@@ -148,33 +169,31 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
             continue;
         }
         // dot product: lane l of 8 sums the terms of points l, l+8, ...
-        // below n - n % 8 in order (fast/quads.c:490-504)
+        // below n - n % 8 in order, then the last n % 8 terms in order
+        // (fast/quads.c:490-504)
+        auto term = [&](uint64_t p) {
+            int x, y, gx, gy;
+            pt_unpack(p, &x, &y, &gx, &gy);
+            float dx = __fsub_rn((float)x, cx), dy = __fsub_rn((float)y, cy);
+            return __fadd_rn(__fmul_rn(dx, (float)gx), __fmul_rn(dy, (float)gy));
+        };
         const uint32_t nv = n - n % 8;
         float acc = 0.0f;
         for (uint32_t c0 = 0; c0 < nv; c0 += kDotChunk) {
             uint32_t len = min((uint32_t)kDotChunk, nv - c0);
-            for (uint32_t j = t; j < len; j += NT) {
-                int x, y, gx, gy;
-                pt_unpack(pts[c0 + j], &x, &y, &gx, &gy);
-                float dx = __fsub_rn((float)x, cx), dy = __fsub_rn((float)y, cy);
-                terms[j] = __fadd_rn(__fmul_rn(dx, (float)gx), __fmul_rn(dy, (float)gy));
-            }
+            for (uint32_t j = t; j < len; j += NT) terms[j] = term(pt_at(c0 + j));
             __syncthreads();
             if (t < 8)
                 for (uint32_t j = t; j < len; j += 8) acc = __fadd_rn(acc, terms[j]);
             __syncthreads();
         }
         if (t < 8) lanes8[t] = acc;
+        if (t < (int)(n - nv)) tail[t] = term(pts[nv + t]);
         __syncthreads();
         if (t == 0) {
             float dot = __fadd_rn(__fadd_rn(__fadd_rn(lanes8[0], lanes8[4]), __fadd_rn(lanes8[2], lanes8[6])),
                                   __fadd_rn(__fadd_rn(lanes8[1], lanes8[5]), __fadd_rn(lanes8[3], lanes8[7])));
-            for (uint32_t i = nv; i < n; i++) {
-                int x, y, gx, gy;
-                pt_unpack(pts[i], &x, &y, &gx, &gy);
-                float dx = __fsub_rn((float)x, cx), dy = __fsub_rn((float)y, cy);
-                dot = __fadd_rn(dot, __fadd_rn(__fmul_rn(dx, (float)gx), __fmul_rn(dy, (float)gy)));
-            }
+            for (uint32_t i = 0; i < n - nv; i++) dot = __fadd_rn(dot, tail[i]);
             bool rev = dot < 0.0f;
             bool keep = !((!P->reversed_border && rev) || (!P->normal_border && !rev));
             F.cflag[c] = keep ? (1u | (rev ? 2u : 0u)) : 0u;
@@ -188,7 +207,7 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
             const int iw = F.qw, ih = F.qh;
             for (uint32_t i = t; i < n; i += NT) {
                 int px, py, gx, gy;
-                pt_unpack(pts[i], &px, &py, &gx, &gy);
+                pt_unpack(pt_at(i), &px, &py, &gx, &gy);
                 int ix = (px >> 1) + (px & 1), iy = (py >> 1) + (py & 1);
                 uint32_t g2 = 0;
                 if (ix > 0 && ix + 1 < iw && iy > 0 && iy + 1 < ih) {
@@ -541,21 +560,6 @@ __device__ __forceinline__ void bitonic_pair(uint32_t q, uint32_t k, uint32_t j,
 }
 
 constexpr uint32_t kSortTile = 2048;  // keys of a cluster sorted together in LDS
-
-// The value of lane ^ J of a wave (J a power of two below 64): a DPP quad
-// permutation for 1 and 2, a swizzle over 32 lanes up to 16, a permute for
-// 32.
-template <int J>
-__device__ __forceinline__ uint32_t lane_xor32(uint32_t x) {
-    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);  // [1,0,3,2]
-    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
-    else if constexpr (J < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (J << 10) | 0x1F);  // xor J
-    else return (uint32_t)__shfl_xor((int)x, J, 64);
-}
-template <int J>
-__device__ __forceinline__ uint64_t lane_xor64(uint64_t v) {
-    return ((uint64_t)lane_xor32<J>((uint32_t)(v >> 32)) << 32) | lane_xor32<J>((uint32_t)v);
-}
 
 // One compare-exchange of distinct keys (or equal padding): the lane takes
 // its partner's key when the partner is the smaller one and the lane keeps
