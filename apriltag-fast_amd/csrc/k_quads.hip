@@ -221,6 +221,54 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
     }
 }
 
+// One compare-exchange of distinct keys (or equal padding): the lane takes
+// its partner's key when the partner is the smaller one and the lane keeps
+// the minimum, or the larger one and it keeps the maximum.
+__device__ __forceinline__ uint64_t cx_take(uint64_t mine, uint64_t other, bool keep_min) {
+    return (other < mine) == keep_min ? other : mine;
+}
+
+// Step (K, J) of a wave's bitonic sort of 64 R keys in registers (key
+// i = r * 64 + lane in v[r]), then the steps after it.
+template <int R, int K, int J>
+__device__ __forceinline__ void wave_bitonic_step(uint64_t (&v)[R]) {
+    const int lane = threadIdx.x & 63;
+    if constexpr (J >= 64) {
+        constexpr int jr = J >> 6;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (r & jr) continue;
+            const int r2 = r | jr;
+            const bool up = ((r << 6) & K) == 0;
+            const uint64_t a = v[r], b = v[r2];
+            const bool sw = (a > b) == up;
+            v[r] = sw ? b : a;
+            v[r2] = sw ? a : b;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint64_t o = lane_xor64<J>(v[r]);
+            const bool up = (((r << 6) | lane) & K) == 0;
+            v[r] = cx_take(v[r], o, ((lane & J) == 0) == up);
+        }
+    }
+    if constexpr (J > 1) wave_bitonic_step<R, K, J / 2>(v);
+}
+template <int R, int K>
+__device__ __forceinline__ void wave_bitonic_merge(uint64_t (&v)[R]) {
+    wave_bitonic_step<R, K, K / 2>(v);
+    if constexpr (K < 64 * R) wave_bitonic_merge<R, 2 * K>(v);
+}
+
+// A wave's bitonic sort of 64 R keys in registers: key i = r * 64 + lane
+// sits in v[r]; partners within a register row by lane exchanges, across
+// rows in the lane's own registers. Ascending.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[R]) {
+    wave_bitonic_merge<R, 2>(v);
+}
+
 // ------------------------------------------------------------ sort fix
 
 __device__ __forceinline__ void kswap(uint64_t *k, int a, int b) {
@@ -317,12 +365,16 @@ __device__ __noinline__ void order_runs_batch(const uint64_t *orig, int64_t n, u
     g1 = min(g1, base + (int)m);
     uint64_t ent = key, bh = key;
     if (act) run_entry(orig, n, keys, rs, m, hi, (uint32_t)j, key, lo, len, g0 - base, g1 - base, &ent, &bh);
-    uint32_t rank = 0;
-    for (int d = 0; d < 64; d++) {
-        const uint64_t b2 = __shfl(bh, min(base + d, 63), 64);
-        rank += (act && d < (int)m && (b2 < bh || (b2 == bh && d < j))) ? 1u : 0u;
-    }
-    if (act) keys[rs + rank] = ent;
+    // every run's entries in (head, position) order at once: a register
+    // sort of (run's first lane, head's low word, lane) (a run's heads share
+    // its slope, the high word); after it, lane p of a run holds the entry
+    // of rank p - base, from its source lane
+    uint64_t cv[1] = {((uint64_t)(act ? base : 127) << 38) | ((bh & 0xffffffffull) << 6) | (uint64_t)lane};
+    wave_bitonic<1>(cv);
+    const int src = (int)(cv[0] & 63u);
+    const uint64_t e = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(ent >> 32), src, 64) << 32) |
+                       (uint32_t)__shfl((int)(uint32_t)ent, src, 64);
+    if (act) keys[rs + (uint32_t)j] = e;
 }
 
 // order_run for a longer run, 64 keys at a time, the entries and heads in
@@ -561,53 +613,6 @@ __device__ __forceinline__ void bitonic_pair(uint32_t q, uint32_t k, uint32_t j,
 
 constexpr uint32_t kSortTile = 2048;  // keys of a cluster sorted together in LDS
 
-// One compare-exchange of distinct keys (or equal padding): the lane takes
-// its partner's key when the partner is the smaller one and the lane keeps
-// the minimum, or the larger one and it keeps the maximum.
-__device__ __forceinline__ uint64_t cx_take(uint64_t mine, uint64_t other, bool keep_min) {
-    return (other < mine) == keep_min ? other : mine;
-}
-
-// Step (K, J) of a wave's bitonic sort of 64 R keys in registers (key
-// i = r * 64 + lane in v[r]), then the steps after it.
-template <int R, int K, int J>
-__device__ __forceinline__ void wave_bitonic_step(uint64_t (&v)[R]) {
-    const int lane = threadIdx.x & 63;
-    if constexpr (J >= 64) {
-        constexpr int jr = J >> 6;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            if (r & jr) continue;
-            const int r2 = r | jr;
-            const bool up = ((r << 6) & K) == 0;
-            const uint64_t a = v[r], b = v[r2];
-            const bool sw = (a > b) == up;
-            v[r] = sw ? b : a;
-            v[r2] = sw ? a : b;
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const uint64_t o = lane_xor64<J>(v[r]);
-            const bool up = (((r << 6) | lane) & K) == 0;
-            v[r] = cx_take(v[r], o, ((lane & J) == 0) == up);
-        }
-    }
-    if constexpr (J > 1) wave_bitonic_step<R, K, J / 2>(v);
-}
-template <int R, int K>
-__device__ __forceinline__ void wave_bitonic_merge(uint64_t (&v)[R]) {
-    wave_bitonic_step<R, K, K / 2>(v);
-    if constexpr (K < 64 * R) wave_bitonic_merge<R, 2 * K>(v);
-}
-
-// A wave's bitonic sort of 64 R keys in registers: key i = r * 64 + lane
-// sits in v[r]; partners within a register row by lane exchanges, across
-// rows in the lane's own registers. Ascending.
-template <int R>
-__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[R]) {
-    wave_bitonic_merge<R, 2>(v);
-}
 
 template <int R>
 __device__ __forceinline__ void wave_sort_keys(const uint64_t *in, uint64_t *out, uint32_t n) {
