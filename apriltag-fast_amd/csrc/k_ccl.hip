@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames, int lds_q
                     out[at++] = ((uint32_t)i << 16) | (uint32_t)k;
             }
         }
-        carry += __shfl(inc, 63, 64);
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     }
     if (lane == 0) {
         if (carry > cap) atomicOr(F.counters + kError, 1u);
@@ -235,7 +235,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             uint32_t c = y < QH ? F.rcnt[y] : 0u;
             uint32_t inc = wave_incl_scan(c);
             if (y < QH) F.row_off[y] = carry + inc - c;
-            carry += __shfl(inc, 63, 64);
+            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
         }
         if (lane == 0) {
             F.row_off[QH] = carry;

@@ -22,6 +22,8 @@
 // they are evaluated in double and rounded once to float.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "atg_internal.h"
 
 #pragma clang fp contract(off)
@@ -394,12 +396,19 @@ __global__ __launch_bounds__(NT) void k_refine(const Frame *frames, const DevPar
                         y0 = DADD(DMUL(alpha, (double)qp[ea][1]), DMUL(DSUB(1.0, alpha), (double)qp[eb][1]));
                         refine_partial(v, range, vsteps, x0, y0, nx, ny, part, &ln, &lc);
                     }
-                    double lns[4], lcs[4];
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        lns[k] = __shfl(ln, (lane & ~3) + k, 64);
-                        lcs[k] = __shfl(lc, (lane & ~3) + k, 64);
-                    }
+                    // the quad's four partial sums in every lane of the quad
+                    // (DPP quad broadcasts)
+                    auto qb = [](double v, auto kc) {
+                        constexpr int ctl = decltype(kc)::value * 0x55;
+                        const uint64_t b = (uint64_t)__double_as_longlong(v);
+                        const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, ctl, 0xF, 0xF, false);
+                        const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), ctl, 0xF, 0xF, false);
+                        return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+                    };
+                    const double lns[4] = {qb(ln, std::integral_constant<int, 0>{}), qb(ln, std::integral_constant<int, 1>{}),
+                                           qb(ln, std::integral_constant<int, 2>{}), qb(ln, std::integral_constant<int, 3>{})};
+                    const double lcs[4] = {qb(lc, std::integral_constant<int, 0>{}), qb(lc, std::integral_constant<int, 1>{}),
+                                           qb(lc, std::integral_constant<int, 2>{}), qb(lc, std::integral_constant<int, 3>{})};
                     if (part == 0) {
                         int ok = 0;
                         if (si < ns) {

@@ -152,10 +152,10 @@ __device__ __forceinline__ void row_runs(const Frame &F, int y, TF tfun) {
         int x = x0 + lane;
         bool valid = x <= xmax;
         uint32_t tv = valid ? tfun(x) : 0u;
-        uint32_t tp = __shfl_up(tv, 1, 64);
-        if (lane == 0) tp = carry_t;
-        uint32_t tn = __shfl_down(tv, 1, 64);
-        if (lane == 63 && x + 1 <= xmax) tn = tfun(x + 1);
+        const uint32_t tp = wave_from_left(tv, carry_t);
+        uint32_t fill = 0;
+        if (lane == 63 && x + 1 <= xmax) fill = tfun(x + 1);
+        const uint32_t tn = wave_from_right(tv, fill);
         bool is_start = valid && (x == 0 || tv != tp);
         bool is_end = valid && (x == xmax || tn != tv);
         uint64_t smask = __ballot(is_start);
@@ -170,7 +170,7 @@ __device__ __forceinline__ void row_runs(const Frame &F, int y, TF tfun) {
         }
         if (smask) carry_start = x0 + 63 - __clzll(smask);
         count += __popcll(emask);
-        carry_t = __shfl(tv, 63, 64);
+        carry_t = (uint32_t)__builtin_amdgcn_readlane((int)tv, 63);
     }
     if (lane == 0) F.rcnt[y] = (uint32_t)count;
 }

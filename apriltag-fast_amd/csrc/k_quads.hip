@@ -474,7 +474,8 @@ __device__ void fix_runs_block(uint64_t *keys, uint32_t n, const uint64_t *orig,
             const uint32_t rs_l = i < b ? ls[i] : 0u, m_l = i < b ? lm[i] : 0u;
             const uint32_t cnt = min(64u, b - i0);
             for (uint32_t q = 0; q < cnt; q++) {
-                const uint32_t rs = (uint32_t)__shfl((int)rs_l, (int)q, 64), m = (uint32_t)__shfl((int)m_l, (int)q, 64);
+                const uint32_t rs = (uint32_t)__builtin_amdgcn_readlane((int)rs_l, (int)q),
+                               m = (uint32_t)__builtin_amdgcn_readlane((int)m_l, (int)q);
                 if (m <= 64) {
                     if (used + (int)m > 64) {
                         order_runs_batch(orig, n, keys, my_rs, my_m, my_base);
@@ -993,7 +994,7 @@ template <int NT>
 __device__ __forceinline__ uint32_t bscan_t(uint32_t v, uint32_t *total) {
     if constexpr (NT == 64) {
         const uint32_t inc = wscan(v);
-        *total = __shfl(inc, 63, 64);
+        *total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
         return inc - v;
     } else {
         return bscan(v, total);
