@@ -299,8 +299,9 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 // no bits at shift; the skipped passes are the last ones. pk: the frame's
 // words are packed where sort_packed says so (the key is their low
 // kSortBits bits).
-// Digit counts of each block's segment, rhist[block][digit]; the lanes of
-// a wave with the same digit add once.
+// Digit counts of each block's segment, rhist[block][digit]; each run of
+// equal digits on adjacent lanes adds its length once (a digit that comes
+// back later in the wave adds again: counts only, no order).
 __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int which, int shift, int parity, int dyn,
                                                     int pk) {
     __shared__ uint32_t h[kRadixBins];
@@ -319,9 +320,14 @@ __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int whi
     for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
         const uint32_t i = c0 + t;
         const bool valid = i < s1;
-        const uint32_t d = valid ? (uint32_t)((B.kin[i] & km) >> shift) & (kRadixBins - 1) : 0u;
-        const uint64_t peers = digit_peers(d, valid);
-        if (valid && !(peers & lt)) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+        const uint32_t d = valid ? (uint32_t)((B.kin[i] & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
+        const uint32_t dl = wave_from_left(d, 0xfffffffeu);  // lane 0: a head
+        const uint64_t heads = __ballot(valid && d != dl) | ~__ballot(valid);  // invalid lanes end runs
+        if (valid && d != dl) {
+            const uint64_t after = heads & ~(lt | (1ull << lane));
+            const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
+            atomicAdd(&h[d], end - (uint32_t)lane);
+        }
     }
     __syncthreads();
     for (int i = t; i < kRadixBins; i += 256) F.rhist[blockIdx.x * kRadixBins + i] = h[i];
