@@ -415,10 +415,36 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
         const bool valid = i < s1;
         const uint64_t key = valid ? B.kin[i] : 0ull;
         const uint32_t val = (valid && !packed) ? (iota ? i : B.vin[i]) : 0u;
-        const uint32_t d = (uint32_t)((key & km) >> shift) & (kRadixBins - 1);
-        const uint64_t peers = digit_peers(d, valid);
-        const uint32_t rank = (uint32_t)__popcll(peers & lt), cnt = (uint32_t)__popcll(peers);
-        const bool lead = valid && rank == 0;
+        const uint32_t d = valid ? (uint32_t)((key & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
+        // the wave's runs of equal digits on adjacent lanes; where every
+        // digit of the wave is one run, a lane's rank is its place in the
+        // run (each run head marks its digit with its lane: a head that reads
+        // back another lane's mark shares the digit with another run, and
+        // the wave then ranks by ballots over the digit bits)
+        const uint32_t dl = wave_from_left(d, 0xfffffffeu);
+        const bool head = valid && d != dl;
+        const uint64_t hm = __ballot(head) | ~__ballot(valid);
+        const uint64_t at_or_before = lt | (1ull << lane);
+        uint32_t rank, cnt;
+        bool lead;
+        if (head) wcnt[w][d] = (uint8_t)(lane + 1);
+        // (LDS fences: the read must see every head's store, not this lane's own)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        const bool clash = head && wcnt[w][d] != (uint8_t)(lane + 1);
+        if (!__ballot(clash)) {
+            const uint64_t upto = hm & at_or_before, after = hm & ~at_or_before;
+            rank = (uint32_t)lane - (uint32_t)(63 - __clzll((long long)upto));
+            cnt = (after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u) - (uint32_t)lane;
+            lead = head;
+        } else {
+            const uint64_t peers = digit_peers(d, valid);
+            rank = (uint32_t)__popcll(peers & lt);
+            cnt = (uint32_t)__popcll(peers);
+            lead = valid && rank == 0;
+        }
+        if (head) wcnt[w][d] = 0;
         if (lead) wcnt[w][d] = (uint8_t)cnt;
         __syncthreads();
         if (valid) {
