@@ -432,8 +432,10 @@ __global__ __launch_bounds__(256) void k_front_dense(const Frame *frames) {
 // to, or kLostNone / kLostRetired), rroot[slot] of each row-y record (the
 // root of its set as replayed: a node or kSym | a previous record's slot).
 //
-// Rows of at most C = 512 runs (each of the two rows) and 2C links replay
-// in LDS (k_replay, a wave per row); the wider rows, which it lists, in
+// Rows of at most C = 448 runs (512 for calls of up to 32 frames; each of the
+// two rows) and replay_links(C) links replay in LDS (k_replay, a wave per
+// row, about 9 KiB at 448: 18 rows in flight a CU); the wider rows, which it
+// lists, in
 // k_replay_wide with LDS for the widest row of the call (C = qw, 3C links); rows
 // beyond that, and all rows when asked, in HBM scratch sized for the row
 // width (k_replay_hbm).
@@ -462,9 +464,19 @@ __host__ __device__ inline ReplayLayout replay_layout(int C, int L, int ix, int 
 
 // LDS capacity (runs per row) of the first replay launch for frames up to
 // max_qw wide, and of the second (the rows wider than the first's).
-int replay_cap(int max_qw) { return std::min(512, (std::max(max_qw, 1) + 63) & ~63); }
+// Links of a row replayed in LDS with cap runs (rows with more go to the
+// wide launch): a quarter more than the runs, in 64s (a 1080p row has at
+// most about 1.05 links a run).
+__host__ __device__ inline int replay_links(int cap) { return (cap + cap / 4 + 63) & ~63; }
+// Runs a row of the first replay launch: 448 for calls of more than 32
+// frames (more rows in flight a CU; the few wider rows spread over the wide
+// launch), 512 for smaller calls, whose wider rows would be a serial tail
+// there.
+int replay_cap(int max_qw, int nframes) {
+    return std::min(nframes > 32 ? 448 : 512, (std::max(max_qw, 1) + 63) & ~63);
+}
 int replay_wide_cap(int max_qw) { return (std::max(max_qw, 1) + 63) & ~63; }
-size_t replay_lds_bytes(int cap, bool pk) { return replay_layout(cap, 2 * cap, 2, pk ? 4 : 8).total; }
+size_t replay_lds_bytes(int cap, bool pk) { return replay_layout(cap, replay_links(cap), 2, pk ? 4 : 8).total; }
 size_t replay_wide_lds_bytes(int cap) { return replay_layout(cap, 3 * cap, 2).total; }
 __host__ __device__ inline size_t replay_hbm_bytes_d(int qw) { return replay_layout(qw, 3 * qw, 4).total; }
 size_t replay_hbm_bytes(int qw) { return replay_hbm_bytes_d(qw); }
@@ -748,7 +760,7 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     for (int i = lane; i < ncur; i += 64) F.rroot[cb + (uint32_t)nxt[i]] = name(ent.find_ro((uint32_t)(m + i)));
 }
 
-// A wave per row, the rows that fit C = cap runs and 2C links; the others
+// A wave per row, the rows that fit C = cap runs and replay_links(C) links; the others
 // are listed (rwide) for the wide launch. PK: packed elements.
 template <bool PK>
 __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
@@ -756,11 +768,11 @@ __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
     const Frame &F = frames[blockIdx.y];
     const int y = 1 + (int)blockIdx.x;
     if (y >= F.qh) return;
-    if (!replay_fits(F, y, cap, 2 * cap)) {
+    if (!replay_fits(F, y, cap, replay_links(cap))) {
         if (threadIdx.x == 0) F.rwide[atomicAdd(F.counters + kNWideRows, 1u)] = (uint32_t)y;
         return;
     }
-    replay_row<false, PK>(F, y, lds, cap, 2 * cap);
+    replay_row<false, PK>(F, y, lds, cap, replay_links(cap));
 }
 
 // The listed rows of every frame (list 3 of the work queue) that fit
@@ -937,11 +949,11 @@ size_t replay_scratch_bytes(int qw, bool force) {
 // The launch shapes of a call: max_lds_qw is the widest frame whose front
 // runs in LDS (0: none), max_qw the widest frame, max_weight the most
 // pixels a set of a frame can weigh (qw qh plus the virtual nodes).
-CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch) {
+CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch, int nframes) {
     CclPlan c{};
     c.front = max_lds_qw > 0 ? (int)front_bytes(max_lds_qw) : 0;
     // rows of up to replay_cap runs in LDS, the others (all when forced) in HBM
-    c.replay_cap = force_scratch ? 0 : replay_cap(max_qw);
+    c.replay_cap = force_scratch ? 0 : replay_cap(max_qw, nframes);
     // packed elements when every weight fits 21 bits (and a parent 11)
     c.replay_pack = c.replay_cap && c.replay_cap < 1023 && max_weight < (1 << 21);
     c.replay = c.replay_cap ? (int)replay_lds_bytes(c.replay_cap, c.replay_pack) : 0;
@@ -1012,7 +1024,7 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
                            dframes, Q, c.replay_wide_cap, 3 * c.replay_wide_cap);
     }
     const int hcap = c.replay_wide_cap ? c.replay_wide_cap : c.replay_cap;
-    const int hlcap = c.replay_wide_cap ? 3 * c.replay_wide_cap : 2 * c.replay_cap;
+    const int hlcap = c.replay_wide_cap ? 3 * c.replay_wide_cap : replay_links(c.replay_cap);
     hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, hcap, hlcap);
     mk->mark(st, "uf_replay");
     for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);

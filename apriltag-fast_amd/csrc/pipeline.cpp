@@ -21,7 +21,7 @@
 namespace atg {
 void launch_init(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q);
 void launch_copy(hipStream_t st, void *dst, const void *src, int64_t bytes);
-CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch);
+CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch, int nframes);
 void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                         int max_qh, bool blur, bool deglitch, const Tune &tu, Marks *mk);
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
@@ -486,7 +486,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
     }
     // the union-find's launch shapes (its HBM state is in each frame's region)
-    const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, max_weight, ctx->force_scratch);
+    const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, max_weight, ctx->force_scratch, (int)n);
     const size_t qoff = total;
     total += queue_bytes(n);
     if (total > ctx->arena_size && ctx->arena_fixed) return fail(ctx, ATG_ERR_LIMIT, "batch over the arena budget");
