@@ -1117,7 +1117,37 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
             // above it are kept, in order
             const uint32_t rank = (uint32_t)P->max_nmaxima;  // 0-based, descending
             double thr;
-            if (nm <= (uint32_t)kMaxima) {
+            if (NT == 64 && nm <= (uint32_t)kMaxima) {
+                // a register sort of the maxima's order-preserving keys
+                // (absent entries 0, first): the value at descending rank
+                // `rank` (which of equal values is taken does not matter)
+                uint64_t kv[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t i = (uint32_t)(r * 64 + lane);
+                    kv[r] = i < nm ? dkey(y[midx[i]]) : 0ull;
+                }
+                int R = 4;
+                if (nm <= 64) {
+                    uint64_t a[1] = {kv[0]};
+                    wave_bitonic<1>(a);
+                    kv[0] = a[0];
+                    R = 1;
+                } else if (nm <= 128) {
+                    uint64_t a[2] = {kv[0], kv[1]};
+                    wave_bitonic<2>(a);
+                    kv[0] = a[0];
+                    kv[1] = a[1];
+                    R = 2;
+                } else {
+                    wave_bitonic<4>(kv);
+                }
+                const uint32_t pos = (uint32_t)(64 * R) - 1u - rank;
+                const uint64_t row = pos < 64 ? kv[0] : pos < 128 ? kv[1] : pos < 192 ? kv[2] : kv[3];
+                const uint64_t k = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)(pos & 63)) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)(pos & 63));
+                thr = __longlong_as_double((long long)((k >> 63) ? (k ^ 0x8000000000000000ull) : ~k));
+            } else if (nm <= (uint32_t)kMaxima) {
                 // the rank of each value among the maxima (ties by position)
                 double *mv = sh + 2 * kLdsPts;
                 for (uint32_t i = t; i < nm; i += NT) mv[i] = y[midx[i]];
