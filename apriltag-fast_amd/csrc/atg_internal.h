@@ -209,7 +209,10 @@ struct CclPlan {
     int replay;                 // LDS bytes of the replay
     int replay_cap;             // runs per row replayed in LDS, 0 = none
     bool replay_pack;           // its elements packed (parent, weight) in one word
-    int replay_wide, replay_wide_cap;  // the second LDS launch (wider rows): bytes, runs; 0 = none
+    int replay_mid, replay_mid_cap;    // the second LDS launch (listed rows up to 704 runs): bytes, runs; 0 = none
+    bool replay_mid_pack;       // its elements packed
+    int replay_mid_grid;        // its persistent grid
+    int replay_wide, replay_wide_cap;  // the third (the wider rows, up to the widest): bytes, runs; 0 = none
     int replay_wide_grid;       // its persistent grid
     int replay_waves;           // waves per frame of k_replay_hbm
 };

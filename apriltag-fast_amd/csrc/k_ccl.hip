@@ -435,10 +435,9 @@ __global__ __launch_bounds__(256) void k_front_dense(const Frame *frames) {
 // Rows of at most C = 448 runs (512 for calls of up to 32 frames; each of the
 // two rows) and replay_links(C) links replay in LDS (k_replay, a wave per
 // row, about 9 KiB at 448: 18 rows in flight a CU); the wider rows, which it
-// lists, in
-// k_replay_wide with LDS for the widest row of the call (C = qw, 3C links); rows
-// beyond that, and all rows when asked, in HBM scratch sized for the row
-// width (k_replay_hbm).
+// lists, in k_replay_wide: first those of up to 704 runs, then the others with
+// LDS for the widest row of the call (C = qw, 3C links); rows beyond that, and
+// all rows when asked, in HBM scratch sized for the row width (k_replay_hbm).
 
 // Byte offsets of a row's replay memory for C runs, L links and ix-byte
 // link indices.
@@ -475,7 +474,13 @@ __host__ __device__ inline int replay_links(int cap) { return (cap + cap / 4 + 6
 int replay_cap(int max_qw, int nframes) {
     return std::min(nframes > 32 ? 448 : 512, (std::max(max_qw, 1) + 63) & ~63);
 }
+// The listed wider rows replay in two LDS launches: rows of up to 704 runs
+// (a 2448x2048 row has at most about 620) with replay_links(704) links, about
+// 14 KiB packed (11 rows in flight a CU), then the rest of the rows up to the
+// widest row of the call (3C links, 3 rows a CU at C = 1280).
+constexpr int kReplayMidCap = 704;
 int replay_wide_cap(int max_qw) { return (std::max(max_qw, 1) + 63) & ~63; }
+int replay_mid_cap(int max_qw) { return std::min(replay_wide_cap(max_qw), kReplayMidCap); }
 size_t replay_lds_bytes(int cap, bool pk) { return replay_layout(cap, replay_links(cap), 2, pk ? 4 : 8).total; }
 size_t replay_wide_lds_bytes(int cap) { return replay_layout(cap, 3 * cap, 2).total; }
 __host__ __device__ inline size_t replay_hbm_bytes_d(int qw) { return replay_layout(qw, 3 * qw, 4).total; }
@@ -776,9 +781,12 @@ __global__ __launch_bounds__(64) void k_replay(const Frame *frames, int cap) {
 }
 
 // The listed rows of every frame (list 3 of the work queue) that fit
-// C = cap runs and lcap links: persistent, blocks taking list entries in
-// strides (a frame cursor).
-__global__ __launch_bounds__(64) void k_replay_wide(const Frame *frames, const WorkQueue *Q, int cap, int lcap) {
+// C = cap runs and lcap links but not pcap runs and plcap links (pcap 0:
+// every listed row that fits): persistent, blocks taking list entries in
+// strides (a frame cursor). PK: packed elements.
+template <bool PK>
+__global__ __launch_bounds__(64) void k_replay_wide(const Frame *frames, const WorkQueue *Q, int cap, int lcap,
+                                                    int pcap, int plcap) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t total = Q->total[3];
     const uint32_t *off = Q->off[3];
@@ -787,7 +795,8 @@ __global__ __launch_bounds__(64) void k_replay_wide(const Frame *frames, const W
         while (off[fi + 1] <= g) fi++;
         const Frame &F = frames[fi];
         const int y = (int)F.rwide[g - off[fi]];
-        if (replay_fits(F, y, cap, lcap)) replay_row<false, false>(F, y, lds, cap, lcap);
+        if (replay_fits(F, y, cap, lcap) && !(pcap && replay_fits(F, y, pcap, plcap)))
+            replay_row<false, PK>(F, y, lds, cap, lcap);
         __syncthreads();  // the next row reuses the memory
     }
 }
@@ -957,21 +966,31 @@ CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scra
     // packed elements when every weight fits 21 bits (and a parent 11)
     c.replay_pack = c.replay_cap && c.replay_cap < 1023 && max_weight < (1 << 21);
     c.replay = c.replay_cap ? (int)replay_lds_bytes(c.replay_cap, c.replay_pack) : 0;
-    // the wider rows: a second LDS launch when the widest row fits LDS
+    // the wider rows: LDS launches up to 704 runs, then up to the widest
+    // row when that fits LDS
+    c.replay_mid_cap = 0;
     c.replay_wide_cap = 0;
     c.replay_wide = 0;
     if (c.replay_cap && replay_wide_cap(max_qw) > c.replay_cap) {
+        c.replay_mid_cap = replay_mid_cap(max_qw);
+        c.replay_mid_pack = c.replay_mid_cap < 1023 && max_weight < (1 << 21);
+        c.replay_mid = (int)replay_lds_bytes(c.replay_mid_cap, c.replay_mid_pack);
+        const void *fm = c.replay_mid_pack ? reinterpret_cast<const void *>(&k_replay_wide<true>)
+                                           : reinterpret_cast<const void *>(&k_replay_wide<false>);
+        c.replay_mid_grid = (int)persistent_grid(fm, 64, (size_t)c.replay_mid, 1 << 16);
+    }
+    if (c.replay_mid_cap && replay_wide_cap(max_qw) > c.replay_mid_cap) {
+        const void *fw = reinterpret_cast<const void *>(&k_replay_wide<false>);
         const size_t wb = replay_wide_lds_bytes(replay_wide_cap(max_qw));
         bool ok = wb <= 64 * 1024;
         if (!ok && wb <= 159 * 1024) {
-            ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_replay_wide),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)wb) == hipSuccess;
+            ok = hipFuncSetAttribute(fw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)wb) == hipSuccess;
             (void)hipGetLastError();
         }
         if (ok) {
             c.replay_wide_cap = replay_wide_cap(max_qw);
             c.replay_wide = (int)wb;
-            c.replay_wide_grid = (int)persistent_grid(reinterpret_cast<const void *>(&k_replay_wide), 64, wb, 1 << 16);
+            c.replay_wide_grid = (int)persistent_grid(fw, 64, wb, 1 << 16);
         }
     }
     c.replay_waves = force_scratch ? kReplayHbmWavesAll : kReplayHbmWaves;
@@ -1018,13 +1037,25 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
         hipLaunchKernelGGL(k_replay<true>, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
     else if (c.replay_cap)
         hipLaunchKernelGGL(k_replay<false>, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
-    if (c.replay_wide_cap) {
+    int hcap = c.replay_cap, hlcap = replay_links(c.replay_cap);
+    if (c.replay_mid_cap) {
         launch_work_offsets(st, dframes, nframes, Q, 3, (int)kNWideRows);
-        hipLaunchKernelGGL(k_replay_wide, dim3((unsigned)c.replay_wide_grid), dim3(64), (size_t)c.replay_wide, st,
-                           dframes, Q, c.replay_wide_cap, 3 * c.replay_wide_cap);
+        const int mcap = c.replay_mid_cap, mlcap = replay_links(mcap);
+        if (c.replay_mid_pack)
+            hipLaunchKernelGGL(k_replay_wide<true>, dim3((unsigned)c.replay_mid_grid), dim3(64), (size_t)c.replay_mid,
+                               st, dframes, Q, mcap, mlcap, 0, 0);
+        else
+            hipLaunchKernelGGL(k_replay_wide<false>, dim3((unsigned)c.replay_mid_grid), dim3(64), (size_t)c.replay_mid,
+                               st, dframes, Q, mcap, mlcap, 0, 0);
+        hcap = mcap;
+        hlcap = mlcap;
     }
-    const int hcap = c.replay_wide_cap ? c.replay_wide_cap : c.replay_cap;
-    const int hlcap = c.replay_wide_cap ? 3 * c.replay_wide_cap : replay_links(c.replay_cap);
+    if (c.replay_wide_cap) {
+        hipLaunchKernelGGL(k_replay_wide<false>, dim3((unsigned)c.replay_wide_grid), dim3(64), (size_t)c.replay_wide,
+                           st, dframes, Q, c.replay_wide_cap, 3 * c.replay_wide_cap, hcap, hlcap);
+        hcap = c.replay_wide_cap;
+        hlcap = 3 * c.replay_wide_cap;
+    }
     hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, hcap, hlcap);
     mk->mark(st, "uf_replay");
     for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);

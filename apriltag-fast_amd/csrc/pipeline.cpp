@@ -621,7 +621,9 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         // one graph per call shape: everything the stream work depends on
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
-                                    ccl.replay_cap, ccl.replay_pack, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid, ccl.replay_waves,        max_lds_qw,
+                                    ccl.replay_cap, ccl.replay_pack, ccl.replay_mid, ccl.replay_mid_cap, ccl.replay_mid_pack,
+                                    ccl.replay_mid_grid, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid,
+                                    ccl.replay_waves, max_lds_qw,
                                     tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners, ctx->hp.refine,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
