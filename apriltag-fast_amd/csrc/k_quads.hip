@@ -37,6 +37,9 @@ namespace atg {
 // Correctly rounded float sqrt and division. The device's single-precision
 // sqrt is not correctly rounded; double has more than 2*24+2 bits, so one
 // double operation rounded once to float is the correctly rounded result.
+#ifndef ATG_SCAN_RING
+#define ATG_SCAN_RING 16
+#endif
 __device__ __forceinline__ float fsqrt_cr(float x) { return (float)__dsqrt_rn((double)x); }
 __device__ __forceinline__ float fdiv_cr(float a, float b) { return (float)__ddiv_rn((double)a, (double)b); }
 
@@ -773,7 +776,7 @@ __global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, con
 // b, b + G, ... (a frame cursor): NT threads a cluster, the running sums
 // in chunks of CH points.
 template <int NT, int CH>
-__global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
+__global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
                                                    int keys_ready, int l) {
     __shared__ double sm[6][CH + 2];  // rows 2 doubles apart in the banks: the scan's lanes conflict-free
     __shared__ uint32_t fls[4 * NT + 1], flm[4 * NT + 1], fcnt;
@@ -842,34 +845,44 @@ __global__ __launch_bounds__(NT) void k_fit_moments(const Frame *frames, int par
             }
             __syncthreads();
             if (t < 6) {
-                // one lane per sum, in order; each of the 8 registers takes
-                // the next group's term as soon as its add has used it
+                // one lane per sum, in order; each of the kRing registers
+                // takes the term kRing ahead as soon as its add has used it,
+                // so a read has kRing - 1 dependent adds to arrive in
+                constexpr uint32_t kRing = ATG_SCAN_RING;
                 double s = run;
                 double *a = sm[t];
-                const uint32_t len8 = len & ~7u;
                 uint32_t j = 0;
-                double v[8];
-                if (len8) {
+                double v[kRing];
+                if (len >= kRing) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) v[q] = a[q];
-                    for (; j + 8 < len8; j += 8) {
+                    for (uint32_t q = 0; q < kRing; q++) v[q] = a[q];
+                    for (; j + 2 * kRing <= len; j += kRing) {
 #pragma unroll
-                        for (int q = 0; q < 8; q++) {
+                        for (uint32_t q = 0; q < kRing; q++) {
                             s = __dadd_rn(s, v[q]);
                             a[j + q] = s;
-                            v[q] = a[j + 8 + q];
+                            v[q] = a[j + kRing + q];
                         }
                     }
 #pragma unroll
-                    for (int q = 0; q < 8; q++) {
+                    for (uint32_t q = 0; q < kRing; q++) {
                         s = __dadd_rn(s, v[q]);
                         a[j + q] = s;
                     }
-                    j += 8;
+                    j += kRing;
                 }
-                for (; j < len; j++) {
-                    s = __dadd_rn(s, a[j]);
-                    a[j] = s;
+                // the rest (fewer than kRing), 8 reads in flight at a time
+                for (; j < len; j += 8) {
+                    double w[8];
+#pragma unroll
+                    for (uint32_t q = 0; q < 8; q++) w[q] = j + q < len ? a[j + q] : 0.0;
+#pragma unroll
+                    for (uint32_t q = 0; q < 8; q++) {
+                        if (j + q < len) {
+                            s = __dadd_rn(s, w[q]);
+                            a[j + q] = s;
+                        }
+                    }
                 }
                 run = s;
             }
@@ -1429,7 +1442,7 @@ static unsigned pgrid(K *fn, int threads, int want) {
 void warm_persistent_grids(int fit_blocks) {
     pgrid(k_angle_sort, 256, fit_blocks);
     pgrid(k_angle_sort_wave, 64, fit_blocks);
-    pgrid(k_fit_moments<256, 512>, 256, fit_blocks);
+    pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks);
     pgrid(k_fit_moments<64, 128>, 64, fit_blocks);
     pgrid(k_fit_corners<256>, 256, fit_blocks);
     pgrid(k_fit_corners<64>, 64, fit_blocks);
@@ -1468,8 +1481,8 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0, 0);
     }
     mk->mark(st, "angle_sort");
-    hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256), 0,
-                       st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+    hipLaunchKernelGGL((k_fit_moments<128, 256>), dim3(pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks)), dim3(128),
+                       0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
     hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, st,
                        dframes, parity, Q, sort_cap ? 1 : 0, 1);
     mk->mark(st, "moments");
