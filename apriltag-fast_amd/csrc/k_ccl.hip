@@ -824,10 +824,15 @@ __global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap,
 // ---------------------------------------------------------------------
 // Step 3: the actual roots. rroot[slot] names a node or a record of the
 // previous row, so the roots resolve by pointer jumping over the slots
-// (k_root_jump, one pass per doubling of the resolved distance, every row
+// (k_root_jump, one pass per quadrupling of the resolved distance, every row
 // at once); then k_lost gives each record root that lost in the next row
 // its winner as parent, and each set that ended its final weight.
 
+// Hops a k_root_jump pass follows. If every slot's value was at least D
+// links up its chain when the pass began, it is at least (kRootHops + 1) D
+// up when the pass ends (each value read spans D more), so P passes with
+// (kRootHops + 1)^P >= qh reach every root (a chain climbs a row a link).
+constexpr int kRootHops = 3;
 __global__ __launch_bounds__(256) void k_root_jump(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63;
@@ -837,7 +842,12 @@ __global__ __launch_bounds__(256) void k_root_jump(const Frame *frames) {
     const int n = (int)F.nrec[y];
     for (int l = lane; l < n; l += 64) {
         uint32_t v = F.rroot[cb + l];
-        if (v & kSym) F.rroot[cb + l] = F.rroot[v & ~kSym];  // an ancestor's value: never a wrong root
+        if (!(v & kSym)) continue;
+        // kRootHops ancestors' values in turn: each is never a wrong root
+#pragma unroll
+        for (int h = 0; h < kRootHops; h++)
+            if (v & kSym) v = F.rroot[v & ~kSym];
+        F.rroot[cb + l] = v;
     }
 }
 
@@ -1058,7 +1068,7 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     }
     hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, hcap, hlcap);
     mk->mark(st, "uf_replay");
-    for (int d = 1; d < max_qh; d <<= 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);
+    for (int64_t d = 1; d < max_qh; d *= kRootHops + 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);
     hipLaunchKernelGGL(k_lost, rg, rb, 0, st, dframes);
     mk->mark(st, "uf_stitch");
     int64_t maxnodes = (int64_t)max_qw * max_qh + max_qh;

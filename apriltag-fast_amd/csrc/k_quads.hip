@@ -1443,6 +1443,7 @@ void warm_persistent_grids(int fit_blocks) {
     pgrid(k_angle_sort, 256, fit_blocks);
     pgrid(k_angle_sort_wave, 64, fit_blocks);
     pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks);
+    pgrid(k_fit_moments<256, 512>, 256, fit_blocks);
     pgrid(k_fit_moments<64, 128>, 64, fit_blocks);
     pgrid(k_fit_corners<256>, 256, fit_blocks);
     pgrid(k_fit_corners<64>, 64, fit_blocks);
@@ -1481,8 +1482,14 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0, 0);
     }
     mk->mark(st, "angle_sort");
-    hipLaunchKernelGGL((k_fit_moments<128, 256>), dim3(pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks)), dim3(128),
-                       0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+    // large clusters: 128-thread blocks for batches (8 a CU: more running
+    // sums in flight), 256 for up to 16 frames (each cluster sooner)
+    if (nframes > 16)
+        hipLaunchKernelGGL((k_fit_moments<128, 256>), dim3(pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks)),
+                           dim3(128), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+    else
+        hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256),
+                           0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
     hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, st,
                        dframes, parity, Q, sort_cap ? 1 : 0, 1);
     mk->mark(st, "moments");
