@@ -119,7 +119,7 @@ enum {
     kNSmall = 9,  // kept clusters of at most kLargeCluster points
     kRootBits = 10,  // bits of the dense ok-root numbers
     kSortBits = 11,  // bits of the cluster sort key
-    kNWideRows = 12,  // rows the first replay launch left (over 512 runs), listed in rwide
+    kNWideRows = 12,  // rows the first replay launch left (over its run or link capacity), listed in rwide
     kNHuge = 13,  // kept clusters above kHugeCluster points
     kNCounters = 16
 };

@@ -192,7 +192,7 @@ __device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, ui
 // row ahead; the rest (rare) from HBM at use.
 constexpr int kStage = 1024;
 
-size_t front_bytes(int qw) { return (size_t)36 * qw + 128; }
+size_t front_bytes(int qw) { return (size_t)48 * qw + 128; }  // two sets of three [2 qw + 2] arrays
 
 // A barrier of the whole block over LDS only (the scratch variant orders
 // HBM too).
@@ -211,22 +211,24 @@ template <bool G, int kFrontThreads>
 __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) {
     constexpr int kPer = kStage / kFrontThreads;  // prefetched runs and links per thread
     using U32 = typename Mem<G>::U32;
-    using U16 = typename Mem<G>::U16;
     (void)lds_small;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int QW = F.qw, QH = F.qh;
     // elements of row y: the records of row y-1 at [0, QW) (each named by
     // its first run in row y-1), the runs of row y at QW + i, the virtual
-    // node at 2 QW
+    // node at 2 QW. Two sets of (parent, weight, least run) arrays take
+    // the rows in turn: row y unites in one while the other holds row y-1
+    // settled (every element under its root), where a run i's record is
+    // pmin[pp[QW + i]] and, at the record's first run, its weight
+    // pacc[pp[QW + i]].
     const int EM = 2 * QW + 2;
     const uint32_t VN = 2u * (uint32_t)QW;
     U32 *p = reinterpret_cast<U32 *>(base);
     U32 *acc = p + EM;
     U32 *minrun = acc + EM;
-    U32 *W = minrun + EM;          // [QW + 1] weight of each record, at its first run
-    U32 *Wn = W + QW + 1;
-    U16 *labA = reinterpret_cast<U16 *>(Wn + QW + 1);  // [QW] record (first run) of each run
-    U16 *labB = labA + QW;
+    U32 *pp = minrun + EM;
+    U32 *pacc = pp + EM;
+    U32 *pmin = pacc + EM;
     // row offsets: exclusive scan of the per-row run counts (wave 0)
     if (wave == 0) {
         uint32_t carry = 0;
@@ -246,12 +248,13 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             F.fw[carry + y] = 1;
         }
     }
-    // row 0: each run is its own set (named by itself)
+    // row 0: each run is its own set (named by itself), as settled
     int pcur = (int)F.rcnt[0];
     for (int i = tid; i < pcur; i += kFrontThreads) {
         uint32_t r = F.runs[i];
-        labA[i] = (uint16_t)i;
-        W[i] = run_end(r) - run_start(r) + 1;
+        pp[QW + i] = (uint32_t)(QW + i);
+        pmin[QW + i] = (uint32_t)i;
+        pacc[QW + i] = run_end(r) - run_start(r) + 1;
         F.parent[i] = (uint32_t)i;
     }
     uint32_t pbase = 0;  // row_off of the previous row
@@ -285,21 +288,23 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             rr[j] = nx_r[j];
             ee[j] = nx_e[j];
         }
-        // the previous row's outputs: each run's record (its first run)
-        // and, at a record's first run, its weight
-        for (int i = tid; i < pcur; i += kFrontThreads) {
-            const uint32_t l = labA[i];
-            F.rlab[pbase + i] = l;
-            if (l == (uint32_t)i) F.recw[pbase + i] = W[i];
-        }
         const uint32_t cbase = pbase + (uint32_t)pcur;
         if (y + 1 < QH) prefetch(y + 1);
         const uint32_t *cur = F.runs + (int64_t)y * QW;
         const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
-        // elements: records of row y-1, runs of row y, the virtual node
+        // the previous row's outputs (each run's record, its first run,
+        // and at a record's first run its weight), and the elements: the
+        // records of row y-1 (weight 0 at the other slots, which no link
+        // names), the runs of row y, the virtual node
         for (int x = tid; x < pcur; x += kFrontThreads) {
+            const uint32_t r = pp[QW + x];
+            const uint32_t l = pmin[r];
+            const uint32_t wgt = l == (uint32_t)x ? pacc[r] : 0u;
+            F.rlab[pbase + x] = l;
+            if (l == (uint32_t)x) F.recw[pbase + x] = wgt;
             p[x] = (uint32_t)x;
-            acc[x] = W[x];
+            acc[x] = wgt;
+            minrun[x] = 0xffffffffu;
         }
         bool lastcol = false;  // fast/unionfind.c:93-97, on the thread of the last run
 #pragma unroll
@@ -320,21 +325,21 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             minrun[QW + i] = 0xffffffffu;
             if (i == ncur - 1) lastcol = (int)run_end(r) == QW - 2 && run_white(r) && t1 == 255 && t2 != 255;
         }
-        for (int x = tid; x < pcur; x += kFrontThreads) minrun[x] = 0xffffffffu;
         if (tid == 0) {
             p[VN] = VN;
             acc[VN] = 1;
             minrun[VN] = 0xffffffffu;
         }
         bsync<G>();
+        // a link joins a current run and the record of a previous-row run
 #pragma unroll
         for (int j = 0; j < kPer; j++) {
             const int e = tid + j * kFrontThreads;
-            if (e < ne) fr_unite<G>(p, (uint32_t)QW + (ee[j] >> 16), (uint32_t)labA[ee[j] & 0xffffu]);
+            if (e < ne) fr_unite<G>(p, (uint32_t)QW + (ee[j] >> 16), pmin[pp[QW + (ee[j] & 0xffffu)]]);
         }
         for (int e = kStage + tid; e < ne; e += kFrontThreads) {
             const uint32_t ed = edges[e];
-            fr_unite<G>(p, (uint32_t)QW + (ed >> 16), (uint32_t)labA[ed & 0xffffu]);
+            fr_unite<G>(p, (uint32_t)QW + (ed >> 16), pmin[pp[QW + (ed & 0xffffu)]]);
         }
         if (lastcol) fr_unite<G>(p, (uint32_t)(QW + ncur - 1), VN);
         bsync<G>();
@@ -352,24 +357,21 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
         for (int i = tid; i < ncur; i += kFrontThreads) settle((uint32_t)(QW + i), true);
         if (tid == 0) settle(VN, false);
         bsync<G>();
-        // row y's records: each run's set named by its first run
-        for (int i = tid; i < ncur; i += kFrontThreads) {
-            const uint32_t r = p[QW + i];
-            const uint32_t l = minrun[r];
-            labB[i] = (uint16_t)l;
-            if (l == (uint32_t)i) Wn[i] = acc[r];
-        }
-        bsync<G>();
+        // row y settled: the previous set for row y + 1 (the links of row y
+        // read the other set, which row y + 1 rewrites, before the barrier
+        // after the unions)
         pbase = cbase;
         pcur = ncur;
-        U32 *tw = W; W = Wn; Wn = tw;
-        U16 *tl = labA; labA = labB; labB = tl;
+        U32 *t0 = p; p = pp; pp = t0;
+        U32 *t1a = acc; acc = pacc; pacc = t1a;
+        U32 *t2m = minrun; minrun = pmin; pmin = t2m;
     }
     // the last row's outputs
     for (int i = tid; i < pcur; i += kFrontThreads) {
-        const uint32_t l = labA[i];
+        const uint32_t r = pp[QW + i];
+        const uint32_t l = pmin[r];
         F.rlab[pbase + i] = l;
-        if (l == (uint32_t)i) F.recw[pbase + i] = W[i];
+        if (l == (uint32_t)i) F.recw[pbase + i] = pacc[r];
     }
 }
 

@@ -474,7 +474,9 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         fp.force = ctx->force_scratch;
         Frame F{};
         fp.size = layout(fp, nfam, !on_device, &F, nullptr);
-        if (!F.fscr) max_lds_qw = std::max<int>(max_lds_qw, (int)fp.qw);
+        // (the sizing layout has no base, so its pointers are all null: the
+        // front's memory is decided here as layout() decides it)
+        if (front_scratch_bytes((int)fp.qw, fp.force) == 0) max_lds_qw = std::max<int>(max_lds_qw, (int)fp.qw);
         max_weight = std::max<int64_t>(max_weight, fp.qw * fp.qh + fp.qh);
         fp.off = total;
         total += fp.size;
