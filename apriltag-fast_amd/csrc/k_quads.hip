@@ -1021,7 +1021,10 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
                                                     int l) {
     constexpr int kMaxNm = CornerShape<NT>::kMaxNm, kLdsPts = CornerShape<NT>::kLdsPts;
     constexpr int kMaxima = CornerShape<NT>::kMaxima;
-    constexpr int kShared = 2 * kLdsPts + kMaxima > 4 * kMaxNm * kMaxNm ? 2 * kLdsPts + kMaxima : 4 * kMaxNm * kMaxNm;
+    // errors and filtered errors, and (256 threads: the one-wave shape
+    // sorts the maxima in registers) the maxima values
+    constexpr int kPart1 = 2 * kLdsPts + (NT == 256 ? kMaxima : 0);
+    constexpr int kShared = kPart1 > 4 * kMaxNm * kMaxNm ? kPart1 : 4 * kMaxNm * kMaxNm;
     constexpr int kWaves = NT / 64;
     // The errors, filtered errors and maxima values of a cluster (first
     // part) share LDS with the pair-fit tables (second part).
@@ -1160,7 +1163,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
                 const uint64_t k = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)(pos & 63)) << 32) |
                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)(pos & 63));
                 thr = __longlong_as_double((long long)((k >> 63) ? (k ^ 0x8000000000000000ull) : ~k));
-            } else if (nm <= (uint32_t)kMaxima) {
+            } else if (NT == 256 && nm <= (uint32_t)kMaxima) {
                 // the rank of each value among the maxima (ties by position)
                 double *mv = sh + 2 * kLdsPts;
                 for (uint32_t i = t; i < nm; i += NT) mv[i] = y[midx[i]];
