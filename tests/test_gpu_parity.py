@@ -92,6 +92,24 @@ def test_batch_of_uneven_frames_equals_single(atg, ctx):
     assert sum(len(a["dets"]) for a in alone) > 20
 
 
+def test_large_batch_of_uneven_frames_equals_single(atg, ctx):
+    """40 frames of four sizes in one call (the shapes of calls of more than
+    32 frames: one-wave fit kernels, 128-thread moment blocks, the 448-run
+    replay rows with the 704-run and widest-row launches for the 2448x2048
+    rows) give what each frame gives alone."""
+    frames = [synth_case(atg, 4, s)[0] for s in range(40)]
+    p = default_params()
+    ctx.set_families([("tag36h11", 2)])
+    prm = atg.params_from_dict(p)
+    print("case large batch: 40 frames, 10 each of 640x480 1280x720 1920x1080 2448x2048", flush=True)
+    alone = [ctx.detect(f, prm) for f in frames]
+    batch = ctx.detect_batch(frames, prm)
+    for k, (a, b) in enumerate(zip(alone, batch)):
+        assert a["nquads"] == b["nquads"], k
+        assert same_dets(a["dets"], b["dets"]), k
+    assert sum(len(a["dets"]) for a in alone) > 200
+
+
 def test_repeatable(atg, ctx):
     img, p, fams = synth_case(atg, 2, 1)
     a = gpu_detect(atg, ctx, img, p, fams)
