@@ -70,9 +70,24 @@ __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames, int lds_q
     const int ncur = (int)F.rcnt[y], nprev = (int)F.rcnt[y - 1];
     const uint32_t *cur = F.runs + (int64_t)y * QW;
     const uint32_t *prv = F.runs + (int64_t)(y - 1) * QW;
+    // the first 64 current runs, read while the previous row is copied
+    uint32_t rnext = lane < ncur ? cur[lane] : 0u;
     if constexpr (L) {
         uint32_t *sp = sprv + (threadIdx.x >> 6) * lds_qw;
-        for (int k = lane; k < nprev; k += 64) sp[k] = prv[k];
+        // 8 loads a lane in flight at once
+        for (int k0 = 0; k0 < nprev; k0 += 512) {
+            uint32_t v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int k = k0 + q * 64 + lane;
+                v[q] = k < nprev ? prv[k] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int k = k0 + q * 64 + lane;
+                if (k < nprev) sp[k] = v[q];
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
@@ -86,8 +101,9 @@ __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames, int lds_q
         int a0 = 0, a1 = -1, klo = nprev;
         bool white = false;
         uint32_t cnt = 0;
+        const uint32_t r = rnext;  // the next 64 read a step ahead
+        if (i + 64 < ncur) rnext = cur[i + 64];
         if (i < ncur) {
-            uint32_t r = cur[i];
             a0 = (int)run_start(r);
             a1 = (int)run_end(r);
             white = run_white(r) != 0;
