@@ -43,7 +43,9 @@ constexpr int kResultDets = kMaxDetsPerFrame;  // detection capacity of a frame'
 constexpr int kInlineDets = 32;            // read back with the counters
 constexpr size_t kCountersBytes = 64;
 constexpr size_t kResultBlock = kCountersBytes + (size_t)kResultDets * sizeof(at_detection);
-constexpr Tune kDefaultTune = {1, 4, 1024, 1024};
+// fit_blocks 2048: the one-wave fit kernels that fit more than 16 waves a CU
+// (the small-cluster angle sort, edge refinement) get them all resident
+constexpr Tune kDefaultTune = {1, 4, 1024, 2048};
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
