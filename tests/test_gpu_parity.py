@@ -60,6 +60,41 @@ def test_stages_match_oracle(atg, ctx, oracle, cfg, seed):
         assert a.shape == b.shape and np.array_equal(a, b), k
 
 
+def _blocky(w, h, b, seed):
+    r = np.random.default_rng(seed)
+    small = (r.random(((h + b - 1) // b, (w + b - 1) // b)) > 0.5).astype(np.uint8) * 255
+    return np.ascontiguousarray(np.kron(small, np.ones((b, b), np.uint8))[:h, :w])
+
+
+def _stripes(w, h, seed):
+    r = np.random.default_rng(seed)
+    img = np.zeros((h, w), np.uint8)
+    for k in range(4):
+        img[:, k::16] = 255
+    n = (r.random((h // 4, w // 4)) > 0.97).astype(np.uint8) * 255
+    return np.ascontiguousarray(img ^ np.kron(n, np.ones((4, 4), np.uint8)))
+
+
+@pytest.mark.parametrize("name", ["wide", "tall", "front_scratch"])
+def test_extreme_shapes_match_oracle(atg, ctx, oracle, name):
+    """Shapes at the union-find's limits: rows of over 1024 runs (past the
+    front's register stage and the 704-run replay launch: the widest-row
+    launch), 2048 quad rows of stripes spanning all of them (six root-jump
+    passes), and a 3600-wide quad image whose front state is too large for
+    LDS (HBM scratch without the debug switch)."""
+    img, dec = {"wide": (_blocky(4096, 128, 2, 1), 2.0), "tall": (_stripes(256, 4096, 2), 2.0),
+                "front_scratch": (_blocky(3600, 96, 2, 3), 1.0)}[name]
+    p, fams = default_params(quad_decimate=dec), [("tag36h11", 2)]
+    print("case", name, img.shape, flush=True)
+    o = oracle.detect(img, p, fams, stages=True)
+    g = gpu_detect(atg, ctx, img, p, fams)
+    st = ctx.stages(0)
+    for k in STAGE_KEYS:
+        a, b = o["stages"][k], st[k]
+        assert a.shape == b.shape and np.array_equal(a, b), k
+    assert same_dets(o["dets"], g["dets"]) and o["nquads"] == g["nquads"]
+
+
 @pytest.mark.parametrize("cfg,seed", [(1, 1), (4, 3)])
 def test_union_find_scratch_path_matches_oracle(atg, ctx, oracle, cfg, seed):
     """The union-find replay with its state in HBM scratch (the path of
