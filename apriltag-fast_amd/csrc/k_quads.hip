@@ -576,8 +576,11 @@ __device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue 
 
 // One block: where each frame's items start in list l (counter `which` of
 // each frame), and the list's size.
-__global__ __launch_bounds__(256) void k_work_offsets(const Frame *frames, int nframes, WorkQueue *Q, int l,
-                                                     int which) {
+// Block b builds list (lw >> 16 b) & 0xff from counter (lw >> 16 b + 8) & 0xff
+// of every frame (one launch for up to four lists).
+__global__ __launch_bounds__(256) void k_work_offsets(const Frame *frames, int nframes, WorkQueue *Q, uint64_t lw) {
+    const uint32_t pair = (uint32_t)(lw >> (16 * blockIdx.x)) & 0xffffu;
+    const int l = (int)(pair & 0xffu), which = (int)(pair >> 8);
     uint32_t carry = 0;
     for (int f0 = 0; f0 < nframes; f0 += 256) {
         const int f = f0 + (int)threadIdx.x;
@@ -594,7 +597,8 @@ __global__ __launch_bounds__(256) void k_work_offsets(const Frame *frames, int n
 }
 
 void launch_work_offsets(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q, int l, int which) {
-    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, l, which);
+    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q,
+                       (uint64_t)(uint32_t)(l | which << 8));
 }
 
 // ------------------------------------------------------------ angle sort
@@ -1473,9 +1477,10 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     else
         hipLaunchKernelGGL(k_fit_prep<256>, dim3(256, nframes), dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 4, (int)kNHuge);
-    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 0, (int)kNLarge);
-    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 1, (int)kNSmall);
+    // the huge, large and small lists, a block each
+    const uint64_t lw = (uint64_t)(4 | kNHuge << 8) | (uint64_t)(0 | kNLarge << 8) << 16 |
+                        (uint64_t)(1 | kNSmall << 8) << 32;
+    hipLaunchKernelGGL(k_work_offsets, dim3(3), dim3(256), 0, st, dframes, nframes, Q, lw);
     mk->mark(st, "fit_prep");
     int parity = 0;
     if (sort_cap) {
@@ -1505,7 +1510,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st,
                            dframes, dparams, Q, 1);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_work_offsets, dim3(1), dim3(256), 0, st, dframes, nframes, Q, 2, (int)kNQuads);
+    launch_work_offsets(st, dframes, nframes, Q, 2, (int)kNQuads);
     mk->mark(st, "corners");
 }
 
