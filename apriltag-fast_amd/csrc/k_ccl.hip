@@ -842,7 +842,7 @@ __global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap,
 // ---------------------------------------------------------------------
 // Step 3: the actual roots. rroot[slot] names a node or a record of the
 // previous row, so the roots resolve by pointer jumping over the slots
-// (k_root_jump, one pass per quadrupling of the resolved distance, every row
+// (k_root_jump, one pass per fivefold growth of the resolved distance, every row
 // at once); then k_lost gives each record root that lost in the next row
 // its winner as parent, and each set that ended its final weight.
 
@@ -850,7 +850,7 @@ __global__ __launch_bounds__(64) void k_replay_hbm(const Frame *frames, int cap,
 // links up its chain when the pass began, it is at least (kRootHops + 1) D
 // up when the pass ends (each value read spans D more), so P passes with
 // (kRootHops + 1)^P >= qh reach every root (a chain climbs a row a link).
-constexpr int kRootHops = 3;
+constexpr int kRootHops = 4;
 __global__ __launch_bounds__(256) void k_root_jump(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63;

@@ -90,9 +90,9 @@ def kernel_bytes(n):
     W, H, qw, qh = n["W"], n["H"], n["qw"], n["qh"]
     R, N, E, P, K, Q = n["R"], n["N"], n["E"], n["P"], n["K"], n["Q"]
     px, tiles = qw * qh, (qw // 4) * (qh // 4)
-    lg, d = 0, 1  # root-jump passes: each follows 3 hops, the resolved distance x4
+    lg, d = 0, 1  # root-jump passes: each follows 4 hops, the resolved distance x5
     while d < qh:
-        d, lg = d * 4, lg + 1
+        d, lg = d * 5, lg + 1
     return {
         "decimate": W * H + px + 2 * tiles,
         "threshold_rle": px + 2 * tiles + px + 4 * R + 4 * qh,

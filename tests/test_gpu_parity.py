@@ -79,7 +79,7 @@ def _stripes(w, h, seed):
 def test_extreme_shapes_match_oracle(atg, ctx, oracle, name):
     """Shapes at the union-find's limits: rows of over 1024 runs (past the
     front's register stage and the 704-run replay launch: the widest-row
-    launch), 2048 quad rows of stripes spanning all of them (six root-jump
+    launch), 2048 quad rows of stripes spanning all of them (five root-jump
     passes), and a 3600-wide quad image whose front state is too large for
     LDS (HBM scratch without the debug switch)."""
     img, dec = {"wide": (_blocky(4096, 128, 2, 1), 2.0), "tall": (_stripes(256, 4096, 2), 2.0),
