@@ -139,6 +139,8 @@ def lib():
     L.atg_debug_copy.restype = C.c_int
     L.atg_debug_force_scratch.argtypes = [vp, C.c_int]
     L.atg_debug_force_scratch.restype = C.c_int
+    L.atg_debug_finish.argtypes = [vp, vp, C.POINTER(i64)]
+    L.atg_debug_finish.restype = C.c_int
     L.atg_get_tune.argtypes = [vp, C.POINTER(Tune)]
     L.atg_get_tune.restype = C.c_int
     L.atg_set_tune.argtypes = [vp, C.POINTER(Tune)]
@@ -343,6 +345,14 @@ class Context:
         best = min(times)
         return 2 * nbytes / (best * 1e-3) / 1e9, times
 
+    def finish(self, dets):
+        """The device's reconcile and stable id sort on a DET_DTYPE array."""
+        d = np.ascontiguousarray(dets, DET_DTYPE).copy()
+        n = C.c_int64(len(d))
+        self._check(self.L.atg_debug_finish(self.h, d.ctypes.data if len(d) else None, C.byref(n)),
+                    "atg_debug_finish")
+        return d[:n.value].copy()
+
     def force_scratch(self, on=True):
         """Keep the union-find replay state in HBM scratch (the wide-frame path)."""
         self._check(self.L.atg_debug_force_scratch(self.h, 1 if on else 0), "atg_debug_force_scratch")
@@ -499,11 +509,15 @@ def synth_stream_frame(spec, seed, step, out=None):
     return img
 
 
-def synth_frame(config, seed, width=None, height=None, ntags=None):
-    """A seeded synthetic frame (uint8 HxW) and its tags (family, id)."""
+def synth_frame(config, seed, width=None, height=None, ntags=None, family=None):
+    """A seeded synthetic frame (uint8 HxW) and its tags (family, id);
+    family: the index of the built-in family its tags come from (the
+    configuration's own by default)."""
     L = lib()
     s = SynthSpec()
     L.atg_synth_default(C.byref(s), config)
+    if family is not None:
+        s.family_a = family
     if width:
         s.width = width
     if height:

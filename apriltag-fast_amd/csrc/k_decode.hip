@@ -784,19 +784,10 @@ __device__ bool g2_overlaps(const double pa[4][2], const double pb[4][2]) {
 }
 __device__ __forceinline__ int prefer(int p, double a, double b) { return p != 0 ? p : a < b ? -1 : b < a ? 1 : 0; }
 
-__global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevParams *P) {
-    const Frame F = frames[blockIdx.y];
-    if (threadIdx.x != 0) return;
-    const uint32_t nq = F.counters[kNQuads];
-    const int nfam = P->nfam;
-    at_detection *d = F.dets;
-    int64_t n = 0;
-    for (uint32_t k = 0; k < nq * (uint32_t)nfam; k++) {
-        if (!F.dflag[k]) continue;
-        if (n < kMaxDetsPerFrame) d[n++] = F.dtmp[k];
-        else atomicOr(F.counters + kError, 16u);
-    }
-    // reconcile (fast/decode.c:860-891)
+// Reconcile (fast/decode.c:860-891) then the stable sort by id
+// (fast/detector.c:164-179) of a frame's detection list, in place by one
+// lane; returns the count kept.
+__device__ int64_t finish_dets(at_detection *d, int64_t n) {
     int64_t i0 = 0;
     while (i0 < n) {
         bool removed0 = false;
@@ -825,14 +816,37 @@ __global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevPar
         }
         if (!removed0) i0++;
     }
-    // stable sort by id
     for (int64_t i = 1; i < n; i++) {
         at_detection v = d[i];
         int64_t j = i;
         while (j > 0 && d[j - 1].id > v.id) { d[j] = d[j - 1]; j--; }
         d[j] = v;
     }
-    F.counters[kNDets] = (uint32_t)n;
+    return n;
+}
+
+__global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevParams *P) {
+    const Frame F = frames[blockIdx.y];
+    if (threadIdx.x != 0) return;
+    const uint32_t nq = F.counters[kNQuads];
+    const int nfam = P->nfam;
+    at_detection *d = F.dets;
+    int64_t n = 0;
+    for (uint32_t k = 0; k < nq * (uint32_t)nfam; k++) {
+        if (!F.dflag[k]) continue;
+        if (n < kMaxDetsPerFrame) d[n++] = F.dtmp[k];
+        else atomicOr(F.counters + kError, 16u);
+    }
+    F.counters[kNDets] = (uint32_t)finish_dets(d, n);
+}
+
+// The finishing step of one detection list given by the caller.
+__global__ __launch_bounds__(64) void k_debug_finish(at_detection *d, int64_t n, int64_t *nout) {
+    if (threadIdx.x == 0) *nout = finish_dets(d, n);
+}
+
+void launch_debug_finish(hipStream_t st, at_detection *d, int64_t n, int64_t *nout) {
+    hipLaunchKernelGGL(k_debug_finish, dim3(1), dim3(64), 0, st, d, n, nout);
 }
 
 __global__ void k_init(const Frame *frames, WorkQueue *Q) {

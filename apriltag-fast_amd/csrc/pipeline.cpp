@@ -33,6 +33,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
 int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          bool refine, WorkQueue *Q, Marks *mk);
+void launch_debug_finish(hipStream_t st, at_detection *d, int64_t n, int64_t *nout);
 }  // namespace atg
 
 using namespace atg;
@@ -959,6 +960,29 @@ extern "C" int atg_debug_copy(atg_ctx *ctx, void *dst, const void *src, int64_t 
     HIPCHK(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
     if (ms) *ms = f;
     return ATG_OK;
+}
+
+extern "C" int atg_debug_finish(atg_ctx *ctx, at_detection *dets, int64_t *n) {
+    if (!ctx || !n || *n < 0 || (*n > 0 && !dets)) return ATG_ERR_ARG;
+    if (*n == 0) return ATG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(at_detection) * (size_t)*n;
+    uint8_t *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, bytes + 256));
+    int64_t *dn = reinterpret_cast<int64_t *>(buf + align256(bytes));
+    int rc = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(buf, dets, bytes, hipMemcpyHostToDevice, ctx->stream));
+        launch_debug_finish(ctx->stream, reinterpret_cast<at_detection *>(buf), *n, dn);
+        HIPCHK(hipGetLastError());
+        int64_t kept = 0;
+        HIPCHK(hipMemcpyAsync(&kept, dn, sizeof kept, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(dets, buf, sizeof(at_detection) * (size_t)kept, hipMemcpyDeviceToHost));
+        *n = kept;
+        return ATG_OK;
+    }();
+    hipFree(buf);
+    return rc;
 }
 
 extern "C" int atg_debug_force_scratch(atg_ctx *ctx, int on) {

@@ -145,6 +145,11 @@ int atg_set_graphs(atg_ctx *ctx, int on);
  * that HBM traffic counters are scaled by. */
 int atg_debug_copy(atg_ctx *ctx, void *dst, const void *src, int64_t bytes, double *ms);
 
+/* The device's last step of a frame on a caller's detection list, in
+ * place: reconcile (fast/decode.c:860-891), then the stable sort by id
+ * (fast/detector.c:164-179); *n becomes the count kept. */
+int atg_debug_finish(atg_ctx *ctx, at_detection *dets, int64_t *n);
+
 /* on != 0: the following calls take the paths of frames too large for
  * LDS (the union-find replay's state in HBM scratch, the angle sort of all
  * points at once in HBM), so that tests can check them on any frame. */

@@ -1090,7 +1090,7 @@ static bool g2d_overlaps(const double pa[4][2], const double pb[4][2]) {
 static int pref(int p, double a, double b) { return p != 0 ? p : a < b ? -1 : b < a ? 1 : 0; }
 
 /* fast/decode.c:860-891 */
-static void reconcile(orc_detection *d, int64_t *n) {
+void orc_reconcile(orc_detection *d, int64_t *n) {
     int64_t i0 = 0;
     while (i0 < *n) {
         bool removed0 = false;
@@ -1247,7 +1247,8 @@ int orc_detect(const orc_params *P, const orc_family *fams, const int32_t *bits,
             }
         }
     }
-    reconcile(dets, &nd);
+    res->nraw = nd;
+    orc_reconcile(dets, &nd);
     /* stable sort by id (fast/detector.c:164-179) */
     for (int64_t i = 1; i < nd; i++) {
         orc_detection v = dets[i];

@@ -61,6 +61,7 @@ typedef struct {
     orc_detection *dets;
     int64_t ndets;
     int64_t nquads;
+    int64_t nraw; /* detections before reconcile */
 } orc_result;
 
 /* Detect tags. `bits` holds bits_corrected per family. stages may be NULL.
@@ -71,5 +72,8 @@ int orc_detect(const orc_params *p, const orc_family *fams, const int32_t *bits,
 void orc_result_free(orc_result *r);
 void orc_stages_free(orc_stages *s);
 void orc_default_params(orc_params *p);
+/* Removes overlapping detections of the same tag in place, keeping the
+ * better one (fast/decode.c:860-891). */
+void orc_reconcile(orc_detection *d, int64_t *n);
 
 #endif

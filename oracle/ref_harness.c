@@ -185,3 +185,55 @@ int ref_stages(at_detector *td, const uint8_t *pixels, int64_t width, int64_t he
 int64_t ref_sizeof_detector(void) { return (int64_t)sizeof(at_detector); }
 int64_t ref_sizeof_detection(void) { return (int64_t)sizeof(at_detection); }
 int64_t ref_sizeof_family(void) { return (int64_t)sizeof(at_family); }
+
+/* The reference's reconcile (fast/decode.c:860-891) on a caller's list. */
+void ref_reconcile(at_detection *dets, int64_t *ndets) { reconcile(dets, ndets); }
+
+/* The detections of at_detect's single-thread path before reconcile and
+ * the sort (fast/detector.c:283-327): decimate, blur, quad_thresh_single,
+ * scale, decode_quads. Copies up to cap of them into out; returns their
+ * count, or -1 when at_detect would return early. */
+int64_t ref_raw_detections(at_detector *td, const uint8_t *pixels, int64_t width, int64_t height, int64_t stride,
+                           at_detection *out, int64_t cap) {
+    struct at_internal *in = td->in;
+    if (in->nfamilies == 0 || width < 8 || height < 8) return -1;
+    int32_t saved_threads = td->nthreads;
+    bool saved_debug = td->debug;
+    td->nthreads = 1;
+    td->debug = false;
+    at_detect(td, pixels, width, height, stride); /* sizes the arenas */
+    at_ctx *cx = &in->mcx;
+    int64_t qw = width, qh = height;
+    if (td->quad_decimate > 1.0f) decimated_size(width, height, td->quad_decimate, &qw, &qh);
+    in->icx.sp = 1;
+    in->buf_qim = STK_TAKE(&in->icx, uint8_t);
+    in->buf_t = STK_TAKE(&in->icx, uint8_t);
+    in->buf_uf = STK_TAKE(&in->icx, UfNode);
+    at_mark mk = stk_enter(cx);
+    uint8_t *qim = in->buf_qim;
+    if (td->quad_decimate > 1.0f)
+        decimate_rows(pixels, stride, td->quad_decimate, qim, qw, 0, qh);
+    else
+        for (int64_t y = 0; y < qh; y++) memcpy(qim + y * qw, pixels + y * stride, (size_t)qw);
+    bool blurred = blur_or_sharpen(cx, qim, qw, qh, td->quad_sigma);
+    ImView view = {pixels, stride, width, height};
+    if (blurred && !(td->quad_decimate > 1.0f)) view = (ImView){qim, width, width, height};
+    vec_quad quads;
+    svec_take(cx, quads);
+    quad_thresh_single(td, qim, qw, qh, &quads);
+    if (td->quad_decimate > 1.0f)
+        for (int64_t i = 0; i < quads.len; i++)
+            for (int j = 0; j < 4; j++) {
+                quads.data[i].p[j][0] *= td->quad_decimate;
+                quads.data[i].p[j][1] *= td->quad_decimate;
+            }
+    in->dets.len = 0;
+    decode_quads(td, quads.data, quads.len, &view);
+    int64_t n = in->dets.len;
+    memcpy(out, in->dets.data, (size_t)(n < cap ? n : cap) * sizeof(at_detection));
+    stk_leave(cx, mk);
+    td->nthreads = saved_threads;
+    td->debug = saved_debug;
+    at_detect(td, pixels, width, height, stride);
+    return n;
+}

@@ -95,7 +95,8 @@ class OrcParams(C.Structure):
 
 
 class OrcResult(C.Structure):
-    _fields_ = [("dets", C.POINTER(Detection)), ("ndets", C.c_int64), ("nquads", C.c_int64)]
+    _fields_ = [("dets", C.POINTER(Detection)), ("ndets", C.c_int64), ("nquads", C.c_int64),
+                ("nraw", C.c_int64)]
 
 
 class Stages(C.Structure):
@@ -244,7 +245,12 @@ class Oracle:
                                         C.POINTER(OrcResult), C.POINTER(Stages)]
         self.lib.orc_result_free.argtypes = [C.POINTER(OrcResult)]
         self.lib.orc_stages_free.argtypes = [C.POINTER(Stages)]
+        self.lib.orc_reconcile.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         self.fams = load_families()
+
+    def reconcile(self, dets):
+        """The oracle's reconcile (fast/decode.c:860-891) on a DET_DTYPE array."""
+        return _reconcile_with(self.lib.orc_reconcile, dets)
 
     def family_struct(self, name):
         f = self.fams[name]
@@ -271,13 +277,13 @@ class Oracle:
                                  img.shape[0], img.strides[0], C.byref(res),
                                  C.byref(st) if stages else None)
         dets = dets_to_array(res.dets, res.ndets)
-        nq = res.nquads
+        nq, nraw = res.nquads, res.nraw
         self.lib.orc_result_free(C.byref(res))
         sd = None
         if stages and rc == 0:
             sd = stages_to_dict(st)
             self.lib.orc_stages_free(C.byref(st))
-        return dict(rc=rc, dets=dets, nquads=nq, stages=sd)
+        return dict(rc=rc, dets=dets, nquads=nq, nraw=nraw, stages=sd)
 
 
 # --------------------------------------------------------------- reference build
@@ -299,6 +305,27 @@ class Reference:
         L.ref_stages.argtypes = [C.POINTER(Detector), C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                  C.POINTER(Stages)]
         L.ref_stages_free.argtypes = [C.POINTER(Stages)]
+        L.ref_reconcile.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.ref_raw_detections.restype = C.c_int64
+        L.ref_raw_detections.argtypes = [C.POINTER(Detector), C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
+                                         C.c_void_p, C.c_int64]
+
+    def reconcile(self, dets):
+        """The reference's own reconcile on a DET_DTYPE array."""
+        return _reconcile_with(self.lib.ref_reconcile, dets)
+
+    def raw_detections(self, img, params, families, cap=1 << 16):
+        """The reference's detections before reconcile and the id sort."""
+        img = np.ascontiguousarray(img, np.uint8)
+        td = self.detector(params, families)
+        try:
+            buf = np.zeros(cap, DET_DTYPE)
+            n = self.lib.ref_raw_detections(td, img.ctypes.data, img.shape[1], img.shape[0], img.strides[0],
+                                            buf.ctypes.data, cap)
+            assert n <= cap
+            return buf[:max(n, 0)].copy()
+        finally:
+            self.lib.at_detector_destroy(td)
 
     def detector(self, params, families):
         td = self.lib.at_detector_create()
@@ -332,6 +359,14 @@ class Reference:
             self.lib.at_detector_destroy(td)
 
 
+def _reconcile_with(fn, dets):
+    d = np.ascontiguousarray(dets, DET_DTYPE).copy()
+    n = C.c_int64(len(d))
+    if len(d):
+        fn(d.ctypes.data, C.byref(n))
+    return d[:n.value].copy()
+
+
 def have_ref():
     return REF_SO.exists()
 
@@ -360,7 +395,56 @@ def frame_hash(img):
     return int.from_bytes(hashlib.blake2b(np.ascontiguousarray(img).tobytes(), digest_size=8).digest(), "little")
 
 
-SYNTH_CASES = [(1, s) for s in range(8)] + [(2, s) for s in range(3)] + [(4, s) for s in range(2)] + [(3, 0), (3, 1)]
+SYNTH_CASES = ([(1, s) for s in range(8)] + [(2, s) for s in range(3)] + [(4, s) for s in range(4)] +
+               [(3, 0), (3, 1)])
+
+# Named fixture cases beyond the configurations' own: name -> (config,
+# seed, family index of the rendered tags or None, parameter overrides,
+# families). "rc_*" are frames on which the reference's reconcile
+# (fast/decode.c:860-891) removes detections; "fam_*" render and detect
+# each built-in family.
+NAMED_CASES = {
+    "rc_c3_0_16h5b3": (3, 0, None, dict(quad_decimate=1.0), [("tag16h5", 3)]),
+    "rc_c3_1_16h5b2": (3, 1, None, dict(quad_decimate=1.0), [("tag16h5", 2)]),
+    "rc_c3_2_16h5b3": (3, 2, None, dict(quad_decimate=1.0), [("tag16h5", 3)]),
+    "rc_c3_2_16h5b3_d2": (3, 2, None, dict(quad_decimate=2.0), [("tag16h5", 3)]),
+}
+for _i, _name in enumerate(FAMILY_NAMES):
+    for _cfg in (1, 2):
+        NAMED_CASES["fam_%s_c%d" % (_name, _cfg)] = (_cfg, 0, _i, {}, [(_name, 2)])
+RECONCILE_CASES = [k for k in NAMED_CASES if k.startswith("rc_")]
+FAMILY_CASES = [k for k in NAMED_CASES if k.startswith("fam_")]
+
+# Cases whose stage arrays the fixtures pin by digest (the reference
+# build's stage dump): production sizes and the heavy-clustering frames.
+STAGE_DIGEST_CASES = ["c2_0", "c2_1", "c2_2", "c4_2", "c4_3", "c3_0", "c3_1"]
+DIGEST_KEYS = ["qim", "t", "runs", "row_off", "rep", "csize", "ok", "cluster_ids", "ccount", "pts", "quads"]
+
+# The uneven 64-frame batch: 16 frames each of four sizes, seeds 100..163.
+BATCH64_SIZES = [(640, 480), (1280, 720), (1920, 1080), (2448, 2048)]
+
+
+def batch64_frames(atg):
+    return [atg.synth_frame(4, 100 + i, width=BATCH64_SIZES[i % 4][0], height=BATCH64_SIZES[i % 4][1])[0]
+            for i in range(64)]
+
+
+def digest(a):
+    """A 64-bit blake2b digest of an array's bytes (fixtures of large outputs)."""
+    import hashlib
+    a = np.ascontiguousarray(a)
+    return int.from_bytes(hashlib.blake2b(a.tobytes(), digest_size=8).digest(), "little")
+
+
+def named_case(atg, name):
+    """The frame, parameters and families of a case: "c<cfg>_<seed>" or a
+    NAMED_CASES name."""
+    if name in NAMED_CASES:
+        cfg, seed, fam, over, fams = NAMED_CASES[name]
+        img, _ = atg.synth_frame(cfg, seed, family=fam)
+        return img, default_params(**over), fams
+    cfg, seed = (int(x) for x in name[1:].split("_"))
+    return synth_case(atg, cfg, seed)
 
 
 def synth_case(atg, cfg, seed):

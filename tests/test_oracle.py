@@ -12,9 +12,10 @@
 import numpy as np
 import pytest
 
-from atg_testlib import (GOLDEN, Oracle, Reference, default_params, det_rows, frame_hash, golden_cases,
-                         have_ref, load_atg, parse_case_args, parse_hex, read_pnm, ref_images, same_dets,
-                         SYNTH_CASES, synth_case)
+from atg_testlib import (DET_DTYPE, DIGEST_KEYS, GOLDEN, NAMED_CASES, RECONCILE_CASES, STAGE_DIGEST_CASES,
+                         Oracle, Reference, batch64_frames, default_params, det_rows, digest, frame_hash,
+                         golden_cases, have_ref, load_atg, named_case, parse_case_args, parse_hex, read_pnm,
+                         ref_images, same_dets, SYNTH_CASES, synth_case)
 
 CASES = golden_cases()
 
@@ -55,16 +56,124 @@ def test_oracle_stages_match_reference_build(orc, imgs, name, args):
         assert same_dets(o["dets"], r["dets"])
 
 
-def test_oracle_matches_synth_fixtures(orc):
+ALL_CASES = ["c%d_%d" % c for c in SYNTH_CASES] + list(NAMED_CASES)
+
+
+@pytest.fixture(scope="module")
+def fx():
+    return np.load(GOLDEN / "synth_expected.npz")
+
+
+@pytest.mark.parametrize("tag", ALL_CASES)
+def test_oracle_matches_reference_fixtures(orc, fx, tag):
+    """Detections, quad count and the count before reconcile equal the
+    reference build's on every seeded case (configurations, reconcile
+    frames, every family)."""
     atg = load_atg()
-    fx = np.load(GOLDEN / "synth_expected.npz")
-    for cfg, seed in SYNTH_CASES:
-        tag = "c%d_%d" % (cfg, seed)
-        img, p, fams = synth_case(atg, cfg, seed)
-        assert int(fx[tag + "_hash"]) == frame_hash(img), tag
-        o = orc.detect(img, p, fams)
-        assert o["dets"].tobytes() == fx[tag + "_dets"].tobytes(), tag
-        assert o["nquads"] == int(fx[tag + "_nquads"]), tag
+    img, p, fams = named_case(atg, tag)
+    assert int(fx[tag + "_hash"]) == frame_hash(img), tag
+    o = orc.detect(img, p, fams)
+    assert o["dets"].tobytes() == fx[tag + "_dets"].tobytes(), tag
+    assert o["nquads"] == int(fx[tag + "_nquads"]), tag
+    assert o["nraw"] == int(fx[tag + "_nraw"]), tag
+
+
+def test_reconcile_cases_remove_detections(fx):
+    """The reconcile fixtures are frames on which the reference removes
+    overlapping detections of one tag (fast/decode.c:860-891)."""
+    removed = [int(fx[t + "_nraw"]) - len(fx[t + "_dets"]) for t in RECONCILE_CASES]
+    assert all(r > 0 for r in removed), removed
+
+
+@pytest.mark.parametrize("tag", STAGE_DIGEST_CASES)
+def test_oracle_stage_digests_match_reference(orc, fx, tag):
+    """Every stage array (quad image, threshold, runs, union-find roots and
+    sizes, ok flags, cluster ids and order, points, quads) of production-
+    size and heavy-clustering frames equals the reference build's dump."""
+    atg = load_atg()
+    img, p, fams = named_case(atg, tag)
+    st = orc.detect(img, p, fams, stages=True)["stages"]
+    for k in DIGEST_KEYS:
+        assert digest(st[k]) == int(fx[tag + "_sd_" + k]), (tag, k)
+
+
+def test_oracle_batch64_matches_reference(orc, fx):
+    """The 64 uneven frames (16 each of 640x480 .. 2448x2048) of the
+    batching workload: each frame's detections equal the reference's."""
+    atg = load_atg()
+    frames = batch64_frames(atg)
+    p, fams = default_params(), [("tag36h11", 2)]
+    for i, f in enumerate(frames):
+        assert frame_hash(f) == int(fx["b64_hash"][i])
+        o = orc.detect(f, p, fams)
+        assert len(o["dets"]) == int(fx["b64_ndets"][i]), i
+        assert o["nquads"] == int(fx["b64_nquads"][i]), i
+        assert digest(o["dets"]) == int(fx["b64_dets"][i]), i
+
+
+def reconcile_lists():
+    z = np.load(GOLDEN / "reconcile_cases.npz")
+    d, off, keep = z["dets"], z["offsets"], z["keep"]
+    return [(d[off[i]:off[i + 1]], keep[off[i]:off[i + 1]]) for i in range(len(off) - 1)]
+
+
+def test_oracle_reconcile_matches_reference_lists(orc):
+    """The oracle's reconcile keeps what the reference's keeps on 400
+    random lists (overlaps, nesting, shared corners, ties, both windings)."""
+    removed = 0
+    for k, (d, keep) in enumerate(reconcile_lists()):
+        got = orc.reconcile(d)
+        assert got.tobytes() == d[keep].tobytes(), k
+        removed += len(d) - len(got)
+    assert removed > 100
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference build (oracle/_ref) not present")
+def test_reference_reconcile_reproduces_fixture():
+    ref = Reference()
+    for k, (d, keep) in enumerate(reconcile_lists()[:50]):
+        assert ref.reconcile(d).tobytes() == d[keep].tobytes(), k
+
+
+def _square(cx, cy, s, rot=0.0, cw=False):
+    a = rot + np.arange(4) * np.pi / 2
+    pts = np.stack([cx + s * np.cos(a), cy + s * np.sin(a)], 1)
+    return (pts[::-1] if cw else pts).reshape(8)
+
+
+def _dets(*rows):
+    d = np.zeros(len(rows), DET_DTYPE)
+    for i, (tid, fam, ham, margin, p) in enumerate(rows):
+        d[i]["id"], d[i]["family"], d[i]["hamming"], d[i]["margin"], d[i]["p"] = tid, fam, ham, margin, p
+    return d
+
+
+def test_reconcile_hand_cases(orc):
+    """Cases whose outcome the rule fixes (of two overlapping detections of
+    one tag and family, the one with fewer corrected bits stays, then the
+    one with the larger margin; detections of other tags or families, or
+    apart, all stay), for the geometry that decides overlap: crossing
+    edges, one quad inside the other, both windings, disjoint quads."""
+    big, small, far = _square(50, 50, 20), _square(50, 50, 5, 0.3), _square(150, 50, 10)
+    shifted = _square(60, 55, 20, 0.2, cw=True)
+    # crossing edges: the lower hamming stays, wherever it is in the list
+    for a, b in ((0, 1), (1, 0)):
+        d = _dets((7, 0, a, 50.0, big), (7, 0, b, 50.0, shifted))
+        got = orc.reconcile(d)
+        assert len(got) == 1 and got[0]["hamming"] == 0
+    # nested (no edge crosses): still one tag; equal hamming, larger margin stays
+    d = _dets((7, 0, 1, 30.0, big), (7, 0, 1, 60.0, small))
+    got = orc.reconcile(d)
+    assert len(got) == 1 and got[0]["margin"] == 60.0
+    d = _dets((7, 0, 1, 60.0, small), (7, 0, 1, 30.0, big))
+    assert orc.reconcile(d)[0]["margin"] == 60.0
+    # apart, other id, other family: all stay, in order
+    d = _dets((7, 0, 0, 50.0, big), (7, 0, 0, 50.0, far), (8, 0, 0, 50.0, shifted), (7, 1, 0, 50.0, small))
+    assert orc.reconcile(d).tobytes() == d.tobytes()
+    # a chain of three over one tag: the best of them stays
+    d = _dets((3, 0, 1, 40.0, big), (3, 0, 0, 20.0, shifted), (3, 0, 2, 90.0, small))
+    got = orc.reconcile(d)
+    assert len(got) == 1 and got[0]["hamming"] == 0
 
 
 def _flood_partition(t):
