@@ -433,12 +433,20 @@ class Server:
                                       C.byref(self.h))
         if rc != 0:
             raise RuntimeError("atg_server_create failed (%d)" % rc)
-        self._keep = []
+        # frames the server borrows, by the id passed to it as the frame's
+        # tag: (caller's tag, array), released when the frame's record is polled
+        self._keep = {}
+        self._next_id = 0
 
     def _check(self, rc, what):
         if rc < 0:
             raise RuntimeError("%s: %s (%d)" % (what, self.L.atg_server_error(self.h).decode(), rc))
         return rc
+
+    def _hold(self, img, tag):
+        self._next_id += 1
+        self._keep[self._next_id] = (tag, img)
+        return self._next_id
 
     def add_stream(self, width, height):
         sid = C.c_int32()
@@ -449,17 +457,25 @@ class Server:
         return self.L.atg_server_now_ns(self.h)
 
     def submit(self, stream, img, tag=0):
-        """img: a uint8 HxW array that stays alive until its result is polled."""
-        self._keep.append(img)
-        return self.L.atg_server_submit(self.h, stream, img.ctypes.data, img.strides[0], tag)
+        """img: a uint8 HxW array, held until its result is polled."""
+        fid = self._hold(img, tag)
+        rc = self.L.atg_server_submit(self.h, stream, img.ctypes.data, img.strides[0], fid)
+        if rc != 0:
+            del self._keep[fid]  # refused or dropped: the server holds nothing
+        return rc
 
     def feed(self, arrivals):
-        """arrivals: list of (stream, at_ns, img, tag); imgs stay alive until polled."""
+        """arrivals: list of (stream, at_ns, img, tag); each img is held until
+        its record (or, if dropped, its ATG_ERR_LIMIT record) is polled."""
         arr = (Arrival * max(1, len(arrivals)))()
-        for i, (sid, at, img, tag) in enumerate(arrivals):
-            arr[i] = Arrival(sid, int(at), img.ctypes.data, img.strides[0], tag)
-            self._keep.append(img)
-        self._check(self.L.atg_server_feed(self.h, arr, len(arrivals)), "atg_server_feed")
+        ids = [self._hold(img, tag) for (_, _, img, tag) in arrivals]
+        for i, (sid, at, img, _) in enumerate(arrivals):
+            arr[i] = Arrival(sid, int(at), img.ctypes.data, img.strides[0], ids[i])
+        rc = self.L.atg_server_feed(self.h, arr, len(arrivals))
+        if rc < 0:
+            for f in ids:
+                del self._keep[f]
+        self._check(rc, "atg_server_feed")
 
     def poll(self, cap=1024):
         out = (Served * cap)()
@@ -467,7 +483,8 @@ class Server:
         res = []
         for i in range(n):
             o = out[i]
-            res.append(dict(stream=o.stream, status=o.status, tag=o.tag, arrival_ns=o.arrival_ns,
+            tag, _ = self._keep.pop(o.tag)
+            res.append(dict(stream=o.stream, status=o.status, tag=tag, arrival_ns=o.arrival_ns,
                             admit_ns=o.admit_ns, done_ns=o.done_ns, nquads=o.nquads, batch_frames=o.batch_frames,
                             dets=_dets(o.dets, o.ndets)))
         return res
@@ -484,7 +501,7 @@ class Server:
         if self.h:
             self.L.atg_server_destroy(self.h)
             self.h = C.c_void_p()
-            self._keep = []
+            self._keep = {}
 
 
 def synth_spec(config, width=None, height=None, ntags=None):

@@ -39,8 +39,15 @@ constexpr int kMaxFamilies = 16;
 constexpr int kSortBlocks = 32;      // radix-sort blocks per frame
 constexpr int kRadixBits = 11;       // bits of a radix-sort digit
 constexpr int kRadixBins = 1 << kRadixBits;
-constexpr int kMaxQuadsPerFrame = 4096;
-constexpr int kMaxDetsPerFrame = 1024;
+constexpr int kInlineDets = 32;      // detections of a frame returned with its counters
+// A cluster of fewer points never makes a quad (the corner search's window
+// ksz = min(20, n / 12) must reach 2, fast/quads.c:789-790), so only
+// clusters of at least this many points reach the fit: a frame's quads are
+// at most its points / kMinFitPoints.
+constexpr uint32_t kMinFitPoints = 24;
+// max_nmaxima at most: the corner search holds the pair tables of this
+// many maxima in LDS.
+constexpr int kMaxMaxima = 32;
 
 struct DevFamily {
     int32_t nbits, wab, tw, reversed;
@@ -163,14 +170,15 @@ struct Frame {
     double *mom;          // [6][pcap] cumulative moments in angle order
     double *errs, *filt;  // [pcap] window errors, filtered
     uint32_t *midx;       // [pcap] maxima positions
-    QuadRec *qtmp;        // [kMaxQuadsPerFrame] fitted quads, any order
-    uint32_t *qcl;        // [kMaxQuadsPerFrame] their cluster index
-    QuadRec *quads;       // [kMaxQuadsPerFrame] in cluster order
-    double *qline;        // [kMaxQuadsPerFrame][4][4] per quad edge: refined line (x, y, cos, sin);
+    uint32_t qcap;        // quad capacity: pcap / kMinFitPoints + 1, more than the frame can have
+    QuadRec *qtmp;        // [qcap] fitted quads, any order
+    uint32_t *qcl;        // [qcap] their cluster index
+    QuadRec *quads;       // [qcap] in cluster order
+    double *qline;        // [qcap][4][4] per quad edge: refined line (x, y, cos, sin);
                           // before k_lines: mean x, mean y and atan2's two arguments
-    at_detection *dtmp;   // [kMaxQuadsPerFrame * nfam] per (quad, family)
-    uint32_t *dflag;      // [kMaxQuadsPerFrame * nfam]
-    at_detection *dets;   // [kMaxDetsPerFrame] final, sorted by id
+    at_detection *dtmp;   // [qcap * nfam] per (quad, family); then the final list, sorted by id, in place
+    uint32_t *dflag;      // [qcap * nfam]
+    at_detection *inl;    // [kInlineDets] the final list's first entries, beside the counters
     uint32_t *counters;
 };
 
@@ -230,18 +238,22 @@ size_t replay_scratch_bytes(int qw, bool force);
 unsigned persistent_grid(const void *fn, int threads, size_t lds, int want);
 void warm_persistent_grids(int fit_blocks);
 
-// Optional timing marks between the stages of one call (HIP events on
-// the call's stream).
+// Optional timing marks between the stages of one call: a one-lane kernel
+// on the call's stream writes the device wall clock into stamps[i] (kernels
+// on one stream run in order, so consecutive stamps bracket the stage
+// between them). Being kernels, the marks are captured in the call's HIP
+// graph like any other launch.
 constexpr int kMaxMarks = 24;
+void launch_stamp(hipStream_t st, uint64_t *slot);
 struct Marks {
-    hipEvent_t ev[kMaxMarks];
+    uint64_t *stamps = nullptr;  // device [kMaxMarks]
     const char *name[kMaxMarks];
     int n = 0;
     bool on = false;
     void mark(hipStream_t st, const char *what) {
         if (!on || n >= kMaxMarks) return;
         name[n] = what;
-        hipEventRecord(ev[n++], st);
+        launch_stamp(st, stamps + n++);
     }
 };
 

@@ -76,6 +76,30 @@ extern "C" const char *at_family_name(const at_detector *td, const at_detection 
     return td->in->fams[(size_t)d->family].name;
 }
 
+// td->timing (fast/apriltag_fast.h:67-69) from the device times of the
+// chain's kernel groups: each reference stage is the sum of the groups that
+// do its work.
+static void fill_timing(atg_ctx *ctx, at_timing *t) {
+    static const struct {
+        const char *group;
+        double at_timing::*field;
+    } map[] = {{"decimate", &at_timing::decimate},   {"blur", &at_timing::blur},
+               {"threshold_rle", &at_timing::threshold}, {"uf_edges", &at_timing::unionfind},
+               {"uf_front", &at_timing::unionfind},   {"uf_replay", &at_timing::unionfind},
+               {"uf_stitch", &at_timing::unionfind},  {"resolve", &at_timing::unionfind},
+               {"labels", &at_timing::clusters},      {"points", &at_timing::clusters},
+               {"cluster_sort", &at_timing::clusters}, {"clusters", &at_timing::clusters},
+               {"fit_prep", &at_timing::fit_quads},   {"angle_sort", &at_timing::fit_quads},
+               {"moments", &at_timing::fit_quads},    {"corners", &at_timing::fit_quads},
+               {"decode", &at_timing::decode},        {"finish", &at_timing::reconcile}};
+    const char *names[32];
+    double ms[32];
+    int n = atg_stage_times(ctx, names, ms, 32);
+    for (int i = 0; i < n && i < 32; i++)
+        for (const auto &m : map)
+            if (strcmp(names[i], m.group) == 0) t->*m.field += ms[i] * 1e-3;
+}
+
 extern "C" void at_detect(at_detector *td, const uint8_t *pixels, int64_t width, int64_t height, int64_t stride) {
     at_internal *in = td->in;
     auto t0 = std::chrono::steady_clock::now();
@@ -87,9 +111,12 @@ extern "C" void at_detect(at_detector *td, const uint8_t *pixels, int64_t width,
     // fast/detector.c:254-255
     if (in->fams.empty()) return;
     if (width < 8 || height < 8) return;
-    if (!in->ctx && atg_ctx_create(0, &in->ctx) != ATG_OK) {
-        fprintf(stderr, "apriltag: no GPU context\n");
-        abort();
+    if (!in->ctx) {
+        if (atg_ctx_create(0, &in->ctx) != ATG_OK) {
+            fprintf(stderr, "apriltag: no GPU context\n");
+            abort();
+        }
+        atg_set_profiling(in->ctx, 1);  // td->timing per stage
     }
     if (in->fams_dirty) {
         std::vector<const at_family *> fp;
@@ -120,9 +147,7 @@ extern "C" void at_detect(at_detector *td, const uint8_t *pixels, int64_t width,
     td->detections = in->dets.data();
     td->ndetections = r.ndets;
     td->nquads = r.nquads;
-    double ms = 0.0;
-    atg_last_timing(in->ctx, &ms);
-    td->timing.decode = ms * 1e-3;
+    fill_timing(in->ctx, &td->timing);
     td->timing.total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

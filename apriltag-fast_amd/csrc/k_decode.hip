@@ -825,19 +825,31 @@ __device__ int64_t finish_dets(at_detection *d, int64_t n) {
     return n;
 }
 
+// A frame's detections: the decoded (quad, family) slots in order,
+// compacted in place (each moves to a slot at or before its own), then
+// reconciled and sorted; the first kInlineDets copied beside the counters.
 __global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevParams *P) {
     const Frame F = frames[blockIdx.y];
-    if (threadIdx.x != 0) return;
-    const uint32_t nq = F.counters[kNQuads];
-    const int nfam = P->nfam;
-    at_detection *d = F.dets;
-    int64_t n = 0;
-    for (uint32_t k = 0; k < nq * (uint32_t)nfam; k++) {
-        if (!F.dflag[k]) continue;
-        if (n < kMaxDetsPerFrame) d[n++] = F.dtmp[k];
-        else atomicOr(F.counters + kError, 16u);
+    __shared__ int64_t s_n;
+    if (threadIdx.x == 0) {
+        const uint32_t nq = F.counters[kNQuads];
+        const int nfam = P->nfam;
+        at_detection *d = F.dtmp;
+        int64_t n = 0;
+        for (uint32_t k = 0; k < nq * (uint32_t)nfam; k++) {
+            if (!F.dflag[k]) continue;
+            if ((int64_t)k != n) d[n] = d[k];
+            n++;
+        }
+        n = finish_dets(d, n);
+        F.counters[kNDets] = (uint32_t)n;
+        s_n = n;
     }
-    F.counters[kNDets] = (uint32_t)finish_dets(d, n);
+    __syncthreads();
+    const int64_t words = min(s_n, (int64_t)kInlineDets) * (int64_t)(sizeof(at_detection) / 8);
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(F.dtmp);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(F.inl);
+    for (int64_t i = threadIdx.x; i < words; i += 64) dst[i] = src[i];
 }
 
 // The finishing step of one detection list given by the caller.

@@ -308,4 +308,10 @@ void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     mk->mark(st, "threshold_rle");
 }
 
+__global__ void k_stamp(uint64_t *slot) {
+    if (threadIdx.x == 0) *slot = wall_clock64();
+}
+
+void launch_stamp(hipStream_t st, uint64_t *slot) { hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, slot); }
+
 }  // namespace atg

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
     for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         const uint64_t *pts = F.cpts + s0;
-        bool in_range = (int64_t)n >= P->mcp && (int64_t)n <= F.max_px;
+        bool in_range = (int64_t)n >= P->mcp && (int64_t)n <= F.max_px && n >= kMinFitPoints;
         uint64_t pv[kR];
 #pragma unroll
         for (int r = 0; r < kR; r++) {
@@ -996,7 +996,7 @@ template <int NT>
 struct CornerShape;
 template <>
 struct CornerShape<256> {
-    static constexpr int kMaxNm = 32;     // maxima kept at most
+    static constexpr int kMaxNm = kMaxMaxima;  // maxima kept at most
     static constexpr int kLdsPts = 1024;  // clusters up to this size keep their errors in LDS
     static constexpr int kMaxima = 1024;  // maxima values in LDS at most
 };
@@ -1239,6 +1239,11 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
             if (t == 0) atomicOr(F.counters + kError, 4u);
             continue;
         }
+        // the selection keeps fewer than four maxima when ties fill the top
+        // max_nmaxima + 1 values (or max_nmaxima < 4): no 4-tuple exists, the
+        // reference's search finds nothing and rejects the cluster
+        // (fast/quads.c:864-956); nm is the same in every thread
+        if (nm < 4) continue;
         // pair fits (fast/quads.c:869-928): (a, b) for a < b, closing (b, a) for b >= a + 3
         for (uint32_t pq = t; pq < nm * nm; pq += NT) {
             uint32_t p = pq / nm, q = pq % nm;
@@ -1389,7 +1394,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
                 q.reversed = (flag & 2u) ? 1 : 0;
                 q.hok = 0;
                 uint32_t slot = atomicAdd(F.counters + kNQuads, 1u);
-                if (slot < (uint32_t)kMaxQuadsPerFrame) {
+                if (slot < F.qcap) {
                     F.qtmp[slot] = q;
                     F.qcl[slot] = c;
                 } else {
@@ -1406,7 +1411,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
 // by its cluster index.
 __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
-    uint32_t nq = min(F.counters[kNQuads], (uint32_t)kMaxQuadsPerFrame);
+    uint32_t nq = min(F.counters[kNQuads], F.qcap);
     for (uint32_t i = threadIdx.x; i < nq; i += 256) {
         uint32_t ci = F.qcl[i], r = 0;
         for (uint32_t j = 0; j < nq; j++) r += F.qcl[j] < ci ? 1u : 0u;

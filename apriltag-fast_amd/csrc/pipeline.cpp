@@ -40,10 +40,10 @@ using namespace atg;
 
 namespace {
 
-constexpr int kResultDets = kMaxDetsPerFrame;  // detection capacity of a frame's result block
-constexpr int kInlineDets = 32;            // read back with the counters
 constexpr size_t kCountersBytes = 64;
-constexpr size_t kResultBlock = kCountersBytes + (size_t)kResultDets * sizeof(at_detection);
+// a frame's result block: its counters and the first detections (the rest
+// stay in its arena region)
+constexpr size_t kResultBlock = kCountersBytes + (size_t)kInlineDets * sizeof(at_detection);
 // fit_blocks 2048: the one-wave fit kernels that fit more than 16 waves a CU
 // (the small-cluster angle sort, edge refinement) get them all resident
 constexpr Tune kDefaultTune = {1, 4, 1024, 2048};
@@ -126,7 +126,11 @@ struct atg_ctx {
     std::vector<int64_t> nquads;
     double chain_ms = 0.0;
     int64_t pcap_scale = 1;           // points per quad-image pixel
-    Marks marks;                      // per-stage events, when profiling
+    Marks marks;                      // per-stage clock stamps, when profiling
+    uint64_t *hstamps = nullptr;      // pinned host copy of the stamps
+    double clock_khz = 0.0;           // the device wall clock's rate
+    int gmarks_n = 0;                 // the marks of the captured graph's chain
+    const char *gmarks_name[kMaxMarks];
     bool profiling = false;
     bool force_scratch = false;       // union-find replay state in HBM even where LDS fits
     bool arena_fixed = false;         // the arena never grows (a serving lane's budget)
@@ -182,6 +186,10 @@ void build_tables(const at_family &f, std::vector<uint32_t> &off, std::vector<ui
 }
 
 int upload_params(atg_ctx *ctx, const atg_params *p) {
+    // (the reference selects the (max_nmaxima+1)-th largest maximum,
+    // fast/quads.c:826-855: a negative value reads outside its array)
+    if (p->max_nmaxima < 0 || p->max_nmaxima > kMaxMaxima)
+        return fail(ctx, ATG_ERR_ARG, "max_nmaxima outside 0..32 (the corner search keeps at most 32 maxima)");
     DevParams &h = ctx->hp;
     memset(&h, 0, sizeof h);
     h.decimate = p->quad_decimate;
@@ -354,12 +362,14 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->errs = (double *)take((size_t)pc * 8);
     F->filt = (double *)take((size_t)pc * 8);
     F->midx = (uint32_t *)take((size_t)pc * 4);
-    F->qtmp = (QuadRec *)take(sizeof(QuadRec) * kMaxQuadsPerFrame);
-    F->qline = (double *)take(sizeof(double) * 16 * kMaxQuadsPerFrame);
-    F->qcl = (uint32_t *)take(4 * kMaxQuadsPerFrame);
-    F->quads = (QuadRec *)take(sizeof(QuadRec) * kMaxQuadsPerFrame);
-    F->dtmp = (at_detection *)take(sizeof(at_detection) * kMaxQuadsPerFrame * std::max(nfam, 1));
-    F->dflag = (uint32_t *)take(4 * (size_t)kMaxQuadsPerFrame * std::max(nfam, 1));
+    const size_t qc = (size_t)(pc / kMinFitPoints + 1);
+    F->qcap = (uint32_t)qc;
+    F->qtmp = (QuadRec *)take(sizeof(QuadRec) * qc);
+    F->qline = (double *)take(sizeof(double) * 16 * qc);
+    F->qcl = (uint32_t *)take(4 * qc);
+    F->quads = (QuadRec *)take(sizeof(QuadRec) * qc);
+    F->dtmp = (at_detection *)take(sizeof(at_detection) * qc * std::max(nfam, 1));
+    F->dflag = (uint32_t *)take(4 * qc * std::max(nfam, 1));
     return at;
 }
 
@@ -391,6 +401,11 @@ extern "C" int atg_ctx_create(int device, atg_ctx **out) {
         HIPCHK(hipEventCreate(&ctx->ev0));
         HIPCHK(hipEventCreate(&ctx->ev1));
         HIPCHK(hipEventCreateWithFlags(&ctx->ev2, hipEventDisableTiming));
+        HIPCHK(hipMalloc(&ctx->marks.stamps, kMaxMarks * sizeof(uint64_t)));
+        HIPCHK(hipHostMalloc(&ctx->hstamps, kMaxMarks * sizeof(uint64_t)));
+        int khz = 0;
+        HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+        ctx->clock_khz = khz > 0 ? khz : 100000.0;
         return ATG_OK;
     }();
     if (!ctx->err.empty()) {
@@ -412,6 +427,8 @@ extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
     hipFree(ctx->dframes);
     hipFree(ctx->dtables);
     hipFree(ctx->dparams);
+    hipFree(ctx->marks.stamps);
+    hipHostFree(ctx->hstamps);
     hipHostFree(ctx->pinned);
     hipHostFree(ctx->pinned_out);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
@@ -566,7 +583,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         F.blurred = ctx->hp.blur_ksz > 1 ? 1 : 0;
         F.pcap = fp.pcap;
         F.counters = reinterpret_cast<uint32_t *>(ctx->results + kResultBlock * (size_t)i);
-        F.dets = reinterpret_cast<at_detection *>(ctx->results + kResultBlock * (size_t)i + kCountersBytes);
+        F.inl = reinterpret_cast<at_detection *>(ctx->results + kResultBlock * (size_t)i + kCountersBytes);
         if (fp.skip) {
             // an 8x8 zero frame (k_init clears it): no tag can come out of it
             F.skip = 1;
@@ -592,7 +609,6 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     copies.push_back(Copy{dq, qstage, sizeof(WorkQueue)});
     hipStream_t st = ctx->stream;
     Marks *mk = &ctx->marks;
-    mk->n = 0;
     mk->on = ctx->profiling;
     const Tune tu = ctx->tune;
     // the angle sort in LDS per cluster, or (clusters too large) of all points in HBM
@@ -605,6 +621,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
     warm_persistent_grids(tu.fit_blocks);  // device queries stay out of stream capture
     auto issue = [&]() -> hipError_t {
+        mk->n = 0;
         for (const Copy &c : copies) {
             hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return e;
@@ -618,11 +635,15 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, ctx->hp.refine != 0, dq, mk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+        if (mk->n > 0) {
+            e = hipMemcpyAsync(ctx->hstamps, mk->stamps, sizeof(uint64_t) * (size_t)mk->n, hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return e;
+        }
         // results: counters and the first detections of every frame in one copy
         return hipMemcpy2DAsync(hout, pre, ctx->results, kResultBlock, pre, (size_t)n, hipMemcpyDeviceToHost, st);
     };
     HIPCHK(hipEventRecord(ctx->ev0, st));
-    if (ctx->graphs && !ctx->profiling) {
+    if (ctx->graphs) {
         // one graph per call shape: everything the stream work depends on
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
@@ -630,6 +651,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
                                     ccl.replay_mid_grid, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid,
                                     ccl.replay_waves, max_lds_qw,
                                     tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners, ctx->hp.refine,
+                                    ctx->profiling,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
                                     (int64_t)(intptr_t)ctx->dparams};
@@ -658,7 +680,12 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
                 return fail(ctx, ATG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
             }
             ctx->gkey = key;
+            ctx->gmarks_n = mk->n;
+            for (int k = 0; k < mk->n; k++) ctx->gmarks_name[k] = mk->name[k];
         }
+        // the marks of the captured chain (a replay issues nothing itself)
+        mk->n = ctx->gmarks_n;
+        for (int k = 0; k < mk->n; k++) mk->name[k] = ctx->gmarks_name[k];
         HIPCHK(hipGraphLaunch(ctx->gexec, st));
     } else {
         HIPCHK(issue());
@@ -688,9 +715,7 @@ int chain_collect(atg_ctx *ctx) {
     ctx->stage_ms.clear();
     ctx->stage_names.clear();
     for (int k = 1; k < mk->n; k++) {
-        float s = 0.0f;
-        HIPCHK(hipEventElapsedTime(&s, mk->ev[k - 1], mk->ev[k]));
-        ctx->stage_ms.push_back(s);
+        ctx->stage_ms.push_back((double)(ctx->hstamps[k] - ctx->hstamps[k - 1]) / ctx->clock_khz);
         ctx->stage_names.push_back(mk->name[k]);
     }
     ctx->dets.assign((size_t)n, {});
@@ -713,16 +738,15 @@ int chain_collect(atg_ctx *ctx) {
             ctx->fstatus[(size_t)i] = ATG_ERR_LIMIT;
             continue;
         }
-        uint32_t nd = std::min<uint32_t>(cnt[kNDets], kResultDets);
+        uint32_t nd = cnt[kNDets];
         ctx->nquads[(size_t)i] = cnt[kNQuads];
         auto &v = ctx->dets[(size_t)i];
         v.resize(nd);
         uint32_t first = std::min<uint32_t>(nd, kInlineDets);
         if (first) memcpy(v.data(), d, first * sizeof(at_detection));
         if (nd > first)
-            HIPCHK(hipMemcpy(v.data() + first, ctx->results + kResultBlock * (size_t)i + kCountersBytes +
-                                                   first * sizeof(at_detection),
-                             (nd - first) * sizeof(at_detection), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(v.data() + first, ctx->hframes[(size_t)i].dtmp + first, (nd - first) * sizeof(at_detection),
+                             hipMemcpyDeviceToHost));
     }
     if (!first_err.empty()) return fail(ctx, ATG_ERR_LIMIT, first_err);
     if (overflow) return 1;  // the caller retries with more point capacity
@@ -842,10 +866,6 @@ extern "C" int atg_copy_to_device(atg_ctx *ctx, void *dst, const void *src, size
 
 extern "C" int atg_set_profiling(atg_ctx *ctx, int on) {
     if (!ctx) return ATG_ERR_ARG;
-    if (on && !ctx->profiling) {
-        HIPCHK(hipSetDevice(ctx->device));
-        for (int k = 0; k < kMaxMarks; k++) HIPCHK(hipEventCreate(&ctx->marks.ev[k]));
-    }
     ctx->profiling = on != 0;
     return ATG_OK;
 }
@@ -1082,7 +1102,7 @@ extern "C" int atg_debug_stages(atg_ctx *ctx, int32_t frame, atg_stages *out) {
     out->cluster_ids = (uint64_t *)malloc(8 * (size_t)nc + 8);
     HIPCHK(hipMemcpy(out->cluster_ids, F.cid, 8 * (size_t)nc, hipMemcpyDeviceToHost));
     for (uint32_t c = 0; c < nc; c++) out->ccount[c] = cs[c + 1] - cs[c];
-    uint32_t nq = std::min<uint32_t>(cnt[kNQuads], kMaxQuadsPerFrame);
+    uint32_t nq = std::min<uint32_t>(cnt[kNQuads], F.qcap);
     out->nquads = nq;
     std::vector<QuadRec> q(nq);
     HIPCHK(hipMemcpy(q.data(), F.quads, sizeof(QuadRec) * nq, hipMemcpyDeviceToHost));

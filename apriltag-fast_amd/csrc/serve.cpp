@@ -215,7 +215,16 @@ void release_arrivals(atg_server *s) {
         StreamS &S = s->streams[(size_t)a.stream];
         s->submitted++;
         if (S.queued >= kQueuePerStream) {
+            // dropped: its record comes back from poll with ATG_ERR_LIMIT
             s->dropped++;
+            Done d;
+            d.rec = atg_served{};
+            d.rec.stream = a.stream;
+            d.rec.status = ATG_ERR_LIMIT;
+            d.rec.tag = a.tag;
+            d.rec.arrival_ns = s->feed_t0 + a.at_ns - s->t0;
+            d.rec.admit_ns = d.rec.done_ns = t - s->t0;
+            s->done.push_back(std::move(d));
             continue;
         }
         S.queued++;

@@ -6,76 +6,126 @@
 // families, matd_t and image_u8_t as those functions need them.
 #include <pthread.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <utility>
+#include <vector>
 
 #include "apriltag.h"
 #include "apriltag_fast.h"
 
 namespace {
 
-// apriltag_family_t -> at_family (fast/families.c:19-35): the codes are
-// borrowed, the bit layout copied.
-bool to_fast_family(const apriltag_family_t *src, at_family *dst) {
-    if (src->nbits > 64) return false;
-    memset(dst, 0, sizeof *dst);
-    strncpy(dst->name, src->name ? src->name : "", sizeof dst->name - 1);
-    dst->codes = src->codes;
-    dst->ncodes = src->ncodes;
-    dst->nbits = (int32_t)src->nbits;
-    dst->h = (int32_t)src->h;
-    dst->width_at_border = src->width_at_border;
-    dst->total_width = src->total_width;
-    dst->reversed_border = src->reversed_border;
-    for (uint32_t i = 0; i < src->nbits; i++) {
-        dst->bit_x[i] = (int32_t)src->bit_x[i];
-        dst->bit_y[i] = (int32_t)src->bit_y[i];
-    }
+// The native family record of an upstream family: the code list is shared
+// (the caller keeps it alive, as for at_add_family), the bit positions are
+// copied into the record's fixed arrays. False for a layout the native
+// record cannot hold (more than 64 bits).
+bool native_family(const apriltag_family_t &up, at_family *out) {
+    if (up.nbits > 64) return false;
+    at_family f{};
+    snprintf(f.name, sizeof f.name, "%s", up.name ? up.name : "");
+    f.codes = up.codes;
+    f.ncodes = up.ncodes;
+    f.nbits = (int32_t)up.nbits;
+    f.h = (int32_t)up.h;
+    f.width_at_border = up.width_at_border;
+    f.total_width = up.total_width;
+    f.reversed_border = up.reversed_border;
+    std::copy(up.bit_x, up.bit_x + up.nbits, f.bit_x);
+    std::copy(up.bit_y, up.bit_y + up.nbits, f.bit_y);
+    *out = f;
     return true;
 }
 
-// The families to the native detector again; one it refuses (bits above
-// 3, a 17th family) is removed so that family indices stay aligned
-// (apriltag.c:97-120).
-void sync_families(apriltag_detector_t *td) {
+// Hands the upstream family list to the native detector. A family it
+// refuses (bits_corrected outside 0..3, a 17th family, a layout it cannot
+// hold) leaves both upstream lists, with a line on stderr, so that a
+// detection's family index names the same family on both sides.
+void push_families(apriltag_detector_t *td) {
+    std::vector<apriltag_family_t *> fams;
+    std::vector<int> bits;
+    for (int k = 0; k < zarray_size(td->tag_families); k++) {
+        apriltag_family_t *up;
+        int b;
+        zarray_get(td->tag_families, k, &up);
+        zarray_get(td->family_bits, k, &b);
+        fams.push_back(up);
+        bits.push_back(b);
+    }
     at_clear_families(td->fast);
-    for (int i = 0; i < zarray_size(td->tag_families);) {
-        apriltag_family_t *fam;
-        int bits;
-        zarray_get(td->tag_families, i, &fam);
-        zarray_get(td->family_bits, i, &bits);
+    zarray_clear(td->tag_families);
+    zarray_clear(td->family_bits);
+    for (size_t k = 0; k < fams.size(); k++) {
         at_family f;
-        if (to_fast_family(fam, &f) && at_add_family(td->fast, &f, bits)) {
-            i++;
+        if (!native_family(*fams[k], &f) || !at_add_family(td->fast, &f, bits[k])) {
+            fprintf(stderr, "apriltag: cannot add family %s (bits_corrected %d), removed\n", fams[k]->name, bits[k]);
             continue;
         }
-        fprintf(stderr, "apriltag: cannot add family %s (bits_corrected %d), removed\n", fam->name, bits);
-        zarray_remove_index(td->tag_families, i, 0);
-        zarray_remove_index(td->family_bits, i, 0);
+        zarray_add(td->tag_families, &fams[k]);
+        zarray_add(td->family_bits, &bits[k]);
     }
     td->families_changed = false;
 }
 
-// td->tp as the reference fills it (apriltag.c:123-138): one stamp per
-// stage time of the native detector.
-void fill_timeprofile(timeprofile_t *tp, const at_timing *t) {
-    int64_t u = tp->utime;
+// td->tp after a detection: one stamp per stage of the upstream detector's
+// profile, in its order and with its names, each at the start time plus
+// the stage times so far.
+void stamp_profile(timeprofile_t *tp, const at_timing &t) {
+    const std::pair<const char *, double> stages[] = {
+        {"init", 0.0},
+        {"decimate", t.decimate},
+        {"blur/sharp", t.blur},
+        {"threshold", t.threshold},
+        {"unionfind", t.unionfind},
+        {"make clusters", t.clusters},
+        {"fit quads to clusters", t.fit_quads},
+        {"decode+refinement", t.decode},
+        {"reconcile", t.reconcile},
+    };
     zarray_clear(tp->stamps);
-    const char *names[] = {"init", "decimate", "blur/sharp", "threshold", "unionfind", "make clusters",
-                           "fit quads to clusters", "decode+refinement", "reconcile"};
-    const double secs[] = {0.0, t->decimate, t->blur, t->threshold, t->unionfind, t->clusters, t->fit_quads,
-                           t->decode, t->reconcile};
-    for (size_t i = 0; i < sizeof secs / sizeof secs[0]; i++) {
-        u += (int64_t)(secs[i] * 1e6);
-        struct timeprofile_entry e;
-        memset(&e, 0, sizeof e);
-        strncpy(e.name, names[i], sizeof e.name - 1);
-        e.utime = u;
+    int64_t at = tp->utime;
+    for (const auto &st : stages) {
+        at += (int64_t)(st.second * 1e6);
+        struct timeprofile_entry e{};
+        snprintf(e.name, sizeof e.name, "%s", st.first);
+        e.utime = at;
         zarray_add(tp->stamps, &e);
     }
+}
+
+// The native detector's parameters from the upstream detector's.
+void copy_params(const apriltag_detector_t &up, at_detector *fd) {
+    fd->nthreads = std::max(up.nthreads, 1);
+    fd->quad_decimate = up.quad_decimate;
+    fd->quad_sigma = up.quad_sigma;
+    fd->refine_edges = up.refine_edges;
+    fd->decode_sharpening = up.decode_sharpening;
+    fd->debug = up.debug;
+    fd->qtp.min_cluster_pixels = up.qtp.min_cluster_pixels;
+    fd->qtp.max_nmaxima = up.qtp.max_nmaxima;
+    fd->qtp.critical_rad = up.qtp.critical_rad;
+    fd->qtp.cos_critical_rad = up.qtp.cos_critical_rad;
+    fd->qtp.max_line_fit_mse = up.qtp.max_line_fit_mse;
+    fd->qtp.min_white_black_diff = up.qtp.min_white_black_diff;
+    fd->qtp.deglitch = up.qtp.deglitch != 0;
+}
+
+// A new upstream detection record (calloc'd, H a new 3x3 matd_t) for a
+// native one; the family pointer is borrowed from the detector's list.
+apriltag_detection_t *upstream_detection(const at_detection &d, zarray_t *families) {
+    apriltag_detection_t *det = (apriltag_detection_t *)calloc(1, sizeof(apriltag_detection_t));
+    zarray_get(families, d.family, &det->family);
+    det->id = d.id;
+    det->hamming = d.hamming;
+    det->decision_margin = d.decision_margin;
+    det->H = matd_create_data(3, 3, d.H);
+    std::copy(d.c, d.c + 2, det->c);
+    std::copy(&d.p[0][0], &d.p[0][0] + 8, &det->p[0][0]);
+    return det;
 }
 
 apriltag_family_t *family_from_builtin(const at_family *f) {
@@ -188,45 +238,25 @@ void apriltag_detector_clear_families(apriltag_detector_t *td) {
 }
 
 zarray_t *apriltag_detector_detect(apriltag_detector_t *td, image_u8_t *im_orig) {
-    zarray_t *detections = zarray_create(sizeof(apriltag_detection_t *));
-    if (zarray_size(td->tag_families) == 0) return detections;      // apriltag.c:156-159
-    if (im_orig->width < 8 || im_orig->height < 8) return detections;  // apriltag.c:160-164
-    if (td->families_changed) sync_families(td);
+    zarray_t *out = zarray_create(sizeof(apriltag_detection_t *));
+    // no families, or an image under 8x8: no detections
+    const bool runs = zarray_size(td->tag_families) > 0 && im_orig->width >= 8 && im_orig->height >= 8;
+    if (!runs) return out;
+    if (td->families_changed) push_families(td);
     at_detector *fd = td->fast;
-    fd->nthreads = td->nthreads > 0 ? td->nthreads : 1;
-    fd->quad_decimate = td->quad_decimate;
-    fd->quad_sigma = td->quad_sigma;
-    fd->refine_edges = td->refine_edges;
-    fd->decode_sharpening = td->decode_sharpening;
-    fd->debug = td->debug;
-    fd->qtp.min_cluster_pixels = td->qtp.min_cluster_pixels;
-    fd->qtp.max_nmaxima = td->qtp.max_nmaxima;
-    fd->qtp.critical_rad = td->qtp.critical_rad;
-    fd->qtp.cos_critical_rad = td->qtp.cos_critical_rad;
-    fd->qtp.max_line_fit_mse = td->qtp.max_line_fit_mse;
-    fd->qtp.min_white_black_diff = td->qtp.min_white_black_diff;
-    fd->qtp.deglitch = td->qtp.deglitch != 0;
+    copy_params(*td, fd);
     td->tp->utime = utime_now();
     at_detect(fd, im_orig->buf, im_orig->width, im_orig->height, im_orig->stride);
-    fill_timeprofile(td->tp, &fd->timing);
+    stamp_profile(td->tp, fd->timing);
     td->nquads = (uint32_t)fd->nquads;
     td->nedges = 0;
     td->nsegments = 0;
-    zarray_ensure_capacity(detections, (int)fd->ndetections);
-    for (int64_t i = 0; i < fd->ndetections; i++) {
-        const at_detection *d = &fd->detections[i];
-        apriltag_detection_t *det = (apriltag_detection_t *)calloc(1, sizeof(apriltag_detection_t));
-        zarray_get(td->tag_families, d->family, &det->family);
-        det->id = d->id;
-        det->hamming = d->hamming;
-        det->decision_margin = d->decision_margin;
-        det->H = matd_create_data(3, 3, d->H);
-        det->c[0] = d->c[0];
-        det->c[1] = d->c[1];
-        memcpy(det->p, d->p, sizeof det->p);
-        zarray_add(detections, &det);
+    zarray_ensure_capacity(out, (int)fd->ndetections);
+    for (const at_detection *d = fd->detections; d != fd->detections + fd->ndetections; ++d) {
+        apriltag_detection_t *det = upstream_detection(*d, td->tag_families);
+        zarray_add(out, &det);
     }
-    return detections;
+    return out;
 }
 
 void apriltag_detection_destroy(apriltag_detection_t *det) {
@@ -245,48 +275,47 @@ void apriltag_detections_destroy(zarray_t *detections) {
 }
 
 void apriltag_detection_copy(apriltag_detection_t *src, apriltag_detection_t *dst) {
-    if (dst->H) matd_destroy(dst->H);
-    dst->H = matd_copy(src->H);
-    dst->c[0] = src->c[0];
-    dst->c[1] = src->c[1];
-    memcpy(dst->p, src->p, sizeof dst->p);
-    dst->id = src->id;
-    dst->family = src->family;
-    dst->hamming = src->hamming;
-    dst->decision_margin = src->decision_margin;
+    matd_t *h = matd_copy(src->H);  // dst's own H is replaced
+    matd_destroy(dst->H);
+    *dst = *src;
+    dst->H = h;
 }
 
 zarray_t *apriltag_detections_copy(zarray_t *detections) {
+    const int n = zarray_size(detections);
     zarray_t *out = zarray_create(sizeof(apriltag_detection_t *));
-    for (int i = 0; i < zarray_size(detections); i++) {
-        apriltag_detection_t *det;
-        zarray_get(detections, i, &det);
-        apriltag_detection_t *c = (apriltag_detection_t *)calloc(1, sizeof(apriltag_detection_t));
-        apriltag_detection_copy(det, c);
-        zarray_add(out, &c);
+    zarray_ensure_capacity(out, n);
+    for (int k = 0; k < n; k++) {
+        apriltag_detection_t *src, *dst = (apriltag_detection_t *)calloc(1, sizeof(apriltag_detection_t));
+        zarray_get(detections, k, &src);
+        apriltag_detection_copy(src, dst);
+        zarray_add(out, &dst);
     }
     return out;
 }
 
-// apriltag.c:263-297: the white ring inside the border, then the code
-// bits, 1 pixel per cell; the rest black.
+// The tag image of code idx, one pixel per cell, total_width square, black
+// but for: the one-cell white ring just inside where the black border
+// ends (for a normal-border family, two cells wider than the data square:
+// the white quiet zone; for a reversed-border family, the data square's own
+// outline), drawn as four strokes of width-1 cells, one from each corner
+// of the ring going clockwise; and the white data bits, bit i (most
+// significant first) at (bit_x[i], bit_y[i]) from the data square's corner.
 image_u8_t *apriltag_to_image(apriltag_family_t *fam, uint32_t idx) {
     if (!fam || idx >= fam->ncodes) return nullptr;
     const int tw = fam->total_width;
     image_u8_t *im = image_u8_create((unsigned)tw, (unsigned)tw);
+    auto white = [&](int x, int y) { im->buf[y * im->stride + x] = 255; };
+    const int ring = fam->width_at_border + (fam->reversed_border ? 0 : 2);
+    const int lo = (tw - ring) / 2, hi = tw - 1 - lo;
+    // strokes: start x, start y, step x, step y
+    const int strokes[4][4] = {{lo, lo, 1, 0}, {hi, lo, 0, 1}, {lo + 1, hi, 1, 0}, {lo, lo + 1, 0, 1}};
+    for (const auto &st : strokes)
+        for (int k = 0; k < ring - 1; k++) white(st[0] + k * st[2], st[1] + k * st[3]);
+    const int data = (tw - fam->width_at_border) / 2;
     const uint64_t code = fam->codes[idx];
-    const int ww = fam->width_at_border + (fam->reversed_border ? 0 : 2);
-    const int w0 = (tw - ww) / 2, w1 = tw - 1 - w0;
-    for (int i = 0; i < ww - 1; i++) {
-        im->buf[w0 * im->stride + w0 + i] = 255;             // top, from the left
-        im->buf[(w0 + i) * im->stride + w1] = 255;           // right, from the top
-        im->buf[w1 * im->stride + w0 + i + 1] = 255;         // bottom, from the second
-        im->buf[(w0 + 1 + i) * im->stride + w0] = 255;       // left, from the second
-    }
-    const int b0 = (tw - fam->width_at_border) / 2;
     for (uint32_t i = 0; i < fam->nbits; i++)
-        if (code & ((uint64_t)1 << (fam->nbits - i - 1)))
-            im->buf[((int)fam->bit_y[i] + b0) * im->stride + (int)fam->bit_x[i] + b0] = 255;
+        if ((code >> (fam->nbits - 1 - i)) & 1u) white(data + (int)fam->bit_x[i], data + (int)fam->bit_y[i]);
     return im;
 }
 

@@ -179,6 +179,8 @@ def serving_sweep(atg, ctx, loads, seconds, nstreams=32, nframes=4, check_ref=Tr
                     break
                 res.extend(got)
             counts = srv.counts()
+            # dropped arrivals come back with ATG_ERR_LIMIT and no detections
+            res = [r for r in res if r["status"] != -3]
         finally:
             srv.close()
         exact = all(r["status"] == 0 and same_dets(r["dets"], alone[(r["tag"] // 1000000, (r["tag"] % 1000000) % nframes)]["dets"])

@@ -43,7 +43,8 @@ extern "C" {
 typedef struct atg_ctx atg_ctx;
 
 /* The detection parameters (the parameter head of at_detector,
- * fast/apriltag_fast.h:73-87). */
+ * fast/apriltag_fast.h:73-87). max_nmaxima must be 0..32 (ATG_ERR_ARG
+ * otherwise): the corner search keeps at most 32 maxima on the device. */
 typedef struct {
     float quad_decimate;
     float quad_sigma;
@@ -159,8 +160,9 @@ int atg_debug_force_scratch(atg_ctx *ctx, int on);
  * detect call's kernel chain. */
 int atg_last_timing(const atg_ctx *ctx, double *chain_ms);
 
-/* Per-stage device times of the following calls: HIP events between the
- * kernel groups on the context's stream (a little launch overhead each). */
+/* Per-stage device times of the following calls: a one-lane kernel
+ * between the kernel groups stamps the device wall clock (a few us each;
+ * graph replay stays on). */
 int atg_set_profiling(atg_ctx *ctx, int on);
 /* The stages of the last call: names[i] and ms[i] for i < the return
  * value (at most cap are written). */
@@ -231,7 +233,9 @@ int atg_server_add_stream(atg_server *s, int32_t width, int32_t height, int32_t 
  * polled). ATG_ERR_LIMIT: the stream's queue is full and the frame dropped. */
 int atg_server_submit(atg_server *s, int32_t stream, const uint8_t *pixels, int64_t stride, uint64_t tag);
 /* Frames that arrive at their times, released by the server's own feeder
- * (precise arrival processes without a caller thread). */
+ * (precise arrival processes without a caller thread). A fed frame that
+ * finds its stream's queue full is dropped: its record comes back from
+ * atg_server_poll with status ATG_ERR_LIMIT and no detections. */
 int atg_server_feed(atg_server *s, const atg_arrival *arrivals, int64_t n);
 /* Completed frames, in completion order: up to cap into out; returns the count. */
 int atg_server_poll(atg_server *s, atg_served *out, int32_t cap);

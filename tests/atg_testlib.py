@@ -458,3 +458,46 @@ def synth_case(atg, cfg, seed):
     fams = [("tag36h11", 2)] if cfg != 3 else [("tag36h11", 2), ("tagStandard52h13", 1)]
     p = default_params(quad_decimate=1.0 if cfg == 3 else 2.0)
     return img, p, fams
+
+
+def tag_cells(atg, family, idx):
+    """The tag image of code idx (apriltag_to_image of the product's
+    upstream API, host code): one byte per cell."""
+    L = atg.lib()
+
+    class ImageU8(C.Structure):
+        _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("stride", C.c_int32),
+                    ("buf", C.POINTER(C.c_uint8))]
+    L.apriltag_to_image.restype = C.POINTER(ImageU8)
+    L.apriltag_to_image.argtypes = [C.c_void_p, C.c_uint32]
+    L.image_u8_destroy.argtypes = [C.POINTER(ImageU8)]
+    create, destroy = getattr(L, family + "_create"), getattr(L, family + "_destroy")
+    create.restype = C.c_void_p
+    destroy.argtypes = [C.c_void_p]
+    fam = create()
+    try:
+        p = L.apriltag_to_image(fam, idx)
+        im = p.contents
+        out = np.ctypeslib.as_array(im.buf, (im.height, im.stride))[:, :im.width].copy()
+        L.image_u8_destroy(p)
+        return out
+    finally:
+        destroy(fam)
+
+
+def crowded_frame(atg):
+    """A 2048x2048 page: 1250 tag16h5 tags of 32 px in the top half (ids
+    cycling over 0..29) and 14450 black 8 px squares in the bottom half.
+    With tag16h5 at decimate 1 it makes 15530 quad candidates and 1250
+    detections."""
+    cells = [np.kron(tag_cells(atg, "tag16h5", i), np.ones((4, 4), np.uint8)) for i in range(30)]
+    img = np.full((2048, 2048), 255, np.uint8)
+    k = 0
+    for y in range(8, 1000, 40):
+        for x in range(8, 2048 - 40, 40):
+            img[y:y + 32, x:x + 32] = cells[k % 30]
+            k += 1
+    for y in range(1040, 2040, 12):
+        for x in range(8, 2040, 12):
+            img[y:y + 8, x:x + 8] = 0
+    return img

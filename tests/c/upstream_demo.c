@@ -86,6 +86,11 @@ int main(int argc, char **argv) {
             printf("\n");
         }
         printf("# %s nquads %u\n", argv[a], td->nquads);
+        for (int i = 0; i < zarray_size(td->tp->stamps); i++) {
+            struct timeprofile_entry e;
+            zarray_get(td->tp->stamps, i, &e);
+            printf("# tp %s|%lld\n", e.name, (long long)(e.utime - td->tp->utime));
+        }
         apriltag_detections_destroy(dets);
         image_u8_destroy(im);
     }

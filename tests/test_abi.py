@@ -129,3 +129,39 @@ def test_upstream_api_builds_with_reference_layouts(tmp_path):
     out = subprocess.run([str(exe), "-s"], check=True, capture_output=True, text=True).stdout
     sizes = {l.split()[0]: int(l.split()[1]) for l in out.splitlines()}
     assert sizes == UPSTREAM_SIZES
+
+
+class _ImageU8(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("stride", C.c_int32), ("buf", C.POINTER(C.c_uint8))]
+
+
+def test_apriltag_to_image_round_trips_through_the_detector():
+    """apriltag_to_image (upstream API, host code): the tag image of code
+    idx of every family, scaled up and set on a white page, decodes to idx
+    with hamming 0 in the oracle (which the reference's goldens pin)."""
+    from atg_testlib import FAMILY_NAMES, Oracle, default_params
+    atg = load_atg()
+    L = atg.lib()
+    L.apriltag_to_image.restype = C.POINTER(_ImageU8)
+    L.apriltag_to_image.argtypes = [C.c_void_p, C.c_uint32]
+    L.image_u8_destroy.argtypes = [C.POINTER(_ImageU8)]
+    orc = Oracle()
+    for name in FAMILY_NAMES:
+        create = getattr(L, name + "_create")
+        create.restype = C.c_void_p
+        destroy = getattr(L, name + "_destroy")
+        destroy.argtypes = [C.c_void_p]
+        fam = create()
+        try:
+            for idx in (0, 3, 11):
+                ptr = L.apriltag_to_image(fam, idx)
+                im = ptr.contents
+                h, w = im.height, im.width
+                cells = np.ctypeslib.as_array(im.buf, (h, im.stride))[:, :w].copy()
+                L.image_u8_destroy(ptr)
+                page = np.full((h * 12 + 80, w * 12 + 80), 255, np.uint8)
+                page[40:40 + h * 12, 40:40 + w * 12] = np.kron(cells, np.ones((12, 12), np.uint8))
+                d = orc.detect(page, default_params(quad_decimate=1.0), [(name, 0)])["dets"]
+                assert [(int(x["id"]), int(x["hamming"])) for x in d] == [(idx, 0)], (name, idx)
+        finally:
+            destroy(fam)

@@ -12,8 +12,8 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from atg_testlib import (GOLDEN, SYNTH_CASES, Detector, default_params, det_rows, frame_hash, golden_cases,
-                         parse_case_args, parse_hex, read_pnm, ref_images, same_dets, synth_case)
+from atg_testlib import (GOLDEN, SYNTH_CASES, Detector, crowded_frame, default_params, det_rows, frame_hash,
+                         golden_cases, parse_case_args, parse_hex, read_pnm, ref_images, same_dets, synth_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -93,6 +93,52 @@ def test_extreme_shapes_match_oracle(atg, ctx, oracle, name):
         a, b = o["stages"][k], st[k]
         assert a.shape == b.shape and np.array_equal(a, b), k
     assert same_dets(o["dets"], g["dets"]) and o["nquads"] == g["nquads"]
+
+
+def test_crowded_frame_matches_oracle(atg, ctx, oracle):
+    """15530 quad candidates and 1250 detections in one frame (past any
+    fixed per-frame capacity): stages and detections equal the oracle's."""
+    img = crowded_frame(atg)
+    p, fams = default_params(quad_decimate=1.0), [("tag16h5", 0)]
+    print("case crowded 2048x2048 tag16h5", flush=True)
+    o = oracle.detect(img, p, fams, stages=True)
+    g = gpu_detect(atg, ctx, img, p, fams)
+    st = ctx.stages(0)
+    for k in STAGE_KEYS:
+        assert np.array_equal(o["stages"][k], st[k]), k
+    assert o["nquads"] == g["nquads"] > 4096 and len(g["dets"]) > 1024
+    assert same_dets(o["dets"], g["dets"])
+
+
+def test_max_nmaxima_limit(atg, ctx, oracle):
+    """max_nmaxima 0..32 runs as the oracle (0 keeps no maxima: no quads;
+    17..32 takes the 256-thread corner search for every cluster); outside
+    it the call is refused with an error, not run."""
+    img, p, fams = synth_case(atg, 2, 0)
+    ctx.set_families(fams)
+    for m in (0, 3, 17, 32):
+        q = dict(p, max_nmaxima=m)
+        g, o = ctx.detect(img, atg.params_from_dict(q)), oracle.detect(img, q, fams)
+        assert g["nquads"] == o["nquads"] and same_dets(g["dets"], o["dets"]), m
+        assert (g["nquads"] > 0) == (m > 3), m
+    for bad in (-1, 33):
+        with pytest.raises(RuntimeError):
+            ctx.detect(img, atg.params_from_dict(dict(p, max_nmaxima=bad)))
+
+
+@pytest.mark.parametrize("mcp", [0, 1, 3])
+def test_small_min_cluster_pixels_match_oracle(atg, ctx, oracle, mcp):
+    """min_cluster_pixels below 4: every component holds points and every
+    cluster is a candidate; stages and detections equal the oracle's."""
+    img, p, fams = synth_case(atg, 2, 1)
+    p = dict(p, min_cluster_pixels=mcp)
+    print("case mcp", mcp, flush=True)
+    o = oracle.detect(img, p, fams, stages=True)
+    g = gpu_detect(atg, ctx, img, p, fams)
+    st = ctx.stages(0)
+    for k in STAGE_KEYS:
+        assert np.array_equal(o["stages"][k], st[k]), k
+    assert o["nquads"] == g["nquads"] and same_dets(o["dets"], g["dets"])
 
 
 @pytest.mark.parametrize("cfg,seed", [(1, 1), (4, 3)])
@@ -177,6 +223,11 @@ def test_native_at_detect_api(atg):
             raw = C.string_at(d.detections, d.ndetections * 168) if d.ndetections else b""
             dets = np.frombuffer(raw, atg.DET_DTYPE)
             assert det_rows(dets) == gold[path.name]
+        # the stage times of the call (fast/detector.c:257-334): each stage ran
+        tm = td.contents.timing
+        stages = [tm.decimate, tm.threshold, tm.unionfind, tm.clusters, tm.fit_quads, tm.decode, tm.reconcile]
+        assert all(t > 0 for t in stages), stages
+        assert sum(stages) + tm.blur <= tm.total
         # under 8x8: nothing, as the reference
         L.at_detect(td, img.ctypes.data, 7, 7, img.shape[1])
         assert td.contents.ndetections == 0 and td.contents.nquads == 0
@@ -220,3 +271,10 @@ def test_upstream_api_golden_hex(tmp_path):
         gold = parse_hex(GOLDEN / "ref_hex" / (name + ".hex"))
         for path in ref_images():
             assert got.get(path.name, []) == gold.get(path.name, []), (name, path.name)
+        # td->tp: the upstream profile's stages, in order, stamped as they end
+        tp = [l[len("# tp "):].split("|") for l in out.splitlines() if l.startswith("# tp ")]
+        names = ["init", "decimate", "blur/sharp", "threshold", "unionfind", "make clusters",
+                 "fit quads to clusters", "decode+refinement", "reconcile"]
+        assert [n for n, _ in tp[:9]] == names
+        us = [int(u) for _, u in tp[:9]]
+        assert us == sorted(us) and us[-1] > 0

@@ -15,7 +15,7 @@ import pytest
 from atg_testlib import (DET_DTYPE, DIGEST_KEYS, GOLDEN, NAMED_CASES, RECONCILE_CASES, STAGE_DIGEST_CASES,
                          Oracle, Reference, batch64_frames, default_params, det_rows, digest, frame_hash,
                          golden_cases, have_ref, load_atg, named_case, parse_case_args, parse_hex, read_pnm,
-                         ref_images, same_dets, SYNTH_CASES, synth_case)
+                         ref_images, same_dets, SYNTH_CASES, synth_case, crowded_frame)
 
 CASES = golden_cases()
 
@@ -133,6 +133,20 @@ def test_reference_reconcile_reproduces_fixture():
     ref = Reference()
     for k, (d, keep) in enumerate(reconcile_lists()[:50]):
         assert ref.reconcile(d).tobytes() == d[keep].tobytes(), k
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference build (oracle/_ref) not present")
+def test_oracle_crowded_frame_matches_reference():
+    """A frame of 15530 quad candidates and 1250 detections: the oracle's
+    stages and detections equal the reference build's."""
+    img = crowded_frame(load_atg())
+    p, fams = default_params(quad_decimate=1.0), [("tag16h5", 0)]
+    o = Oracle().detect(img, p, fams, stages=True)
+    r = Reference().detect(img, p, fams, stages=True)
+    assert o["nquads"] == r["nquads"] > 4096 and len(r["dets"]) > 1024
+    assert same_dets(o["dets"], r["dets"])
+    for k in ("rep", "cluster_ids", "pts", "quads"):
+        assert np.array_equal(o["stages"][k], r["stages"][k]), k
 
 
 def _square(cx, cy, s, rot=0.0, cw=False):
