@@ -1425,23 +1425,41 @@ void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int wh
 
 void warm_decode_grid(int fit_blocks);
 
+// Resident blocks per kernel shape, looked up in a fixed table (a lookup
+// never allocates, so calls that must not allocate can launch through it).
 unsigned persistent_grid(const void *fn, int threads, size_t lds, int want) {
+    struct Entry {
+        const void *fn;
+        int threads;
+        size_t lds;
+        int resident;
+    };
+    constexpr int kSlots = 256;
     static std::mutex mu;
-    static std::map<std::tuple<const void *, int, size_t>, int> cap;
-    int resident;
+    static Entry table[kSlots];
+    int resident = 0;
     {
         std::lock_guard<std::mutex> g(mu);
-        auto it = cap.find({fn, threads, lds});
-        if (it == cap.end()) {
-            int dev = 0, cus = 0, per_cu = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess)
-                cus = per_cu = 0;
-            (void)hipGetLastError();
-            it = cap.emplace(std::make_tuple(fn, threads, lds), std::max(1, per_cu * cus)).first;
+        size_t h = (reinterpret_cast<size_t>(fn) >> 4) * 31 + (size_t)threads * 7 + lds;
+        for (int k = 0; k < kSlots; k++) {
+            Entry &e = table[(h + (size_t)k) % kSlots];
+            if (e.fn == fn && e.threads == threads && e.lds == lds) {
+                resident = e.resident;
+                break;
+            }
+            if (e.fn == nullptr) {
+                int dev = 0, cus = 0, per_cu = 0;
+                if (hipGetDevice(&dev) != hipSuccess ||
+                    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess)
+                    cus = per_cu = 0;
+                (void)hipGetLastError();
+                e = Entry{fn, threads, lds, std::max(1, per_cu * cus)};
+                resident = e.resident;
+                break;
+            }
         }
-        resident = it->second;
+        if (resident == 0) resident = 1;  // a full table (never: a few dozen shapes exist)
     }
     return (unsigned)std::max(1, std::min(want, resident));
 }

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -104,6 +105,7 @@ struct atg_ctx {
     std::vector<at_family> fams;
     std::vector<int32_t> bits;
     uint8_t *dtables = nullptr;
+    size_t tab_off[kMaxFamilies][3] = {};  // per family: offsets, ids, codes within dtables
     DevParams hp{};
     DevParams *dparams = nullptr;
     bool params_valid = false;
@@ -144,6 +146,10 @@ struct atg_ctx {
     std::vector<int> fstatus;         // per frame of the last call: ATG_OK, 1 (point capacity), ATG_ERR_LIMIT
     std::vector<double> stage_ms;
     std::vector<const char *> stage_names;
+    // atg_detect_noalloc: this device's static scratch, and pinned staging
+    // for its descriptors and results (both set up at creation)
+    uint8_t *scratch = nullptr;
+    uint8_t *na_stage = nullptr;
 };
 
 namespace {
@@ -185,12 +191,17 @@ void build_tables(const at_family &f, std::vector<uint32_t> &off, std::vector<ui
     *cap = capacity;
 }
 
-int upload_params(atg_ctx *ctx, const atg_params *p) {
+// The device parameters of a call (host computation only, nothing
+// allocated): the detector parameters, min_tag_width, the blur kernel and
+// host-libm constants as the reference computes them, and each family's
+// geometry with the device addresses of its decode tables (resident since
+// atg_set_families).
+int fill_params(atg_ctx *ctx, const atg_params *p, DevParams *out) {
     // (the reference selects the (max_nmaxima+1)-th largest maximum,
     // fast/quads.c:826-855: a negative value reads outside its array)
     if (p->max_nmaxima < 0 || p->max_nmaxima > kMaxMaxima)
         return fail(ctx, ATG_ERR_ARG, "max_nmaxima outside 0..32 (the corner search keeps at most 32 maxima)");
-    DevParams &h = ctx->hp;
+    DevParams &h = *out;
     memset(&h, 0, sizeof h);
     h.decimate = p->quad_decimate;
     h.sigma = p->quad_sigma;
@@ -225,15 +236,13 @@ int upload_params(atg_ctx *ctx, const atg_params *p) {
     if (ksz > 63) return fail(ctx, ATG_ERR_ARG, "quad_sigma too large (blur kernel over 63 taps)");
     h.blur_ksz = ksz > 1 ? (int32_t)ksz : 0;
     if (ksz > 1) {
-        double sigma = fabsf(p->quad_sigma);
-        std::vector<double> dk((size_t)ksz);
+        double sigma = fabsf(p->quad_sigma), dk[64], acc = 0.0;
         for (int64_t i = 0; i < ksz; i++) {
             double t = (double)(-(ksz / 2) + i) / sigma;
-            dk[(size_t)i] = exp(-0.5 * (t * t));
+            dk[i] = exp(-0.5 * (t * t));
         }
-        double acc = 0.0;
-        for (int64_t i = 0; i < ksz; i++) acc += dk[(size_t)i];
-        for (int64_t i = 0; i < ksz; i++) h.blur_k[i] = (uint8_t)(dk[(size_t)i] / acc * 255.0);
+        for (int64_t i = 0; i < ksz; i++) acc += dk[i];
+        for (int64_t i = 0; i < ksz; i++) h.blur_k[i] = (uint8_t)(dk[i] / acc * 255.0);
     }
     // qsm_kernel_init (fast/quads.c:740-745), make_detection's rotations
     // (fast/detector.c:85-86): host libm, as the reference
@@ -243,53 +252,65 @@ int upload_params(atg_ctx *ctx, const atg_params *p) {
         h.rot_cs[r][0] = cos(th);
         h.rot_cs[r][1] = sin(th);
     }
-    // families: geometry and table pointers into dtables
-    size_t at = 0;
-    std::vector<uint8_t> blob;
     for (size_t i = 0; i < ctx->fams.size(); i++) {
         const at_family &f = ctx->fams[i];
         DevFamily &d = h.fam[i];
-        std::vector<uint32_t> off;
-        std::vector<uint16_t> ids;
-        std::vector<uint64_t> codes;
-        int64_t cs, cap;
-        build_tables(f, off, ids, codes, &cs, &cap);
         d.nbits = f.nbits;
         d.wab = f.width_at_border;
         d.tw = f.total_width;
         d.reversed = f.reversed_border ? 1 : 0;
         d.maxhamming = ctx->bits[i];
-        d.chunk_size = (int32_t)cs;
+        d.chunk_size = (int32_t)((f.nbits + 3) / 4);
         d.ncodes = (int32_t)f.ncodes;
-        d.capacity = cap;
+        d.capacity = (int64_t)1 << d.chunk_size;
         for (int k = 0; k < 64; k++) {
             d.bit_x[k] = f.bit_x[k];
             d.bit_y[k] = f.bit_y[k];
         }
+        d.offsets = reinterpret_cast<const uint32_t *>(ctx->dtables + ctx->tab_off[i][0]);
+        d.ids = reinterpret_cast<const uint16_t *>(ctx->dtables + ctx->tab_off[i][1]);
+        d.codes = reinterpret_cast<const uint64_t *>(ctx->dtables + ctx->tab_off[i][2]);
+    }
+    return ATG_OK;
+}
+
+// The decode tables of the context's families into HBM, where they stay
+// until the families change (set-up time: this allocates).
+int upload_tables(atg_ctx *ctx) {
+    size_t at = 0;
+    std::vector<uint8_t> blob;
+    for (size_t i = 0; i < ctx->fams.size(); i++) {
+        std::vector<uint32_t> off;
+        std::vector<uint16_t> ids;
+        std::vector<uint64_t> codes;
+        int64_t cs, cap;
+        build_tables(ctx->fams[i], off, ids, codes, &cs, &cap);
         size_t o_off = align256(at), o_ids = align256(o_off + off.size() * 4), o_codes = align256(o_ids + ids.size() * 2);
         at = o_codes + codes.size() * 8;
         blob.resize(align256(at));
         memcpy(blob.data() + o_off, off.data(), off.size() * 4);
         memcpy(blob.data() + o_ids, ids.data(), ids.size() * 2);
         memcpy(blob.data() + o_codes, codes.data(), codes.size() * 8);
-        d.offsets = reinterpret_cast<const uint32_t *>(o_off);  // rebased below
-        d.ids = reinterpret_cast<const uint16_t *>(o_ids);
-        d.codes = reinterpret_cast<const uint64_t *>(o_codes);
+        ctx->tab_off[i][0] = o_off;
+        ctx->tab_off[i][1] = o_ids;
+        ctx->tab_off[i][2] = o_codes;
     }
+    HIPCHK(hipSetDevice(ctx->device));
+    if (ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->dtables) HIPCHK(hipFree(ctx->dtables));
     ctx->dtables = nullptr;
     if (!blob.empty()) {
         HIPCHK(hipMalloc(&ctx->dtables, blob.size()));
         HIPCHK(hipMemcpy(ctx->dtables, blob.data(), blob.size(), hipMemcpyHostToDevice));
     }
-    for (size_t i = 0; i < ctx->fams.size(); i++) {
-        DevFamily &d = h.fam[i];
-        d.offsets = reinterpret_cast<const uint32_t *>(ctx->dtables + reinterpret_cast<size_t>(d.offsets));
-        d.ids = reinterpret_cast<const uint16_t *>(ctx->dtables + reinterpret_cast<size_t>(d.ids));
-        d.codes = reinterpret_cast<const uint64_t *>(ctx->dtables + reinterpret_cast<size_t>(d.codes));
-    }
+    return ATG_OK;
+}
+
+int upload_params(atg_ctx *ctx, const atg_params *p) {
+    int rc = fill_params(ctx, p, &ctx->hp);
+    if (rc) return rc;
     if (!ctx->dparams) HIPCHK(hipMalloc(&ctx->dparams, sizeof(DevParams)));
-    HIPCHK(hipMemcpy(ctx->dparams, &h, sizeof h, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->dparams, &ctx->hp, sizeof(DevParams), hipMemcpyHostToDevice));
     ctx->last_params = *p;
     ctx->params_valid = true;
     return ATG_OK;
@@ -390,6 +411,49 @@ extern "C" void atg_params_default(atg_params *p) {
     p->deglitch = false;
 }
 
+// ------------------------------------------------------------------
+// The scratch of atg_detect_noalloc: every buffer of its one-frame call
+// (the frame descriptor, the work lists' header, the parameters, the
+// counters and first detections, the frame's arena region) lies in this
+// array. Each GPU holds its own copy for the life of the process. Host
+// code never passes the array's name to a kernel or a copy: its device
+// address on each device is looked up once (hipGetSymbolAddress, with that
+// device current) and kept.
+static __device__ __align__(256) uint8_t scratch[ATG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch[kMaxDevices];
+
+// The offsets of the scratch's parts (256-byte aligned): frame descriptor,
+// work lists, parameters, result block, then the frame's arena region.
+constexpr size_t kNaFrame = 0;
+constexpr size_t kNaQueue = (sizeof(Frame) + 255) & ~(size_t)255;
+constexpr size_t kNaParams = kNaQueue + ((sizeof(WorkQueue) + 255) & ~(size_t)255) + kLists * 256;
+constexpr size_t kNaResult = kNaParams + ((sizeof(DevParams) + 255) & ~(size_t)255);
+constexpr size_t kNaArena = kNaResult + ((kResultBlock + 255) & ~(size_t)255);
+// pinned staging: the same head (descriptor, work lists, parameters, result)
+constexpr size_t kNoAllocStage = kNaArena;
+static_assert(kNaQueue + ((sizeof(WorkQueue) + 255) & ~(size_t)255) >= kNaQueue + 256, "queue header");
+
+// The device address of this device's scratch (set-up: once per device,
+// with the context's device current).
+int noalloc_setup(atg_ctx *ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return fail(ctx, ATG_ERR_ARG, "device ordinal over 63");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[ctx->device]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch[ctx->device] = static_cast<uint8_t *>(ptr);
+    }
+    ctx->scratch = g_scratch[ctx->device];
+    return ATG_OK;
+}
+
+}  // namespace
+
 extern "C" int atg_ctx_create(int device, atg_ctx **out) {
     *out = nullptr;
     atg_ctx *ctx = new atg_ctx();
@@ -406,6 +470,10 @@ extern "C" int atg_ctx_create(int device, atg_ctx **out) {
         int khz = 0;
         HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
         ctx->clock_khz = khz > 0 ? khz : 100000.0;
+        int rc = noalloc_setup(ctx);
+        if (rc) return rc;
+        HIPCHK(hipHostMalloc(&ctx->na_stage, kNoAllocStage));
+        warm_persistent_grids(ctx->tune.fit_blocks);
         return ATG_OK;
     }();
     if (!ctx->err.empty()) {
@@ -429,6 +497,7 @@ extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
     hipFree(ctx->dparams);
     hipFree(ctx->marks.stamps);
     hipHostFree(ctx->hstamps);
+    hipHostFree(ctx->na_stage);
     hipHostFree(ctx->pinned);
     hipHostFree(ctx->pinned_out);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
@@ -456,10 +525,98 @@ extern "C" int atg_set_families(atg_ctx *ctx, const at_family *const *fams, cons
         ctx->bits.push_back(bits[i]);
     }
     ctx->params_valid = false;
-    return ATG_OK;
+    return upload_tables(ctx);
 }
 
 namespace atg {
+
+// The plan of one frame: its sizes, point capacity and the bytes of its
+// arena region (a frame the reference returns nothing for becomes an 8x8
+// zero frame).
+int plan_frame(atg_ctx *ctx, const atg_params *p, int nfam, const atg_frame &f, int on_device, FramePlan &fp) {
+    fp = FramePlan{};
+    fp.W = f.width;
+    fp.H = f.height;
+    fp.stride = f.stride;
+    fp.skip = nfam == 0 || fp.W < 8 || fp.H < 8 || !f.pixels;
+    decimated_size(fp.W, fp.H, p->quad_decimate, &fp.qw, &fp.qh);
+    if (!fp.skip && (fp.qw < 4 || fp.qh < 4)) fp.skip = true;
+    if (fp.skip) { fp.qw = fp.qh = 4; fp.W = fp.H = 8; }
+    if (fp.qw > 32767 || fp.qh > 32767) return fail(ctx, ATG_ERR_LIMIT, "quad image wider or taller than 32767");
+    const int64_t nodes = fp.qw * fp.qh + fp.qh;
+    if (nodes >= ((int64_t)1 << 24)) return fail(ctx, ATG_ERR_LIMIT, "quad image over 2^24 pixels");
+    fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
+    fp.force = ctx->force_scratch;
+    Frame F{};
+    fp.size = layout(fp, nfam, !on_device, &F, nullptr);
+    return ATG_OK;
+}
+
+// The launch shapes of a call, from its frames' plans.
+struct ChainShape {
+    int max_qw = 4, max_qh = 4, key_bits = 0, cl_bits = 0, max_lds_qw = 0;
+    int64_t max_weight = 0;
+    int max_cluster = 0;
+    void add(const FramePlan &fp) {
+        // (the sizing layout decides the front's memory as layout() does)
+        if (front_scratch_bytes((int)fp.qw, fp.force) == 0) max_lds_qw = std::max<int>(max_lds_qw, (int)fp.qw);
+        const int64_t nodes = fp.qw * fp.qh + fp.qh;
+        max_weight = std::max<int64_t>(max_weight, nodes);
+        max_qw = std::max<int>(max_qw, (int)fp.qw);
+        max_qh = std::max<int>(max_qh, (int)fp.qh);
+        key_bits = std::max(key_bits, ceil_log2((uint64_t)cluster_map_size(fp.qw, fp.qh)) + 2 * ceil_log2((uint64_t)nodes));
+        cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
+        max_cluster = std::max(max_cluster, (int)(2 * (2 * fp.qw + 2 * fp.qh)));
+    }
+};
+
+// The descriptor of a planned frame whose arena region starts at `region`
+// and whose counters and first detections go to `result` (the image
+// pointer is set by the caller).
+void describe_frame(const DevParams &hp, const FramePlan &fp, int nfam, bool host_input, int key_bits,
+                    uint8_t *region, uint8_t *result, Frame &F) {
+    F = Frame{};
+    layout(fp, nfam, host_input, &F, region);
+    F.W = (int32_t)fp.W;
+    F.H = (int32_t)fp.H;
+    F.qw = (int32_t)fp.qw;
+    F.qh = (int32_t)fp.qh;
+    F.tw = (int32_t)(fp.qw / 4);
+    F.th = (int32_t)(fp.qh / 4);
+    F.nbits_node = ceil_log2((uint64_t)(fp.qw * fp.qh + fp.qh));
+    F.bucket_mask = (uint64_t)cluster_map_size(fp.qw, fp.qh) - 1;
+    F.key_bits = key_bits;
+    F.max_px = (int32_t)(2 * (2 * fp.qw + 2 * fp.qh));
+    F.blurred = hp.blur_ksz > 1 ? 1 : 0;
+    F.pcap = fp.pcap;
+    F.counters = reinterpret_cast<uint32_t *>(result);
+    F.inl = reinterpret_cast<at_detection *>(result + kCountersBytes);
+    if (fp.skip) {
+        // an 8x8 zero frame (k_init clears it): no tag can come out of it
+        F.skip = 1;
+        F.stride = 8;
+    }
+}
+
+// The kernel chain of a call over n described frames (dframes, dparams
+// and the work lists' header dq in device memory), on stream st.
+void launch_chain(hipStream_t st, const ChainShape &cs, const CclPlan &ccl, const DevParams &hp, bool deglitch,
+                  const Frame *dframes, const DevParams *dparams, int n, WorkQueue *dq, const Tune &tu, bool force,
+                  Marks *mk) {
+    // the angle sort in LDS per cluster, or (clusters too large) of all points in HBM
+    const int sort_cap = angle_sort_cap(cs.max_cluster, force);
+    // the corner search of the small clusters a wave each, when its pair
+    // tables hold the maxima kept
+    const bool wave_corners = hp.max_nmaxima <= 16;
+    const bool blur = hp.blur_ksz > 1;
+    launch_init(st, dframes, n, dq);
+    mk->mark(st, "init");
+    launch_image_stage(st, dframes, dparams, n, cs.max_qw, cs.max_qh, blur, deglitch, tu, mk);
+    launch_ccl_stage(st, dframes, dparams, n, cs.max_qw, cs.max_qh, ccl, dq, tu, mk);
+    launch_points_stage(st, dframes, n, cs.max_qh, cs.key_bits, tu, mk);
+    launch_quads_stage(st, dframes, dparams, n, cs.cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
+    launch_decode_stage(st, dframes, dparams, n, tu.fit_blocks, hp.refine != 0, dq, mk);
+}
 
 // The first half of a call: plans, arena, frame descriptors, the kernel
 // chain and the readback copy, all queued on the context's stream; the
@@ -475,40 +632,20 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     ctx->plans.assign((size_t)n, FramePlan{});
     ctx->hframes.assign((size_t)n, Frame{});
     size_t total = 0, pinned_need = 0;
-    int max_qw = 4, max_qh = 4, key_bits = 0, cl_bits = 0;
-    int max_lds_qw = 0;
-    int64_t max_weight = 0;
+    ChainShape cs;
     for (int32_t i = 0; i < n; i++) {
         FramePlan &fp = ctx->plans[(size_t)i];
-        fp.W = frames[i].width;
-        fp.H = frames[i].height;
-        fp.stride = frames[i].stride;
-        fp.skip = nfam == 0 || fp.W < 8 || fp.H < 8 || !frames[i].pixels;
-        decimated_size(fp.W, fp.H, p->quad_decimate, &fp.qw, &fp.qh);
-        if (!fp.skip && (fp.qw < 4 || fp.qh < 4)) fp.skip = true;
-        if (fp.skip) { fp.qw = fp.qh = 4; fp.W = fp.H = 8; }
-        if (fp.qw > 32767 || fp.qh > 32767) return fail(ctx, ATG_ERR_LIMIT, "quad image wider or taller than 32767");
-        int64_t nodes = fp.qw * fp.qh + fp.qh;
-        if (nodes >= ((int64_t)1 << 24)) return fail(ctx, ATG_ERR_LIMIT, "quad image over 2^24 pixels");
-        fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
-        fp.force = ctx->force_scratch;
-        Frame F{};
-        fp.size = layout(fp, nfam, !on_device, &F, nullptr);
-        // (the sizing layout has no base, so its pointers are all null: the
-        // front's memory is decided here as layout() decides it)
-        if (front_scratch_bytes((int)fp.qw, fp.force) == 0) max_lds_qw = std::max<int>(max_lds_qw, (int)fp.qw);
-        max_weight = std::max<int64_t>(max_weight, fp.qw * fp.qh + fp.qh);
+        int rc = plan_frame(ctx, p, nfam, frames[i], on_device, fp);
+        if (rc) return rc;
+        cs.add(fp);
         fp.off = total;
         total += fp.size;
         if (!on_device && !fp.skip) pinned_need += align256((size_t)(fp.W * fp.H));
-        max_qw = std::max<int>(max_qw, (int)fp.qw);
-        max_qh = std::max<int>(max_qh, (int)fp.qh);
-        int nb = ceil_log2((uint64_t)nodes);
-        key_bits = std::max(key_bits, ceil_log2((uint64_t)cluster_map_size(fp.qw, fp.qh)) + 2 * nb);
-        cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
     }
+    const int max_qw = cs.max_qw, max_qh = cs.max_qh, key_bits = cs.key_bits, cl_bits = cs.cl_bits;
+    const int max_lds_qw = cs.max_lds_qw;
     // the union-find's launch shapes (its HBM state is in each frame's region)
-    const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, max_weight, ctx->force_scratch, (int)n);
+    const CclPlan ccl = ccl_plan(max_lds_qw, max_qw, cs.max_weight, ctx->force_scratch, (int)n);
     const size_t qoff = total;
     total += queue_bytes(n);
     if (total > ctx->arena_size && ctx->arena_fixed) return fail(ctx, ATG_ERR_LIMIT, "batch over the arena budget");
@@ -569,25 +706,9 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     for (int32_t i = 0; i < n; i++) {
         FramePlan &fp = ctx->plans[(size_t)i];
         Frame &F = ctx->hframes[(size_t)i];
-        layout(fp, nfam, !on_device, &F, ctx->arena + fp.off);
-        F.W = (int32_t)fp.W;
-        F.H = (int32_t)fp.H;
-        F.qw = (int32_t)fp.qw;
-        F.qh = (int32_t)fp.qh;
-        F.tw = (int32_t)(fp.qw / 4);
-        F.th = (int32_t)(fp.qh / 4);
-        F.nbits_node = ceil_log2((uint64_t)(fp.qw * fp.qh + fp.qh));
-        F.bucket_mask = (uint64_t)cluster_map_size(fp.qw, fp.qh) - 1;
-        F.key_bits = key_bits;
-        F.max_px = (int32_t)(2 * (2 * fp.qw + 2 * fp.qh));
-        F.blurred = ctx->hp.blur_ksz > 1 ? 1 : 0;
-        F.pcap = fp.pcap;
-        F.counters = reinterpret_cast<uint32_t *>(ctx->results + kResultBlock * (size_t)i);
-        F.inl = reinterpret_cast<at_detection *>(ctx->results + kResultBlock * (size_t)i + kCountersBytes);
+        describe_frame(ctx->hp, fp, nfam, !on_device, key_bits, ctx->arena + fp.off,
+                       ctx->results + kResultBlock * (size_t)i, F);
         if (fp.skip) {
-            // an 8x8 zero frame (k_init clears it): no tag can come out of it
-            F.skip = 1;
-            F.stride = 8;
         } else if (on_device) {
             F.img = frames[i].pixels;
             F.stride = fp.stride;
@@ -611,12 +732,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     Marks *mk = &ctx->marks;
     mk->on = ctx->profiling;
     const Tune tu = ctx->tune;
-    // the angle sort in LDS per cluster, or (clusters too large) of all points in HBM
-    int max_cluster = 0;
-    for (int32_t i = 0; i < n; i++) max_cluster = std::max(max_cluster, ctx->hframes[(size_t)i].max_px);
-    const int sort_cap = angle_sort_cap(max_cluster, ctx->force_scratch);
-    // the corner search of the small clusters a wave each, when its pair
-    // tables hold the maxima kept
+    const int sort_cap = angle_sort_cap(cs.max_cluster, ctx->force_scratch);
     const bool wave_corners = ctx->hp.max_nmaxima <= 16;
     const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
     warm_persistent_grids(tu.fit_blocks);  // device queries stay out of stream capture
@@ -626,13 +742,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
             hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return e;
         }
-        launch_init(st, ctx->dframes, n, dq);
-        mk->mark(st, "init");
-        launch_image_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, blur, deglitch, tu, mk);
-        launch_ccl_stage(st, ctx->dframes, ctx->dparams, n, max_qw, max_qh, ccl, dq, tu, mk);
-        launch_points_stage(st, ctx->dframes, n, max_qh, key_bits, tu, mk);
-        launch_quads_stage(st, ctx->dframes, ctx->dparams, n, cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
-        launch_decode_stage(st, ctx->dframes, ctx->dparams, n, tu.fit_blocks, ctx->hp.refine != 0, dq, mk);
+        launch_chain(st, cs, ccl, ctx->hp, deglitch, ctx->dframes, ctx->dparams, n, dq, tu, ctx->force_scratch, mk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (mk->n > 0) {
@@ -841,6 +951,86 @@ extern "C" int atg_detect(atg_ctx *ctx, const atg_params *p, const uint8_t *pixe
                           int64_t stride, int on_device, atg_result *out) {
     atg_frame f{pixels, width, height, stride};
     return atg_detect_batch(ctx, p, &f, 1, on_device, out);
+}
+
+// One frame with every buffer in the device's static scratch: no device,
+// pinned or host allocation (the decode tables are resident since
+// atg_set_families; descriptors go through the context's pinned staging).
+extern "C" int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8_t *pixels, int64_t width,
+                                  int64_t height, int64_t stride, int on_device, at_detection *dets, int64_t cap,
+                                  int64_t *ndets, int64_t *nquads) {
+    if (!ctx || !p || !ndets || !nquads || cap < 0 || (cap > 0 && !dets)) return ATG_ERR_ARG;
+    ctx->err.clear();
+    *ndets = 0;
+    *nquads = 0;
+    const int nfam = (int)ctx->fams.size();
+    // no families, or under 8x8: nothing (fast/detector.c:254-255)
+    if (nfam == 0 || width < 8 || height < 8 || !pixels) return ATG_OK;
+    if (stride < width) return fail(ctx, ATG_ERR_ARG, "stride under the width");
+    if (!ctx->scratch || !ctx->na_stage) return fail(ctx, ATG_ERR_HIP, "no device scratch on this context");
+    HIPCHK(hipSetDevice(ctx->device));
+    uint8_t *stage = ctx->na_stage;
+    DevParams &hp = *reinterpret_cast<DevParams *>(stage + kNaParams);
+    int rc = fill_params(ctx, p, &hp);
+    if (rc) return rc;
+    const atg_frame f{pixels, width, height, stride};
+    hipStream_t st = ctx->stream;
+    const int64_t saved_scale = ctx->pcap_scale;
+    for (int64_t scale = 1; scale <= 4; scale *= 4) {
+        ctx->pcap_scale = scale;
+        FramePlan fp;
+        rc = plan_frame(ctx, p, nfam, f, on_device, fp);
+        ctx->pcap_scale = saved_scale;
+        if (rc) return rc;
+        if (fp.skip) return ATG_OK;  // a quad image under 4x4: nothing
+        if (kNaArena + fp.size > ATG_NOALLOC_SCRATCH_BYTES) {
+            char m[200];
+            snprintf(m, sizeof m, "the frame needs %zu bytes of device scratch; atg_detect_noalloc holds %zu",
+                     kNaArena + fp.size, (size_t)ATG_NOALLOC_SCRATCH_BYTES);
+            return fail(ctx, ATG_ERR_LIMIT, m);
+        }
+        ChainShape cs;
+        cs.add(fp);
+        const CclPlan ccl = ccl_plan(cs.max_lds_qw, cs.max_qw, cs.max_weight, ctx->force_scratch, 1);
+        uint8_t *base = ctx->scratch;
+        Frame &F = *reinterpret_cast<Frame *>(stage + kNaFrame);
+        describe_frame(hp, fp, nfam, !on_device, cs.key_bits, base + kNaArena, base + kNaResult, F);
+        F.stride = on_device ? stride : width;
+        if (on_device) F.img = pixels;
+        WorkQueue &q = *reinterpret_cast<WorkQueue *>(stage + kNaQueue);
+        memset(&q, 0, sizeof q);
+        q.nframes = 1;
+        for (int l = 0; l < kLists; l++)
+            q.off[l] = reinterpret_cast<uint32_t *>(base + kNaQueue + align256(sizeof(WorkQueue)) + (size_t)l * 256);
+        if (!on_device)
+            HIPCHK(hipMemcpy2DAsync(const_cast<uint8_t *>(F.img), (size_t)width, pixels, (size_t)stride, (size_t)width,
+                                    (size_t)height, hipMemcpyHostToDevice, st));
+        // descriptor, work lists' header and parameters in one copy
+        HIPCHK(hipMemcpyAsync(base, stage, kNaResult, hipMemcpyHostToDevice, st));
+        Marks off;
+        launch_chain(st, cs, ccl, hp, p->deglitch, reinterpret_cast<const Frame *>(base + kNaFrame),
+                     reinterpret_cast<const DevParams *>(base + kNaParams), 1,
+                     reinterpret_cast<WorkQueue *>(base + kNaQueue), ctx->tune, ctx->force_scratch, &off);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(stage + kNaResult, base + kNaResult, kResultBlock, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const uint32_t *cnt = reinterpret_cast<const uint32_t *>(stage + kNaResult);
+        if (cnt[kError] & 2u) continue;  // more points than pixels: again with the worst-case capacity
+        if (cnt[kError]) {
+            char m[96];
+            snprintf(m, sizeof m, "device capacity exceeded (flags 0x%x)", cnt[kError]);
+            return fail(ctx, ATG_ERR_LIMIT, m);
+        }
+        const int64_t nd = cnt[kNDets], take = std::min(nd, cap), first = std::min<int64_t>(take, kInlineDets);
+        *ndets = nd;
+        *nquads = cnt[kNQuads];
+        if (first) memcpy(dets, stage + kNaResult + kCountersBytes, (size_t)first * sizeof(at_detection));
+        if (take > first)
+            HIPCHK(hipMemcpy(dets + first, F.dtmp + first, (size_t)(take - first) * sizeof(at_detection),
+                             hipMemcpyDeviceToHost));
+        return ATG_OK;
+    }
+    return fail(ctx, ATG_ERR_LIMIT, "point capacity exceeded");
 }
 
 extern "C" int atg_device_alloc(atg_ctx *ctx, size_t bytes, void **out) {

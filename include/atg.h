@@ -168,6 +168,27 @@ int atg_set_profiling(atg_ctx *ctx, int on);
  * value (at most cap are written). */
 int atg_stage_times(const atg_ctx *ctx, const char **names, double *ms, int32_t cap);
 
+/* Bytes of the static device scratch of atg_detect_noalloc. Every GPU the
+ * library runs on holds one such array until the process ends. */
+#define ATG_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/* No counterpart in the reference: atg_detect for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid hipMalloc). One
+ * frame (on_device != 0: pixels is a device pointer); every buffer of the
+ * call is the library's static device scratch of ATG_NOALLOC_SCRATCH_BYTES,
+ * so the call allocates nothing: no device, pinned or host memory (the
+ * decode tables are put on the device by atg_set_families, the context's
+ * pinned staging by atg_ctx_create). A frame whose buffers do not fit
+ * (a 1920x1080 frame at decimate 2 takes about 110 MB) is refused with
+ * ATG_ERR_LIMIT. The detections, sorted by id as atg_detect's, go to
+ * dets: *ndets is the frame's count, of which at most cap are written;
+ * *nquads the quad count. All calls on one GPU share its scratch, which
+ * keeps the last call's contents (the chain clears what it counts):
+ * calls that overlap on one GPU, from any contexts, race. */
+int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8_t *pixels, int64_t width, int64_t height,
+                       int64_t stride, int on_device, at_detection *dets, int64_t cap, int64_t *ndets,
+                       int64_t *nquads);
+
 /* Device buffers on the context's device, for callers without their own
  * allocator (frames kept resident in HBM between calls). */
 int atg_device_alloc(atg_ctx *ctx, size_t bytes, void **out);
