@@ -73,7 +73,9 @@ class SynthTag(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("thr_tile_rows", C.c_int32), ("row_waves", C.c_int32), ("front_threads", C.c_int32),
-                ("fit_blocks", C.c_int32)]
+                ("fit_blocks", C.c_int32), ("front_bands", C.c_int32)]
+
+TUNE_FIELDS = [f for f, _ in Tune._fields_]
 
 
 class ServerConfig(C.Structure):
@@ -309,10 +311,11 @@ class Context:
     def get_tune(self):
         t = Tune()
         self._check(self.L.atg_get_tune(self.h, C.byref(t)), "atg_get_tune")
-        return (t.thr_tile_rows, t.row_waves, t.front_threads, t.fit_blocks)
+        return tuple(getattr(t, f) for f in TUNE_FIELDS)
 
-    def set_tune(self, thr_tile_rows, row_waves, front_threads, fit_blocks):
-        t = Tune(thr_tile_rows, row_waves, front_threads, fit_blocks)
+    def set_tune(self, thr_tile_rows, row_waves, front_threads, fit_blocks, front_bands=0):
+        """front_bands 0: the union-find front's row bands chosen by the call's size."""
+        t = Tune(thr_tile_rows, row_waves, front_threads, fit_blocks, front_bands)
         self._check(self.L.atg_set_tune(self.h, C.byref(t)), "atg_set_tune")
 
     def set_graphs(self, on=True):
@@ -326,7 +329,7 @@ class Context:
         ms = C.c_double()
         self._check(self.L.atg_autotune(self.h, C.byref(p), fr, len(imgs), 1 if on_device else 0, C.byref(t),
                                         C.byref(ms)), "atg_autotune")
-        return (t.thr_tile_rows, t.row_waves, t.front_threads, t.fit_blocks), ms.value
+        return tuple(getattr(t, f) for f in TUNE_FIELDS), ms.value
 
     def copy_bandwidth(self, nbytes=1 << 29, reps=3):
         """A plain copy kernel over nbytes, reps times: (GB/s moved, ms each)."""

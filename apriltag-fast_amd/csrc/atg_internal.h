@@ -208,12 +208,15 @@ struct Tune {
                             // (their launch bounds are 256 threads)
     int32_t front_threads;  // k_front block: 128, 256, 512
     int32_t fit_blocks;     // blocks of the persistent fit and decode kernels: 32..4096
+    int32_t front_bands;    // row bands of a frame's union-find front at most (1: one sweep; 0: by the
+                            // call's size)
 };
 
 // The replay kernels' memory plan for one call (LDS where it fits, else
 // HBM scratch).
 struct CclPlan {
     int front;                  // LDS bytes of the front (frames without fscr)
+    int front_band;             // LDS bytes of the split front's kernels, 0 = not used
     int replay;                 // LDS bytes of the replay
     int replay_cap;             // runs per row replayed in LDS, 0 = none
     bool replay_pack;           // its elements packed (parent, weight) in one word

@@ -209,6 +209,7 @@ __device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, ui
 constexpr int kStage = 1024;
 
 size_t front_bytes(int qw) { return (size_t)48 * qw + 128; }  // two sets of three [2 qw + 2] arrays
+size_t front_band_bytes(int qw) { return (size_t)68 * qw + 128; }  // and the ports' least ports and records
 
 // A barrier of the whole block over LDS only (the scratch variant orders
 // HBM too).
@@ -223,11 +224,31 @@ __device__ __forceinline__ void bsync() {
     }
 }
 
-template <bool G, int kFrontThreads>
-__device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) {
+// How a sweep starts and what it writes. The whole frame (kSweepFull)
+// starts from row 0, whose runs are each their own set. A band of rows
+// y0..y1 of a split frame starts from row y0 - 1: as ports (kSweepLocal),
+// or from row y0 - 1's final records and weights (kSweepFinal).
+//
+// kSweepLocal: each run of row y0 - 1 (a port) is a set of weight 0; the
+// band's sets that hold ports are its port components, each named by its
+// least port. The sweep keeps each set's least port (as its least run)
+// and follows each port's record row by row; a component ends either in
+// row y1 (live) or earlier (its set has no run in the next row). It
+// writes, at the port row's slots: rlost[slot + x] = the component of port
+// x; rroot[slot + g] for each component g = kPortLive | its record in row
+// y1, or its weight (its pixels in the band) if it ended earlier.
+enum { kSweepFull = 0, kSweepLocal = 1, kSweepFinal = 2 };
+constexpr uint32_t kPortEnded = 0x80000000u;  // (port tracking) the port's component ended: | its component
+constexpr uint32_t kPortLive = 0x80000000u;   // (component record) reaches row y1: | its record there
+
+// The sweep of rows y0..y1 (kSweepFull: 0..QH-1). Writes rlab/recw of
+// rows wlo..y1 (kSweepLocal: row y1 only, its sets without what lies above
+// the band); row_off and the virtual nodes of rows olo..ohi (the last
+// band also row_off[QH] and the run count).
+template <bool G, int kFrontThreads, int mode>
+__device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, int y0, int y1, int olo, int ohi) {
     constexpr int kPer = kStage / kFrontThreads;  // prefetched runs and links per thread
     using U32 = typename Mem<G>::U32;
-    (void)lds_small;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int QW = F.qw, QH = F.qh;
     // elements of row y: the records of row y-1 at [0, QW) (each named by
@@ -245,35 +266,62 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
     U32 *pp = minrun + EM;
     U32 *pacc = pp + EM;
     U32 *pmin = pacc + EM;
-    // row offsets: exclusive scan of the per-row run counts (wave 0)
+    // kSweepLocal: the least port of each set (two sets, as minrun), and
+    // each port's record in the last settled row (or kPortEnded | component)
+    U32 *mp = pmin + EM;
+    U32 *pmp = mp + EM;
+    U32 *port = pmp + EM;
+    const int yp = mode == kSweepFull ? 0 : y0 - 1;  // the row the sweep starts from
+    const int wlo = mode == kSweepFull ? 0 : mode == kSweepFinal ? y0 : QH;  // rows written in the loop
+    // row offsets: exclusive scan of the per-row run counts (wave 0); the
+    // base of row yp for the sweep
     if (wave == 0) {
         uint32_t carry = 0;
-        for (int y0 = 0; y0 < QH; y0 += 64) {
-            int y = y0 + lane;
+        for (int y00 = 0; y00 < QH; y00 += 64) {
+            int y = y00 + lane;
             uint32_t c = y < QH ? F.rcnt[y] : 0u;
             uint32_t inc = wave_incl_scan(c);
-            if (y < QH) F.row_off[y] = carry + inc - c;
+            if (y < QH && y >= olo && y <= ohi) F.row_off[y] = carry + inc - c;
+            if (y == yp) lds_small[0] = carry + inc - c;
             carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
         }
-        if (lane == 0) {
+        if (lane == 0 && ohi == QH - 1) {
             F.row_off[QH] = carry;
             F.counters[kNRuns] = carry;
         }
-        for (int y = lane; y < QH; y += 64) {
+        for (int y = olo + lane; y <= ohi; y += 64) {
             F.parent[carry + y] = carry + y;
             F.fw[carry + y] = 1;
         }
     }
-    // row 0: each run is its own set (named by itself), as settled
-    int pcur = (int)F.rcnt[0];
+    __syncthreads();
+    uint32_t pbase = lds_small[0];  // row_off of the previous row
+    // the starting row, as settled
+    int pcur = (int)F.rcnt[yp];
     for (int i = tid; i < pcur; i += kFrontThreads) {
-        uint32_t r = F.runs[i];
-        pp[QW + i] = (uint32_t)(QW + i);
-        pmin[QW + i] = (uint32_t)i;
-        pacc[QW + i] = run_end(r) - run_start(r) + 1;
-        F.parent[i] = (uint32_t)i;
+        if (mode == kSweepFull) {
+            const uint32_t r = F.runs[i];
+            pp[QW + i] = (uint32_t)(QW + i);
+            pmin[QW + i] = (uint32_t)i;
+            pacc[QW + i] = run_end(r) - run_start(r) + 1;
+            F.parent[i] = (uint32_t)i;
+        } else if (mode == kSweepLocal) {
+            pp[QW + i] = (uint32_t)(QW + i);
+            pmin[QW + i] = (uint32_t)i;
+            pacc[QW + i] = 0u;
+            pmp[QW + i] = (uint32_t)i;
+            port[i] = (uint32_t)i;
+        } else {
+            const uint32_t l = F.rlab[pbase + i];
+            pp[QW + i] = (uint32_t)QW + l;
+            if (l == (uint32_t)i) {
+                pmin[QW + i] = (uint32_t)i;
+                pacc[QW + i] = F.recw[pbase + i];
+            }
+        }
     }
-    uint32_t pbase = 0;  // row_off of the previous row
+    const int nport = mode == kSweepLocal ? pcur : 0;
+    const uint32_t portslot = pbase;  // the port row's slots
     // the next row, loaded one row ahead: thread tid holds runs and links
     // tid + j kFrontThreads
     uint32_t nx_rc = 0, nx_ec = 0, nx_r[kPer], nx_e[kPer];
@@ -293,9 +341,9 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
         nx_t1 = trow[QW - 1];
         nx_t2 = trow[QW - 2];
     };
-    if (QH > 1) prefetch(1);
+    if (yp + 1 <= y1) prefetch(yp + 1);
     bsync<G>();
-    for (int y = 1; y < QH; y++) {
+    for (int y = yp + 1; y <= y1; y++) {
         const int ncur = (int)nx_rc, ne = (int)nx_ec;
         const uint8_t t1 = nx_t1, t2 = nx_t2;
         uint32_t rr[kPer], ee[kPer];
@@ -305,23 +353,42 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             ee[j] = nx_e[j];
         }
         const uint32_t cbase = pbase + (uint32_t)pcur;
-        if (y + 1 < QH) prefetch(y + 1);
+        if (y + 1 <= y1) prefetch(y + 1);
         const uint32_t *cur = F.runs + (int64_t)y * QW;
         const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
         // the previous row's outputs (each run's record, its first run,
         // and at a record's first run its weight), and the elements: the
         // records of row y-1 (weight 0 at the other slots, which no link
         // names), the runs of row y, the virtual node
+        const bool out = y - 1 >= wlo;
         for (int x = tid; x < pcur; x += kFrontThreads) {
             const uint32_t r = pp[QW + x];
             const uint32_t l = pmin[r];
             const uint32_t wgt = l == (uint32_t)x ? pacc[r] : 0u;
-            F.rlab[pbase + x] = l;
-            if (l == (uint32_t)x) F.recw[pbase + x] = wgt;
+            if (out) {
+                F.rlab[pbase + x] = l;
+                if (l == (uint32_t)x) F.recw[pbase + x] = wgt;
+            }
             p[x] = (uint32_t)x;
             acc[x] = wgt;
             minrun[x] = 0xffffffffu;
+            if (mode == kSweepLocal) mp[x] = l == (uint32_t)x ? pmp[r] : 0xffffffffu;
         }
+        // each port's record in row y-1 (from its record in row y-2), or
+        // the end of its component in row y-2
+        if (mode == kSweepLocal && y > yp + 1)
+            for (int x = tid; x < nport; x += kFrontThreads) {
+                const uint32_t q = port[x];
+                if (q & kPortEnded) continue;
+                const uint32_t r = pp[q], m = pmin[r];
+                if (m != 0xffffffffu) {
+                    port[x] = m;
+                } else {
+                    const uint32_t g = pmp[r];
+                    port[x] = kPortEnded | g;
+                    F.rroot[portslot + g] = pacc[r];  // (every port of the component writes the same)
+                }
+            }
         bool lastcol = false;  // fast/unionfind.c:93-97, on the thread of the last run
 #pragma unroll
         for (int j = 0; j < kPer; j++) {
@@ -331,6 +398,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
                 p[QW + i] = (uint32_t)(QW + i);
                 acc[QW + i] = run_end(r) - run_start(r) + 1;
                 minrun[QW + i] = 0xffffffffu;
+                if (mode == kSweepLocal) mp[QW + i] = 0xffffffffu;
                 if (i == ncur - 1) lastcol = (int)run_end(r) == QW - 2 && run_white(r) && t1 == 255 && t2 != 255;
             }
         }
@@ -339,12 +407,14 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             p[QW + i] = (uint32_t)(QW + i);
             acc[QW + i] = run_end(r) - run_start(r) + 1;
             minrun[QW + i] = 0xffffffffu;
+            if (mode == kSweepLocal) mp[QW + i] = 0xffffffffu;
             if (i == ncur - 1) lastcol = (int)run_end(r) == QW - 2 && run_white(r) && t1 == 255 && t2 != 255;
         }
         if (tid == 0) {
             p[VN] = VN;
             acc[VN] = 1;
             minrun[VN] = 0xffffffffu;
+            if (mode == kSweepLocal) mp[VN] = 0xffffffffu;
         }
         bsync<G>();
         // a link joins a current run and the record of a previous-row run
@@ -366,6 +436,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
             if (r != x) {
                 p[x] = r;
                 atomicAdd((uint32_t *)&acc[r], (uint32_t)acc[x]);  // acc of a non-root never changes
+                if (mode == kSweepLocal && !run) atomicMin((uint32_t *)&mp[r], (uint32_t)mp[x]);
             }
             if (run) atomicMin((uint32_t *)&minrun[r], x - (uint32_t)QW);
         };
@@ -381,13 +452,28 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small) 
         U32 *t0 = p; p = pp; pp = t0;
         U32 *t1a = acc; acc = pacc; pacc = t1a;
         U32 *t2m = minrun; minrun = pmin; pmin = t2m;
+        U32 *t3 = mp; mp = pmp; pmp = t3;
     }
-    // the last row's outputs
+    // the last row's outputs, and (kSweepLocal) each port's record in it
     for (int i = tid; i < pcur; i += kFrontThreads) {
         const uint32_t r = pp[QW + i];
         const uint32_t l = pmin[r];
         F.rlab[pbase + i] = l;
         if (l == (uint32_t)i) F.recw[pbase + i] = pacc[r];
+    }
+    if (mode == kSweepLocal) {
+        for (int x = tid; x < nport; x += kFrontThreads) {
+            const uint32_t q = port[x];  // its record in row y1 - 1 (row yp when y1 = yp + 1)
+            uint32_t g;
+            if (q & kPortEnded) {
+                g = q & ~kPortEnded;
+            } else {
+                const uint32_t r = pp[q], m = pmin[r];
+                g = pmp[r];
+                F.rroot[portslot + g] = m != 0xffffffffu ? kPortLive | m : pacc[r];
+            }
+            F.rlost[portslot + x] = g;
+        }
     }
 }
 
@@ -397,9 +483,127 @@ __global__ __launch_bounds__(T) void k_front(const Frame *frames) {
     __shared__ uint32_t small[T / 64];
     const Frame F = frames[blockIdx.y];
     if (F.fscr)
-        front_sweep<true, T>(F, F.fscr, small);
+        front_sweep<true, T, kSweepFull>(F, F.fscr, small, 0, F.qh - 1, 0, F.qh - 1);
     else
-        front_sweep<false, T>(F, lds, small);
+        front_sweep<false, T, kSweepFull>(F, lds, small, 0, F.qh - 1, 0, F.qh - 1);
+}
+
+constexpr int kBandRows = 16;  // rows of a band of the split front at least
+
+// Row bands of a frame for the split front: band k of nb holds rows
+// [k qh / nb, (k + 1) qh / nb); nb = qh / band_rows, at most max_bands
+// (one band: the whole-frame sweep). Frames whose front state is in HBM
+// are not split.
+__device__ __forceinline__ int front_bands(const Frame &F, int band_rows, int max_bands) {
+    if (F.fscr) return 1;
+    return max(1, min(max_bands, F.qh / max(band_rows, 2)));
+}
+__device__ __forceinline__ int band_start(const Frame &F, int k, int nb) { return (int)((int64_t)k * F.qh / nb); }
+
+// Step 1 of the split front: band 0 sweeps rows from 0 as the whole-frame
+// sweep does (its outputs are final); every other band sweeps its rows
+// from ports (kSweepLocal). A frame of one band is swept whole here.
+template <int T>
+__global__ __launch_bounds__(T) void k_front_local(const Frame *frames, int band_rows, int max_bands) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t small[T / 64];
+    const Frame F = frames[blockIdx.y];
+    const int nb = front_bands(F, band_rows, max_bands), k = (int)blockIdx.x;
+    if (k >= nb) return;
+    const int y0 = band_start(F, k, nb), y1 = band_start(F, k + 1, nb) - 1;
+    if (F.fscr)
+        front_sweep<true, T, kSweepFull>(F, F.fscr, small, 0, F.qh - 1, 0, F.qh - 1);
+    else if (k == 0)
+        front_sweep<false, T, kSweepFull>(F, lds, small, 0, y1, 0, y1);
+    else
+        front_sweep<false, T, kSweepLocal>(F, lds, small, y0, y1, y0, y1);
+}
+
+// Step 2: band by band (one block per frame), the final records of each
+// band's last row: the sets of the row above the band (final records and
+// weights), the band's port components and its sets in its last row
+// (records and weights without what lies above) joined. Elements: the
+// records above at [0, QW), the band's last-row records at QW + [0, QW),
+// the port components (by least port) at 2 QW + [0, QW).
+template <int T>
+__global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band_rows, int max_bands) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const Frame F = frames[blockIdx.y];
+    const int nb = front_bands(F, band_rows, max_bands);
+    if (nb < 2) return;
+    const uint32_t QW = (uint32_t)F.qw;
+    const int tid = threadIdx.x;
+    uint32_t *p = reinterpret_cast<uint32_t *>(lds);
+    uint32_t *w = p + 3 * QW;
+    uint32_t *mn = w + 3 * QW;
+    auto settle = [&](uint32_t x) {
+        const uint32_t r = fr_find<false>(p, x);
+        if (r != x) {
+            p[x] = r;
+            atomicAdd(&w[r], w[x]);  // (roots no longer change: a non-root's weight is read once)
+        }
+        return r;
+    };
+    for (int k = 1; k < nb; k++) {
+        const int r0 = band_start(F, k, nb) - 1, e = band_start(F, k + 1, nb) - 1;
+        const uint32_t b0 = F.row_off[r0], n0 = F.rcnt[r0], be = F.row_off[e], ne = F.rcnt[e];
+        for (uint32_t x = tid; x < n0; x += T) {
+            p[x] = x;
+            w[x] = F.rlab[b0 + x] == x ? F.recw[b0 + x] : 0u;
+            mn[x] = 0xffffffffu;
+            // a component's element: its weight if it ended inside the band
+            // (a live one's pixels are in its last-row record's weight)
+            const bool comp = F.rlost[b0 + x] == x;
+            const uint32_t info = comp ? F.rroot[b0 + x] : 0u;
+            p[2 * QW + x] = 2 * QW + x;
+            w[2 * QW + x] = comp && !(info & kPortLive) ? info : 0u;
+            mn[2 * QW + x] = 0xffffffffu;
+        }
+        for (uint32_t i = tid; i < ne; i += T) {
+            p[QW + i] = QW + i;
+            w[QW + i] = F.rlab[be + i] == i ? F.recw[be + i] : 0u;
+            mn[QW + i] = 0xffffffffu;
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < n0; x += T) {
+            const uint32_t g = F.rlost[b0 + x];
+            fr_unite<false>(p, F.rlab[b0 + x], 2 * QW + g);
+            if (g == x) {
+                const uint32_t info = F.rroot[b0 + x];
+                if (info & kPortLive) fr_unite<false>(p, 2 * QW + x, QW + (info & ~kPortLive));
+            }
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < n0; x += T) {
+            settle(x);
+            settle(2 * QW + x);
+        }
+        for (uint32_t i = tid; i < ne; i += T) {
+            const uint32_t r = settle(QW + i);
+            if (F.rlab[be + i] == i) atomicMin(&mn[r], i);  // a record of the band: its first run
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < ne; i += T) {
+            const uint32_t r = p[QW + F.rlab[be + i]], l = mn[r];
+            F.rlab[be + i] = l;
+            if (l == i) F.recw[be + i] = w[r];
+        }
+        __syncthreads();
+    }
+}
+
+// Step 3: every band but the first sweeps its rows again from the final
+// records above it (its last row is final already).
+template <int T>
+__global__ __launch_bounds__(T) void k_front_final(const Frame *frames, int band_rows, int max_bands) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t small[T / 64];
+    const Frame F = frames[blockIdx.y];
+    const int nb = front_bands(F, band_rows, max_bands), k = (int)blockIdx.x + 1;
+    if (k >= nb) return;
+    const int y0 = band_start(F, k, nb), y1 = band_start(F, k + 1, nb) - 2;
+    if (y1 < y0) return;
+    front_sweep<false, T, kSweepFinal>(F, lds, small, y0, y1, 1, 0);
 }
 
 // A wave per row: the front names each record of a row by its first run;
@@ -949,6 +1153,17 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     }
 }
 
+// Lets the front's kernels of T threads take `bytes` of dynamic LDS.
+template <int T>
+static bool front_lds_allow(int bytes) {
+    const void *fs[] = {reinterpret_cast<const void *>(&k_front<T>), reinterpret_cast<const void *>(&k_front_local<T>),
+                        reinterpret_cast<const void *>(&k_front_carry<T>),
+                        reinterpret_cast<const void *>(&k_front_final<T>)};
+    for (const void *f : fs)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return false;
+    return true;
+}
+
 // The front's dynamic LDS limit: 64 KiB, or most of the 160 KiB of a CU
 // where the kernels accept that much (asked once; a refusal is not left
 // behind as the runtime's last error).
@@ -956,16 +1171,8 @@ static size_t front_lds_limit() {
     static size_t lim = 0;
     if (!lim) {
         const int big = 159 * 1024;
-        bool ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<64>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
-                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<128>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
-                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<256>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
-                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<512>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess &&
-                  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front<1024>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, big) == hipSuccess;
+        bool ok = front_lds_allow<64>(big) && front_lds_allow<128>(big) && front_lds_allow<256>(big) &&
+                  front_lds_allow<512>(big) && front_lds_allow<1024>(big);
         (void)hipGetLastError();
         lim = ok ? (size_t)big : (size_t)64 * 1024;
     }
@@ -989,6 +1196,8 @@ size_t replay_scratch_bytes(int qw, bool force) {
 CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch, int nframes) {
     CclPlan c{};
     c.front = max_lds_qw > 0 ? (int)front_bytes(max_lds_qw) : 0;
+    // the split front, when its state (with the ports) fits LDS
+    c.front_band = max_lds_qw > 0 && front_band_bytes(max_lds_qw) <= front_lds_limit() ? (int)front_band_bytes(max_lds_qw) : 0;
     // rows of up to replay_cap runs in LDS, the others (all when forced) in HBM
     c.replay_cap = force_scratch ? 0 : replay_cap(max_qw, nframes);
     // packed elements when every weight fits 21 bits (and a parent 11)
@@ -1042,7 +1251,23 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     else
         hipLaunchKernelGGL(k_uf_edges<false>, rg, rb, 0, st, dframes, 0);
     mk->mark(st, "uf_edges");
-    switch (tu.front_threads) {
+    // the split front: bands of at least kBandRows rows, at most mb a frame
+    // (tu.front_bands, or by the call's size: more bands for fewer frames)
+    const int mb = tu.front_bands > 0 ? tu.front_bands : std::max(1, std::min(64, 64 / std::max(1, nframes)));
+    if (mb > 1 && c.front_band > 0) {
+#define ATG_FRONT_BANDS(T)                                                                                     \
+    hipLaunchKernelGGL(k_front_local<T>, dim3(mb, nframes), dim3(T), (size_t)c.front_band, st, dframes, kBandRows, mb); \
+    hipLaunchKernelGGL(k_front_carry<T>, dim3(1, nframes), dim3(T), (size_t)c.front_band, st, dframes, kBandRows, mb); \
+    hipLaunchKernelGGL(k_front_final<T>, dim3(mb - 1, nframes), dim3(T), (size_t)c.front_band, st, dframes, kBandRows, mb)
+        switch (tu.front_threads) {
+        case 64: ATG_FRONT_BANDS(64); break;
+        case 128: ATG_FRONT_BANDS(128); break;
+        case 512: ATG_FRONT_BANDS(512); break;
+        case 1024: ATG_FRONT_BANDS(1024); break;
+        default: ATG_FRONT_BANDS(256);
+        }
+#undef ATG_FRONT_BANDS
+    } else switch (tu.front_threads) {
     case 64:
         hipLaunchKernelGGL(k_front<64>, dim3(1, nframes), dim3(64), (size_t)c.front, st, dframes);
         break;

@@ -47,7 +47,7 @@ constexpr size_t kCountersBytes = 64;
 constexpr size_t kResultBlock = kCountersBytes + (size_t)kInlineDets * sizeof(at_detection);
 // fit_blocks 2048: the one-wave fit kernels that fit more than 16 waves a CU
 // (the small-cluster angle sort, edge refinement) get them all resident
-constexpr Tune kDefaultTune = {1, 4, 1024, 2048};
+constexpr Tune kDefaultTune = {1, 4, 1024, 2048, 0};
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -760,7 +760,8 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
                                     ccl.replay_cap, ccl.replay_pack, ccl.replay_mid, ccl.replay_mid_cap, ccl.replay_mid_pack,
                                     ccl.replay_mid_grid, ccl.replay_wide, ccl.replay_wide_cap, ccl.replay_wide_grid,
                                     ccl.replay_waves, max_lds_qw,
-                                    tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, sort_cap, wave_corners, ctx->hp.refine,
+                                    tu.thr_tile_rows, tu.row_waves, tu.front_threads, tu.fit_blocks, tu.front_bands,
+                                    ccl.front_band, sort_cap, wave_corners, ctx->hp.refine,
                                     ctx->profiling,
                                     (int64_t)(intptr_t)ctx->arena, (int64_t)(intptr_t)ctx->results,
                                     (int64_t)(intptr_t)ctx->dframes, (int64_t)(intptr_t)ctx->pinned_out,
@@ -1065,7 +1066,7 @@ static bool valid_tune(const Tune &t) {
     // row_waves: the per-row kernels are compiled for blocks of at most 256
     // threads (4 waves)
     return in(t.thr_tile_rows, {1, 2, 4}) && in(t.row_waves, {1, 2, 4}) && in(t.front_threads, {64, 128, 256, 512, 1024}) &&
-           t.fit_blocks >= 32 && t.fit_blocks <= 4096;
+           t.fit_blocks >= 32 && t.fit_blocks <= 4096 && t.front_bands >= 0 && t.front_bands <= 64;
 }
 
 extern "C" int atg_get_tune(const atg_ctx *ctx, atg_tune *out) {
@@ -1133,11 +1134,15 @@ extern "C" int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *
     Tune best = start;
     double bms = 0.0;
     int rc = measure(best, &bms);
-    const std::vector<int> cand[4] = {{1, 2, 4}, {1, 2, 4}, {128, 256, 512, 1024}, {256, 512, 1024, 2048, 4096}};
-    for (int k = 0; k < 4 && rc == ATG_OK; k++) {
+    const std::vector<int> cand[5] = {{1, 2, 4},
+                                      {1, 2, 4},
+                                      {128, 256, 512, 1024},
+                                      {256, 512, 1024, 2048, 4096},
+                                      {0, 1, 2, 4, 8, 16, 32, 64}};
+    for (int k = 0; k < 5 && rc == ATG_OK; k++) {
         for (int v : cand[k]) {
             Tune t = best;
-            int32_t *f[4] = {&t.thr_tile_rows, &t.row_waves, &t.front_threads, &t.fit_blocks};
+            int32_t *f[5] = {&t.thr_tile_rows, &t.row_waves, &t.front_threads, &t.fit_blocks, &t.front_bands};
             if (*f[k] == v) continue;
             *f[k] = v;
             double ms = 0.0;

@@ -236,9 +236,8 @@ def batching_config4(atg, ctx, steps):
         for d in dev:
             ctx.free_device(d[0])
     return dict(frames="64: 16 each of 640x480, 1280x720, 1920x1080, 2448x2048", fps_default=round(fps0, 1),
-                fps_autotuned=round(fps1, 1), autotuned=dict(zip(("thr_tile_rows", "row_waves", "front_threads",
-                                                                    "fit_blocks"), tune)),
-                default=dict(zip(("thr_tile_rows", "row_waves", "front_threads", "fit_blocks"), default_tune)),
+                fps_autotuned=round(fps1, 1), autotuned=dict(zip(atg.TUNE_FIELDS, tune)),
+                default=dict(zip(atg.TUNE_FIELDS, default_tune)),
                 bit_exact_vs_single=bool(exact), detections=int(sum(len(a["dets"]) for a in alone)))
 
 

@@ -125,9 +125,11 @@ int atg_debug_counters(atg_ctx *ctx, int32_t frame, uint32_t *out16);
  * threshold tile rows per block (1, 2, 4), rows per block of the per-row
  * kernels (1, 2, 4), threads of the union-find front sweep (64, 128, 256,
  * 512, 1024), blocks of the persistent fit and decode kernels (32..4096; at most
- * as many as are resident at once are launched). */
+ * as many as are resident at once are launched), row bands of a frame's
+ * union-find front at most (1..64, each of 16 rows or more, swept at once;
+ * 0: chosen by the number of frames of the call). */
 typedef struct {
-    int32_t thr_tile_rows, row_waves, front_threads, fit_blocks;
+    int32_t thr_tile_rows, row_waves, front_threads, fit_blocks, front_bands;
 } atg_tune;
 int atg_get_tune(const atg_ctx *ctx, atg_tune *out);
 int atg_set_tune(atg_ctx *ctx, const atg_tune *t);

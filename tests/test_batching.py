@@ -25,8 +25,8 @@ def same(a, b):
                                     for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("tune", [(2, 1, 128, 64), (4, 4, 512, 1024), (1, 2, 256, 32), (2, 4, 128, 512),
-                                  (1, 4, 64, 4096)])
+@pytest.mark.parametrize("tune", [(2, 1, 128, 64, 0), (4, 4, 512, 1024, 1), (1, 2, 256, 32, 3), (2, 4, 128, 512, 64),
+                                  (1, 4, 64, 4096, 7), (1, 4, 1024, 2048, 2)])
 def test_work_splits_give_the_same_detections(atg, ctx, tune):
     frames = uneven_frames(atg)
     ctx.set_families(FAMS)

@@ -141,6 +141,31 @@ def test_small_min_cluster_pixels_match_oracle(atg, ctx, oracle, mcp):
     assert o["nquads"] == g["nquads"] and same_dets(o["dets"], g["dets"])
 
 
+@pytest.mark.parametrize("case,bands", [("c2", 2), ("c2", 3), ("c2", 16), ("c2", 64), ("c4", 5), ("c4", 64),
+                                        ("tall", 64), ("wide", 7), ("c3", 13)])
+def test_split_front_matches_oracle(atg, ctx, oracle, case, bands):
+    """The union-find front split into row bands (swept from ports, joined
+    band by band, swept again): roots, sizes, clusters and detections equal
+    the oracle's for every band count."""
+    if case == "tall":
+        img, p, fams = _stripes(256, 4096, 2), default_params(), [("tag36h11", 2)]
+    elif case == "wide":
+        img, p, fams = _blocky(4096, 128, 2, 1), default_params(), [("tag36h11", 2)]
+    else:
+        img, p, fams = synth_case(atg, {"c2": 2, "c4": 4, "c3": 3}[case], {"c2": 1, "c4": 3, "c3": 0}[case])
+    print("case split front", case, bands, img.shape, flush=True)
+    o = oracle.detect(img, p, fams, stages=True)
+    ctx.set_tune(1, 4, 1024, 2048, bands)
+    try:
+        g = gpu_detect(atg, ctx, img, p, fams)
+        st = ctx.stages(0)
+    finally:
+        ctx.set_tune(1, 4, 1024, 2048, 0)
+    for k in STAGE_KEYS:
+        assert np.array_equal(o["stages"][k], st[k]), k
+    assert same_dets(o["dets"], g["dets"]) and o["nquads"] == g["nquads"]
+
+
 @pytest.mark.parametrize("cfg,seed", [(1, 1), (4, 3)])
 def test_union_find_scratch_path_matches_oracle(atg, ctx, oracle, cfg, seed):
     """The union-find replay with its state in HBM scratch (the path of
