@@ -306,7 +306,8 @@ class Context:
         self._check(self.L.atg_device_free(self.h, ptr), "atg_device_free")
 
     def set_profiling(self, on=True):
-        self._check(self.L.atg_set_profiling(self.h, 1 if on else 0), "atg_set_profiling")
+        """on: True/1 device clock stamps (graphs stay on), 2 HIP events, False/0 off."""
+        self._check(self.L.atg_set_profiling(self.h, int(on)), "atg_set_profiling")
 
     def get_tune(self):
         t = Tune()
@@ -361,10 +362,11 @@ class Context:
         self._check(self.L.atg_debug_force_scratch(self.h, 1 if on else 0), "atg_debug_force_scratch")
 
     def stage_times(self):
-        names = (C.c_char_p * 32)()
-        ms = (C.c_double * 32)()
-        n = self.L.atg_stage_times(self.h, names, ms, 32)
-        return [(names[i].decode(), ms[i]) for i in range(min(n, 32))]
+        """[(group.kernel, ms)] of the last call with profiling on."""
+        names = (C.c_char_p * 64)()
+        ms = (C.c_double * 64)()
+        n = self.L.atg_stage_times(self.h, names, ms, 64)
+        return [(names[i].decode(), ms[i]) for i in range(min(n, 64))]
 
     def counters(self, frame=0):
         out = (C.c_uint32 * 16)()

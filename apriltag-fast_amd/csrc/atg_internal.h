@@ -176,7 +176,8 @@ struct Frame {
     QuadRec *quads;       // [qcap] in cluster order
     double *qline;        // [qcap][4][4] per quad edge: refined line (x, y, cos, sin);
                           // before k_lines: mean x, mean y and atan2's two arguments
-    at_detection *dtmp;   // [qcap * nfam] per (quad, family); then the final list, sorted by id, in place
+    at_detection *dtmp;   // [qcap * nfam] per (quad, family)
+    at_detection *dfin;   // [qcap * nfam] the final list, sorted by id
     uint32_t *dflag;      // [qcap * nfam]
     at_detection *inl;    // [kInlineDets] the final list's first entries, beside the counters
     uint32_t *counters;
@@ -241,22 +242,26 @@ size_t replay_scratch_bytes(int qw, bool force);
 unsigned persistent_grid(const void *fn, int threads, size_t lds, int want);
 void warm_persistent_grids(int fit_blocks);
 
-// Optional timing marks between the stages of one call: a one-lane kernel
-// on the call's stream writes the device wall clock into stamps[i] (kernels
-// on one stream run in order, so consecutive stamps bracket the stage
-// between them). Being kernels, the marks are captured in the call's HIP
-// graph like any other launch.
-constexpr int kMaxMarks = 24;
+// Optional timing marks between the kernels of one call (consecutive
+// marks on one stream bracket the kernels launched between them). Level 1:
+// a one-lane kernel writes the device wall clock into stamps[i]; being a
+// kernel, the mark is captured in the call's HIP graph like any other
+// launch. Level 2: HIP events (the call then runs without a graph).
+constexpr int kMaxMarks = 48;
 void launch_stamp(hipStream_t st, uint64_t *slot);
 struct Marks {
     uint64_t *stamps = nullptr;  // device [kMaxMarks]
+    hipEvent_t ev[kMaxMarks] = {};
     const char *name[kMaxMarks];
     int n = 0;
-    bool on = false;
+    int level = 0;
     void mark(hipStream_t st, const char *what) {
-        if (!on || n >= kMaxMarks) return;
+        if (!level || n >= kMaxMarks) return;
         name[n] = what;
-        launch_stamp(st, stamps + n++);
+        if (level == 2)
+            hipEventRecord(ev[n++], st);
+        else
+            launch_stamp(st, stamps + n++);
     }
 };
 

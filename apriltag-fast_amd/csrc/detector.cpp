@@ -77,8 +77,8 @@ extern "C" const char *at_family_name(const at_detector *td, const at_detection 
 }
 
 // td->timing (fast/apriltag_fast.h:67-69) from the device times of the
-// chain's kernel groups: each reference stage is the sum of the groups that
-// do its work.
+// chain's kernels (named group.kernel): each reference stage is the sum of
+// the groups that do its work.
 static void fill_timing(atg_ctx *ctx, at_timing *t) {
     static const struct {
         const char *group;
@@ -92,12 +92,15 @@ static void fill_timing(atg_ctx *ctx, at_timing *t) {
                {"fit_prep", &at_timing::fit_quads},   {"angle_sort", &at_timing::fit_quads},
                {"moments", &at_timing::fit_quads},    {"corners", &at_timing::fit_quads},
                {"decode", &at_timing::decode},        {"finish", &at_timing::reconcile}};
-    const char *names[32];
-    double ms[32];
-    int n = atg_stage_times(ctx, names, ms, 32);
-    for (int i = 0; i < n && i < 32; i++)
+    const char *names[64];
+    double ms[64];
+    int n = atg_stage_times(ctx, names, ms, 64);
+    for (int i = 0; i < n && i < 64; i++) {
+        const char *dot = strchr(names[i], '.');
+        const size_t len = dot ? (size_t)(dot - names[i]) : strlen(names[i]);
         for (const auto &m : map)
-            if (strcmp(names[i], m.group) == 0) t->*m.field += ms[i] * 1e-3;
+            if (strlen(m.group) == len && strncmp(names[i], m.group, len) == 0) t->*m.field += ms[i] * 1e-3;
+    }
 }
 
 extern "C" void at_detect(at_detector *td, const uint8_t *pixels, int64_t width, int64_t height, int64_t stride) {

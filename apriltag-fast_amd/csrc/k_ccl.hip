@@ -488,15 +488,18 @@ __global__ __launch_bounds__(T) void k_front(const Frame *frames) {
         front_sweep<false, T, kSweepFull>(F, lds, small, 0, F.qh - 1, 0, F.qh - 1);
 }
 
-constexpr int kBandRows = 16;  // rows of a band of the split front at least
+constexpr int kBandRows = 16;      // rows of a band of the split front at least
+constexpr int kMaxFrontBands = 64;  // bands of a frame at most
 
 // Row bands of a frame for the split front: band k of nb holds rows
 // [k qh / nb, (k + 1) qh / nb); nb = qh / band_rows, at most max_bands
-// (one band: the whole-frame sweep). Frames whose front state is in HBM
-// are not split.
+// (one band: the whole-frame sweep). A frame whose front state is in HBM
+// has a region of it for each band (front_band_stride bytes apart).
 __device__ __forceinline__ int front_bands(const Frame &F, int band_rows, int max_bands) {
-    if (F.fscr) return 1;
     return max(1, min(max_bands, F.qh / max(band_rows, 2)));
+}
+__host__ __device__ __forceinline__ size_t front_band_stride(int qw) {
+    return ((size_t)68 * qw + 128 + 255) & ~(size_t)255;
 }
 __device__ __forceinline__ int band_start(const Frame &F, int k, int nb) { return (int)((int64_t)k * F.qh / nb); }
 
@@ -511,12 +514,17 @@ __global__ __launch_bounds__(T) void k_front_local(const Frame *frames, int band
     const int nb = front_bands(F, band_rows, max_bands), k = (int)blockIdx.x;
     if (k >= nb) return;
     const int y0 = band_start(F, k, nb), y1 = band_start(F, k + 1, nb) - 1;
-    if (F.fscr)
-        front_sweep<true, T, kSweepFull>(F, F.fscr, small, 0, F.qh - 1, 0, F.qh - 1);
-    else if (k == 0)
+    if (F.fscr) {
+        uint8_t *mine = F.fscr + (size_t)k * front_band_stride(F.qw);
+        if (k == 0)
+            front_sweep<true, T, kSweepFull>(F, mine, small, 0, y1, 0, y1);
+        else
+            front_sweep<true, T, kSweepLocal>(F, mine, small, y0, y1, y0, y1);
+    } else if (k == 0) {
         front_sweep<false, T, kSweepFull>(F, lds, small, 0, y1, 0, y1);
-    else
+    } else {
         front_sweep<false, T, kSweepLocal>(F, lds, small, y0, y1, y0, y1);
+    }
 }
 
 // Step 2: band by band (one block per frame), the final records of each
@@ -603,7 +611,10 @@ __global__ __launch_bounds__(T) void k_front_final(const Frame *frames, int band
     if (k >= nb) return;
     const int y0 = band_start(F, k, nb), y1 = band_start(F, k + 1, nb) - 2;
     if (y1 < y0) return;
-    front_sweep<false, T, kSweepFinal>(F, lds, small, y0, y1, 1, 0);
+    if (F.fscr)
+        front_sweep<true, T, kSweepFinal>(F, F.fscr + (size_t)k * front_band_stride(F.qw), small, y0, y1, 1, 0);
+    else
+        front_sweep<false, T, kSweepFinal>(F, lds, small, y0, y1, 1, 0);
 }
 
 // A wave per row: the front names each record of a row by its first run;
@@ -1181,9 +1192,11 @@ static size_t front_lds_limit() {
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// A frame whose front state does not fit LDS (or forced) keeps it in HBM:
+// a region for each band the split front can give it.
 size_t front_scratch_bytes(int qw, bool force) {
     size_t f = front_bytes(qw);
-    return (force || f > front_lds_limit()) ? al256(f) : 0;
+    return (force || f > front_lds_limit()) ? front_band_stride(qw) * (size_t)kMaxFrontBands : 0;
 }
 
 size_t replay_scratch_bytes(int qw, bool force) {
@@ -1196,8 +1209,14 @@ size_t replay_scratch_bytes(int qw, bool force) {
 CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scratch, int nframes) {
     CclPlan c{};
     c.front = max_lds_qw > 0 ? (int)front_bytes(max_lds_qw) : 0;
-    // the split front, when its state (with the ports) fits LDS
-    c.front_band = max_lds_qw > 0 && front_band_bytes(max_lds_qw) <= front_lds_limit() ? (int)front_band_bytes(max_lds_qw) : 0;
+    // the split front: LDS for the state of its LDS frames (with the
+    // ports) and for the band-by-band join of the widest frame (three
+    // arrays of 3 qw), when both fit
+    {
+        const size_t need = std::max(max_lds_qw > 0 ? front_band_bytes(max_lds_qw) : (size_t)0,
+                                     (size_t)36 * (size_t)max_qw + 64);
+        c.front_band = need <= front_lds_limit() ? (int)need : 0;
+    }
     // rows of up to replay_cap runs in LDS, the others (all when forced) in HBM
     c.replay_cap = force_scratch ? 0 : replay_cap(max_qw, nframes);
     // packed elements when every weight fits 21 bits (and a parent 11)
@@ -1253,7 +1272,8 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     mk->mark(st, "uf_edges");
     // the split front: bands of at least kBandRows rows, at most mb a frame
     // (tu.front_bands, or by the call's size: more bands for fewer frames)
-    const int mb = tu.front_bands > 0 ? tu.front_bands : std::max(1, std::min(64, 64 / std::max(1, nframes)));
+    const int mb = tu.front_bands > 0 ? std::min(tu.front_bands, kMaxFrontBands)
+                                      : std::max(1, std::min(kMaxFrontBands, 256 / std::max(1, nframes)));
     if (mb > 1 && c.front_band > 0) {
 #define ATG_FRONT_BANDS(T)                                                                                     \
     hipLaunchKernelGGL(k_front_local<T>, dim3(mb, nframes), dim3(T), (size_t)c.front_band, st, dframes, kBandRows, mb); \
@@ -1283,13 +1303,15 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     default:
         hipLaunchKernelGGL(k_front<256>, dim3(1, nframes), dim3(256), (size_t)c.front, st, dframes);
     }
+    mk->mark(st, "uf_front.sweep");
     hipLaunchKernelGGL(k_front_dense, rg, rb, 0, st, dframes);
-    mk->mark(st, "uf_front");
+    mk->mark(st, "uf_front.dense");
     const dim3 rows((unsigned)std::max(1, max_qh - 1), nframes);
     if (c.replay_cap && c.replay_pack)
         hipLaunchKernelGGL(k_replay<true>, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
     else if (c.replay_cap)
         hipLaunchKernelGGL(k_replay<false>, rows, dim3(64), (size_t)c.replay, st, dframes, c.replay_cap);
+    mk->mark(st, "uf_replay.rows");
     int hcap = c.replay_cap, hlcap = replay_links(c.replay_cap);
     if (c.replay_mid_cap) {
         launch_work_offsets(st, dframes, nframes, Q, 3, (int)kNWideRows);
@@ -1310,17 +1332,19 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
         hlcap = 3 * c.replay_wide_cap;
     }
     hipLaunchKernelGGL(k_replay_hbm, dim3((unsigned)c.replay_waves, nframes), dim3(64), 0, st, dframes, hcap, hlcap);
-    mk->mark(st, "uf_replay");
+    mk->mark(st, "uf_replay.wide");
     for (int64_t d = 1; d < max_qh; d *= kRootHops + 1) hipLaunchKernelGGL(k_root_jump, rg, rb, 0, st, dframes);
+    mk->mark(st, "uf_stitch.jump");
     hipLaunchKernelGGL(k_lost, rg, rb, 0, st, dframes);
-    mk->mark(st, "uf_stitch");
+    mk->mark(st, "uf_stitch.lost");
     int64_t maxnodes = (int64_t)max_qw * max_qh + max_qh;
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,
                        dparams);
     mk->mark(st, "resolve");
     launch_root_numbers(st, dframes, nframes);  // the dense numbers k_labels writes
+    mk->mark(st, "labels.roots");
     hipLaunchKernelGGL(k_labels, rg, rb, 0, st, dframes);
-    mk->mark(st, "labels");
+    mk->mark(st, "labels.rows");
 }
 
 }  // namespace atg

@@ -784,81 +784,129 @@ __device__ bool g2_overlaps(const double pa[4][2], const double pb[4][2]) {
 }
 __device__ __forceinline__ int prefer(int p, double a, double b) { return p != 0 ? p : a < b ? -1 : b < a ? 1 : 0; }
 
-// Reconcile (fast/decode.c:860-891) then the stable sort by id
-// (fast/detector.c:164-179) of a frame's detection list, in place by one
-// lane; returns the count kept.
-__device__ int64_t finish_dets(at_detection *d, int64_t n) {
-    int64_t i0 = 0;
-    while (i0 < n) {
-        bool removed0 = false;
-        int64_t i1 = i0 + 1;
-        while (i1 < n) {
-            if (d[i0].id != d[i1].id || d[i0].family != d[i1].family) { i1++; continue; }
-            if (g2_overlaps(d[i0].p, d[i1].p)) {
-                int p = 0;
-                p = prefer(p, (double)d[i0].hamming, (double)d[i1].hamming);
-                p = prefer(p, -(double)d[i0].decision_margin, -(double)d[i1].decision_margin);
+// The last step of a frame, by one wave: the detections of the decoded
+// slots (src[k] where flag[k], all nslots when flag is null), in slot
+// order, reconciled (fast/decode.c:860-891) and stably sorted by id
+// (fast/detector.c:164-179) into out; returns their count. Scratch: A the
+// kept slots in order, K their (id, family) keys, B the list ordered by
+// key (stable), R removal flags, S the survivors (each nslots u32).
+//
+// Reconcile only compares detections of the same id and family, and a
+// removal never changes the order of the others, so the reference's one
+// pass over the list gives, within each (id, family) group, what the same
+// pass over that group alone gives: each lane runs the pass over a group.
+__device__ __forceinline__ uint64_t wave_lt_mask() { return (1ull << (threadIdx.x & 63)) - 1ull; }
+
+__device__ uint32_t finish_list(const at_detection *src, const uint32_t *flag, uint32_t nslots, at_detection *out,
+                                uint32_t *A, uint32_t *K, uint32_t *B, uint32_t *R, uint32_t *S) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt = wave_lt_mask();
+    // the decoded slots in order (ballot compaction)
+    uint32_t n = 0;
+    for (uint32_t c0 = 0; c0 < nslots; c0 += 64) {
+        const uint32_t k = c0 + lane;
+        const bool f = k < nslots && (flag == nullptr || flag[k] != 0u);
+        const uint64_t b = __ballot(f);
+        if (f) {
+            const uint32_t at = n + (uint32_t)__popcll(b & lt);
+            A[at] = k;
+            K[at] = (uint32_t)src[k].id | ((uint32_t)src[k].family << 16);
+            R[at] = 0u;
+        }
+        n += (uint32_t)__popcll(b);
+    }
+    __threadfence_block();
+    // B: positions ordered by key, stable (rank of each)
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t ki = K[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < n; j++) {
+            const uint32_t kj = K[j];
+            r += (kj < ki || (kj == ki && j < i)) ? 1u : 0u;
+        }
+        B[r] = i;
+    }
+    __threadfence_block();
+    // one lane a group: the reference's pass over the group's detections
+    for (uint32_t g0 = 0; g0 < n; g0 += 64) {
+        const uint32_t g = g0 + lane;
+        if (g >= n || (g > 0 && K[B[g]] == K[B[g - 1]])) continue;  // not a group's first
+        uint32_t e = g + 1;
+        while (e < n && K[B[e]] == K[B[g]]) e++;
+        for (uint32_t i0 = g; i0 < e; i0++) {
+            const uint32_t p0 = B[i0];
+            if (R[p0]) continue;
+            const at_detection &d0 = src[A[p0]];
+            for (uint32_t i1 = i0 + 1; i1 < e; i1++) {
+                const uint32_t p1 = B[i1];
+                if (R[p1]) continue;
+                const at_detection &d1 = src[A[p1]];
+                if (!g2_overlaps(d0.p, d1.p)) continue;
+                int pr = 0;
+                pr = prefer(pr, (double)d0.hamming, (double)d1.hamming);
+                pr = prefer(pr, -(double)d0.decision_margin, -(double)d1.decision_margin);
                 for (int i = 0; i < 4; i++) {
-                    p = prefer(p, d[i0].p[i][0], d[i1].p[i][0]);
-                    p = prefer(p, d[i0].p[i][1], d[i1].p[i][1]);
+                    pr = prefer(pr, d0.p[i][0], d1.p[i][0]);
+                    pr = prefer(pr, d0.p[i][1], d1.p[i][1]);
                 }
-                if (p < 0) {
-                    for (int64_t k = i1; k + 1 < n; k++) d[k] = d[k + 1];
-                    n--;
+                if (pr < 0) {
+                    R[p1] = 1u;  // the later one goes; the pass goes on from the next
                     continue;
                 }
-                for (int64_t k = i0; k + 1 < n; k++) d[k] = d[k + 1];
-                n--;
-                removed0 = true;
+                R[p0] = 1u;  // the earlier one goes: its pass ends
                 break;
             }
-            i1++;
         }
-        if (!removed0) i0++;
     }
-    for (int64_t i = 1; i < n; i++) {
-        at_detection v = d[i];
-        int64_t j = i;
-        while (j > 0 && d[j - 1].id > v.id) { d[j] = d[j - 1]; j--; }
-        d[j] = v;
+    __threadfence_block();
+    // the survivors in order
+    uint32_t m = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t i = c0 + lane;
+        const bool f = i < n && R[i] == 0u;
+        const uint64_t b = __ballot(f);
+        if (f) S[m + (uint32_t)__popcll(b & lt)] = A[i];
+        m += (uint32_t)__popcll(b);
     }
-    return n;
+    __threadfence_block();
+    // stable by id: the rank of each survivor, its record copied there
+    for (uint32_t i = lane; i < m; i += 64) {
+        const int32_t id = src[S[i]].id;
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < m; j++) {
+            const int32_t o = src[S[j]].id;
+            r += (o < id || (o == id && j < i)) ? 1u : 0u;
+        }
+        out[r] = src[S[i]];
+    }
+    return m;
 }
 
-// A frame's detections: the decoded (quad, family) slots in order,
-// compacted in place (each moves to a slot at or before its own), then
-// reconciled and sorted; the first kInlineDets copied beside the counters.
 __global__ __launch_bounds__(64) void k_finish(const Frame *frames, const DevParams *P) {
     const Frame F = frames[blockIdx.y];
-    __shared__ int64_t s_n;
-    if (threadIdx.x == 0) {
-        const uint32_t nq = F.counters[kNQuads];
-        const int nfam = P->nfam;
-        at_detection *d = F.dtmp;
-        int64_t n = 0;
-        for (uint32_t k = 0; k < nq * (uint32_t)nfam; k++) {
-            if (!F.dflag[k]) continue;
-            if ((int64_t)k != n) d[n] = d[k];
-            n++;
-        }
-        n = finish_dets(d, n);
-        F.counters[kNDets] = (uint32_t)n;
-        s_n = n;
-    }
-    __syncthreads();
-    const int64_t words = min(s_n, (int64_t)kInlineDets) * (int64_t)(sizeof(at_detection) / 8);
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(F.dtmp);
-    uint64_t *dst = reinterpret_cast<uint64_t *>(F.inl);
-    for (int64_t i = threadIdx.x; i < words; i += 64) dst[i] = src[i];
+    const uint32_t nslots = F.counters[kNQuads] * (uint32_t)P->nfam;
+    // scratch: the corner search's arrays, free by now
+    uint32_t *A = F.midx, *K = reinterpret_cast<uint32_t *>(F.errs), *R = reinterpret_cast<uint32_t *>(F.filt);
+    const uint32_t n = finish_list(F.dtmp, F.dflag, nslots, F.dfin, A, K, K + nslots, R, R + nslots);
+    if (threadIdx.x == 0) F.counters[kNDets] = n;
+    __threadfence_block();
+    const uint32_t words = min(n, (uint32_t)kInlineDets) * (uint32_t)(sizeof(at_detection) / 8);
+    const uint64_t *s = reinterpret_cast<const uint64_t *>(F.dfin);
+    uint64_t *d = reinterpret_cast<uint64_t *>(F.inl);
+    for (uint32_t i = threadIdx.x; i < words; i += 64) d[i] = s[i];
 }
 
-// The finishing step of one detection list given by the caller.
-__global__ __launch_bounds__(64) void k_debug_finish(at_detection *d, int64_t n, int64_t *nout) {
-    if (threadIdx.x == 0) *nout = finish_dets(d, n);
+// The last step on a caller's list (debug entry point): buf holds the n
+// records, then n records of output, then 5 n u32 of scratch.
+__global__ __launch_bounds__(64) void k_debug_finish(uint8_t *buf, uint32_t n, int64_t *nout) {
+    at_detection *in = reinterpret_cast<at_detection *>(buf), *out = in + n;
+    uint32_t *s = reinterpret_cast<uint32_t *>(out + n);
+    const uint32_t m = finish_list(in, nullptr, n, out, s, s + n, s + 2 * n, s + 3 * n, s + 4 * n);
+    if (threadIdx.x == 0) *nout = m;
 }
 
-void launch_debug_finish(hipStream_t st, at_detection *d, int64_t n, int64_t *nout) {
-    hipLaunchKernelGGL(k_debug_finish, dim3(1), dim3(64), 0, st, d, n, nout);
+void launch_debug_finish(hipStream_t st, uint8_t *buf, uint32_t n, int64_t *nout) {
+    hipLaunchKernelGGL(k_debug_finish, dim3(1), dim3(64), 0, st, buf, n, nout);
 }
 
 __global__ void k_init(const Frame *frames, WorkQueue *Q) {
@@ -894,8 +942,10 @@ void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *
             hipLaunchKernelGGL(k_refine<256>,
                                dim3(persistent_grid(reinterpret_cast<const void *>(&k_refine<256>), 256, 0, blocks)),
                                dim3(256), 0, st, dframes, dparams, Q);
+        mk->mark(st, "decode.refine");
         hipLaunchKernelGGL(k_lines, dim3(persistent_grid(reinterpret_cast<const void *>(&k_lines), 256, 0, blocks)),
                            dim3(256), 0, st, dframes, Q);
+        mk->mark(st, "decode.lines");
     }
     if (nframes > 4)
         hipLaunchKernelGGL(k_decode<64>,
@@ -905,7 +955,7 @@ void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *
         hipLaunchKernelGGL(k_decode<256>,
                            dim3(persistent_grid(reinterpret_cast<const void *>(&k_decode<256>), 256, 0, blocks)),
                            dim3(256), 0, st, dframes, dparams, Q);
-    mk->mark(st, "decode");
+    mk->mark(st, "decode.sample");
     hipLaunchKernelGGL(k_finish, dim3(1, nframes), dim3(64), 0, st, dframes, dparams);
     mk->mark(st, "finish");
 }

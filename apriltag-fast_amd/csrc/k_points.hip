@@ -557,15 +557,17 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     dim3 g((unsigned)((max_qh + rw - 1) / rw), nframes), b(64 * rw);
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 0);
     hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
+    mk->mark(st, "points.count");
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);
-    mk->mark(st, "points");
+    mk->mark(st, "points.write");
     // key_bits: the widest key the call's frames can have
     int parity = 0;
     launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 1, 1);
     mk->mark(st, "cluster_sort");
     hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+    mk->mark(st, "clusters.gather");
     hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-    mk->mark(st, "clusters");
+    mk->mark(st, "clusters.write");
 }
 
 }  // namespace atg

@@ -1411,12 +1411,26 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
 // by its cluster index.
 __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
-    uint32_t nq = min(F.counters[kNQuads], F.qcap);
-    for (uint32_t i = threadIdx.x; i < nq; i += 256) {
-        uint32_t ci = F.qcl[i], r = 0;
-        for (uint32_t j = 0; j < nq; j++) r += F.qcl[j] < ci ? 1u : 0u;
-        F.quads[r] = F.qtmp[i];
+    const uint32_t nq = min(F.counters[kNQuads], F.qcap), ncl = F.counters[kNClusters];
+    // a quad's rank = the clusters before its own that have a quad (one a
+    // cluster at most): marks over the clusters (midx, free after the
+    // corner search), scanned
+    uint32_t *mark = F.midx;
+    for (uint32_t c = threadIdx.x; c < ncl; c += 256) mark[c] = 0u;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nq; i += 256) mark[F.qcl[i]] = 1u;
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < ncl; c0 += 256) {
+        const uint32_t c = c0 + threadIdx.x;
+        const uint32_t v = c < ncl ? mark[c] : 0u;
+        uint32_t tot;
+        const uint32_t pre = bscan(v, &tot);
+        if (c < ncl) mark[c] = carry + pre;
+        carry += tot;
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nq; i += 256) F.quads[mark[F.qcl[i]]] = F.qtmp[i];
     if (threadIdx.x == 0) F.counters[kNQuads] = nq;
 }
 
@@ -1480,10 +1494,12 @@ void warm_persistent_grids(int fit_blocks) {
     warm_decode_grid(fit_blocks);
 }
 
-// Whether the angle sort runs per cluster (else over all points at once in
-// HBM): clusters up to 16384 points.
+// Whether the angle sort runs per cluster (any size: its steps wider than
+// a 2048-key tile go over the cluster's keys in HBM), or (forced, for
+// tests) over all points at once.
 int angle_sort_cap(int max_cluster, bool force_global) {
-    return (force_global || max_cluster > 16384) ? 0 : 1;
+    (void)max_cluster;
+    return force_global ? 0 : 1;
 }
 
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
@@ -1499,20 +1515,23 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_fit_prep<64>, dim3(512, nframes), dim3(64), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     else
         hipLaunchKernelGGL(k_fit_prep<256>, dim3(256, nframes), dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
+    mk->mark(st, "fit_prep.prep");
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
     // the huge, large and small lists, a block each
     const uint64_t lw = (uint64_t)(4 | kNHuge << 8) | (uint64_t)(0 | kNLarge << 8) << 16 |
                         (uint64_t)(1 | kNSmall << 8) << 32;
     hipLaunchKernelGGL(k_work_offsets, dim3(3), dim3(256), 0, st, dframes, nframes, Q, lw);
-    mk->mark(st, "fit_prep");
+    mk->mark(st, "fit_prep.lists");
     int parity = 0;
     if (sort_cap) {
         hipLaunchKernelGGL(k_angle_sort, dim3(pgrid(k_angle_sort, 256, fit_blocks)), dim3(256), 0, st, dframes, Q);
+        mk->mark(st, "angle_sort.large");
         hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, st, dframes, Q);
+        mk->mark(st, "angle_sort.small");
     } else {
         launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0, 0);
+        mk->mark(st, "angle_sort.all");
     }
-    mk->mark(st, "angle_sort");
     // large clusters: 128-thread blocks for batches (8 a CU: more running
     // sums in flight), 256 for up to 16 frames (each cluster sooner)
     if (nframes > 16)
@@ -1521,20 +1540,23 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     else
         hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256),
                            0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+    mk->mark(st, "moments.large");
     hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, st,
                        dframes, parity, Q, sort_cap ? 1 : 0, 1);
-    mk->mark(st, "moments");
+    mk->mark(st, "moments.small");
     hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st, dframes,
                        dparams, Q, 0);
+    mk->mark(st, "corners.large");
     if (wave_corners)
         hipLaunchKernelGGL(k_fit_corners<64>, dim3(pgrid(k_fit_corners<64>, 64, wf)), dim3(64), 0, st, dframes, dparams,
                            Q, 1);
     else
         hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st,
                            dframes, dparams, Q, 1);
+    mk->mark(st, "corners.small");
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
     launch_work_offsets(st, dframes, nframes, Q, 2, (int)kNQuads);
-    mk->mark(st, "corners");
+    mk->mark(st, "corners.order");
 }
 
 }  // namespace atg

@@ -34,7 +34,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
 int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          bool refine, WorkQueue *Q, Marks *mk);
-void launch_debug_finish(hipStream_t st, at_detection *d, int64_t n, int64_t *nout);
+void launch_debug_finish(hipStream_t st, uint8_t *buf, uint32_t n, int64_t *nout);
 }  // namespace atg
 
 using namespace atg;
@@ -133,7 +133,7 @@ struct atg_ctx {
     double clock_khz = 0.0;           // the device wall clock's rate
     int gmarks_n = 0;                 // the marks of the captured graph's chain
     const char *gmarks_name[kMaxMarks];
-    bool profiling = false;
+    int profiling = 0;                // marks: 0 none, 1 clock stamps, 2 HIP events
     bool force_scratch = false;       // union-find replay state in HBM even where LDS fits
     bool arena_fixed = false;         // the arena never grows (a serving lane's budget)
     hipEvent_t ev2 = nullptr;         // the end of the last enqueued chain (with its readback)
@@ -390,6 +390,7 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->qcl = (uint32_t *)take(4 * qc);
     F->quads = (QuadRec *)take(sizeof(QuadRec) * qc);
     F->dtmp = (at_detection *)take(sizeof(at_detection) * qc * std::max(nfam, 1));
+    F->dfin = (at_detection *)take(sizeof(at_detection) * qc * std::max(nfam, 1));
     F->dflag = (uint32_t *)take(4 * qc * std::max(nfam, 1));
     return at;
 }
@@ -496,6 +497,8 @@ extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
     hipFree(ctx->dtables);
     hipFree(ctx->dparams);
     hipFree(ctx->marks.stamps);
+    for (hipEvent_t e : ctx->marks.ev)
+        if (e) hipEventDestroy(e);
     hipHostFree(ctx->hstamps);
     hipHostFree(ctx->na_stage);
     hipHostFree(ctx->pinned);
@@ -633,6 +636,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     ctx->hframes.assign((size_t)n, Frame{});
     size_t total = 0, pinned_need = 0;
     ChainShape cs;
+    std::vector<uint8_t> pinned_src((size_t)n, 0);
     for (int32_t i = 0; i < n; i++) {
         FramePlan &fp = ctx->plans[(size_t)i];
         int rc = plan_frame(ctx, p, nfam, frames[i], on_device, fp);
@@ -640,7 +644,17 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         cs.add(fp);
         fp.off = total;
         total += fp.size;
-        if (!on_device && !fp.skip) pinned_need += align256((size_t)(fp.W * fp.H));
+        if (!on_device && !fp.skip) {
+            // host frames: dense rows in page-locked memory go by DMA, the
+            // others through the context's pinned staging
+            hipPointerAttribute_t attr{};
+            const bool locked = fp.stride == fp.W &&
+                                hipPointerGetAttributes(&attr, frames[i].pixels) == hipSuccess &&
+                                attr.type == hipMemoryTypeHost;
+            (void)hipGetLastError();  // (pageable memory is not known to HIP: not an error here)
+            pinned_src[(size_t)i] = locked ? 1 : 0;
+            if (!locked) pinned_need += align256((size_t)(fp.W * fp.H));
+        }
     }
     const int max_qw = cs.max_qw, max_qh = cs.max_qh, key_bits = cs.key_bits, cl_bits = cs.cl_bits;
     const int max_lds_qw = cs.max_lds_qw;
@@ -712,6 +726,10 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         } else if (on_device) {
             F.img = frames[i].pixels;
             F.stride = fp.stride;
+        } else if (pinned_src[(size_t)i]) {
+            // page-locked caller memory: copied in by DMA as it lies
+            copies.push_back(Copy{(void *)F.img, frames[i].pixels, (size_t)(fp.W * fp.H)});
+            F.stride = fp.W;
         } else {
             uint8_t *dst = ctx->pinned + pin_at;
             for (int64_t y = 0; y < fp.H; y++) memcpy(dst + y * fp.W, frames[i].pixels + y * fp.stride, (size_t)fp.W);
@@ -730,7 +748,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     copies.push_back(Copy{dq, qstage, sizeof(WorkQueue)});
     hipStream_t st = ctx->stream;
     Marks *mk = &ctx->marks;
-    mk->on = ctx->profiling;
+    mk->level = ctx->profiling;
     const Tune tu = ctx->tune;
     const int sort_cap = angle_sort_cap(cs.max_cluster, ctx->force_scratch);
     const bool wave_corners = ctx->hp.max_nmaxima <= 16;
@@ -745,7 +763,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         launch_chain(st, cs, ccl, ctx->hp, deglitch, ctx->dframes, ctx->dparams, n, dq, tu, ctx->force_scratch, mk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if (mk->n > 0) {
+        if (mk->n > 0 && mk->level == 1) {
             e = hipMemcpyAsync(ctx->hstamps, mk->stamps, sizeof(uint64_t) * (size_t)mk->n, hipMemcpyDeviceToHost, st);
             if (e != hipSuccess) return e;
         }
@@ -753,7 +771,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
         return hipMemcpy2DAsync(hout, pre, ctx->results, kResultBlock, pre, (size_t)n, hipMemcpyDeviceToHost, st);
     };
     HIPCHK(hipEventRecord(ctx->ev0, st));
-    if (ctx->graphs) {
+    if (ctx->graphs && ctx->profiling != 2) {
         // one graph per call shape: everything the stream work depends on
         std::vector<int64_t> key = {n,       max_qw,         max_qh,         key_bits,      cl_bits,
                                     blur,    deglitch,       nfam,           ccl.front,     ccl.replay,
@@ -826,7 +844,10 @@ int chain_collect(atg_ctx *ctx) {
     ctx->stage_ms.clear();
     ctx->stage_names.clear();
     for (int k = 1; k < mk->n; k++) {
-        ctx->stage_ms.push_back((double)(ctx->hstamps[k] - ctx->hstamps[k - 1]) / ctx->clock_khz);
+        float ev_ms = 0.0f;
+        if (mk->level == 2) HIPCHK(hipEventElapsedTime(&ev_ms, mk->ev[k - 1], mk->ev[k]));
+        ctx->stage_ms.push_back(mk->level == 2 ? (double)ev_ms
+                                               : (double)(ctx->hstamps[k] - ctx->hstamps[k - 1]) / ctx->clock_khz);
         ctx->stage_names.push_back(mk->name[k]);
     }
     ctx->dets.assign((size_t)n, {});
@@ -856,7 +877,7 @@ int chain_collect(atg_ctx *ctx) {
         uint32_t first = std::min<uint32_t>(nd, kInlineDets);
         if (first) memcpy(v.data(), d, first * sizeof(at_detection));
         if (nd > first)
-            HIPCHK(hipMemcpy(v.data() + first, ctx->hframes[(size_t)i].dtmp + first, (nd - first) * sizeof(at_detection),
+            HIPCHK(hipMemcpy(v.data() + first, ctx->hframes[(size_t)i].dfin + first, (nd - first) * sizeof(at_detection),
                              hipMemcpyDeviceToHost));
     }
     if (!first_err.empty()) return fail(ctx, ATG_ERR_LIMIT, first_err);
@@ -1027,7 +1048,7 @@ extern "C" int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8
         *nquads = cnt[kNQuads];
         if (first) memcpy(dets, stage + kNaResult + kCountersBytes, (size_t)first * sizeof(at_detection));
         if (take > first)
-            HIPCHK(hipMemcpy(dets + first, F.dtmp + first, (size_t)(take - first) * sizeof(at_detection),
+            HIPCHK(hipMemcpy(dets + first, F.dfin + first, (size_t)(take - first) * sizeof(at_detection),
                              hipMemcpyDeviceToHost));
         return ATG_OK;
     }
@@ -1056,8 +1077,12 @@ extern "C" int atg_copy_to_device(atg_ctx *ctx, void *dst, const void *src, size
 }
 
 extern "C" int atg_set_profiling(atg_ctx *ctx, int on) {
-    if (!ctx) return ATG_ERR_ARG;
-    ctx->profiling = on != 0;
+    if (!ctx || on < 0 || on > 2) return ATG_ERR_ARG;
+    if (on == 2 && !ctx->marks.ev[0]) {
+        HIPCHK(hipSetDevice(ctx->device));
+        for (int k = 0; k < kMaxMarks; k++) HIPCHK(hipEventCreate(&ctx->marks.ev[k]));
+    }
+    ctx->profiling = on;
     return ATG_OK;
 }
 
@@ -1099,9 +1124,10 @@ extern "C" int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *
                             atg_tune *best_out, double *best_ms) {
     if (!ctx || !p || !frames || n < 1) return ATG_ERR_ARG;
     const Tune start = ctx->tune;
-    const bool graphs = ctx->graphs, prof = ctx->profiling;
+    const bool graphs = ctx->graphs;
+    const int prof = ctx->profiling;
     ctx->graphs = false;
-    ctx->profiling = false;
+    ctx->profiling = 0;
     std::vector<std::vector<at_detection>> want;
     std::vector<int64_t> want_q;
     auto measure = [&](const Tune &t, double *ms) -> int {
@@ -1178,21 +1204,23 @@ extern "C" int atg_debug_copy(atg_ctx *ctx, void *dst, const void *src, int64_t 
 }
 
 extern "C" int atg_debug_finish(atg_ctx *ctx, at_detection *dets, int64_t *n) {
-    if (!ctx || !n || *n < 0 || (*n > 0 && !dets)) return ATG_ERR_ARG;
+    if (!ctx || !n || *n < 0 || *n > (int64_t)1 << 24 || (*n > 0 && !dets)) return ATG_ERR_ARG;
     if (*n == 0) return ATG_OK;
     HIPCHK(hipSetDevice(ctx->device));
     const size_t bytes = sizeof(at_detection) * (size_t)*n;
+    // the list, the output, 5 n u32 of scratch, the count
+    const size_t total = align256(2 * bytes + 20 * (size_t)*n);
     uint8_t *buf = nullptr;
-    HIPCHK(hipMalloc(&buf, bytes + 256));
-    int64_t *dn = reinterpret_cast<int64_t *>(buf + align256(bytes));
+    HIPCHK(hipMalloc(&buf, total + 256));
+    int64_t *dn = reinterpret_cast<int64_t *>(buf + total);
     int rc = [&]() -> int {
         HIPCHK(hipMemcpyAsync(buf, dets, bytes, hipMemcpyHostToDevice, ctx->stream));
-        launch_debug_finish(ctx->stream, reinterpret_cast<at_detection *>(buf), *n, dn);
+        launch_debug_finish(ctx->stream, buf, (uint32_t)*n, dn);
         HIPCHK(hipGetLastError());
         int64_t kept = 0;
         HIPCHK(hipMemcpyAsync(&kept, dn, sizeof kept, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipMemcpy(dets, buf, sizeof(at_detection) * (size_t)kept, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dets, buf + bytes, sizeof(at_detection) * (size_t)kept, hipMemcpyDeviceToHost));
         *n = kept;
         return ATG_OK;
     }();

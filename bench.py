@@ -10,14 +10,19 @@ The frames are resident in HBM before the timed region.
 A step is one pass of the detection chain over one batch of B frames
 (the 10 frames repeated); `value` is the batched frames/s. Beside it:
 the single-frame path (one frame per call) frames/s and its p50/p99
-latency, bit-exactness of both against the oracle, per-stage device
-times, the roofline of the dominant kernel, and the reference CPU
-detector timed on the host (1 thread and all cores).
+latency, bit-exactness of both against the oracle, per-kernel device
+times (HIP events on the chain's stream), the roofline of the dominant
+kernel on the algorithmic bytes of its stage, the reference CPU detector
+timed on the host (1 thread and all cores, median of 3 runs, alternating
+with the GPU), the 640x480 and 3840x2160 configurations one frame at a
+time against the CPU, batches from host memory, the uneven 64-frame batch
+and the serving sweep.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -41,19 +46,61 @@ def pctl(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# The kernels behind each timing mark of the chain (the names rocprofv3
+# shows), and the stage of the detection path each belongs to: (a)
+# decimate and threshold (with the run-length encoding fused into it), (b)
+# union-find, (c) roots, points and cluster order, (d) quad fit, (e) decode.
+MARK_KERNELS = {
+    "decimate": ["k_decimate_tiles"], "blur": ["k_blur_rows", "k_blur_cols", "k_tiles"],
+    "threshold_rle": ["k_threshold"], "uf_edges": ["k_uf_edges<true>", "k_uf_edges<false>"],
+    "uf_front.sweep": ["k_front<1024>", "k_front<512>", "k_front<256>", "k_front<128>", "k_front<64>",
+                       "k_front_local<1024>", "k_front_carry<1024>", "k_front_final<1024>"],
+    "uf_front.dense": ["k_front_dense"], "uf_replay.rows": ["k_replay<true>", "k_replay<false>"],
+    "uf_replay.wide": ["k_replay_wide<true>", "k_replay_wide<false>", "k_replay_hbm", "k_work_offsets"],
+    "uf_stitch.jump": ["k_root_jump"], "uf_stitch.lost": ["k_lost"], "resolve": ["k_resolve"],
+    "labels.roots": ["k_root_count", "k_root_dense"], "labels.rows": ["k_labels"],
+    "points.count": ["k_point_rows", "k_point_scan"], "points.write": ["k_point_rows"],
+    "cluster_sort": ["k_radix_hist", "k_radix_offsets", "k_radix_scatter"],
+    "clusters.gather": ["k_cluster_count"], "clusters.write": ["k_cluster_write"],
+}
+
+
+def stage_of(mark):
+    g = mark.split(".")[0]
+    if g in ("decimate", "blur", "threshold_rle"):
+        return "a"
+    if g.startswith("uf_") or g == "resolve":
+        return "b"
+    if g in ("labels", "points", "cluster_sort", "clusters"):
+        return "c"
+    if g in ("fit_prep", "angle_sort", "moments", "corners"):
+        return "d"
+    return "e"
+
+
 def measured_traffic(batch):
-    """HBM bytes per launch of each kernel group at this batch size, as the
+    """HBM bytes per launch of each kernel at this batch size, as the
     rocprofv3 FETCH_SIZE and WRITE_SIZE runs measured them (scaled by the
-    copy kernel's readings): the table of profiles/traffic_1080p_batch<B>.txt,
-    if that profile exists."""
+    copy kernel's readings): the kernel rows of
+    profiles/traffic_1080p_batch<B>.txt, if that profile exists."""
     path = REPO / "profiles" / ("traffic_1080p_batch%d.txt" % batch)
     out = {}
     if not path.exists():
         return out
     for line in path.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0].startswith("group:"):
-            out[cells[0][len("group:"):].strip()] = int(cells[3])
+        if len(cells) >= 5 and cells[0].startswith("kernel:"):
+            out[cells[0][len("kernel:"):].strip()] = int(cells[4])
     return out
 
 
@@ -73,20 +120,24 @@ def frame_counts(ctx, i, W, H):
                 S=(c[11] + 10) // 11, SB=24 if packed else 32)
 
 
-def kernel_bytes(n):
-    """Algorithmic HBM bytes per frame of each kernel group: every array
-    it reads or writes, once (a 4-byte word per run, node, link and record
-    slot; an 8-byte word per point; 48 bytes of moments per fitted point).
-    Stages (a)-(c) follow the per-frame formulas of the byte model split by
-    kernel, the cluster sort with the frame's own number of radix passes
-    (S) and bytes a point per pass (SB: 8-byte words read by the histogram
-    and read and written by the scatter, 24, or 12-byte key and index
-    pairs, 32); labels count the ok roots' numbering (8N), the gathers of
-    run labels and dense numbers (8N), t and the two label planes; points
-    read t and plab (count pass) and t, plab and dlab (write pass) and
-    write 16 bytes a point; (d)-(e) count the arrays of the fit and, for
-    decode, the pixels each quad samples (refine: 4 edges x 16 samples x 25
-    steps x 2 bilinear reads of 4 pixels)."""
+def survey_bytes(n, decimated=True):
+    """Algorithmic HBM bytes per frame of stages (a)-(c) (each logical
+    array written once by its producer and read once by each consumer):
+    (a) W H + 2 qw qh (read the frame; write the quad image and the
+    thresholded image; W H + qw qh without decimation), (b) qw qh + 10 R +
+    4 N + 4 qh (read t; write and read runs; write roots; row offsets), (c)
+    qw qh + 5 R + 4 N + 8 P (read t, runs and roots; write points)."""
+    W, H, qw, qh, R, N, P = n["W"], n["H"], n["qw"], n["qh"], n["R"], n["N"], n["P"]
+    px = qw * qh
+    return {"a": W * H + (2 if decimated else 1) * px, "b": px + 10 * R + 4 * N + 4 * qh,
+            "c": px + 5 * R + 4 * N + 8 * P}
+
+
+def traffic_model(n):
+    """The implementation's own byte model per kernel group (every array a
+    kernel group reads or writes, once): what the chain moves, which is
+    more than the algorithmic bytes (two label planes, radix passes,
+    gathers); reported beside them, never as the roofline."""
     W, H, qw, qh = n["W"], n["H"], n["qw"], n["qh"]
     R, N, E, P, K, Q = n["R"], n["N"], n["E"], n["P"], n["K"], n["Q"]
     px, tiles = qw * qh, (qw // 4) * (qh // 4)
@@ -241,41 +292,136 @@ def batching_config4(atg, ctx, steps):
                 bit_exact_vs_single=bool(exact), detections=int(sum(len(a["dets"]) for a in alone)))
 
 
-def cpu_baseline(frames, sample_s):
+class CpuRef:
     """The reference detector (oracle/_ref, built from the reference's own
-    sources) timed on this host at 1 thread and at all cores, on a bounded
-    sample of the same frames; the C restatement if the reference build
-    is absent."""
-    from atg_testlib import Oracle, Reference, default_params, have_ref
-    cores = os.cpu_count() or 1
-    fams = [("tag36h11", 2)]
-    res = {}
-    if have_ref():
-        ref = Reference()
-        for nt in (1, cores):
-            p = default_params(nthreads=nt)
-            td = ref.detector(p, fams)
-            n, t0 = 0, time.perf_counter()
-            while time.perf_counter() - t0 < sample_s:
-                img = frames[n % len(frames)]
-                ref.lib.at_detect(td, img.ctypes.data, img.shape[1], img.shape[0], img.strides[0])
-                n += 1
-            dt = time.perf_counter() - t0
-            ref.lib.at_detector_destroy(td)
-            res[nt] = (n / dt, n)
-        return dict(value=round(res[cores][0], 2), unit="frames/s", cores=cores, kind="reference",
-                    value_1thread=round(res[1][0], 2), cores_1thread=1,
-                    sample="%d and %d frames (the 10 workload frames, cycled; ~%.0f s each) at %d and 1 threads"
-                           % (res[cores][1], res[1][1], sample_s, cores))
-    orc = Oracle()
-    p = default_params()
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < sample_s:
-        orc.detect(frames[n % len(frames)], p, fams)
-        n += 1
-    dt = time.perf_counter() - t0
-    return dict(value=round(n / dt, 2), unit="frames/s", cores=1, kind="port",
-                sample="%d frames of the workload, ~%.0f s, 1 thread" % (n, sample_s))
+    sources with its single-thread flags) timed on this host; the C
+    restatement (oracle/) where the reference build is absent."""
+
+    def __init__(self):
+        from atg_testlib import Oracle, Reference, have_ref
+        self.kind = "reference" if have_ref() else "port"
+        self.ref = Reference() if have_ref() else None
+        self.orc = None if have_ref() else Oracle()
+        self.dets = {}
+
+    def runner(self, params, fams, nthreads):
+        """A function running the detector on one frame."""
+        from atg_testlib import default_params
+        p = dict(params, nthreads=nthreads)
+        if self.ref is not None:
+            key = (json.dumps(p, sort_keys=True), tuple(fams))
+            if key not in self.dets:
+                self.dets[key] = self.ref.detector(p, fams)
+            td, lib = self.dets[key], self.ref.lib
+            return lambda img: lib.at_detect(td, img.ctypes.data, img.shape[1], img.shape[0], img.strides[0])
+        return lambda img: self.orc.detect(img, default_params(**p), fams)
+
+    def close(self):
+        for td in self.dets.values():
+            self.ref.lib.at_detector_destroy(td)
+        self.dets = {}
+
+
+def gpu_single_run(ctx, dev, prm, iters):
+    """One frame a call over the device-resident frames (cycled): latencies (s)."""
+    lat = []
+    for k in range(iters):
+        t = time.perf_counter()
+        ctx.detect(None, prm, on_device_ptr=dev[k % len(dev)])
+        lat.append(time.perf_counter() - t)
+    return lat
+
+
+def cpu_run(fn, frames, iters):
+    t0 = time.perf_counter()
+    for k in range(iters):
+        fn(frames[k % len(frames)])
+    return iters / (time.perf_counter() - t0)
+
+
+def compare_single(atg, ctx, cpu, frames, pdict, fams, iters, cpu_iters, cores):
+    """A configuration one frame at a time: the GPU single-frame path and
+    the CPU detector at 1 thread and at all cores, 3 runs each, the sides
+    alternating (runs of `iters` GPU calls and at least 15 CPU frames);
+    medians of the runs."""
+    ctx.set_families(fams)
+    prm = atg.params_from_dict(pdict)
+    dev = [ctx.to_device(f) for f in frames]
+    try:
+        gpu_single_run(ctx, dev, prm, len(dev))  # warm-up: graphs of the call shapes
+        one, alln = cpu.runner(pdict, fams, 1), cpu.runner(pdict, fams, cores)
+        one(frames[0])
+        alln(frames[0])
+        lat, g_fps, c1, cn = [], [], [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            run = gpu_single_run(ctx, dev, prm, iters)
+            g_fps.append(iters / (time.perf_counter() - t0))
+            lat += run
+            c1.append(cpu_run(one, frames, cpu_iters[0]))
+            cn.append(cpu_run(alln, frames, cpu_iters[1]))
+    finally:
+        for d in dev:
+            ctx.free_device(d[0])
+    return dict(gpu_fps=round(float(np.median(g_fps)), 2), gpu_p50_ms=round(1e3 * pctl(lat, 50), 3),
+                gpu_p99_ms=round(1e3 * pctl(lat, 99), 3), cpu_fps_1thread=round(float(np.median(c1)), 2),
+                cpu_fps_all_cores=round(float(np.median(cn)), 2),
+                runs="3 alternating runs: %d GPU calls, %d and %d CPU frames (1 thread, %d threads)"
+                     % (iters, cpu_iters[0], cpu_iters[1], cores))
+
+
+def host_input_batches(atg, frames, fams, batch, calls):
+    """Batches whose frames are in pinned host memory (copied in by DMA each
+    call, PCIe included): two contexts on two threads, so that one batch's
+    copies overlap the other's kernels; frames/s over all calls."""
+    pins = []
+    for f in frames:
+        pf = atg.PinnedFrame(*f.shape)
+        pf.array[:] = f
+        pins.append(pf)
+    prm = atg.default_params()
+    ctxs = [atg.Context(0) for _ in range(2)]
+    try:
+        for c in ctxs:
+            c.set_families(fams)
+        imgs = [pins[i % len(pins)].array for i in range(batch)]
+        for c in ctxs:
+            c.detect_batch(imgs, prm)  # warm-up
+
+        def work(c):
+            for _ in range(calls):
+                c.detect_batch(imgs, prm)
+        th = [threading.Thread(target=work, args=(c,)) for c in ctxs]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        wall = time.perf_counter() - t0
+    finally:
+        for c in ctxs:
+            c.close()
+        for pf in pins:
+            pf.free()
+    return dict(fps=round(2 * calls * batch / wall, 1), batch=batch, calls_per_context=calls, contexts=2,
+                frame_bytes=int(frames[0].nbytes))
+
+
+def kernel_profile(ctx, batch, prm, calls=5):
+    """Device time of each marked kernel of a batched call: HIP events
+    between the kernels on the chain's stream (the calls without a graph),
+    averaged over `calls` calls."""
+    ctx.set_profiling(2)
+    try:
+        ctx.detect_batch(batch, prm, on_device=True)
+        acc = {}
+        for _ in range(calls):
+            ctx.detect_batch(batch, prm, on_device=True)
+            for name, ms in ctx.stage_times():
+                acc[name] = acc.get(name, 0.0) + ms / calls
+    finally:
+        ctx.set_profiling(0)
+    return acc
 
 
 def main():
@@ -284,10 +430,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--cpu-sample-s", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (config 4) measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 640x480 and 3840x2160 comparisons")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-loads", default="500,1000,2000,4000,8000,20000")
     args = ap.parse_args()
@@ -319,21 +465,7 @@ def main():
         all(same_dets(w["dets"], g["dets"]) and w["nquads"] == g["nquads"] for w, g in zip(want, got_s))
     ndets = sum(len(w["dets"]) for w in want)
 
-    # single-frame path
-    log("timing the single-frame path")
-    for _ in range(args.warmup):
-        for d in dev:
-            ctx.detect(None, prm, on_device_ptr=d)
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(max(1, args.steps // 2)):
-        for d in dev:
-            t = time.perf_counter()
-            ctx.detect(None, prm, on_device_ptr=d)
-            lat.append(time.perf_counter() - t)
-    single_fps = len(lat) / (time.perf_counter() - t0)
-
-    # batched path
+    # batched path: the headline
     log("timing the batched path")
     batch = [dev[i % len(dev)] for i in range(args.batch)]
     for _ in range(args.warmup):
@@ -346,34 +478,79 @@ def main():
         step_t.append(time.perf_counter() - t)
     wall = time.perf_counter() - t0
     batched_fps = args.batch * args.steps / wall
-
-    # per-stage device times of one batched call (HIP events on the stream)
-    ctx.set_profiling(True)
-    ctx.detect_batch(batch, prm, on_device=True)
-    stages = ctx.stage_times()
-    ctx.set_profiling(False)
     chain_ms = ctx.chain_ms()
-    # algorithmic bytes of the batch, from the first 10 frames' counts
+
+    # per-kernel device times of the batched call, and the bytes they move
+    log("per-kernel device times")
+    kt = kernel_profile(ctx, batch, prm)
     ctx.detect_batch(dev, prm, on_device=True)
     counts = [frame_counts(ctx, i, W, H) for i in range(len(dev))]
     reps = args.batch / len(dev)
-    kb = {}
+    sb = {"a": 0.0, "b": 0.0, "c": 0.0}
+    model = {}
     for n in counts:
-        for k, v in kernel_bytes(n).items():
-            kb[k] = kb.get(k, 0) + v * reps
-    dom_name, dom_ms = max(stages, key=lambda kv: kv[1])
-    ach = kb[dom_name] / (dom_ms * 1e-3) / 1e9
-    bound = "hbm"
-    traffic = measured_traffic(args.batch).get(dom_name)
-    roof = dict(kernel=dom_name, bound=bound, achieved=round(ach, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(ach / HBM_PEAK_GBS, 6), traffic=traffic, bytes_per_launch=int(kb[dom_name]),
-                ms_per_launch=round(dom_ms, 4))
-    ac = ("decimate", "threshold_rle", "uf_edges", "uf_front", "uf_replay", "uf_stitch", "resolve", "labels",
-          "points", "cluster_sort", "clusters")
-    t_ac = sum(v for k, v in stages if k in ac)
-    b_ac = sum(kb[k] for k in ac if k in kb)
-    per_kernel = {k: {"ms": round(v, 4), "GB/s": round(kb.get(k, 0) / (v * 1e-3) / 1e9, 2)} for k, v in stages}
+        for k, v in survey_bytes(n).items():
+            sb[k] += v * reps
+        for k, v in traffic_model(n).items():
+            model[k] = model.get(k, 0.0) + v * reps
+    dom, dom_ms = max(kt.items(), key=lambda kv: kv[1])
+    st = stage_of(dom)
+    traffic_tab = measured_traffic(args.batch)
+    kernels = MARK_KERNELS.get(dom, [])
+    traffic = sum(traffic_tab[k] for k in kernels if k in traffic_tab) or None
+    if st in sb:
+        dom_bytes = sb[st]
+        bytes_note = "stage (%s) algorithmic bytes of the batch (all of them credited to this kernel)" % st
+    else:
+        dom_bytes = model[dom.split(".")[0]]
+        bytes_note = "the implementation byte model of its group (stages d-e have no algorithmic byte figure)"
+    ach = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roof = dict(kernel=dom, kernels=kernels, stage=st, bound="hbm", achieved=round(ach, 3), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 6), traffic=traffic, bytes_per_launch=int(dom_bytes),
+                ms_per_launch=round(dom_ms, 4), bytes=bytes_note)
+    t_ac = sum(v for k, v in kt.items() if stage_of(k) in sb)
+    b_ac = sum(sb.values())
+    groups = {}
+    for k, v in kt.items():
+        groups[k.split(".")[0]] = groups.get(k.split(".")[0], 0.0) + v
+    ac_traffic = sum(v for k, v in traffic_tab.items()
+                     if any(k in MARK_KERNELS[m] for m in MARK_KERNELS if stage_of(m) in sb)) or None
+    stages_a_c = {"bytes": int(b_ac), "bytes_per_stage": {k: int(v) for k, v in sb.items()}, "ms": round(t_ac, 4),
+                  "GB/s": round(b_ac / (t_ac * 1e-3) / 1e9, 2), "frac": round(b_ac / (t_ac * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+                  "measured_traffic": ac_traffic,
+                  "traffic_over_algorithmic": round(ac_traffic / b_ac, 2) if ac_traffic else None}
+    model_gbs = {g: {"ms": round(groups[g], 4), "model_bytes": int(model[g]),
+                     "GB/s": round(model[g] / (groups[g] * 1e-3) / 1e9, 2)} for g in model if g in groups}
 
+    cores = os.cpu_count() or 1
+    cpu = None if args.no_cpu else CpuRef()
+    # the headline configuration one frame at a time, GPU and CPU alternating
+    log("single-frame path against the CPU detector")
+    if cpu is not None:
+        c2 = compare_single(atg, ctx, cpu, frames, pdict, fams, 10 * max(1, args.steps // 2), (30, 90), cores)
+    else:
+        ctx.set_families(fams)
+        gpu_single_run(ctx, dev, prm, len(dev))
+        t0 = time.perf_counter()
+        lat = gpu_single_run(ctx, dev, prm, 10 * max(1, args.steps // 2))
+        c2 = dict(gpu_fps=round(len(lat) / (time.perf_counter() - t0), 2), gpu_p50_ms=round(1e3 * pctl(lat, 50), 3),
+                  gpu_p99_ms=round(1e3 * pctl(lat, 99), 3))
+    ctx.set_families(fams)
+    configs = None
+    if cpu is not None and not args.no_configs:
+        log("configurations 1 and 3 one frame at a time")
+        c1_frames = [atg.synth_frame(1, s)[0] for s in range(8)]
+        c3_frames = [atg.synth_frame(3, s)[0] for s in range(2)]
+        configs = {
+            "640x480 tag36h11 decimate 2 (8 frames)": compare_single(atg, ctx, cpu, c1_frames, pdict, fams, 80,
+                                                                    (60, 120), cores),
+            "3840x2160 decimate 1, tag36h11 (2) + tagStandard52h13 (1), clutter (2 frames)": compare_single(
+                atg, ctx, cpu, c3_frames, default_params(quad_decimate=1.0),
+                [("tag36h11", 2), ("tagStandard52h13", 1)], 20, (16, 30), cores),
+        }
+        ctx.set_families(fams)
+    log("batches from pinned host memory")
+    host_in = host_input_batches(atg, frames, fams, args.batch, max(2, args.steps // 4))
     batching = None
     if not args.no_batching:
         log("config 4: uneven batch")
@@ -386,7 +563,13 @@ def main():
     if not args.no_serve:
         serving = serving_sweep(atg, ctx, [int(x) for x in args.serve_loads.split(",")], args.serve_seconds,
                                 check_ref=not args.no_cpu)
-    cpu = None if args.no_cpu else cpu_baseline(frames, args.cpu_sample_s)
+    cpu_line = None
+    if cpu is not None:
+        cpu_line = dict(value=c2["cpu_fps_all_cores"], unit="frames/s", cores=cores, kind=cpu.kind,
+                        value_1thread=c2["cpu_fps_1thread"], cores_1thread=1, cpu_model=cpu_model(),
+                        sample="the 10 workload frames cycled: median of 3 runs of 30 frames (1 thread) and "
+                               "90 frames (%d threads), alternating with GPU single-frame runs" % cores)
+        cpu.close()
     out = {
         "metric": "frames/s @1080p tag36h11 batched vs single-frame; p50/p99 latency, bit-exact",
         "value": round(batched_fps, 2),
@@ -403,21 +586,24 @@ def main():
         "config": {"workload": "10 seeded 1920x1080 frames, 20 tag36h11 tags each, decimate 2, "
                                "refine_edges 1, bits_corrected 2; batch of %d frames (the 10 cycled)" % args.batch,
                    "batch": args.batch, "frame": "1920x1080 u8"},
-        "single_frame_fps": round(single_fps, 2),
-        "batched_over_single": round(batched_fps / single_fps, 2),
-        "latency_ms_single": {"p50": round(1000 * pctl(lat, 50), 3), "p99": round(1000 * pctl(lat, 99), 3)},
+        "single_frame_fps": c2["gpu_fps"],
+        "batched_over_single": round(batched_fps / c2["gpu_fps"], 2),
+        "latency_ms_single": {"p50": c2["gpu_p50_ms"], "p99": c2["gpu_p99_ms"]},
         "latency_ms_batch": {"p50": round(1000 * pctl(step_t, 50), 3), "p99": round(1000 * pctl(step_t, 99), 3)},
         "bit_exact": bool(bit_exact),
         "detections_per_10_frames": int(ndets),
         "device_chain_ms": round(chain_ms, 3),
-        "stage_ms": {k: round(v, 4) for k, v in stages},
-        "kernel_gbs": per_kernel,
-        "stages_a_c": {"bytes": int(b_ac), "ms": round(t_ac, 4), "GB/s": round(b_ac / (t_ac * 1e-3) / 1e9, 2)},
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
         "roofline": roof,
+        "stages_a_c": stages_a_c,
+        "traffic_model": model_gbs,
         "copy_kernel_gbs": round(copy_gbs, 1),
+        "single_frame_vs_cpu": c2,
+        "configs_single_frame": configs,
+        "batched_host_input": host_in,
         "batching_uneven": batching,
         "serving": serving,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_line,
     }
     for d in dev:
         ctx.free_device(d[0])

@@ -162,9 +162,10 @@ int atg_debug_force_scratch(atg_ctx *ctx, int on);
  * detect call's kernel chain. */
 int atg_last_timing(const atg_ctx *ctx, double *chain_ms);
 
-/* Per-stage device times of the following calls: a one-lane kernel
- * between the kernel groups stamps the device wall clock (a few us each;
- * graph replay stays on). */
+/* Per-kernel device times of the following calls (on: 1) a one-lane
+ * kernel between the kernel groups stamps the device wall clock (a few us
+ * each; graph replay stays on); 2) HIP events between them on the
+ * context's stream (the calls then run without graphs); 0) off. */
 int atg_set_profiling(atg_ctx *ctx, int on);
 /* The stages of the last call: names[i] and ms[i] for i < the return
  * value (at most cap are written). */

@@ -91,3 +91,23 @@ def test_work_split_outside_the_kernels_bounds_is_refused(atg, ctx):
     with pytest.raises(RuntimeError):
         ctx.set_tune(1, 8, 256, 256)
     assert ctx.get_tune()[1] <= 4
+
+
+def test_pinned_host_frames_equal_pageable(atg, ctx):
+    """Host frames in page-locked memory are copied in by DMA as they lie;
+    pageable ones through the context's staging: the same detections."""
+    frames = uneven_frames(atg)
+    ctx.set_families(FAMS)
+    pins = []
+    try:
+        for f in frames:
+            pf = atg.PinnedFrame(*f.shape)
+            pf.array[:] = f
+            pins.append(pf)
+        a = ctx.detect_batch(frames)
+        b = ctx.detect_batch([p.array for p in pins])
+        c = ctx.detect_batch([p.array for p in pins])  # the graph of the DMA call shape, replayed
+    finally:
+        for pf in pins:
+            pf.free()
+    assert same(a, b) and same(a, c)
