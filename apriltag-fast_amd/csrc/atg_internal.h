@@ -17,7 +17,6 @@
 //                              weight, what its root lost to, its root
 //   nrec    u32 [qh]           records (sets touching the row) per row
 //   plab    u32 [qh][qw]       per-pixel nlab (ok 0 where t is 127)
-//   dlab    u32 [qh][qw]       per-pixel dense number of the root (ok pixels)
 //   pcnt/poff u32 [qh + 1]     points per row / their offsets
 //   pts     u64 [pcap]         boundary points in emission order
 //   skey/sval, skey2/sval2     radix-sort ping-pong (u64 key, u32 value)
@@ -124,8 +123,7 @@ enum {
     kNKeptPts,  // points of the clusters that go on to the quad fit
     kNLarge = 8,  // kept clusters above kLargeCluster points, at most kHugeCluster
     kNSmall = 9,  // kept clusters of at most kLargeCluster points
-    kRootBits = 10,  // bits of the dense ok-root numbers
-    kSortBits = 11,  // bits of the cluster sort key
+    kSortBits = 11,  // bits of the cluster sort key: the cluster ranks' bits
     kNWideRows = 12,  // rows the first replay launch left (over its run or link capacity), listed in rwide
     kNHuge = 13,  // kept clusters above kHugeCluster points
     kNCounters = 16
@@ -136,8 +134,6 @@ struct Frame {
     int64_t stride;
     int32_t W, H;
     int32_t qw, qh, tw, th;
-    int32_t nbits_node;   // node ids < 2^nbits_node
-    int32_t key_bits;     // bits of the cluster sort key
     uint64_t bucket_mask; // cluster hash buckets - 1 (fast/clusters.c:97-103)
     int32_t max_px;       // 2 * (2 qw + 2 qh) (fast/quads.c:1150)
     int32_t blurred;      // the quad image was blurred or sharpened
@@ -151,7 +147,6 @@ struct Frame {
     uint32_t *rlab;           // = nlab until k_resolve: record of each run within its row
     uint32_t *recw, *rlost, *rroot, *nrec;
     uint32_t *uedge, *ecnt;   // [qh][3 qw] union-find links per row, [qh] their counts
-    uint32_t *dlab;           // [qh][qw] the dense number of each ok pixel's root (uedge's memory, free by then)
     uint32_t *pcnt, *poff;
     uint64_t *pts, *cpts;
     uint64_t *skey, *skey2;
@@ -160,8 +155,13 @@ struct Frame {
     uint64_t *pw;         // [pcap] kept clusters' points: x | y << 16 | squared gradient << 32,
                           // in cluster order (the memory of pts, free by then)
     uint32_t *rwide;      // [qh] rows for the replay's wide launch
-    uint32_t *rdense;     // [nodes] dense number of each ok root, in node order
-    uint32_t *rinv;       // [nodes] the ok root of each dense number
+    // the frame's cluster table (open addressing, hmask + 1 slots): each
+    // cluster's id (kSlotEmpty where none), point count and rank; the
+    // slots in insertion order ([hmask + 1) / 2]); the sort's scratch
+    uint32_t hmask;
+    uint64_t *hkey;
+    uint32_t *hcnt, *hrank, *clist;
+    uint64_t *csort;
     uint32_t *cstart, *ccount, *cflag;
     uint32_t *klist;      // [2][kcap] kept clusters: the huge ones, then the large ones; at kcap the small ones
     uint32_t kcap;

@@ -1126,8 +1126,7 @@ __global__ void k_resolve(const Frame *frames, const DevParams *P) {
 }
 
 // One wave per row: plab[y][x] = nlab of the run under pixel x (0 where
-// t is 127), the row's virtual node at x = qw-1 (fast/clusters.c:123-142);
-// dlab[y][x] = the dense number of its root where that root is ok.
+// t is 127), the row's virtual node at x = qw-1 (fast/clusters.c:123-142).
 __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     int y = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1135,7 +1134,6 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     const int lane = threadIdx.x & 63;
     const uint8_t *trow = F.t + (int64_t)y * F.qw;
     uint32_t *out = F.plab + (int64_t)y * F.qw;
-    uint32_t *dout = F.dlab + (int64_t)y * F.qw;
     const uint32_t *nl = F.nlab + F.row_off[y];
     const int xmax = F.qw - 2;
     int count = 0;
@@ -1150,17 +1148,13 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
         uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
         if (valid) {
             int idx = count + __popcll(sm & le) - 1;
-            const uint32_t v = tv == 127u ? 0u : nl[idx];
-            out[x] = v;
-            dout[x] = (v >> 31) ? F.rdense[v & 0x7fffffffu] : 0u;
+            out[x] = tv == 127u ? 0u : nl[idx];
         }
         count += __popcll(sm);
         carry_t = (uint32_t)__builtin_amdgcn_readlane((int)tv, 63);
     }
     if (lane == 0) {
-        const uint32_t v = F.nlab[F.row_off[F.qh] + y];
-        out[F.qw - 1] = v;
-        dout[F.qw - 1] = (v >> 31) ? F.rdense[v & 0x7fffffffu] : 0u;
+        out[F.qw - 1] = F.nlab[F.row_off[F.qh] + y];
     }
 }
 
@@ -1254,7 +1248,6 @@ CclPlan ccl_plan(int max_lds_qw, int max_qw, int64_t max_weight, bool force_scra
 }
 
 void launch_work_offsets(hipStream_t st, const Frame *dframes, int nframes, WorkQueue *Q, int l, int which);
-void launch_root_numbers(hipStream_t st, const Frame *dframes, int nframes);
 
 void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int max_qw,
                       int max_qh, const CclPlan &c, WorkQueue *Q, const Tune &tu, Marks *mk) {
@@ -1341,8 +1334,6 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
     hipLaunchKernelGGL(k_resolve, dim3((unsigned)((maxnodes + 255) / 256), nframes), dim3(256), 0, st, dframes,
                        dparams);
     mk->mark(st, "resolve");
-    launch_root_numbers(st, dframes, nframes);  // the dense numbers k_labels writes
-    mk->mark(st, "labels.roots");
     hipLaunchKernelGGL(k_labels, rg, rb, 0, st, dframes);
     mk->mark(st, "labels.rows");
 }

@@ -1,28 +1,24 @@
 // Stage (c): boundary points, their clusters and the cluster order.
 //
-//   k_root_*       the roots of clusters that can hold points (the ok
-//                  roots) numbered densely in node order (launched before
-//                  k_labels, which writes each pixel's number to dlab).
 //   k_point_rows   one wave per row pair (y, y+1), y in 1..qh-2: the 4-bit
 //                  neighbour mask of each pixel (fast/clusters.c:150-174);
-//                  pass 0 counts points per row, pass 1 writes each point
-//                  at its emission index (row offset + wave prefix) with
-//                  its cluster sort key.
+//                  pass 0 counts points per row (and clears the frame's
+//                  cluster table), pass 1 writes each point at its
+//                  emission index with its cluster's table slot: each run
+//                  of points of one cluster (id = max_root << 32 |
+//                  min_root, fast/clusters.c:229) finds or inserts the id
+//                  in the table and adds its length to the cluster's count.
 //   k_point_scan   the row offsets of the points.
-//   radix sort     stable LSD sort of (key, index), kRadixBits a pass: the
-//                  keys order the clusters by (hash bucket, id) as the
-//                  reference's cluster table does (fast/clusters.c:338-356);
-//                  stability keeps each cluster's points in emission order.
-//                  Where the key and the point index fit one word (the
-//                  index above the key's bits: a packed frame) the sort
-//                  moves that word alone.
-//   k_cluster_*    the points regrouped and the cluster starts compacted.
-//
-// Cluster sort key of a point: bucket << 2D | dense(max_root) << D |
-// dense(min_root), D the bits of the ok-root count. The dense numbers keep
-// the roots' order, so comparing keys equals comparing (bucket, id) for
-// id = max_root << 32 | min_root; the passes beyond the key's 2D + bucket
-// bits (known on the device only) are skipped.
+//   k_cluster_rank one block per frame: the distinct clusters sorted by
+//                  (hash bucket, id), the order of the reference's cluster
+//                  table (fast/clusters.c:338-356), giving each its rank,
+//                  its id and its first point (a scan of the counts).
+//   k_rank_words   each point's word: its cluster's rank | its index above
+//                  the rank's bits.
+//   radix sort     stable LSD sort of the words, kRadixBits a pass (one
+//                  pass up to 2048 clusters): stability keeps each
+//                  cluster's points in emission order; the last pass
+//                  writes each point itself in place.
 #include <hip/hip_runtime.h>
 
 #include "atg_internal.h"
@@ -50,18 +46,44 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
 
 __device__ __forceinline__ uint32_t opp(uint32_t a, uint32_t b) { return (a ^ b) == 255u ? 1u : 0u; }
 
-// Whether the frame's cluster sort is packed: its key bits (counters
-// [kSortBits], known once the ok roots are numbered) and the bits of a
-// point index (below pcap) fit one word.
+// Whether a frame's radix sort moves packed words: its key bits
+// (counters[kSortBits]: the cluster ranks' bits) and the bits of a point
+// index (below pcap) fit one word, as they do for the cluster sort.
 __device__ __forceinline__ bool sort_packed(const Frame &F) {
     return F.counters[kSortBits] + (64u - (uint32_t)__clzll((unsigned long long)F.pcap)) <= 64u;
+}
+
+constexpr uint64_t kSlotEmpty = ~0ull;  // (no cluster id: its two roots differ)
+
+// The slot of cluster `id` in the frame's table, inserted if new (its slot
+// then listed in clist, counted in counters[kNClusters]); linear probing.
+// A table fuller than half sets error flag 128 (the call is re-run with a
+// table sized for the worst case).
+__device__ uint32_t cluster_slot(const Frame &F, uint64_t id) {
+    uint32_t s = (uint32_t)((id * 0x9E3779B97F4A7C15ull) >> 32) & F.hmask;
+    for (uint32_t probe = 0; probe <= F.hmask; probe++, s = (s + 1) & F.hmask) {
+        const uint64_t k = __atomic_load_n(&F.hkey[s], __ATOMIC_RELAXED);
+        if (k == id) return s;
+        if (k != kSlotEmpty) continue;
+        const uint64_t old = atomicCAS((unsigned long long *)&F.hkey[s], (unsigned long long)kSlotEmpty,
+                                       (unsigned long long)id);
+        if (old == kSlotEmpty) {
+            const uint32_t at = atomicAdd(F.counters + kNClusters, 1u);
+            if (at < (F.hmask + 1) / 2) F.clist[at] = s;
+            else atomicOr(F.counters + kError, 128u);
+            return s;
+        }
+        if (old == id) return s;
+    }
+    atomicOr(F.counters + kError, 128u);
+    return 0;
 }
 __device__ __forceinline__ uint64_t low_bits(uint32_t b) { return b >= 64 ? ~0ull : (1ull << b) - 1ull; }
 
 // One pixel column of the row pair: t of rows a and b, their plab (root
-// | ok << 31) and, for the write pass, their dlab (dense root numbers).
+// | ok << 31).
 struct PixCol {
-    uint32_t ta, tb, la, lb, da, db;
+    uint32_t ta, tb, la, lb;
 };
 
 // The point mask of pixel x of row a against row b (fast/clusters.c:
@@ -84,26 +106,27 @@ __device__ __forceinline__ uint32_t point_mask(const PixCol &c, const PixCol &l,
 // and lane 63's column in every lane.
 __device__ __forceinline__ PixCol col_from_left(const PixCol &c, const PixCol &fill) {
     return {wave_from_left(c.ta, fill.ta), wave_from_left(c.tb, fill.tb), wave_from_left(c.la, fill.la),
-            wave_from_left(c.lb, fill.lb), wave_from_left(c.da, fill.da), wave_from_left(c.db, fill.db)};
+            wave_from_left(c.lb, fill.lb)};
 }
 __device__ __forceinline__ PixCol col_from_right(const PixCol &c, const PixCol &fill) {
     return {wave_from_right(c.ta, fill.ta), wave_from_right(c.tb, fill.tb), wave_from_right(c.la, fill.la),
-            wave_from_right(c.lb, fill.lb), wave_from_right(c.da, fill.da), wave_from_right(c.db, fill.db)};
+            wave_from_right(c.lb, fill.lb)};
 }
 __device__ __forceinline__ PixCol col_of_lane63(const PixCol &c) {
     return {(uint32_t)__builtin_amdgcn_readlane((int)c.ta, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.tb, 63),
-            (uint32_t)__builtin_amdgcn_readlane((int)c.la, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.lb, 63),
-            (uint32_t)__builtin_amdgcn_readlane((int)c.da, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.db, 63)};
+            (uint32_t)__builtin_amdgcn_readlane((int)c.la, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.lb, 63)};
 }
 
 // A wave per row pair (y, y + 1), 64 pixels a step: each lane's column of
 // the step, the column right of the step (for lane 63) and both of the
 // next step loaded during this one; neighbours by shuffles, the left one
 // of lane 0 kept from the step before. Pass 0 counts the points of each
-// row; pass 1 stages each step's points (at most 4 a pixel) in LDS and
-// writes them out contiguously with their cluster sort keys (bucket << 2D
-// | dense(max root) << D | dense(min root), and the point's index above
-// the key's bits when the frame's sort is packed).
+// row and clears the frame's cluster table (the rows' waves share it out);
+// pass 1 stages each step's points (at most 4 a pixel) with their cluster
+// ids in LDS and writes them out contiguously with their clusters' slots:
+// the first point of each run of one cluster (in emission order, across
+// steps) looks the cluster up, and each run within a 64-point chunk adds
+// its length to the cluster's count.
 __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pass) {
     __shared__ uint64_t st_pt[4][256], st_key[4][256];
     const Frame F = frames[blockIdx.y];
@@ -111,30 +134,31 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
     int y = 1 + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (y > F.qh - 2) return;
     const int QW = F.qw;
+    if (!pass) {
+        const uint32_t nrows = (uint32_t)(F.qh - 2), slots = F.hmask + 1;
+        for (uint32_t i = (uint32_t)(y - 1) * 64u + (uint32_t)lane; i < slots; i += nrows * 64u) {
+            F.hkey[i] = kSlotEmpty;
+            F.hcnt[i] = 0u;
+        }
+    }
     const uint8_t *a = F.t + (int64_t)y * QW, *b = a + QW;
     const uint32_t *la = F.plab + (int64_t)y * QW, *lb = la + QW;
-    const uint32_t *da = F.dlab + (int64_t)y * QW, *db = da + QW;
     auto load = [&](int x) -> PixCol {
-        PixCol c{0, 0, 0, 0, 0, 0};
+        PixCol c{0, 0, 0, 0};
         if (x < QW) {
             c.ta = a[x];
             c.tb = b[x];
             c.la = la[x];
             c.lb = lb[x];
-            if (pass) {
-                c.da = da[x];
-                c.db = db[x];
-            }
         }
         return c;
     };
     uint32_t count = 0;
     uint32_t base = pass ? F.poff[y] : 0u;
-    const uint32_t D = pass ? F.counters[kRootBits] : 0u;
-    const uint32_t sbits = pass ? F.counters[kSortBits] : 0u;
-    const bool packed = pass && sort_packed(F);
-    const uint64_t mask = F.bucket_mask;
-    PixCol cur = load(lane), cr = load(64), left{0, 0, 0, 0, 0, 0};
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint64_t prev_id = kSlotEmpty;  // the cluster of the row pair's last written point (none yet)
+    uint32_t prev_slot = 0;
+    PixCol cur = load(lane), cr = load(64), left{0, 0, 0, 0};
     for (int x0 = 0; x0 < QW - 1; x0 += 64) {
         const PixCol nxt = load(x0 + 64 + lane), nr = load(x0 + 128);
         const int x = x0 + lane;
@@ -146,23 +170,18 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
             uint32_t k = inc - c;  // this lane's first point in the step
             if (m) {
                 const int vd = 255 - 2 * (int)cur.ta;
-                const uint32_t r0 = cur.la & 0x7fffffffu, d0 = cur.da;
+                const uint32_t r0 = cur.la & 0x7fffffffu;
                 for (int j = 0; j < 4; j++) {
                     if (!((m >> j) & 1u)) continue;
                     // partner (dx, dy): (1, 0), (0, 1), (-1, 1), (1, 1)
                     const int dx = j == 0 ? 1 : j == 1 ? 0 : j == 2 ? -1 : 1;
                     const int dy = j == 0 ? 0 : 1;
                     const uint32_t r1 = (j == 0 ? r.la : j == 1 ? cur.lb : j == 2 ? l.lb : r.lb) & 0x7fffffffu;
-                    const uint32_t d1 = j == 0 ? r.da : j == 1 ? cur.db : j == 2 ? l.db : r.db;
-                    // dense numbers keep the roots' order
                     const uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
-                    const uint32_t dh = r0 > r1 ? d0 : d1, dl = r0 > r1 ? d1 : d0;
-                    const uint64_t id = ((uint64_t)hi << 32) + lo;
-                    const uint64_t bucket = ((id * 2654435761ull) >> 32) & mask;
                     const uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
                     const uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
                     st_pt[wv][k] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
-                    st_key[wv][k] = (bucket << (2 * D)) | ((uint64_t)dh << D) | dl;
+                    st_key[wv][k] = ((uint64_t)hi << 32) | lo;
                     k++;
                 }
             }
@@ -170,11 +189,37 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             const uint32_t at0 = base + count;
-            for (uint32_t q = (uint32_t)lane; q < tot; q += 64) {
+            for (uint32_t q0 = 0; q0 < tot; q0 += 64) {
+                const uint32_t q = q0 + (uint32_t)lane;
+                const bool valid = q < tot;
+                const uint64_t id = valid ? st_key[wv][q] : 0ull;
+                const uint64_t before = !valid ? 0ull : q == 0 ? prev_id : st_key[wv][q - 1];
+                const bool head = valid && id != before;  // the first point of a run of its cluster
+                uint32_t slot = head ? cluster_slot(F, id) : 0u;
+                // the slot of the last head at or before each lane; lanes
+                // before the chunk's first head continue the previous run
+                const uint64_t upto = __ballot(head) & (lt | (1ull << lane));
+                const int src = upto ? 63 - __clzll((long long)upto) : 0;
+                const uint32_t from = (uint32_t)__shfl((int)slot, src, 64);
+                slot = upto ? from : prev_slot;
+                // each run of equal ids within the chunk adds its length
+                const uint32_t sl = wave_from_left(slot, 0xffffffffu);
+                const bool piece = valid && (lane == 0 || sl != slot);
+                const uint64_t pm = __ballot(piece) | ~__ballot(valid);
+                if (piece) {
+                    const uint64_t after = pm & ~(lt | (1ull << lane));
+                    const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
+                    atomicAdd(F.hcnt + slot, end - (uint32_t)lane);
+                }
                 const int64_t at = (int64_t)at0 + q;
-                if (at >= F.pcap) break;  // flagged by k_point_scan
-                F.pts[at] = st_pt[wv][q];
-                F.skey[at] = st_key[wv][q] | (packed ? (uint64_t)at << sbits : 0ull);
+                if (valid && at < F.pcap) {  // (beyond pcap: flagged by k_point_scan)
+                    F.pts[at] = st_pt[wv][q];
+                    F.sval[at] = slot;
+                }
+                const int lastl = (int)(min(tot - q0, 64u) - 1u);
+                prev_id = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(id >> 32), lastl, 64) << 32) |
+                          (uint32_t)__shfl((int)(uint32_t)id, lastl, 64);
+                prev_slot = (uint32_t)__shfl((int)slot, lastl, 64);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             __builtin_amdgcn_wave_barrier();
@@ -233,56 +278,6 @@ __device__ __forceinline__ void seg_of(uint32_t n, int b, uint32_t *s0, uint32_t
     *s1 = min(n, (uint32_t)(b + 1) * seg);
 }
 
-// count = counters[which]
-// The ok roots (nlab = root | ok << 31 names itself) of each segment of
-// nodes: counts, then dense numbers in node order (rdense) and back
-// (rinv); D and the sort key's bits into the counters.
-__device__ __forceinline__ bool ok_root(const Frame &F, uint32_t i) {
-    const uint32_t v = F.nlab[i];
-    return (v >> 31) && (v & 0x7fffffffu) == i;
-}
-
-__global__ __launch_bounds__(256) void k_root_count(const Frame *frames) {
-    const Frame F = frames[blockIdx.y];
-    const uint32_t nn = F.counters[kNRuns] + (uint32_t)F.qh;
-    uint32_t s0, s1;
-    seg_of(nn, blockIdx.x, &s0, &s1);
-    uint32_t c = 0;
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) c += ok_root(F, i) ? 1u : 0u;
-    uint32_t tot;
-    block_excl_scan(c, &tot);
-    if (threadIdx.x == 0) F.rhist[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(256) void k_root_dense(const Frame *frames) {
-    const Frame F = frames[blockIdx.y];
-    const uint32_t nn = F.counters[kNRuns] + (uint32_t)F.qh;
-    uint32_t base = 0, all = 0;
-    for (int b = 0; b < kSortBlocks; b++) {
-        const uint32_t v = F.rhist[b];
-        if (b < (int)blockIdx.x) base += v;
-        all += v;
-    }
-    uint32_t s0, s1;
-    seg_of(nn, blockIdx.x, &s0, &s1);
-    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
-        const uint32_t i = c0 + threadIdx.x;
-        const uint32_t f = (i < s1 && ok_root(F, i)) ? 1u : 0u;
-        uint32_t tot;
-        const uint32_t pre = block_excl_scan(f, &tot);
-        if (f) {
-            F.rdense[i] = base + pre;
-            F.rinv[base + pre] = i;
-        }
-        base += tot;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const uint32_t d = all <= 1 ? 1u : 32u - (uint32_t)__clz(all - 1);
-        F.counters[kRootBits] = d;
-        F.counters[kSortBits] = (uint32_t)__popcll(F.bucket_mask) + 2 * d;
-    }
-}
-
 // The lanes of the wave holding the same digit as this lane (of those
 // with `valid`).
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
@@ -308,6 +303,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int whi
     const Frame F = frames[blockIdx.y];
     const uint32_t sb = F.counters[kSortBits];
     if (dyn && shift >= (int)sb) return;
+    if (F.counters[kError] & (2u | 128u)) return;  // a call to re-run: nothing to sort
     const uint64_t km = pk && sort_packed(F) ? low_bits(sb) : ~0ull;
     const uint32_t n = F.counters[which];
     const SortBufs B = sort_bufs(F, parity);
@@ -362,6 +358,7 @@ __global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *fram
     static_assert(kPer * kOffThreads == kRadixBins, "whole digits a thread");
     const Frame F = frames[blockIdx.y];
     if (dyn && shift >= (int)F.counters[kSortBits]) return;
+    if (F.counters[kError] & (2u | 128u)) return;
     const int t = threadIdx.x;
     uint32_t carry = 0;
     for (int q = 0; q < kPer; q++) {
@@ -391,16 +388,19 @@ __global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *fram
 // first pass of an unpacked cluster sort; its points were emitted in
 // order), not read.
 __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity,
-                                                       int dyn, int iota, int pk) {
+                                                       int dyn, int iota, int pk, int gather) {
     __shared__ uint32_t off[kRadixBins];
     __shared__ uint8_t wcnt[4][kRadixBins];
     const Frame F = frames[blockIdx.y];
     const uint32_t sb = F.counters[kSortBits];
     if (dyn && shift >= (int)sb) return;
+    if (F.counters[kError] & (2u | 128u)) return;
     const bool packed = pk && sort_packed(F);
     const uint64_t km = packed ? low_bits(sb) : ~0ull;
     const uint32_t n = F.counters[which];
     const SortBufs B = sort_bufs(F, parity);
+    // gather (the cluster sort's last pass): the point of each word to its place
+    const bool to_points = gather && packed && shift + kRadixBits >= (int)sb;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int i = t; i < kRadixBins; i += 256) {
         off[i] = F.rhist[blockIdx.x * kRadixBins + i];
@@ -450,8 +450,12 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
         if (valid) {
             uint32_t pos = off[d] + rank;
             for (int k = 0; k < w; k++) pos += wcnt[k][d];
-            B.kout[pos] = key;
-            if (!packed) B.vout[pos] = val;
+            if (to_points) {
+                F.cpts[pos] = F.pts[(uint32_t)(key >> sb)];
+            } else {
+                B.kout[pos] = key;
+                if (!packed) B.vout[pos] = val;
+            }
         }
         __syncthreads();
         if (lead) {
@@ -464,89 +468,115 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
 
 // ---------------------------------------------------------------- clusters
 
-// The buffer of the cluster sort's result: one swap per executed pass.
-__device__ __forceinline__ int sorted_parity(const Frame &F) {
-    return (int)(((F.counters[kSortBits] + kRadixBits - 1) / kRadixBits) & 1);
+// The reference's cluster order: (hash bucket, id) ascending, bucket =
+// ((id * 2654435761) >> 32) & (buckets - 1) (fast/clusters.c:44,338-356).
+// The padding entries (kSlotEmpty) sort last.
+__device__ __forceinline__ bool cluster_before(uint64_t a, uint64_t b, uint64_t bm) {
+    if (a == kSlotEmpty || b == kSlotEmpty) return b == kSlotEmpty && a != kSlotEmpty;
+    const uint64_t ba = ((a * 2654435761ull) >> 32) & bm, bb = ((b * 2654435761ull) >> 32) & bm;
+    return ba < bb || (ba == bb && a < b);
 }
 
-// After the sort: the points in cluster order, and the number of cluster
-// starts in each block's segment.
-__global__ __launch_bounds__(256) void k_cluster_count(const Frame *frames) {
-    const Frame F = frames[blockIdx.y];
-    const int parity = sorted_parity(F);
-    uint32_t n = F.counters[kNPts];
-    const uint64_t *key = parity ? F.skey2 : F.skey;
-    const uint32_t *val = parity ? F.sval2 : F.sval;
-    const uint32_t sb = F.counters[kSortBits];
-    const bool packed = sort_packed(F);
-    const uint64_t km = packed ? low_bits(sb) : ~0ull;
-    uint32_t s0, s1;
-    seg_of(n, blockIdx.x, &s0, &s1);
-    uint32_t c = 0;
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += 256) {
-        const uint64_t k = key[i];
-        F.cpts[i] = F.pts[packed ? (uint32_t)(k >> sb) : val[i]];
-        c += (i == 0 || (k & km) != (key[i - 1] & km)) ? 1u : 0u;
-    }
-    uint32_t tot;
-    block_excl_scan(c, &tot);
-    if (threadIdx.x == 0) F.rhist[blockIdx.x] = tot;
-}
+constexpr int kRankThreads = 1024;
+constexpr uint32_t kRankLds = 8192;  // clusters sorted in LDS at most (else in the frame's csort)
 
-__global__ __launch_bounds__(256) void k_cluster_write(const Frame *frames) {
+// One block per frame: the clusters of the table in the reference's order
+// (bitonic network; keys distinct), then each cluster's rank (a binary
+// search in the sorted ids), its id and count at its rank, the counts
+// scanned into the clusters' first points, and the rank bits as the sort
+// key's.
+__global__ __launch_bounds__(kRankThreads) void k_cluster_rank(const Frame *frames) {
+    __shared__ uint64_t lk[kRankLds];
+    __shared__ uint32_t wsum[kRankThreads / 64];
     const Frame F = frames[blockIdx.y];
-    const int parity = sorted_parity(F);
-    uint32_t n = F.counters[kNPts];
-    const uint64_t *key = parity ? F.skey2 : F.skey;
-    const uint64_t km = sort_packed(F) ? low_bits(F.counters[kSortBits]) : ~0ull;
-    uint32_t base = 0, all = 0;
-    for (int b = 0; b < kSortBlocks; b++) {
-        uint32_t v = F.rhist[b];
-        if (b < (int)blockIdx.x) base += v;
-        all += v;
-    }
-    uint32_t s0, s1;
-    seg_of(n, blockIdx.x, &s0, &s1);
-    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
-        uint32_t i = c0 + threadIdx.x;
-        const uint64_t k = i < s1 ? key[i] & km : 0ull;
-        uint32_t f = (i < s1 && (i == 0 || k != (key[i - 1] & km))) ? 1u : 0u;
-        uint32_t tot;
-        uint32_t pre = block_excl_scan(f, &tot);
-        if (f) {
-            F.cstart[base + pre] = i;
-            // the cluster id back from the sort key's dense roots
-            const uint32_t D = F.counters[kRootBits];
-            const uint64_t dm = ((uint64_t)1 << D) - 1;
-            F.cid[base + pre] = ((uint64_t)F.rinv[(k >> D) & dm] << 32) | F.rinv[k & dm];
+    const uint32_t t = threadIdx.x;
+    if (F.counters[kError] & (2u | 128u)) {
+        // the call is re-run with more room: the later stages see no cluster
+        if (t == 0) {
+            F.counters[kNClusters] = 0;
+            F.cstart[0] = 0;
         }
-        base += tot;
+        return;
     }
-    if (blockIdx.x == kSortBlocks - 1 && threadIdx.x == 0) {
-        F.counters[kNClusters] = all;
-        F.cstart[all] = n;
+    const uint32_t ncl = F.counters[kNClusters];
+    uint32_t m = 1;
+    while (m < ncl) m <<= 1;
+    uint64_t *keys = m <= kRankLds ? lk : F.csort;
+    for (uint32_t i = t; i < m; i += kRankThreads) keys[i] = i < ncl ? F.hkey[F.clist[i]] : kSlotEmpty;
+    __syncthreads();
+    const uint64_t bm = F.bucket_mask;
+    for (uint32_t k = 2; k <= m; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < m; i += kRankThreads) {
+                const uint32_t p = i ^ j;
+                if (p <= i) continue;
+                const uint64_t a = keys[i], b = keys[p];
+                const bool up = (i & k) == 0;
+                if (up ? cluster_before(b, a, bm) : cluster_before(a, b, bm)) {
+                    keys[i] = b;
+                    keys[p] = a;
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = t; i < ncl; i += kRankThreads) {
+        const uint32_t slot = F.clist[i];
+        const uint64_t id = F.hkey[slot];
+        uint32_t lo = 0, hi = ncl;  // the first sorted key not before id
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cluster_before(keys[mid], id, bm)) lo = mid + 1; else hi = mid;
+        }
+        F.hrank[slot] = lo;
+        F.cid[lo] = id;
+        F.cstart[lo] = F.hcnt[slot];
     }
+    __syncthreads();
+    // counts -> first points (exclusive scan in rank order)
+    const uint32_t lane = t & 63, w = t >> 6;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < ncl; c0 += kRankThreads) {
+        const uint32_t i = c0 + t;
+        const uint32_t v = i < ncl ? F.cstart[i] : 0u;
+        const uint32_t inc = wave_incl_scan_u(v);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (uint32_t q = 0; q < kRankThreads / 64; q++) {
+            pre += q < w ? wsum[q] : 0u;
+            tot += wsum[q];
+        }
+        if (i < ncl) F.cstart[i] = carry + pre + inc - v;
+        carry += tot;
+        __syncthreads();
+    }
+    if (t == 0) {
+        F.cstart[ncl] = carry;
+        F.counters[kSortBits] = ncl <= 1 ? 1u : 32u - (uint32_t)__clz(ncl - 1);
+    }
+}
+
+// Each point's sort word: its cluster's rank | its index above the rank bits.
+__global__ __launch_bounds__(256) void k_rank_words(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    if (F.counters[kError] & (2u | 128u)) return;
+    const uint32_t n = F.counters[kNPts], sb = F.counters[kSortBits];
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        F.skey[i] = (uint64_t)F.hrank[F.sval[i]] | ((uint64_t)i << sb);
 }
 
 // ---------------------------------------------------------------- launchers
 
-// The ok roots numbered densely in node order (before k_labels, which
-// writes each pixel's number).
-void launch_root_numbers(hipStream_t st, const Frame *dframes, int nframes) {
-    hipLaunchKernelGGL(k_root_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_root_dense, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-}
-
 // pk: the cluster sort (packed frames move one word); else (key, value)
 // pairs throughout.
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
-                       int dyn, int iota, int pk) {
+                       int dyn, int iota, int pk, int gather) {
     for (int shift = 0; shift < nbits; shift += kRadixBits) {
         hipLaunchKernelGGL(k_radix_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
                            *parity, dyn, pk);
         hipLaunchKernelGGL(k_radix_offsets, dim3(1, nframes), dim3(kOffThreads), 0, st, dframes, shift, dyn);
         hipLaunchKernelGGL(k_radix_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity, dyn, shift == 0 ? iota : 0, pk);
+                           *parity, dyn, shift == 0 ? iota : 0, pk, gather);
         *parity ^= 1;
     }
 }
@@ -560,14 +590,15 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     mk->mark(st, "points.count");
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);
     mk->mark(st, "points.write");
-    // key_bits: the widest key the call's frames can have
+    hipLaunchKernelGGL(k_cluster_rank, dim3(1, nframes), dim3(kRankThreads), 0, st, dframes);
+    mk->mark(st, "clusters.rank");
+    hipLaunchKernelGGL(k_rank_words, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+    // key_bits: the most rank bits the call's frames can have; the passes
+    // past a frame's own return at once, the last one it runs writes the
+    // points in place
     int parity = 0;
-    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 1, 1);
+    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 0, 1, 1);
     mk->mark(st, "cluster_sort");
-    hipLaunchKernelGGL(k_cluster_count, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-    mk->mark(st, "clusters.gather");
-    hipLaunchKernelGGL(k_cluster_write, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
-    mk->mark(st, "clusters.write");
 }
 
 }  // namespace atg

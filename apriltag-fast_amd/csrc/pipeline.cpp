@@ -84,8 +84,19 @@ size_t queue_bytes(int64_t nframes) {
 // capacity bounds a frame's list of them.
 int64_t cluster_cap(int64_t pcap) { return pcap / 4 + 64; }
 
+// Slots of a frame's cluster table (a power of two, filled at most half):
+// first sized for a cluster per 32 points; after an overflow (error flag
+// 128) for every point a cluster of its own.
+int64_t hash_slots(int64_t pcap, bool full) {
+    const int64_t want = full ? 2 * pcap + 2 : std::max<int64_t>(2048, pcap / 16);
+    int64_t n = 1;
+    while (n < want) n <<= 1;
+    return n;
+}
+
 struct FramePlan {
     bool skip;            // the reference returns nothing for this frame
+    bool hfull;           // its cluster table sized for the worst case
     int64_t W, H, stride;
     int64_t qw, qh;
     int64_t pcap;
@@ -128,6 +139,8 @@ struct atg_ctx {
     std::vector<int64_t> nquads;
     double chain_ms = 0.0;
     int64_t pcap_scale = 1;           // points per quad-image pixel
+    bool hash_full = false;           // cluster tables sized for the worst case
+    bool need_points = false, need_table = false;  // the last call ran out of point or table room
     Marks marks;                      // per-stage clock stamps, when profiling
     uint64_t *hstamps = nullptr;      // pinned host copy of the stamps
     double clock_khz = 0.0;           // the device wall clock's rate
@@ -354,7 +367,6 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->nrec = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->plab = (uint32_t *)take((size_t)px * 4);
     F->uedge = (uint32_t *)take((size_t)px * 12);
-    F->dlab = F->uedge;  // written after the union-find's last use of uedge
     F->ecnt = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->pcnt = (uint32_t *)take((size_t)(qh + 1) * 4);
     F->poff = (uint32_t *)take((size_t)(qh + 2) * 4);
@@ -367,8 +379,13 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->sval2 = (uint32_t *)take((size_t)pc * 4);
     F->rhist = (uint32_t *)take((size_t)kRadixBins * kSortBlocks * 4);
     F->rwide = (uint32_t *)take((size_t)(qh + 1) * 4);
-    F->rdense = (uint32_t *)take((size_t)nodes * 4);
-    F->rinv = (uint32_t *)take((size_t)nodes * 4);
+    const int64_t hs = hash_slots(pc, fp.hfull);
+    F->hmask = (uint32_t)(hs - 1);
+    F->hkey = (uint64_t *)take((size_t)hs * 8);
+    F->hcnt = (uint32_t *)take((size_t)hs * 4);
+    F->hrank = (uint32_t *)take((size_t)hs * 4);
+    F->clist = (uint32_t *)take((size_t)hs / 2 * 4);
+    F->csort = (uint64_t *)take((size_t)hs / 2 * 8);
     const size_t fs = front_scratch_bytes((int)qw, fp.force);
     F->fscr = fs ? take(fs) : nullptr;
     F->rscr = take(replay_scratch_bytes((int)qw, fp.force));
@@ -546,10 +563,13 @@ int plan_frame(atg_ctx *ctx, const atg_params *p, int nfam, const atg_frame &f, 
     if (!fp.skip && (fp.qw < 4 || fp.qh < 4)) fp.skip = true;
     if (fp.skip) { fp.qw = fp.qh = 4; fp.W = fp.H = 8; }
     if (fp.qw > 32767 || fp.qh > 32767) return fail(ctx, ATG_ERR_LIMIT, "quad image wider or taller than 32767");
+    // node ids (runs and a virtual node a row) take 31 bits (the ok flag is
+    // the 32nd of a label)
     const int64_t nodes = fp.qw * fp.qh + fp.qh;
-    if (nodes >= ((int64_t)1 << 24)) return fail(ctx, ATG_ERR_LIMIT, "quad image over 2^24 pixels");
+    if (nodes >= ((int64_t)1 << 31)) return fail(ctx, ATG_ERR_LIMIT, "quad image over 2^31 pixels");
     fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
     fp.force = ctx->force_scratch;
+    fp.hfull = ctx->hash_full;
     Frame F{};
     fp.size = layout(fp, nfam, !on_device, &F, nullptr);
     return ATG_OK;
@@ -567,7 +587,7 @@ struct ChainShape {
         max_weight = std::max<int64_t>(max_weight, nodes);
         max_qw = std::max<int>(max_qw, (int)fp.qw);
         max_qh = std::max<int>(max_qh, (int)fp.qh);
-        key_bits = std::max(key_bits, ceil_log2((uint64_t)cluster_map_size(fp.qw, fp.qh)) + 2 * ceil_log2((uint64_t)nodes));
+        key_bits = std::max(key_bits, ceil_log2((uint64_t)hash_slots(fp.pcap, fp.hfull) / 2));  // cluster ranks
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
         max_cluster = std::max(max_cluster, (int)(2 * (2 * fp.qw + 2 * fp.qh)));
     }
@@ -586,9 +606,8 @@ void describe_frame(const DevParams &hp, const FramePlan &fp, int nfam, bool hos
     F.qh = (int32_t)fp.qh;
     F.tw = (int32_t)(fp.qw / 4);
     F.th = (int32_t)(fp.qh / 4);
-    F.nbits_node = ceil_log2((uint64_t)(fp.qw * fp.qh + fp.qh));
+    (void)key_bits;
     F.bucket_mask = (uint64_t)cluster_map_size(fp.qw, fp.qh) - 1;
-    F.key_bits = key_bits;
     F.max_px = (int32_t)(2 * (2 * fp.qw + 2 * fp.qh));
     F.blurred = hp.blur_ksz > 1 ? 1 : 0;
     F.pcap = fp.pcap;
@@ -853,17 +872,22 @@ int chain_collect(atg_ctx *ctx) {
     ctx->dets.assign((size_t)n, {});
     ctx->nquads.assign((size_t)n, 0);
     ctx->fstatus.assign((size_t)n, ATG_OK);
+    ctx->need_points = ctx->need_table = false;
     bool overflow = false;
     std::string first_err;
     for (int32_t i = 0; i < n; i++) {
         const uint32_t *cnt = reinterpret_cast<const uint32_t *>(hout + pre * (size_t)i);
         const at_detection *d = reinterpret_cast<const at_detection *>(hout + pre * (size_t)i + kCountersBytes);
         if (ctx->plans[(size_t)i].skip) continue;
-        if (cnt[kError] & 2u) {
+        if (cnt[kError] & (2u | 128u)) {
+            // out of point (2) or cluster-table (128) room: re-run with more
             overflow = true;
             ctx->fstatus[(size_t)i] = 1;
+            ctx->need_points = ctx->need_points || (cnt[kError] & 2u);
+            ctx->need_table = ctx->need_table || (cnt[kError] & 128u);
+            continue;
         }
-        if (cnt[kError] & ~2u) {
+        if (cnt[kError]) {
             char m[128];
             snprintf(m, sizeof m, "frame %d: device capacity exceeded (flags 0x%x)", i, cnt[kError]);
             if (first_err.empty()) first_err = m;
@@ -881,7 +905,7 @@ int chain_collect(atg_ctx *ctx) {
                              hipMemcpyDeviceToHost));
     }
     if (!first_err.empty()) return fail(ctx, ATG_ERR_LIMIT, first_err);
-    if (overflow) return 1;  // the caller retries with more point capacity
+    if (overflow) return 1;  // the caller retries with more room (retry_room)
     return ATG_OK;
 }
 
@@ -889,6 +913,31 @@ int run_chain(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_
     int rc = chain_enqueue(ctx, p, frames, n, on_device);
     if (rc) return rc;
     return chain_collect(ctx);
+}
+
+// After a call that ran out of room: more room for the next run (false
+// when the worst-case sizes were used already).
+bool retry_room(atg_ctx *ctx) {
+    bool more = false;
+    if (ctx->need_points && ctx->pcap_scale < 4) {
+        ctx->pcap_scale = 4;  // more boundary points than quad-image pixels: the worst case is 4
+        more = true;
+    }
+    if (ctx->need_table && !ctx->hash_full) {
+        ctx->hash_full = true;
+        more = true;
+    }
+    return more;
+}
+
+// A call with retries until the frames fit (at most two: the worst cases).
+int run_chain_fit(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, int32_t n, int on_device) {
+    ctx->pcap_scale = 1;
+    ctx->hash_full = false;
+    int rc = run_chain(ctx, p, frames, n, on_device);
+    while (rc == 1 && retry_room(ctx)) rc = run_chain(ctx, p, frames, n, on_device);
+    if (rc == 1) rc = fail(ctx, ATG_ERR_LIMIT, "point or cluster capacity exceeded");
+    return rc;
 }
 
 // Arena bytes of a call over these frames (what chain_enqueue allocates).
@@ -907,6 +956,7 @@ size_t batch_arena_bytes(const atg_ctx *ctx, const atg_params *p, const atg_fram
         if (fp.skip) { fp.qw = fp.qh = 4; fp.W = fp.H = 8; }
         fp.pcap = fp.qw * fp.qh * ctx->pcap_scale + 64;
         fp.force = ctx->force_scratch;
+        fp.hfull = ctx->hash_full;
         Frame F{};
         total += layout(fp, nfam, !on_device, &F, nullptr);
     }
@@ -952,14 +1002,7 @@ extern "C" int atg_detect_batch(atg_ctx *ctx, const atg_params *p, const atg_fra
     if (!ctx || !p || (n > 0 && (!frames || !out)) || n < 0) return ATG_ERR_ARG;
     if (n == 0) return ATG_OK;
     ctx->err.clear();
-    ctx->pcap_scale = 1;
-    int rc = run_chain(ctx, p, frames, n, on_device);
-    if (rc == 1) {
-        // more boundary points than quad-image pixels: the worst case is 4
-        ctx->pcap_scale = 4;
-        rc = run_chain(ctx, p, frames, n, on_device);
-        if (rc == 1) rc = fail(ctx, ATG_ERR_LIMIT, "point capacity exceeded");
-    }
+    int rc = run_chain_fit(ctx, p, frames, n, on_device);
     if (rc) return rc;
     for (int32_t i = 0; i < n; i++) {
         out[i].dets = ctx->dets[(size_t)i].data();
@@ -997,12 +1040,21 @@ extern "C" int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8
     if (rc) return rc;
     const atg_frame f{pixels, width, height, stride};
     hipStream_t st = ctx->stream;
-    const int64_t saved_scale = ctx->pcap_scale;
-    for (int64_t scale = 1; scale <= 4; scale *= 4) {
-        ctx->pcap_scale = scale;
+    // the context's sizing, restored on every way out
+    struct Restore {
+        atg_ctx *c;
+        int64_t scale;
+        bool full;
+        ~Restore() {
+            c->pcap_scale = scale;
+            c->hash_full = full;
+        }
+    } restore{ctx, ctx->pcap_scale, ctx->hash_full};
+    ctx->pcap_scale = 1;
+    ctx->hash_full = false;
+    for (int attempt = 0; attempt < 3; attempt++) {
         FramePlan fp;
         rc = plan_frame(ctx, p, nfam, f, on_device, fp);
-        ctx->pcap_scale = saved_scale;
         if (rc) return rc;
         if (fp.skip) return ATG_OK;  // a quad image under 4x4: nothing
         if (kNaArena + fp.size > ATG_NOALLOC_SCRATCH_BYTES) {
@@ -1037,7 +1089,13 @@ extern "C" int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8
         HIPCHK(hipMemcpyAsync(stage + kNaResult, base + kNaResult, kResultBlock, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         const uint32_t *cnt = reinterpret_cast<const uint32_t *>(stage + kNaResult);
-        if (cnt[kError] & 2u) continue;  // more points than pixels: again with the worst-case capacity
+        if (cnt[kError] & (2u | 128u)) {
+            // out of point or cluster-table room: again with the worst case
+            ctx->need_points = (cnt[kError] & 2u) != 0;
+            ctx->need_table = (cnt[kError] & 128u) != 0;
+            if (retry_room(ctx)) continue;
+            break;
+        }
         if (cnt[kError]) {
             char m[96];
             snprintf(m, sizeof m, "device capacity exceeded (flags 0x%x)", cnt[kError]);
@@ -1052,7 +1110,7 @@ extern "C" int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8
                              hipMemcpyDeviceToHost));
         return ATG_OK;
     }
-    return fail(ctx, ATG_ERR_LIMIT, "point capacity exceeded");
+    return fail(ctx, ATG_ERR_LIMIT, "point or cluster capacity exceeded");
 }
 
 extern "C" int atg_device_alloc(atg_ctx *ctx, size_t bytes, void **out) {
@@ -1132,15 +1190,10 @@ extern "C" int atg_autotune(atg_ctx *ctx, const atg_params *p, const atg_frame *
     std::vector<int64_t> want_q;
     auto measure = [&](const Tune &t, double *ms) -> int {
         ctx->tune = t;
-        ctx->pcap_scale = 1;
         std::vector<double> v;
         for (int rep = 0; rep < 4; rep++) {
-            int rc = run_chain(ctx, p, frames, n, on_device);
-            if (rc == 1) {
-                ctx->pcap_scale = 4;
-                rc = run_chain(ctx, p, frames, n, on_device);
-            }
-            if (rc) return rc < 0 ? rc : ATG_ERR_LIMIT;
+            int rc = run_chain_fit(ctx, p, frames, n, on_device);
+            if (rc) return rc;
             if (rep > 0) v.push_back(ctx->chain_ms);
         }
         std::sort(v.begin(), v.end());

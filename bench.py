@@ -68,10 +68,10 @@ MARK_KERNELS = {
     "uf_front.dense": ["k_front_dense"], "uf_replay.rows": ["k_replay<true>", "k_replay<false>"],
     "uf_replay.wide": ["k_replay_wide<true>", "k_replay_wide<false>", "k_replay_hbm", "k_work_offsets"],
     "uf_stitch.jump": ["k_root_jump"], "uf_stitch.lost": ["k_lost"], "resolve": ["k_resolve"],
-    "labels.roots": ["k_root_count", "k_root_dense"], "labels.rows": ["k_labels"],
+    "labels.rows": ["k_labels"],
     "points.count": ["k_point_rows", "k_point_scan"], "points.write": ["k_point_rows"],
-    "cluster_sort": ["k_radix_hist", "k_radix_offsets", "k_radix_scatter"],
-    "clusters.gather": ["k_cluster_count"], "clusters.write": ["k_cluster_write"],
+    "clusters.rank": ["k_cluster_rank"],
+    "cluster_sort": ["k_rank_words", "k_radix_hist", "k_radix_offsets", "k_radix_scatter"],
 }
 
 
@@ -81,7 +81,7 @@ def stage_of(mark):
         return "a"
     if g.startswith("uf_") or g == "resolve":
         return "b"
-    if g in ("labels", "points", "cluster_sort", "clusters"):
+    if g in ("labels", "points", "clusters", "cluster_sort"):
         return "c"
     if g in ("fit_prep", "angle_sort", "moments", "corners"):
         return "d"
@@ -112,12 +112,9 @@ def frame_counts(ctx, i, W, H):
     st = ctx.stages(i)
     qw, qh = st["qw"], st["qh"]
     R = c[0]
-    # the cluster sort: 11-bit radix passes over the key's bits; a frame
-    # whose key and point index fit one 64-bit word (pcap = qw qh + 64)
-    # sorts that word alone
-    packed = c[11] + (qw * qh + 64).bit_length() <= 64
+    # the cluster sort: 11-bit radix passes over the cluster ranks' bits
     return dict(W=W, H=H, qw=qw, qh=qh, R=R, N=R + qh, E=c[6], P=c[1], C=c[2], K=c[7], Q=c[3],
-                S=(c[11] + 10) // 11, SB=24 if packed else 32)
+                S=(c[11] + 10) // 11)
 
 
 def survey_bytes(n, decimated=True):
@@ -152,10 +149,10 @@ def traffic_model(n):
         "uf_replay": 4 * R + 4 * E + 4 * R + 4 * N + 12 * N,
         "uf_stitch": lg * 8 * N + 16 * N,
         "resolve": 16 * N,
-        "labels": 16 * N + px + 8 * px,
-        "points": (px + 4 * px) + (px + 8 * px) + 16 * P,
-        "cluster_sort": n["S"] * n["SB"] * P,
-        "clusters": 24 * P,
+        "labels": 4 * N + px + 4 * px,
+        "points": 2 * (px + 4 * px) + 12 * P,
+        "clusters": 28 * n["C"],
+        "cluster_sort": 16 * P + n["S"] * 16 * P + (n["S"] - 1) * 8 * P + 16 * P,
         "fit_prep": 8 * P + 20 * P + 12 * K,
         "angle_sort": 16 * K,
         "moments": 72 * K,
