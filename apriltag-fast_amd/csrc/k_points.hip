@@ -6,17 +6,19 @@
 //                  cluster table), pass 1 writes each point at its
 //                  emission index with its cluster's table slot: each run
 //                  of points of one cluster (id = max_root << 32 |
-//                  min_root, fast/clusters.c:229) finds or inserts the id
-//                  in the table and adds its length to the cluster's count.
+//                  min_root, fast/clusters.c:229) finds or inserts the id.
 //   k_point_scan   the row offsets of the points.
 //   k_cluster_rank one block per frame: the distinct clusters sorted by
 //                  (hash bucket, id), the order of the reference's cluster
-//                  table (fast/clusters.c:338-356), giving each its rank,
-//                  its id and its first point (a scan of the counts).
+//                  table (fast/clusters.c:338-356), giving each its rank
+//                  and its id.
 //   k_rank_words   each point's word: its cluster's rank | its index above
-//                  the rank's bits.
+//                  the rank's bits (frames of 2048 clusters or more also
+//                  count each cluster's points here, k_cluster_starts scans
+//                  the counts into the clusters' first points).
 //   radix sort     stable LSD sort of the words, kRadixBits a pass (one
-//                  pass up to 2048 clusters): stability keeps each
+//                  pass below 2048 clusters, whose digit offsets are then
+//                  the clusters' first points): stability keeps each
 //                  cluster's points in emission order; the last pass
 //                  writes each point itself in place.
 #include <hip/hip_runtime.h>
@@ -54,6 +56,9 @@ __device__ __forceinline__ bool sort_packed(const Frame &F) {
 }
 
 constexpr uint64_t kSlotEmpty = ~0ull;  // (no cluster id: its two roots differ)
+// Clusters below which the cluster sort's one pass gives their first points
+// (digit d = rank d, and the end at digit ncl); from it on they are counted.
+constexpr uint32_t kOnePass = 1u << kRadixBits;
 
 // The slot of cluster `id` in the frame's table, inserted if new (its slot
 // then listed in clist, counted in counters[kNClusters]); linear probing.
@@ -125,8 +130,7 @@ __device__ __forceinline__ PixCol col_of_lane63(const PixCol &c) {
 // pass 1 stages each step's points (at most 4 a pixel) with their cluster
 // ids in LDS and writes them out contiguously with their clusters' slots:
 // the first point of each run of one cluster (in emission order, across
-// steps) looks the cluster up, and each run within a 64-point chunk adds
-// its length to the cluster's count.
+// steps) looks the cluster up.
 __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pass) {
     __shared__ uint64_t st_pt[4][256], st_key[4][256];
     const Frame F = frames[blockIdx.y];
@@ -136,10 +140,7 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
     const int QW = F.qw;
     if (!pass) {
         const uint32_t nrows = (uint32_t)(F.qh - 2), slots = F.hmask + 1;
-        for (uint32_t i = (uint32_t)(y - 1) * 64u + (uint32_t)lane; i < slots; i += nrows * 64u) {
-            F.hkey[i] = kSlotEmpty;
-            F.hcnt[i] = 0u;
-        }
+        for (uint32_t i = (uint32_t)(y - 1) * 64u + (uint32_t)lane; i < slots; i += nrows * 64u) F.hkey[i] = kSlotEmpty;
     }
     const uint8_t *a = F.t + (int64_t)y * QW, *b = a + QW;
     const uint32_t *la = F.plab + (int64_t)y * QW, *lb = la + QW;
@@ -202,15 +203,6 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
                 const int src = upto ? 63 - __clzll((long long)upto) : 0;
                 const uint32_t from = (uint32_t)__shfl((int)slot, src, 64);
                 slot = upto ? from : prev_slot;
-                // each run of equal ids within the chunk adds its length
-                const uint32_t sl = wave_from_left(slot, 0xffffffffu);
-                const bool piece = valid && (lane == 0 || sl != slot);
-                const uint64_t pm = __ballot(piece) | ~__ballot(valid);
-                if (piece) {
-                    const uint64_t after = pm & ~(lt | (1ull << lane));
-                    const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
-                    atomicAdd(F.hcnt + slot, end - (uint32_t)lane);
-                }
                 const int64_t at = (int64_t)at0 + q;
                 if (valid && at < F.pcap) {  // (beyond pcap: flagged by k_point_scan)
                     F.pts[at] = st_pt[wv][q];
@@ -353,7 +345,7 @@ __device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t *tota
 // t + kOffThreads, ...: a digit's total, its base among the digits (a
 // block scan per group), then its blocks' positions.
 constexpr int kOffThreads = 1024;
-__global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *frames, int shift, int dyn) {
+__global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *frames, int shift, int dyn, int starts) {
     constexpr int kPer = kRadixBins / kOffThreads;
     static_assert(kPer * kOffThreads == kRadixBins, "whole digits a thread");
     const Frame F = frames[blockIdx.y];
@@ -372,6 +364,9 @@ __global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *fram
         uint32_t all;
         uint32_t run = carry + block_excl_scan_t<kOffThreads>(tot, &all);
         carry += all;
+        // a one-pass cluster sort: digit d is cluster rank d, its start
+        // the cluster's first point
+        if (starts && F.counters[kNClusters] < kOnePass && d <= (int)F.counters[kNClusters]) F.cstart[d] = run;
 #pragma unroll
         for (int b = 0; b < kSortBlocks; b++) {
             F.rhist[b * kRadixBins + d] = run;
@@ -478,16 +473,15 @@ __device__ __forceinline__ bool cluster_before(uint64_t a, uint64_t b, uint64_t 
 }
 
 constexpr int kRankThreads = 1024;
-constexpr uint32_t kRankLds = 8192;  // clusters sorted in LDS at most (else in the frame's csort)
+constexpr uint32_t kRankLds = 16384;  // clusters sorted in LDS at most (else in the frame's csort)
+
 
 // One block per frame: the clusters of the table in the reference's order
 // (bitonic network; keys distinct), then each cluster's rank (a binary
-// search in the sorted ids), its id and count at its rank, the counts
-// scanned into the clusters' first points, and the rank bits as the sort
-// key's.
+// search in the sorted ids) and its id at its rank; the rank bits are the
+// sort key's. (LDS: kRankLds ids, dynamic.)
 __global__ __launch_bounds__(kRankThreads) void k_cluster_rank(const Frame *frames) {
-    __shared__ uint64_t lk[kRankLds];
-    __shared__ uint32_t wsum[kRankThreads / 64];
+    extern __shared__ uint64_t lk[];
     const Frame F = frames[blockIdx.y];
     const uint32_t t = threadIdx.x;
     if (F.counters[kError] & (2u | 128u)) {
@@ -529,15 +523,22 @@ __global__ __launch_bounds__(kRankThreads) void k_cluster_rank(const Frame *fram
         }
         F.hrank[slot] = lo;
         F.cid[lo] = id;
-        F.cstart[lo] = F.hcnt[slot];
     }
-    __syncthreads();
-    // counts -> first points (exclusive scan in rank order)
-    const uint32_t lane = t & 63, w = t >> 6;
+    if (t == 0) F.counters[kSortBits] = ncl <= 1 ? 1u : 32u - (uint32_t)__clz(ncl - 1);
+}
+
+// Frames of kOnePass clusters or more: the counts (k_rank_words) scanned
+// into the clusters' first points, in rank order (one block a frame).
+__global__ __launch_bounds__(kRankThreads) void k_cluster_starts(const Frame *frames) {
+    __shared__ uint32_t wsum[kRankThreads / 64];
+    const Frame F = frames[blockIdx.y];
+    const uint32_t ncl = F.counters[kNClusters];
+    if (ncl < kOnePass || (F.counters[kError] & (2u | 128u))) return;
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     uint32_t carry = 0;
     for (uint32_t c0 = 0; c0 < ncl; c0 += kRankThreads) {
         const uint32_t i = c0 + t;
-        const uint32_t v = i < ncl ? F.cstart[i] : 0u;
+        const uint32_t v = i < ncl ? F.hcnt[i] : 0u;  // counts by rank
         const uint32_t inc = wave_incl_scan_u(v);
         if (lane == 63) wsum[w] = inc;
         __syncthreads();
@@ -550,19 +551,47 @@ __global__ __launch_bounds__(kRankThreads) void k_cluster_rank(const Frame *fram
         carry += tot;
         __syncthreads();
     }
-    if (t == 0) {
-        F.cstart[ncl] = carry;
-        F.counters[kSortBits] = ncl <= 1 ? 1u : 32u - (uint32_t)__clz(ncl - 1);
-    }
+    if (t == 0) F.cstart[ncl] = carry;
 }
 
-// Each point's sort word: its cluster's rank | its index above the rank bits.
+// Each point's sort word: its cluster's rank | its index above the rank
+// bits. Frames of kOnePass clusters or more count each cluster's points
+// (by rank, in hcnt, free since the ranks were read: cleared by the first
+// block, each run of one rank on adjacent lanes adding its length).
 __global__ __launch_bounds__(256) void k_rank_words(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     if (F.counters[kError] & (2u | 128u)) return;
-    const uint32_t n = F.counters[kNPts], sb = F.counters[kSortBits];
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-        F.skey[i] = (uint64_t)F.hrank[F.sval[i]] | ((uint64_t)i << sb);
+    const uint32_t n = F.counters[kNPts], sb = F.counters[kSortBits], ncl = F.counters[kNClusters];
+    const bool counts = ncl >= kOnePass;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t s0, s1;
+    seg_of(n, blockIdx.x, &s0, &s1);
+    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
+        const uint32_t i = c0 + threadIdx.x;
+        const bool valid = i < s1;
+        const uint32_t r = valid ? F.hrank[F.sval[i]] : 0xffffffffu;
+        if (valid) F.skey[i] = (uint64_t)r | ((uint64_t)i << sb);
+        if (counts) {
+            const uint32_t rl = wave_from_left(r, 0xfffffffeu);
+            const bool head = valid && r != rl;
+            const uint64_t hm = __ballot(head) | ~__ballot(valid);
+            if (head) {
+                const uint64_t after = hm & ~(lt | (1ull << lane));
+                const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
+                atomicAdd(F.hcnt + r, end - (uint32_t)lane);
+            }
+        }
+    }
+}
+
+// Frames of kOnePass clusters or more: the counts by rank cleared before
+// k_rank_words adds to them (hcnt holds the slots' nothing by then).
+__global__ __launch_bounds__(256) void k_rank_clear(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    const uint32_t ncl = F.counters[kNClusters];
+    if (ncl < kOnePass) return;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < ncl; i += gridDim.x * 256) F.hcnt[i] = 0u;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -574,11 +603,22 @@ void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int wh
     for (int shift = 0; shift < nbits; shift += kRadixBits) {
         hipLaunchKernelGGL(k_radix_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
                            *parity, dyn, pk);
-        hipLaunchKernelGGL(k_radix_offsets, dim3(1, nframes), dim3(kOffThreads), 0, st, dframes, shift, dyn);
+        hipLaunchKernelGGL(k_radix_offsets, dim3(1, nframes), dim3(kOffThreads), 0, st, dframes, shift, dyn,
+                           gather && shift == 0 ? 1 : 0);
         hipLaunchKernelGGL(k_radix_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
                            *parity, dyn, shift == 0 ? iota : 0, pk, gather);
         *parity ^= 1;
     }
+}
+
+// Host set-up of the stage's kernels (before any stream capture): the rank
+// kernel's dynamic LDS.
+void prepare_points_stage() {
+    static bool done = false;
+    if (done) return;
+    done = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cluster_rank),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kRankLds * 8)) == hipSuccess;
+    (void)hipGetLastError();
 }
 
 void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
@@ -590,9 +630,11 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     mk->mark(st, "points.count");
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);
     mk->mark(st, "points.write");
-    hipLaunchKernelGGL(k_cluster_rank, dim3(1, nframes), dim3(kRankThreads), 0, st, dframes);
+    hipLaunchKernelGGL(k_cluster_rank, dim3(1, nframes), dim3(kRankThreads), kRankLds * 8, st, dframes);
     mk->mark(st, "clusters.rank");
+    hipLaunchKernelGGL(k_rank_clear, dim3(8, nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_rank_words, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_cluster_starts, dim3(1, nframes), dim3(kRankThreads), 0, st, dframes);
     // key_bits: the most rank bits the call's frames can have; the passes
     // past a frame's own return at once, the last one it runs writes the
     // points in place

@@ -35,6 +35,7 @@ int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          bool refine, WorkQueue *Q, Marks *mk);
 void launch_debug_finish(hipStream_t st, uint8_t *buf, uint32_t n, int64_t *nout);
+void prepare_points_stage();
 }  // namespace atg
 
 using namespace atg;
@@ -492,6 +493,7 @@ extern "C" int atg_ctx_create(int device, atg_ctx **out) {
         if (rc) return rc;
         HIPCHK(hipHostMalloc(&ctx->na_stage, kNoAllocStage));
         warm_persistent_grids(ctx->tune.fit_blocks);
+        prepare_points_stage();
         return ATG_OK;
     }();
     if (!ctx->err.empty()) {
@@ -773,6 +775,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     const bool wave_corners = ctx->hp.max_nmaxima <= 16;
     const bool blur = ctx->hp.blur_ksz > 1, deglitch = p->deglitch;
     warm_persistent_grids(tu.fit_blocks);  // device queries stay out of stream capture
+    prepare_points_stage();
     auto issue = [&]() -> hipError_t {
         mk->n = 0;
         for (const Copy &c : copies) {

@@ -75,6 +75,34 @@ def _stripes(w, h, seed):
     return np.ascontiguousarray(img ^ np.kron(n, np.ones((4, 4), np.uint8)))
 
 
+def test_noise_frame_retries_match_oracle(atg, ctx, oracle):
+    """Random binary noise at decimate 1 with min_cluster_pixels 0: more
+    points than pixels and more clusters than the first cluster table
+    holds; the call re-runs with the worst-case sizes and equals the
+    oracle stage by stage."""
+    r = np.random.default_rng(5)
+    img = (r.random((512, 512)) > 0.5).astype(np.uint8) * 255
+    p, fams = default_params(quad_decimate=1.0, min_cluster_pixels=0), [("tag36h11", 2)]
+    print("case noise 512x512", flush=True)
+    o = oracle.detect(img, p, fams, stages=True)
+    assert len(o["stages"]["pts"]) > 512 * 512 and len(o["stages"]["cluster_ids"]) > 16384
+    g = gpu_detect(atg, ctx, img, p, fams)
+    st = ctx.stages(0)
+    for k in STAGE_KEYS:
+        assert np.array_equal(o["stages"][k], st[k]), k
+    assert same_dets(o["dets"], g["dets"]) and o["nquads"] == g["nquads"]
+
+
+def test_4096_square_decimate_1_matches_oracle(atg, ctx, oracle):
+    """A 4096x4096 frame at decimate 1 (over 2^24 union-find nodes)."""
+    img, _ = atg.synth_frame(2, 3, width=4096, height=4096)
+    p, fams = default_params(quad_decimate=1.0), [("tag36h11", 2)]
+    print("case 4096x4096 decimate 1", flush=True)
+    o = oracle.detect(img, p, fams)
+    g = gpu_detect(atg, ctx, img, p, fams)
+    assert o["nquads"] == g["nquads"] and same_dets(o["dets"], g["dets"]) and len(g["dets"]) > 0
+
+
 @pytest.mark.parametrize("name", ["wide", "tall", "front_scratch"])
 def test_extreme_shapes_match_oracle(atg, ctx, oracle, name):
     """Shapes at the union-find's limits: rows of over 1024 runs (past the
