@@ -431,6 +431,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (config 4) measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the 640x480 and 3840x2160 comparisons")
+    ap.add_argument("--no-host-input", action="store_true", help="skip the batches from pinned host memory")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-loads", default="500,1000,2000,4000,8000,20000")
     args = ap.parse_args()
@@ -546,8 +547,10 @@ def main():
                 [("tag36h11", 2), ("tagStandard52h13", 1)], 20, (16, 30), cores),
         }
         ctx.set_families(fams)
-    log("batches from pinned host memory")
-    host_in = host_input_batches(atg, frames, fams, args.batch, max(2, args.steps // 4))
+    host_in = None
+    if not args.no_host_input:
+        log("batches from pinned host memory")
+        host_in = host_input_batches(atg, frames, fams, args.batch, max(2, args.steps // 4))
     batching = None
     if not args.no_batching:
         log("config 4: uneven batch")
