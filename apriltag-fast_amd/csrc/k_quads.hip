@@ -438,33 +438,64 @@ __device__ void fix_runs_block(uint64_t *keys, uint32_t n, const uint64_t *orig,
         const uint32_t g1 = min(n, g0 + kSeg);
         if (t == 0) *lcnt = 0;
         __syncthreads();
-        for (uint32_t k = g0 + t; k < g1; k += NT) {
-            const uint64_t key = keys[k];
-            const uint64_t hk = key >> 32;
-            if (k > 0 && (keys[k - 1] >> 32) == hk) continue;  // not a run start
-            uint32_t e = k + 1;
-            uint64_t prev = key;
-            bool need = false, more = true;
-            while (more) {
-                uint64_t v[4];
+        // thread t lists the runs starting at positions [p0, p0 + 8) of
+        // the segment, p0 = g0 + 8 t: from its keys p0 - 1 .. p0 + 15 in
+        // registers, bit r of `same` says position p0 + r continues the run
+        // of p0 + r - 1 and bit r of `near` that it is also less than 5 from
+        // it (fast/quads.c:313-318); a run is followed through the registers
+        // and, past them (runs of more than 8 keys), through `keys`
+        const uint32_t p0 = g0 + 8u * (uint32_t)t;
+        uint64_t kr[17];
 #pragma unroll
-                for (int q = 0; q < 4; q++) v[q] = e + q < n ? keys[e + q] : 0ull;
-                const uint32_t e0 = e;
+        for (int r = 0; r < 17; r++) {
+            const uint32_t k = p0 + (uint32_t)r - 1u;
+            kr[r] = (r > 0 || p0 > 0) && k < n ? keys[k] : 0ull;
+        }
+        uint32_t same = 0, near = 0;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    if (more && e0 + q < n && (v[q] >> 32) == hk) {
-                        need = need || v[q] - prev < 5;
-                        prev = v[q];
-                        e++;
-                    } else {
-                        more = false;
+        for (int r = 0; r < 16; r++) {
+            const uint32_t k = p0 + (uint32_t)r;
+            const bool sm = k > 0 && k < n && (kr[r + 1] >> 32) == (kr[r] >> 32);
+            same |= (sm ? 1u : 0u) << r;
+            near |= (sm && kr[r + 1] - kr[r] < 5 ? 1u : 0u) << r;
+        }
+        // starts among the first 8 positions (inside the segment)
+        const uint32_t lim = g1 > p0 ? min(8u, g1 - p0) : 0u;
+        uint32_t starts = ~same & ((1u << lim) - 1u);
+        while (starts) {
+            const int r0 = __ffs((int)starts) - 1;
+            starts &= starts - 1;
+            // the run's positions r0 + 1 .. e - 1 continue it
+            const uint32_t rest = ~(same >> (r0 + 1)) & 0xffffu;
+            const int len_in = __ffs((int)rest) - 1;  // keys after r0 in the registers, at most 15 - r0
+            uint32_t e = p0 + (uint32_t)r0 + 1u + (uint32_t)len_in;
+            bool need = ((near >> (r0 + 1)) & ((1u << len_in) - 1u)) != 0;
+            if (r0 + 1 + len_in == 16 && e < n) {
+                // the run goes on past the registers
+                const uint64_t hk = kr[16] >> 32;
+                uint64_t prev = kr[16];
+                bool more = true;
+                while (more) {
+                    uint64_t v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = e + q < n ? keys[e + q] : 0ull;
+                    const uint32_t e0 = e;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (more && e0 + q < n && (v[q] >> 32) == hk) {
+                            need = need || v[q] - prev < 5;
+                            prev = v[q];
+                            e++;
+                        } else {
+                            more = false;
+                        }
                     }
                 }
             }
             if (need) {
                 const uint32_t slot = atomicAdd(lcnt, 1u);
-                ls[slot] = k;
-                lm[slot] = e - k;
+                ls[slot] = p0 + (uint32_t)r0;
+                lm[slot] = e - (p0 + (uint32_t)r0);
             }
         }
         __syncthreads();
@@ -776,15 +807,99 @@ __global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, con
 
 // ------------------------------------------------------------ moments
 
+// Lane q < 6 of a wave: running sum q continued from s over the terms
+// a[0, len) (LDS), each partial sum stored to out[0, len) (the cluster's
+// moment array q in HBM, straight from the chain: no LDS write-back on the
+// dependent path). Each of the kRing registers takes the term kRing ahead
+// as soon as its add has used it. Returns the sum.
+__device__ __forceinline__ double scan_chunk(const double *a, uint32_t len, double s, double *out) {
+    constexpr uint32_t kRing = ATG_SCAN_RING;
+    static_assert(kRing % 2 == 0, "pairs of sums a store");
+    uint32_t j = 0;
+    if (len >= kRing) {
+        double v[kRing];
+#pragma unroll
+        for (uint32_t q = 0; q < kRing; q++) v[q] = a[q];
+        for (; j + 2 * kRing <= len; j += kRing) {
+#pragma unroll
+            for (uint32_t q = 0; q < kRing; q += 2) {
+                const double s0 = __dadd_rn(s, v[q]);
+                const double s1 = __dadd_rn(s0, v[q + 1]);
+                s = s1;
+                v[q] = a[j + kRing + q];
+                v[q + 1] = a[j + kRing + q + 1];
+                out[j + q] = s0;
+                out[j + q + 1] = s1;
+            }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kRing; q++) {
+            s = __dadd_rn(s, v[q]);
+            out[j + q] = s;
+        }
+        j += kRing;
+    }
+    // the rest (fewer than kRing), its reads in flight together
+    if (j < len) {
+        double v[kRing];
+#pragma unroll
+        for (uint32_t q = 0; q < kRing; q++) v[q] = j + q < len ? a[j + q] : 0.0;
+#pragma unroll
+        for (uint32_t q = 0; q < kRing; q++) {
+            if (j + q < len) {
+                s = __dadd_rn(s, v[q]);
+                out[j + q] = s;
+            }
+        }
+    }
+    return s;
+}
+
+// The six moment terms of points [c0, c0 + len) of a cluster in key order
+// (fast/quads.c:968-1035: weight sqrt(g2) + 1, centres x / 2 + 1 / 2) into
+// b[q][j], for j = t0, t0 + stride, ... (KP of them at most: the point
+// loads are issued together).
+template <int KP, int CH>
+__device__ __forceinline__ void moment_terms(const Frame &F, uint32_t s0, const uint64_t *keys, uint32_t c0,
+                                             uint32_t len, double (*b)[CH + 2], int t0, int stride) {
+    uint64_t pwv[KP];
+#pragma unroll
+    for (int r = 0; r < KP; r++) {
+        const uint32_t j = (uint32_t)(t0 + r * stride);
+        pwv[r] = j < len ? F.pw[s0 + (uint32_t)~(uint32_t)keys[c0 + j]] : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < KP; r++) {
+        const uint32_t j = (uint32_t)(t0 + r * stride);
+        if (j >= len) break;
+        const int px = (int)(pwv[r] & 0xffffu), py = (int)((pwv[r] >> 16) & 0xffffu);
+        const uint32_t g2 = (uint32_t)(pwv[r] >> 32);
+        const double wgt = __dadd_rn(__dsqrt_rn((double)g2), 1.0);
+        const double fx = __dadd_rn(__dmul_rn((double)px, 0.5), 0.5);
+        const double fy = __dadd_rn(__dmul_rn((double)py, 0.5), 0.5);
+        const double wfx = __dmul_rn(wgt, fx), wfy = __dmul_rn(wgt, fy);
+        b[0][j] = wfx;
+        b[1][j] = wfy;
+        b[2][j] = __dmul_rn(wfx, fx);
+        b[3][j] = __dmul_rn(wfx, fy);
+        b[4][j] = __dmul_rn(wfy, fy);
+        b[5][j] = wgt;
+    }
+}
+
 // Persistent over list l of kept clusters, block b taking global indices
 // b, b + G, ... (a frame cursor): NT threads a cluster, the running sums
-// in chunks of CH points.
+// in chunks of CH points. With more than one wave the chunks are double
+// buffered: lanes 0-5 of wave 0 run the six sums over one chunk while the
+// other waves form the next chunk's terms.
 template <int NT, int CH>
 __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
                                                    int keys_ready, int l) {
-    __shared__ double sm[6][CH + 2];  // rows 2 doubles apart in the banks: the scan's lanes conflict-free
+    constexpr int kBufs = NT > 64 ? 2 : 1;
+    // rows 2 doubles apart in the banks: the scan's lanes conflict-free
+    __shared__ double sm[kBufs][6][CH + 2];
     __shared__ uint32_t fls[4 * NT + 1], flm[4 * NT + 1], fcnt;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, w = t >> 6;
     {
     TakeCursor cur;
     uint32_t fi, c;
@@ -802,11 +917,11 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
                 keys[k] = ((keys[k] & 0xffffffffull) << 32) | (uint32_t)~sval[s0 + k];
             __syncthreads();
         }
-        // fix_runs (fast/quads.c:304-327), the keys in LDS (the moment
-        // buffer, free until then) when they fit
+        // fix_runs (fast/quads.c:304-327), the keys in LDS (the term
+        // buffers, free until then) when they fit
         {
-            constexpr uint32_t kFixLds = 6 * CH;
-            uint64_t *kl = reinterpret_cast<uint64_t *>(&sm[0][0]);
+            constexpr uint32_t kFixLds = kBufs * 6 * CH;
+            uint64_t *kl = reinterpret_cast<uint64_t *>(&sm[0][0][0]);
             const bool in_lds = n <= kFixLds;
             if (in_lds) {
                 for (uint32_t i = t; i < n; i += NT) kl[i] = keys[i];
@@ -819,81 +934,33 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
                 __syncthreads();
             }
         }
-        // weights (fast/quads.c:968-1004) and the six running sums
+        // the six running sums (fast/quads.c:1006-1035), stored by the
+        // chain lanes into the moment arrays
         double run = 0.0;  // lane q < 6: running sum q
-        for (uint32_t c0 = 0; c0 < n; c0 += CH) {
-            uint32_t len = min((uint32_t)CH, n - c0);
-            constexpr int kPer = CH / NT;  // points a thread, their loads issued together
-            uint64_t pwv[kPer];
-#pragma unroll
-            for (int r = 0; r < kPer; r++) {
-                const uint32_t j = t + r * NT;
-                pwv[r] = j < len ? F.pw[s0 + (uint32_t)~(uint32_t)keys[c0 + j]] : 0ull;
-            }
-#pragma unroll
-            for (int r = 0; r < kPer; r++) {
-                const uint32_t j = t + r * NT;
-                if (j >= len) break;
-                const int px = (int)(pwv[r] & 0xffffu), py = (int)((pwv[r] >> 16) & 0xffffu);
-                const uint32_t g2 = (uint32_t)(pwv[r] >> 32);
-                double wgt = __dadd_rn(__dsqrt_rn((double)g2), 1.0);
-                double fx = __dadd_rn(__dmul_rn((double)px, 0.5), 0.5);
-                double fy = __dadd_rn(__dmul_rn((double)py, 0.5), 0.5);
-                double wfx = __dmul_rn(wgt, fx), wfy = __dmul_rn(wgt, fy);
-                sm[0][j] = wfx;
-                sm[1][j] = wfy;
-                sm[2][j] = __dmul_rn(wfx, fx);
-                sm[3][j] = __dmul_rn(wfx, fy);
-                sm[4][j] = __dmul_rn(wfy, fy);
-                sm[5][j] = wgt;
-            }
+        double *out = F.mom + (t < 6 ? t : 0) * pc + s0;
+        const uint32_t nch = (n + CH - 1) / CH;
+        if constexpr (kBufs == 2) {
+            moment_terms<(CH + NT - 1) / NT, CH>(F, s0, keys, 0, min((uint32_t)CH, n), sm[0], t, NT);
             __syncthreads();
-            if (t < 6) {
-                // one lane per sum, in order; each of the kRing registers
-                // takes the term kRing ahead as soon as its add has used it,
-                // so a read has kRing - 1 dependent adds to arrive in
-                constexpr uint32_t kRing = ATG_SCAN_RING;
-                double s = run;
-                double *a = sm[t];
-                uint32_t j = 0;
-                double v[kRing];
-                if (len >= kRing) {
-#pragma unroll
-                    for (uint32_t q = 0; q < kRing; q++) v[q] = a[q];
-                    for (; j + 2 * kRing <= len; j += kRing) {
-#pragma unroll
-                        for (uint32_t q = 0; q < kRing; q++) {
-                            s = __dadd_rn(s, v[q]);
-                            a[j + q] = s;
-                            v[q] = a[j + kRing + q];
-                        }
-                    }
-#pragma unroll
-                    for (uint32_t q = 0; q < kRing; q++) {
-                        s = __dadd_rn(s, v[q]);
-                        a[j + q] = s;
-                    }
-                    j += kRing;
+            for (uint32_t ci = 0; ci < nch; ci++) {
+                const uint32_t c0 = ci * CH, len = min((uint32_t)CH, n - c0);
+                if (w == 0) {
+                    if (t < 6) run = scan_chunk(sm[ci & 1][t], len, run, out + c0);
+                } else if (ci + 1 < nch) {
+                    const uint32_t c1 = c0 + CH;
+                    moment_terms<(CH + NT - 65) / (NT - 64), CH>(F, s0, keys, c1, min((uint32_t)CH, n - c1),
+                                                                 sm[(ci + 1) & 1], t - 64, NT - 64);
                 }
-                // the rest (fewer than kRing), 8 reads in flight at a time
-                for (; j < len; j += 8) {
-                    double w[8];
-#pragma unroll
-                    for (uint32_t q = 0; q < 8; q++) w[q] = j + q < len ? a[j + q] : 0.0;
-#pragma unroll
-                    for (uint32_t q = 0; q < 8; q++) {
-                        if (j + q < len) {
-                            s = __dadd_rn(s, w[q]);
-                            a[j + q] = s;
-                        }
-                    }
-                }
-                run = s;
+                __syncthreads();
             }
-            __syncthreads();
-            for (uint32_t j = t; j < len; j += NT)
-                for (int q = 0; q < 6; q++) F.mom[q * pc + s0 + c0 + j] = sm[q][j];
-            __syncthreads();
+        } else {
+            for (uint32_t ci = 0; ci < nch; ci++) {
+                const uint32_t c0 = ci * CH, len = min((uint32_t)CH, n - c0);
+                moment_terms<(CH + NT - 1) / NT, CH>(F, s0, keys, c0, len, sm[0], t, NT);
+                __syncthreads();
+                if (t < 6) run = scan_chunk(sm[0][t], len, run, out + c0);
+                __syncthreads();
+            }
         }
     }
     }
@@ -1036,9 +1103,10 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
     __shared__ uint32_t lidx[kMaxima];
     __shared__ uint32_t mx[kMaxNm];
     __shared__ uint32_t hist[NT == 256 ? 256 : 1];
+    constexpr uint32_t kSegPts = NT == 256 ? 16384 : (uint32_t)kLdsPts;  // maxima flags a pass
+    __shared__ uint64_t fw[kSegPts / 64];
     __shared__ uint64_t s_prefix;
     __shared__ uint32_t s_rank;
-    __shared__ double s_thr;
     __shared__ double s_wbest[kWaves];
     __shared__ uint32_t s_word[kWaves];
     __shared__ int64_t s_idx[4];
@@ -1113,23 +1181,36 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
             y[i] = acc;
         }
         __syncthreads();
-        // cyclic local maxima, in order (fast/quads.c:805-823)
+        // cyclic local maxima, in order (fast/quads.c:805-823): by segments
+        // of kSegPts points, the flags as ballot words in LDS (coalesced
+        // reads of the filtered errors), then each thread lists the maxima
+        // of a contiguous run of words after one block scan of the counts
         uint32_t *midx = inl ? lidx : F.midx + s0;
         uint32_t base = 0;
-        for (int64_t c0 = 0; c0 < sz; c0 += NT) {
-            int64_t i = c0 + t;
-            uint32_t f = 0;
-            if (i < sz) {
-                double e = y[i], l = y[i == 0 ? sz - 1 : i - 1], r = y[i == sz - 1 ? 0 : i + 1];
-                f = (e > l && e > r) ? 1u : 0u;
+        for (int64_t g0 = 0; g0 < sz; g0 += kSegPts) {
+            const uint32_t gn = (uint32_t)min((int64_t)kSegPts, sz - g0), nw = (gn + 63) / 64;
+            for (uint32_t c0 = 0; c0 < gn; c0 += NT) {
+                const int64_t i = g0 + c0 + t;
+                bool f = false;
+                if (i < sz) {
+                    const double e = y[i], yl = y[i == 0 ? sz - 1 : i - 1], yr = y[i == sz - 1 ? 0 : i + 1];
+                    f = e > yl && e > yr;
+                }
+                const uint64_t m = __ballot(f);
+                if (lane == 0) fw[(c0 >> 6) + (uint32_t)w] = m;
             }
+            __syncthreads();
+            const uint32_t per = (nw + NT - 1) / NT, w0 = min(nw, (uint32_t)t * per), w1 = min(nw, w0 + per);
+            uint32_t cnt = 0;
+            for (uint32_t k = w0; k < w1; k++) cnt += (uint32_t)__popcll(fw[k]);
             uint32_t tot;
-            uint32_t pre = bscan_t<NT>(f, &tot);
-            if (f) midx[base + pre] = (uint32_t)i;
+            uint32_t o = base + bscan_t<NT>(cnt, &tot);
+            for (uint32_t k = w0; k < w1; k++)
+                for (uint64_t m = fw[k]; m; m &= m - 1) midx[o++] = (uint32_t)(g0 + 64 * k + (uint32_t)__ffsll((long long)m) - 1);
             base += tot;
+            __syncthreads();
         }
         uint32_t nm = base;
-        __syncthreads();
         if (nm < 4) continue;
         if ((int64_t)nm > P->max_nmaxima) {
             // the (max_nmaxima+1)-th largest maximum error (fast/quads.c:
@@ -1168,21 +1249,60 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)(pos & 63));
                 thr = __longlong_as_double((long long)((k >> 63) ? (k ^ 0x8000000000000000ull) : ~k));
             } else if (NT == 256 && nm <= (uint32_t)kMaxima) {
-                // the rank of each value among the maxima (ties by position)
-                double *mv = sh + 2 * kLdsPts;
-                for (uint32_t i = t; i < nm; i += NT) mv[i] = y[midx[i]];
-                __syncthreads();
-                for (uint32_t i = t; i < nm; i += NT) {
-                    const double v = mv[i];
-                    uint32_t r = 0;
-                    for (uint32_t j = 0; j < nm; j++) {
-                        const double u = mv[j];
-                        r += (u > v || (u == v && j < i)) ? 1u : 0u;
-                    }
-                    if (r == rank) s_thr = v;
+                // rank + 1 rounds, each taking the largest remaining value
+                // (ties by position) out of the maxima held in registers,
+                // four a thread: the last one taken is the value at
+                // descending rank `rank`
+                constexpr int kPer = kMaxima / 256;
+                double v[kPer];
+                bool live[kPer];
+#pragma unroll
+                for (int r = 0; r < kPer; r++) {
+                    const uint32_t i = (uint32_t)(t + 256 * r);
+                    live[r] = i < nm;
+                    v[r] = live[r] ? y[midx[i]] : 0.0;
                 }
-                __syncthreads();
-                thr = s_thr;
+                thr = 0.0;
+                for (uint32_t it = 0; it <= rank; it++) {
+                    double bv = 0.0;
+                    uint32_t bi = 0xffffffffu;
+#pragma unroll
+                    for (int r = 0; r < kPer; r++) {
+                        const uint32_t i = (uint32_t)(t + 256 * r);
+                        if (live[r] && (bi == 0xffffffffu || v[r] > bv)) {
+                            bv = v[r];
+                            bi = i;
+                        }
+                    }
+                    for (int d = 32; d >= 1; d >>= 1) {
+                        const double v2 = __shfl_xor(bv, d, 64);
+                        const uint32_t i2 = (uint32_t)__shfl_xor((int)bi, d, 64);
+                        if (i2 != 0xffffffffu && (bi == 0xffffffffu || v2 > bv || (v2 == bv && i2 < bi))) {
+                            bv = v2;
+                            bi = i2;
+                        }
+                    }
+                    if (lane == 0) {
+                        s_wbest[w] = bv;
+                        s_word[w] = bi;
+                    }
+                    __syncthreads();
+                    bv = s_wbest[0];
+                    bi = s_word[0];
+                    for (int k = 1; k < kWaves; k++) {
+                        const double v2 = s_wbest[k];
+                        const uint32_t i2 = s_word[k];
+                        if (i2 != 0xffffffffu && (bi == 0xffffffffu || v2 > bv || (v2 == bv && i2 < bi))) {
+                            bv = v2;
+                            bi = i2;
+                        }
+                    }
+                    __syncthreads();
+                    thr = bv;
+#pragma unroll
+                    for (int r = 0; r < kPer; r++)
+                        if ((uint32_t)(t + 256 * r) == bi) live[r] = false;
+                }
             } else if constexpr (NT == 256) {
                 // radix select over the order-preserving bits
                 uint32_t rk = rank;
