@@ -35,7 +35,7 @@
 namespace atg {
 
 constexpr int kMaxFamilies = 16;
-constexpr int kSortBlocks = 32;      // radix-sort blocks per frame
+constexpr int kSortBlocks = 64;      // radix-sort blocks per frame at most
 constexpr int kRadixBits = 11;       // bits of a radix-sort digit
 constexpr int kRadixBins = 1 << kRadixBits;
 constexpr int kInlineDets = 32;      // detections of a frame returned with its counters

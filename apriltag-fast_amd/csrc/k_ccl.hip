@@ -533,17 +533,17 @@ __global__ __launch_bounds__(T) void k_front_local(const Frame *frames, int band
 // (records and weights without what lies above) joined. Elements: the
 // records above at [0, QW), the band's last-row records at QW + [0, QW),
 // the port components (by least port) at 2 QW + [0, QW).
+//
+// The bands form a chain through the row above each band only: band k's
+// final last-row records are band k + 1's records above. Element x of a
+// row belongs to thread x mod T (slot x / T of its registers) in both
+// roles, so each step hands its results to the next in registers, and the
+// step's other inputs (the ports of the row above, the band's last-row
+// records) are loaded one band ahead: a step waits on LDS only.
 template <int T>
-__global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band_rows, int max_bands) {
-    extern __shared__ __align__(16) uint8_t lds[];
-    const Frame F = frames[blockIdx.y];
-    const int nb = front_bands(F, band_rows, max_bands);
-    if (nb < 2) return;
+__device__ void front_carry_loop(const Frame &F, int nb, uint32_t *p, uint32_t *w, uint32_t *mn) {
     const uint32_t QW = (uint32_t)F.qw;
     const int tid = threadIdx.x;
-    uint32_t *p = reinterpret_cast<uint32_t *>(lds);
-    uint32_t *w = p + 3 * QW;
-    uint32_t *mn = w + 3 * QW;
     auto settle = [&](uint32_t x) {
         const uint32_t r = fr_find<false>(p, x);
         if (r != x) {
@@ -597,6 +597,157 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
             if (l == i) F.recw[be + i] = w[r];
         }
         __syncthreads();
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band_rows, int max_bands) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int kR = 4;  // elements of a row a thread at most (rows of up to kR T runs)
+    const Frame F = frames[blockIdx.y];
+    const int nb = front_bands(F, band_rows, max_bands);
+    if (nb < 2) return;
+    const uint32_t QW = (uint32_t)F.qw;
+    uint32_t *p = reinterpret_cast<uint32_t *>(lds);
+    uint32_t *w = p + 3 * QW;
+    uint32_t *mn = w + 3 * QW;
+    if (QW > (uint32_t)(kR * T)) {  // a row may hold more runs than the registers: every input from HBM
+        front_carry_loop<T>(F, nb, p, w, mn);
+        return;
+    }
+    const int tid = threadIdx.x;
+    auto settle = [&](uint32_t x) {
+        const uint32_t r = fr_find<false>(p, x);
+        if (r != x) {
+            p[x] = r;
+            atomicAdd(&w[r], w[x]);  // (roots no longer change: a non-root's weight is read once)
+        }
+        return r;
+    };
+    // each band's row above and last row: first slot and run count, all
+    // read at once
+    __shared__ uint32_t rows[kMaxFrontBands][4];
+    for (int k = tid; k < nb; k += T) {
+        const int r0 = band_start(F, k, nb) - 1, e = band_start(F, k + 1, nb) - 1;
+        if (k > 0) {
+            rows[k][0] = F.row_off[r0];
+            rows[k][1] = F.rcnt[r0];
+            rows[k][2] = F.row_off[e];
+            rows[k][3] = F.rcnt[e];
+        }
+    }
+    __syncthreads();
+    // a band's inputs: the ports of its row above (component of each port,
+    // and the component's weight or live record) and its last-row records
+    struct In {
+        uint32_t b0, n0, be, ne;
+        uint32_t g[kR], info[kR], lab[kR], wt[kR];
+    };
+    auto load = [&](int k, In &in) {
+        in.b0 = rows[k][0];
+        in.n0 = rows[k][1];
+        in.be = rows[k][2];
+        in.ne = rows[k][3];
+#pragma unroll
+        for (int j = 0; j < kR; j++) {
+            const uint32_t x = (uint32_t)(tid + j * T);
+            in.g[j] = x < in.n0 ? F.rlost[in.b0 + x] : 0u;
+            in.info[j] = x < in.n0 ? F.rroot[in.b0 + x] : 0u;
+            in.lab[j] = x < in.ne ? F.rlab[in.be + x] : 0u;
+            in.wt[j] = x < in.ne ? F.recw[in.be + x] : 0u;
+        }
+    };
+    // the final records above band 1: band 0's last row (final after step 1)
+    uint32_t alab[kR], aw[kR];
+    {
+        const uint32_t b0 = rows[1][0], n0 = rows[1][1];
+#pragma unroll
+        for (int j = 0; j < kR; j++) {
+            const uint32_t x = (uint32_t)(tid + j * T);
+            alab[j] = x < n0 ? F.rlab[b0 + x] : 0u;
+            aw[j] = x < n0 && alab[j] == x ? F.recw[b0 + x] : 0u;
+        }
+    }
+    In cur, nxt;
+    load(1, cur);
+    for (int k = 1; k < nb; k++) {
+        if (k + 1 < nb) load(k + 1, nxt);  // in flight through this band's step
+        const uint32_t n0 = cur.n0, ne = cur.ne;
+#pragma unroll
+        for (int j = 0; j < kR; j++) {
+            const uint32_t x = (uint32_t)(tid + j * T);
+            if (x < n0) {
+                p[x] = x;
+                w[x] = aw[j];
+                mn[x] = 0xffffffffu;
+                // a component's element: its weight if it ended inside the
+                // band (a live one's pixels are in its last-row record's
+                // weight); its mn slot holds its least record above
+                const bool comp = cur.g[j] == x;
+                p[2 * QW + x] = 2 * QW + x;
+                w[2 * QW + x] = comp && !(cur.info[j] & kPortLive) ? cur.info[j] : 0u;
+                mn[2 * QW + x] = 0xffffffffu;
+            }
+            if (x < ne) {
+                p[QW + x] = QW + x;
+                w[QW + x] = cur.lab[j] == x ? cur.wt[j] : 0u;
+                mn[QW + x] = 0xffffffffu;
+            }
+        }
+        bsync<false>();  // LDS only: the next band's loads stay in flight
+        // each component's least record above (the records above are roots
+        // here: singletons until the unions below)
+#pragma unroll
+        for (int j = 0; j < kR; j++) {
+            const uint32_t x = (uint32_t)(tid + j * T);
+            if (x < n0) atomicMin(&mn[2 * QW + cur.g[j]], alab[j]);
+        }
+        bsync<false>();
+        // the component under its least record, every port's record united
+        // with it, and a live component with its last-row record: the sets
+        // of the port-by-port unions, with fewer hooks racing on one root
+        // (a component's node is never a root after this, so its mn slot is
+        // free; roots keep their mn)
+#pragma unroll
+        for (int j = 0; j < kR; j++) {
+            const uint32_t x = (uint32_t)(tid + j * T);
+            if (x < n0) {
+                const uint32_t g = cur.g[j], m = mn[2 * QW + g];
+                if (g == x) p[2 * QW + x] = m;
+                if (alab[j] != m) fr_unite<false>(p, alab[j], m);
+                if (g == x && (cur.info[j] & kPortLive)) fr_unite<false>(p, 2 * QW + x, QW + (cur.info[j] & ~kPortLive));
+            }
+        }
+        bsync<false>();
+#pragma unroll
+        for (int j = 0; j < kR; j++) {
+            const uint32_t x = (uint32_t)(tid + j * T);
+            if (x < n0) {
+                settle(x);
+                settle(2 * QW + x);
+            }
+            if (x < ne) {
+                const uint32_t r = settle(QW + x);
+                if (cur.lab[j] == x) atomicMin(&mn[r], x);  // a record of the band: its first run
+            }
+        }
+        bsync<false>();
+        // the band's final last-row records: to HBM, and (the next band's
+        // records above) to the registers of the same elements
+#pragma unroll
+        for (int j = 0; j < kR; j++) {
+            const uint32_t i = (uint32_t)(tid + j * T);
+            if (i < ne) {
+                const uint32_t r = p[QW + cur.lab[j]], l = mn[r];
+                const uint32_t wr = w[r];
+                F.rlab[cur.be + i] = l;
+                if (l == i) F.recw[cur.be + i] = wr;
+                alab[j] = l;
+                aw[j] = l == i ? wr : 0u;
+            }
+        }
+        bsync<false>();
+        cur = nxt;
     }
 }
 

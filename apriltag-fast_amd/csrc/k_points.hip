@@ -264,8 +264,10 @@ __device__ __forceinline__ SortBufs sort_bufs(const Frame &F, int parity) {
     return {F.skey2, F.sval2, F.skey, F.sval};
 }
 
+// Block b's segment of n items, gridDim.x blocks a frame (256-item steps).
 __device__ __forceinline__ void seg_of(uint32_t n, int b, uint32_t *s0, uint32_t *s1) {
-    uint32_t seg = ((n + kSortBlocks - 1) / kSortBlocks + 255u) & ~255u;
+    const uint32_t nb = gridDim.x;
+    uint32_t seg = ((n + nb - 1) / nb + 255u) & ~255u;
     *s0 = min(n, (uint32_t)b * seg);
     *s1 = min(n, (uint32_t)(b + 1) * seg);
 }
@@ -282,6 +284,13 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
     return peers;
 }
 
+// The cluster sort's word of point i: its cluster's rank | i above the
+// rank bits (formed where the first pass reads it, from the point's table
+// slot and the slots' ranks).
+__device__ __forceinline__ uint64_t rank_word(const Frame &F, uint32_t i, uint32_t sb) {
+    return (uint64_t)F.hrank[F.sval[i]] | ((uint64_t)i << sb);
+}
+
 // dyn: the pass is skipped where the frame's key (counters[kSortBits]) has
 // no bits at shift; the skipped passes are the last ones. pk: the frame's
 // words are packed where sort_packed says so (the key is their low
@@ -290,7 +299,7 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 // equal digits on adjacent lanes adds its length once (a digit that comes
 // back later in the wave adds again: counts only, no order).
 __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int which, int shift, int parity, int dyn,
-                                                    int pk) {
+                                                    int pk, int ranks) {
     __shared__ uint32_t h[kRadixBins];
     const Frame F = frames[blockIdx.y];
     const uint32_t sb = F.counters[kSortBits];
@@ -308,7 +317,8 @@ __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int whi
     for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
         const uint32_t i = c0 + t;
         const bool valid = i < s1;
-        const uint32_t d = valid ? (uint32_t)((B.kin[i] & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
+        const uint64_t key = !valid ? 0ull : ranks ? rank_word(F, i, sb) : B.kin[i];
+        const uint32_t d = valid ? (uint32_t)((key & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
         const uint32_t dl = wave_from_left(d, 0xfffffffeu);  // lane 0: a head
         const uint64_t heads = __ballot(valid && d != dl) | ~__ballot(valid);  // invalid lanes end runs
         if (valid && d != dl) {
@@ -345,7 +355,8 @@ __device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t *tota
 // t + kOffThreads, ...: a digit's total, its base among the digits (a
 // block scan per group), then its blocks' positions.
 constexpr int kOffThreads = 1024;
-__global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *frames, int shift, int dyn, int starts) {
+__global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *frames, int shift, int dyn, int starts,
+                                                                int nb) {
     constexpr int kPer = kRadixBins / kOffThreads;
     static_assert(kPer * kOffThreads == kRadixBins, "whole digits a thread");
     const Frame F = frames[blockIdx.y];
@@ -358,7 +369,7 @@ __global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *fram
         uint32_t v[kSortBlocks], tot = 0;
 #pragma unroll
         for (int b = 0; b < kSortBlocks; b++) {
-            v[b] = F.rhist[b * kRadixBins + d];
+            v[b] = b < nb ? F.rhist[b * kRadixBins + d] : 0u;
             tot += v[b];
         }
         uint32_t all;
@@ -369,7 +380,7 @@ __global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *fram
         if (starts && F.counters[kNClusters] < kOnePass && d <= (int)F.counters[kNClusters]) F.cstart[d] = run;
 #pragma unroll
         for (int b = 0; b < kSortBlocks; b++) {
-            F.rhist[b * kRadixBins + d] = run;
+            if (b < nb) F.rhist[b * kRadixBins + d] = run;
             run += v[b];
         }
     }
@@ -383,7 +394,7 @@ __global__ __launch_bounds__(kOffThreads) void k_radix_offsets(const Frame *fram
 // first pass of an unpacked cluster sort; its points were emitted in
 // order), not read.
 __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int which, int shift, int parity,
-                                                       int dyn, int iota, int pk, int gather) {
+                                                       int dyn, int iota, int pk, int gather, int ranks) {
     __shared__ uint32_t off[kRadixBins];
     __shared__ uint8_t wcnt[4][kRadixBins];
     const Frame F = frames[blockIdx.y];
@@ -408,7 +419,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
     for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
         const uint32_t i = c0 + t;
         const bool valid = i < s1;
-        const uint64_t key = valid ? B.kin[i] : 0ull;
+        const uint64_t key = !valid ? 0ull : ranks ? rank_word(F, i, sb) : B.kin[i];
         const uint32_t val = (valid && !packed) ? (iota ? i : B.vin[i]) : 0u;
         const uint32_t d = valid ? (uint32_t)((key & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
         // the wave's runs of equal digits on adjacent lanes; where every
@@ -554,15 +565,15 @@ __global__ __launch_bounds__(kRankThreads) void k_cluster_starts(const Frame *fr
     if (t == 0) F.cstart[ncl] = carry;
 }
 
-// Each point's sort word: its cluster's rank | its index above the rank
-// bits. Frames of kOnePass clusters or more count each cluster's points
-// (by rank, in hcnt, free since the ranks were read: cleared by the first
-// block, each run of one rank on adjacent lanes adding its length).
-__global__ __launch_bounds__(256) void k_rank_words(const Frame *frames) {
+// Frames of kOnePass clusters or more: each cluster's points counted (by
+// rank, in hcnt, free since the ranks were read: cleared by k_rank_clear),
+// each run of one rank on adjacent lanes adding its length. (Smaller frames
+// take their cluster starts from the sort's one pass.)
+__global__ __launch_bounds__(256) void k_rank_counts(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     if (F.counters[kError] & (2u | 128u)) return;
-    const uint32_t n = F.counters[kNPts], sb = F.counters[kSortBits], ncl = F.counters[kNClusters];
-    const bool counts = ncl >= kOnePass;
+    const uint32_t n = F.counters[kNPts], ncl = F.counters[kNClusters];
+    if (ncl < kOnePass) return;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t s0, s1;
@@ -571,22 +582,19 @@ __global__ __launch_bounds__(256) void k_rank_words(const Frame *frames) {
         const uint32_t i = c0 + threadIdx.x;
         const bool valid = i < s1;
         const uint32_t r = valid ? F.hrank[F.sval[i]] : 0xffffffffu;
-        if (valid) F.skey[i] = (uint64_t)r | ((uint64_t)i << sb);
-        if (counts) {
-            const uint32_t rl = wave_from_left(r, 0xfffffffeu);
-            const bool head = valid && r != rl;
-            const uint64_t hm = __ballot(head) | ~__ballot(valid);
-            if (head) {
-                const uint64_t after = hm & ~(lt | (1ull << lane));
-                const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
-                atomicAdd(F.hcnt + r, end - (uint32_t)lane);
-            }
+        const uint32_t rl = wave_from_left(r, 0xfffffffeu);
+        const bool head = valid && r != rl;
+        const uint64_t hm = __ballot(head) | ~__ballot(valid);
+        if (head) {
+            const uint64_t after = hm & ~(lt | (1ull << lane));
+            const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
+            atomicAdd(F.hcnt + r, end - (uint32_t)lane);
         }
     }
 }
 
 // Frames of kOnePass clusters or more: the counts by rank cleared before
-// k_rank_words adds to them (hcnt holds the slots' nothing by then).
+// k_rank_counts adds to them (hcnt holds the slots' nothing by then).
 __global__ __launch_bounds__(256) void k_rank_clear(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
     const uint32_t ncl = F.counters[kNClusters];
@@ -598,15 +606,21 @@ __global__ __launch_bounds__(256) void k_rank_clear(const Frame *frames) {
 
 // pk: the cluster sort (packed frames move one word); else (key, value)
 // pairs throughout.
+// Radix-sort blocks a frame: all kSortBlocks for calls of a few frames
+// (each frame's sort spread over more of the device), half for batches.
+int sort_blocks(int nframes) { return nframes <= 16 ? kSortBlocks : kSortBlocks / 2; }
+
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
-                       int dyn, int iota, int pk, int gather) {
+                       int dyn, int iota, int pk, int gather, int ranks) {
+    const int nb = sort_blocks(nframes);
     for (int shift = 0; shift < nbits; shift += kRadixBits) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity, dyn, pk);
+        const int rk = ranks && shift == 0 ? 1 : 0;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb, nframes), dim3(256), 0, st, dframes, which, shift, *parity, dyn,
+                           pk, rk);
         hipLaunchKernelGGL(k_radix_offsets, dim3(1, nframes), dim3(kOffThreads), 0, st, dframes, shift, dyn,
-                           gather && shift == 0 ? 1 : 0);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes, which, shift,
-                           *parity, dyn, shift == 0 ? iota : 0, pk, gather);
+                           gather && shift == 0 ? 1 : 0, nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb, nframes), dim3(256), 0, st, dframes, which, shift, *parity,
+                           dyn, shift == 0 ? iota : 0, pk, gather, rk);
         *parity ^= 1;
     }
 }
@@ -633,13 +647,13 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     hipLaunchKernelGGL(k_cluster_rank, dim3(1, nframes), dim3(kRankThreads), kRankLds * 8, st, dframes);
     mk->mark(st, "clusters.rank");
     hipLaunchKernelGGL(k_rank_clear, dim3(8, nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_rank_words, dim3(kSortBlocks, nframes), dim3(256), 0, st, dframes);
+    hipLaunchKernelGGL(k_rank_counts, dim3(sort_blocks(nframes), nframes), dim3(256), 0, st, dframes);
     hipLaunchKernelGGL(k_cluster_starts, dim3(1, nframes), dim3(kRankThreads), 0, st, dframes);
     // key_bits: the most rank bits the call's frames can have; the passes
     // past a frame's own return at once, the last one it runs writes the
     // points in place
     int parity = 0;
-    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 0, 1, 1);
+    launch_radix_sort(st, dframes, nframes, kNPts, key_bits, &parity, 1, 0, 1, 1, 1);
     mk->mark(st, "cluster_sort");
 }
 

@@ -1555,7 +1555,7 @@ __global__ __launch_bounds__(256) void k_quad_order(const Frame *frames) {
 }
 
 void launch_radix_sort(hipStream_t st, const Frame *dframes, int nframes, int which, int nbits, int *parity,
-                       int dyn, int iota, int pk, int gather);
+                       int dyn, int iota, int pk, int gather, int ranks);
 
 void warm_decode_grid(int fit_blocks);
 
@@ -1649,7 +1649,7 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, st, dframes, Q);
         mk->mark(st, "angle_sort.small");
     } else {
-        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0, 0, 0);
+        launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0, 0, 0, 0);
         mk->mark(st, "angle_sort.all");
     }
     // large clusters: 128-thread blocks for batches (8 a CU: more running

@@ -71,7 +71,8 @@ MARK_KERNELS = {
     "labels.rows": ["k_labels"],
     "points.count": ["k_point_rows", "k_point_scan"], "points.write": ["k_point_rows"],
     "clusters.rank": ["k_cluster_rank"],
-    "cluster_sort": ["k_rank_words", "k_radix_hist", "k_radix_offsets", "k_radix_scatter"],
+    "cluster_sort": ["k_rank_clear", "k_rank_counts", "k_cluster_starts", "k_radix_hist", "k_radix_offsets",
+                     "k_radix_scatter"],
 }
 
 
