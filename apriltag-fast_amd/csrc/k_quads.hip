@@ -651,6 +651,7 @@ __device__ __forceinline__ void bitonic_pair(uint32_t q, uint32_t k, uint32_t j,
 }
 
 constexpr uint32_t kSortTile = 2048;  // keys of a cluster sorted together in LDS
+constexpr uint32_t kBigSort = 8192;   // keys of a cluster sorted by one 1024-thread block
 
 
 template <int R>
@@ -750,7 +751,7 @@ __device__ __forceinline__ void sort_tiles(const uint64_t *src, uint64_t *dst, u
 // wider steps over the cluster's keys in HBM. The keys are distinct, so
 // any sort gives the full-key order the reference's sort starts from
 // (fast/quads.c:379-427); its tie fix follows in k_fit_moments.
-__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q) {
+__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q, int big) {
     static_assert(kSortTile == 2048, "4 waves x 512 keys");
     __shared__ uint64_t sk[kSortTile];
     const uint32_t t = threadIdx.x;
@@ -759,6 +760,7 @@ __global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const W
     for (uint32_t g = blockIdx.x; cluster_at(frames, Q, 0, g, cur, &fi, &c); g += gridDim.x) {
         const Frame &F = frames[fi];
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
+        if (big && n > kHugeCluster && n <= kBigSort) continue;  // k_angle_sort_big's
         uint64_t *keys = F.skey + s0;
         uint32_t m = kSortTile;
         while (m < n) m <<= 1;
@@ -778,6 +780,45 @@ __global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const W
                 __syncthreads();
             }
             sort_tiles(keys, keys, n, sk, k);
+        }
+    }
+}
+
+// The angle sort of the clusters of kHugeCluster + 1 to kBigSort points
+// (of the huge list), for calls of a few frames, a cluster per block of
+// 1024 threads: the keys in the
+// registers of 16 waves (512 a wave), each wave's sorted in registers, the
+// merges across waves through LDS (skb, dynamic), the steps inside a wave
+// in registers; padding keys (all ones) sort last.
+__global__ __launch_bounds__(1024) void k_angle_sort_big(const Frame *frames, const WorkQueue *Q) {
+    static_assert(kBigSort == 16 * 512, "16 waves x 512 keys");
+    extern __shared__ uint64_t skb[];
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const uint32_t total = Q->total[4];
+    TakeCursor cur;
+    uint32_t fi, c;
+    for (uint32_t g = blockIdx.x; g < total && cluster_at(frames, Q, 0, g, cur, &fi, &c); g += gridDim.x) {
+        const Frame &F = frames[fi];
+        const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
+        if (n <= kHugeCluster || n > kBigSort) continue;
+        uint32_t m = 1024;
+        while (m < n) m <<= 1;
+        uint64_t v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
+            v[r] = i < n ? F.okey[s0 + i] : ~0ull;
+        }
+        wave_bitonic<8>(v);
+        for (uint32_t k = 1024; k <= m; k <<= 1) {
+            lds_xor_step(v, skb, k - 1);  // the merge's mirror step
+            for (uint32_t j = k >> 2; j >= 512; j >>= 1) lds_xor_step(v, skb, j);
+            reg_half_clean(v);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
+            if (i < n) F.skey[s0 + i] = v[r];
         }
     }
 }
@@ -1605,6 +1646,14 @@ static unsigned pgrid(K *fn, int threads, int want) {
 
 void warm_persistent_grids(int fit_blocks) {
     pgrid(k_angle_sort, 256, fit_blocks);
+    {
+        static bool big_lds = false;  // the big sort's LDS: its keys across waves
+        if (!big_lds)
+            big_lds = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_angle_sort_big),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kBigSort * 8)) == hipSuccess;
+        (void)hipGetLastError();
+        persistent_grid(reinterpret_cast<const void *>(&k_angle_sort_big), 1024, kBigSort * 8, fit_blocks);
+    }
     pgrid(k_angle_sort_wave, 64, fit_blocks);
     pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks);
     pgrid(k_fit_moments<256, 512>, 256, fit_blocks);
@@ -1644,7 +1693,15 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     mk->mark(st, "fit_prep.lists");
     int parity = 0;
     if (sort_cap) {
-        hipLaunchKernelGGL(k_angle_sort, dim3(pgrid(k_angle_sort, 256, fit_blocks)), dim3(256), 0, st, dframes, Q);
+        // calls of a few frames: the clusters above kHugeCluster points
+        // (a few dozen a frame) each by a 1024-thread block
+        const int big = nframes <= 16 ? 1 : 0;
+        hipLaunchKernelGGL(k_angle_sort, dim3(pgrid(k_angle_sort, 256, fit_blocks)), dim3(256), 0, st, dframes, Q, big);
+        if (big)
+            hipLaunchKernelGGL(k_angle_sort_big,
+                               dim3(persistent_grid(reinterpret_cast<const void *>(&k_angle_sort_big), 1024,
+                                                    kBigSort * 8, std::min(fit_blocks, 64 * nframes))),
+                               dim3(1024), kBigSort * 8, st, dframes, Q);
         mk->mark(st, "angle_sort.large");
         hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, st, dframes, Q);
         mk->mark(st, "angle_sort.small");
