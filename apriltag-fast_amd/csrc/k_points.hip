@@ -506,10 +506,54 @@ __global__ __launch_bounds__(kRankThreads) void k_cluster_rank(const Frame *fram
     const uint32_t ncl = F.counters[kNClusters];
     uint32_t m = 1;
     while (m < ncl) m <<= 1;
+    const uint64_t bm = F.bucket_mask;
+    if (m <= kRankThreads) {
+        // a cluster per thread in registers: (bucket, id, table slot),
+        // bitonic, partners in the wave by lane exchanges, the others
+        // through LDS; after it thread t holds the cluster of rank t
+        uint32_t slot = t < ncl ? F.clist[t] : 0u;
+        uint64_t id = t < ncl ? F.hkey[slot] : kSlotEmpty;
+        uint32_t bk = id == kSlotEmpty ? 0xffffffffu : (uint32_t)(((id * 2654435761ull) >> 32) & bm);
+        uint64_t *xid = lk;
+        uint32_t *xbk = reinterpret_cast<uint32_t *>(lk + kRankThreads), *xsl = xbk + kRankThreads;
+        for (uint32_t k = 2; k <= m; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t p = t ^ j;
+                uint32_t obk, osl;
+                uint64_t oid;
+                if (j >= 64) {
+                    xid[t] = id;
+                    xbk[t] = bk;
+                    xsl[t] = slot;
+                    __syncthreads();
+                    oid = xid[p];
+                    obk = xbk[p];
+                    osl = xsl[p];
+                    __syncthreads();
+                } else {
+                    obk = (uint32_t)__shfl_xor((int)bk, (int)j, 64);
+                    osl = (uint32_t)__shfl_xor((int)slot, (int)j, 64);
+                    oid = ((uint64_t)(uint32_t)__shfl_xor((int)(uint32_t)(id >> 32), (int)j, 64) << 32) |
+                          (uint32_t)__shfl_xor((int)(uint32_t)id, (int)j, 64);
+                }
+                const bool ob = obk < bk || (obk == bk && oid < id), mb = bk < obk || (bk == obk && id < oid);
+                // ascending blocks keep the smaller at the lower index
+                if ((t < p) == ((t & k) == 0) ? ob : mb) {
+                    bk = obk;
+                    id = oid;
+                    slot = osl;
+                }
+            }
+        if (t < ncl) {
+            F.hrank[slot] = t;
+            F.cid[t] = id;
+        }
+        if (t == 0) F.counters[kSortBits] = ncl <= 1 ? 1u : 32u - (uint32_t)__clz(ncl - 1);
+        return;
+    }
     uint64_t *keys = m <= kRankLds ? lk : F.csort;
     for (uint32_t i = t; i < m; i += kRankThreads) keys[i] = i < ncl ? F.hkey[F.clist[i]] : kSlotEmpty;
     __syncthreads();
-    const uint64_t bm = F.bucket_mask;
     for (uint32_t k = 2; k <= m; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t i = t; i < m; i += kRankThreads) {
