@@ -92,6 +92,15 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31, rows 2, 3
     return v;
 }
+// The global-memory view of a pointer into device memory (HBM). Through
+// a plain pointer loaded from a frame descriptor the compiler uses flat
+// instructions, which also count against the LDS wait counter: a wait for
+// an LDS load then waits for the HBM accesses in flight as well.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // x of lane i - 1; lane 0 gets fill
 __device__ __forceinline__ uint32_t wave_from_left(uint32_t x, uint32_t fill) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)x, 0x138, 0xF, 0xF, false);  // wave_shr:1

@@ -248,6 +248,21 @@ constexpr uint32_t kPortLive = 0x80000000u;   // (component record) reaches row 
 template <bool G, int kFrontThreads, int mode>
 __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, int y0, int y1, int olo, int ohi) {
     constexpr int kPer = kStage / kFrontThreads;  // prefetched runs and links per thread
+    // the frame's arrays in HBM, as global pointers (flat accesses would
+    // hold up the waits for the LDS union-find)
+    const auto g_runs = gmem(F.runs);
+    const auto g_uedge = gmem(F.uedge);
+    const auto g_rcnt = gmem(F.rcnt);
+    const auto g_ecnt = gmem(F.ecnt);
+    const auto g_t = gmem(F.t);
+    const auto g_rlab = gmem(F.rlab);
+    const auto g_recw = gmem(F.recw);
+    const auto g_rroot = gmem(F.rroot);
+    const auto g_rlost = gmem(F.rlost);
+    const auto g_parent = gmem(F.parent);
+    const auto g_fw = gmem(F.fw);
+    const auto g_row_off = gmem(F.row_off);
+    const auto g_counters = gmem(F.counters);
     using U32 = typename Mem<G>::U32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int QW = F.qw, QH = F.qh;
@@ -279,32 +294,32 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
         uint32_t carry = 0;
         for (int y00 = 0; y00 < QH; y00 += 64) {
             int y = y00 + lane;
-            uint32_t c = y < QH ? F.rcnt[y] : 0u;
+            uint32_t c = y < QH ? g_rcnt[y] : 0u;
             uint32_t inc = wave_incl_scan(c);
-            if (y < QH && y >= olo && y <= ohi) F.row_off[y] = carry + inc - c;
+            if (y < QH && y >= olo && y <= ohi) g_row_off[y] = carry + inc - c;
             if (y == yp) lds_small[0] = carry + inc - c;
             carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
         }
         if (lane == 0 && ohi == QH - 1) {
-            F.row_off[QH] = carry;
-            F.counters[kNRuns] = carry;
+            g_row_off[QH] = carry;
+            g_counters[kNRuns] = carry;
         }
         for (int y = olo + lane; y <= ohi; y += 64) {
-            F.parent[carry + y] = carry + y;
-            F.fw[carry + y] = 1;
+            g_parent[carry + y] = carry + y;
+            g_fw[carry + y] = 1;
         }
     }
     __syncthreads();
     uint32_t pbase = lds_small[0];  // row_off of the previous row
     // the starting row, as settled
-    int pcur = (int)F.rcnt[yp];
+    int pcur = (int)g_rcnt[yp];
     for (int i = tid; i < pcur; i += kFrontThreads) {
         if (mode == kSweepFull) {
-            const uint32_t r = F.runs[i];
+            const uint32_t r = g_runs[i];
             pp[QW + i] = (uint32_t)(QW + i);
             pmin[QW + i] = (uint32_t)i;
             pacc[QW + i] = run_end(r) - run_start(r) + 1;
-            F.parent[i] = (uint32_t)i;
+            g_parent[i] = (uint32_t)i;
         } else if (mode == kSweepLocal) {
             pp[QW + i] = (uint32_t)(QW + i);
             pmin[QW + i] = (uint32_t)i;
@@ -312,11 +327,11 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
             pmp[QW + i] = (uint32_t)i;
             port[i] = (uint32_t)i;
         } else {
-            const uint32_t l = F.rlab[pbase + i];
+            const uint32_t l = g_rlab[pbase + i];
             pp[QW + i] = (uint32_t)QW + l;
             if (l == (uint32_t)i) {
                 pmin[QW + i] = (uint32_t)i;
-                pacc[QW + i] = F.recw[pbase + i];
+                pacc[QW + i] = g_recw[pbase + i];
             }
         }
     }
@@ -327,17 +342,17 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
     uint32_t nx_rc = 0, nx_ec = 0, nx_r[kPer], nx_e[kPer];
     uint8_t nx_t1 = 0, nx_t2 = 0;
     auto prefetch = [&](int y) {
-        const uint32_t *cur = F.runs + (int64_t)y * QW;
-        const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
-        nx_rc = F.rcnt[y];
-        nx_ec = F.ecnt[y];
+        const auto cur = g_runs + (int64_t)y * QW;
+        const auto edges = g_uedge + (int64_t)y * 3 * QW;
+        nx_rc = g_rcnt[y];
+        nx_ec = g_ecnt[y];
 #pragma unroll
         for (int j = 0; j < kPer; j++) {
             int i = tid + j * kFrontThreads;
             nx_r[j] = i < QW ? cur[i] : 0u;
             nx_e[j] = i < 3 * QW ? edges[i] : 0u;
         }
-        const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
+        const auto trow = g_t + (int64_t)(y - 1) * QW;
         nx_t1 = trow[QW - 1];
         nx_t2 = trow[QW - 2];
     };
@@ -354,8 +369,8 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
         }
         const uint32_t cbase = pbase + (uint32_t)pcur;
         if (y + 1 <= y1) prefetch(y + 1);
-        const uint32_t *cur = F.runs + (int64_t)y * QW;
-        const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
+        const auto cur = g_runs + (int64_t)y * QW;
+        const auto edges = g_uedge + (int64_t)y * 3 * QW;
         // the previous row's outputs (each run's record, its first run,
         // and at a record's first run its weight), and the elements: the
         // records of row y-1 (weight 0 at the other slots, which no link
@@ -366,8 +381,8 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
             const uint32_t l = pmin[r];
             const uint32_t wgt = l == (uint32_t)x ? pacc[r] : 0u;
             if (out) {
-                F.rlab[pbase + x] = l;
-                if (l == (uint32_t)x) F.recw[pbase + x] = wgt;
+                g_rlab[pbase + x] = l;
+                if (l == (uint32_t)x) g_recw[pbase + x] = wgt;
             }
             p[x] = (uint32_t)x;
             acc[x] = wgt;
@@ -386,7 +401,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
                 } else {
                     const uint32_t g = pmp[r];
                     port[x] = kPortEnded | g;
-                    F.rroot[portslot + g] = pacc[r];  // (every port of the component writes the same)
+                    g_rroot[portslot + g] = pacc[r];  // (every port of the component writes the same)
                 }
             }
         bool lastcol = false;  // fast/unionfind.c:93-97, on the thread of the last run
@@ -458,8 +473,8 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
     for (int i = tid; i < pcur; i += kFrontThreads) {
         const uint32_t r = pp[QW + i];
         const uint32_t l = pmin[r];
-        F.rlab[pbase + i] = l;
-        if (l == (uint32_t)i) F.recw[pbase + i] = pacc[r];
+        g_rlab[pbase + i] = l;
+        if (l == (uint32_t)i) g_recw[pbase + i] = pacc[r];
     }
     if (mode == kSweepLocal) {
         for (int x = tid; x < nport; x += kFrontThreads) {
@@ -470,9 +485,9 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
             } else {
                 const uint32_t r = pp[q], m = pmin[r];
                 g = pmp[r];
-                F.rroot[portslot + g] = m != 0xffffffffu ? kPortLive | m : pacc[r];
+                g_rroot[portslot + g] = m != 0xffffffffu ? kPortLive | m : pacc[r];
             }
-            F.rlost[portslot + x] = g;
+            g_rlost[portslot + x] = g;
         }
     }
 }
@@ -616,6 +631,8 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
         return;
     }
     const int tid = threadIdx.x;
+    const auto g_rlost = gmem(F.rlost), g_rroot = gmem(F.rroot), g_rlab = gmem(F.rlab), g_recw = gmem(F.recw);
+    const auto g_row_off = gmem(F.row_off), g_rcnt = gmem(F.rcnt);
     auto settle = [&](uint32_t x) {
         const uint32_t r = fr_find<false>(p, x);
         if (r != x) {
@@ -630,10 +647,10 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
     for (int k = tid; k < nb; k += T) {
         const int r0 = band_start(F, k, nb) - 1, e = band_start(F, k + 1, nb) - 1;
         if (k > 0) {
-            rows[k][0] = F.row_off[r0];
-            rows[k][1] = F.rcnt[r0];
-            rows[k][2] = F.row_off[e];
-            rows[k][3] = F.rcnt[e];
+            rows[k][0] = g_row_off[r0];
+            rows[k][1] = g_rcnt[r0];
+            rows[k][2] = g_row_off[e];
+            rows[k][3] = g_rcnt[e];
         }
     }
     __syncthreads();
@@ -651,10 +668,10 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
 #pragma unroll
         for (int j = 0; j < kR; j++) {
             const uint32_t x = (uint32_t)(tid + j * T);
-            in.g[j] = x < in.n0 ? F.rlost[in.b0 + x] : 0u;
-            in.info[j] = x < in.n0 ? F.rroot[in.b0 + x] : 0u;
-            in.lab[j] = x < in.ne ? F.rlab[in.be + x] : 0u;
-            in.wt[j] = x < in.ne ? F.recw[in.be + x] : 0u;
+            in.g[j] = x < in.n0 ? g_rlost[in.b0 + x] : 0u;
+            in.info[j] = x < in.n0 ? g_rroot[in.b0 + x] : 0u;
+            in.lab[j] = x < in.ne ? g_rlab[in.be + x] : 0u;
+            in.wt[j] = x < in.ne ? g_recw[in.be + x] : 0u;
         }
     };
     // the final records above band 1: band 0's last row (final after step 1)
@@ -664,8 +681,8 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
 #pragma unroll
         for (int j = 0; j < kR; j++) {
             const uint32_t x = (uint32_t)(tid + j * T);
-            alab[j] = x < n0 ? F.rlab[b0 + x] : 0u;
-            aw[j] = x < n0 && alab[j] == x ? F.recw[b0 + x] : 0u;
+            alab[j] = x < n0 ? g_rlab[b0 + x] : 0u;
+            aw[j] = x < n0 && alab[j] == x ? g_recw[b0 + x] : 0u;
         }
     }
     In cur, nxt;
@@ -740,8 +757,8 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
             if (i < ne) {
                 const uint32_t r = p[QW + cur.lab[j]], l = mn[r];
                 const uint32_t wr = w[r];
-                F.rlab[cur.be + i] = l;
-                if (l == i) F.recw[cur.be + i] = wr;
+                g_rlab[cur.be + i] = l;
+                if (l == i) g_recw[cur.be + i] = wr;
                 alab[j] = l;
                 aw[j] = l == i ? wr : 0u;
             }
@@ -949,6 +966,19 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     using IX = typename std::conditional<G, volatile uint32_t, uint16_t>::type;
     using U8 = typename std::conditional<G, volatile uint8_t, uint8_t>::type;
     const int lane = threadIdx.x;
+    // the frame's arrays in HBM as global pointers (no flat accesses among the LDS work)
+    const auto g_row_off = gmem(F.row_off);
+    const auto g_nrec = gmem(F.nrec);
+    const auto g_rcnt = gmem(F.rcnt);
+    const auto g_ecnt = gmem(F.ecnt);
+    const auto g_runs = gmem(F.runs);
+    const auto g_uedge = gmem(F.uedge);
+    const auto g_rlab = gmem(F.rlab);
+    const auto g_recw = gmem(F.recw);
+    const auto g_t = gmem(F.t);
+    const auto g_rroot = gmem(F.rroot);
+    const auto g_parent = gmem(F.parent);
+    const auto g_rlost = gmem(F.rlost);
     const int QW = F.qw;
     const ReplayLayout ly = replay_layout(C, L, G ? 4 : 2, PK ? 4 : 8);
     const Ents<G, PK> ent{reinterpret_cast<U32 *>(base)};
@@ -957,12 +987,12 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     IX *head = reinterpret_cast<IX *>(base + ly.head);
     U16 *labA = reinterpret_cast<U16 *>(base + ly.labA);
     U8 *hasl = reinterpret_cast<U8 *>(base + ly.hasl);
-    const uint32_t pb = F.row_off[y - 1], cb = F.row_off[y], vb = F.row_off[F.qh];
-    const int m = (int)F.nrec[y - 1], nprev = (int)F.rcnt[y - 1];
-    const int ncur = (int)F.rcnt[y], ne = (int)F.ecnt[y], ng = (int)F.nrec[y];
-    const uint32_t *cur = F.runs + (int64_t)y * QW;
-    const uint32_t *edges = F.uedge + (int64_t)y * 3 * QW;
-    const uint32_t *rlab_cur = F.rlab + cb;
+    const uint32_t pb = g_row_off[y - 1], cb = g_row_off[y], vb = g_row_off[F.qh];
+    const int m = (int)g_nrec[y - 1], nprev = (int)g_rcnt[y - 1];
+    const int ncur = (int)g_rcnt[y], ne = (int)g_ecnt[y], ng = (int)g_nrec[y];
+    const auto cur = g_runs + (int64_t)y * QW;
+    const auto edges = g_uedge + (int64_t)y * 3 * QW;
+    const auto rlab_cur = g_rlab + cb;
     const int ev = m + ncur;
     // the elements (records with their weights, current runs with their
     // lengths), each previous run's record (labA) and each current run's
@@ -973,10 +1003,10 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int i = b0 + 64 * q + lane;
-            rw[q] = i < m ? F.recw[pb + i] : 0u;
+            rw[q] = i < m ? g_recw[pb + i] : 0u;
             rr[q] = i < ncur ? cur[i] : 0u;
             rc[q] = i < ncur ? rlab_cur[i] : 0u;
-            rp[q] = i < nprev ? F.rlab[pb + i] : 0u;
+            rp[q] = i < nprev ? g_rlab[pb + i] : 0u;
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -998,7 +1028,7 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     bool lastcol = false;
     if (ncur > 0) {
         uint32_t lr = cur[ncur - 1];
-        const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
+        const auto trow = g_t + (int64_t)(y - 1) * QW;
         lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
     }
     wsync<G>();
@@ -1143,10 +1173,10 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     };
     for (int l = lane; l < m; l += 64) {
         uint32_t q = ent.par((uint32_t)l);
-        F.rlost[pb + l] = q == (uint32_t)l ? (hasl[l] ? kLostNone : kLostRetired) : name(q);
+        g_rlost[pb + l] = q == (uint32_t)l ? (hasl[l] ? kLostNone : kLostRetired) : name(q);
     }
-    for (int x = m + lane; x <= ev; x += 64) F.parent[name((uint32_t)x)] = name(ent.par((uint32_t)x));
-    for (int i = lane; i < ncur; i += 64) F.rroot[cb + (uint32_t)nxt[i]] = name(ent.find_ro((uint32_t)(m + i)));
+    for (int x = m + lane; x <= ev; x += 64) g_parent[name((uint32_t)x)] = name(ent.par((uint32_t)x));
+    for (int i = lane; i < ncur; i += 64) g_rroot[cb + (uint32_t)nxt[i]] = name(ent.find_ro((uint32_t)(m + i)));
 }
 
 // A wave per row, the rows that fit C = cap runs and replay_links(C) links; the others

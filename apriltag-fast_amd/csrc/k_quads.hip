@@ -853,7 +853,8 @@ __global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, con
 // moment array q in HBM, straight from the chain: no LDS write-back on the
 // dependent path). Each of the kRing registers takes the term kRing ahead
 // as soon as its add has used it. Returns the sum.
-__device__ __forceinline__ double scan_chunk(const double *a, uint32_t len, double s, double *out) {
+__device__ __forceinline__ double scan_chunk(const double *a, uint32_t len, double s, double *out_p) {
+    auto out = gmem(out_p);
     constexpr uint32_t kRing = ATG_SCAN_RING;
     static_assert(kRing % 2 == 0, "pairs of sums a store");
     uint32_t j = 0;
@@ -896,19 +897,32 @@ __device__ __forceinline__ double scan_chunk(const double *a, uint32_t len, doub
     return s;
 }
 
-// The six moment terms of points [c0, c0 + len) of a cluster in key order
-// (fast/quads.c:968-1035: weight sqrt(g2) + 1, centres x / 2 + 1 / 2) into
-// b[q][j], for j = t0, t0 + stride, ... (KP of them at most: the point
-// loads are issued together).
-template <int KP, int CH>
-__device__ __forceinline__ void moment_terms(const Frame &F, uint32_t s0, const uint64_t *keys, uint32_t c0,
-                                             uint32_t len, double (*b)[CH + 2], int t0, int stride) {
-    uint64_t pwv[KP];
+// The moment terms of points [c0, c0 + len) of a cluster in key order, in
+// three steps (so that a caller can keep the loads a chunk or two ahead):
+// the points' indices from the keys, their packed (x, y, g2) words, and
+// the six terms (fast/quads.c:968-1035: weight sqrt(g2) + 1, centres
+// x / 2 + 1 / 2) into b[q][j], for j = t0, t0 + stride, ... (KP of them).
+template <int KP>
+__device__ __forceinline__ void term_points(const uint64_t *keys, uint32_t c0, uint32_t len, int t0, int stride,
+                                            uint32_t (&ix)[KP]) {
 #pragma unroll
     for (int r = 0; r < KP; r++) {
         const uint32_t j = (uint32_t)(t0 + r * stride);
-        pwv[r] = j < len ? F.pw[s0 + (uint32_t)~(uint32_t)keys[c0 + j]] : 0ull;
+        ix[r] = j < len ? (uint32_t)~(uint32_t)gmem(keys)[c0 + j] : 0u;
     }
+}
+template <int KP>
+__device__ __forceinline__ void term_words(const Frame &F, uint32_t s0, uint32_t len, int t0, int stride,
+                                           const uint32_t (&ix)[KP], uint64_t (&pwv)[KP]) {
+#pragma unroll
+    for (int r = 0; r < KP; r++) {
+        const uint32_t j = (uint32_t)(t0 + r * stride);
+        pwv[r] = j < len ? gmem(F.pw)[s0 + ix[r]] : 0ull;
+    }
+}
+template <int KP, int CH>
+__device__ __forceinline__ void term_values(const uint64_t (&pwv)[KP], uint32_t len, double (*b)[CH + 2], int t0,
+                                            int stride) {
 #pragma unroll
     for (int r = 0; r < KP; r++) {
         const uint32_t j = (uint32_t)(t0 + r * stride);
@@ -926,6 +940,23 @@ __device__ __forceinline__ void moment_terms(const Frame &F, uint32_t s0, const 
         b[4][j] = __dmul_rn(wfy, fy);
         b[5][j] = wgt;
     }
+}
+template <int KP, int CH>
+__device__ __forceinline__ void moment_terms(const Frame &F, uint32_t s0, const uint64_t *keys, uint32_t c0,
+                                             uint32_t len, double (*b)[CH + 2], int t0, int stride) {
+    uint32_t ix[KP];
+    uint64_t pwv[KP];
+    term_points<KP>(keys, c0, len, t0, stride, ix);
+    term_words<KP>(F, s0, len, t0, stride, ix, pwv);
+    term_values<KP, CH>(pwv, len, b, t0, stride);
+}
+
+// A barrier of the block that orders LDS only (loads from HBM issued before
+// it stay in flight).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Persistent over list l of kept clusters, block b taking global indices
@@ -981,18 +1012,31 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
         double *out = F.mom + (t < 6 ? t : 0) * pc + s0;
         const uint32_t nch = (n + CH - 1) / CH;
         if constexpr (kBufs == 2) {
-            moment_terms<(CH + NT - 1) / NT, CH>(F, s0, keys, 0, min((uint32_t)CH, n), sm[0], t, NT);
+            // the other waves keep their loads ahead of the chain: the
+            // points of the chunk after next and the words of the next one
+            // in registers (the barriers order LDS only)
+            constexpr int KP = (CH + NT - 65) / (NT - 64);
+            const int th = t - 64, sh = NT - 64;
+            auto clen = [&](uint32_t ci) { return ci < nch ? min((uint32_t)CH, n - ci * CH) : 0u; };
+            uint32_t ix[KP];
+            uint64_t pwv[KP];
+            moment_terms<(CH + NT - 1) / NT, CH>(F, s0, keys, 0, clen(0), sm[0], t, NT);
+            if (w > 0) {
+                term_points<KP>(keys, CH, clen(1), th, sh, ix);
+                term_words<KP>(F, s0, clen(1), th, sh, ix, pwv);
+                term_points<KP>(keys, 2 * CH, clen(2), th, sh, ix);
+            }
             __syncthreads();
             for (uint32_t ci = 0; ci < nch; ci++) {
-                const uint32_t c0 = ci * CH, len = min((uint32_t)CH, n - c0);
+                const uint32_t c0 = ci * CH, len = clen(ci);
                 if (w == 0) {
                     if (t < 6) run = scan_chunk(sm[ci & 1][t], len, run, out + c0);
                 } else if (ci + 1 < nch) {
-                    const uint32_t c1 = c0 + CH;
-                    moment_terms<(CH + NT - 65) / (NT - 64), CH>(F, s0, keys, c1, min((uint32_t)CH, n - c1),
-                                                                 sm[(ci + 1) & 1], t - 64, NT - 64);
+                    term_values<KP, CH>(pwv, clen(ci + 1), sm[(ci + 1) & 1], th, sh);
+                    term_words<KP>(F, s0, clen(ci + 2), th, sh, ix, pwv);
+                    term_points<KP>(keys, (ci + 3) * CH, clen(ci + 3), th, sh, ix);
                 }
-                __syncthreads();
+                lds_barrier();
             }
         } else {
             for (uint32_t ci = 0; ci < nch; ci++) {
@@ -1683,7 +1727,8 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     if (nframes > 4)
         hipLaunchKernelGGL(k_fit_prep<64>, dim3(512, nframes), dim3(64), 0, st, dframes, dparams, sort_cap ? 0 : 1);
     else
-        hipLaunchKernelGGL(k_fit_prep<256>, dim3(256, nframes), dim3(256), 0, st, dframes, dparams, sort_cap ? 0 : 1);
+        hipLaunchKernelGGL(k_fit_prep<256>, dim3(nframes <= 2 ? 1024 : 256, nframes), dim3(256), 0, st, dframes, dparams,
+                           sort_cap ? 0 : 1);
     mk->mark(st, "fit_prep.prep");
     hipLaunchKernelGGL(k_keep_lists, dim3(1, nframes), dim3(256), 0, st, dframes);
     // the huge, large and small lists, a block each
