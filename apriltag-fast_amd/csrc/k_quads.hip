@@ -116,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
-        const uint64_t *pts = F.cpts + s0;
+        const auto pts = gmem(F.cpts) + s0;
         bool in_range = (int64_t)n >= P->mcp && (int64_t)n <= F.max_px && n >= kMinFitPoints;
         uint64_t pv[kR];
 #pragma unroll
@@ -160,11 +160,11 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
                 pt_unpack(pt_at(i), &x, &y, &gx, &gy);
                 slope = point_slope(__fsub_rn((float)x, cx), __fsub_rn((float)y, cy));
             }
-            F.okey[s0 + i] = ((uint64_t)slope << 32) | (uint32_t)~i;
+            gmem(F.okey)[s0 + i] = ((uint64_t)slope << 32) | (uint32_t)~i;
             if (items) {  // for the sort of all points at once
                 uint32_t pos = s0 + (n - 1 - i);
-                F.skey[pos] = ((uint64_t)c << 32) | slope;
-                F.sval[pos] = i;
+                gmem(F.skey)[pos] = ((uint64_t)c << 32) | slope;
+                gmem(F.sval)[pos] = i;
             }
         }
         if (!ok) {
@@ -214,11 +214,11 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
                 int ix = (px >> 1) + (px & 1), iy = (py >> 1) + (py & 1);
                 uint32_t g2 = 0;
                 if (ix > 0 && ix + 1 < iw && iy > 0 && iy + 1 < ih) {
-                    const uint8_t *im = F.qim + (int64_t)iy * iw + ix;
+                    const auto im = gmem(F.qim) + (int64_t)iy * iw + ix;
                     int ggx = (int)im[1] - (int)im[-1], ggy = (int)im[iw] - (int)im[-iw];
                     g2 = (uint32_t)(ggx * ggx + ggy * ggy);
                 }
-                F.pw[s0 + i] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)g2 << 32);
+                gmem(F.pw)[s0 + i] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)g2 << 32);
             }
         }
     }
@@ -328,7 +328,7 @@ __device__ __forceinline__ void run_entry(const uint64_t *orig, int64_t n, const
     if (g1 - g0 <= 1) return;
     uint64_t lf[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) lf[q] = q < len ? orig[lo + q] : 0ull;
+    for (int q = 0; q < 5; q++) lf[q] = q < len ? gmem(orig)[lo + q] : 0ull;
     leaf_network(lf, (int)len);
     const int j = (int)i - g0;
     uint64_t top = 0;
@@ -402,20 +402,20 @@ __device__ __noinline__ void order_run_long(const uint64_t *orig, int64_t n, uin
             while (g1 < (int)m && leaf_lo((uint32_t)g1) == lo) g1++;
             uint64_t e, b;
             run_entry(orig, n, keys, s, m, hi, i, key, lo, len, g0, g1, &e, &b);
-            ent_s[i] = e;
-            bh_s[i] = b;
+            gmem(ent_s)[i] = e;
+            gmem(bh_s)[i] = b;
         }
     }
     __threadfence_block();
     for (uint32_t r0 = 0; r0 < m; r0 += 64) {
         const uint32_t i = r0 + (uint32_t)lane;
-        const uint64_t b = i < m ? bh_s[i] : 0ull;
+        const uint64_t b = i < m ? gmem(bh_s)[i] : 0ull;
         uint32_t rank = 0;
         for (uint32_t j = 0; j < m; j++) {
-            const uint64_t b2 = bh_s[j];
+            const uint64_t b2 = gmem(bh_s)[j];
             rank += (b2 < b || (b2 == b && j < i)) ? 1u : 0u;
         }
-        if (i < m) keys[s + rank] = ent_s[i];
+        if (i < m) keys[s + rank] = gmem(ent_s)[i];
     }
 }
 
@@ -661,13 +661,13 @@ __device__ __forceinline__ void wave_sort_keys(const uint64_t *in, uint64_t *out
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint32_t i = (uint32_t)(r * 64 + lane);
-        v[r] = i < n ? in[i] : ~0ull;
+        v[r] = i < n ? gmem(in)[i] : ~0ull;
     }
     wave_bitonic<R>(v);
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint32_t i = (uint32_t)(r * 64 + lane);
-        if (i < n) out[i] = v[r];
+        if (i < n) gmem(out)[i] = v[r];
     }
 }
 
@@ -722,7 +722,7 @@ __device__ __forceinline__ void sort_tiles(const uint64_t *src, uint64_t *dst, u
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const uint32_t i = w * 512 + r * 64 + lane;
-            v[r] = i < tn ? src[t0 + i] : ~0ull;
+            v[r] = i < tn ? gmem(src)[t0 + i] : ~0ull;
         }
         if (kk == 0) {
             wave_bitonic<8>(v);        // each wave's 512 keys, ascending
@@ -739,7 +739,7 @@ __device__ __forceinline__ void sort_tiles(const uint64_t *src, uint64_t *dst, u
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const uint32_t i = w * 512 + r * 64 + lane;
-            if (i < tn) dst[t0 + i] = v[r];
+            if (i < tn) gmem(dst)[t0 + i] = v[r];
         }
         __syncthreads();
     }
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(1024) void k_angle_sort_big(const Frame *frames, co
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
-            v[r] = i < n ? F.okey[s0 + i] : ~0ull;
+            v[r] = i < n ? gmem(F.okey)[s0 + i] : ~0ull;
         }
         wave_bitonic<8>(v);
         for (uint32_t k = 1024; k <= m; k <<= 1) {
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(1024) void k_angle_sort_big(const Frame *frames, co
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
-            if (i < n) F.skey[s0 + i] = v[r];
+            if (i < n) gmem(F.skey)[s0 + i] = v[r];
         }
     }
 }
@@ -1056,9 +1056,9 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
 // The six cumulative moment arrays of a cluster (mx, my, mxx, mxy, myy, w),
 // cluster-relative: array q starts at b + q * stride.
 struct Mom {
-    const double *b;
+    const __attribute__((address_space(1))) double *b;
     int64_t stride;
-    __device__ __forceinline__ const double *a(int q) const { return b + q * stride; }
+    __device__ __forceinline__ const __attribute__((address_space(1))) double *a(int q) const { return b + q * stride; }
 };
 
 // The window sums of [i0, i1], wrapping, as fit_line forms them
@@ -1071,7 +1071,7 @@ __device__ __forceinline__ void range_sums(const Mom &M, int64_t sz, int64_t i0,
     double a[6], b[6], c[6];
     #pragma unroll
     for (int q = 0; q < 6; q++) {
-        const double *v = M.a(q);
+        const auto v = M.a(q);
         a[q] = v[sz - 1];
         b[q] = v[ib];
         c[q] = v[i1];
@@ -1212,7 +1212,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
         if (!(flag & 1u)) continue;
         const uint32_t s0 = F.cstart[c];
         const int64_t sz = F.cstart[c + 1] - s0;
-        const Mom M = {F.mom + s0, pc};
+        const Mom M = {gmem(F.mom) + s0, pc};
         int64_t ksz = min((int64_t)20, sz / 12);
         if (ksz < 2) continue;
         double nn = (double)(2 * ksz + 1);
@@ -1235,7 +1235,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
             double b[6], c[6], s[6];
             #pragma unroll
             for (int q = 0; q < 6; q++) {
-                const double *v = M.a(q);
+                const auto v = M.a(q);
                 b[q] = v[ib];
                 c[q] = v[ic];
             }
