@@ -166,8 +166,9 @@ struct Frame {
     uint32_t *rwide;      // [qh] rows for the replay's wide launch
     // the frame's cluster table (open addressing, hmask + 1 slots): each
     // cluster's id (kSlotEmpty where none), point count and rank; the
-    // slots in insertion order ([hmask + 1) / 2]); the sort's scratch
-    uint32_t hmask;
+    // slots in insertion order ([hmask + 1) / 2]); the sort's scratch;
+    // ccap clusters at most (more: error flag 128)
+    uint32_t hmask, ccap;
     uint64_t *hkey;
     uint32_t *hcnt, *hrank, *clist;
     uint64_t *csort;

@@ -67,6 +67,9 @@ constexpr uint32_t kOnePass = 1u << kRadixBits;
 __device__ uint32_t cluster_slot(const Frame &F, uint64_t id) {
     uint32_t s = (uint32_t)((id * 0x9E3779B97F4A7C15ull) >> 32) & F.hmask;
     for (uint32_t probe = 0; probe <= F.hmask; probe++, s = (s + 1) & F.hmask) {
+        // a long probe: past an overflow the call is re-run with a larger
+        // table, so an overfull table is not searched to its end
+        if ((probe & 63u) == 63u && (__atomic_load_n(F.counters + kError, __ATOMIC_RELAXED) & 128u)) return 0;
         const uint64_t k = __atomic_load_n(&F.hkey[s], __ATOMIC_RELAXED);
         if (k == id) return s;
         if (k != kSlotEmpty) continue;
@@ -74,7 +77,7 @@ __device__ uint32_t cluster_slot(const Frame &F, uint64_t id) {
                                        (unsigned long long)id);
         if (old == kSlotEmpty) {
             const uint32_t at = atomicAdd(F.counters + kNClusters, 1u);
-            if (at < (F.hmask + 1) / 2) F.clist[at] = s;
+            if (at < F.ccap) F.clist[at] = s;
             else atomicOr(F.counters + kError, 128u);
             return s;
         }
@@ -690,9 +693,13 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     mk->mark(st, "points.write");
     hipLaunchKernelGGL(k_cluster_rank, dim3(1, nframes), dim3(kRankThreads), kRankLds * 8, st, dframes);
     mk->mark(st, "clusters.rank");
-    hipLaunchKernelGGL(k_rank_clear, dim3(8, nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_rank_counts, dim3(sort_blocks(nframes), nframes), dim3(256), 0, st, dframes);
-    hipLaunchKernelGGL(k_cluster_starts, dim3(1, nframes), dim3(kRankThreads), 0, st, dframes);
+    // the point counts of frames of kOnePass clusters or more (none where
+    // every frame's table holds fewer: one pass gives the starts)
+    if (key_bits > kRadixBits) {
+        hipLaunchKernelGGL(k_rank_clear, dim3(8, nframes), dim3(256), 0, st, dframes);
+        hipLaunchKernelGGL(k_rank_counts, dim3(sort_blocks(nframes), nframes), dim3(256), 0, st, dframes);
+        hipLaunchKernelGGL(k_cluster_starts, dim3(1, nframes), dim3(kRankThreads), 0, st, dframes);
+    }
     // key_bits: the most rank bits the call's frames can have; the passes
     // past a frame's own return at once, the last one it runs writes the
     // points in place

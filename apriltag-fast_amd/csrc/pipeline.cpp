@@ -86,9 +86,15 @@ size_t queue_bytes(int64_t nframes) {
 int64_t cluster_cap(int64_t pcap) { return pcap / 4 + 64; }
 
 // Slots of a frame's cluster table (a power of two, filled at most half):
-// first sized for a cluster per 32 points; after an overflow (error flag
-// 128) for every point a cluster of its own.
-int64_t hash_slots(int64_t pcap, bool full) {
+// first, for quad images of up to kOnePassPixels, kOnePassSlots (a
+// 960x540 image of the workload has about a thousand clusters: room for
+// 2047, whose ranks take one radix pass), for larger ones a cluster per
+// 32 points; after an overflow (error flag 128) for every point a cluster
+// of its own.
+constexpr int64_t kOnePassPixels = 960 * 540, kOnePassSlots = 4096;
+bool one_pass_table(int64_t px, bool full) { return !full && px <= kOnePassPixels; }
+int64_t hash_slots(int64_t pcap, int64_t px, bool full) {
+    if (one_pass_table(px, full)) return kOnePassSlots;
     const int64_t want = full ? 2 * pcap + 2 : std::max<int64_t>(2048, pcap / 16);
     int64_t n = 1;
     while (n < want) n <<= 1;
@@ -380,8 +386,11 @@ size_t layout(const FramePlan &fp, int nfam, bool host_input, Frame *F, uint8_t 
     F->sval2 = (uint32_t *)take((size_t)pc * 4);
     F->rhist = (uint32_t *)take((size_t)kRadixBins * kSortBlocks * 4);
     F->rwide = (uint32_t *)take((size_t)(qh + 1) * 4);
-    const int64_t hs = hash_slots(pc, fp.hfull);
+    const int64_t hs = hash_slots(pc, px, fp.hfull);
     F->hmask = (uint32_t)(hs - 1);
+    // clusters the table takes before the overflow flag: one radix pass's
+    // worth (2047) for a one-pass table
+    F->ccap = (uint32_t)(one_pass_table(px, fp.hfull) ? (1 << kRadixBits) - 1 : hs / 2);
     F->hkey = (uint64_t *)take((size_t)hs * 8);
     F->hcnt = (uint32_t *)take((size_t)hs * 4);
     F->hrank = (uint32_t *)take((size_t)hs * 4);
@@ -589,7 +598,7 @@ struct ChainShape {
         max_weight = std::max<int64_t>(max_weight, nodes);
         max_qw = std::max<int>(max_qw, (int)fp.qw);
         max_qh = std::max<int>(max_qh, (int)fp.qh);
-        key_bits = std::max(key_bits, ceil_log2((uint64_t)hash_slots(fp.pcap, fp.hfull) / 2));  // cluster ranks
+        key_bits = std::max(key_bits, ceil_log2((uint64_t)hash_slots(fp.pcap, fp.qw * fp.qh, fp.hfull) / 2));  // cluster ranks
         cl_bits = std::max(cl_bits, ceil_log2((uint64_t)fp.pcap + 1));
         max_cluster = std::max(max_cluster, (int)(2 * (2 * fp.qw + 2 * fp.qh)));
     }
