@@ -425,7 +425,7 @@ class PinnedFrame:
 class Server:
     """The serving loop (atg_server): streams, arrivals, polled results."""
 
-    def __init__(self, families, params=None, budget_bytes=16 << 30, lanes=3, max_batch=128, slots=2, device=0):
+    def __init__(self, families, params=None, budget_bytes=16 << 30, lanes=3, max_batch=128, slots=0, device=0):
         self.L = lib()
         cfg = ServerConfig()
         self.L.atg_server_config_default(C.byref(cfg))

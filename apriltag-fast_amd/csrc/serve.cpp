@@ -1,17 +1,19 @@
 // The serving loop: frames from many camera streams, admitted as they
 // arrive into batches that overlap on the device.
 //
-// Each stream has a few input slots resident in HBM (the frame's H2D copy
-// lands there) and a bounded queue. A worker thread owns the device: it
-// releases fed arrivals at their times, collects finished batches, and
-// whenever a lane (a kernel chain on its own stream, with its own fixed
-// arena) is idle, forms the next batch from the oldest waiting frames
-// whose stream has a free slot and whose arena still fits the lane's
-// budget, copies them in and launches the chain. With several lanes,
-// frames that arrive while batches run are admitted as soon as a lane and
-// slots free up, and their work overlaps the running batches on the
-// device: continuous batching. The budget covers the lanes' arenas and
-// result buffers and the streams' slots.
+// Each stream has a bounded queue of waiting frames (host memory). A
+// worker thread owns the device: it releases fed arrivals at their times,
+// collects finished batches, and whenever a lane (a kernel chain on its
+// own stream, with its own fixed arena) is idle, forms the next batch from
+// the oldest waiting frames while the lane's arena holds them (and, if
+// the configuration caps them, while a stream has fewer frames in flight
+// than its cap) and launches the chain, which copies each frame into the
+// batch's arena (by DMA from page-locked memory). A frame's device memory
+// is its batch's: frames in flight are bounded by the lanes' arenas only.
+// With several lanes, frames that arrive while batches run are admitted as
+// soon as a lane frees up, and their work overlaps the running batches on
+// the device: continuous batching. The budget covers the lanes' arenas
+// and result buffers.
 //
 // A frame's detections do not depend on the batch it runs in: each frame
 // has its own arena region and every kernel indexes by frame.
@@ -50,7 +52,6 @@ struct Req {
     int64_t stride;
     uint64_t tag;
     int64_t arrival, admit;
-    int slot;
 };
 
 struct Lane {
@@ -62,10 +63,9 @@ struct Lane {
 
 struct StreamS {
     int32_t w, h;
-    size_t frame_bytes;  // arena bytes of one frame
-    std::vector<uint8_t *> slots;
-    std::vector<char> used;
-    int queued = 0;
+    size_t frame_bytes;  // arena bytes of one frame (its pixels included)
+    int queued = 0;      // waiting or in flight
+    int inflight = 0;    // in a launched batch
 };
 
 struct Done {
@@ -86,7 +86,6 @@ struct atg_server {
     std::vector<Lane> lanes;
     atg_ctx *spill = nullptr;  // frames that need more point capacity, one at a time
     bool started = false;
-    size_t slot_bytes = 0;
     // shared with the worker
     std::mutex mu;
     std::condition_variable cv;
@@ -120,7 +119,7 @@ void finish(atg_server *s, const Req &r, int status, int32_t nb, std::vector<at_
     d.rec.batch_frames = nb;
     d.dets = std::move(dets);
     std::lock_guard<std::mutex> g(s->mu);
-    s->streams[(size_t)r.stream].used[(size_t)r.slot] = 0;
+    s->streams[(size_t)r.stream].inflight--;
     s->streams[(size_t)r.stream].queued--;
     s->done.push_back(std::move(d));
     s->completed++;
@@ -139,7 +138,7 @@ void collect(atg_server *s, Lane &L) {
             // the worst-case capacity
             const StreamS &S = s->streams[(size_t)r.stream];
             atg_result res;
-            st = atg_detect(s->spill, &s->params, S.slots[(size_t)r.slot], S.w, S.h, S.w, 1, &res);
+            st = atg_detect(s->spill, &s->params, r.px, S.w, S.h, r.stride, 0, &res);
             std::vector<at_detection> v;
             if (st == ATG_OK) v.assign(res.dets, res.dets + res.ndets);
             finish(s, r, st, nb, std::move(v), st == ATG_OK ? res.nquads : 0);
@@ -157,24 +156,24 @@ void collect(atg_server *s, Lane &L) {
     L.busy = false;
 }
 
-// The next batch for lane L: the oldest waiting frames whose stream has a
-// free slot, while the lane's arena holds them.
+// The next batch for lane L: the oldest waiting frames while the lane's
+// arena holds them (and their stream is under its in-flight cap, if any).
 bool launch(atg_server *s, Lane &L) {
     std::vector<Req> take;
     {
         std::lock_guard<std::mutex> g(s->mu);
         if (s->pending.empty()) return false;
         size_t bytes = 0;
+        const int cap = s->cfg.slots_per_stream;
         for (auto it = s->pending.begin(); it != s->pending.end() && (int32_t)take.size() < s->cfg.max_batch;) {
             StreamS &S = s->streams[(size_t)it->stream];
-            int slot = -1;
-            for (size_t k = 0; k < S.used.size(); k++)
-                if (!S.used[k]) { slot = (int)k; break; }
-            if (slot < 0 || bytes + S.frame_bytes > L.cap) { ++it; continue; }
-            S.used[(size_t)slot] = 1;
+            if ((cap > 0 && S.inflight >= cap) || bytes + S.frame_bytes > L.cap) {
+                ++it;
+                continue;
+            }
+            S.inflight++;
             bytes += S.frame_bytes;
             Req r = *it;
-            r.slot = slot;
             r.admit = now_ns();
             take.push_back(r);
             it = s->pending.erase(it);
@@ -182,18 +181,12 @@ bool launch(atg_server *s, Lane &L) {
         s->inflight += (int64_t)take.size();
     }
     if (take.empty()) return false;
-    hipStream_t st = ctx_stream(L.ctx);
     std::vector<atg_frame> fr(take.size());
-    int rc = ATG_OK;
     for (size_t i = 0; i < take.size(); i++) {
         const StreamS &S = s->streams[(size_t)take[i].stream];
-        uint8_t *dst = S.slots[(size_t)take[i].slot];
-        if (hipMemcpy2DAsync(dst, (size_t)S.w, take[i].px, (size_t)take[i].stride, (size_t)S.w, (size_t)S.h,
-                             hipMemcpyHostToDevice, st) != hipSuccess)
-            rc = ATG_ERR_HIP;
-        fr[i] = atg_frame{dst, S.w, S.h, S.w};
+        fr[i] = atg_frame{take[i].px, S.w, S.h, take[i].stride};
     }
-    if (rc == ATG_OK) rc = chain_enqueue(L.ctx, &s->params, fr.data(), (int32_t)fr.size(), 1);
+    const int rc = chain_enqueue(L.ctx, &s->params, fr.data(), (int32_t)fr.size(), 0);
     L.batch = std::move(take);
     if (rc != ATG_OK) {
         for (const Req &r : L.batch) finish(s, r, rc, 0, {}, 0);
@@ -228,7 +221,7 @@ void release_arrivals(atg_server *s) {
             continue;
         }
         S.queued++;
-        s->pending.push_back(Req{a.stream, a.pixels, a.stride, a.tag, s->feed_t0 + a.at_ns, 0, -1});
+        s->pending.push_back(Req{a.stream, a.pixels, a.stride, a.tag, s->feed_t0 + a.at_ns, 0});
     }
 }
 
@@ -272,13 +265,15 @@ int start(atg_server *s) {
         rc = atg_set_families(L.ctx, s->fams.data(), s->bits.data(), (int32_t)s->fams.size());
         if (rc) return sfail(s, rc, std::string("lane families: ") + atg_last_error(L.ctx));
     }
-    // a frame's arena bytes, by stream (the same for every frame of it)
+    // a frame's arena bytes with its pixels, by stream (the same for every
+    // frame of it; the sizing reads no pixel)
+    static const uint8_t kSizing = 0;
     for (auto &S : s->streams) {
-        atg_frame f{S.slots[0], S.w, S.h, S.w};
-        S.frame_bytes = batch_arena_bytes(s->lanes[0].ctx, &s->params, &f, 1, 1);
+        atg_frame f{&kSizing, S.w, S.h, S.w};
+        S.frame_bytes = batch_arena_bytes(s->lanes[0].ctx, &s->params, &f, 1, 0);
     }
-    size_t fixed = s->slot_bytes + (size_t)s->cfg.lanes * ctx_result_bytes(s->cfg.max_batch);
-    if (fixed >= s->cfg.budget_bytes) return sfail(s, ATG_ERR_LIMIT, "the streams' slots fill the budget");
+    size_t fixed = (size_t)s->cfg.lanes * ctx_result_bytes(s->cfg.max_batch);
+    if (fixed >= s->cfg.budget_bytes) return sfail(s, ATG_ERR_LIMIT, "the lanes' result buffers fill the budget");
     size_t cap = ((s->cfg.budget_bytes - fixed) / (size_t)s->cfg.lanes) & ~(size_t)255;
     for (auto &S : s->streams)
         if (S.frame_bytes > cap) return sfail(s, ATG_ERR_LIMIT, "a stream's frame does not fit a lane's budget");
@@ -302,14 +297,14 @@ extern "C" void atg_server_config_default(atg_server_config *c) {
     c->budget_bytes = (size_t)16 << 30;
     c->lanes = 3;
     c->max_batch = 128;
-    c->slots_per_stream = 2;
+    c->slots_per_stream = 0;  // no cap on a stream's frames in flight
 }
 
 extern "C" int atg_server_create(const atg_server_config *c, const atg_params *p, const at_family *const *fams,
                                  const int32_t *bits, int32_t nfam, atg_server **out) {
     *out = nullptr;
     if (!c || !p || nfam < 1 || nfam > 16 || c->lanes < 1 || c->lanes > kMaxLanes || c->max_batch < 1 ||
-        c->slots_per_stream < 1)
+        c->slots_per_stream < 0)
         return ATG_ERR_ARG;
     atg_server *s = new atg_server();
     s->cfg = *c;
@@ -334,9 +329,6 @@ extern "C" void atg_server_destroy(atg_server *s) {
     }
     for (auto &L : s->lanes) atg_ctx_destroy(L.ctx);
     atg_ctx_destroy(s->spill);
-    hipSetDevice(s->cfg.device);
-    for (auto &S : s->streams)
-        for (auto *p : S.slots) hipFree(p);
     delete s;
 }
 
@@ -351,14 +343,6 @@ extern "C" int atg_server_add_stream(atg_server *s, int32_t width, int32_t heigh
     S.w = width;
     S.h = height;
     S.frame_bytes = 0;  // known once the lanes exist
-    if (hipSetDevice(s->cfg.device) != hipSuccess) return sfail(s, ATG_ERR_HIP, "hipSetDevice");
-    for (int k = 0; k < s->cfg.slots_per_stream; k++) {
-        void *p = nullptr;
-        if (hipMalloc(&p, (size_t)width * (size_t)height) != hipSuccess) return sfail(s, ATG_ERR_HIP, "slot alloc");
-        S.slots.push_back(static_cast<uint8_t *>(p));
-        S.used.push_back(0);
-        s->slot_bytes += ((size_t)width * (size_t)height + 255) & ~(size_t)255;
-    }
     *stream_id = (int32_t)s->streams.size();
     s->streams.push_back(std::move(S));
     return ATG_OK;
@@ -377,7 +361,7 @@ extern "C" int atg_server_submit(atg_server *s, int32_t stream, const uint8_t *p
         return ATG_ERR_LIMIT;
     }
     S.queued++;
-    s->pending.push_back(Req{stream, pixels, stride, tag, now_ns(), 0, -1});
+    s->pending.push_back(Req{stream, pixels, stride, tag, now_ns(), 0});
     s->cv.notify_all();
     return ATG_OK;
 }
@@ -440,7 +424,7 @@ extern "C" int atg_server_stats(atg_server *s, atg_server_counts *out) {
     out->max_batch_seen = s->max_batch_seen;
     out->lanes = (int32_t)s->lanes.size();
     out->lane_arena_bytes = s->lanes.empty() ? 0 : s->lanes[0].cap;
-    out->slot_bytes = s->slot_bytes;
+    out->slot_bytes = 0;  // (frames live in the lanes' arenas)
     return ATG_OK;
 }
 

@@ -201,12 +201,12 @@ int atg_copy_to_device(atg_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* ------------------------------------------------------------------
  * Serving (no counterpart in the reference): frames from many camera
  * streams, admitted as they arrive into batches that overlap on the
- * device. Each stream keeps slots_per_stream input buffers resident in
- * HBM; `lanes` batches run at once, each a kernel chain on its own stream
- * with a fixed arena; frames are admitted while the lane's arena holds
- * them. The budget covers the lanes' arenas and result buffers and the
- * streams' slots. A frame's detections are the ones atg_detect gives for
- * it alone. */
+ * device. `lanes` batches run at once, each a kernel chain on its own
+ * stream with a fixed arena; a lane takes the oldest waiting frames while
+ * its arena holds them, and the chain copies each frame from host memory
+ * into the batch's arena (by DMA from page-locked memory). The budget
+ * covers the lanes' arenas (frames included) and result buffers. A
+ * frame's detections are the ones atg_detect gives for it alone. */
 typedef struct atg_server atg_server;
 
 typedef struct {
@@ -214,7 +214,7 @@ typedef struct {
     size_t budget_bytes;       /* HBM for frames in flight */
     int32_t lanes;             /* batches in flight at once (1..8) */
     int32_t max_batch;         /* frames per batch at most */
-    int32_t slots_per_stream;  /* resident input buffers per stream */
+    int32_t slots_per_stream;  /* a stream's frames in flight at most (0: no cap) */
 } atg_server_config;
 
 /* One completed frame. Times are on the server clock (atg_server_now_ns). */
@@ -241,10 +241,10 @@ typedef struct {
 typedef struct {
     int64_t submitted, dropped, completed, batches, max_batch_seen;
     int32_t lanes;
-    size_t lane_arena_bytes, slot_bytes;
+    size_t lane_arena_bytes, slot_bytes;  /* slot_bytes: 0 (frames live in the lanes' arenas) */
 } atg_server_counts;
 
-void atg_server_config_default(atg_server_config *c);  /* device 0, 16 GiB, 3 lanes, 128, 2 */
+void atg_server_config_default(atg_server_config *c);  /* device 0, 16 GiB, 3 lanes, 128, no cap */
 int atg_server_create(const atg_server_config *c, const atg_params *p, const at_family *const *fams,
                       const int32_t *bits, int32_t nfam, atg_server **out);
 /* Stops the worker; frames not yet arrived are cancelled, admitted ones finish. */
