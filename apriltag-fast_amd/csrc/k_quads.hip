@@ -751,9 +751,12 @@ __device__ __forceinline__ void sort_tiles(const uint64_t *src, uint64_t *dst, u
 // wider steps over the cluster's keys in HBM. The keys are distinct, so
 // any sort gives the full-key order the reference's sort starts from
 // (fast/quads.c:379-427); its tie fix follows in k_fit_moments.
-__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q, int big) {
+template <bool FIX>
+__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q) {
     static_assert(kSortTile == 2048, "4 waves x 512 keys");
+    constexpr bool big = FIX;  // calls of a few frames: the big sort takes the huge clusters, their keys fixed here
     __shared__ uint64_t sk[kSortTile];
+    __shared__ uint32_t fls[FIX ? 4 * 256 + 1 : 1], flm[FIX ? 4 * 256 + 1 : 1], fcnt;
     const uint32_t t = threadIdx.x;
     TakeCursor cur;
     uint32_t fi, c;
@@ -781,6 +784,20 @@ __global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const W
             }
             sort_tiles(keys, keys, n, sk, k);
         }
+        // the tie fix on the sorted keys (in LDS when they fit a tile);
+        // for batches k_fit_moments fixes the large clusters' keys
+        if constexpr (!FIX) {
+        } else if (n <= kSortTile) {
+            for (uint32_t i = t; i < n; i += 256) sk[i] = gmem(keys)[i];
+            __syncthreads();
+            fix_runs_block<256>(sk, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
+                                reinterpret_cast<uint64_t *>(F.filt + s0), fls, flm, &fcnt);
+            for (uint32_t i = t; i < n; i += 256) gmem(keys)[i] = sk[i];
+            __syncthreads();
+        } else {
+            fix_runs_block<256>(keys, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
+                                reinterpret_cast<uint64_t *>(F.filt + s0), fls, flm, &fcnt);
+        }
     }
 }
 
@@ -793,6 +810,7 @@ __global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const W
 __global__ __launch_bounds__(1024) void k_angle_sort_big(const Frame *frames, const WorkQueue *Q) {
     static_assert(kBigSort == 16 * 512, "16 waves x 512 keys");
     extern __shared__ uint64_t skb[];
+    __shared__ uint32_t fls[4 * 1024 + 1], flm[4 * 1024 + 1], fcnt;
     const uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
     const uint32_t total = Q->total[4];
     TakeCursor cur;
@@ -815,20 +833,46 @@ __global__ __launch_bounds__(1024) void k_angle_sort_big(const Frame *frames, co
             for (uint32_t j = k >> 2; j >= 512; j >>= 1) lds_xor_step(v, skb, j);
             reg_half_clean(v);
         }
+        // the tie fix on the sorted keys, in LDS (one segment of 8192)
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
-            if (i < n) gmem(F.skey)[s0 + i] = v[r];
+            skb[i] = v[r];
         }
+        __syncthreads();
+        fix_runs_block<1024>(skb, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
+                             reinterpret_cast<uint64_t *>(F.filt + s0), fls, flm, &fcnt);
+        for (uint32_t i = t; i < n; i += 1024) gmem(F.skey)[s0 + i] = skb[i];
+        __syncthreads();
     }
 }
 
 // The angle sort of list 1 (at most kLargeCluster points), a wave per
 // cluster, persistent.
+template <int R>
+__device__ __forceinline__ void wave_sort_keys_lds(const uint64_t *in, uint64_t *out, uint32_t n) {
+    const int lane = threadIdx.x & 63;
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)(r * 64 + lane);
+        v[r] = i < n ? gmem(in)[i] : ~0ull;
+    }
+    wave_bitonic<R>(v);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)(r * 64 + lane);
+        if (i < n) out[i] = v[r];
+    }
+}
+
 __global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, const WorkQueue *Q) {
     static_assert(kLargeCluster <= 512, "eight keys a lane at most");
+    __shared__ uint64_t kl[kLargeCluster];
+    __shared__ uint32_t fls[4 * 64 + 1], flm[4 * 64 + 1], fcnt;
     const uint32_t total = Q->total[1];
     const uint32_t *off = Q->off[1];
+    const int lane = threadIdx.x;
     uint32_t fi = 0;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         while (off[fi + 1] <= g) fi++;
@@ -836,13 +880,19 @@ __global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, con
         const uint32_t c = F.klist[F.kcap + (g - off[fi])];
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         if (n <= 64)
-            wave_sort_keys<1>(F.okey + s0, F.skey + s0, n);
+            wave_sort_keys_lds<1>(F.okey + s0, kl, n);
         else if (n <= 128)
-            wave_sort_keys<2>(F.okey + s0, F.skey + s0, n);
+            wave_sort_keys_lds<2>(F.okey + s0, kl, n);
         else if (n <= 256)
-            wave_sort_keys<4>(F.okey + s0, F.skey + s0, n);
+            wave_sort_keys_lds<4>(F.okey + s0, kl, n);
         else
-            wave_sort_keys<8>(F.okey + s0, F.skey + s0, n);
+            wave_sort_keys_lds<8>(F.okey + s0, kl, n);
+        __syncthreads();
+        // the tie fix on the sorted keys (fast/quads.c:304-327)
+        fix_runs_block<64>(kl, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
+                           reinterpret_cast<uint64_t *>(F.filt + s0), fls, flm, &fcnt);
+        for (uint32_t i = (uint32_t)lane; i < n; i += 64) gmem(F.skey)[s0 + i] = kl[i];
+        __syncthreads();
     }
 }
 
@@ -966,7 +1016,7 @@ __device__ __forceinline__ void lds_barrier() {
 // other waves form the next chunk's terms.
 template <int NT, int CH>
 __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
-                                                   int keys_ready, int l) {
+                                                   int keys_ready, int l, int fix) {
     constexpr int kBufs = NT > 64 ? 2 : 1;
     // rows 2 doubles apart in the banks: the scan's lanes conflict-free
     __shared__ double sm[kBufs][6][CH + 2];
@@ -990,8 +1040,9 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
             __syncthreads();
         }
         // fix_runs (fast/quads.c:304-327), the keys in LDS (the term
-        // buffers, free until then) when they fit
-        {
+        // buffers, free until then) when they fit, unless the angle sort
+        // fixed them (fix 0)
+        if (fix) {
             constexpr uint32_t kFixLds = kBufs * 6 * CH;
             uint64_t *kl = reinterpret_cast<uint64_t *>(&sm[0][0][0]);
             const bool in_lds = n <= kFixLds;
@@ -1048,6 +1099,108 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
             }
         }
     }
+    }
+}
+
+// The moments of batches, G clusters a wave (persistent over groups of G
+// consecutive entries of list l; their keys sorted and fixed by the angle
+// sort): the six running sums of all G clusters advance together, lane 6 k + q holding sum q of cluster
+// k: for each chunk of C positions the wave forms the terms of every
+// cluster's points there into LDS, the chain lanes run their sums over
+// them in registers (each sum in its cluster's order, as the reference
+// adds them), and the wave copies the partial sums out to the moment
+// arrays. A step of the chains costs one wave instruction for G clusters.
+template <int G, int C>
+__global__ __launch_bounds__(64) void k_fit_moments_packed(const Frame *frames, const WorkQueue *Q, int l) {
+    static_assert(6 * G <= 64 && (G * C) % 64 == 0, "a lane a chain; whole items a lane");
+    __shared__ double tb[G][6][C + 2];  // rows 2 doubles apart: the chain lanes' reads spread over the banks
+    // each cluster of the group: its keys, point words and moment arrays
+    // (at its first point), the arrays' stride, its size
+    __shared__ const uint64_t *gkey[G];
+    __shared__ const uint64_t *gpw[G];
+    __shared__ double *gmom[G];
+    __shared__ int64_t gpc[G];
+    __shared__ uint32_t gn[G];
+    const int lane = threadIdx.x;
+    const uint32_t total = Q->total[l] + (l == 0 ? Q->total[4] : 0u);
+    TakeCursor cur;  // this lane's (entries g0 + lane, lane < G)
+    for (uint32_t g0 = blockIdx.x * G; g0 < total; g0 += gridDim.x * G) {
+        if (lane < G) {
+            uint32_t fi = 0, c = 0;
+            const bool ok = g0 + lane < total && cluster_at(frames, Q, l, g0 + lane, cur, &fi, &c);
+            const Frame &F = frames[ok ? fi : 0];
+            const uint32_t s0 = ok ? F.cstart[c] : 0u;
+            gn[lane] = ok ? F.cstart[c + 1] - s0 : 0u;
+            gkey[lane] = F.skey + s0;
+            gpw[lane] = F.pw + s0;
+            gmom[lane] = F.mom + s0;
+            gpc[lane] = F.pcap;
+        }
+        __syncthreads();
+        uint32_t maxlen = 0;
+        for (int k = 0; k < G; k++) maxlen = max(maxlen, gn[k]);
+        // the packed running sums
+        const int ck = lane / 6, cq = lane % 6;
+        const bool chain = lane < 6 * G;
+        double run = 0.0;
+        for (uint32_t c0 = 0; c0 < maxlen; c0 += C) {
+            // terms of (cluster k, position c0 + j), item k C + j, loads first
+            constexpr int kItems = G * C / 64;
+            uint64_t pwv[kItems];
+            uint32_t ix[kItems];
+#pragma unroll
+            for (int r = 0; r < kItems; r++) {
+                const int it = lane + 64 * r, k = it / C, j = it % C;
+                ix[r] = c0 + (uint32_t)j < gn[k] ? (uint32_t)~(uint32_t)gmem(gkey[k])[c0 + j] : 0xffffffffu;
+            }
+#pragma unroll
+            for (int r = 0; r < kItems; r++) {
+                const int it = lane + 64 * r, k = it / C;
+                pwv[r] = ix[r] != 0xffffffffu ? gmem(gpw[k])[ix[r]] : 0ull;
+            }
+#pragma unroll
+            for (int r = 0; r < kItems; r++) {
+                const int it = lane + 64 * r, k = it / C, j = it % C;
+                if (ix[r] == 0xffffffffu) continue;
+                const int px = (int)(pwv[r] & 0xffffu), py = (int)((pwv[r] >> 16) & 0xffffu);
+                const uint32_t g2 = (uint32_t)(pwv[r] >> 32);
+                const double wgt = __dadd_rn(__dsqrt_rn((double)g2), 1.0);
+                const double fx = __dadd_rn(__dmul_rn((double)px, 0.5), 0.5);
+                const double fy = __dadd_rn(__dmul_rn((double)py, 0.5), 0.5);
+                const double wfx = __dmul_rn(wgt, fx), wfy = __dmul_rn(wgt, fy);
+                tb[k][0][j] = wfx;
+                tb[k][1][j] = wfy;
+                tb[k][2][j] = __dmul_rn(wfx, fx);
+                tb[k][3][j] = __dmul_rn(wfx, fy);
+                tb[k][4][j] = __dmul_rn(wfy, fy);
+                tb[k][5][j] = wgt;
+            }
+            __syncthreads();
+            // the chains: each lane its sum over its cluster's positions here
+            if (chain) {
+                const uint32_t n = gn[ck];
+                const uint32_t len = c0 < n ? min((uint32_t)C, n - c0) : 0u;
+                double *a = tb[ck][cq];
+                double v[C];
+#pragma unroll
+                for (int j = 0; j < C; j++) v[j] = a[j];
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    if ((uint32_t)j < len) {
+                        run = __dadd_rn(run, v[j]);
+                        v[j] = run;
+                    }
+#pragma unroll
+                for (int j = 0; j < C; j++) a[j] = v[j];
+            }
+            __syncthreads();
+            // out to the moment arrays, consecutive positions on consecutive lanes
+            for (int it = lane; it < G * 6 * C; it += 64) {
+                const int k = it / (6 * C), q = (it / C) % 6, j = it % C;
+                if (c0 + (uint32_t)j < gn[k]) gmem(gmom[k])[(int64_t)q * gpc[k] + c0 + j] = tb[k][q][j];
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -1689,7 +1842,8 @@ static unsigned pgrid(K *fn, int threads, int want) {
 }
 
 void warm_persistent_grids(int fit_blocks) {
-    pgrid(k_angle_sort, 256, fit_blocks);
+    pgrid(k_angle_sort<true>, 256, fit_blocks);
+    pgrid(k_angle_sort<false>, 256, fit_blocks);
     {
         static bool big_lds = false;  // the big sort's LDS: its keys across waves
         if (!big_lds)
@@ -1702,6 +1856,7 @@ void warm_persistent_grids(int fit_blocks) {
     pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks);
     pgrid(k_fit_moments<256, 512>, 256, fit_blocks);
     pgrid(k_fit_moments<64, 128>, 64, fit_blocks);
+    pgrid(k_fit_moments_packed<10, 32>, 64, fit_blocks);
     pgrid(k_fit_corners<256>, 256, fit_blocks);
     pgrid(k_fit_corners<64>, 64, fit_blocks);
     warm_decode_grid(fit_blocks);
@@ -1741,7 +1896,12 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         // calls of a few frames: the clusters above kHugeCluster points
         // (a few dozen a frame) each by a 1024-thread block
         const int big = nframes <= 16 ? 1 : 0;
-        hipLaunchKernelGGL(k_angle_sort, dim3(pgrid(k_angle_sort, 256, fit_blocks)), dim3(256), 0, st, dframes, Q, big);
+        if (big)
+            hipLaunchKernelGGL(k_angle_sort<true>, dim3(pgrid(k_angle_sort<true>, 256, fit_blocks)), dim3(256), 0, st,
+                               dframes, Q);
+        else
+            hipLaunchKernelGGL(k_angle_sort<false>, dim3(pgrid(k_angle_sort<false>, 256, fit_blocks)), dim3(256), 0,
+                               st, dframes, Q);
         if (big)
             hipLaunchKernelGGL(k_angle_sort_big,
                                dim3(persistent_grid(reinterpret_cast<const void *>(&k_angle_sort_big), 1024,
@@ -1758,13 +1918,17 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
     // sums in flight), 256 for up to 16 frames (each cluster sooner)
     if (nframes > 16)
         hipLaunchKernelGGL((k_fit_moments<128, 256>), dim3(pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks)),
-                           dim3(128), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+                           dim3(128), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0, 1);
     else
         hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256),
-                           0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0);
+                           0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0, sort_cap ? 0 : 1);
     mk->mark(st, "moments.large");
-    hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, st,
-                       dframes, parity, Q, sort_cap ? 1 : 0, 1);
+    if (nframes > 16 && sort_cap)  // batches: ten small clusters' sums a wave
+        hipLaunchKernelGGL((k_fit_moments_packed<10, 32>), dim3(pgrid(k_fit_moments_packed<10, 32>, 64, wf)),
+                           dim3(64), 0, st, dframes, Q, 1);
+    else
+        hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, st,
+                       dframes, parity, Q, sort_cap ? 1 : 0, 1, sort_cap ? 0 : 1);
     mk->mark(st, "moments.small");
     hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st, dframes,
                        dparams, Q, 0);
