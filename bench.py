@@ -433,6 +433,7 @@ def main():
     ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (config 4) measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the 640x480 and 3840x2160 comparisons")
     ap.add_argument("--no-host-input", action="store_true", help="skip the batches from pinned host memory")
+    ap.add_argument("--streams", type=int, default=3, help="contexts (streams) the batched steps are spread over")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-loads", default="500,1000,2000,4000,8000,20000")
     args = ap.parse_args()
@@ -467,6 +468,8 @@ def main():
     # batched path: the headline
     log("timing the batched path")
     batch = [dev[i % len(dev)] for i in range(args.batch)]
+    # one context, calls back to back: each call's host work (planning,
+    # launch, results) between the device's chains
     for _ in range(args.warmup):
         ctx.detect_batch(batch, prm, on_device=True)
     step_t = []
@@ -475,9 +478,33 @@ def main():
         t = time.perf_counter()
         ctx.detect_batch(batch, prm, on_device=True)
         step_t.append(time.perf_counter() - t)
-    wall = time.perf_counter() - t0
-    batched_fps = args.batch * args.steps / wall
+    wall1 = time.perf_counter() - t0
     chain_ms = ctx.chain_ms()
+    # the headline: the same K steps spread over `streams` contexts, each a
+    # chain on its own stream driven by its own host thread, so that one
+    # call's host work and its low-occupancy phases overlap another's chain
+    extra = [atg.Context(0) for _ in range(args.streams - 1)]
+    lanes = [ctx] + extra
+    for c in extra:
+        c.set_families(fams)
+        for _ in range(args.warmup):
+            c.detect_batch(batch, prm, on_device=True)
+    share = [args.steps // len(lanes) + (1 if i < args.steps % len(lanes) else 0) for i in range(len(lanes))]
+
+    def lane_run(c, k):
+        for _ in range(k):
+            c.detect_batch(batch, prm, on_device=True)
+
+    threads = [threading.Thread(target=lane_run, args=(lanes[i], share[i])) for i in range(len(lanes))]
+    t0 = time.perf_counter()
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    wall = time.perf_counter() - t0
+    for c in extra:
+        c.close()
+    batched_fps = args.batch * args.steps / wall
 
     # per-kernel device times of the batched call, and the bytes they move
     log("per-kernel device times")
@@ -591,6 +618,9 @@ def main():
         "batched_over_single": round(batched_fps / c2["gpu_fps"], 2),
         "latency_ms_single": {"p50": c2["gpu_p50_ms"], "p99": c2["gpu_p99_ms"]},
         "latency_ms_batch": {"p50": round(1000 * pctl(step_t, 50), 3), "p99": round(1000 * pctl(step_t, 99), 3)},
+        "batched_streams": args.streams,
+        "batched_one_stream": {"fps": round(args.batch * args.steps / wall1, 2),
+                               "ms_per_step": round(1000.0 * wall1 / args.steps, 3)},
         "bit_exact": bool(bit_exact),
         "detections_per_10_frames": int(ndets),
         "device_chain_ms": round(chain_ms, 3),
