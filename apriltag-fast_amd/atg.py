@@ -104,6 +104,14 @@ class ServerCounts(C.Structure):
 DET_DTYPE = np.dtype([("family", "<i4"), ("id", "<i4"), ("hamming", "<i4"), ("margin", "<f4"),
                       ("H", "<f8", 9), ("c", "<f8", 2), ("p", "<f8", 8)])
 
+# atg_pose_in / atg_pose_out (include/atg.h)
+POSE_IN_DTYPE = np.dtype([("H", "<f8", 9), ("p", "<f8", 8), ("tagsize", "<f8"), ("fx", "<f8"), ("fy", "<f8"),
+                          ("cx", "<f8"), ("cy", "<f8")])
+POSE_OUT_DTYPE = np.dtype([("hpose", "<f8", 12), ("R0", "<f8", 9), ("t0", "<f8", 3), ("R1", "<f8", 9),
+                           ("t1", "<f8", 3), ("err1", "<f8"), ("R2", "<f8", 9), ("t2", "<f8", 3), ("err2", "<f8"),
+                           ("has2", "<i4"), ("best", "<i4")])
+POSE_HOMOGRAPHY_ONLY = 1
+
 _lib = None
 
 
@@ -192,6 +200,8 @@ def lib():
     L.atg_host_alloc.restype = C.c_int
     L.atg_host_free.argtypes = [vp]
     L.atg_host_free.restype = C.c_int
+    L.atg_estimate_poses.argtypes = [vp, vp, i64, i32, i32, vp]
+    L.atg_estimate_poses.restype = C.c_int
     _lib = L
     return L
 
@@ -360,6 +370,15 @@ class Context:
     def force_scratch(self, on=True):
         """Keep the union-find replay state in HBM scratch (the wide-frame path)."""
         self._check(self.L.atg_debug_force_scratch(self.h, 1 if on else 0), "atg_debug_force_scratch")
+
+    def estimate_poses(self, tags, n_iters=50, flags=0):
+        """Poses of many tags in one launch: tags is a POSE_IN_DTYPE array;
+        returns a POSE_OUT_DTYPE array."""
+        t = np.ascontiguousarray(tags, POSE_IN_DTYPE)
+        out = np.zeros(len(t), POSE_OUT_DTYPE)
+        self._check(self.L.atg_estimate_poses(self.h, t.ctypes.data if len(t) else None, len(t), n_iters, flags,
+                                              out.ctypes.data if len(t) else None), "atg_estimate_poses")
+        return out
 
     def stage_times(self):
         """[(group.kernel, ms)] of the last call with profiling on."""

@@ -35,6 +35,7 @@ int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          bool refine, WorkQueue *Q, Marks *mk);
 void launch_debug_finish(hipStream_t st, uint8_t *buf, uint32_t n, int64_t *nout);
+hipError_t launch_pose(hipStream_t st, const atg_pose_in *in, int64_t n, int iters, int flags, atg_pose_out *out);
 void prepare_points_stage();
 }  // namespace atg
 
@@ -170,6 +171,9 @@ struct atg_ctx {
     // for its descriptors and results (both set up at creation)
     uint8_t *scratch = nullptr;
     uint8_t *na_stage = nullptr;
+    // atg_estimate_poses: device copies of the tags in and the poses out
+    uint8_t *pose_buf = nullptr;
+    size_t pose_cap = 0;
 };
 
 namespace {
@@ -531,6 +535,7 @@ extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
     hipHostFree(ctx->na_stage);
     hipHostFree(ctx->pinned);
     hipHostFree(ctx->pinned_out);
+    hipFree(ctx->pose_buf);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->ev2) hipEventDestroy(ctx->ev2);
@@ -1291,6 +1296,33 @@ extern "C" int atg_debug_finish(atg_ctx *ctx, at_detection *dets, int64_t *n) {
     }();
     hipFree(buf);
     return rc;
+}
+
+extern "C" int atg_estimate_poses(atg_ctx *ctx, const atg_pose_in *in, int64_t n, int32_t n_iters, int32_t flags,
+                                  atg_pose_out *out) {
+    if (!ctx) return ATG_ERR_ARG;
+    if (n < 0 || n > ((int64_t)1 << 24) || (n > 0 && (!in || !out)))
+        return fail(ctx, ATG_ERR_ARG, "atg_estimate_poses: 0..2^24 tags, non-null arrays");
+    if (flags & ~ATG_POSE_HOMOGRAPHY_ONLY) return fail(ctx, ATG_ERR_ARG, "atg_estimate_poses: unknown flags");
+    if (n == 0) return ATG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bin = sizeof(atg_pose_in) * (size_t)n, bout = sizeof(atg_pose_out) * (size_t)n;
+    const size_t need = align256(bin) + bout;
+    if (need > ctx->pose_cap) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->pose_buf) HIPCHK(hipFree(ctx->pose_buf));
+        ctx->pose_buf = nullptr;
+        ctx->pose_cap = 0;
+        HIPCHK(hipMalloc(&ctx->pose_buf, need));
+        ctx->pose_cap = need;
+    }
+    auto *din = reinterpret_cast<atg_pose_in *>(ctx->pose_buf);
+    auto *dout = reinterpret_cast<atg_pose_out *>(ctx->pose_buf + align256(bin));
+    HIPCHK(hipMemcpyAsync(din, in, bin, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_pose(ctx->stream, din, n, n_iters, flags, dout));
+    HIPCHK(hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ATG_OK;
 }
 
 extern "C" int atg_debug_force_scratch(atg_ctx *ctx, int on) {

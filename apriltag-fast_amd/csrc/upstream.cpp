@@ -338,6 +338,7 @@ ATG_FAMILY(tagStandard52h13, at_tagStandard52h13)
 // ---------------------------------------------------------------- matd_t
 
 matd_t *matd_create(int rows, int cols) {
+    if (rows == 0 || cols == 0) return matd_create_scalar(0.0);
     matd_t *m = (matd_t *)calloc(1, sizeof(matd_t));
     m->nrows = (unsigned)rows;
     m->ncols = (unsigned)cols;

@@ -192,6 +192,51 @@ int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8_t *pixels,
                        int64_t stride, int on_device, at_detection *dets, int64_t cap, int64_t *ndets,
                        int64_t *nquads);
 
+/* ------------------------------------------------------------------
+ * Tag pose (apriltag_pose.h:1-77 in the reference, the consumer of the
+ * detections): many tags in one launch, a device thread each.
+ *
+ * One tag: its detection's homography and corners (apriltag_detection_t
+ * H and p, apriltag.h:215-250) and the camera (apriltag_detection_info_t,
+ * apriltag_pose.h:11-18). */
+typedef struct {
+    double H[9];          /* row-major */
+    double p[4][2];       /* corners, as the detection's */
+    double tagsize;       /* metres */
+    double fx, fy, cx, cy;
+} atg_pose_in;
+
+/* The poses of one tag (R row-major 3x3, t 3x1):
+ *   hpose   homography_to_pose(H, -fx, fy, cx, cy) (common/homography.c:
+ *           275-351), the top three rows of its 4x4 [R | T]
+ *   R0, t0  estimate_pose_for_tag_homography (apriltag_pose.c:455-488)
+ *   R1..err2  estimate_tag_pose_orthogonal_iteration (apriltag_pose.c:
+ *           490-526): the refined homography pose and its object-space
+ *           error, and the second local minimum (has2 = 0 and err2 =
+ *           HUGE_VAL when there is none)
+ *   best    estimate_tag_pose's choice (apriltag_pose.c:528-547): 1 when
+ *           err1 <= err2, else 2 */
+typedef struct {
+    double hpose[12];
+    double R0[9], t0[3];
+    double R1[9], t1[3], err1;
+    double R2[9], t2[3], err2;
+    int32_t has2, best;
+} atg_pose_out;
+
+/* flags of atg_estimate_poses */
+#define ATG_POSE_HOMOGRAPHY_ONLY 1  /* hpose, R0 and t0 only */
+
+/* No counterpart as one call in the reference (which estimates one tag a
+ * call, apriltag_pose.c:490-547): the poses of n tags, host arrays in and
+ * out, n_iters orthogonal-iteration steps (the reference's estimate_tag_pose
+ * uses 50). Floating point follows the reference's formulas; its SVD
+ * polar factors and 3x3 inverses are formed by other, equally accurate
+ * methods, so results agree to rounding (tests bound them by 1e-9
+ * relative), not bit for bit. */
+int atg_estimate_poses(atg_ctx *ctx, const atg_pose_in *in, int64_t n, int32_t n_iters, int32_t flags,
+                       atg_pose_out *out);
+
 /* Device buffers on the context's device, for callers without their own
  * allocator (frames kept resident in HBM between calls). */
 int atg_device_alloc(atg_ctx *ctx, size_t bytes, void **out);

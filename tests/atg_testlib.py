@@ -16,6 +16,7 @@ REPO = Path(__file__).resolve().parent.parent
 PKG = REPO / "apriltag-fast_amd"
 ORACLE_SO = REPO / "oracle" / "_build" / "liboracle.so"
 REF_SO = REPO / "oracle" / "_ref" / "libatref.so"
+POSE_REF_SO = REPO / "oracle" / "_ref" / "libposeref.so"
 GOLDEN = REPO / "tests" / "golden"
 
 
@@ -369,6 +370,82 @@ def _reconcile_with(fn, dets):
 
 def have_ref():
     return REF_SO.exists()
+
+
+# --------------------------------------------------------------- pose
+
+# atg_pose_in / atg_pose_out (include/atg.h); the reference harness
+# (oracle/pose_harness.c) fills the same output layout
+POSE_IN_DTYPE = np.dtype([("H", "<f8", 9), ("p", "<f8", 8), ("tagsize", "<f8"), ("fx", "<f8"), ("fy", "<f8"),
+                          ("cx", "<f8"), ("cy", "<f8")])
+POSE_OUT_DTYPE = np.dtype([("hpose", "<f8", 12), ("R0", "<f8", 9), ("t0", "<f8", 3), ("R1", "<f8", 9),
+                           ("t1", "<f8", 3), ("err1", "<f8"), ("R2", "<f8", 9), ("t2", "<f8", 3), ("err2", "<f8"),
+                           ("has2", "<i4"), ("best", "<i4")])
+
+
+class RefPose:
+    """The reference's pose estimation built from its own sources
+    (oracle/_ref/libposeref.so)."""
+
+    def __init__(self):
+        self.lib = C.CDLL(str(POSE_REF_SO))
+        dp = C.POINTER(C.c_double)
+        d = C.c_double
+        self.lib.ref_pose.argtypes = [dp, dp, d, d, d, d, d, C.c_int, C.c_void_p]
+        self.lib.ref_homography_to_pose.argtypes = [dp, d, d, d, d, dp]
+        self.lib.ref_estimate_tag_pose.restype = d
+        self.lib.ref_estimate_tag_pose.argtypes = [dp, dp, d, d, d, d, d, dp, dp]
+
+    def poses(self, tags, iters=50):
+        tags = np.ascontiguousarray(tags, POSE_IN_DTYPE)
+        out = np.zeros(len(tags), POSE_OUT_DTYPE)
+        dp = C.POINTER(C.c_double)
+        for i, t in enumerate(tags):
+            H = np.ascontiguousarray(t["H"])
+            p = np.ascontiguousarray(t["p"])
+            self.lib.ref_pose(H.ctypes.data_as(dp), p.ctypes.data_as(dp), t["tagsize"], t["fx"], t["fy"], t["cx"],
+                              t["cy"], int(iters), out[i:i + 1].ctypes.data)
+        return out
+
+    def homography_to_pose(self, H, fx, fy, cx, cy):
+        H = np.ascontiguousarray(H, np.float64).ravel()
+        m = np.zeros(16)
+        dp = C.POINTER(C.c_double)
+        self.lib.ref_homography_to_pose(H.ctypes.data_as(dp), fx, fy, cx, cy, m.ctypes.data_as(dp))
+        return m.reshape(4, 4)
+
+    def estimate_tag_pose(self, tag):
+        R, t = np.zeros(9), np.zeros(3)
+        dp = C.POINTER(C.c_double)
+        H = np.ascontiguousarray(tag["H"])
+        p = np.ascontiguousarray(tag["p"])
+        e = self.lib.ref_estimate_tag_pose(H.ctypes.data_as(dp), p.ctypes.data_as(dp), tag["tagsize"], tag["fx"],
+                                           tag["fy"], tag["cx"], tag["cy"], R.ctypes.data_as(dp), t.ctypes.data_as(dp))
+        return e, R.reshape(3, 3), t
+
+
+def rotation(axis, angle):
+    """Rodrigues' rotation about axis by angle."""
+    a = np.asarray(axis, float)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+def tag_seen(R, t, tagsize, fx, fy, cx, cy):
+    """A tag at pose (R, t) seen by the pinhole camera (fx, fy, cx, cy): the
+    detection's homography (tag coordinates in [-1, 1] to pixels, as
+    apriltag_detection_t.H) and its corners in the order the pose code
+    pairs with the object points (-s, s), (s, s), (s, -s), (-s, -s)."""
+    s = tagsize / 2
+    K = np.array([[fx, 0, cx], [0, fy, cy], [0, 0, 1.0]])
+    H = K @ np.column_stack([s * R[:, 0], s * R[:, 1], t])
+    H = H / H[2, 2]
+    p = []
+    for x, y in [(-1, 1), (1, 1), (1, -1), (-1, -1)]:
+        q = H @ [x, y, 1.0]
+        p += [q[0] / q[2], q[1] / q[2]]
+    return H.ravel(), np.array(p)
 
 
 def same_dets(a, b):

@@ -11,6 +11,8 @@
 #include <string.h>
 
 #include "apriltag.h"
+#include "apriltag_pose.h"
+#include "common/homography.h"
 #include "tag16h5.h"
 #include "tag25h9.h"
 #include "tag36h10.h"
@@ -42,9 +44,10 @@ int main(int argc, char **argv) {
         const char *a = argv[first];
         if (!strcmp(a, "-s")) {
             printf("apriltag_detector_t %zu\napriltag_detection_t %zu\napriltag_family_t %zu\nimage_u8_t %zu\n"
-                   "zarray_t %zu\nmatd_t %zu\n",
+                   "zarray_t %zu\nmatd_t %zu\napriltag_detection_info_t %zu\napriltag_pose_t %zu\n",
                    sizeof(apriltag_detector_t), sizeof(apriltag_detection_t), sizeof(apriltag_family_t),
-                   sizeof(image_u8_t), sizeof(zarray_t), sizeof(matd_t));
+                   sizeof(image_u8_t), sizeof(zarray_t), sizeof(matd_t), sizeof(apriltag_detection_info_t),
+                   sizeof(apriltag_pose_t));
             return 0;
         } else if (!strcmp(a, "-0")) {
             refine = 0;

@@ -11,14 +11,16 @@ import pytest
 from atg_testlib import PKG, REPO, frame_hash, load_atg, load_families
 
 HEADERS = [REPO / "include" / "apriltag_fast.h", REPO / "include" / "atg.h", REPO / "include" / "apriltag.h",
-           REPO / "include" / "common" / "matd.h", REPO / "include" / "common" / "image_u8.h"] + \
+           REPO / "include" / "common" / "matd.h", REPO / "include" / "common" / "image_u8.h",
+           REPO / "include" / "apriltag_pose.h", REPO / "include" / "common" / "homography.h"] + \
     sorted((REPO / "include").glob("tag*.h"))
 
 # sizeof of the upstream API's public structs in the reference (its
 # apriltag.h and common/ headers, x86-64 gcc): code built against them
 # must see the same layouts here.
 UPSTREAM_SIZES = {"apriltag_detector_t": 160, "apriltag_detection_t": 112, "apriltag_family_t": 72,
-                  "image_u8_t": 24, "zarray_t": 24, "matd_t": 16}
+                  "image_u8_t": 24, "zarray_t": 24, "matd_t": 16, "apriltag_detection_info_t": 48,
+                  "apriltag_pose_t": 16}
 
 
 def declared_functions():
