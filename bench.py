@@ -15,8 +15,9 @@ times (HIP events on the chain's stream), the roofline of the dominant
 kernel on the algorithmic bytes of its stage, the reference CPU detector
 timed on the host (1 thread and all cores, median of 3 runs, alternating
 with the GPU), the 640x480 and 3840x2160 configurations one frame at a
-time against the CPU, batches from host memory, the uneven 64-frame batch
-and the serving sweep.
+time against the CPU, batches from host memory, the uneven 64-frame batch,
+the serving sweep, and the pose of the batch's detections (GPU against the
+reference's estimate_tag_pose on one host thread).
 """
 import argparse
 import json
@@ -405,6 +406,46 @@ def host_input_batches(atg, frames, fams, batch, calls):
                 frame_bytes=int(frames[0].nbytes))
 
 
+def pose_leg(atg, ctx, dets_per_frame, W, H, batch, cpu_seconds):
+    """The pose of every detection of a batched call (the workload's
+    detections repeated to the batch): the GPU (atg_estimate_poses, one
+    launch, host arrays in and out) against the reference's own
+    estimate_tag_pose (oracle/_ref/libposeref.so) on this host, one
+    thread, over a bounded sample; the errors of the sample compared."""
+    import ctypes as C
+    from atg_testlib import POSE_IN_DTYPE, POSE_REF_SO
+    rows = [d for k in range(batch) for d in dets_per_frame[k % len(dets_per_frame)]]
+    tags = np.zeros(len(rows), POSE_IN_DTYPE)
+    for i, d in enumerate(rows):
+        tags[i] = (d["H"], d["p"], 0.1, 1400.0, 1400.0, W / 2.0, H / 2.0)
+    ctx.estimate_poses(tags[:64])
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        got = ctx.estimate_poses(tags)
+    gpu_s = (time.perf_counter() - t0) / reps
+    res = dict(tags=len(tags), gpu_ms_per_call=round(1e3 * gpu_s, 3), gpu_tags_per_s=round(len(tags) / gpu_s, 1),
+               n_iters=50, camera="f 1400 px, centre at the frame centre, tag 0.1 m")
+    if POSE_REF_SO.exists():
+        L = C.CDLL(str(POSE_REF_SO))
+        L.ref_estimate_many.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        n, done, t0 = 0, [], time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds and n < len(tags):
+            k = min(64, len(tags) - n)
+            part = np.ascontiguousarray(tags[n:n + k])
+            e = np.zeros(k)
+            L.ref_estimate_many(part.ctypes.data, k, e.ctypes.data)
+            done.append(e)
+            n += k
+        cpu_s = time.perf_counter() - t0
+        e_ref = np.concatenate(done)
+        e_gpu = np.where(got["best"][:n] == 1, got["err1"][:n], got["err2"][:n])
+        res.update(cpu_tags_per_s_1thread=round(n / cpu_s, 1), cpu_sample_tags=n,
+                   cpu_kind="reference", gpu_over_cpu=round(len(tags) / gpu_s / (n / cpu_s), 1),
+                   errors_within_1e9=bool((np.abs(e_gpu - e_ref) <= 1e-9 * np.abs(e_ref) + 1e-18).all()))
+    return res
+
+
 def kernel_profile(ctx, batch, prm, calls=5):
     """Device time of each marked kernel of a batched call: HIP events
     between the kernels on the chain's stream (the calls without a graph),
@@ -433,6 +474,7 @@ def main():
     ap.add_argument("--no-batching", action="store_true", help="skip the uneven-batch (config 4) measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the 640x480 and 3840x2160 comparisons")
     ap.add_argument("--no-host-input", action="store_true", help="skip the batches from pinned host memory")
+    ap.add_argument("--no-pose", action="store_true", help="skip the pose of the batch's detections")
     ap.add_argument("--streams", type=int, default=3, help="contexts (streams) the batched steps are spread over")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-loads", default="500,1000,2000,4000,8000,20000")
@@ -584,6 +626,10 @@ def main():
         log("config 4: uneven batch")
         batching = batching_config4(atg, ctx, max(2, args.steps // 4))
         ctx.set_families(fams)
+    pose = None
+    if not args.no_pose:
+        log("pose of the batch's detections")
+        pose = pose_leg(atg, ctx, [w["dets"] for w in want], W, H, args.batch, 0.0 if args.no_cpu else 3.0)
     # a plain copy kernel of known size: the reference for the traffic
     # counters and for what HBM delivers to a simple kernel here
     copy_gbs, _ = ctx.copy_bandwidth(1 << 29)
@@ -634,6 +680,7 @@ def main():
         "batched_host_input": host_in,
         "batching_uneven": batching,
         "serving": serving,
+        "pose": pose,
         "cpu_baseline": cpu_line,
     }
     for d in dev:

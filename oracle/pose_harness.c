@@ -101,3 +101,14 @@ double ref_estimate_tag_pose(const double *H, const double *corners, double tags
     matd_destroy(det.H);
     return e;
 }
+
+/* estimate_tag_pose over n tags in the atg_pose_in layout (H[9], p[8],
+ * tagsize, fx, fy, cx, cy: 22 doubles a tag), each error into err[i]: the
+ * reference CPU side of the pose timing. */
+void ref_estimate_many(const double *tags, int64_t n, double *err) {
+    for (int64_t i = 0; i < n; i++) {
+        const double *t = tags + 22 * i;
+        double R[9], tt[3];
+        err[i] = ref_estimate_tag_pose(t, t + 9, t[17], t[18], t[19], t[20], t[21], R, tt);
+    }
+}
