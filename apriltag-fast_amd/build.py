@@ -64,6 +64,14 @@ def compile_one(src, force):
     return obj, None
 
 
+def link(objs, out, soname):
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,-soname," + soname, "-o", str(out) + ".tmp"]
+    r = subprocess.run(cmd + [str(o) for o in objs], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n%s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(str(out) + ".tmp", out)
+
+
 def build(force=False, verbose=True):
     BUILD.mkdir(exist_ok=True)
     srcs = sources()
@@ -73,11 +81,7 @@ def build(force=False, verbose=True):
     if errs:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
     objs = [str(o) for o, _ in res]
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,-soname," + SONAME, "-o", str(OUT) + ".tmp"] + objs
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("link failed:\n%s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
-    os.replace(str(OUT) + ".tmp", OUT)
+    link(objs, OUT, OUT.name)
     if verbose:
         print("built", OUT)
     return OUT
@@ -128,18 +132,19 @@ endif()
 def install(prefix):
     """Install the built library under prefix (see the module docstring)."""
     prefix = Path(prefix).resolve()
-    if not OUT.exists():
-        build()
+    build()
     lib, inc = prefix / "lib", prefix / "include" / "apriltag"
     for d in (lib / "pkgconfig", lib / "apriltag" / "cmake", inc / "common", inc / "fast"):
         d.mkdir(parents=True, exist_ok=True)
     real = lib / ("libapriltag.so." + VERSION)
-    shutil.copy2(OUT, real)
+    # the same objects linked again under the upstream library's SONAME
+    # (the in-tree library keeps its own name)
+    link(sorted(BUILD.glob("*.o")), real, SONAME)
     for name, target in ((SONAME, real.name), ("libapriltag.so", SONAME)):
-        link = lib / name
-        if link.is_symlink() or link.exists():
-            link.unlink()
-        link.symlink_to(target)
+        ln = lib / name
+        if ln.is_symlink() or ln.exists():
+            ln.unlink()
+        ln.symlink_to(target)
     src = REPO / "include"
     for h in sorted(src.glob("*.h")):
         shutil.copy2(h, inc / h.name)

@@ -259,6 +259,30 @@ void warm_persistent_grids(int fit_blocks);
 // launch. Level 2: HIP events (the call then runs without a graph).
 constexpr int kMaxMarks = 48;
 void launch_stamp(hipStream_t st, uint64_t *slot);
+// Streams for chain work that is independent of the main stream's:
+// begin() forks kForks streams off the main stream, end() joins them
+// back. Under stream capture they become parallel branches of the graph.
+struct Fork {
+    static constexpr int kForks = 2;
+    hipStream_t s[kForks] = {};
+    hipEvent_t go = nullptr, done[kForks] = {};
+    bool on = false;
+    // the streams to use: the forked ones, or the main one when off
+    void begin(hipStream_t st, hipStream_t (&out)[kForks]) {
+        for (int k = 0; k < kForks; k++) out[k] = on ? s[k] : st;
+        if (!on) return;
+        hipEventRecord(go, st);
+        for (int k = 0; k < kForks; k++) hipStreamWaitEvent(s[k], go, 0);
+    }
+    void end(hipStream_t st) {
+        if (!on) return;
+        for (int k = 0; k < kForks; k++) {
+            hipEventRecord(done[k], s[k]);
+            hipStreamWaitEvent(st, done[k], 0);
+        }
+    }
+};
+
 struct Marks {
     uint64_t *stamps = nullptr;  // device [kMaxMarks]
     hipEvent_t ev[kMaxMarks] = {};

@@ -577,15 +577,16 @@ __global__ __launch_bounds__(256) void k_keep_lists(const Frame *frames) {
 }
 
 // The cluster at global index g of the kernels of one cluster a block
-// (list 0: the huge clusters of every frame, list 4, then those of list 0;
-// list 1: the small ones): frame fi and cluster c, false past the end.
-// The cursors follow g, which only grows.
+// (l = 0: the huge clusters of every frame, list 4, then those of list 0;
+// l = 4: the huge ones alone; l = 5: list 0 alone; l = 1: the small ones):
+// frame fi and cluster c, false past the end. The cursors follow g, which
+// only grows.
 struct TakeCursor {
     uint32_t fh = 0, fl = 0;
 };
 __device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue *Q, int l, uint32_t g,
                                            TakeCursor &cur, uint32_t *fi, uint32_t *c) {
-    if (l == 0) {
+    if (l == 0 || l == 4) {
         const uint32_t th = Q->total[4];
         if (g < th) {
             const uint32_t *off = Q->off[4];
@@ -594,8 +595,10 @@ __device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue 
             *c = frames[cur.fh].klist[g - off[cur.fh]];
             return true;
         }
+        if (l == 4) return false;
         g -= th;
     }
+    if (l == 5) l = 0;
     if (g >= Q->total[l]) return false;
     const uint32_t *off = Q->off[l];
     while (off[cur.fl + 1] <= g) cur.fl++;
@@ -752,7 +755,7 @@ __device__ __forceinline__ void sort_tiles(const uint64_t *src, uint64_t *dst, u
 // any sort gives the full-key order the reference's sort starts from
 // (fast/quads.c:379-427); its tie fix follows in k_fit_moments.
 template <bool FIX>
-__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q) {
+__global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const WorkQueue *Q, int l) {
     static_assert(kSortTile == 2048, "4 waves x 512 keys");
     constexpr bool big = FIX;  // calls of a few frames: the big sort takes the huge clusters, their keys fixed here
     __shared__ uint64_t sk[kSortTile];
@@ -760,7 +763,7 @@ __global__ __launch_bounds__(256) void k_angle_sort(const Frame *frames, const W
     const uint32_t t = threadIdx.x;
     TakeCursor cur;
     uint32_t fi, c;
-    for (uint32_t g = blockIdx.x; cluster_at(frames, Q, 0, g, cur, &fi, &c); g += gridDim.x) {
+    for (uint32_t g = blockIdx.x; cluster_at(frames, Q, l, g, cur, &fi, &c); g += gridDim.x) {
         const Frame &F = frames[fi];
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
         if (big && n > kHugeCluster && n <= kBigSort) continue;  // k_angle_sort_big's
@@ -1865,13 +1868,16 @@ void warm_persistent_grids(int fit_blocks) {
 // Whether the angle sort runs per cluster (any size: its steps wider than
 // a 2048-key tile go over the cluster's keys in HBM), or (forced, for
 // tests) over all points at once.
+// 0: the angle sort of all points at once in HBM (forced); otherwise per
+// cluster, 2 when a cluster may exceed kBigSort points (the frame bounds
+// a fitted cluster by 2 (2 qw + 2 qh), fast/quads.c:1148-1154), else 1.
 int angle_sort_cap(int max_cluster, bool force_global) {
-    (void)max_cluster;
-    return force_global ? 0 : 1;
+    if (force_global) return 0;
+    return max_cluster > (int)kBigSort ? 2 : 1;
 }
 
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk) {
+                        int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk, Fork *fk) {
     // k_fit_prep: every cluster of every frame, a grid per frame; the fit
     // kernels after it: persistent over the work lists, fit_blocks blocks of
     // 256 threads or four times as many waves, at most as many as are
@@ -1891,55 +1897,81 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
                         (uint64_t)(1 | kNSmall << 8) << 32;
     hipLaunchKernelGGL(k_work_offsets, dim3(3), dim3(256), 0, st, dframes, nframes, Q, lw);
     mk->mark(st, "fit_prep.lists");
+    // The clusters' fits are independent from here to the quad order. With
+    // the fork on, the small ones (list 1) go on one forked stream and, for
+    // calls of a few frames, the large ones (list 0) on the other, beside
+    // the huge ones (list 4) on the main stream: parallel branches of the
+    // call's graph. Batches keep the huge and large ones together.
     int parity = 0;
+    hipStream_t fs[Fork::kForks] = {st, st};
+    const bool few = nframes <= 16;
     if (sort_cap) {
-        // calls of a few frames: the clusters above kHugeCluster points
-        // (a few dozen a frame) each by a 1024-thread block
-        const int big = nframes <= 16 ? 1 : 0;
-        if (big)
-            hipLaunchKernelGGL(k_angle_sort<true>, dim3(pgrid(k_angle_sort<true>, 256, fit_blocks)), dim3(256), 0, st,
-                               dframes, Q);
-        else
-            hipLaunchKernelGGL(k_angle_sort<false>, dim3(pgrid(k_angle_sort<false>, 256, fit_blocks)), dim3(256), 0,
-                               st, dframes, Q);
-        if (big)
+        fk->begin(st, fs);
+        hipStream_t sl = few ? fs[1] : st;  // the large clusters' stream
+        if (few) {
+            // the huge clusters of up to kBigSort points each by a
+            // 1024-thread block, larger ones (possible from 4096 quad-image
+            // pixels of perimeter on) by the tiled sort
             hipLaunchKernelGGL(k_angle_sort_big,
                                dim3(persistent_grid(reinterpret_cast<const void *>(&k_angle_sort_big), 1024,
                                                     kBigSort * 8, std::min(fit_blocks, 64 * nframes))),
                                dim3(1024), kBigSort * 8, st, dframes, Q);
+            if (sort_cap == 2)
+                hipLaunchKernelGGL(k_angle_sort<true>, dim3(pgrid(k_angle_sort<true>, 256, fit_blocks)), dim3(256), 0,
+                                   st, dframes, Q, 4);
+            hipLaunchKernelGGL(k_angle_sort<true>, dim3(pgrid(k_angle_sort<true>, 256, fit_blocks)), dim3(256), 0, sl,
+                               dframes, Q, 5);
+        } else {
+            hipLaunchKernelGGL(k_angle_sort<false>, dim3(pgrid(k_angle_sort<false>, 256, fit_blocks)), dim3(256), 0,
+                               st, dframes, Q, 0);
+        }
         mk->mark(st, "angle_sort.large");
-        hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, st, dframes, Q);
-        mk->mark(st, "angle_sort.small");
+        hipLaunchKernelGGL(k_angle_sort_wave, dim3(pgrid(k_angle_sort_wave, 64, wf)), dim3(64), 0, fs[0], dframes, Q);
+        mk->mark(fs[0], "angle_sort.small");
     } else {
         launch_radix_sort(st, dframes, nframes, kNPts, 32 + cl_bits, &parity, 0, 0, 0, 0, 0);
         mk->mark(st, "angle_sort.all");
+        fk->begin(st, fs);
     }
-    // large clusters: 128-thread blocks for batches (8 a CU: more running
-    // sums in flight), 256 for up to 16 frames (each cluster sooner)
-    if (nframes > 16)
+    // the huge and large clusters: 128-thread blocks for batches (8 a CU:
+    // more running sums in flight), 256 for up to 16 frames (each cluster
+    // sooner), one launch a list then
+    if (!few) {
         hipLaunchKernelGGL((k_fit_moments<128, 256>), dim3(pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks)),
                            dim3(128), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0, 1);
-    else
+    } else {
+        const int fix = sort_cap ? 0 : 1;
         hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256),
-                           0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0, sort_cap ? 0 : 1);
+                           0, st, dframes, parity, Q, sort_cap ? 1 : 0, 4, fix);
+        hipLaunchKernelGGL((k_fit_moments<256, 512>), dim3(pgrid(k_fit_moments<256, 512>, 256, fit_blocks)), dim3(256),
+                           0, fs[1], dframes, parity, Q, sort_cap ? 1 : 0, 5, fix);
+    }
     mk->mark(st, "moments.large");
-    if (nframes > 16 && sort_cap)  // batches: ten small clusters' sums a wave
+    if (!few && sort_cap)  // batches: ten small clusters' sums a wave
         hipLaunchKernelGGL((k_fit_moments_packed<10, 32>), dim3(pgrid(k_fit_moments_packed<10, 32>, 64, wf)),
-                           dim3(64), 0, st, dframes, Q, 1);
+                           dim3(64), 0, fs[0], dframes, Q, 1);
     else
-        hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, st,
-                       dframes, parity, Q, sort_cap ? 1 : 0, 1, sort_cap ? 0 : 1);
-    mk->mark(st, "moments.small");
-    hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st, dframes,
-                       dparams, Q, 0);
+        hipLaunchKernelGGL((k_fit_moments<64, 128>), dim3(pgrid(k_fit_moments<64, 128>, 64, wf)), dim3(64), 0, fs[0],
+                           dframes, parity, Q, sort_cap ? 1 : 0, 1, sort_cap ? 0 : 1);
+    mk->mark(fs[0], "moments.small");
+    if (!few) {
+        hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st,
+                           dframes, dparams, Q, 0);
+    } else {
+        hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st,
+                           dframes, dparams, Q, 4);
+        hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, fs[1],
+                           dframes, dparams, Q, 5);
+    }
     mk->mark(st, "corners.large");
     if (wave_corners)
-        hipLaunchKernelGGL(k_fit_corners<64>, dim3(pgrid(k_fit_corners<64>, 64, wf)), dim3(64), 0, st, dframes, dparams,
-                           Q, 1);
+        hipLaunchKernelGGL(k_fit_corners<64>, dim3(pgrid(k_fit_corners<64>, 64, wf)), dim3(64), 0, fs[0], dframes,
+                           dparams, Q, 1);
     else
-        hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_fit_corners<256>, dim3(pgrid(k_fit_corners<256>, 256, fit_blocks)), dim3(256), 0, fs[0],
                            dframes, dparams, Q, 1);
-    mk->mark(st, "corners.small");
+    mk->mark(fs[0], "corners.small");
+    fk->end(st);
     hipLaunchKernelGGL(k_quad_order, dim3(1, nframes), dim3(256), 0, st, dframes);
     launch_work_offsets(st, dframes, nframes, Q, 2, (int)kNQuads);
     mk->mark(st, "corners.order");

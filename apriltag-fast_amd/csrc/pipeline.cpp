@@ -30,7 +30,7 @@ void launch_ccl_stage(hipStream_t st, const Frame *dframes, const DevParams *dpa
 void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int max_qh, int key_bits, const Tune &tu,
                          Marks *mk);
 void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int cl_bits,
-                        int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk);
+                        int fit_blocks, int sort_cap, bool wave_corners, WorkQueue *Q, Marks *mk, Fork *fk);
 int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          bool refine, WorkQueue *Q, Marks *mk);
@@ -171,6 +171,9 @@ struct atg_ctx {
     // for its descriptors and results (both set up at creation)
     uint8_t *scratch = nullptr;
     uint8_t *na_stage = nullptr;
+    // the chain's forked streams (the fit of the small, large and huge
+    // clusters side by side); used when no per-stage timing is taken
+    Fork fork;
     // atg_estimate_poses: device copies of the tags in and the poses out
     uint8_t *pose_buf = nullptr;
     size_t pose_cap = 0;
@@ -497,6 +500,11 @@ extern "C" int atg_ctx_create(int device, atg_ctx **out) {
         HIPCHK(hipEventCreate(&ctx->ev0));
         HIPCHK(hipEventCreate(&ctx->ev1));
         HIPCHK(hipEventCreateWithFlags(&ctx->ev2, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ctx->fork.go, hipEventDisableTiming));
+        for (int k = 0; k < Fork::kForks; k++) {
+            HIPCHK(hipStreamCreateWithFlags(&ctx->fork.s[k], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&ctx->fork.done[k], hipEventDisableTiming));
+        }
         HIPCHK(hipMalloc(&ctx->marks.stamps, kMaxMarks * sizeof(uint64_t)));
         HIPCHK(hipHostMalloc(&ctx->hstamps, kMaxMarks * sizeof(uint64_t)));
         int khz = 0;
@@ -539,6 +547,11 @@ extern "C" void atg_ctx_destroy(atg_ctx *ctx) {
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->ev2) hipEventDestroy(ctx->ev2);
+    if (ctx->fork.go) hipEventDestroy(ctx->fork.go);
+    for (int k = 0; k < Fork::kForks; k++) {
+        if (ctx->fork.done[k]) hipEventDestroy(ctx->fork.done[k]);
+        if (ctx->fork.s[k]) hipStreamDestroy(ctx->fork.s[k]);
+    }
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -640,7 +653,7 @@ void describe_frame(const DevParams &hp, const FramePlan &fp, int nfam, bool hos
 // and the work lists' header dq in device memory), on stream st.
 void launch_chain(hipStream_t st, const ChainShape &cs, const CclPlan &ccl, const DevParams &hp, bool deglitch,
                   const Frame *dframes, const DevParams *dparams, int n, WorkQueue *dq, const Tune &tu, bool force,
-                  Marks *mk) {
+                  Marks *mk, Fork *fk) {
     // the angle sort in LDS per cluster, or (clusters too large) of all points in HBM
     const int sort_cap = angle_sort_cap(cs.max_cluster, force);
     // the corner search of the small clusters a wave each, when its pair
@@ -652,7 +665,7 @@ void launch_chain(hipStream_t st, const ChainShape &cs, const CclPlan &ccl, cons
     launch_image_stage(st, dframes, dparams, n, cs.max_qw, cs.max_qh, blur, deglitch, tu, mk);
     launch_ccl_stage(st, dframes, dparams, n, cs.max_qw, cs.max_qh, ccl, dq, tu, mk);
     launch_points_stage(st, dframes, n, cs.max_qh, cs.key_bits, tu, mk);
-    launch_quads_stage(st, dframes, dparams, n, cs.cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk);
+    launch_quads_stage(st, dframes, dparams, n, cs.cl_bits, tu.fit_blocks, sort_cap, wave_corners, dq, mk, fk);
     launch_decode_stage(st, dframes, dparams, n, tu.fit_blocks, hp.refine != 0, dq, mk);
 }
 
@@ -784,6 +797,7 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
     hipStream_t st = ctx->stream;
     Marks *mk = &ctx->marks;
     mk->level = ctx->profiling;
+    ctx->fork.on = ctx->profiling == 0;
     const Tune tu = ctx->tune;
     const int sort_cap = angle_sort_cap(cs.max_cluster, ctx->force_scratch);
     const bool wave_corners = ctx->hp.max_nmaxima <= 16;
@@ -796,7 +810,8 @@ int chain_enqueue(atg_ctx *ctx, const atg_params *p, const atg_frame *frames, in
             hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return e;
         }
-        launch_chain(st, cs, ccl, ctx->hp, deglitch, ctx->dframes, ctx->dparams, n, dq, tu, ctx->force_scratch, mk);
+        launch_chain(st, cs, ccl, ctx->hp, deglitch, ctx->dframes, ctx->dparams, n, dq, tu, ctx->force_scratch, mk,
+                     &ctx->fork);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (mk->n > 0 && mk->level == 1) {
@@ -1099,9 +1114,10 @@ extern "C" int atg_detect_noalloc(atg_ctx *ctx, const atg_params *p, const uint8
         // descriptor, work lists' header and parameters in one copy
         HIPCHK(hipMemcpyAsync(base, stage, kNaResult, hipMemcpyHostToDevice, st));
         Marks off;
+        Fork nofork;
         launch_chain(st, cs, ccl, hp, p->deglitch, reinterpret_cast<const Frame *>(base + kNaFrame),
                      reinterpret_cast<const DevParams *>(base + kNaParams), 1,
-                     reinterpret_cast<WorkQueue *>(base + kNaQueue), ctx->tune, ctx->force_scratch, &off);
+                     reinterpret_cast<WorkQueue *>(base + kNaQueue), ctx->tune, ctx->force_scratch, &off, &nofork);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(stage + kNaResult, base + kNaResult, kResultBlock, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
