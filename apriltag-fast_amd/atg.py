@@ -200,6 +200,8 @@ def lib():
     L.atg_host_alloc.restype = C.c_int
     L.atg_host_free.argtypes = [vp]
     L.atg_host_free.restype = C.c_int
+    L.atg_debug_trig.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, C.c_int]
+    L.atg_debug_trig.restype = C.c_int
     L.atg_estimate_poses.argtypes = [vp, vp, i64, i32, i32, vp]
     L.atg_estimate_poses.restype = C.c_int
     _lib = L
@@ -379,6 +381,17 @@ class Context:
         self._check(self.L.atg_estimate_poses(self.h, t.ctypes.data if len(t) else None, len(t), n_iters, flags,
                                               out.ctypes.data if len(t) else None), "atg_estimate_poses")
         return out
+
+    def trig(self, y, x, th, slow=False):
+        """The refinement's correctly rounded atan2f(y, x), cosf(th),
+        sinf(th) on the device (equal-length float32 arrays)."""
+        y, x, th = (np.ascontiguousarray(a, np.float32) for a in (y, x, th))
+        n = len(y)
+        assert len(x) == n and len(th) == n
+        at, co, si = (np.zeros(n, np.float32) for _ in range(3))
+        self._check(self.L.atg_debug_trig(self.h, y.ctypes.data, x.ctypes.data, th.ctypes.data, n, at.ctypes.data,
+                                          co.ctypes.data, si.ctypes.data, 1 if slow else 0), "atg_debug_trig")
+        return at, co, si
 
     def stage_times(self):
         """[(group.kernel, ms)] of the last call with profiling on."""

@@ -18,13 +18,14 @@
 //              order, reconcile (:860-891), stable sort by id
 //              (fast/detector.c:164-179).
 //
-// atan2f/cosf/sinf must be correctly rounded (the reference's goldens are);
-// they are evaluated in double and rounded once to float.
+// atan2f/cosf/sinf must be correctly rounded (the reference's goldens are):
+// cr_trig.h.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
 #include "atg_internal.h"
+#include "cr_trig.h"
 
 #pragma clang fp contract(off)
 
@@ -465,11 +466,30 @@ __global__ __launch_bounds__(256) void k_lines(const Frame *frames, const WorkQu
     for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < total; g += gridDim.x * 256) {
         while (4 * off[fi + 1] <= g) fi++;
         double *ln = frames[fi].qline + (size_t)(g - 4 * off[fi]) * 4;
-        float th2 = (float)atan2(ln[2], ln[3]);
-        double theta = DMUL(0.5, (double)th2);
-        ln[2] = (double)(float)cos((double)(float)theta);
-        ln[3] = (double)(float)sin((double)(float)theta);
+        const float th2 = cr::atan2f_cr((float)ln[2], (float)ln[3]);
+        const double theta = DMUL(0.5, (double)th2);
+        ln[2] = (double)cr::cosf_cr((float)theta);
+        ln[3] = (double)cr::sinf_cr((float)theta);
     }
+}
+
+// The refinement's correctly rounded atan2f, cosf and sinf on caller
+// arrays (a check of cr_trig.h): a thread an element; slow: the
+// double-double path for every element.
+__global__ __launch_bounds__(256) void k_debug_trig(const float *y, const float *x, const float *th, int64_t n,
+                                                    float *at, float *co, float *si, int slow) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    at[i] = cr::atan2f_cr(y[i], x[i], slow != 0);
+    co[i] = cr::cosf_cr(th[i], slow != 0);
+    si[i] = cr::sinf_cr(th[i], slow != 0);
+}
+
+void launch_debug_trig(hipStream_t st, const float *y, const float *x, const float *th, int64_t n, float *at,
+                       float *co, float *si, int slow) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_debug_trig, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, y, x, th, n, at, co, si,
+                           slow);
 }
 
 // Persistent: the quads of every frame, block b taking global indices b,

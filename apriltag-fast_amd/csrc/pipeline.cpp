@@ -35,6 +35,8 @@ int angle_sort_cap(int max_cluster, bool force_global);
 void launch_decode_stage(hipStream_t st, const Frame *dframes, const DevParams *dparams, int nframes, int blocks,
                          bool refine, WorkQueue *Q, Marks *mk);
 void launch_debug_finish(hipStream_t st, uint8_t *buf, uint32_t n, int64_t *nout);
+void launch_debug_trig(hipStream_t st, const float *y, const float *x, const float *th, int64_t n, float *at,
+                       float *co, float *si, int slow);
 hipError_t launch_pose(hipStream_t st, const atg_pose_in *in, int64_t n, int iters, int flags, atg_pose_out *out);
 void prepare_points_stage();
 }  // namespace atg
@@ -1339,6 +1341,31 @@ extern "C" int atg_estimate_poses(atg_ctx *ctx, const atg_pose_in *in, int64_t n
     HIPCHK(hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return ATG_OK;
+}
+
+extern "C" int atg_debug_trig(atg_ctx *ctx, const float *y, const float *x, const float *th, int64_t n,
+                              float *atan2_out, float *cos_out, float *sin_out, int slow) {
+    if (!ctx || n < 0 || n > ((int64_t)1 << 26) || (n > 0 && (!y || !x || !th || !atan2_out || !cos_out || !sin_out)))
+        return ATG_ERR_ARG;
+    if (n == 0) return ATG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t b = sizeof(float) * (size_t)n;
+    float *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, 6 * b));
+    int rc = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(buf, y, b, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(buf + n, x, b, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(buf + 2 * n, th, b, hipMemcpyHostToDevice, ctx->stream));
+        launch_debug_trig(ctx->stream, buf, buf + n, buf + 2 * n, n, buf + 3 * n, buf + 4 * n, buf + 5 * n, slow);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(atan2_out, buf + 3 * n, b, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(cos_out, buf + 4 * n, b, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(sin_out, buf + 5 * n, b, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return ATG_OK;
+    }();
+    hipFree(buf);
+    return rc;
 }
 
 extern "C" int atg_debug_force_scratch(atg_ctx *ctx, int on) {

@@ -153,6 +153,15 @@ int atg_debug_copy(atg_ctx *ctx, void *dst, const void *src, int64_t bytes, doub
  * (fast/detector.c:164-179); *n becomes the count kept. */
 int atg_debug_finish(atg_ctx *ctx, at_detection *dets, int64_t *n);
 
+/* The refinement's correctly rounded float functions (the libm calls of
+ * refine_edges, fast/decode.c:740-744) on host arrays of n: atan2_out[i]
+ * = atan2f(y[i], x[i]), cos_out[i] = cosf(th[i]), sin_out[i] = sinf(th[i])
+ * as the device computes them (|th| <= 4). slow != 0: by the
+ * double-double path wherever it applies (which the device otherwise takes
+ * only next to a float rounding boundary). */
+int atg_debug_trig(atg_ctx *ctx, const float *y, const float *x, const float *th, int64_t n, float *atan2_out,
+                   float *cos_out, float *sin_out, int slow);
+
 /* on != 0: the following calls take the paths of frames too large for
  * LDS (the union-find replay's state in HBM scratch, the angle sort of all
  * points at once in HBM), so that tests can check them on any frame. */

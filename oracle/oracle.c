@@ -777,13 +777,13 @@ static void refine_edges(float qd, const OView *v, OQuad *q) {
         }
         double ex = mx / nn, ey = my / nn;
         double cxx = mxx / nn - ex * ex, cxy = mxy / nn - ex * ey, cyy = myy / nn - ey * ey;
-        /* correctly rounded float atan2/cos/sin: evaluated in double, rounded once */
-        float th2 = (float)atan2((double)(float)(-2.0 * cxy), (double)(float)(cyy - cxx));
+        /* correctly rounded float atan2/cos/sin (cr_libm.c) */
+        float th2 = cr_atan2f((float)(-2.0 * cxy), (float)(cyy - cxx));
         double theta = 0.5 * (double)th2;
         lines[edge][0] = ex;
         lines[edge][1] = ey;
-        lines[edge][2] = (float)cos((double)(float)theta);
-        lines[edge][3] = (float)sin((double)(float)theta);
+        lines[edge][2] = cr_cosf((float)theta);
+        lines[edge][3] = cr_sinf((float)theta);
     }
     for (int i = 0; i < 4; i++) {
         int j = (i + 1) & 3;

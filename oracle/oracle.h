@@ -76,4 +76,10 @@ void orc_default_params(orc_params *p);
  * better one (fast/decode.c:860-891). */
 void orc_reconcile(orc_detection *d, int64_t *n);
 
+/* Correctly rounded float atan2, cos and sin (cr_libm.c): the libm calls of
+ * refine_edges (fast/decode.c:740-744). */
+float cr_atan2f(float y, float x);
+float cr_cosf(float x);
+float cr_sinf(float x);
+
 #endif
