@@ -1856,7 +1856,7 @@ void warm_persistent_grids(int fit_blocks) {
         persistent_grid(reinterpret_cast<const void *>(&k_angle_sort_big), 1024, kBigSort * 8, fit_blocks);
     }
     pgrid(k_angle_sort_wave, 64, fit_blocks);
-    pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks);
+    pgrid(k_fit_moments<128, 128>, 128, 2 * fit_blocks);
     pgrid(k_fit_moments<256, 512>, 256, fit_blocks);
     pgrid(k_fit_moments<64, 128>, 64, fit_blocks);
     pgrid(k_fit_moments_packed<10, 32>, 64, fit_blocks);
@@ -1933,11 +1933,12 @@ void launch_quads_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         mk->mark(st, "angle_sort.all");
         fk->begin(st, fs);
     }
-    // the huge and large clusters: 128-thread blocks for batches (8 a CU:
-    // more running sums in flight), 256 for up to 16 frames (each cluster
-    // sooner), one launch a list then
+    // the huge and large clusters: 128-thread blocks of 128-point chunks for
+    // batches (more blocks a CU: more running sums in flight; 1.48 -> 1.36
+    // ms at 256 frames against 256-point chunks), 256 threads for up to 16
+    // frames (each cluster sooner), one launch a list then
     if (!few) {
-        hipLaunchKernelGGL((k_fit_moments<128, 256>), dim3(pgrid(k_fit_moments<128, 256>, 128, 2 * fit_blocks)),
+        hipLaunchKernelGGL((k_fit_moments<128, 128>), dim3(pgrid(k_fit_moments<128, 128>, 128, 2 * fit_blocks)),
                            dim3(128), 0, st, dframes, parity, Q, sort_cap ? 1 : 0, 0, 1);
     } else {
         const int fix = sort_cap ? 0 : 1;
