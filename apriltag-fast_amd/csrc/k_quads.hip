@@ -1329,7 +1329,7 @@ __device__ __forceinline__ uint32_t bscan_t(uint32_t v, uint32_t *total) {
 // Persistent over list l of kept clusters, a cluster per block of NT threads.
 template <int NT>
 // (the 256-thread shape held to 128 registers: four blocks a CU, as its LDS allows)
-__global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q,
+__global__ __launch_bounds__(NT, 4) void k_fit_corners(const Frame *frames, const DevParams *P, const WorkQueue *Q,
                                                     int l) {
     constexpr int kMaxNm = CornerShape<NT>::kMaxNm, kLdsPts = CornerShape<NT>::kLdsPts;
     constexpr int kMaxima = CornerShape<NT>::kMaxima;
@@ -1353,7 +1353,8 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
     __shared__ int64_t s_idx[4];
     __shared__ double s_lines[4][4];
     __shared__ float s_qp[4][2];
-    __shared__ int s_ok, s_lok[4];
+    __shared__ int s_ok, s_lok[4], s_dok[4], s_cvx[4];
+    __shared__ double s_side[6];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const double max_mse = (double)P->max_mse, max_dot = (double)P->cos_crit;
     const double INF = __longlong_as_double(0x7ff0000000000000ll);
@@ -1695,62 +1696,68 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_fit_corners(const Fra
             s_lok[t] = mse > max_mse ? 0 : 1;
         }
         __syncthreads();
-        if (t == 0) {
-            bool ok = s_lok[0] && s_lok[1] && s_lok[2] && s_lok[3];
-            for (int i = 0; ok && i < 4; i++) {
-                int j = (i + 1) & 3;
-                double a00 = s_lines[i][3], a01 = -s_lines[j][3], a10 = -s_lines[i][2], a11 = s_lines[j][2];
-                double b0 = __dadd_rn(-s_lines[i][0], s_lines[j][0]), b1 = __dadd_rn(-s_lines[i][1], s_lines[j][1]);
-                double det = __dsub_rn(__dmul_rn(a00, a11), __dmul_rn(a10, a01));
-                if (fabs(det) < 0.001) { ok = false; break; }
+        // the corners of lines i and i + 1 (fast/quads.c:1077-1092), one
+        // thread each; then the six side lengths of the area's two Heron
+        // triangles (:1094-1110) and the four convexity checks (:1112-1124)
+        // on threads of their own, and their reduction (in the reference's
+        // order) on thread 0. A corner, side or angle of a quad rejected
+        // earlier in the reference's order is computed and not used.
+        auto sq = [](double v) { return __dmul_rn(v, v); };
+        auto fd = [](float a, float b) { return (double)__fsub_rn(a, b); };
+        if (t < 4) {
+            const int i = t, j = (i + 1) & 3;
+            double a00 = s_lines[i][3], a01 = -s_lines[j][3], a10 = -s_lines[i][2], a11 = s_lines[j][2];
+            double b0 = __dadd_rn(-s_lines[i][0], s_lines[j][0]), b1 = __dadd_rn(-s_lines[i][1], s_lines[j][1]);
+            double det = __dsub_rn(__dmul_rn(a00, a11), __dmul_rn(a10, a01));
+            const bool dok = !(fabs(det) < 0.001);
+            s_dok[i] = dok ? 1 : 0;
+            if (dok) {
                 double w00 = __ddiv_rn(a11, det), w01 = __ddiv_rn(-a01, det);
                 double l0 = __dadd_rn(__dmul_rn(w00, b0), __dmul_rn(w01, b1));
                 s_qp[i][0] = (float)__dadd_rn(s_lines[i][0], __dmul_rn(l0, a00));
                 s_qp[i][1] = (float)__dadd_rn(s_lines[i][1], __dmul_rn(l0, a10));
             }
-            QuadRec q;
-            if (ok)
-                for (int i = 0; i < 4; i++) {
-                    q.p[i][0] = s_qp[i][0];
-                    q.p[i][1] = s_qp[i][1];
-                }
-            if (ok) {
-                // area (two Heron triangles) and convexity (:1094-1124);
-                // coordinate differences in float
-                auto sq = [](double v) { return __dmul_rn(v, v); };
-                auto fd = [](float a, float b) { return (double)__fsub_rn(a, b); };
-                double len[3], area = 0.0;
-                for (int i = 0; i < 3; i++) {
-                    int ia = i, ib = (i + 1) % 3;
-                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(s_qp[ib][0], s_qp[ia][0])), sq(fd(s_qp[ib][1], s_qp[ia][1]))));
-                }
-                double pp = __ddiv_rn(__dadd_rn(__dadd_rn(len[0], len[1]), len[2]), 2.0);
-                area = __dadd_rn(area, __dsqrt_rn(__dmul_rn(__dmul_rn(__dmul_rn(pp, __dsub_rn(pp, len[0])),
-                                                                      __dsub_rn(pp, len[1])),
-                                                            __dsub_rn(pp, len[2]))));
-                const int idxs[4] = {2, 3, 0, 2};
-                for (int i = 0; i < 3; i++) {
-                    int ia = idxs[i], ib = idxs[i + 1];
-                    len[i] = __dsqrt_rn(__dadd_rn(sq(fd(s_qp[ib][0], s_qp[ia][0])), sq(fd(s_qp[ib][1], s_qp[ia][1]))));
-                }
-                pp = __ddiv_rn(__dadd_rn(__dadd_rn(len[0], len[1]), len[2]), 2.0);
-                area = __dadd_rn(area, __dsqrt_rn(__dmul_rn(__dmul_rn(__dmul_rn(pp, __dsub_rn(pp, len[0])),
-                                                                      __dsub_rn(pp, len[1])),
-                                                            __dsub_rn(pp, len[2]))));
-                double tw = (double)P->tag_width;
-                if (area < __dmul_rn(__dmul_rn(0.95, tw), tw)) ok = false;
-                double ccr = (double)P->cos_crit;
-                for (int i = 0; ok && i < 4; i++) {
-                    int i0 = i, i1 = (i + 1) & 3, i2 = (i + 2) & 3;
-                    double dx1 = fd(s_qp[i1][0], s_qp[i0][0]), dy1 = fd(s_qp[i1][1], s_qp[i0][1]);
-                    double dx2 = fd(s_qp[i2][0], s_qp[i1][0]), dy2 = fd(s_qp[i2][1], s_qp[i1][1]);
-                    double den = __dsqrt_rn(__dmul_rn(__dadd_rn(__dmul_rn(dx1, dx1), __dmul_rn(dy1, dy1)),
-                                                      __dadd_rn(__dmul_rn(dx2, dx2), __dmul_rn(dy2, dy2))));
-                    if (den == 0.0) { ok = false; break; }
-                    double cd = __ddiv_rn(__dadd_rn(__dmul_rn(dx1, dx2), __dmul_rn(dy1, dy2)), den);
-                    if ((cd > ccr || cd < -ccr) || __dmul_rn(dx1, dy2) < __dmul_rn(dy1, dx2)) ok = false;
-                }
+        }
+        __syncthreads();
+        if (!(s_lok[0] && s_lok[1] && s_lok[2] && s_lok[3] && s_dok[0] && s_dok[1] && s_dok[2] && s_dok[3])) continue;
+        if (t < 6) {
+            // triangle (0, 1, 2) then (2, 3, 0): sides (0,1) (1,2) (2,0) (2,3) (3,0) (0,2)
+            const int ia = t < 3 ? t : (t == 3 ? 2 : (t == 4 ? 3 : 0));
+            const int ib = t < 3 ? (t + 1) % 3 : (t == 3 ? 3 : (t == 4 ? 0 : 2));
+            s_side[t] = __dsqrt_rn(__dadd_rn(sq(fd(s_qp[ib][0], s_qp[ia][0])), sq(fd(s_qp[ib][1], s_qp[ia][1]))));
+        } else if (t >= 8 && t < 12) {
+            const int i0 = t - 8, i1 = (i0 + 1) & 3, i2 = (i0 + 2) & 3;
+            const double ccr = (double)P->cos_crit;
+            double dx1 = fd(s_qp[i1][0], s_qp[i0][0]), dy1 = fd(s_qp[i1][1], s_qp[i0][1]);
+            double dx2 = fd(s_qp[i2][0], s_qp[i1][0]), dy2 = fd(s_qp[i2][1], s_qp[i1][1]);
+            double den = __dsqrt_rn(__dmul_rn(__dadd_rn(__dmul_rn(dx1, dx1), __dmul_rn(dy1, dy1)),
+                                              __dadd_rn(__dmul_rn(dx2, dx2), __dmul_rn(dy2, dy2))));
+            bool cok = den != 0.0;
+            if (cok) {
+                double cd = __ddiv_rn(__dadd_rn(__dmul_rn(dx1, dx2), __dmul_rn(dy1, dy2)), den);
+                if ((cd > ccr || cd < -ccr) || __dmul_rn(dx1, dy2) < __dmul_rn(dy1, dx2)) cok = false;
             }
+            s_cvx[i0] = cok ? 1 : 0;
+        }
+        __syncthreads();
+        if (t == 0) {
+            bool ok = true;
+            QuadRec q;
+            for (int i = 0; i < 4; i++) {
+                q.p[i][0] = s_qp[i][0];
+                q.p[i][1] = s_qp[i][1];
+            }
+            double area = 0.0;
+            for (int k = 0; k < 2; k++) {
+                const double l0 = s_side[3 * k], l1 = s_side[3 * k + 1], l2 = s_side[3 * k + 2];
+                double pp = __ddiv_rn(__dadd_rn(__dadd_rn(l0, l1), l2), 2.0);
+                area = __dadd_rn(area, __dsqrt_rn(__dmul_rn(__dmul_rn(__dmul_rn(pp, __dsub_rn(pp, l0)),
+                                                                      __dsub_rn(pp, l1)),
+                                                            __dsub_rn(pp, l2))));
+            }
+            double tw = (double)P->tag_width;
+            if (area < __dmul_rn(__dmul_rn(0.95, tw), tw)) ok = false;
+            ok = ok && s_cvx[0] && s_cvx[1] && s_cvx[2] && s_cvx[3];
             if (ok) {
                 q.reversed = (flag & 2u) ? 1 : 0;
                 q.hok = 0;
