@@ -869,7 +869,8 @@ __device__ __forceinline__ void wave_sort_keys_lds(const uint64_t *in, uint64_t 
     }
 }
 
-__global__ __launch_bounds__(64) void k_angle_sort_wave(const Frame *frames, const WorkQueue *Q) {
+// (96 registers: five waves a SIMD instead of four)
+__global__ __launch_bounds__(64, 5) void k_angle_sort_wave(const Frame *frames, const WorkQueue *Q) {
     static_assert(kLargeCluster <= 512, "eight keys a lane at most");
     __shared__ uint64_t kl[kLargeCluster];
     __shared__ uint32_t fls[4 * 64 + 1], flm[4 * 64 + 1], fcnt;
