@@ -54,6 +54,13 @@ __device__ __forceinline__ bool runs_join(int a0, int a1, int b0, int b1, bool w
     return max(max(a0, b0 - 1), 1) <= min(a1, b1 - 1);
 }
 
+// ecnt[y]: row y's link count, and in this bit whether its last run is
+// also linked to the row's virtual node (fast/unionfind.c:93-97: a white
+// run ending at column qw - 2 under a white pixel at qw - 1 whose left
+// neighbour is black, in row y - 1)
+constexpr uint32_t kLastColLink = 0x80000000u;
+__device__ __forceinline__ int edge_count(uint32_t ecnt) { return (int)(ecnt & ~kLastColLink); }
+
 // The links of every row, in the reference's order (current run
 // ascending, then previous run ascending): one wave per row, all rows at
 // once. uedge[y][e] = i << 16 | k (current run i of row y, previous run k).
@@ -70,6 +77,16 @@ __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames, int lds_q
     const int ncur = (int)F.rcnt[y], nprev = (int)F.rcnt[y - 1];
     const uint32_t *cur = F.runs + (int64_t)y * QW;
     const uint32_t *prv = F.runs + (int64_t)(y - 1) * QW;
+    // the virtual-node link of the row's last run (its loads in flight
+    // through the row's links)
+    uint32_t lastrun = 0u;
+    uint8_t tq1 = 0, tq2 = 0;
+    if (lane == 0 && ncur > 0) {
+        const uint8_t *trow = F.t + (int64_t)(y - 1) * QW;
+        lastrun = cur[ncur - 1];
+        tq1 = trow[QW - 1];
+        tq2 = trow[QW - 2];
+    }
     // the first 64 current runs, read while the previous row is copied
     uint32_t rnext = lane < ncur ? cur[lane] : 0u;
     if constexpr (L) {
@@ -134,7 +151,8 @@ __global__ __launch_bounds__(256) void k_uf_edges(const Frame *frames, int lds_q
     }
     if (lane == 0) {
         if (carry > cap) atomicOr(F.counters + kError, 1u);
-        F.ecnt[y] = min(carry, cap);
+        const bool lastcol = ncur > 0 && (int)run_end(lastrun) == QW - 2 && run_white(lastrun) && tq1 == 255 && tq2 != 255;
+        F.ecnt[y] = min(carry, cap) | (lastcol ? kLastColLink : 0u);
         atomicAdd(F.counters + kNEdges, min(carry, cap));
     }
 }
@@ -254,7 +272,6 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
     const auto g_uedge = gmem(F.uedge);
     const auto g_rcnt = gmem(F.rcnt);
     const auto g_ecnt = gmem(F.ecnt);
-    const auto g_t = gmem(F.t);
     const auto g_rlab = gmem(F.rlab);
     const auto g_recw = gmem(F.recw);
     const auto g_rroot = gmem(F.rroot);
@@ -340,7 +357,6 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
     // the next row, loaded one row ahead: thread tid holds runs and links
     // tid + j kFrontThreads
     uint32_t nx_rc = 0, nx_ec = 0, nx_r[kPer], nx_e[kPer];
-    uint8_t nx_t1 = 0, nx_t2 = 0;
     auto prefetch = [&](int y) {
         const auto cur = g_runs + (int64_t)y * QW;
         const auto edges = g_uedge + (int64_t)y * 3 * QW;
@@ -352,15 +368,14 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
             nx_r[j] = i < QW ? cur[i] : 0u;
             nx_e[j] = i < 3 * QW ? edges[i] : 0u;
         }
-        const auto trow = g_t + (int64_t)(y - 1) * QW;
-        nx_t1 = trow[QW - 1];
-        nx_t2 = trow[QW - 2];
     };
     if (yp + 1 <= y1) prefetch(yp + 1);
     bsync<G>();
     for (int y = yp + 1; y <= y1; y++) {
-        const int ncur = (int)nx_rc, ne = (int)nx_ec;
-        const uint8_t t1 = nx_t1, t2 = nx_t2;
+        // (the link count and the virtual-node flag stay in vector
+        // registers until here: no wait on the loads in flight)
+        const int ncur = (int)nx_rc, ne = edge_count(nx_ec);
+        const bool vlink = (nx_ec & kLastColLink) != 0;
         uint32_t rr[kPer], ee[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; j++) {
@@ -414,7 +429,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
                 acc[QW + i] = run_end(r) - run_start(r) + 1;
                 minrun[QW + i] = 0xffffffffu;
                 if (mode == kSweepLocal) mp[QW + i] = 0xffffffffu;
-                if (i == ncur - 1) lastcol = (int)run_end(r) == QW - 2 && run_white(r) && t1 == 255 && t2 != 255;
+                if (i == ncur - 1) lastcol = vlink;
             }
         }
         for (int i = kStage + tid; i < ncur; i += kFrontThreads) {
@@ -423,7 +438,7 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
             acc[QW + i] = run_end(r) - run_start(r) + 1;
             minrun[QW + i] = 0xffffffffu;
             if (mode == kSweepLocal) mp[QW + i] = 0xffffffffu;
-            if (i == ncur - 1) lastcol = (int)run_end(r) == QW - 2 && run_white(r) && t1 == 255 && t2 != 255;
+            if (i == ncur - 1) lastcol = vlink;
         }
         if (tid == 0) {
             p[VN] = VN;
@@ -892,7 +907,7 @@ size_t replay_hbm_bytes(int qw) { return replay_hbm_bytes_d(qw); }
 constexpr int kReplayHbmWaves = 16, kReplayHbmWavesAll = 64;
 
 __device__ __forceinline__ bool replay_fits(const Frame &F, int y, int cap, int lcap) {
-    return (int)F.rcnt[y - 1] <= cap && (int)F.rcnt[y] <= cap && (int)F.ecnt[y] <= lcap;
+    return (int)F.rcnt[y - 1] <= cap && (int)F.rcnt[y] <= cap && edge_count(F.ecnt[y]) <= lcap;
 }
 
 // A row's elements: (parent, weight) pairs, or (PK) packed in one word as
@@ -975,7 +990,6 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     const auto g_uedge = gmem(F.uedge);
     const auto g_rlab = gmem(F.rlab);
     const auto g_recw = gmem(F.recw);
-    const auto g_t = gmem(F.t);
     const auto g_rroot = gmem(F.rroot);
     const auto g_parent = gmem(F.parent);
     const auto g_rlost = gmem(F.rlost);
@@ -989,7 +1003,8 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     U8 *hasl = reinterpret_cast<U8 *>(base + ly.hasl);
     const uint32_t pb = g_row_off[y - 1], cb = g_row_off[y], vb = g_row_off[F.qh];
     const int m = (int)g_nrec[y - 1], nprev = (int)g_rcnt[y - 1];
-    const int ncur = (int)g_rcnt[y], ne = (int)g_ecnt[y], ng = (int)g_nrec[y];
+    const uint32_t ecnt = g_ecnt[y];
+    const int ncur = (int)g_rcnt[y], ne = edge_count(ecnt), ng = (int)g_nrec[y];
     const auto cur = g_runs + (int64_t)y * QW;
     const auto edges = g_uedge + (int64_t)y * 3 * QW;
     const auto rlab_cur = g_rlab + cb;
@@ -1025,12 +1040,7 @@ __device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     if (lane == 0) {
         ent.set((uint32_t)ev, (uint32_t)ev, 1u);
     }
-    bool lastcol = false;
-    if (ncur > 0) {
-        uint32_t lr = cur[ncur - 1];
-        const auto trow = g_t + (int64_t)(y - 1) * QW;
-        lastcol = (int)run_end(lr) == QW - 2 && run_white(lr) && trow[QW - 1] == 255 && trow[QW - 2] != 255;
-    }
+    const bool lastcol = ncur > 0 && (ecnt & kLastColLink) != 0;
     wsync<G>();
     // the links by group, stably (each group's links in order, contiguous
     // in le; group g's links from head[g - 1] (0 for g = 0) to head[g]):
