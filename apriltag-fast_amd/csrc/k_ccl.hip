@@ -355,35 +355,34 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
     const int nport = mode == kSweepLocal ? pcur : 0;
     const uint32_t portslot = pbase;  // the port row's slots
     // the next row, loaded one row ahead: thread tid holds runs and links
-    // tid + j kFrontThreads
-    uint32_t nx_rc = 0, nx_ec = 0, nx_r[kPer], nx_e[kPer];
-    auto prefetch = [&](int y) {
+    // tid + j kFrontThreads. Two buffers take the rows in turn (the loop
+    // body twice), so that no copy of a buffer waits for its loads.
+    // The row's run and link counts are loaded lane by lane (lane 0 the
+    // run count, lane 1 the link count) and read out of the lanes where the
+    // row begins: a uniform load would be moved to scalar registers as it
+    // returns, a wait on every load in flight.
+    struct Ahead {
+        uint32_t cnt, r[kPer], e[kPer];
+    };
+    auto prefetch = [&](int y, Ahead &nx) {
         const auto cur = g_runs + (int64_t)y * QW;
         const auto edges = g_uedge + (int64_t)y * 3 * QW;
-        nx_rc = g_rcnt[y];
-        nx_ec = g_ecnt[y];
+        nx.cnt = (lane & 1) ? g_ecnt[y] : g_rcnt[y];
 #pragma unroll
         for (int j = 0; j < kPer; j++) {
             int i = tid + j * kFrontThreads;
-            nx_r[j] = i < QW ? cur[i] : 0u;
-            nx_e[j] = i < 3 * QW ? edges[i] : 0u;
+            nx.r[j] = i < QW ? cur[i] : 0u;
+            nx.e[j] = i < 3 * QW ? edges[i] : 0u;
         }
     };
-    if (yp + 1 <= y1) prefetch(yp + 1);
-    bsync<G>();
-    for (int y = yp + 1; y <= y1; y++) {
-        // (the link count and the virtual-node flag stay in vector
-        // registers until here: no wait on the loads in flight)
-        const int ncur = (int)nx_rc, ne = edge_count(nx_ec);
-        const bool vlink = (nx_ec & kLastColLink) != 0;
-        uint32_t rr[kPer], ee[kPer];
-#pragma unroll
-        for (int j = 0; j < kPer; j++) {
-            rr[j] = nx_r[j];
-            ee[j] = nx_e[j];
-        }
+    auto sweep_row = [&](int y, const Ahead &cu, Ahead &nx) {
+        const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane((int)cu.cnt, 0);
+        const uint32_t ec = (uint32_t)__builtin_amdgcn_readlane((int)cu.cnt, 1);
+        const int ncur = (int)rc, ne = edge_count(ec);
+        const bool vlink = (ec & kLastColLink) != 0;
+        const uint32_t *rr = cu.r, *ee = cu.e;
         const uint32_t cbase = pbase + (uint32_t)pcur;
-        if (y + 1 <= y1) prefetch(y + 1);
+        if (y + 1 <= y1) prefetch(y + 1, nx);
         const auto cur = g_runs + (int64_t)y * QW;
         const auto edges = g_uedge + (int64_t)y * 3 * QW;
         // the previous row's outputs (each run's record, its first run,
@@ -483,6 +482,13 @@ __device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, 
         U32 *t1a = acc; acc = pacc; pacc = t1a;
         U32 *t2m = minrun; minrun = pmin; pmin = t2m;
         U32 *t3 = mp; mp = pmp; pmp = t3;
+    };
+    Ahead ah0{}, ah1{};
+    if (yp + 1 <= y1) prefetch(yp + 1, ah0);
+    bsync<G>();
+    for (int y = yp + 1; y <= y1; y += 2) {
+        sweep_row(y, ah0, ah1);
+        if (y + 1 <= y1) sweep_row(y + 1, ah1, ah0);
     }
     // the last row's outputs, and (kSweepLocal) each port's record in it
     for (int i = tid; i < pcur; i += kFrontThreads) {
