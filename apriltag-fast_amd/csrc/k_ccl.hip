@@ -264,7 +264,7 @@ constexpr uint32_t kPortLive = 0x80000000u;   // (component record) reaches row 
 // the band); row_off and the virtual nodes of rows olo..ohi (the last
 // band also row_off[QH] and the run count).
 template <bool G, int kFrontThreads, int mode>
-__device__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, int y0, int y1, int olo, int ohi) {
+__device__ __forceinline__ void front_sweep(const Frame &F, uint8_t *base, uint32_t *lds_small, int y0, int y1, int olo, int ohi) {
     constexpr int kPer = kStage / kFrontThreads;  // prefetched runs and links per thread
     // the frame's arrays in HBM, as global pointers (flat accesses would
     // hold up the waits for the LDS union-find)
@@ -981,7 +981,7 @@ struct Ents {
 
 // Replay of row y with capacity C runs and L links (the row fits).
 template <bool G, bool PK>
-__device__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
+__device__ __forceinline__ void replay_row(const Frame &F, int y, uint8_t *base, int C, int L) {
     using U32 = typename Mem<G>::U32;
     using U16 = typename Mem<G>::U16;
     using IX = typename std::conditional<G, volatile uint32_t, uint16_t>::type;
