@@ -222,6 +222,56 @@ __device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, ui
     }
 }
 
+// The roots of a and b found together, each step's reads of the two paths
+// issued at once: the two dependent chains of LDS reads overlap. qa, qb:
+// the parents of a and b, already read. Halve: path halving (while sets
+// are united); without it the finds only read, so that an element's own
+// write of its root (when the sets are settled) is the only write to it.
+template <bool G, bool Halve>
+__device__ __forceinline__ void fr_find_pair(typename Mem<G>::U32 *p, uint32_t &a, uint32_t qa, uint32_t &b,
+                                             uint32_t qb) {
+    bool da = qa == a, db = qb == b;
+    while (!(da && db)) {
+        const uint32_t ga = da ? a : (uint32_t)p[qa];
+        const uint32_t gb = db ? b : (uint32_t)p[qb];
+        if (!da) {
+            if (ga == qa) {
+                a = qa;
+                da = true;
+            } else {
+                if (Halve) p[a] = ga;
+                a = ga;
+            }
+        }
+        if (!db) {
+            if (gb == qb) {
+                b = qb;
+                db = true;
+            } else {
+                if (Halve) p[b] = gb;
+                b = gb;
+            }
+        }
+        if (!da) qa = p[a];
+        if (!db) qb = p[b];
+    }
+}
+
+// fr_unite with the two finds together (fr_find_pair)
+template <bool G>
+__device__ __forceinline__ void fr_unite_pair(typename Mem<G>::U32 *p, uint32_t a, uint32_t b) {
+    for (;;) {
+        fr_find_pair<G, true>(p, a, p[a], b, p[b]);
+        if (a == b) return;
+        if (a < b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS((uint32_t *)&p[a], a, b) == a) return;
+    }
+}
+
 // The first kStage runs and links of a row come from registers loaded one
 // row ahead; the rest (rare) from HBM at use.
 constexpr int kStage = 1024;
@@ -389,11 +439,13 @@ __device__ __forceinline__ void front_sweep(const Frame &F, uint8_t *base, uint3
         // and at a record's first run its weight), and the elements: the
         // records of row y-1 (weight 0 at the other slots, which no link
         // names), the runs of row y, the virtual node
+        // (pp[QW + x] then holds run x's record, which its links read)
         const bool out = y - 1 >= wlo;
         for (int x = tid; x < pcur; x += kFrontThreads) {
             const uint32_t r = pp[QW + x];
-            const uint32_t l = pmin[r];
-            const uint32_t wgt = l == (uint32_t)x ? pacc[r] : 0u;
+            const uint32_t l = pmin[r], pw = pacc[r];
+            const uint32_t wgt = l == (uint32_t)x ? pw : 0u;
+            pp[QW + x] = l;
             if (out) {
                 g_rlab[pbase + x] = l;
                 if (l == (uint32_t)x) g_recw[pbase + x] = wgt;
@@ -439,7 +491,10 @@ __device__ __forceinline__ void front_sweep(const Frame &F, uint8_t *base, uint3
             if (mode == kSweepLocal) mp[QW + i] = 0xffffffffu;
             if (i == ncur - 1) lastcol = vlink;
         }
-        if (tid == 0) {
+        // the virtual node on the last thread (the one least likely to hold
+        // a record or a run)
+        const bool vthread = tid == kFrontThreads - 1;
+        if (vthread) {
             p[VN] = VN;
             acc[VN] = 1;
             minrun[VN] = 0xffffffffu;
@@ -450,28 +505,50 @@ __device__ __forceinline__ void front_sweep(const Frame &F, uint8_t *base, uint3
 #pragma unroll
         for (int j = 0; j < kPer; j++) {
             const int e = tid + j * kFrontThreads;
-            if (e < ne) fr_unite<G>(p, (uint32_t)QW + (ee[j] >> 16), pmin[pp[QW + (ee[j] & 0xffffu)]]);
+            if (e < ne) fr_unite_pair<G>(p, (uint32_t)QW + (ee[j] >> 16), pp[QW + (ee[j] & 0xffffu)]);
         }
         for (int e = kStage + tid; e < ne; e += kFrontThreads) {
             const uint32_t ed = edges[e];
-            fr_unite<G>(p, (uint32_t)QW + (ed >> 16), pmin[pp[QW + (ed & 0xffffu)]]);
+            fr_unite_pair<G>(p, (uint32_t)QW + (ed >> 16), pp[QW + (ed & 0xffffu)]);
         }
-        if (lastcol) fr_unite<G>(p, (uint32_t)(QW + ncur - 1), VN);
+        if (lastcol) fr_unite_pair<G>(p, (uint32_t)(QW + ncur - 1), VN);
         bsync<G>();
         // each element under its root: weights into the roots, and the
-        // least run of each set that holds one
-        auto settle = [&](uint32_t x, bool run) {
-            const uint32_t r = fr_find<G>(p, x);
-            if (r != x) {
-                p[x] = r;
-                atomicAdd((uint32_t *)&acc[r], (uint32_t)acc[x]);  // acc of a non-root never changes
-                if (mode == kSweepLocal && !run) atomicMin((uint32_t *)&mp[r], (uint32_t)mp[x]);
+        // least run of each set that holds one. A thread's record x and run
+        // QW + x go together, their reads issued at once (the weight of a
+        // non-root never changes: it may be read before its root is known).
+        const int npair = max(pcur, ncur);
+        for (int x = tid; x < npair; x += kFrontThreads) {
+            const bool hr = x < pcur, hn = x < ncur;
+            // (a thread with one of the two takes it on both sides)
+            const uint32_t er = hr ? (uint32_t)x : (uint32_t)QW + (uint32_t)x;
+            const uint32_t en = hn ? (uint32_t)QW + (uint32_t)x : er;
+            const uint32_t qr = p[er], qn = p[en], wr = acc[er], wn = acc[en];
+            const uint32_t mr = mode == kSweepLocal ? (uint32_t)mp[er] : 0u;
+            uint32_t rr0 = er, rn0 = en;
+            fr_find_pair<G, false>(p, rr0, qr, rn0, qn);
+            if (hr && rr0 != er) {
+                p[er] = rr0;
+                atomicAdd((uint32_t *)&acc[rr0], wr);
+                if (mode == kSweepLocal) atomicMin((uint32_t *)&mp[rr0], mr);
             }
-            if (run) atomicMin((uint32_t *)&minrun[r], x - (uint32_t)QW);
-        };
-        for (int x = tid; x < pcur; x += kFrontThreads) settle((uint32_t)x, false);
-        for (int i = tid; i < ncur; i += kFrontThreads) settle((uint32_t)(QW + i), true);
-        if (tid == 0) settle(VN, false);
+            if (hn) {
+                if (rn0 != en) {
+                    p[en] = rn0;
+                    atomicAdd((uint32_t *)&acc[rn0], wn);
+                }
+                atomicMin((uint32_t *)&minrun[rn0], (uint32_t)x);
+            }
+        }
+        if (vthread) {
+            uint32_t rv = VN, dummy = VN;
+            fr_find_pair<G, false>(p, rv, p[VN], dummy, VN);
+            if (rv != VN) {
+                p[VN] = rv;
+                atomicAdd((uint32_t *)&acc[rv], (uint32_t)acc[VN]);
+                if (mode == kSweepLocal) atomicMin((uint32_t *)&mp[rv], (uint32_t)mp[VN]);
+            }
+        }
         bsync<G>();
         // row y settled: the previous set for row y + 1 (the links of row y
         // read the other set, which row y + 1 rewrites, before the barrier
