@@ -222,12 +222,10 @@ __device__ __forceinline__ void fr_unite(typename Mem<G>::U32 *p, uint32_t a, ui
     }
 }
 
-// The roots of a and b found together, each step's reads of the two paths
-// issued at once: the two dependent chains of LDS reads overlap. qa, qb:
-// the parents of a and b, already read. Halve: path halving (while sets
-// are united); without it the finds only read, so that an element's own
-// write of its root (when the sets are settled) is the only write to it.
-template <bool G, bool Halve>
+// The roots of a and b found together with path halving, each step's reads
+// of the two paths issued at once: the two dependent chains of LDS reads
+// overlap. qa, qb: the parents of a and b, already read.
+template <bool G>
 __device__ __forceinline__ void fr_find_pair(typename Mem<G>::U32 *p, uint32_t &a, uint32_t qa, uint32_t &b,
                                              uint32_t qb) {
     bool da = qa == a, db = qb == b;
@@ -239,7 +237,7 @@ __device__ __forceinline__ void fr_find_pair(typename Mem<G>::U32 *p, uint32_t &
                 a = qa;
                 da = true;
             } else {
-                if (Halve) p[a] = ga;
+                p[a] = ga;
                 a = ga;
             }
         }
@@ -248,7 +246,7 @@ __device__ __forceinline__ void fr_find_pair(typename Mem<G>::U32 *p, uint32_t &
                 b = qb;
                 db = true;
             } else {
-                if (Halve) p[b] = gb;
+                p[b] = gb;
                 b = gb;
             }
         }
@@ -257,11 +255,36 @@ __device__ __forceinline__ void fr_find_pair(typename Mem<G>::U32 *p, uint32_t &
     }
 }
 
+// The roots of N elements a[i] (q[i]: their parents, already read), found
+// together by reads only (one read a path a step, the N issued at once): a
+// settling element's own write of its root is then the only write to it.
+template <bool G, int N>
+__device__ __forceinline__ void fr_roots(typename Mem<G>::U32 *p, uint32_t (&a)[N], uint32_t (&q)[N]) {
+    for (;;) {
+        bool more = false;
+        uint32_t g[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const bool root = q[i] == a[i];
+            g[i] = root ? q[i] : (uint32_t)p[q[i]];
+            more |= !root;
+        }
+        if (!more) return;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            if (q[i] != a[i]) {
+                a[i] = q[i];
+                q[i] = g[i];
+            }
+        }
+    }
+}
+
 // fr_unite with the two finds together (fr_find_pair)
 template <bool G>
 __device__ __forceinline__ void fr_unite_pair(typename Mem<G>::U32 *p, uint32_t a, uint32_t b) {
     for (;;) {
-        fr_find_pair<G, true>(p, a, p[a], b, p[b]);
+        fr_find_pair<G>(p, a, p[a], b, p[b]);
         if (a == b) return;
         if (a < b) {
             const uint32_t t = a;
@@ -523,10 +546,11 @@ __device__ __forceinline__ void front_sweep(const Frame &F, uint8_t *base, uint3
             // (a thread with one of the two takes it on both sides)
             const uint32_t er = hr ? (uint32_t)x : (uint32_t)QW + (uint32_t)x;
             const uint32_t en = hn ? (uint32_t)QW + (uint32_t)x : er;
-            const uint32_t qr = p[er], qn = p[en], wr = acc[er], wn = acc[en];
+            uint32_t rt[2] = {er, en}, qt[2] = {p[er], p[en]};
+            const uint32_t wr = acc[er], wn = acc[en];
             const uint32_t mr = mode == kSweepLocal ? (uint32_t)mp[er] : 0u;
-            uint32_t rr0 = er, rn0 = en;
-            fr_find_pair<G, false>(p, rr0, qr, rn0, qn);
+            fr_roots<G, 2>(p, rt, qt);
+            const uint32_t rr0 = rt[0], rn0 = rt[1];
             if (hr && rr0 != er) {
                 p[er] = rr0;
                 atomicAdd((uint32_t *)&acc[rr0], wr);
@@ -541,8 +565,9 @@ __device__ __forceinline__ void front_sweep(const Frame &F, uint8_t *base, uint3
             }
         }
         if (vthread) {
-            uint32_t rv = VN, dummy = VN;
-            fr_find_pair<G, false>(p, rv, p[VN], dummy, VN);
+            uint32_t rt[1] = {VN}, qt[1] = {p[VN]};
+            fr_roots<G, 1>(p, rt, qt);
+            const uint32_t rv = rt[0];
             if (rv != VN) {
                 p[VN] = rv;
                 atomicAdd((uint32_t *)&acc[rv], (uint32_t)acc[VN]);
@@ -657,8 +682,10 @@ template <int T>
 __device__ void front_carry_loop(const Frame &F, int nb, uint32_t *p, uint32_t *w, uint32_t *mn) {
     const uint32_t QW = (uint32_t)F.qw;
     const int tid = threadIdx.x;
-    auto settle = [&](uint32_t x) {
-        const uint32_t r = fr_find<false>(p, x);
+    auto settle = [&](uint32_t x) {  // (reads only until the element's own root write)
+        uint32_t rt[1] = {x}, qt[1] = {p[x]};
+        fr_roots<false, 1>(p, rt, qt);
+        const uint32_t r = rt[0];
         if (r != x) {
             p[x] = r;
             atomicAdd(&w[r], w[x]);  // (roots no longer change: a non-root's weight is read once)
@@ -731,14 +758,6 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
     const int tid = threadIdx.x;
     const auto g_rlost = gmem(F.rlost), g_rroot = gmem(F.rroot), g_rlab = gmem(F.rlab), g_recw = gmem(F.recw);
     const auto g_row_off = gmem(F.row_off), g_rcnt = gmem(F.rcnt);
-    auto settle = [&](uint32_t x) {
-        const uint32_t r = fr_find<false>(p, x);
-        if (r != x) {
-            p[x] = r;
-            atomicAdd(&w[r], w[x]);  // (roots no longer change: a non-root's weight is read once)
-        }
-        return r;
-    };
     // each band's row above and last row: first slot and run count, all
     // read at once
     __shared__ uint32_t rows[kMaxFrontBands][4];
@@ -829,21 +848,40 @@ __global__ __launch_bounds__(T) void k_front_carry(const Frame *frames, int band
             if (x < n0) {
                 const uint32_t g = cur.g[j], m = mn[2 * QW + g];
                 if (g == x) p[2 * QW + x] = m;
-                if (alab[j] != m) fr_unite<false>(p, alab[j], m);
-                if (g == x && (cur.info[j] & kPortLive)) fr_unite<false>(p, 2 * QW + x, QW + (cur.info[j] & ~kPortLive));
+                if (alab[j] != m) fr_unite_pair<false>(p, alab[j], m);
+                if (g == x && (cur.info[j] & kPortLive))
+                    fr_unite_pair<false>(p, 2 * QW + x, QW + (cur.info[j] & ~kPortLive));
             }
         }
         bsync<false>();
+        // each element under its root (a position's three elements found
+        // together; a missing one takes another's place), the weights of
+        // the non-roots into the roots
 #pragma unroll
         for (int j = 0; j < kR; j++) {
             const uint32_t x = (uint32_t)(tid + j * T);
-            if (x < n0) {
-                settle(x);
-                settle(2 * QW + x);
+            const bool ha = x < n0, hl = x < ne;
+            if (!ha && !hl) continue;
+            const uint32_t ea = ha ? x : QW + x, ec = ha ? 2 * QW + x : QW + x, el = hl ? QW + x : x;
+            uint32_t rt[3] = {ea, ec, el}, qt[3] = {p[ea], p[ec], p[el]};
+            const uint32_t wa = w[ea], wc = w[ec], wl = w[el];
+            fr_roots<false, 3>(p, rt, qt);
+            if (ha) {
+                if (rt[0] != ea) {
+                    p[ea] = rt[0];
+                    atomicAdd(&w[rt[0]], wa);
+                }
+                if (rt[1] != ec) {
+                    p[ec] = rt[1];
+                    atomicAdd(&w[rt[1]], wc);
+                }
             }
-            if (x < ne) {
-                const uint32_t r = settle(QW + x);
-                if (cur.lab[j] == x) atomicMin(&mn[r], x);  // a record of the band: its first run
+            if (hl) {
+                if (rt[2] != el) {
+                    p[el] = rt[2];
+                    atomicAdd(&w[rt[2]], wl);
+                }
+                if (cur.lab[j] == x) atomicMin(&mn[rt[2]], x);  // a record of the band: its first run
             }
         }
         bsync<false>();
