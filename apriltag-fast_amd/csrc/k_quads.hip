@@ -153,14 +153,16 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
         float cx = (float)((double)(xmin + xmax) * 0.5 + 0.05118);
         float cy = (float)((double)(ymin + ymax) * 0.5 + -0.028581);
         // sort items (reverse point order) and the full keys of the points
-        for (uint32_t i = t; i < n; i += NT) {
+        // (a rejected cluster's keys are never read: a third of the points
+        // lie in the few clusters over max_px)
+        for (uint32_t i = t; i < n && (ok || items); i += NT) {
             uint32_t slope = 0;
             if (ok) {
                 int x, y, gx, gy;
                 pt_unpack(pt_at(i), &x, &y, &gx, &gy);
                 slope = point_slope(__fsub_rn((float)x, cx), __fsub_rn((float)y, cy));
+                gmem(F.okey)[s0 + i] = ((uint64_t)slope << 32) | (uint32_t)~i;
             }
-            gmem(F.okey)[s0 + i] = ((uint64_t)slope << 32) | (uint32_t)~i;
             if (items) {  // for the sort of all points at once
                 uint32_t pos = s0 + (n - 1 - i);
                 gmem(F.skey)[pos] = ((uint64_t)c << 32) | slope;
@@ -186,8 +188,18 @@ __global__ __launch_bounds__(NT) void k_fit_prep(const Frame *frames, const DevP
             uint32_t len = min((uint32_t)kDotChunk, nv - c0);
             for (uint32_t j = t; j < len; j += NT) terms[j] = term(pt_at(c0 + j));
             __syncthreads();
-            if (t < 8)
-                for (uint32_t j = t; j < len; j += 8) acc = __fadd_rn(acc, terms[j]);
+            if (t < 8) {
+                // in order, 8 terms a step read at once (one LDS wait a step)
+                uint32_t j = t;
+                for (; j + 56 < len; j += 64) {
+                    float v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) v[q] = terms[j + 8 * q];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) acc = __fadd_rn(acc, v[q]);
+                }
+                for (; j < len; j += 8) acc = __fadd_rn(acc, terms[j]);
+            }
             __syncthreads();
         }
         if (t < 8) lanes8[t] = acc;
