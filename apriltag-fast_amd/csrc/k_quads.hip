@@ -711,16 +711,22 @@ __device__ __forceinline__ void reg_half_clean(uint64_t (&v)[8]) {
 
 // One step of the network whose partners sit in other waves: through LDS
 // (sk, kSortTile keys), partner p = position ^ x.
-__device__ __forceinline__ void lds_xor_step(uint64_t (&v)[8], uint64_t *sk, uint32_t x) {
+// act: the wave takes part (a wave whose keys lie beyond every pair of
+// the step only meets the barriers)
+__device__ __forceinline__ void lds_xor_step(uint64_t (&v)[8], uint64_t *sk, uint32_t x, bool act = true) {
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (act) {
 #pragma unroll
-    for (int r = 0; r < 8; r++) sk[w * 512 + r * 64 + lane] = v[r];
+        for (int r = 0; r < 8; r++) sk[w * 512 + r * 64 + lane] = v[r];
+    }
     __syncthreads();
+    if (act) {
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-        const uint32_t i = w * 512 + r * 64 + lane, p = i ^ x;
-        const uint64_t o = sk[p];
-        v[r] = i < p ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
+        for (int r = 0; r < 8; r++) {
+            const uint32_t i = w * 512 + r * 64 + lane, p = i ^ x;
+            const uint64_t o = sk[p];
+            v[r] = i < p ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
+        }
     }
     __syncthreads();
 }
@@ -836,23 +842,28 @@ __global__ __launch_bounds__(1024) void k_angle_sort_big(const Frame *frames, co
         if (n <= kHugeCluster || n > kBigSort) continue;
         uint32_t m = 1024;
         while (m < n) m <<= 1;
+        // the waves of keys [0, m): every step pairs keys below m, so the
+        // waves beyond (padding only, sorted) just meet the barriers
+        const bool act = w * 512 < m;
         uint64_t v[8];
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
             v[r] = i < n ? gmem(F.okey)[s0 + i] : ~0ull;
         }
-        wave_bitonic<8>(v);
+        if (act) wave_bitonic<8>(v);
         for (uint32_t k = 1024; k <= m; k <<= 1) {
-            lds_xor_step(v, skb, k - 1);  // the merge's mirror step
-            for (uint32_t j = k >> 2; j >= 512; j >>= 1) lds_xor_step(v, skb, j);
-            reg_half_clean(v);
+            lds_xor_step(v, skb, k - 1, act);  // the merge's mirror step
+            for (uint32_t j = k >> 2; j >= 512; j >>= 1) lds_xor_step(v, skb, j, act);
+            if (act) reg_half_clean(v);
         }
         // the tie fix on the sorted keys, in LDS (one segment of 8192)
+        if (act) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
-            skb[i] = v[r];
+            for (int r = 0; r < 8; r++) {
+                const uint32_t i = w * 512 + (uint32_t)r * 64 + lane;
+                skb[i] = v[r];
+            }
         }
         __syncthreads();
         fix_runs_block<1024>(skb, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
