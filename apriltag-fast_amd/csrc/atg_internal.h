@@ -211,6 +211,21 @@ struct WorkQueue {
     uint32_t *off[kLists];   // [nframes + 1] each
 };
 
+// The frame of entry g of a work list: the last f with off[f] <= g (off[]
+// ascending, off[nframes] = the list's total > g), by binary search from a
+// frame lo with off[lo] <= g (a block's entries ascend, so its last frame
+// is a valid start). A persistent block's stride crosses many frames at
+// once: a search costs log2(nframes) reads where a walk costs one a frame.
+__device__ __forceinline__ uint32_t frame_of(const uint32_t *off, uint32_t nframes, uint32_t g, uint32_t lo) {
+    uint32_t hi = nframes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Work splits of the kernel chain (the autotuner's choices); any values
 // give the same results.
 struct Tune {

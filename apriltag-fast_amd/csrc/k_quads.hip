@@ -602,7 +602,7 @@ __device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue 
         const uint32_t th = Q->total[4];
         if (g < th) {
             const uint32_t *off = Q->off[4];
-            while (off[cur.fh + 1] <= g) cur.fh++;
+            cur.fh = frame_of(off, Q->nframes, g, cur.fh);
             *fi = cur.fh;
             *c = frames[cur.fh].klist[g - off[cur.fh]];
             return true;
@@ -613,7 +613,7 @@ __device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue 
     if (l == 5) l = 0;
     if (g >= Q->total[l]) return false;
     const uint32_t *off = Q->off[l];
-    while (off[cur.fl + 1] <= g) cur.fl++;
+    cur.fl = frame_of(off, Q->nframes, g, cur.fl);
     const Frame &F = frames[cur.fl];
     *fi = cur.fl;
     *c = F.klist[(l ? F.kcap : F.counters[kNHuge]) + (g - off[cur.fl])];
