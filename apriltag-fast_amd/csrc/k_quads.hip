@@ -926,12 +926,20 @@ __global__ __launch_bounds__(64, 5) void k_angle_sort_wave(const Frame *frames, 
 // ------------------------------------------------------------ moments
 
 // Lane q < 6 of a wave: running sum q continued from s over the terms
-// a[0, len) (LDS), each partial sum stored to out[0, len) (the cluster's
-// moment array q in HBM, straight from the chain: no LDS write-back on the
-// dependent path). Each of the kRing registers takes the term kRing ahead
-// as soon as its add has used it. Returns the sum.
-__device__ __forceinline__ double scan_chunk(const double *a, uint32_t len, double s, double *out_p) {
-    auto out = gmem(out_p);
+// a[0, len) (LDS), each partial sum stored to out[0, len): in place over
+// its term (InPlace: a position is rewritten only after its term was read
+// into the ring; the block then copies the sums out with whole-wave
+// stores), or the cluster's moment array q in HBM straight from the chain
+// (no LDS writes beside the ring's reads: faster where the chain itself is
+// the critical path, the long chunks of single frames). Each of the kRing
+// registers takes the term kRing ahead as soon as its add has used it.
+// Returns the sum.
+template <bool InPlace>
+__device__ __forceinline__ double scan_chunk(double *a, uint32_t len, double s, double *out_p) {
+    auto out = [&]() {
+        if constexpr (InPlace) return a;
+        else return gmem(out_p);
+    }();
     constexpr uint32_t kRing = ATG_SCAN_RING;
     static_assert(kRing % 2 == 0, "pairs of sums a store");
     uint32_t j = 0;
@@ -1028,6 +1036,22 @@ __device__ __forceinline__ void moment_terms(const Frame &F, uint32_t s0, const 
     term_values<KP, CH>(pwv, len, b, t0, stride);
 }
 
+// The running sums of positions j = t0, t0 + stride, ... (KP of them, below
+// len) of a chunk at c0 from LDS to the cluster's six moment arrays
+// (array q at mom + q pc), consecutive positions on consecutive lanes.
+template <int KP, int CH>
+__device__ __forceinline__ void sums_out(const double (*b)[CH + 2], uint32_t len, double *mom, int64_t pc, uint32_t c0,
+                                         int t0, int stride) {
+    auto m = gmem(mom);
+#pragma unroll
+    for (int r = 0; r < KP; r++) {
+        const uint32_t j = (uint32_t)(t0 + r * stride);
+        if (j >= len) break;
+#pragma unroll
+        for (int q = 0; q < 6; q++) m[(int64_t)q * pc + c0 + j] = b[q][j];
+    }
+}
+
 // A barrier of the block that orders LDS only (loads from HBM issued before
 // it stay in flight).
 __device__ __forceinline__ void lds_barrier() {
@@ -1045,6 +1069,9 @@ template <int NT, int CH>
 __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Frame *frames, int parity, const WorkQueue *Q,
                                                    int keys_ready, int l, int fix) {
     constexpr int kBufs = NT > 64 ? 2 : 1;
+    // the sums in place and copied out by the block (short chunks of many
+    // clusters at once), or stored by the chain lanes
+    constexpr bool kInPlace = kBufs == 2 && CH <= 128;
     // rows 2 doubles apart in the banks: the scan's lanes conflict-free
     __shared__ double sm[kBufs][6][CH + 2];
     __shared__ uint32_t fls[4 * NT + 1], flm[4 * NT + 1], fcnt;
@@ -1087,7 +1114,8 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
         // the six running sums (fast/quads.c:1006-1035), stored by the
         // chain lanes into the moment arrays
         double run = 0.0;  // lane q < 6: running sum q
-        double *out = F.mom + (t < 6 ? t : 0) * pc + s0;
+        double *mom = F.mom + s0;
+        double *out = mom + (t < 6 ? t : 0) * pc;
         const uint32_t nch = (n + CH - 1) / CH;
         if constexpr (kBufs == 2) {
             // the other waves keep their loads ahead of the chain: the
@@ -1106,22 +1134,33 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
             }
             __syncthreads();
             for (uint32_t ci = 0; ci < nch; ci++) {
-                const uint32_t c0 = ci * CH, len = clen(ci);
+                const uint32_t len = clen(ci);
                 if (w == 0) {
-                    if (t < 6) run = scan_chunk(sm[ci & 1][t], len, run, out + c0);
-                } else if (ci + 1 < nch) {
-                    term_values<KP, CH>(pwv, clen(ci + 1), sm[(ci + 1) & 1], th, sh);
-                    term_words<KP>(F, s0, clen(ci + 2), th, sh, ix, pwv);
-                    term_points<KP>(keys, (ci + 3) * CH, clen(ci + 3), th, sh, ix);
+                    if (t < 6) run = scan_chunk<kInPlace>(sm[ci & 1][t], len, run, out + ci * CH);
+                } else {
+                    // the sums of the chunk before out of the other buffer,
+                    // each thread its own positions, which it then refills
+                    // with the next chunk's terms
+                    if (kInPlace && ci > 0)
+                        sums_out<KP, CH>(sm[(ci + 1) & 1], clen(ci - 1), mom, pc, (ci - 1) * CH, th, sh);
+                    if (ci + 1 < nch) {
+                        term_values<KP, CH>(pwv, clen(ci + 1), sm[(ci + 1) & 1], th, sh);
+                        term_words<KP>(F, s0, clen(ci + 2), th, sh, ix, pwv);
+                        term_points<KP>(keys, (ci + 3) * CH, clen(ci + 3), th, sh, ix);
+                    }
                 }
                 lds_barrier();
+            }
+            if (kInPlace && nch > 0) {
+                sums_out<(CH + NT - 1) / NT, CH>(sm[(nch - 1) & 1], clen(nch - 1), mom, pc, (nch - 1) * CH, t, NT);
+                __syncthreads();  // the buffers are the next cluster's
             }
         } else {
             for (uint32_t ci = 0; ci < nch; ci++) {
                 const uint32_t c0 = ci * CH, len = min((uint32_t)CH, n - c0);
                 moment_terms<(CH + NT - 1) / NT, CH>(F, s0, keys, c0, len, sm[0], t, NT);
                 __syncthreads();
-                if (t < 6) run = scan_chunk(sm[0][t], len, run, out + c0);
+                if (t < 6) run = scan_chunk<false>(sm[0][t], len, run, out + c0);
                 __syncthreads();
             }
         }
