@@ -1209,27 +1209,49 @@ __global__ __launch_bounds__(64) void k_fit_moments_packed(const Frame *frames, 
         const int ck = lane / 6, cq = lane % 6;
         const bool chain = lane < 6 * G;
         double run = 0.0;
-        for (uint32_t c0 = 0; c0 < maxlen; c0 += C) {
-            // terms of (cluster k, position c0 + j), item k C + j, loads first
-            constexpr int kItems = G * C / 64;
-            uint64_t pwv[kItems];
-            uint32_t ix[kItems];
+        // terms of (cluster k, position c0 + j), item k C + j: the points'
+        // indices two chunks ahead and their words one chunk ahead, so that
+        // a chunk's loads are in flight through the chunk before it
+        constexpr int kItems = G * C / 64;
+        auto load_ix = [&](uint32_t c0, uint32_t (&ix)[kItems]) {
 #pragma unroll
             for (int r = 0; r < kItems; r++) {
                 const int it = lane + 64 * r, k = it / C, j = it % C;
                 ix[r] = c0 + (uint32_t)j < gn[k] ? (uint32_t)~(uint32_t)gmem(gkey[k])[c0 + j] : 0xffffffffu;
             }
+        };
+        auto load_pw = [&](const uint32_t (&ix)[kItems], uint64_t (&pwv)[kItems]) {
 #pragma unroll
             for (int r = 0; r < kItems; r++) {
                 const int it = lane + 64 * r, k = it / C;
                 pwv[r] = ix[r] != 0xffffffffu ? gmem(gpw[k])[ix[r]] : 0ull;
             }
+        };
+        uint32_t ix[kItems], ixn[kItems];
+        uint64_t pwv[kItems];
+        load_ix(0, ix);
+        load_pw(ix, pwv);
+        load_ix(C, ixn);
+        for (uint32_t c0 = 0; c0 < maxlen; c0 += C) {
+            uint32_t ixc[kItems];
+            uint64_t pwc[kItems];
+#pragma unroll
+            for (int r = 0; r < kItems; r++) {
+                ixc[r] = ix[r];
+                pwc[r] = pwv[r];
+            }
+            if (c0 + C < maxlen) {
+#pragma unroll
+                for (int r = 0; r < kItems; r++) ix[r] = ixn[r];
+                load_pw(ix, pwv);
+                load_ix(c0 + 2 * C, ixn);
+            }
 #pragma unroll
             for (int r = 0; r < kItems; r++) {
                 const int it = lane + 64 * r, k = it / C, j = it % C;
-                if (ix[r] == 0xffffffffu) continue;
-                const int px = (int)(pwv[r] & 0xffffu), py = (int)((pwv[r] >> 16) & 0xffffu);
-                const uint32_t g2 = (uint32_t)(pwv[r] >> 32);
+                if (ixc[r] == 0xffffffffu) continue;
+                const int px = (int)(pwc[r] & 0xffffu), py = (int)((pwc[r] >> 16) & 0xffffu);
+                const uint32_t g2 = (uint32_t)(pwc[r] >> 32);
                 const double wgt = __dadd_rn(__dsqrt_rn((double)g2), 1.0);
                 const double fx = __dadd_rn(__dmul_rn((double)px, 0.5), 0.5);
                 const double fy = __dadd_rn(__dmul_rn((double)py, 0.5), 0.5);
