@@ -1068,12 +1068,15 @@ struct Ents {
             e[2 * x + 1] = w;
         }
     }
+    // a root x hooked under p: its weight is never read again (a find
+    // returns its root's entry), so the packed word is written, not merged
     __device__ __forceinline__ void set_par(uint32_t x, uint32_t p) const {
-        if constexpr (PK) e[x] = (p << kWB) | (e[x] & kWM);
+        if constexpr (PK) e[x] = p << kWB;
         else e[2 * x] = p;
     }
+    // the weight of a root x (its own parent): the packed word written whole
     __device__ __forceinline__ void set_wt(uint32_t x, uint32_t w) const {
-        if constexpr (PK) e[x] = (e[x] & ~kWM) | w;
+        if constexpr (PK) e[x] = (x << kWB) | w;
         else e[2 * x + 1] = w;
     }
     // find with path halving; returns the root's entry with .x = the root
