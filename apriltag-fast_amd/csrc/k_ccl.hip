@@ -1185,38 +1185,82 @@ __device__ __forceinline__ void replay_row(const Frame &F, int y, uint8_t *base,
     };
     for (int g = lane; g < ng; g += 64) head[g] = 0;
     wsync<G>();
-    uint32_t edn = lane < ne ? edges[lane] : 0u;
-    for (int b0 = 0; b0 < ne; b0 += 64) {
-        const int e = b0 + lane;
-        const bool valid = e < ne;
-        const uint32_t ed = edn;
-        edn = e + 64 < ne ? edges[e + 64] : 0u;
-        const uint32_t g = valid ? (uint32_t)nxt[ed >> 16] : 0u;
-        if (valid) hasl[labA[ed & 0xffffu]] = 1;
-        const uint64_t peers = group_peers(g, valid);
-        if (valid && !(peers & lt)) head[g] = (uint32_t)head[g] + (uint32_t)__popcll(peers);
+    auto scan_heads = [&]() {  // each group's count to its first link
+        for (int g0 = 0, carry = 0; g0 < ng; g0 += 64) {
+            const int g = g0 + lane;
+            const uint32_t c = g < ng ? (uint32_t)head[g] : 0u;
+            const uint32_t inc = wave_scan_incl(c);
+            if (g < ng) head[g] = (uint32_t)carry + inc - c;
+            carry += __builtin_amdgcn_readlane((int)inc, 63);
+        }
         wsync<G>();
-    }
-    for (int g0 = 0, carry = 0; g0 < ng; g0 += 64) {
-        const int g = g0 + lane;
-        const uint32_t c = g < ng ? (uint32_t)head[g] : 0u;
-        const uint32_t inc = wave_scan_incl(c);
-        if (g < ng) head[g] = (uint32_t)carry + inc - c;
-        carry += __builtin_amdgcn_readlane((int)inc, 63);
-    }
-    wsync<G>();
-    edn = lane < ne ? edges[lane] : 0u;
-    for (int b0 = 0; b0 < ne; b0 += 64) {
-        const int e = b0 + lane;
-        const bool valid = e < ne;
-        const uint32_t ed = edn;
-        edn = e + 64 < ne ? edges[e + 64] : 0u;
-        const uint32_t g = valid ? (uint32_t)nxt[ed >> 16] : 0u;
-        const uint64_t peers = group_peers(g, valid);
-        if (valid) le[(uint32_t)head[g] + (uint32_t)__popcll(peers & lt)] = (ed & 0xffff0000u) | labA[ed & 0xffffu];
-        wsync<G>();
-        if (valid && !(peers & lt)) head[g] = (uint32_t)head[g] + (uint32_t)__popcll(peers);
-        wsync<G>();
+    };
+    // Rows of up to kRegChunks chunks of links (every row of the first
+    // launch): the links, their groups and records and each chunk's peer
+    // masks stay in registers from the count to the placement, the loads
+    // of all chunks (HBM, then LDS) issued at once. Other rows: each pass
+    // streams the links, a chunk loaded ahead.
+    constexpr int kRegChunks = 10;
+    if (ne <= 64 * kRegChunks) {
+        uint32_t eds[kRegChunks], gs[kRegChunks], las[kRegChunks];
+        uint64_t prs[kRegChunks];
+#pragma unroll
+        for (int c = 0; c < kRegChunks; c++) {
+            const int e = 64 * c + lane;
+            eds[c] = e < ne ? (uint32_t)edges[e] : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < kRegChunks; c++) {
+            const bool valid = 64 * c + lane < ne;
+            gs[c] = valid ? (uint32_t)nxt[eds[c] >> 16] : 0u;
+            las[c] = valid ? (uint32_t)labA[eds[c] & 0xffffu] : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < kRegChunks; c++) {
+            if (64 * c >= ne) break;
+            const bool valid = 64 * c + lane < ne;
+            if (valid) hasl[las[c]] = 1;
+            prs[c] = group_peers(gs[c], valid);
+            if (valid && !(prs[c] & lt)) head[gs[c]] = (uint32_t)head[gs[c]] + (uint32_t)__popcll(prs[c]);
+            wsync<G>();
+        }
+        scan_heads();
+#pragma unroll
+        for (int c = 0; c < kRegChunks; c++) {
+            if (64 * c >= ne) break;
+            const bool valid = 64 * c + lane < ne;
+            if (valid) le[(uint32_t)head[gs[c]] + (uint32_t)__popcll(prs[c] & lt)] = (eds[c] & 0xffff0000u) | las[c];
+            wsync<G>();
+            if (valid && !(prs[c] & lt)) head[gs[c]] = (uint32_t)head[gs[c]] + (uint32_t)__popcll(prs[c]);
+            wsync<G>();
+        }
+    } else {
+        uint32_t edn = lane < ne ? edges[lane] : 0u;
+        for (int b0 = 0; b0 < ne; b0 += 64) {
+            const int e = b0 + lane;
+            const bool valid = e < ne;
+            const uint32_t ed = edn;
+            edn = e + 64 < ne ? edges[e + 64] : 0u;
+            const uint32_t g = valid ? (uint32_t)nxt[ed >> 16] : 0u;
+            if (valid) hasl[labA[ed & 0xffffu]] = 1;
+            const uint64_t peers = group_peers(g, valid);
+            if (valid && !(peers & lt)) head[g] = (uint32_t)head[g] + (uint32_t)__popcll(peers);
+            wsync<G>();
+        }
+        scan_heads();
+        edn = lane < ne ? edges[lane] : 0u;
+        for (int b0 = 0; b0 < ne; b0 += 64) {
+            const int e = b0 + lane;
+            const bool valid = e < ne;
+            const uint32_t ed = edn;
+            edn = e + 64 < ne ? edges[e + 64] : 0u;
+            const uint32_t g = valid ? (uint32_t)nxt[ed >> 16] : 0u;
+            const uint64_t peers = group_peers(g, valid);
+            if (valid) le[(uint32_t)head[g] + (uint32_t)__popcll(peers & lt)] = (ed & 0xffff0000u) | labA[ed & 0xffffu];
+            wsync<G>();
+            if (valid && !(peers & lt)) head[g] = (uint32_t)head[g] + (uint32_t)__popcll(peers);
+            wsync<G>();
+        }
     }
     auto gstart = [&](uint32_t g) -> uint32_t { return g ? (uint32_t)head[g - 1] : 0u; };
     // the groups that have links, listed (in labA, free by now), so that
