@@ -226,6 +226,23 @@ __device__ __forceinline__ uint32_t frame_of(const uint32_t *off, uint32_t nfram
     return lo;
 }
 
+// frame_of for an entry g the same in every lane of the wave (every lane
+// takes part): a 64-way search, each lane reading one candidate frame's
+// offset and a ballot keeping the last at or below g, two rounds of reads
+// for up to 4096 frames where the binary search takes log2(nframes).
+__device__ __forceinline__ uint32_t frame_of_wave(const uint32_t *off, uint32_t nframes, uint32_t g, uint32_t lo) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t hi = nframes;  // off[lo] <= g < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63u) >> 6;
+        const uint32_t idx = lo + lane * step;
+        const uint64_t at_or_below = __ballot(idx < hi && off[idx] <= g);  // a prefix of the lanes, lane 0 set
+        lo += (uint32_t)(63 - __clzll((long long)at_or_below)) * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
 // Work splits of the kernel chain (the autotuner's choices); any values
 // give the same results.
 struct Tune {

@@ -1384,7 +1384,7 @@ __global__ __launch_bounds__(64) void k_replay_wide(const Frame *frames, const W
     const uint32_t *off = Q->off[3];
     uint32_t fi = 0;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        fi = frame_of(off, Q->nframes, g, fi);
+        fi = frame_of_wave(off, Q->nframes, g, fi);
         const Frame &F = frames[fi];
         const int y = (int)F.rwide[g - off[fi]];
         if (replay_fits(F, y, cap, lcap) && !(pcap && replay_fits(F, y, pcap, plcap)))

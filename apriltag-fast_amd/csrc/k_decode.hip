@@ -353,7 +353,7 @@ __global__ __launch_bounds__(NT) void k_refine(const Frame *frames, const DevPar
     const uint32_t *off = Q->off[2];
     uint32_t fi = 0;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (off[fi + 1] <= g) fi++;
+        fi = frame_of_wave(off, Q->nframes, g, fi);
         const Frame &F = frames[fi];
         const uint32_t qi = g - off[fi];
         View v = {F.img, F.stride, F.W, F.H};
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(256) void k_lines(const Frame *frames, const WorkQu
     const uint32_t *off = Q->off[2];
     uint32_t fi = 0;
     for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < total; g += gridDim.x * 256) {
-        while (4 * off[fi + 1] <= g) fi++;
+        fi = frame_of(off, Q->nframes, g >> 2, fi);  // (4 off[f] <= g iff off[f] <= g / 4)
         double *ln = frames[fi].qline + (size_t)(g - 4 * off[fi]) * 4;
         const float th2 = cr::atan2f_cr((float)ln[2], (float)ln[3]);
         const double theta = DMUL(0.5, (double)th2);
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 4 : 1) void k_decode(const Frame *f
     const uint32_t *off = Q->off[2];
     uint32_t fi = 0;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (off[fi + 1] <= g) fi++;
+        fi = frame_of_wave(off, Q->nframes, g, fi);
         const Frame &F = frames[fi];
         const uint32_t qi = g - off[fi];
         View v = {F.img, F.stride, F.W, F.H};

@@ -596,13 +596,16 @@ __global__ __launch_bounds__(256) void k_keep_lists(const Frame *frames) {
 struct TakeCursor {
     uint32_t fh = 0, fl = 0;
 };
+// Wave: g is the same in every lane of the wave, which all take part (the
+// frame found by the wave's 64-way search); otherwise by each lane alone.
+template <bool Wave = true>
 __device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue *Q, int l, uint32_t g,
                                            TakeCursor &cur, uint32_t *fi, uint32_t *c) {
     if (l == 0 || l == 4) {
         const uint32_t th = Q->total[4];
         if (g < th) {
             const uint32_t *off = Q->off[4];
-            cur.fh = frame_of(off, Q->nframes, g, cur.fh);
+            cur.fh = Wave ? frame_of_wave(off, Q->nframes, g, cur.fh) : frame_of(off, Q->nframes, g, cur.fh);
             *fi = cur.fh;
             *c = frames[cur.fh].klist[g - off[cur.fh]];
             return true;
@@ -613,7 +616,7 @@ __device__ __forceinline__ bool cluster_at(const Frame *frames, const WorkQueue 
     if (l == 5) l = 0;
     if (g >= Q->total[l]) return false;
     const uint32_t *off = Q->off[l];
-    cur.fl = frame_of(off, Q->nframes, g, cur.fl);
+    cur.fl = Wave ? frame_of_wave(off, Q->nframes, g, cur.fl) : frame_of(off, Q->nframes, g, cur.fl);
     const Frame &F = frames[cur.fl];
     *fi = cur.fl;
     *c = F.klist[(l ? F.kcap : F.counters[kNHuge]) + (g - off[cur.fl])];
@@ -902,7 +905,7 @@ __global__ __launch_bounds__(64, 5) void k_angle_sort_wave(const Frame *frames, 
     const int lane = threadIdx.x;
     uint32_t fi = 0;
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (off[fi + 1] <= g) fi++;
+        fi = frame_of_wave(off, Q->nframes, g, fi);
         const Frame &F = frames[fi];
         const uint32_t c = F.klist[F.kcap + (g - off[fi])];
         const uint32_t s0 = F.cstart[c], n = F.cstart[c + 1] - s0;
@@ -1193,7 +1196,7 @@ __global__ __launch_bounds__(64) void k_fit_moments_packed(const Frame *frames, 
     for (uint32_t g0 = blockIdx.x * G; g0 < total; g0 += gridDim.x * G) {
         if (lane < G) {
             uint32_t fi = 0, c = 0;
-            const bool ok = g0 + lane < total && cluster_at(frames, Q, l, g0 + lane, cur, &fi, &c);
+            const bool ok = g0 + lane < total && cluster_at<false>(frames, Q, l, g0 + lane, cur, &fi, &c);
             const Frame &F = frames[ok ? fi : 0];
             const uint32_t s0 = ok ? F.cstart[c] : 0u;
             gn[lane] = ok ? F.cstart[c + 1] - s0 : 0u;
