@@ -419,11 +419,27 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
     uint32_t s0, s1;
     seg_of(n, blockIdx.x, &s0, &s1);
     const uint64_t lt = (1ull << lane) - 1ull;
+    // the chunks' loads ahead of their use: a ranked word's slot two chunks
+    // ahead, each word (and value) one chunk ahead, the point a gather moves
+    // at the top of its chunk (read during the ranking)
+    auto slot_at = [&](uint32_t i) -> uint32_t { return ranks && i < s1 ? F.sval[i] : 0u; };
+    auto word_at = [&](uint32_t i, uint32_t slot) -> uint64_t {
+        if (i >= s1) return 0ull;
+        return ranks ? ((uint64_t)F.hrank[slot] | ((uint64_t)i << sb)) : B.kin[i];
+    };
+    auto val_at = [&](uint32_t i) -> uint32_t { return i < s1 && !packed ? (iota ? i : B.vin[i]) : 0u; };
+    uint64_t key_n = word_at(s0 + t, slot_at(s0 + t));
+    uint32_t val_n = val_at(s0 + t);
+    uint32_t slot_n2 = slot_at(s0 + 256 + t);
     for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
         const uint32_t i = c0 + t;
         const bool valid = i < s1;
-        const uint64_t key = !valid ? 0ull : ranks ? rank_word(F, i, sb) : B.kin[i];
-        const uint32_t val = (valid && !packed) ? (iota ? i : B.vin[i]) : 0u;
+        const uint64_t key = key_n;
+        const uint32_t val = val_n;
+        const uint64_t pt = valid && to_points ? F.pts[(uint32_t)(key >> sb)] : 0ull;
+        key_n = word_at(i + 256, slot_n2);
+        val_n = val_at(i + 256);
+        slot_n2 = slot_at(i + 512);
         const uint32_t d = valid ? (uint32_t)((key & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
         // the wave's runs of equal digits on adjacent lanes; where every
         // digit of the wave is one run, a lane's rank is its place in the
@@ -460,7 +476,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
             uint32_t pos = off[d] + rank;
             for (int k = 0; k < w; k++) pos += wcnt[k][d];
             if (to_points) {
-                F.cpts[pos] = F.pts[(uint32_t)(key >> sb)];
+                F.cpts[pos] = pt;
             } else {
                 B.kout[pos] = key;
                 if (!packed) B.vout[pos] = val;
