@@ -1497,20 +1497,41 @@ __global__ __launch_bounds__(256) void k_labels(const Frame *frames) {
     const int xmax = F.qw - 2;
     int count = 0;
     uint32_t carry_t = 0;
-    for (int x0 = 0; x0 <= xmax; x0 += 64) {
-        int x = x0 + lane;
-        bool valid = x <= xmax;
-        uint32_t tv = valid ? trow[x] : 0u;
-        const uint32_t tp = wave_from_left(tv, carry_t);
-        bool st = valid && (x == 0 || tv != tp) && tv != 127u;
-        uint64_t sm = __ballot(st);
-        uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-        if (valid) {
-            int idx = count + __popcll(sm & le) - 1;
-            out[x] = tv == 127u ? 0u : nl[idx];
+    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    // kG chunks of 64 pixels at a time: their bytes loaded at once, each
+    // pixel's run index from the chunks' ballots in order, then the runs'
+    // labels gathered at once (two rounds of reads for 1024 pixels)
+    constexpr int kG = 16;
+    for (int g0 = 0; g0 <= xmax; g0 += 64 * kG) {
+        uint32_t tv[kG], lab[kG];
+        int idx[kG];
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const int x = g0 + 64 * j + lane;
+            tv[j] = x <= xmax ? trow[x] : 0u;
         }
-        count += __popcll(sm);
-        carry_t = (uint32_t)__builtin_amdgcn_readlane((int)tv, 63);
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            if (g0 + 64 * j > xmax) break;
+            const int x = g0 + 64 * j + lane;
+            const bool valid = x <= xmax;
+            const uint32_t tp = wave_from_left(tv[j], carry_t);
+            const bool st = valid && (x == 0 || tv[j] != tp) && tv[j] != 127u;
+            const uint64_t sm = __ballot(st);
+            idx[j] = count + __popcll(sm & le) - 1;
+            count += __popcll(sm);
+            carry_t = (uint32_t)__builtin_amdgcn_readlane((int)tv[j], 63);
+        }
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const int x = g0 + 64 * j + lane;
+            lab[j] = x <= xmax && tv[j] != 127u ? nl[idx[j]] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const int x = g0 + 64 * j + lane;
+            if (x <= xmax) out[x] = lab[j];
+        }
     }
     if (lane == 0) {
         out[F.qw - 1] = F.nlab[F.row_off[F.qh] + y];
