@@ -934,20 +934,58 @@ __global__ __launch_bounds__(256) void k_front_dense(const Frame *frames) {
     const uint32_t b = F.row_off[y], n = F.rcnt[y];
     const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t carry = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
-        const uint32_t i = c0 + (uint32_t)lane;
-        const bool first = i < n && F.rlab[b + i] == i;
-        const uint64_t fb = __ballot(first);
-        const uint32_t d = carry + (uint32_t)__popcll(fb & lt);
-        const uint32_t w = first ? F.recw[b + i] : 0u;  // read before any lane of the chunk writes
-        if (first) {
-            F.rroot[b + i] = d;
-            F.recw[b + d] = w;  // d <= i: slots already read
+    constexpr int kG = 16;  // rows of up to 1024 runs: every chunk's reads at once
+    if (n <= 64u * kG) {
+        uint32_t lab[kG], wv[kG], dn[kG];
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const uint32_t i = 64u * j + (uint32_t)lane;
+            lab[j] = i < n ? F.rlab[b + i] : 0xffffffffu;
+            wv[j] = i < n ? F.recw[b + i] : 0u;  // (every read before any write)
         }
-        carry += (uint32_t)__popcll(fb);
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const uint32_t i = 64u * j + (uint32_t)lane;
+            const uint64_t fb = __ballot(lab[j] == i);
+            dn[j] = carry + (uint32_t)__popcll(fb & lt);
+            carry += (uint32_t)__popcll(fb);
+        }
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const uint32_t i = 64u * j + (uint32_t)lane;
+            if (lab[j] == i) {
+                F.rroot[b + i] = dn[j];
+                F.recw[b + dn[j]] = wv[j];
+            }
+        }
+        __threadfence_block();
+        uint32_t nl[kG];
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const uint32_t i = 64u * j + (uint32_t)lane;
+            nl[j] = i < n ? F.rroot[b + lab[j]] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const uint32_t i = 64u * j + (uint32_t)lane;
+            if (i < n) F.rlab[b + i] = nl[j];
+        }
+    } else {
+        for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+            const uint32_t i = c0 + (uint32_t)lane;
+            const bool first = i < n && F.rlab[b + i] == i;
+            const uint64_t fb = __ballot(first);
+            const uint32_t d = carry + (uint32_t)__popcll(fb & lt);
+            const uint32_t w = first ? F.recw[b + i] : 0u;  // read before any lane of the chunk writes
+            if (first) {
+                F.rroot[b + i] = d;
+                F.recw[b + d] = w;  // d <= i: slots already read
+            }
+            carry += (uint32_t)__popcll(fb);
+        }
+        __threadfence_block();
+        for (uint32_t i = (uint32_t)lane; i < n; i += 64) F.rlab[b + i] = F.rroot[b + F.rlab[b + i]];
     }
-    __threadfence_block();
-    for (uint32_t i = (uint32_t)lane; i < n; i += 64) F.rlab[b + i] = F.rroot[b + F.rlab[b + i]];
     if (lane == 0) F.nrec[y] = carry;
 }
 
