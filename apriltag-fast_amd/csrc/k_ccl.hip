@@ -1470,10 +1470,31 @@ __global__ __launch_bounds__(256) void k_root_jump(const Frame *frames) {
     if (y < 1 || y >= F.qh) return;
     const uint32_t cb = F.row_off[y];
     const int n = (int)F.nrec[y];
+    constexpr int kG = 16;  // rows of up to 1024 records: each hop's reads of all records at once
+    if (n <= 64 * kG) {
+        uint32_t v[kG];
+        bool sym[kG];
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const int l = 64 * j + lane;
+            v[j] = l < n ? F.rroot[cb + l] : 0u;
+            sym[j] = (v[j] & kSym) != 0;
+        }
+        // kRootHops ancestors' values in turn: each is never a wrong root
+#pragma unroll
+        for (int h = 0; h < kRootHops; h++) {
+#pragma unroll
+            for (int j = 0; j < kG; j++)
+                if (v[j] & kSym) v[j] = F.rroot[v[j] & ~kSym];
+        }
+#pragma unroll
+        for (int j = 0; j < kG; j++)
+            if (sym[j]) F.rroot[cb + 64 * j + lane] = v[j];
+        return;
+    }
     for (int l = lane; l < n; l += 64) {
         uint32_t v = F.rroot[cb + l];
         if (!(v & kSym)) continue;
-        // kRootHops ancestors' values in turn: each is never a wrong root
 #pragma unroll
         for (int h = 0; h < kRootHops; h++)
             if (v & kSym) v = F.rroot[v & ~kSym];
@@ -1489,6 +1510,26 @@ __global__ __launch_bounds__(256) void k_lost(const Frame *frames) {
     if (y < 1 || y > F.qh) return;
     const uint32_t pb = F.row_off[y - 1];
     const int m = (int)F.nrec[y - 1];
+    constexpr int kG = 16;  // rows of up to 1024 records: the reads of all records at once
+    if (m <= 64 * kG) {
+        uint32_t root[kG], x[kG], w[kG];
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const int l = 64 * j + lane;
+            root[j] = l < m ? F.rroot[pb + l] : 0u;
+            x[j] = l < m ? (y < F.qh ? F.rlost[pb + l] : kLostRetired) : kLostNone;
+            w[j] = l < m ? F.recw[pb + l] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kG; j++)
+            if (x[j] != kLostNone && x[j] != kLostRetired && (x[j] & kSym)) x[j] = F.rroot[x[j] & ~kSym];
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            if (x[j] == kLostRetired) F.fw[root[j]] = w[j];
+            else if (x[j] != kLostNone) F.parent[root[j]] = x[j];
+        }
+        return;
+    }
     for (int l = lane; l < m; l += 64) {
         uint32_t root = F.rroot[pb + l];
         uint32_t x = y < F.qh ? F.rlost[pb + l] : kLostRetired;
