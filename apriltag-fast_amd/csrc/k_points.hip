@@ -230,6 +230,59 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
     if (!pass && lane == 0) F.pcnt[y] = count;
 }
 
+// Pass 0 of the emission on its own: a wave per row pair counts its
+// points (the masks of k_point_rows) and clears its share of the frame's
+// cluster table; kG steps of 64 columns loaded at once (the count's work a
+// step is a few instructions: one round of reads a step was its time).
+__global__ __launch_bounds__(256) void k_point_count(const Frame *frames) {
+    const Frame F = frames[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int y = 1 + (int)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (y > F.qh - 2) return;
+    const int QW = F.qw;
+    {
+        const uint32_t nrows = (uint32_t)(F.qh - 2), slots = F.hmask + 1;
+        for (uint32_t i = (uint32_t)(y - 1) * 64u + (uint32_t)lane; i < slots; i += nrows * 64u) F.hkey[i] = kSlotEmpty;
+    }
+    const uint8_t *a = F.t + (int64_t)y * QW, *b = a + QW;
+    const uint32_t *la = F.plab + (int64_t)y * QW, *lb = la + QW;
+    auto load = [&](int x) -> PixCol {
+        PixCol c{0, 0, 0, 0};
+        if (x < QW) {
+            c.ta = a[x];
+            c.tb = b[x];
+            c.la = la[x];
+            c.lb = lb[x];
+        }
+        return c;
+    };
+    constexpr int kG = 4;
+    uint32_t count = 0;
+    PixCol left{0, 0, 0, 0};
+    for (int g0 = 0; g0 < QW - 1; g0 += 64 * kG) {
+        PixCol col[kG];
+#pragma unroll
+        for (int j = 0; j < kG; j++) col[j] = load(g0 + 64 * j + lane);
+        const PixCol after = load(g0 + 64 * kG);  // the column right of the group
+#pragma unroll
+        for (int j = 0; j < kG; j++) {
+            const int x0 = g0 + 64 * j;
+            if (x0 >= QW - 1) break;
+            const int x = x0 + lane;
+            const PixCol fill = j + 1 < kG ? PixCol{(uint32_t)__builtin_amdgcn_readlane((int)col[j + 1].ta, 0),
+                                                    (uint32_t)__builtin_amdgcn_readlane((int)col[j + 1].tb, 0),
+                                                    (uint32_t)__builtin_amdgcn_readlane((int)col[j + 1].la, 0),
+                                                    (uint32_t)__builtin_amdgcn_readlane((int)col[j + 1].lb, 0)}
+                                           : after;
+            const PixCol r = col_from_right(col[j], fill), l = col_from_left(col[j], left);
+            const uint32_t m = x < QW - 1 ? point_mask(col[j], l, r, x) : 0u;
+            count += (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_u(__popc(m)), 63);
+            left = col_of_lane63(col[j]);
+        }
+    }
+    if (lane == 0) F.pcnt[y] = count;
+}
+
 // Row offsets of the points (one block per frame).
 __global__ __launch_bounds__(256) void k_point_scan(const Frame *frames) {
     const Frame F = frames[blockIdx.y];
@@ -702,7 +755,7 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
                          Marks *mk) {
     const int rw = tu.row_waves < 1 ? 1 : tu.row_waves > 4 ? 4 : tu.row_waves;  // launch bounds: 256 threads
     dim3 g((unsigned)((max_qh + rw - 1) / rw), nframes), b(64 * rw);
-    hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 0);
+    hipLaunchKernelGGL(k_point_count, g, b, 0, st, dframes);
     hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
     mk->mark(st, "points.count");
     hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);

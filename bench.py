@@ -71,7 +71,7 @@ MARK_KERNELS = {
     "uf_replay.wide": ["k_replay_wide<true>", "k_replay_wide<false>", "k_replay_hbm", "k_work_offsets"],
     "uf_stitch.jump": ["k_root_jump"], "uf_stitch.lost": ["k_lost"], "resolve": ["k_resolve"],
     "labels.rows": ["k_labels"],
-    "points.count": ["k_point_rows", "k_point_scan"], "points.write": ["k_point_rows"],
+    "points.count": ["k_point_count", "k_point_scan"], "points.write": ["k_point_rows"],
     "clusters.rank": ["k_cluster_rank"],
     "cluster_sort": ["k_rank_clear", "k_rank_counts", "k_cluster_starts", "k_radix_hist", "k_radix_offsets",
                      "k_radix_scatter"],
