@@ -1,10 +1,11 @@
 // Stage (c): boundary points, their clusters and the cluster order.
 //
-//   k_point_rows   one wave per row pair (y, y+1), y in 1..qh-2: the 4-bit
-//                  neighbour mask of each pixel (fast/clusters.c:150-174);
-//                  pass 0 counts points per row (and clears the frame's
-//                  cluster table), pass 1 writes each point at its
-//                  emission index with its cluster's table slot: each run
+//   k_point_count  one wave per row pair (y, y+1), y in 1..qh-2: the 4-bit
+//                  neighbour mask of each pixel (fast/clusters.c:150-174),
+//                  the points of each row counted (and the frame's cluster
+//                  table cleared).
+//   k_point_rows   the same masks again: each point written at its
+//                  emission index with its cluster's table slot; each run
 //                  of points of one cluster (id = max_root << 32 |
 //                  min_root, fast/clusters.c:229) finds or inserts the id.
 //   k_point_scan   the row offsets of the points.
@@ -125,26 +126,20 @@ __device__ __forceinline__ PixCol col_of_lane63(const PixCol &c) {
             (uint32_t)__builtin_amdgcn_readlane((int)c.la, 63), (uint32_t)__builtin_amdgcn_readlane((int)c.lb, 63)};
 }
 
-// A wave per row pair (y, y + 1), 64 pixels a step: each lane's column of
-// the step, the column right of the step (for lane 63) and both of the
-// next step loaded during this one; neighbours by shuffles, the left one
-// of lane 0 kept from the step before. Pass 0 counts the points of each
-// row and clears the frame's cluster table (the rows' waves share it out);
-// pass 1 stages each step's points (at most 4 a pixel) with their cluster
-// ids in LDS and writes them out contiguously with their clusters' slots:
-// the first point of each run of one cluster (in emission order, across
-// steps) looks the cluster up.
-__global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pass) {
+// A wave per row pair (y, y + 1), 64 pixels a step, kG steps of columns
+// loaded at once (each lane's column, and the column right of the group
+// for lane 63 of the last step); neighbours by shuffles, the left one of
+// lane 0 kept from the step before. Each step's points (at most 4 a pixel)
+// are staged with their cluster ids in LDS and written out contiguously
+// with their clusters' slots: the first point of each run of one cluster
+// (in emission order, across steps) looks the cluster up.
+__global__ __launch_bounds__(256) void k_point_rows(const Frame *frames) {
     __shared__ uint64_t st_pt[4][256], st_key[4][256];
     const Frame F = frames[blockIdx.y];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int y = 1 + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (y > F.qh - 2) return;
     const int QW = F.qw;
-    if (!pass) {
-        const uint32_t nrows = (uint32_t)(F.qh - 2), slots = F.hmask + 1;
-        for (uint32_t i = (uint32_t)(y - 1) * 64u + (uint32_t)lane; i < slots; i += nrows * 64u) F.hkey[i] = kSlotEmpty;
-    }
     const uint8_t *a = F.t + (int64_t)y * QW, *b = a + QW;
     const uint32_t *la = F.plab + (int64_t)y * QW, *lb = la + QW;
     auto load = [&](int x) -> PixCol {
@@ -158,76 +153,84 @@ __global__ __launch_bounds__(256) void k_point_rows(const Frame *frames, int pas
         return c;
     };
     uint32_t count = 0;
-    uint32_t base = pass ? F.poff[y] : 0u;
+    const uint32_t base = F.poff[y];
     const uint64_t lt = (1ull << lane) - 1ull;
     uint64_t prev_id = kSlotEmpty;  // the cluster of the row pair's last written point (none yet)
     uint32_t prev_slot = 0;
-    PixCol cur = load(lane), cr = load(64), left{0, 0, 0, 0};
-    for (int x0 = 0; x0 < QW - 1; x0 += 64) {
-        const PixCol nxt = load(x0 + 64 + lane), nr = load(x0 + 128);
-        const int x = x0 + lane;
-        const PixCol r = col_from_right(cur, cr), l = col_from_left(cur, left);
-        const uint32_t m = x < QW - 1 ? point_mask(cur, l, r, x) : 0u;
-        const uint32_t c = __popc(m);
-        if (pass) {
-            const uint32_t inc = wave_incl_scan_u(c), tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-            uint32_t k = inc - c;  // this lane's first point in the step
-            if (m) {
-                const int vd = 255 - 2 * (int)cur.ta;
-                const uint32_t r0 = cur.la & 0x7fffffffu;
-                for (int j = 0; j < 4; j++) {
-                    if (!((m >> j) & 1u)) continue;
-                    // partner (dx, dy): (1, 0), (0, 1), (-1, 1), (1, 1)
-                    const int dx = j == 0 ? 1 : j == 1 ? 0 : j == 2 ? -1 : 1;
-                    const int dy = j == 0 ? 0 : 1;
-                    const uint32_t r1 = (j == 0 ? r.la : j == 1 ? cur.lb : j == 2 ? l.lb : r.lb) & 0x7fffffffu;
-                    const uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
-                    const uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
-                    const uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
-                    st_pt[wv][k] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
-                    st_key[wv][k] = ((uint64_t)hi << 32) | lo;
-                    k++;
+    constexpr int kG = 4;
+    PixCol left{0, 0, 0, 0};
+    for (int g0 = 0; g0 < QW - 1; g0 += 64 * kG) {
+        PixCol col[kG];
+#pragma unroll
+        for (int jg = 0; jg < kG; jg++) col[jg] = load(g0 + 64 * jg + lane);
+        const PixCol after = load(g0 + 64 * kG);  // the column right of the group
+#pragma unroll
+        for (int jg = 0; jg < kG; jg++) {
+            const int x0 = g0 + 64 * jg;
+            if (x0 >= QW - 1) break;
+            const PixCol cur = col[jg];
+            const PixCol cr = jg + 1 < kG ? PixCol{(uint32_t)__builtin_amdgcn_readlane((int)col[jg + 1].ta, 0),
+                                                   (uint32_t)__builtin_amdgcn_readlane((int)col[jg + 1].tb, 0),
+                                                   (uint32_t)__builtin_amdgcn_readlane((int)col[jg + 1].la, 0),
+                                                   (uint32_t)__builtin_amdgcn_readlane((int)col[jg + 1].lb, 0)}
+                                          : after;
+            const int x = x0 + lane;
+            const PixCol r = col_from_right(cur, cr), l = col_from_left(cur, left);
+            const uint32_t m = x < QW - 1 ? point_mask(cur, l, r, x) : 0u;
+            const uint32_t c = __popc(m);
+                const uint32_t inc = wave_incl_scan_u(c), tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+                uint32_t k = inc - c;  // this lane's first point in the step
+                if (m) {
+                    const int vd = 255 - 2 * (int)cur.ta;
+                    const uint32_t r0 = cur.la & 0x7fffffffu;
+                    for (int j = 0; j < 4; j++) {
+                        if (!((m >> j) & 1u)) continue;
+                        // partner (dx, dy): (1, 0), (0, 1), (-1, 1), (1, 1)
+                        const int dx = j == 0 ? 1 : j == 1 ? 0 : j == 2 ? -1 : 1;
+                        const int dy = j == 0 ? 0 : 1;
+                        const uint32_t r1 = (j == 0 ? r.la : j == 1 ? cur.lb : j == 2 ? l.lb : r.lb) & 0x7fffffffu;
+                        const uint32_t hi = r0 > r1 ? r0 : r1, lo = r0 > r1 ? r1 : r0;
+                        const uint32_t px = (uint32_t)(2 * x + dx), py = (uint32_t)(2 * y + dy);
+                        const uint32_t gx = (uint32_t)(dx * vd) & 0xffffu, gy = (uint32_t)(dy * vd) & 0xffffu;
+                        st_pt[wv][k] = (uint64_t)px | ((uint64_t)py << 16) | ((uint64_t)gx << 32) | ((uint64_t)gy << 48);
+                        st_key[wv][k] = ((uint64_t)hi << 32) | lo;
+                        k++;
+                    }
                 }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-            const uint32_t at0 = base + count;
-            for (uint32_t q0 = 0; q0 < tot; q0 += 64) {
-                const uint32_t q = q0 + (uint32_t)lane;
-                const bool valid = q < tot;
-                const uint64_t id = valid ? st_key[wv][q] : 0ull;
-                const uint64_t before = !valid ? 0ull : q == 0 ? prev_id : st_key[wv][q - 1];
-                const bool head = valid && id != before;  // the first point of a run of its cluster
-                uint32_t slot = head ? cluster_slot(F, id) : 0u;
-                // the slot of the last head at or before each lane; lanes
-                // before the chunk's first head continue the previous run
-                const uint64_t upto = __ballot(head) & (lt | (1ull << lane));
-                const int src = upto ? 63 - __clzll((long long)upto) : 0;
-                const uint32_t from = (uint32_t)__shfl((int)slot, src, 64);
-                slot = upto ? from : prev_slot;
-                const int64_t at = (int64_t)at0 + q;
-                if (valid && at < F.pcap) {  // (beyond pcap: flagged by k_point_scan)
-                    F.pts[at] = st_pt[wv][q];
-                    F.sval[at] = slot;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                const uint32_t at0 = base + count;
+                for (uint32_t q0 = 0; q0 < tot; q0 += 64) {
+                    const uint32_t q = q0 + (uint32_t)lane;
+                    const bool valid = q < tot;
+                    const uint64_t id = valid ? st_key[wv][q] : 0ull;
+                    const uint64_t before = !valid ? 0ull : q == 0 ? prev_id : st_key[wv][q - 1];
+                    const bool head = valid && id != before;  // the first point of a run of its cluster
+                    uint32_t slot = head ? cluster_slot(F, id) : 0u;
+                    // the slot of the last head at or before each lane; lanes
+                    // before the chunk's first head continue the previous run
+                    const uint64_t upto = __ballot(head) & (lt | (1ull << lane));
+                    const int src = upto ? 63 - __clzll((long long)upto) : 0;
+                    const uint32_t from = (uint32_t)__shfl((int)slot, src, 64);
+                    slot = upto ? from : prev_slot;
+                    const int64_t at = (int64_t)at0 + q;
+                    if (valid && at < F.pcap) {  // (beyond pcap: flagged by k_point_scan)
+                        F.pts[at] = st_pt[wv][q];
+                        F.sval[at] = slot;
+                    }
+                    const int lastl = (int)(min(tot - q0, 64u) - 1u);
+                    prev_id = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(id >> 32), lastl, 64) << 32) |
+                              (uint32_t)__shfl((int)(uint32_t)id, lastl, 64);
+                    prev_slot = (uint32_t)__shfl((int)slot, lastl, 64);
                 }
-                const int lastl = (int)(min(tot - q0, 64u) - 1u);
-                prev_id = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(id >> 32), lastl, 64) << 32) |
-                          (uint32_t)__shfl((int)(uint32_t)id, lastl, 64);
-                prev_slot = (uint32_t)__shfl((int)slot, lastl, 64);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-            count += tot;
-        } else {
-            count += (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_u(c), 63);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                count += tot;
+            left = col_of_lane63(cur);
         }
-        left = col_of_lane63(cur);
-        cur = nxt;
-        cr = nr;
     }
-    if (!pass && lane == 0) F.pcnt[y] = count;
 }
 
 // Pass 0 of the emission on its own: a wave per row pair counts its
@@ -758,7 +761,7 @@ void launch_points_stage(hipStream_t st, const Frame *dframes, int nframes, int 
     hipLaunchKernelGGL(k_point_count, g, b, 0, st, dframes);
     hipLaunchKernelGGL(k_point_scan, dim3(1, nframes), dim3(256), 0, st, dframes);
     mk->mark(st, "points.count");
-    hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes, 1);
+    hipLaunchKernelGGL(k_point_rows, g, b, 0, st, dframes);
     mk->mark(st, "points.write");
     hipLaunchKernelGGL(k_cluster_rank, dim3(1, nframes), dim3(kRankThreads), kRankLds * 8, st, dframes);
     mk->mark(st, "clusters.rank");
