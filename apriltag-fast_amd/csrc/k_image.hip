@@ -183,7 +183,11 @@ __device__ __forceinline__ void row_runs(const Frame &F, int y, TF tfun) {
 // Eight waves a SIMD (64 VGPRs, two of them spilled to scratch): the rows'
 // loads in flight cover more of the latency than at six (80 VGPRs, no
 // spills): 0.56 -> 0.49 ms for 256 frames of 1080p.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_threshold(const Frame *frames, const DevParams *P, int do_rle, int trows) {
+// Each wave first copies its pixel row into LDS (rboff: the row buffers'
+// offset, rbstride bytes a wave), its loads issued eight steps at a time:
+// the threshold and the run extraction then read the row from LDS.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_threshold(const Frame *frames, const DevParams *P, int do_rle, int trows,
+                                                                                          int rboff, int rbstride) {
     extern __shared__ uint8_t lds[];
     const Frame F = frames[blockIdx.y];
     const int t0 = (int)blockIdx.x * trows;
@@ -214,8 +218,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         const int ty = y >> 2;
         const bool tail = ty == F.th;
         const uint8_t *hx = lds + 2 * tw * (ty - t0), *hn = hx + tw;
-        const uint8_t *row = F.qim + (int64_t)y * F.qw;
+        const uint8_t *grow = F.qim + (int64_t)y * F.qw;
         uint8_t *trow = F.t + (int64_t)y * F.qw;
+        uint8_t *row = lds + rboff + wave * rbstride;
+        for (int x0 = 0; x0 < F.qw; x0 += 512) {
+            uint32_t v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int x = x0 + 64 * q + lane;
+                v[q] = x < F.qw ? grow[x] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int x = x0 + 64 * q + lane;
+                if (x < F.qw) row[x] = (uint8_t)v[q];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         auto tfun = [&](int x) -> uint32_t {
             bool xin = x < 4 * tw;
             int tx = xin ? x >> 2 : tw - 1;
@@ -297,9 +318,12 @@ void launch_image_stage(hipStream_t st, const Frame *dframes, const DevParams *d
         mk->mark(st, "blur");
     }
     int max_th = max_qh / 4, trows = tu.thr_tile_rows;
-    size_t lds = (size_t)2 * (max_qw / 4) * trows + 16;
+    // the tile extrema of trows tile rows, then a pixel row a wave
+    const int rboff = (int)(((size_t)2 * (max_qw / 4) * trows + 16 + 15) & ~(size_t)15);
+    const int rbstride = (max_qw + 15) & ~15;
+    size_t lds = (size_t)rboff + (size_t)4 * rbstride;
     dim3 g2(cdiv(max_th + 1, trows), nframes);
-    hipLaunchKernelGGL(k_threshold, g2, dim3(256), lds, st, dframes, dparams, deglitch ? 0 : 1, trows);
+    hipLaunchKernelGGL(k_threshold, g2, dim3(256), lds, st, dframes, dparams, deglitch ? 0 : 1, trows, rboff, rbstride);
     if (deglitch) {
         dim3 gp(cdiv(maxpx, 256), nframes);
         hipLaunchKernelGGL(k_deglitch_max, gp, dim3(256), 0, st, dframes);
