@@ -1103,14 +1103,39 @@ __global__ __launch_bounds__(NT, NT == 128 ? 8 : 1) void k_fit_moments(const Fra
             constexpr uint32_t kFixLds = kBufs * 6 * CH;
             uint64_t *kl = reinterpret_cast<uint64_t *>(&sm[0][0][0]);
             const bool in_lds = n <= kFixLds;
+            // (the copies four keys a thread at a time, their loads together)
             if (in_lds) {
-                for (uint32_t i = t; i < n; i += NT) kl[i] = keys[i];
+                for (uint32_t i0 = 0; i0 < n; i0 += 4 * NT) {
+                    uint64_t cv[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t i = i0 + t + (uint32_t)(r * NT);
+                        cv[r] = i < n ? gmem(keys)[i] : 0ull;
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t i = i0 + t + (uint32_t)(r * NT);
+                        if (i < n) kl[i] = cv[r];
+                    }
+                }
                 __syncthreads();
             }
             fix_runs_block<NT>(in_lds ? kl : keys, n, F.okey + s0, reinterpret_cast<uint64_t *>(F.errs + s0),
                                reinterpret_cast<uint64_t *>(F.filt + s0), fls, flm, &fcnt);
             if (in_lds) {
-                for (uint32_t i = t; i < n; i += NT) keys[i] = kl[i];
+                for (uint32_t i0 = 0; i0 < n; i0 += 4 * NT) {
+                    uint64_t cv[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t i = i0 + t + (uint32_t)(r * NT);
+                        cv[r] = i < n ? kl[i] : 0ull;
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t i = i0 + t + (uint32_t)(r * NT);
+                        if (i < n) gmem(keys)[i] = cv[r];
+                    }
+                }
                 __syncthreads();
             }
         }
