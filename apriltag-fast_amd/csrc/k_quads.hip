@@ -1397,6 +1397,34 @@ __device__ __forceinline__ uint64_t dkey(double d) {
     uint64_t b = (uint64_t)__double_as_longlong(d);
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
+// the double of an order-preserving integer
+__device__ __forceinline__ double dkey_value(uint64_t k) {
+    return __longlong_as_double((long long)((k >> 63) ? (k ^ 0x8000000000000000ull) : ~k));
+}
+
+// The first R rows of a wave's keys (R = 1, 2 or 4; key i = r * 64 + lane
+// in kv[r]) sorted ascending in place.
+__device__ __forceinline__ void wave_sort_rows(uint64_t (&kv)[4], int R) {
+    if (R == 1) {
+        uint64_t a[1] = {kv[0]};
+        wave_bitonic<1>(a);
+        kv[0] = a[0];
+    } else if (R == 2) {
+        uint64_t a[2] = {kv[0], kv[1]};
+        wave_bitonic<2>(a);
+        kv[0] = a[0];
+        kv[1] = a[1];
+    } else {
+        wave_bitonic<4>(kv);
+    }
+}
+
+// Key pos (wave-uniform) of a wave's rows, in every lane.
+__device__ __forceinline__ uint64_t wave_key_at(const uint64_t (&kv)[4], uint32_t pos) {
+    const uint64_t row = pos < 64 ? kv[0] : pos < 128 ? kv[1] : pos < 192 ? kv[2] : kv[3];
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)(pos & 63)) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)(pos & 63));
+}
 
 // The triples m0 < m1 < m2 in colex order (by m2, then m1, then m0): the
 // first C(k, 3) of them are those with m2 < k. Packed m0 | m1 << 5 | m2 << 10.
@@ -1446,9 +1474,9 @@ __global__ __launch_bounds__(NT, 4) void k_fit_corners(const Frame *frames, cons
                                                     int l) {
     constexpr int kMaxNm = CornerShape<NT>::kMaxNm, kLdsPts = CornerShape<NT>::kLdsPts;
     constexpr int kMaxima = CornerShape<NT>::kMaxima;
-    // errors and filtered errors, and (256 threads: the one-wave shape
-    // sorts the maxima in registers) the maxima values
-    constexpr int kPart1 = 2 * kLdsPts + (NT == 256 ? kMaxima : 0);
+    // errors and filtered errors, and (256 threads) the waves' top maxima
+    // keys of the threshold's merge
+    constexpr int kPart1 = 2 * kLdsPts + (NT == 256 ? 256 : 0);
     constexpr int kShared = kPart1 > 4 * kMaxNm * kMaxNm ? kPart1 : 4 * kMaxNm * kMaxNm;
     constexpr int kWaves = NT / 64;
     // The errors, filtered errors and maxima values of a cluster (first
@@ -1583,81 +1611,48 @@ __global__ __launch_bounds__(NT, 4) void k_fit_corners(const Frame *frames, cons
                     const uint32_t i = (uint32_t)(r * 64 + lane);
                     kv[r] = i < nm ? dkey(y[midx[i]]) : 0ull;
                 }
-                int R = 4;
-                if (nm <= 64) {
-                    uint64_t a[1] = {kv[0]};
-                    wave_bitonic<1>(a);
-                    kv[0] = a[0];
-                    R = 1;
-                } else if (nm <= 128) {
-                    uint64_t a[2] = {kv[0], kv[1]};
-                    wave_bitonic<2>(a);
-                    kv[0] = a[0];
-                    kv[1] = a[1];
-                    R = 2;
-                } else {
-                    wave_bitonic<4>(kv);
-                }
-                const uint32_t pos = (uint32_t)(64 * R) - 1u - rank;
-                const uint64_t row = pos < 64 ? kv[0] : pos < 128 ? kv[1] : pos < 192 ? kv[2] : kv[3];
-                const uint64_t k = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(row >> 32), (int)(pos & 63)) << 32) |
-                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)(pos & 63));
-                thr = __longlong_as_double((long long)((k >> 63) ? (k ^ 0x8000000000000000ull) : ~k));
-            } else if (NT == 256 && nm <= (uint32_t)kMaxima) {
-                // rank + 1 rounds, each taking the largest remaining value
-                // (ties by position) out of the maxima held in registers,
-                // four a thread: the last one taken is the value at
-                // descending rank `rank`
-                constexpr int kPer = kMaxima / 256;
-                double v[kPer];
-                bool live[kPer];
+                const int R = nm <= 64 ? 1 : nm <= 128 ? 2 : 4;
+                wave_sort_rows(kv, R);
+                thr = dkey_value(wave_key_at(kv, (uint32_t)(64 * R) - 1u - rank));
+            } else if (NT == 256 && nm <= (uint32_t)kMaxima && rank < 64) {
+                // the maxima's keys sorted in registers a wave at a time,
+                // 256 a wave (absent entries 0, first); past one wave's
+                // worth, the waves' top 64 keys (which hold the overall top
+                // 64) sorted again by wave 0. The value goes to every wave
+                // through LDS.
+                const uint32_t nwv = (nm + 255) / 256;
+                uint64_t *top = (uint64_t *)(sh + 2 * kLdsPts);
+                if ((uint32_t)w < nwv) {
+                    const uint32_t cnt = min(256u, nm - (uint32_t)w * 256u);
+                    const int R = cnt <= 64 ? 1 : cnt <= 128 ? 2 : 4;
+                    uint64_t kv[4];
 #pragma unroll
-                for (int r = 0; r < kPer; r++) {
-                    const uint32_t i = (uint32_t)(t + 256 * r);
-                    live[r] = i < nm;
-                    v[r] = live[r] ? y[midx[i]] : 0.0;
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t i = (uint32_t)(w * 256 + r * 64 + lane);
+                        kv[r] = (r < R && i < nm) ? dkey(y[midx[i]]) : 0ull;
+                    }
+                    wave_sort_rows(kv, R);
+                    if (nwv == 1) {
+                        const double v = dkey_value(wave_key_at(kv, (uint32_t)(64 * R) - 1u - rank));
+                        if (lane == 0) s_wbest[0] = v;
+                    } else {
+                        top[w * 64 + lane] = R == 1 ? kv[0] : R == 2 ? kv[1] : kv[3];
+                    }
                 }
-                thr = 0.0;
-                for (uint32_t it = 0; it <= rank; it++) {
-                    double bv = 0.0;
-                    uint32_t bi = 0xffffffffu;
-#pragma unroll
-                    for (int r = 0; r < kPer; r++) {
-                        const uint32_t i = (uint32_t)(t + 256 * r);
-                        if (live[r] && (bi == 0xffffffffu || v[r] > bv)) {
-                            bv = v[r];
-                            bi = i;
-                        }
-                    }
-                    for (int d = 32; d >= 1; d >>= 1) {
-                        const double v2 = __shfl_xor(bv, d, 64);
-                        const uint32_t i2 = (uint32_t)__shfl_xor((int)bi, d, 64);
-                        if (i2 != 0xffffffffu && (bi == 0xffffffffu || v2 > bv || (v2 == bv && i2 < bi))) {
-                            bv = v2;
-                            bi = i2;
-                        }
-                    }
-                    if (lane == 0) {
-                        s_wbest[w] = bv;
-                        s_word[w] = bi;
-                    }
+                if (nwv > 1) {
                     __syncthreads();
-                    bv = s_wbest[0];
-                    bi = s_word[0];
-                    for (int k = 1; k < kWaves; k++) {
-                        const double v2 = s_wbest[k];
-                        const uint32_t i2 = s_word[k];
-                        if (i2 != 0xffffffffu && (bi == 0xffffffffu || v2 > bv || (v2 == bv && i2 < bi))) {
-                            bv = v2;
-                            bi = i2;
-                        }
-                    }
-                    __syncthreads();
-                    thr = bv;
+                    if (w == 0) {
+                        uint64_t kv[4];
 #pragma unroll
-                    for (int r = 0; r < kPer; r++)
-                        if ((uint32_t)(t + 256 * r) == bi) live[r] = false;
+                        for (int r = 0; r < 4; r++) kv[r] = (uint32_t)r < nwv ? top[r * 64 + lane] : 0ull;
+                        const int R = nwv <= 2 ? 2 : 4;
+                        wave_sort_rows(kv, R);
+                        const double v = dkey_value(wave_key_at(kv, (uint32_t)(64 * R) - 1u - rank));
+                        if (lane == 0) s_wbest[0] = v;
+                    }
                 }
+                __syncthreads();
+                thr = s_wbest[0];
             } else if constexpr (NT == 256) {
                 // radix select over the order-preserving bits
                 uint32_t rk = rank;
@@ -1686,7 +1681,7 @@ __global__ __launch_bounds__(NT, 4) void k_fit_corners(const Frame *frames, cons
                     rk = s_rank;
                     __syncthreads();
                 }
-                thr = __longlong_as_double((long long)((prefix >> 63) ? (prefix ^ 0x8000000000000000ull) : ~prefix));
+                thr = dkey_value(prefix);
             } else {
                 thr = 0.0;  // the wave shape holds at most kMaxima maxima
             }
