@@ -345,9 +345,9 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 
 // The cluster sort's word of point i: its cluster's rank | i above the
 // rank bits (formed where the first pass reads it, from the point's table
-// slot and the slots' ranks).
-__device__ __forceinline__ uint64_t rank_word(const Frame &F, uint32_t i, uint32_t sb) {
-    return (uint64_t)F.hrank[F.sval[i]] | ((uint64_t)i << sb);
+// slot, sval[i], and the slots' ranks).
+__device__ __forceinline__ uint64_t rank_word(const Frame &F, uint32_t slot, uint32_t i, uint32_t sb) {
+    return (uint64_t)F.hrank[slot] | ((uint64_t)i << sb);
 }
 
 // dyn: the pass is skipped where the frame's key (counters[kSortBits]) has
@@ -373,17 +373,42 @@ __global__ __launch_bounds__(256) void k_radix_hist(const Frame *frames, int whi
     uint32_t s0, s1;
     seg_of(n, blockIdx.x, &s0, &s1);
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
-        const uint32_t i = c0 + t;
-        const bool valid = i < s1;
-        const uint64_t key = !valid ? 0ull : ranks ? rank_word(F, i, sb) : B.kin[i];
-        const uint32_t d = valid ? (uint32_t)((key & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
-        const uint32_t dl = wave_from_left(d, 0xfffffffeu);  // lane 0: a head
-        const uint64_t heads = __ballot(valid && d != dl) | ~__ballot(valid);  // invalid lanes end runs
-        if (valid && d != dl) {
-            const uint64_t after = heads & ~(lt | (1ull << lane));
-            const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
-            atomicAdd(&h[d], end - (uint32_t)lane);
+    // four chunks a step: their slots' reads issued together, then their
+    // words' (the counts do not depend on the order they are added in)
+    constexpr int kG = 4;
+    for (uint32_t c0 = s0; c0 < s1; c0 += 256 * kG) {
+        uint64_t keys[kG];
+        if (ranks) {
+            uint32_t slot[kG];
+#pragma unroll
+            for (int g = 0; g < kG; g++) {
+                const uint32_t i = c0 + (uint32_t)(g * 256 + t);
+                slot[g] = i < s1 ? F.sval[i] : 0u;
+            }
+#pragma unroll
+            for (int g = 0; g < kG; g++) {
+                const uint32_t i = c0 + (uint32_t)(g * 256 + t);
+                keys[g] = i < s1 ? rank_word(F, slot[g], i, sb) : 0ull;
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < kG; g++) {
+                const uint32_t i = c0 + (uint32_t)(g * 256 + t);
+                keys[g] = i < s1 ? B.kin[i] : 0ull;
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kG; g++) {
+            const uint32_t i = c0 + (uint32_t)(g * 256 + t);
+            const bool valid = i < s1;
+            const uint32_t d = valid ? (uint32_t)((keys[g] & km) >> shift) & (kRadixBins - 1) : 0xffffffffu;
+            const uint32_t dl = wave_from_left(d, 0xfffffffeu);  // lane 0: a head
+            const uint64_t heads = __ballot(valid && d != dl) | ~__ballot(valid);  // invalid lanes end runs
+            if (valid && d != dl) {
+                const uint64_t after = heads & ~(lt | (1ull << lane));
+                const uint32_t end = after ? (uint32_t)(__ffsll((unsigned long long)after) - 1) : 64u;
+                atomicAdd(&h[d], end - (uint32_t)lane);
+            }
         }
     }
     __syncthreads();
@@ -481,7 +506,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const Frame *frames, int 
     auto slot_at = [&](uint32_t i) -> uint32_t { return ranks && i < s1 ? F.sval[i] : 0u; };
     auto word_at = [&](uint32_t i, uint32_t slot) -> uint64_t {
         if (i >= s1) return 0ull;
-        return ranks ? ((uint64_t)F.hrank[slot] | ((uint64_t)i << sb)) : B.kin[i];
+        return ranks ? rank_word(F, slot, i, sb) : B.kin[i];
     };
     auto val_at = [&](uint32_t i) -> uint32_t { return i < s1 && !packed ? (iota ? i : B.vin[i]) : 0u; };
     uint64_t key_n = word_at(s0 + t, slot_at(s0 + t));
